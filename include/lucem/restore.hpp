@@ -1,0 +1,218 @@
+// C++ host mirror of lucem-ops' Richardson-Lucy interface, header-only over the C ABI
+// (lucem_hip.h). Names, argument meaning and error behaviour follow the reference:
+//
+//   lucem::PadMode, PadMode parse        crates/lucem-ops/src/core/pad.rs:13-57
+//   lucem::OpsError / OpsResult          crates/lucem-ops/src/error.rs:8-59
+//   lucem::RichardsonLucy                crates/lucem-ops/src/restore/deconvolve.rs:19-41
+//   lucem::richardson_lucy_into          crates/lucem-ops/src/restore/deconvolve.rs:45-149
+//   lucem::richardson_lucy               crates/lucem-ops/src/restore/deconvolve.rs:152-162
+//   lucem::Client, Workspace, DeviceArray crates/lucem-core/src/compute/target.rs:95-115,
+//                                        memory/workspace.rs:17-37, memory/array.rs:20-151
+//
+// Every call enqueues on the client's stream; Client::sync() is the barrier
+// (crates/lucem-core/src/compute/bench.rs:10-13).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../lucem_hip.h"
+
+namespace lucem {
+
+enum class PadMode : uint32_t { Reflect = 0, Constant = 1, Nearest = 2, Mirror = 3, Wrap = 4 };
+
+// An error of the C ABI: `status` is the OpsError / CoreError variant, what() its message.
+class OpsError : public std::runtime_error {
+   public:
+    OpsError(lucem_status st, const std::string& msg) : std::runtime_error(msg), status(st) {}
+    lucem_status status;
+};
+
+// OpsResult<T>: a value or an OpsError (Result<T, OpsError> of the reference).
+template <class T>
+class OpsResult {
+   public:
+    OpsResult(T v) : value_(std::move(v)), status_(LUCEM_OK) {}
+    OpsResult(lucem_status st, std::string msg) : status_(st), message_(std::move(msg)) {}
+    bool is_ok() const { return status_ == LUCEM_OK; }
+    lucem_status status() const { return status_; }
+    const std::string& message() const { return message_; }
+    T& unwrap() {
+        if (!is_ok()) throw OpsError(status_, message_);
+        return *value_;
+    }
+
+   private:
+    std::optional<T> value_;
+    lucem_status status_;
+    std::string message_;
+};
+
+struct Unit {};
+
+inline OpsResult<Unit> status_result(lucem_status st) {
+    if (st == LUCEM_OK) return Unit{};
+    return OpsResult<Unit>(st, lucem_last_error());
+}
+
+inline OpsResult<PadMode> parse_pad_mode(const std::string& name) {
+    uint32_t id = 0;
+    lucem_status st = lucem_pad_mode_parse(name.c_str(), &id);
+    if (st != LUCEM_OK) return OpsResult<PadMode>(st, lucem_last_error());
+    return static_cast<PadMode>(id);
+}
+
+struct RichardsonLucy {
+    size_t num_iter = 30;
+    bool clip = true;
+    std::optional<double> filter_epsilon;
+    PadMode mode = PadMode::Reflect;
+
+    lucem_rl_settings to_c() const {
+        lucem_rl_settings s;
+        lucem_rl_settings_default(&s);
+        s.num_iter = num_iter;
+        s.clip = clip ? 1 : 0;
+        s.has_filter_epsilon = filter_epsilon.has_value() ? 1 : 0;
+        s.filter_epsilon = filter_epsilon.value_or(0.0);
+        s.mode = static_cast<uint32_t>(mode);
+        return s;
+    }
+};
+
+// One device and one stream (own stream unless `stream` is given).
+class Client {
+   public:
+    explicit Client(int device = 0, hipStream_t stream = nullptr, bool own_stream = true) {
+        lucem_status st = lucem_client_create(device, stream, own_stream ? 1 : 0, &h_);
+        if (st != LUCEM_OK) throw OpsError(st, lucem_last_error());
+    }
+    ~Client() {
+        if (h_) lucem_client_destroy(h_);
+    }
+    Client(const Client&) = delete;
+    Client& operator=(const Client&) = delete;
+    lucem_client* handle() const { return h_; }
+    hipStream_t stream() const { return static_cast<hipStream_t>(lucem_client_stream(h_)); }
+    void sync() const {
+        lucem_status st = lucem_client_sync(h_);
+        if (st != LUCEM_OK) throw OpsError(st, lucem_last_error());
+    }
+
+   private:
+    lucem_client* h_ = nullptr;
+};
+
+class Workspace {
+   public:
+    explicit Workspace(const Client& c) {
+        lucem_status st = lucem_workspace_create(c.handle(), &h_);
+        if (st != LUCEM_OK) throw OpsError(st, lucem_last_error());
+    }
+    ~Workspace() {
+        if (h_) lucem_workspace_destroy(h_);
+    }
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    lucem_workspace* handle() const { return h_; }
+    uint64_t capacity_bytes() const { return lucem_workspace_capacity_bytes(h_); }
+
+   private:
+    lucem_workspace* h_ = nullptr;
+};
+
+// A C-contiguous device array. Copies share the buffer (as DeviceArray clones do).
+template <class F>
+class DeviceArray {
+   public:
+    static OpsResult<DeviceArray> empty(const Client&, std::vector<uint64_t> dims) {
+        DeviceArray a;
+        a.dims_ = std::move(dims);
+        const size_t n = a.len();
+        if (n) {
+            void* p = nullptr;
+            hipError_t e = hipMalloc(&p, n * sizeof(F));
+            if (e != hipSuccess) return OpsResult<DeviceArray>(LUCEM_OUT_OF_DEVICE_MEMORY, hipGetErrorString(e));
+            a.buf_ = std::shared_ptr<F>(static_cast<F*>(p), [](F* q) { (void)hipFree(q); });
+        }
+        return a;
+    }
+    static OpsResult<DeviceArray> from_host(const Client& c, std::vector<uint64_t> dims, const std::vector<F>& host) {
+        auto r = empty(c, std::move(dims));
+        if (!r.is_ok()) return r;
+        DeviceArray a = r.unwrap();
+        if (host.size() != a.len()) return OpsResult<DeviceArray>(LUCEM_SHAPE_MISMATCH, "host data length != shape");
+        if (a.len()) {
+            hipError_t e = hipMemcpyAsync(a.data(), host.data(), a.len() * sizeof(F), hipMemcpyHostToDevice, c.stream());
+            if (e == hipSuccess) e = hipStreamSynchronize(c.stream());
+            if (e != hipSuccess) return OpsResult<DeviceArray>(LUCEM_HIP_ERROR, hipGetErrorString(e));
+        }
+        return a;
+    }
+    std::vector<F> to_host(const Client& c) const {
+        std::vector<F> out(len());
+        if (len()) {
+            hipError_t e = hipMemcpyAsync(out.data(), data(), len() * sizeof(F), hipMemcpyDeviceToHost, c.stream());
+            if (e == hipSuccess) e = hipStreamSynchronize(c.stream());
+            if (e != hipSuccess) throw OpsError(LUCEM_HIP_ERROR, hipGetErrorString(e));
+        }
+        return out;
+    }
+    size_t len() const {
+        size_t n = 1;
+        for (auto d : dims_) n *= d;
+        return n;
+    }
+    bool is_empty() const { return len() == 0; }
+    const std::vector<uint64_t>& dims() const { return dims_; }
+    F* data() const { return buf_.get(); }
+    bool shares_buffer_with(const DeviceArray& o) const { return buf_ && buf_ == o.buf_; }
+
+   private:
+    std::vector<uint64_t> dims_;
+    std::shared_ptr<F> buf_;
+};
+
+// richardson_lucy_into::<f32> (deconvolve.rs:45-149).
+inline OpsResult<Unit> richardson_lucy_into(const Client& client, const DeviceArray<float>& input,
+                                            const DeviceArray<float>& psf, const DeviceArray<float>& output,
+                                            const RichardsonLucy& settings, Workspace& ws) {
+    const lucem_rl_settings s = settings.to_c();
+    return status_result(lucem_richardson_lucy_into_f32(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
+        psf.dims().data(), (uint32_t)psf.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+// richardson_lucy::<f32> (deconvolve.rs:152-162): allocates the output and a workspace.
+inline OpsResult<DeviceArray<float>> richardson_lucy(const Client& client, const DeviceArray<float>& input,
+                                                     const DeviceArray<float>& psf, const RichardsonLucy& settings) {
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    Workspace ws(client);
+    auto r = richardson_lucy_into(client, input, psf, out.unwrap(), settings, ws);
+    if (!r.is_ok()) return OpsResult<DeviceArray<float>>(r.status(), r.message());
+    client.sync();  // the workspace is released here; its buffers must not be in use
+    return out;
+}
+
+// The N-D (2-D or 3-D) form of richardson_lucy_into.
+inline OpsResult<Unit> richardson_lucy_nd_into(const Client& client, const DeviceArray<float>& input,
+                                               const DeviceArray<float>& psf, const DeviceArray<float>& output,
+                                               const RichardsonLucy& settings, Workspace& ws) {
+    const lucem_rl_settings s = settings.to_c();
+    return status_result(lucem_richardson_lucy_nd_into_f32(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
+        psf.dims().data(), (uint32_t)psf.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+}  // namespace lucem

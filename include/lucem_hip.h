@@ -1,0 +1,146 @@
+/*
+ * lucem_hip.h — C ABI of the MI355X (gfx950) Richardson–Lucy deconvolution library.
+ *
+ * Every entry point takes device pointers owned by the caller, plain sizes and POD settings,
+ * enqueues work on the client's HIP stream and returns without blocking. Status codes are
+ * returned in the same order as the reference's checks.
+ *
+ * Reference interfaces replaced (paths in neuro-kitchen/lucem):
+ *   lucem_client_*        lucem_core::compute::Client (crates/lucem-core/src/compute/target.rs:95-115)
+ *                         and its sync (crates/lucem-core/src/compute/bench.rs:10-13)
+ *   lucem_workspace_*     lucem_core::memory::Workspace (crates/lucem-core/src/memory/workspace.rs:17-37)
+ *   lucem_status          lucem_ops::error::OpsError (crates/lucem-ops/src/error.rs:8-59) and
+ *                         lucem_core::CoreError::{TooManyElements, OutOfDeviceMemory}
+ *                         (crates/lucem-core/src/compute/limits.rs:11-22, memory/budget.rs:232-244)
+ *   lucem_pad_mode_parse  lucem_ops::core::PadMode::parse (crates/lucem-ops/src/core/pad.rs:48-57)
+ *   lucem_rl_settings     lucem_ops::restore::RichardsonLucy (crates/lucem-ops/src/restore/deconvolve.rs:19-41)
+ *   lucem_richardson_lucy_into_f32
+ *                         lucem_ops::restore::richardson_lucy_into::<f32>
+ *                         (crates/lucem-ops/src/restore/deconvolve.rs:45-149)
+ *   lucem_richardson_lucy_f32
+ *                         lucem_ops::restore::richardson_lucy::<f32> (deconvolve.rs:152-162): the
+ *                         output is caller-allocated here, the workspace is the client's own.
+ *   lucem_richardson_lucy_nd_into_f32
+ *                         the same operator for 2-D or 3-D arrays (no reference counterpart: the
+ *                         reference accepts 2-D only, deconvolve.rs:55-56)
+ *   lucem_rl_batch_*      many tiles of uneven size in one graph-captured launch sequence; each
+ *                         tile is what a lucem-pipeline custom::transform closure receives
+ *                         (engine/lucem-pipeline/src/custom.rs:233-251, Block geometry
+ *                         crates/lucem-core/src/tiling.rs:13-23); no reference counterpart
+ *   lucem_rl_grad_*       differentiable unrolled RL (forward with checkpoints, adjoint); no
+ *                         reference counterpart
+ */
+#ifndef LUCEM_HIP_H
+#define LUCEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. Order and meaning follow OpsError / CoreError. */
+typedef enum lucem_status {
+    LUCEM_OK = 0,
+    LUCEM_SHAPE_MISMATCH = 1,      /* OpsError::ShapeMismatch       (core/check.rs:9-26)  */
+    LUCEM_ALIASED_BUFFERS = 2,     /* OpsError::AliasedBuffers      (core/check.rs:47-57) */
+    LUCEM_INVALID_DIMENSION = 3,   /* OpsError::InvalidDimension    (core/check.rs:84-91) */
+    LUCEM_INVALID_WEIGHTS = 4,     /* OpsError::InvalidWeights      (deconvolve.rs:57-62) */
+    LUCEM_INVALID_PARAMETER = 5,   /* OpsError::InvalidParameter    (deconvolve.rs:63-68) */
+    LUCEM_TOO_MANY_ELEMENTS = 6,   /* CoreError::TooManyElements    (compute/limits.rs:16-22) */
+    LUCEM_OUT_OF_DEVICE_MEMORY = 7,/* CoreError::OutOfDeviceMemory  (memory/budget.rs:232-244) */
+    LUCEM_UNKNOWN_PAD_MODE = 8,    /* OpsError::UnknownPadMode      (core/pad.rs:55)      */
+    LUCEM_UNSUPPORTED_SHAPE = 9,   /* an axis longer than this library's single-pass FFT limit */
+    LUCEM_HIP_ERROR = 10,          /* a HIP runtime call failed; see lucem_last_error()   */
+    LUCEM_INVALID_HANDLE = 11      /* NULL client / workspace / settings pointer          */
+} lucem_status;
+
+/* PadMode ids (core/pad.rs:29-33). RL always uses cval = 0. */
+enum {
+    LUCEM_PAD_REFLECT = 0,
+    LUCEM_PAD_CONSTANT = 1,
+    LUCEM_PAD_NEAREST = 2,
+    LUCEM_PAD_MIRROR = 3,
+    LUCEM_PAD_WRAP = 4
+};
+
+/* Largest padded FFT length per axis handled in one LDS-resident pass. */
+#define LUCEM_MAX_FFT_LEN 4096
+
+/* RichardsonLucy settings (deconvolve.rs:19-41). Defaults: num_iter 30, clip 1,
+ * has_filter_epsilon 0 (=> 1e-12), mode LUCEM_PAD_REFLECT. The epsilon used is
+ * max(filter_epsilon or 1e-12, 1e-15) rounded to f32 (deconvolve.rs:104). */
+typedef struct lucem_rl_settings {
+    uint64_t num_iter;
+    int32_t clip;
+    int32_t has_filter_epsilon;
+    double filter_epsilon;
+    uint32_t mode;
+    uint32_t reserved;
+} lucem_rl_settings;
+
+typedef struct lucem_client lucem_client;
+typedef struct lucem_workspace lucem_workspace;
+
+/* The message of the last failing call on this thread ("" if none). */
+const char* lucem_last_error(void);
+/* Library version string. */
+const char* lucem_version(void);
+
+/* Default settings, as RichardsonLucy::default(). */
+void lucem_rl_settings_default(lucem_rl_settings* out);
+/* PadMode::parse: names and aliases of core/pad.rs:48-57; LUCEM_UNKNOWN_PAD_MODE otherwise. */
+lucem_status lucem_pad_mode_parse(const char* name, uint32_t* mode_out);
+
+/* A client binds one device and one HIP stream. own_stream != 0 creates a stream owned by the
+ * client (default priority, `stream` ignored); otherwise the client enqueues on `stream` (a
+ * hipStream_t; NULL is the device's null stream). Not thread-safe: one client per thread of work. */
+lucem_status lucem_client_create(int device, void* stream, int32_t own_stream, lucem_client** out);
+lucem_status lucem_client_destroy(lucem_client* client);
+/* Waits for every operation queued on the client's stream. */
+lucem_status lucem_client_sync(lucem_client* client);
+/* The client's hipStream_t. */
+void* lucem_client_stream(lucem_client* client);
+/* Number of compute units of the client's device. */
+int lucem_client_num_cus(lucem_client* client);
+
+/* Grow-only device buffers reused across calls (Workspace semantics: one per thread of work,
+ * never shared between concurrent calls). */
+lucem_status lucem_workspace_create(lucem_client* client, lucem_workspace** out);
+lucem_status lucem_workspace_destroy(lucem_workspace* ws);
+uint64_t lucem_workspace_capacity_bytes(const lucem_workspace* ws);
+
+/* richardson_lucy_into::<f32>. input, psf and output are device pointers, C-contiguous.
+ * Checks, in order: same shape (input, output); no overlap of input and output; input ndim == 2;
+ * psf ndim == 2; psf non-empty; num_iter >= 1; empty input => LUCEM_OK with nothing done. */
+lucem_status lucem_richardson_lucy_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                            uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
+                                            uint32_t psf_ndim, float* output, const uint64_t* out_dims,
+                                            uint32_t out_ndim, const lucem_rl_settings* settings,
+                                            lucem_workspace* ws);
+
+/* richardson_lucy::<f32> with a caller-allocated output and the client's internal workspace. */
+lucem_status lucem_richardson_lucy_f32(lucem_client* client, const float* input, const uint64_t* dims,
+                                       uint32_t ndim, const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim,
+                                       float* output, const lucem_rl_settings* settings);
+
+/* The same operator for 2-D or 3-D arrays (psf of the same rank as the input). */
+lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                               uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
+                                               uint32_t psf_ndim, float* output, const uint64_t* out_dims,
+                                               uint32_t out_ndim, const lucem_rl_settings* settings,
+                                               lucem_workspace* ws);
+
+/* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
+ * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is
+ * unnormalised. L must be 7-smooth and <= LUCEM_MAX_FFT_LEN. The engine the RL passes use; no
+ * reference counterpart (the reference's naive DFT is crates/lucem-ops/src/core/fft.rs:13-51). */
+lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const float* src, float* dst, uint32_t L, uint64_t nlines,
+                                     int32_t inverse);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUCEM_HIP_H */

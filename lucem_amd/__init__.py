@@ -1,0 +1,21 @@
+"""MI355X-native (gfx950) Richardson-Lucy deconvolution: a drop-in for lucem-ops'
+``richardson_lucy_into`` behind a C ABI (``include/lucem_hip.h``), with HIP FFT kernels.
+
+Public surface (mirrors the reference's names):
+
+- :class:`PadMode`, :class:`RichardsonLucy`        crates/lucem-ops/src/core/pad.rs:13-57,
+                                                   restore/deconvolve.rs:19-41
+- :func:`richardson_lucy_into`, :func:`richardson_lucy`
+                                                   restore/deconvolve.rs:45-162
+- :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
+"""
+
+from .restore import (  # noqa: F401
+    PadMode,
+    RichardsonLucy,
+    richardson_lucy,
+    richardson_lucy_into,
+    richardson_lucy_nd,
+    richardson_lucy_nd_into,
+)
+from ._native import Client, LucemError, Workspace, version  # noqa: F401

@@ -1,0 +1,241 @@
+"""ctypes binding of ``liblucem_hip.so`` (the C ABI declared in ``include/lucem_hip.h``).
+
+This is the Python side of the drop-in boundary: arrays cross it as raw device pointers taken
+from torch tensors (or anything exporting DLPack, converted with ``torch.from_dlpack``). The
+library must be built (``python -m lucem_amd.build``); there is no fallback path, and a missing
+library or device raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "liblucem_hip.so"
+
+LUCEM_OK = 0
+STATUS_NAMES = {
+    0: "OK",
+    1: "SHAPE_MISMATCH",
+    2: "ALIASED_BUFFERS",
+    3: "INVALID_DIMENSION",
+    4: "INVALID_WEIGHTS",
+    5: "INVALID_PARAMETER",
+    6: "TOO_MANY_ELEMENTS",
+    7: "OUT_OF_DEVICE_MEMORY",
+    8: "UNKNOWN_PAD_MODE",
+    9: "UNSUPPORTED_SHAPE",
+    10: "HIP_ERROR",
+    11: "INVALID_HANDLE",
+}
+
+# Every symbol include/lucem_hip.h declares (checked by the CPU tests).
+EXPORTED = [
+    "lucem_last_error", "lucem_version", "lucem_rl_settings_default", "lucem_pad_mode_parse",
+    "lucem_client_create", "lucem_client_destroy", "lucem_client_sync", "lucem_client_stream",
+    "lucem_client_num_cus", "lucem_workspace_create", "lucem_workspace_destroy",
+    "lucem_workspace_capacity_bytes", "lucem_richardson_lucy_into_f32", "lucem_richardson_lucy_f32",
+    "lucem_richardson_lucy_nd_into_f32", "lucem_fft_c2c_lines_f32",
+]
+
+
+class LucemError(RuntimeError):
+    """A non-OK status of the C ABI (mirrors lucem_ops::OpsError variants)."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{self.kind}: {message}")
+
+
+class RlSettings(ctypes.Structure):
+    _fields_ = [
+        ("num_iter", ctypes.c_uint64),
+        ("clip", ctypes.c_int32),
+        ("has_filter_epsilon", ctypes.c_int32),
+        ("filter_epsilon", ctypes.c_double),
+        ("mode", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library. Raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH} is not built: run `python -m lucem_amd.build` (hipcc, gfx950)")
+            L = ctypes.CDLL(str(LIB_PATH))
+            L.lucem_last_error.restype = ctypes.c_char_p
+            L.lucem_version.restype = ctypes.c_char_p
+            L.lucem_rl_settings_default.argtypes = [ctypes.POINTER(RlSettings)]
+            L.lucem_pad_mode_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
+            L.lucem_client_create.argtypes = [ctypes.c_int, _vp, ctypes.c_int32, ctypes.POINTER(_vp)]
+            L.lucem_client_destroy.argtypes = [_vp]
+            L.lucem_client_sync.argtypes = [_vp]
+            L.lucem_client_stream.argtypes = [_vp]
+            L.lucem_client_stream.restype = _vp
+            L.lucem_client_num_cus.argtypes = [_vp]
+            L.lucem_workspace_create.argtypes = [_vp, ctypes.POINTER(_vp)]
+            L.lucem_workspace_destroy.argtypes = [_vp]
+            L.lucem_workspace_capacity_bytes.argtypes = [_vp]
+            L.lucem_workspace_capacity_bytes.restype = ctypes.c_uint64
+            rl = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
+                  ctypes.POINTER(RlSettings), _vp]
+            L.lucem_richardson_lucy_into_f32.argtypes = rl
+            L.lucem_richardson_lucy_nd_into_f32.argtypes = rl
+            L.lucem_richardson_lucy_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
+                                                    _vp, ctypes.POINTER(RlSettings)]
+            L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
+            _register_optional(L)
+            _lib = L
+        return _lib
+
+
+def _register_optional(L: ctypes.CDLL) -> None:
+    """Signatures of the batched and gradient entry points (bound in their modules)."""
+    from . import _native_ext
+
+    _native_ext.register(L)
+
+
+def check(status: int) -> None:
+    if status != LUCEM_OK:
+        msg = lib().lucem_last_error().decode(errors="replace")
+        raise LucemError(int(status), msg)
+
+
+def dims(shape) -> ctypes.Array:
+    shape = [int(s) for s in shape]
+    return (ctypes.c_uint64 * max(len(shape), 1))(*shape)
+
+
+def version() -> str:
+    return lib().lucem_version().decode()
+
+
+def last_error() -> str:
+    return lib().lucem_last_error().decode(errors="replace")
+
+
+class Client:
+    """A device and a HIP stream (mirrors lucem_core::compute::Client).
+
+    By default the client runs on torch's current stream of ``device`` so that library work is
+    ordered with the torch operations that produce its inputs and consume its outputs.
+    """
+
+    def __init__(self, device: int = 0, stream: int | None = None, own_stream: bool = False):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("no HIP device is visible: lucem_amd runs on MI355X only")
+        if stream is None and not own_stream:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = _vp()
+        check(lib().lucem_client_create(int(device), _vp(stream or None), 1 if own_stream else 0, ctypes.byref(h)))
+        self._h = h
+        self.device = int(device)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def sync(self) -> None:
+        check(lib().lucem_client_sync(self._h))
+
+    @property
+    def stream(self) -> int:
+        return int(lib().lucem_client_stream(self._h) or 0)
+
+    @property
+    def num_cus(self) -> int:
+        return int(lib().lucem_client_num_cus(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().lucem_client_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Workspace:
+    """Grow-only device buffers reused across calls (mirrors lucem_core::memory::Workspace)."""
+
+    def __init__(self, client: Client):
+        h = _vp()
+        check(lib().lucem_workspace_create(client.handle, ctypes.byref(h)))
+        self._h = h
+        self._client = client
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def capacity_bytes(self) -> int:
+        return int(lib().lucem_workspace_capacity_bytes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().lucem_workspace_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_clients: dict[tuple[int, int], Client] = {}
+
+
+def default_client(device: int | None = None) -> Client:
+    """One client per (device, torch current stream)."""
+    import torch
+
+    dev = torch.cuda.current_device() if device is None else int(device)
+    key = (dev, int(torch.cuda.current_stream(dev).cuda_stream))
+    c = _default_clients.get(key)
+    if c is None:
+        c = Client(dev, key[1])
+        _default_clients[key] = c
+    return c
+
+
+def as_tensor(x):
+    """A torch tensor view of ``x`` (torch tensor or any DLPack exporter)."""
+    import torch
+
+    if isinstance(x, torch.Tensor):
+        return x
+    if hasattr(x, "__dlpack__"):
+        return torch.from_dlpack(x)
+    raise TypeError(f"expected a torch tensor or a DLPack-exporting array, got {type(x).__name__}")
+
+
+def device_ptr(t, what: str) -> int:
+    """Raw device pointer of a contiguous float32 HIP tensor."""
+    import torch
+
+    if not t.is_cuda:
+        raise ValueError(f"{what} must be a device (HIP) tensor")
+    if t.dtype != torch.float32:
+        raise ValueError(f"{what} must be float32, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be C-contiguous")
+    return int(t.data_ptr())

@@ -1,0 +1,217 @@
+// LDS-resident mixed-radix FFT for gfx950 (wave64, 64 KiB of LDS per workgroup).
+//
+// B lines of length N are stored interleaved in LDS, element i of line b at buf[i*B + b], so
+// consecutive lanes touch consecutive LDS words in every pass. Each Stockham pass of radix R
+// loads all of its butterfly inputs into registers, waits at a barrier, then computes and writes
+// in place: a pass needs N*B/threads complex values per thread of registers and no second LDS
+// buffer. Twiddles come from a table tw[m] = exp(-2*pi*i*m/N) built on the host in double
+// precision and rounded once to float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rl_internal.hpp"
+
+namespace lucem {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// -i * a (forward) or +i * a (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// Radix butterflies: v[s] <- sum_t v[t] * exp(-+2*pi*i*t*s/R).
+template <int R, bool INV>
+struct Butterfly;
+
+template <bool INV>
+struct Butterfly<2, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV>
+struct Butterfly<4, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <bool INV>
+struct Butterfly<8, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float c = 0.7071067811865476f;  // sqrt(1/2)
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Butterfly<4, INV>::run(e);
+        Butterfly<4, INV>::run(o);
+        // o[s] *= exp(-+2*pi*i*s/8)
+        const float s1 = INV ? c : -c;
+        o[1] = make_float2(c * (o[1].x) - s1 * (o[1].y), c * (o[1].y) + s1 * (o[1].x));
+        o[2] = mul_mi<INV>(o[2]);
+        o[3] = make_float2(-c * (o[3].x) - s1 * (o[3].y), -c * (o[3].y) + s1 * (o[3].x));
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            v[s] = cadd(e[s], o[s]);
+            v[s + 4] = csub(e[s], o[s]);
+        }
+    }
+};
+
+template <bool INV>
+struct Butterfly<3, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float s3 = 0.8660254037844387f;  // sin(2*pi/3)
+        const float2 t = cadd(v[1], v[2]);
+        const float2 d = cscale(mul_mi<INV>(csub(v[1], v[2])), s3);
+        const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, d);
+        v[2] = csub(m, d);
+    }
+};
+
+template <bool INV>
+struct Butterfly<5, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float c1 = 0.30901699437494745f, c2 = -0.8090169943749473f;  // cos(2pi/5), cos(4pi/5)
+        constexpr float s1 = 0.9510565162951535f, s2 = 0.5877852522924732f;    // sin(2pi/5), sin(4pi/5)
+        const float2 a0 = v[0];
+        const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        const float2 m1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+        const float2 m2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+        const float2 n1 = mul_mi<INV>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+        const float2 n2 = mul_mi<INV>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+        v[0] = cadd(a0, cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+};
+
+template <bool INV>
+struct Butterfly<7, INV> {
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float C1 = 0.6234898018587336f, C2 = -0.22252093395631434f, C3 = -0.900968867902419f;
+        constexpr float S1 = 0.7818314824680298f, S2 = 0.9749279121818236f, S3 = 0.43388373911755823f;
+        const float2 a0 = v[0];
+        const float2 t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+        const float2 d1 = csub(v[1], v[6]), d2 = csub(v[2], v[5]), d3 = csub(v[3], v[4]);
+        const float2 m1 = make_float2(a0.x + C1 * t1.x + C2 * t2.x + C3 * t3.x, a0.y + C1 * t1.y + C2 * t2.y + C3 * t3.y);
+        const float2 m2 = make_float2(a0.x + C2 * t1.x + C3 * t2.x + C1 * t3.x, a0.y + C2 * t1.y + C3 * t2.y + C1 * t3.y);
+        const float2 m3 = make_float2(a0.x + C3 * t1.x + C1 * t2.x + C2 * t3.x, a0.y + C3 * t1.y + C1 * t2.y + C2 * t3.y);
+        const float2 n1 = mul_mi<INV>(make_float2(S1 * d1.x + S2 * d2.x + S3 * d3.x, S1 * d1.y + S2 * d2.y + S3 * d3.y));
+        const float2 n2 = mul_mi<INV>(make_float2(S2 * d1.x - S3 * d2.x - S1 * d3.x, S2 * d1.y - S3 * d2.y - S1 * d3.y));
+        const float2 n3 = mul_mi<INV>(make_float2(S3 * d1.x - S1 * d2.x + S2 * d3.x, S3 * d1.y - S1 * d2.y + S2 * d3.y));
+        v[0] = cadd(a0, cadd(t1, cadd(t2, t3)));
+        v[1] = cadd(m1, n1);
+        v[6] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[5] = csub(m2, n2);
+        v[3] = cadd(m3, n3);
+        v[4] = csub(m3, n3);
+    }
+};
+
+// One in-place Stockham pass of radix R over B = 1 << lgB interleaved lines of length N.
+// Input element j + t*N/R of each butterfly j is multiplied by w^(t*k), k = j mod Ns,
+// w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns.
+template <int R, bool INV, int T, int E>
+__device__ __forceinline__ void stockham_pass(float2* __restrict__ buf, int N, int lgB, int Ns,
+                                              const float2* __restrict__ tw) {
+    constexpr int S = (E + R - 1) / R;
+    const int B = 1 << lgB;
+    const int NR = N / R;
+    const int nbf = NR << lgB;
+    const int tws = N / (Ns * R);
+    float2 v[S][R];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u = threadIdx.x + s * T;
+        if (u < nbf) {
+            const int b = u & (B - 1), j = u >> lgB;
+#pragma unroll
+            for (int t = 0; t < R; t++) v[s][t] = buf[((j + t * NR) << lgB) + b];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u = threadIdx.x + s * T;
+        if (u < nbf) {
+            const int b = u & (B - 1), j = u >> lgB;
+            const int k = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int t = 1; t < R; t++) {
+                    float2 w = tw[t * k * tws];
+                    if (INV) w.y = -w.y;
+                    v[s][t] = cmul(v[s][t], w);
+                }
+            }
+            Butterfly<R, INV>::run(v[s]);
+            const int o = (j - k) * R + k;
+#pragma unroll
+            for (int t = 0; t < R; t++) buf[((o + t * Ns) << lgB) + b] = v[s][t];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform of the interleaved lines. The caller has written buf and synchronised.
+template <bool INV, int T, int E>
+__device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, const float2* __restrict__ twbase) {
+    const float2* tw = twbase + p.tw;
+    const int N = p.L;
+    int Ns = 1;
+    for (int q = 0; q < p.npass; q++) {
+        const int R = p.rad[q];
+        switch (R) {
+            case 8: stockham_pass<8, INV, T, E>(buf, N, lgB, Ns, tw); break;
+            case 4: stockham_pass<4, INV, T, E>(buf, N, lgB, Ns, tw); break;
+            case 2: stockham_pass<2, INV, T, E>(buf, N, lgB, Ns, tw); break;
+            case 3: stockham_pass<3, INV, T, E>(buf, N, lgB, Ns, tw); break;
+            case 5: stockham_pass<5, INV, T, E>(buf, N, lgB, Ns, tw); break;
+            default: stockham_pass<7, INV, T, E>(buf, N, lgB, Ns, tw); break;
+        }
+        Ns *= R;
+    }
+}
+
+// Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len).
+__device__ __forceinline__ int pad_index(int p, int len, int mode) {
+    if (p >= 0 && p < len) return p;
+    switch (mode) {
+        case 1: return -1;
+        case 2: return p < 0 ? 0 : len - 1;
+        case 4: return ((p % len) + len) % len;
+        case 0: {
+            const int period = 2 * len;
+            const int r = ((p % period) + period) % period;
+            return r < len ? r : period - 1 - r;
+        }
+        default: {
+            const int period = max(2 * len - 2, 1);
+            const int r = ((p % period) + period) % period;
+            return r < len ? r : period - r;
+        }
+    }
+}
+
+}  // namespace lucem

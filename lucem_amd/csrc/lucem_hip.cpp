@@ -1,0 +1,666 @@
+// C ABI of the MI355X Richardson-Lucy library: client (device + stream), grow-only workspace,
+// FFT planning and the plain 2-D / N-D entry points.
+//
+// Host-side contract mirrored from the reference:
+//   richardson_lucy_into   crates/lucem-ops/src/restore/deconvolve.rs:45-149 (check order :53-71)
+//   richardson_lucy        deconvolve.rs:152-162
+//   Workspace              crates/lucem-core/src/memory/workspace.rs:17-37
+//   PadMode::parse         crates/lucem-ops/src/core/pad.rs:48-57
+//   element limit          crates/lucem-core/src/compute/limits.rs:11-22
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lucem_hip.h"
+#include "lucem_internal_host.hpp"
+#include "rl_internal.hpp"
+
+using namespace lucem;
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+lucem_status lucem::fail(lucem_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+lucem_status lucem::hip_fail(int err, const char* what) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString((hipError_t)err));
+    if (err == hipErrorOutOfMemory || err == hipErrorMemoryAllocation)
+        return fail(LUCEM_OUT_OF_DEVICE_MEMORY, buf);
+    return fail(LUCEM_HIP_ERROR, buf);
+}
+
+std::string lucem::dims_str(const uint64_t* dims, uint32_t ndim) {
+    std::string s = "[";
+    for (uint32_t i = 0; i < ndim; i++) {
+        if (i) s += ", ";
+        s += std::to_string(dims ? dims[i] : 0);
+    }
+    return s + "]";
+}
+
+extern "C" const char* lucem_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* lucem_version(void) { return "lucem-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" void lucem_rl_settings_default(lucem_rl_settings* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof *out);
+    out->num_iter = 30;
+    out->clip = 1;
+    out->has_filter_epsilon = 0;
+    out->filter_epsilon = 1e-12;
+    out->mode = LUCEM_PAD_REFLECT;
+}
+
+extern "C" lucem_status lucem_pad_mode_parse(const char* name, uint32_t* mode_out) {
+    if (!name || !mode_out) return fail(LUCEM_INVALID_HANDLE, "pad mode name or output is NULL");
+    std::string s(name);
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    s = a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+    for (auto& c : s) c = (char)std::tolower((unsigned char)c);
+    static const std::pair<const char*, uint32_t> table[] = {
+        {"reflect", 0},  {"symmetric", 0}, {"grid-mirror", 0}, {"constant", 1}, {"grid-constant", 1},
+        {"zeros", 1},    {"nearest", 2},   {"edge", 2},        {"replicate", 2}, {"mirror", 3},
+        {"wrap", 4},     {"grid-wrap", 4}, {"periodic", 4},
+    };
+    for (const auto& e : table)
+        if (s == e.first) {
+            *mode_out = e.second;
+            return LUCEM_OK;
+        }
+    return fail(LUCEM_UNKNOWN_PAD_MODE, "boundary pad mode '" + s +
+                                            "' is not supported (expected reflect, constant, nearest, mirror, or wrap)");
+}
+
+// ---------------------------------------------------------------------------------------------
+// client
+// ---------------------------------------------------------------------------------------------
+extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t own_stream, lucem_client** out) {
+    if (!out) return fail(LUCEM_INVALID_HANDLE, "client output pointer is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(LUCEM_HIP_ERROR, "no HIP device is available");
+    if (device < 0 || device >= count) return fail(LUCEM_INVALID_PARAMETER, "device index out of range");
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto c = std::make_unique<lucem_client>();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    c->num_cus = prop.multiProcessorCount;
+    if (!own_stream) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+            return hip_fail(e, "hipStreamCreate");
+        c->own_stream = true;
+    }
+    c->internal_ws = std::make_unique<lucem_workspace>();
+    c->internal_ws->client = c.get();
+    *out = c.release();
+    return LUCEM_OK;
+}
+
+lucem_client::~lucem_client() {
+    (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(stream);
+    internal_ws.reset();
+    for (auto& kv : twiddles) (void)hipFree(kv.second);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" lucem_status lucem_client_destroy(lucem_client* client) {
+    delete client;
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_client_sync(lucem_client* client) {
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    hipError_t e = hipStreamSynchronize(client->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return LUCEM_OK;
+}
+
+extern "C" void* lucem_client_stream(lucem_client* client) { return client ? (void*)client->stream : nullptr; }
+extern "C" int lucem_client_num_cus(lucem_client* client) { return client ? client->num_cus : 0; }
+
+// Twiddle table exp(-2 pi i m / L), m < L, computed in double and rounded once to float.
+lucem_status lucem::twiddles_for(lucem_client* c, int L, const float2** out) {
+    auto it = c->twiddles.find(L);
+    if (it != c->twiddles.end()) {
+        *out = it->second;
+        return LUCEM_OK;
+    }
+    std::vector<float2> host(L);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int m = 0; m < L; m++) {
+        const double a = two_pi * (double)m / (double)L;
+        host[m] = make_float2((float)std::cos(a), (float)(-std::sin(a)));
+    }
+    float2* dev = nullptr;
+    hipError_t e = hipMalloc(&dev, sizeof(float2) * (size_t)std::max(L, 1));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
+    e = hipMemcpy(dev, host.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return hip_fail(e, "hipMemcpy(twiddles)");
+    }
+    c->twiddles[L] = dev;
+    *out = dev;
+    return LUCEM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------------------------
+lucem_workspace::~lucem_workspace() {
+    if (client) (void)hipStreamSynchronize(client->stream);
+    for (auto& kv : slots) (void)hipFree(kv.second.ptr);
+}
+
+lucem_status lucem::ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out) {
+    Slot& s = ws->slots[name];
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            // the old buffer may still be read by queued work
+            (void)hipStreamSynchronize(ws->client->stream);
+            (void)hipFree(s.ptr);
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "workspace slot '%s' of %zu bytes", name, bytes);
+            return hip_fail(e, buf);
+        }
+        s.ptr = p;
+        s.bytes = bytes;
+    }
+    *out = s.ptr;
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_workspace_create(lucem_client* client, lucem_workspace** out) {
+    if (!client || !out) return fail(LUCEM_INVALID_HANDLE, "client or workspace output is NULL");
+    auto w = std::make_unique<lucem_workspace>();
+    w->client = client;
+    *out = w.release();
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_workspace_destroy(lucem_workspace* ws) {
+    delete ws;
+    return LUCEM_OK;
+}
+
+extern "C" uint64_t lucem_workspace_capacity_bytes(const lucem_workspace* ws) {
+    if (!ws) return 0;
+    uint64_t total = 0;
+    for (const auto& kv : ws->slots) total += kv.second.bytes;
+    return total;
+}
+
+// ---------------------------------------------------------------------------------------------
+// planning
+// ---------------------------------------------------------------------------------------------
+static bool smooth7(int v) {
+    for (int p : {2, 3, 5, 7})
+        while (v % p == 0) v /= p;
+    return v == 1;
+}
+
+int lucem::choose_fft_len(int target) {
+    int L = std::max(target, 1);
+    while (!smooth7(L)) L++;
+    return L;
+}
+
+FftPlan lucem::make_plan(int L) {
+    FftPlan p{};
+    p.L = L;
+    p.npass = 0;
+    p.tw = 0;
+    int r = L;
+    while (r % 8 == 0 && r > 1) {
+        p.rad[p.npass++] = 8;
+        r /= 8;
+    }
+    while (r % 4 == 0) {
+        p.rad[p.npass++] = 4;
+        r /= 4;
+    }
+    while (r % 2 == 0) {
+        p.rad[p.npass++] = 2;
+        r /= 2;
+    }
+    for (int q : {3, 5, 7})
+        while (r % q == 0) {
+            p.rad[p.npass++] = (unsigned char)q;
+            r /= q;
+        }
+    return p;
+}
+
+static int ilog2(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v) l++;
+    return l;
+}
+
+int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
+    const int pairs = (rows + 1) / 2;
+    int lgP = 3;
+    while (lgP > 0 && ((size_t)((1 << lgP) * (L + nx)) > (size_t)kLdsComplex)) lgP--;
+    while (lgP > 0 && (pairs >> lgP) < 2 * num_cus) lgP--;
+    return lgP;
+}
+
+int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus) {
+    int lgB = 4;
+    while (lgB > 0 && ((size_t)L << lgB) > (size_t)kLdsComplex) lgB--;
+    while (lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 2 * num_cus) lgB--;
+    return lgB;
+}
+
+bool lucem::make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g) {
+    std::memset(g, 0, sizeof *g);
+    g->ndim = ndim;
+    for (int a = 0; a < 3; a++) {
+        g->n[a] = (int)n[a];
+        g->k[a] = (int)k[a];
+        const int64_t need = n[a] + k[a] - 1;
+        if (need > LUCEM_MAX_FFT_LEN) return false;
+        g->L[a] = (ndim == 2 && a == 0) ? 1 : choose_fft_len((int)need);
+        if (g->L[a] > LUCEM_MAX_FFT_LEN) return false;
+    }
+    g->Lc = g->L[2] / 2 + 1;
+    g->pitch = (g->Lc + 15) & ~15;
+    // the x pass must hold L_x + n_x complex-equivalents for one row pair
+    if (g->L[2] + g->n[2] > kLdsComplex) return false;
+    if (g->L[2] + g->k[2] > kLdsComplex) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// RL driver for one array (2-D or 3-D)
+// ---------------------------------------------------------------------------------------------
+static lucem_status launch_check(int err, const char* what) {
+    if (err != hipSuccess) return hip_fail(err, what);
+    return LUCEM_OK;
+}
+
+// Forward-transform a real array (dims m) into the full spectrum `dst` ([L_z][L_y][pitch]) with
+// Constant padding and zero left offset (zero-padding of a kernel), storing conj(v)*scale.
+static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3], const float* w, float2* tmp_a,
+                              float2* tmp_b, float2* otf, const float2* const tws[3]) {
+    const float scale = (float)(0.5 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]));
+    RowPass r{};
+    r.op = ROW_LOAD;
+    r.nx = m[2];
+    r.rows = m[0] * m[1];
+    r.left = 0;
+    r.mode = LUCEM_PAD_CONSTANT;
+    r.Lc = g.Lc;
+    r.pitch = g.pitch;
+    r.lgP = choose_lgP(g.L[2], m[2], r.rows, c->num_cus);
+    r.f = make_plan(g.L[2]);
+    r.src = w;
+    r.spec = tmp_a;
+    r.tw = tws[2];
+    const int pairs = (r.rows + 1) / 2;
+    lucem_status st = launch_check(launch_row_pass(r, (pairs + (1 << r.lgP) - 1) >> r.lgP, c->stream), "row pass");
+    if (st) return st;
+
+    ColPass y{};
+    y.op = COL_FWD;
+    y.n_src = m[1];
+    y.left = 0;
+    y.mode = LUCEM_PAD_CONSTANT;
+    y.n_dst = g.L[1];
+    y.Lc = g.Lc;
+    y.f = make_plan(g.L[1]);
+    y.tw = tws[1];
+    y.src = tmp_a;
+    y.src_outer = (int64_t)m[1] * g.pitch;
+    y.src_axis = g.pitch;
+    if (g.ndim == 2) {
+        y.finalize = 1;
+        y.scale = scale;
+        y.dst = otf;
+        y.dst_outer = 0;
+        y.dst_axis = g.pitch;
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
+        y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+        return launch_check(launch_col_pass(y, y.nkb, c->stream), "line pass (y)");
+    }
+    y.dst = tmp_b;
+    y.dst_outer = (int64_t)g.L[1] * g.pitch;
+    y.dst_axis = g.pitch;
+    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], c->num_cus);
+    y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+    st = launch_check(launch_col_pass(y, m[0] * y.nkb, c->stream), "line pass (y)");
+    if (st) return st;
+
+    ColPass z{};
+    z.op = COL_FWD;
+    z.n_src = m[0];
+    z.left = 0;
+    z.mode = LUCEM_PAD_CONSTANT;
+    z.n_dst = g.L[0];
+    z.Lc = g.Lc;
+    z.f = make_plan(g.L[0]);
+    z.tw = tws[0];
+    z.src = tmp_b;
+    z.src_outer = g.pitch;
+    z.src_axis = (int64_t)g.L[1] * g.pitch;
+    z.dst = otf;
+    z.dst_outer = g.pitch;
+    z.dst_axis = (int64_t)g.L[1] * g.pitch;
+    z.finalize = 1;
+    z.scale = scale;
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
+    z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
+    return launch_check(launch_col_pass(z, g.L[1] * z.nkb, c->stream), "line pass (z)");
+}
+
+// Runs the line passes of one correlation over spec_a (and spec_b in 3-D) with transfer `otf`.
+static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* sa, float2* sb, const float2* otf,
+                                    const float2* const tws[3]) {
+    ColPass y{};
+    y.Lc = g.Lc;
+    y.f = make_plan(g.L[1]);
+    y.tw = tws[1];
+    y.left = g.k[1] / 2;
+    y.mode = g.mode;
+    if (g.ndim == 2) {
+        y.op = COL_FUSED;
+        y.n_src = g.n[1];
+        y.n_dst = g.n[1];
+        y.src = sa;
+        y.dst = sa;
+        y.src_outer = y.dst_outer = 0;
+        y.src_axis = y.dst_axis = g.pitch;
+        y.otf = otf;
+        y.otf_outer = 0;
+        y.otf_axis = g.pitch;
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
+        y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+        return launch_check(launch_col_pass(y, y.nkb, c->stream), "line pass (y)");
+    }
+    // y forward: spec_a [nz][ny] -> spec_b [nz][Ly]
+    y.op = COL_FWD;
+    y.n_src = g.n[1];
+    y.n_dst = g.L[1];
+    y.src = sa;
+    y.src_outer = (int64_t)g.n[1] * g.pitch;
+    y.src_axis = g.pitch;
+    y.dst = sb;
+    y.dst_outer = (int64_t)g.L[1] * g.pitch;
+    y.dst_axis = g.pitch;
+    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], c->num_cus);
+    y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+    lucem_status st = launch_check(launch_col_pass(y, g.n[0] * y.nkb, c->stream), "line pass (y)");
+    if (st) return st;
+    // z fused, in place on spec_b
+    ColPass z{};
+    z.op = COL_FUSED;
+    z.Lc = g.Lc;
+    z.f = make_plan(g.L[0]);
+    z.tw = tws[0];
+    z.left = g.k[0] / 2;
+    z.mode = g.mode;
+    z.n_src = g.n[0];
+    z.n_dst = g.n[0];
+    z.src = sb;
+    z.dst = sb;
+    z.src_outer = z.dst_outer = g.pitch;
+    z.src_axis = z.dst_axis = (int64_t)g.L[1] * g.pitch;
+    z.otf = otf;
+    z.otf_outer = g.pitch;
+    z.otf_axis = (int64_t)g.L[1] * g.pitch;
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
+    z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
+    st = launch_check(launch_col_pass(z, g.L[1] * z.nkb, c->stream), "line pass (z)");
+    if (st) return st;
+    // y inverse: spec_b [nz][Ly] -> spec_a [nz][ny]
+    ColPass yi = y;
+    yi.op = COL_INV;
+    yi.n_src = g.L[1];
+    yi.n_dst = g.n[1];
+    yi.src = sb;
+    yi.src_outer = (int64_t)g.L[1] * g.pitch;
+    yi.dst = sa;
+    yi.dst_outer = (int64_t)g.n[1] * g.pitch;
+    return launch_check(launch_col_pass(yi, g.n[0] * yi.nkb, c->stream), "line pass (y inverse)");
+}
+
+lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
+                           const Geometry& g, const lucem_rl_settings& s) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t nk = (int64_t)g.k[0] * g.k[1] * g.k[2];
+    const int64_t rows = (int64_t)g.n[0] * g.n[1];
+    const int64_t krows = (int64_t)g.k[0] * g.k[1];
+    const size_t c8 = sizeof(float2);
+    float *est, *h, *rev;
+    float2 *sa, *sb = nullptr, *otf1, *otf2, *pa, *pb = nullptr;
+    lucem_status st;
+    if ((st = ws_slot(ws, "rl_est", sizeof(float) * n, (void**)&est))) return st;
+    if ((st = ws_slot(ws, "rl_spec_a", c8 * rows * g.pitch, (void**)&sa))) return st;
+    if (g.ndim == 3 && (st = ws_slot(ws, "rl_spec_b", c8 * (int64_t)g.n[0] * g.L[1] * g.pitch, (void**)&sb)))
+        return st;
+    const int64_t otf_elems = (int64_t)g.L[0] * g.L[1] * g.pitch;
+    if ((st = ws_slot(ws, "rl_otf_conv", c8 * otf_elems, (void**)&otf1))) return st;
+    if ((st = ws_slot(ws, "rl_otf_mirror", c8 * otf_elems, (void**)&otf2))) return st;
+    if ((st = ws_slot(ws, "rl_psf_h", sizeof(float) * nk, (void**)&h))) return st;
+    if ((st = ws_slot(ws, "rl_psf_rev", sizeof(float) * nk, (void**)&rev))) return st;
+    if ((st = ws_slot(ws, "rl_psf_spec_a", c8 * krows * g.pitch, (void**)&pa))) return st;
+    if (g.ndim == 3 &&
+        (st = ws_slot(ws, "rl_psf_spec_b", c8 * (int64_t)g.k[0] * g.L[1] * g.pitch, (void**)&pb)))
+        return st;
+    const float2* tws[3] = {nullptr, nullptr, nullptr};
+    for (int a = (g.ndim == 2 ? 1 : 0); a < 3; a++)
+        if ((st = twiddles_for(c, g.L[a], &tws[a]))) return st;
+
+    // PSF normalisation and flip (deconvolve.rs:73-90), then the two transfer functions:
+    // otf_conv for correlate(est, rev) (:120), otf_mirror for correlate(rel, h) (:133).
+    PsfPrep pp{nk, psf, h, rev};
+    if ((st = launch_check(launch_psf_prep(pp, c->stream), "psf prep"))) return st;
+    const int m[3] = {g.k[0], g.k[1], g.k[2]};
+    if ((st = build_otf(c, g, m, rev, pa, pb, otf1, tws))) return st;
+    if ((st = build_otf(c, g, m, h, pa, pb, otf2, tws))) return st;
+
+    const double e64 = std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
+    RowPass r{};
+    r.nx = g.n[2];
+    r.rows = (int)rows;
+    r.left = g.k[2] / 2;
+    r.mode = g.mode;
+    r.Lc = g.Lc;
+    r.pitch = g.pitch;
+    r.lgP = choose_lgP(g.L[2], g.n[2], (int)rows, c->num_cus);
+    r.eps = (float)e64;
+    r.init = 0.5f;
+    r.f = make_plan(g.L[2]);
+    r.obs = input;
+    r.est = est;
+    r.out = output;
+    r.spec = sa;
+    r.tw = tws[2];
+    const int row_blocks = (int)((((rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP);
+
+    // est_0 = 0.5 everywhere (deconvolve.rs:93): transform the constant directly.
+    r.op = ROW_LOAD_CONST;
+    if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+    for (uint64_t it = 0; it < s.num_iter; it++) {
+        const bool last = it + 1 == s.num_iter;
+        if ((st = correlate_lines(c, g, sa, sb, otf1, tws))) return st;
+        r.op = ROW_RATIO;
+        if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+        if ((st = correlate_lines(c, g, sa, sb, otf2, tws))) return st;
+        r.op = ROW_UPDATE;
+        r.first = it == 0;
+        r.last = last;
+        r.clip = s.clip ? 1 : 0;
+        if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+    }
+    return LUCEM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// validation in the reference's order (deconvolve.rs:53-71)
+// ---------------------------------------------------------------------------------------------
+lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
+                                const uint64_t* psf_dims, uint32_t psf_ndim, const float* output,
+                                const uint64_t* out_dims, uint32_t out_ndim, const lucem_rl_settings* s,
+                                uint32_t max_ndim, bool* empty, Geometry* g) {
+    (void)psf;
+    *empty = false;
+    if (!s) return fail(LUCEM_INVALID_HANDLE, "settings is NULL");
+    if ((in_ndim && !in_dims) || (out_ndim && !out_dims) || (psf_ndim && !psf_dims))
+        return fail(LUCEM_INVALID_HANDLE, "dims pointer is NULL");
+    bool same = in_ndim == out_ndim;
+    for (uint32_t i = 0; same && i < in_ndim; i++) same = in_dims[i] == out_dims[i];
+    if (!same)
+        return fail(LUCEM_SHAPE_MISMATCH, "input shape " + dims_str(in_dims, in_ndim) + " does not match output shape " +
+                                              dims_str(out_dims, out_ndim));
+    uint64_t n = 1;
+    bool overflow = false;
+    for (uint32_t i = 0; i < in_ndim; i++) {
+        if (in_dims[i] && n > UINT64_MAX / in_dims[i]) overflow = true;
+        n *= in_dims[i];
+    }
+    if (n > 0 && !overflow) {
+        const char* a = (const char*)input;
+        const char* b = (const char*)output;
+        const uint64_t bytes = n * sizeof(float);
+        if (a < b + bytes && b < a + bytes)
+            return fail(LUCEM_ALIASED_BUFFERS, "input and output arrays must not share the same device buffer");
+    }
+    const std::string expected = max_ndim == 2 ? "2" : "2 or 3";
+    if (in_ndim < 2 || in_ndim > max_ndim)
+        return fail(LUCEM_INVALID_DIMENSION, "richardson_lucy input expects " + expected + "-D array, got " +
+                                                 std::to_string(in_ndim) + "-D shape " + dims_str(in_dims, in_ndim));
+    if (psf_ndim < 2 || psf_ndim > max_ndim || (max_ndim > 2 && psf_ndim != in_ndim))
+        return fail(LUCEM_INVALID_DIMENSION, "richardson_lucy psf expects " +
+                                                 (max_ndim == 2 ? expected : std::to_string(in_ndim)) +
+                                                 "-D array, got " + std::to_string(psf_ndim) + "-D shape " +
+                                                 dims_str(psf_dims, psf_ndim));
+    uint64_t nk = 1;
+    for (uint32_t i = 0; i < psf_ndim; i++) nk *= psf_dims[i];
+    if (nk == 0) return fail(LUCEM_INVALID_WEIGHTS, "psf: PSF cannot be empty");
+    if (s->num_iter == 0) return fail(LUCEM_INVALID_PARAMETER, "invalid parameter `num_iter`: num_iter must be >= 1");
+    if (s->mode > 4)
+        return fail(LUCEM_UNKNOWN_PAD_MODE, "boundary pad mode id " + std::to_string(s->mode) +
+                                                " is not supported (expected 0..4: reflect, constant, nearest, "
+                                                "mirror, wrap)");
+    if (n == 0) {
+        *empty = true;
+        return LUCEM_OK;
+    }
+    if (overflow || n > 0xFFFFFFFFull)
+        return fail(LUCEM_TOO_MANY_ELEMENTS, "rl_est: dims " + dims_str(in_dims, in_ndim) +
+                                                 " exceed the device element limit of 4294967295");
+    int64_t n3[3] = {1, 1, 1}, k3[3] = {1, 1, 1};
+    for (uint32_t i = 0; i < in_ndim; i++) n3[3 - in_ndim + i] = (int64_t)in_dims[i];
+    for (uint32_t i = 0; i < psf_ndim; i++) k3[3 - psf_ndim + i] = (int64_t)psf_dims[i];
+    if (!make_geometry(n3, k3, (int)in_ndim, g))
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with psf " +
+                                                 dims_str(psf_dims, psf_ndim) +
+                                                 ": an axis needs an FFT longer than 4096 (n + k - 1 per axis)");
+    g->mode = (int)s->mode;
+    return LUCEM_OK;
+}
+
+static lucem_status rl_entry(lucem_client* client, const float* input, const uint64_t* in_dims, uint32_t in_ndim,
+                             const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim, float* output,
+                             const uint64_t* out_dims, uint32_t out_ndim, const lucem_rl_settings* settings,
+                             lucem_workspace* ws, uint32_t max_ndim) {
+    // Argument checks come first and touch no device memory, so their order can be tested
+    // without a device (and without a client).
+    bool empty = false;
+    Geometry g;
+    lucem_status st = validate_rl(input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim,
+                                  settings, max_ndim, &empty, &g);
+    if (st != LUCEM_OK || empty) return st;
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    if (!input || !psf || !output) return fail(LUCEM_INVALID_HANDLE, "input, psf or output pointer is NULL");
+    if (!ws) ws = client->internal_ws.get();
+    if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
+    return rl_run(client, ws, input, psf, output, g, *settings);
+}
+
+extern "C" lucem_status lucem_richardson_lucy_into_f32(lucem_client* client, const float* input,
+                                                       const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
+                                                       const uint64_t* psf_dims, uint32_t psf_ndim, float* output,
+                                                       const uint64_t* out_dims, uint32_t out_ndim,
+                                                       const lucem_rl_settings* settings, lucem_workspace* ws) {
+    return rl_entry(client, input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim, settings,
+                    ws, 2);
+}
+
+extern "C" lucem_status lucem_richardson_lucy_f32(lucem_client* client, const float* input, const uint64_t* dims,
+                                                  uint32_t ndim, const float* psf, const uint64_t* psf_dims,
+                                                  uint32_t psf_ndim, float* output,
+                                                  const lucem_rl_settings* settings) {
+    return rl_entry(client, input, dims, ndim, psf, psf_dims, psf_ndim, output, dims, ndim, settings, nullptr, 2);
+}
+
+extern "C" lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input,
+                                                          const uint64_t* in_dims, uint32_t in_ndim,
+                                                          const float* psf, const uint64_t* psf_dims,
+                                                          uint32_t psf_ndim, float* output, const uint64_t* out_dims,
+                                                          uint32_t out_ndim, const lucem_rl_settings* settings,
+                                                          lucem_workspace* ws) {
+    return rl_entry(client, input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim, settings,
+                    ws, 3);
+}
+
+// Batched complex line FFTs (contiguous lines of length L, interleaved re/im).
+extern "C" lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const float* src, float* dst, uint32_t L,
+                                                uint64_t nlines, int32_t inverse) {
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    if (L == 0 || nlines == 0) return LUCEM_OK;
+    if (L > LUCEM_MAX_FFT_LEN || !smooth7((int)L))
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "FFT length must be 7-smooth and at most 4096");
+    if (nlines > 0x7FFFFFFFull) return fail(LUCEM_TOO_MANY_ELEMENTS, "too many lines");
+    hipError_t e = hipSetDevice(client->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const float2* tw;
+    lucem_status st = twiddles_for(client, (int)L, &tw);
+    if (st) return st;
+    ColPass p{};
+    p.op = inverse ? COL_INV : COL_FWD;
+    p.n_src = (int)L;
+    p.left = 0;
+    p.mode = LUCEM_PAD_WRAP;
+    p.n_dst = (int)L;
+    p.Lc = 1;
+    p.lgB = 0;
+    p.nkb = 1;
+    p.src_outer = L;
+    p.src_axis = 1;
+    p.dst_outer = L;
+    p.dst_axis = 1;
+    p.f = make_plan((int)L);
+    p.tw = tw;
+    p.src = (const float2*)src;
+    p.dst = (float2*)dst;
+    return launch_check(launch_col_pass(p, (int)nlines, client->stream), "fft lines");
+}

@@ -1,0 +1,70 @@
+// Host-side internals shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "../../include/lucem_hip.h"
+#include "rl_internal.hpp"
+
+struct lucem_workspace;
+
+struct lucem_client {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::map<int, float2*> twiddles;               // FFT length -> device twiddle table
+    std::unique_ptr<lucem_workspace> internal_ws;  // used by the allocating-style entry points
+    ~lucem_client();
+};
+
+namespace lucem {
+struct Slot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+}  // namespace lucem
+
+struct lucem_workspace {
+    lucem_client* client = nullptr;
+    std::map<std::string, lucem::Slot> slots;
+    ~lucem_workspace();
+};
+
+namespace lucem {
+
+// Geometry of one RL array: sizes per axis (z, y, x), FFT lengths and half-spectrum layout.
+struct Geometry {
+    int ndim;
+    int n[3];
+    int k[3];
+    int L[3];
+    int Lc;
+    int pitch;
+    int mode;
+};
+
+lucem_status fail(lucem_status st, const std::string& msg);
+lucem_status hip_fail(int err, const char* what);
+std::string dims_str(const uint64_t* dims, uint32_t ndim);
+lucem_status twiddles_for(lucem_client* c, int L, const float2** out);
+lucem_status ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out);
+
+int choose_fft_len(int target);
+FftPlan make_plan(int L);
+int choose_lgP(int L, int nx, int rows, int num_cus);
+int choose_lgB(int L, int Lc, int64_t outers, int num_cus);
+bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g);
+
+lucem_status validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
+                         const uint64_t* psf_dims, uint32_t psf_ndim, const float* output, const uint64_t* out_dims,
+                         uint32_t out_ndim, const lucem_rl_settings* s, uint32_t max_ndim, bool* empty, Geometry* g);
+lucem_status rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
+                    const Geometry& g, const lucem_rl_settings& s);
+
+}  // namespace lucem

@@ -1,0 +1,98 @@
+// Internal interface between the C ABI (lucem_hip.cpp) and the HIP kernels (rl_kernels.hip).
+//
+// The deconvolution runs every correlation of the reference (filter/kernels/convolve.rs:61-118)
+// as an FFT linear correlation on a padded domain L >= n + k - 1 per axis:
+//   Q = P_mode(x) gathered on [0, L) with Q[i] = x[pad_index(i - k/2)],
+//   corr = IFFT( FFT(Q) * conj(FFT(w zero-padded to L)) ), cropped to [0, n).
+// Axis order is (z, y, x), x contiguous. The x axis is transformed two rows at a time
+// (z = row_a + i*row_b, one complex FFT of length L_x, then split into two half spectra of
+// L_x/2 + 1 bins); the other axes are complex line FFTs over those half spectra.
+#pragma once
+
+#include <cstdint>
+
+namespace lucem {
+
+constexpr int kMaxPasses = 16;
+constexpr int kThreads = 512;         // threads per workgroup of every FFT kernel
+constexpr int kElemsPerThread = 16;   // complex values held per thread in one radix pass
+constexpr int kLdsComplex = kThreads * kElemsPerThread;  // 8192 complex = 64 KiB of LDS
+
+// One FFT length and its Stockham radix sequence.
+struct FftPlan {
+    int L;
+    int npass;
+    int tw;                            // offset (in float2) of this length's twiddle table
+    unsigned char rad[kMaxPasses];
+};
+
+enum RowOp : int {
+    ROW_LOAD = 0,        // gather a real array, forward R2C
+    ROW_LOAD_CONST = 1,  // a constant array (the 0.5 start), forward R2C
+    ROW_RATIO = 2,       // C2R, ratio y / safe(c), forward R2C
+    ROW_UPDATE = 3,      // C2R, est' = est * c (clip on the last), forward R2C unless last
+};
+
+enum ColOp : int {
+    COL_FWD = 0,    // gather along the axis, forward FFT, store all L bins
+    COL_FUSED = 1,  // gather, forward FFT, multiply by the transfer function, inverse FFT, crop
+    COL_INV = 2,    // load L bins, inverse FFT, crop
+};
+
+// Row (x axis) pass over `rows` real rows of length nx, paired two by two.
+struct RowPass {
+    int op;
+    int nx, rows;
+    int left, mode;      // pad gather along x
+    int Lc, pitch;       // half-spectrum bins per row, row pitch (complex) of `spec`
+    int lgP;             // log2(row pairs per workgroup)
+    int first;           // ROW_UPDATE: the previous estimate is the constant `init`
+    int last;            // ROW_UPDATE: write `out`, no forward transform
+    int clip;            // ROW_UPDATE && last: clamp to [-1, 1]
+    float eps;           // ROW_RATIO: |d| < eps => d = eps
+    float init;          // ROW_LOAD_CONST / first ROW_UPDATE: the constant start value
+    FftPlan f;
+    const float* src;    // ROW_LOAD: the real rows
+    const float* obs;    // ROW_RATIO: observed image
+    float* est;          // ROW_UPDATE: estimate read (unless first) and written (unless last)
+    float* out;          // ROW_UPDATE && last: output
+    float2* spec;        // half spectra, read (C2R) and written (R2C) in place
+    const float2* tw;    // twiddle tables
+};
+
+// Line pass along the y or z axis of the half spectra.
+struct ColPass {
+    int op;
+    int n_src;            // rows gathered along the axis (FWD/FUSED) or L (INV)
+    int left, mode;       // gather along the axis (FWD/FUSED)
+    int n_dst;            // rows stored along the axis
+    int Lc;               // valid columns
+    int lgB;              // log2(columns per workgroup)
+    int nkb;              // column blocks per outer index
+    int finalize;         // COL_FWD: store conj(v) * scale (transfer function build)
+    float scale;
+    int64_t src_outer, src_axis, dst_outer, dst_axis, otf_outer, otf_axis;
+    FftPlan f;
+    const float2* src;
+    float2* dst;
+    const float2* otf;
+    const float2* tw;
+};
+
+// PSF preparation (deconvolve.rs:73-90 on the device): f64 sum, norm = 1/sum when |sum| > 1e-12
+// else 1, h = psf*norm rounded to f32, rev = 180-degree flip of h.
+struct PsfPrep {
+    int64_t nk;
+    const float* psf;
+    float* h;
+    float* rev;
+};
+
+// Launchers (rl_kernels.hip). Each returns the hipError_t of the launch as int.
+int launch_row_pass(const RowPass& p, int blocks, void* stream);
+int launch_col_pass(const ColPass& p, int blocks, void* stream);
+int launch_psf_prep(const PsfPrep& p, void* stream);
+size_t row_pass_lds_bytes(int L, int nx, int lgP);
+size_t col_pass_lds_bytes(int L, int lgB);
+
+}  // namespace lucem

@@ -1,0 +1,228 @@
+"""CPU oracle for lucem-ops Richardson-Lucy deconvolution (ctypes over ``liblucem_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg
+of ``bench.py`` load this module as the checker. The product path never imports it.
+
+Functions and the reference code each one restates (paths relative to the reference tree):
+
+- :func:`pad_index`           crates/lucem-ops/src/core/kernels/pad.rs:11-34
+- :func:`correlate`           crates/lucem-ops/src/filter/kernels/convolve.rs:61-118 and
+                              filter/convolve.rs:125-166 (N-D extension for 3-D)
+- :func:`richardson_lucy`     crates/lucem-ops/src/restore/deconvolve.rs:45-149 with
+                              restore/kernels/deconvolve.rs:7-43 (FP32 or FP64)
+- :func:`rl_fft64`            the same RL in FP64 with each correlation done as an FFT linear
+                              correlation of the padded array (for shapes too large for the direct
+                              FP64 oracle); pinned against the direct FP64 oracle on small shapes
+- :func:`rl_nll_grad64`       FP64 reverse-mode gradient of unrolled RL + Poisson NLL (no
+                              reference counterpart); pinned by central finite differences
+
+Parity pins (see tests/test_oracle.py): pad_index against the reference's own host restatement
+(crates/lucem-ops/src/test_support.rs:21-45, restated in the test) and numpy.pad; correlation
+against scipy.ndimage.correlate for every mode and the reference KAT pattern
+(filter/convolve.rs:211-228); RL against the reference's identity and box-PSF KATs
+(restore/deconvolve.rs:517-566), closed-form fixed points, brute-force FP64 and the FFT form.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "liblucem_oracle.so"
+_lib = None
+
+PAD_REFLECT, PAD_CONSTANT, PAD_NEAREST, PAD_MIRROR, PAD_WRAP = 0, 1, 2, 3, 4
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def build(force: bool = False) -> Path:
+    """Compiles the oracle with ``make`` (g++), in place."""
+    if force or not _LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(_HERE), "liblucem_oracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIB_PATH))
+        L.lo_pad_index.restype = ctypes.c_int64
+        L.lo_pad_index.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        L.lo_num_threads.restype = ctypes.c_int
+        L.lo_set_num_threads.argtypes = [ctypes.c_int]
+        L.lo_correlate_f32.argtypes = [_f32p, _i64p, ctypes.c_int, _f32p, _i64p, _i64p, ctypes.c_int,
+                                       ctypes.c_float, _f32p]
+        L.lo_correlate_f64.argtypes = [_f64p, _i64p, ctypes.c_int, _f64p, _i64p, _i64p, ctypes.c_int,
+                                       ctypes.c_double, _f64p]
+        rl_args = [_i64p, ctypes.c_int, None, _i64p, ctypes.c_int, None, _i64p, ctypes.c_int, ctypes.c_uint64,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.lo_rl_f32.restype = ctypes.c_int
+        L.lo_rl_f32.argtypes = [_f32p] + [(_f32p if a is None else a) for a in rl_args]
+        L.lo_rl_f64.restype = ctypes.c_int
+        L.lo_rl_f64.argtypes = [_f64p] + [(_f64p if a is None else a) for a in rl_args]
+        L.lo_rl_nll_grad_f64.restype = ctypes.c_double
+        L.lo_rl_nll_grad_f64.argtypes = [_f64p, _f64p, _i64p, ctypes.c_int, _f64p, _i64p, ctypes.c_uint64,
+                                         ctypes.c_double, ctypes.c_int, _f64p, _f64p]
+        L.lo_rl_nll_f64.restype = ctypes.c_double
+        L.lo_rl_nll_f64.argtypes = [_f64p, _f64p, _i64p, ctypes.c_int, _f64p, _i64p, ctypes.c_uint64,
+                                    ctypes.c_double, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def set_num_threads(n: int) -> None:
+    lib().lo_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().lo_num_threads())
+
+
+def _dims(shape) -> ctypes.Array:
+    return (ctypes.c_int64 * max(len(shape), 1))(*[int(s) for s in shape])
+
+
+def _ptr(a: np.ndarray, kind):
+    return a.ctypes.data_as(kind)
+
+
+def pad_index(p: int, n: int, mode: int) -> int:
+    """Reference ``pad_index`` (core/kernels/pad.rs:11-34); -1 for Constant outside ``[0, n)``."""
+    return int(lib().lo_pad_index(int(p), int(n), int(mode)))
+
+
+def correlate(x: np.ndarray, w: np.ndarray, mode: int = PAD_REFLECT, cval: float = 0.0, origin=None) -> np.ndarray:
+    """Direct correlation ``out[p] = sum_k w[k] * P_mode(x)[p - (k_dim//2 + origin) + k]``.
+
+    FP32 when ``x`` is float32 (sequential FP32 accumulation, as the reference kernel), else FP64.
+    """
+    f64 = x.dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    x = np.ascontiguousarray(x, dtype=dt)
+    w = np.ascontiguousarray(w, dtype=dt)
+    assert x.ndim == w.ndim and 1 <= x.ndim <= 3
+    out = np.empty_like(x)
+    org = None if origin is None else _dims(origin)
+    if f64:
+        lib().lo_correlate_f64(_ptr(x, _f64p), _dims(x.shape), x.ndim, _ptr(w, _f64p), _dims(w.shape), org, mode,
+                               float(cval), _ptr(out, _f64p))
+    else:
+        lib().lo_correlate_f32(_ptr(x, _f32p), _dims(x.shape), x.ndim, _ptr(w, _f32p), _dims(w.shape), org, mode,
+                               float(cval), _ptr(out, _f32p))
+    return out
+
+
+def richardson_lucy(image: np.ndarray, psf: np.ndarray, num_iter: int = 30, clip: bool = True,
+                    filter_epsilon: float | None = None, mode: int = PAD_REFLECT, nd: bool = False,
+                    out: np.ndarray | None = None) -> tuple[int, np.ndarray]:
+    """Reference ``richardson_lucy_into`` restated on the CPU. Returns ``(status, output)``.
+
+    Status codes follow the reference's check order (deconvolve.rs:53-71). ``nd=True`` allows the
+    3-D extension; otherwise ndim must be 2 as in the reference. Precision follows ``image.dtype``.
+    """
+    f64 = image.dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    image = np.ascontiguousarray(image, dtype=dt)
+    psf = np.ascontiguousarray(psf, dtype=dt)
+    if out is None:
+        out = np.zeros_like(image)
+    kind = _f64p if f64 else _f32p
+    fn = lib().lo_rl_f64 if f64 else lib().lo_rl_f32
+    st = fn(_ptr(image, kind), _dims(image.shape), image.ndim, _ptr(psf, kind), _dims(psf.shape), psf.ndim,
+            _ptr(out, kind), _dims(out.shape), out.ndim, int(num_iter), int(bool(clip)),
+            int(filter_epsilon is not None), float(filter_epsilon or 0.0), int(mode), 3 if nd else 2)
+    return int(st), out
+
+
+def _normalised(psf: np.ndarray):
+    psf = np.asarray(psf, dtype=np.float64)
+    s = float(np.sum(psf.ravel()))
+    norm = 1.0 / s if abs(s) > 1e-12 else 1.0
+    h = psf * norm
+    rev = h[tuple(slice(None, None, -1) for _ in range(h.ndim))]
+    return h, rev
+
+
+def _pad_indices(n: int, k: int, mode: int) -> np.ndarray:
+    return np.array([pad_index(i - k // 2, n, mode) for i in range(n + k - 1)], dtype=np.int64)
+
+
+def correlate_fft64(x: np.ndarray, w: np.ndarray, mode: int = PAD_REFLECT) -> np.ndarray:
+    """FP64 correlation (cval 0, origin 0) as an FFT linear correlation of the padded array."""
+    x = np.asarray(x, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    q = x
+    for ax in range(x.ndim):
+        idx = _pad_indices(x.shape[ax], w.shape[ax], mode)
+        valid = idx >= 0
+        q = np.take(q, np.where(valid, idx, 0), axis=ax)
+        if not valid.all():
+            sl = [slice(None)] * x.ndim
+            sl[ax] = ~valid
+            q[tuple(sl)] = 0.0
+    L = q.shape
+    axes = list(range(x.ndim))
+    Q = np.fft.rfftn(q, L, axes=axes)
+    W = np.fft.rfftn(w, L, axes=axes)
+    c = np.fft.irfftn(Q * np.conj(W), L, axes=axes)
+    return np.ascontiguousarray(c[tuple(slice(0, s) for s in x.shape)])
+
+
+def rl_fft64(image: np.ndarray, psf: np.ndarray, num_iter: int, mode: int = PAD_REFLECT,
+             filter_epsilon: float | None = None, x0: np.ndarray | None = None) -> np.ndarray:
+    """FP64 RL with the reference's semantics (0.5 start, eps rule, no clip), FFT correlations."""
+    y = np.asarray(image, dtype=np.float64)
+    h, rev = _normalised(psf)
+    eps = max(1e-12 if filter_epsilon is None else float(filter_epsilon), 1e-15)
+    x = np.full_like(y, 0.5) if x0 is None else np.array(x0, dtype=np.float64)
+    for _ in range(int(num_iter)):
+        u = correlate_fft64(x, rev, mode)
+        r = y / np.where(np.abs(u) < eps, eps, u)
+        x = x * correlate_fft64(r, h, mode)
+    return x
+
+
+def rl_nll_grad64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
+                  mode: int = PAD_REFLECT):
+    """FP64 loss and gradients ``(loss, d/dx0, d/dpsf)`` of ``sum(u - y log u)``, ``u = A x_T``."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    psf = np.ascontiguousarray(psf, dtype=np.float64)
+    gx = np.zeros_like(x0)
+    gp = np.zeros_like(psf)
+    loss = lib().lo_rl_nll_grad_f64(_ptr(y, _f64p), _ptr(x0, _f64p), _dims(y.shape), y.ndim, _ptr(psf, _f64p),
+                                    _dims(psf.shape), int(num_iter), float(eps), int(mode), _ptr(gx, _f64p),
+                                    _ptr(gp, _f64p))
+    return float(loss), gx, gp
+
+
+def rl_nll64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
+             mode: int = PAD_REFLECT) -> float:
+    """FP64 loss ``sum(u - y log u)`` after ``num_iter`` RL iterations from ``x0``."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    psf = np.ascontiguousarray(psf, dtype=np.float64)
+    return float(lib().lo_rl_nll_f64(_ptr(y, _f64p), _ptr(x0, _f64p), _dims(y.shape), y.ndim, _ptr(psf, _f64p),
+                                     _dims(psf.shape), int(num_iter), float(eps), int(mode)))
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.linalg.norm((a - b).ravel())
+    n = np.linalg.norm(b.ravel())
+    return float(d / n) if n > 0 else float(d)
+
+
+if os.environ.get("LUCEM_ORACLE_THREADS"):
+    set_num_threads(int(os.environ["LUCEM_ORACLE_THREADS"]))

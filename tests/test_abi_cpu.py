@@ -1,0 +1,126 @@
+"""The C ABI without a device: the library loads, exports every declared symbol, and its host
+logic (PadMode::parse, defaults, argument checks in the reference's order) behaves as the
+reference's (crates/lucem-ops/src/core/pad.rs:48-84, restore/deconvolve.rs:32-71)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def L():
+    from lucem_amd import _native as N
+
+    if not N.LIB_PATH.exists():
+        from lucem_amd import build
+
+        build.build()
+    return N.lib()
+
+
+def declared_symbols() -> set[str]:
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(lucem_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(L):
+    from lucem_amd import _native as N
+
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), f"{name} declared in include/ but not exported"
+    assert set(N.EXPORTED) <= names
+
+
+def test_version(L):
+    assert b"gfx950" in L.lucem_version()
+
+
+def test_settings_default_matches_reference(L):
+    from lucem_amd import _native as N
+
+    s = N.RlSettings()
+    L.lucem_rl_settings_default(ctypes.byref(s))
+    assert (s.num_iter, s.clip, s.has_filter_epsilon, s.mode) == (30, 1, 0, 0)
+
+
+@pytest.mark.parametrize("name,want", [
+    ("reflect", 0), ("symmetric", 0), ("grid-mirror", 0), ("constant", 1), ("grid-constant", 1), ("zeros", 1),
+    ("nearest", 2), ("edge", 2), ("replicate", 2), ("mirror", 3), ("wrap", 4), ("grid-wrap", 4),
+    ("periodic", 4), ("  Reflect ", 0), ("WRAP", 4),
+])
+def test_pad_mode_parse(L, name, want):
+    out = ctypes.c_uint32(99)
+    assert L.lucem_pad_mode_parse(name.encode(), ctypes.byref(out)) == 0
+    assert out.value == want
+
+
+def test_pad_mode_parse_rejects(L):
+    out = ctypes.c_uint32(99)
+    assert L.lucem_pad_mode_parse(b"invalid", ctypes.byref(out)) == 8
+    assert b"not supported" in L.lucem_last_error()
+
+
+def test_python_padmode_parse():
+    from lucem_amd import PadMode
+
+    assert PadMode.parse("grid-mirror") is PadMode.Reflect
+    assert PadMode.parse("periodic") is PadMode.Wrap
+    with pytest.raises(Exception):
+        PadMode.parse("bogus")
+
+
+def _call(L, in_dims, psf_dims, out_dims, num_iter=3, mode=0, inp=0x10000, out=0x20000000, nd=False):
+    from lucem_amd import _native as N
+
+    s = N.RlSettings()
+    L.lucem_rl_settings_default(ctypes.byref(s))
+    s.num_iter = num_iter
+    s.mode = mode
+    fn = L.lucem_richardson_lucy_nd_into_f32 if nd else L.lucem_richardson_lucy_into_f32
+    # NULL client: the argument checks run first and never touch the (fake) device pointers
+    return fn(None, N._vp(inp), N.dims(in_dims), len(in_dims), N._vp(0x30000000), N.dims(psf_dims), len(psf_dims),
+              N._vp(out), N.dims(out_dims), len(out_dims), ctypes.byref(s), None)
+
+
+def test_check_order_without_device(L):
+    assert _call(L, (4, 5), (3, 3), (4, 6)) == 1  # same_shape
+    assert b"does not match" in L.lucem_last_error()
+    assert _call(L, (4, 5), (3, 3), (4, 5), inp=0x1000, out=0x1000 + 40) == 2  # overlapping buffers
+    assert _call(L, (2, 4, 5), (3, 3), (2, 4, 5)) == 3  # input ndim
+    assert b"expects 2-D" in L.lucem_last_error()
+    assert _call(L, (4, 5), (3,), (4, 5)) == 3  # psf ndim
+    assert _call(L, (4, 5), (0, 3), (4, 5)) == 4  # empty psf
+    assert _call(L, (4, 5), (3, 3), (4, 5), num_iter=0) == 5
+    assert _call(L, (4, 5), (3, 3), (4, 5), mode=7) == 8
+    assert _call(L, (0, 5), (3, 3), (0, 5)) == 0  # empty input: Ok, nothing done
+    assert _call(L, (4, 5), (3, 3), (4, 5)) == 11  # valid arguments, no client
+    # a shape mismatch wins over every later check
+    assert _call(L, (2, 4, 5), (0,), (2, 4), num_iter=0) == 1
+
+
+def test_check_order_nd_without_device(L):
+    assert _call(L, (2, 4, 5), (3, 3, 3), (2, 4, 5), nd=True) == 11
+    assert _call(L, (2, 4, 5), (3, 3), (2, 4, 5), nd=True) == 3  # psf rank must match in N-D
+    assert _call(L, (2, 2, 4, 5), (3, 3, 3, 3), (2, 2, 4, 5), nd=True) == 3
+    assert _call(L, (65536, 65536, 2), (1, 1, 1), (65536, 65536, 2), nd=True, out=1 << 45) == 6  # > u32::MAX
+    assert _call(L, (8, 9000), (3, 3), (8, 9000)) == 9  # axis beyond the LDS FFT limit
+
+
+def test_oracle_is_not_imported_by_the_product():
+    """The product path never imports the oracle (it is test infrastructure only)."""
+    for p in (ROOT / "lucem_amd").rglob("*.py"):
+        text = p.read_text()
+        assert "oracle" not in re.sub(r"#.*", "", text).replace("oracle/", ""), p

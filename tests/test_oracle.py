@@ -1,0 +1,271 @@
+"""Pins the CPU oracle before anything is compared with it.
+
+Each check is chosen so that a plausible mistake in the oracle (a wrong pad period, a missing
+flip, an off-by-one centre, a wrong accumulation order, a dropped term in the gradient) fails:
+
+- pad_index: the reference's own host restatement (crates/lucem-ops/src/test_support.rs:21-45,
+  rewritten below) by brute force, and numpy.pad's definition of each mode;
+- correlation: scipy.ndimage.correlate (same mode names and centring, k//2 + origin) and the
+  reference's correlate_1d KAT (crates/lucem-ops/src/filter/convolve.rs:211-228, 1e-5);
+- RL: the reference's identity and box-PSF KATs (crates/lucem-ops/src/restore/deconvolve.rs:517-566),
+  the closed-form result for a shifted delta PSF, flat fixed points, FP32 vs FP64, the FFT form;
+- gradient: FP64 central finite differences.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+
+MODES = {0: "reflect", 1: "constant", 2: "nearest", 3: "mirror", 4: "wrap"}
+
+
+def host_pad_index(p: int, n: int, mode: int):
+    """The reference's f64 host restatement (test_support.rs:21-45), in Python."""
+    if 0 <= p < n:
+        return p
+    if mode == 1:
+        return None
+    if mode == 2:
+        return min(max(p, 0), n - 1)
+    if mode == 4:
+        return p % n
+    if mode == 0:
+        period = 2 * n
+        r = p % period
+        return r if r < n else period - 1 - r
+    if n <= 1:
+        return 0
+    period = 2 * n - 2
+    r = p % period
+    return r if r < n else period - r
+
+
+def sample_f64(n: int) -> np.ndarray:
+    """crates/lucem-ops/src/test_support.rs:10-12."""
+    i = np.arange(n, dtype=np.int64)
+    return ((i * 7919 + 104_729) % 1013).astype(np.float64) * 0.1 - 50.0
+
+
+def assert_close(got, want, rtol):
+    """crates/lucem-core/src/test_support.rs:42-50: |g - w| <= rtol*|w| + rtol*max|want|."""
+    got = np.asarray(got, dtype=np.float64).ravel()
+    want = np.asarray(want, dtype=np.float64).ravel()
+    atol = rtol * np.abs(want).max()
+    bad = np.abs(got - want) > rtol * np.abs(want) + atol
+    assert not bad.any(), f"index {np.argmax(bad)}: got {got[bad][0]}, expected {want[bad][0]}"
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_pad_index_matches_reference_host_restatement(oracle, mode):
+    for n in range(1, 9):
+        for p in range(-40, 41):
+            want = host_pad_index(p, n, mode)
+            assert oracle.pad_index(p, n, mode) == (-1 if want is None else want), (p, n, mode)
+
+
+@pytest.mark.parametrize("mode,np_mode", [(0, "symmetric"), (2, "edge"), (3, "reflect"), (4, "wrap")])
+def test_pad_index_matches_numpy_pad(oracle, mode, np_mode):
+    for n in range(2, 7):
+        x = np.arange(n)
+        padded = np.pad(x, 3 * n, mode=np_mode)
+        for i, v in enumerate(padded):
+            assert oracle.pad_index(i - 3 * n, n, mode) == v
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_correlate_matches_scipy(oracle, mode):
+    rng = np.random.default_rng(10 + mode)
+    cases = [((17, 23), (5, 3)), ((7, 9), (4, 6)), ((3, 2), (9, 11)), ((6, 5, 9), (3, 4, 5)), ((1, 1), (3, 3))]
+    for shp, ks in cases:
+        x = rng.standard_normal(shp)
+        w = rng.standard_normal(ks)
+        got = oracle.correlate(x, w, mode, 0.0)
+        want = ndi.correlate(x, w, mode=MODES[mode], cval=0.0)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+        got32 = oracle.correlate(x.astype(np.float32), w.astype(np.float32), mode, 0.0)
+        assert_close(got32, want, 1e-5)
+
+
+def host_correlate_1d(x, dims, w, axis, mode, cval, origin):
+    """filter/convolve.rs:185-209 restated: f64 sums over host_pad_index."""
+    x = np.asarray(x, dtype=np.float64).reshape(dims)
+    left = len(w) // 2 + origin
+    out = np.zeros_like(x)
+    n = dims[axis]
+    for idx in np.ndindex(*dims):
+        s = 0.0
+        for t, wt in enumerate(w):
+            q = host_pad_index(idx[axis] - left + t, n, mode)
+            if q is None:
+                v = cval
+            else:
+                j = list(idx)
+                j[axis] = q
+                v = x[tuple(j)]
+            s += wt * v
+        out[idx] = s
+    return out
+
+
+def test_correlate_1d_reference_kat(oracle):
+    """The reference KAT at filter/convolve.rs:211-228 (dims, weight sets, origins, cval 1.5)."""
+    for dims in [(1,), (7,), (32, 33), (5, 1, 9)]:
+        x = sample_f64(int(np.prod(dims)))
+        for axis in range(len(dims)):
+            for w, origin in [([1.0, -2.0, 3.0], 0), ([0.5] * 4, -1)]:
+                for mode in MODES:
+                    kshape = [1] * len(dims)
+                    kshape[axis] = len(w)
+                    org = [0] * len(dims)
+                    org[axis] = origin
+                    got = oracle.correlate(x.astype(np.float32).reshape(dims),
+                                           np.array(w, np.float32).reshape(kshape), mode, 1.5, org)
+                    want = host_correlate_1d(x, dims, w, axis, mode, 1.5, origin)
+                    assert_close(got, want, 1e-5)
+
+
+def test_rl_identity_kat(oracle):
+    """deconvolve.rs:519-539: delta PSF, 15 iterations, no clip -> the input within 1e-4."""
+    img = ((np.arange(64, dtype=np.float32) / 64.0) * 0.8 + 0.1).reshape(8, 8).astype(np.float32)
+    delta = np.zeros((3, 3), np.float32)
+    delta[1, 1] = 1.0
+    st, out = oracle.richardson_lucy(img, delta, 15, clip=False)
+    assert st == 0
+    assert_close(out, img.astype(np.float64), 1e-4)
+
+
+def test_rl_box_sharpening_kat(oracle):
+    """deconvolve.rs:551-566: RL raises the peak of a box-blurred spot by more than 0.15."""
+    spot = np.full((8, 8), 0.1, np.float32)
+    spot[4, 4] = 0.9
+    box = np.full((3, 3), 1.0 / 9.0, np.float32)
+    blurred = oracle.correlate(spot, box, 0, 0.0)
+    st, restored = oracle.richardson_lucy(blurred, box, 25, clip=True)
+    assert st == 0
+    assert restored[4, 4] > blurred[4, 4] + 0.15
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3, 4])
+def test_rl_shifted_delta_closed_form(oracle, mode):
+    """A delta PSF at (0, 1) of a 3x3 support: correlate(x, flip(h)) reads x[p + (1, 0)] and
+    correlate(r, h) reads r[p - (1, 0)], so RL returns y shifted down one row (row 0 is the
+    padded row -1) after any number of iterations. A missing flip shifts the other way."""
+    rng = np.random.default_rng(3)
+    y = rng.uniform(1.0, 2.0, (9, 7))
+    h = np.zeros((3, 3))
+    h[0, 1] = 1.0
+    for it in (1, 4):
+        st, out = oracle.richardson_lucy(y, h, it, clip=False, mode=mode)
+        assert st == 0
+        want = np.empty_like(y)
+        want[1:] = y[:-1]
+        want[0] = y[oracle.pad_index(-1, 9, mode)]
+        np.testing.assert_allclose(out, want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3, 4])
+def test_rl_flat_fixed_point(oracle, mode):
+    rng = np.random.default_rng(4)
+    psf = rng.uniform(0.1, 1.0, (5, 4))
+    y = np.full((11, 13), 37.5)
+    st, out = oracle.richardson_lucy(y, psf, 6, clip=False, mode=mode)
+    np.testing.assert_allclose(out, y, rtol=1e-12)
+
+
+def test_rl_f32_close_to_f64_and_fft64(oracle):
+    rng = np.random.default_rng(5)
+    y = rng.uniform(50, 150, (23, 31))
+    psf = rng.uniform(0.0, 1.0, (7, 6))
+    for mode in MODES:
+        _, a64 = oracle.richardson_lucy(y, psf, 8, clip=False, mode=mode)
+        _, a32 = oracle.richardson_lucy(y.astype(np.float32), psf.astype(np.float32), 8, clip=False, mode=mode)
+        f64 = oracle.rl_fft64(y, psf, 8, mode=mode)
+        assert oracle.rel_l2(f64, a64) < 1e-12
+        assert oracle.rel_l2(a32, a64) < 1e-5
+
+
+def test_rl_3d_fft64_matches_direct(oracle):
+    rng = np.random.default_rng(6)
+    y = rng.uniform(50, 150, (6, 9, 8))
+    psf = rng.uniform(0.0, 1.0, (3, 4, 5))
+    for mode in MODES:
+        st, a = oracle.richardson_lucy(y, psf, 4, clip=False, mode=mode, nd=True)
+        assert st == 0
+        assert oracle.rel_l2(oracle.rl_fft64(y, psf, 4, mode=mode), a) < 1e-12
+
+
+def test_rl_error_order(oracle):
+    """The reference check order (deconvolve.rs:53-71)."""
+    y = np.ones((4, 5), np.float32)
+    psf = np.ones((3, 3), np.float32)
+    assert oracle.richardson_lucy(y, psf, 3, out=np.zeros((4, 6), np.float32))[0] == 1
+    assert oracle.richardson_lucy(np.ones((2, 3, 4), np.float32), psf, 3)[0] == 3
+    assert oracle.richardson_lucy(y, np.ones((3,), np.float32), 3)[0] == 3
+    assert oracle.richardson_lucy(y, np.ones((0, 3), np.float32), 3)[0] == 4
+    assert oracle.richardson_lucy(y, psf, 0)[0] == 5
+    assert oracle.richardson_lucy(np.ones((0, 5), np.float32), psf, 3)[0] == 0
+    # shape mismatch is reported before the dimension check
+    assert oracle.richardson_lucy(np.ones((2, 3, 4), np.float32), psf, 0, out=np.zeros((2, 3), np.float32))[0] == 1
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_gradient_matches_finite_differences(oracle, mode):
+    rng = np.random.default_rng(7 + mode)
+    x0 = rng.uniform(0.5, 2.0, (9, 10))
+    y = rng.uniform(50, 150, (9, 10))
+    psf = rng.uniform(0.1, 1.0, (3, 5))
+    _, gx, gp = oracle.rl_nll_grad64(y, x0, psf, 4, mode=mode)
+    for idx in [(0, 0), (4, 5), (8, 9), (2, 7)]:
+        h = 1e-5
+        xp, xm = x0.copy(), x0.copy()
+        xp[idx] += h
+        xm[idx] -= h
+        fd = (oracle.rl_nll64(y, xp, psf, 4, mode=mode) - oracle.rl_nll64(y, xm, psf, 4, mode=mode)) / (2 * h)
+        assert abs(fd - gx[idx]) <= 1e-4 * abs(fd) + 1e-7
+    for idx in [(0, 0), (1, 2), (2, 4)]:
+        h = 1e-6
+        pp, pm = psf.copy(), psf.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (oracle.rl_nll64(y, x0, pp, 4, mode=mode) - oracle.rl_nll64(y, x0, pm, 4, mode=mode)) / (2 * h)
+        assert abs(fd - gp[idx]) <= 1e-4 * max(abs(fd), 1e-3)
+
+
+def test_gradient_3d_matches_finite_differences(oracle):
+    rng = np.random.default_rng(9)
+    x0 = rng.uniform(0.5, 2.0, (4, 5, 6))
+    y = rng.uniform(50, 150, (4, 5, 6))
+    psf = rng.uniform(0.1, 1.0, (3, 3, 2))
+    _, gx, gp = oracle.rl_nll_grad64(y, x0, psf, 3)
+    for idx in [(0, 0, 0), (2, 3, 4), (3, 4, 5)]:
+        h = 1e-5
+        xp, xm = x0.copy(), x0.copy()
+        xp[idx] += h
+        xm[idx] -= h
+        fd = (oracle.rl_nll64(y, xp, psf, 3) - oracle.rl_nll64(y, xm, psf, 3)) / (2 * h)
+        assert abs(fd - gx[idx]) <= 1e-4 * abs(fd) + 1e-7
+    idx = (1, 2, 1)
+    pp, pm = psf.copy(), psf.copy()
+    pp[idx] += 1e-6
+    pm[idx] -= 1e-6
+    fd = (oracle.rl_nll64(y, x0, pp, 3) - oracle.rl_nll64(y, x0, pm, 3)) / 2e-6
+    assert abs(fd - gp[idx]) <= 1e-4 * max(abs(fd), 1e-3)
+
+
+def test_planted_flip_mistake_is_caught(oracle):
+    """A restatement that forgets the 180-degree flip (correlating with h in both steps) fails
+    the shifted-delta closed form: the pin above is sensitive to that mistake."""
+    rng = np.random.default_rng(11)
+    y = rng.uniform(1.0, 2.0, (9, 7))
+    h = np.zeros((3, 3))
+    h[0, 1] = 1.0
+    x = np.full_like(y, 0.5)
+    for _ in range(2):  # the planted mistake: both correlations use h
+        u = oracle.correlate(x, h, 0)
+        x = x * oracle.correlate(y / u, h, 0)
+    want = np.empty_like(y)
+    want[1:] = y[:-1]
+    want[0] = y[0]
+    assert not np.allclose(x, want, rtol=1e-6)
