@@ -1,0 +1,196 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X Richardson-Lucy deconvolution (one JSON line on stdout).
+
+Metric: RL Mvoxel-iterations per second = voxels x iterations x steps / seconds / 1e6, with the
+input already resident in HBM. A step is one complete deconvolution call (all iterations) on
+the workload:
+
+- ``plain3d`` (default): 128x512x512 float32 volume, 31x15x15 Gaussian PSF, 50 iterations
+  (the largest single-array workload; the reference RL is 2-D only, this is its N-D form);
+- ``plain2d``: 1024x1024, 25x25 PSF, 30 iterations (the reference's own 2-D entry point).
+
+Alongside the throughput the line carries:
+
+- ``roofline``: the dominant kernel's algorithmic HBM bytes per launch divided by its average
+  launch time (HIP events on the library's stream over the timed region), against the MI355X
+  HBM3E peak of 8 TB/s; ``traffic`` is the measured HBM bytes per launch of that kernel from the
+  committed rocprofv3 counter summary under profiles/ (null when there is none);
+- ``iteration``: the whole RL iteration's algorithmic bytes (24 n + 80 N_c in 2-D, 24 n + 144 N_c
+  in 3-D, on the minimal linear-convolution domain n + k - 1) over the measured time per iteration;
+- ``cpu_baseline``: the CPU restatement of the reference algorithm (direct correlation, FP32,
+  OpenMP) timed on this host on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def minimal_domain_bytes(shape, kshape) -> float:
+    """Algorithmic bytes of one RL iteration on the minimal domain L = n + k - 1."""
+    n = 1
+    for s in shape:
+        n *= s
+    L = [a + b - 1 for a, b in zip(shape, kshape)]
+    nc = 1
+    for a in L[:-1]:
+        nc *= a
+    nc *= L[-1] // 2 + 1
+    per_complex = 80 if len(shape) == 2 else 144
+    return 24.0 * n + per_complex * nc
+
+
+def load_traffic(workload: str, kernel: str):
+    p = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(prob, workload: str, want_seconds: float = 15.0):
+    from oracle import oracle as O
+
+    O.build()
+    y = prob.image
+    if workload == "plain3d":
+        iters = 2
+        sample = f"full 128x512x512 volume, {iters} of the 50 iterations (cost per iteration is constant)"
+    else:
+        iters = prob.num_iter
+        sample = f"full 1024x1024 image, all {iters} iterations"
+    t0 = time.perf_counter()
+    st, _ = O.richardson_lucy(y, prob.psf, iters, clip=False, nd=(y.ndim == 3))
+    dt = time.perf_counter() - t0
+    assert st == 0
+    return {
+        "value": round(y.size * iters / dt / 1e6, 3),
+        "unit": "Mvoxel-iters/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": sample + f"; FP32 direct correlation, {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="plain3d", choices=["plain3d", "plain2d"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {max(args.gpus, world)}): the RL path is one "
+              "array per GPU; tiles and volumes shard across GPUs without exchange", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+    from lucem_amd import synth
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    prob = synth.plain3d() if args.workload == "plain3d" else synth.plain2d()
+    y = torch.from_numpy(prob.image).cuda()
+    psf = torch.from_numpy(prob.psf).cuda()
+    out = torch.empty_like(y)
+    client = N.default_client(0)
+    ws = N.Workspace(client)
+    settings = LA.RichardsonLucy(num_iter=prob.num_iter, clip=False)
+    run = LA.richardson_lucy_nd_into if y.dim() == 3 else LA.richardson_lucy_into
+
+    for _ in range(args.warmup):
+        run(y, psf, out, settings, ws, client)
+    torch.cuda.synchronize()
+    client.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run(y, psf, out, settings, ws, client)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = client.profile_read()
+    client.profile(False)
+    assert bool(torch.isfinite(out).all()), "non-finite output"
+
+    nvox = int(np.prod(prob.image.shape))
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = nvox * prob.num_iter * args.steps / elapsed / 1e6
+    iter_kernels = {k: v for k, v in stats.items() if not k.startswith("otf") and k != "psf_prep"}
+    dom_name, dom = max(iter_kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(args.workload, dom_name)
+    kernel_ms = sum(v["total_ms"] for v in stats.values())
+    iter_bytes = minimal_domain_bytes(prob.image.shape, prob.psf.shape)
+    iter_s = elapsed / (args.steps * prob.num_iter)
+    line = {
+        "metric": "RL Mvoxel-iters/s",
+        "value": round(value, 2),
+        "unit": "Mvoxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": prob.meta["workload"] + f", {prob.num_iter} iterations, Reflect, clip off",
+            "shape": list(prob.image.shape),
+            "psf": list(prob.psf.shape),
+            "num_iter": prob.num_iter,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(avg_ms, 5),
+        },
+        "iteration": {
+            "algorithmic_bytes": iter_bytes,
+            "ms": round(iter_s * 1e3, 4),
+            "achieved_GBps": round(iter_bytes / iter_s / 1e9, 1),
+            "frac": round(iter_bytes / iter_s / 1e9 / HBM_PEAK_GBPS, 4),
+        },
+        "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 5),
+                        "share": round(v["total_ms"] / kernel_ms, 4),
+                        "GBps": round(v["bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)}
+                    for k, v in stats.items()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(prob, args.workload)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

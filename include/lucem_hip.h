@@ -105,6 +105,21 @@ void* lucem_client_stream(lucem_client* client);
 /* Number of compute units of the client's device. */
 int lucem_client_num_cus(lucem_client* client);
 
+/* Per-kernel timing of the client's launches (HIP events on the client stream around each
+ * launch) with each launch's algorithmic HBM bytes (the minimum traffic of that pass: its
+ * distinct inputs and outputs). Enabling clears earlier records. No reference counterpart
+ * (the reference times whole calls, crates/lucem-core/src/compute/bench.rs:17-30). */
+typedef struct lucem_kernel_stat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    double bytes;
+} lucem_kernel_stat;
+lucem_status lucem_client_profile(lucem_client* client, int32_t enable);
+/* Waits for the stream, aggregates the records by kernel name into out[0..max), sets *count
+ * to the number of distinct names, and clears the records. */
+lucem_status lucem_client_profile_read(lucem_client* client, lucem_kernel_stat* out, uint32_t max, uint32_t* count);
+
 /* Grow-only device buffers reused across calls (Workspace semantics: one per thread of work,
  * never shared between concurrent calls). */
 lucem_status lucem_workspace_create(lucem_client* client, lucem_workspace** out);

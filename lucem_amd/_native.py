@@ -37,7 +37,8 @@ EXPORTED = [
     "lucem_client_create", "lucem_client_destroy", "lucem_client_sync", "lucem_client_stream",
     "lucem_client_num_cus", "lucem_workspace_create", "lucem_workspace_destroy",
     "lucem_workspace_capacity_bytes", "lucem_richardson_lucy_into_f32", "lucem_richardson_lucy_f32",
-    "lucem_richardson_lucy_nd_into_f32", "lucem_fft_c2c_lines_f32",
+    "lucem_richardson_lucy_nd_into_f32", "lucem_fft_c2c_lines_f32", "lucem_client_profile",
+    "lucem_client_profile_read",
 ]
 
 
@@ -48,6 +49,15 @@ class LucemError(RuntimeError):
         self.status = status
         self.kind = STATUS_NAMES.get(status, str(status))
         super().__init__(f"{self.kind}: {message}")
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", ctypes.c_uint64),
+        ("total_ms", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+    ]
 
 
 class RlSettings(ctypes.Structure):
@@ -96,6 +106,9 @@ def lib() -> ctypes.CDLL:
             L.lucem_richardson_lucy_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                     _vp, ctypes.POINTER(RlSettings)]
             L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
+            L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
+            L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
+                                                    ctypes.POINTER(ctypes.c_uint32)]
             _register_optional(L)
             _lib = L
         return _lib
@@ -160,6 +173,22 @@ class Client:
     @property
     def num_cus(self) -> int:
         return int(lib().lucem_client_num_cus(self._h))
+
+    def profile(self, enable: bool = True) -> None:
+        """Starts (clearing earlier records) or stops per-kernel event timing."""
+        check(lib().lucem_client_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """``{kernel: {"launches", "total_ms", "bytes"}}`` since profiling was enabled (clears)."""
+        arr = (KernelStat * 64)()
+        n = ctypes.c_uint32()
+        check(lib().lucem_client_profile_read(self._h, arr, 64, ctypes.byref(n)))
+        out = {}
+        for i in range(min(n.value, 64)):
+            s = arr[i]
+            out[s.name.decode()] = {"launches": int(s.launches), "total_ms": float(s.total_ms),
+                                    "bytes": float(s.bytes)}
+        return out
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

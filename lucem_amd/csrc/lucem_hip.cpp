@@ -118,6 +118,11 @@ lucem_client::~lucem_client() {
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
     internal_ws.reset();
+    for (auto& r : records) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    for (auto ev : event_pool) (void)hipEventDestroy(ev);
     for (auto& kv : twiddles) (void)hipFree(kv.second);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -136,6 +141,106 @@ extern "C" lucem_status lucem_client_sync(lucem_client* client) {
 
 extern "C" void* lucem_client_stream(lucem_client* client) { return client ? (void*)client->stream : nullptr; }
 extern "C" int lucem_client_num_cus(lucem_client* client) { return client ? client->num_cus : 0; }
+
+// ---------------------------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------------------------
+static lucem_status take_event(lucem_client* c, hipEvent_t* ev) {
+    if (!c->event_pool.empty()) {
+        *ev = c->event_pool.back();
+        c->event_pool.pop_back();
+        return LUCEM_OK;
+    }
+    hipError_t e = hipEventCreate(ev);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_client_profile(lucem_client* client, int32_t enable) {
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    (void)hipStreamSynchronize(client->stream);
+    for (auto& r : client->records) {
+        client->event_pool.push_back(r.start);
+        client->event_pool.push_back(r.stop);
+    }
+    client->records.clear();
+    client->profiling = enable != 0;
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_client_profile_read(lucem_client* client, lucem_kernel_stat* out, uint32_t max,
+                                                  uint32_t* count) {
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    hipError_t e = hipStreamSynchronize(client->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    std::vector<lucem_kernel_stat> agg;
+    for (auto& r : client->records) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, r.start, r.stop);
+        if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+        lucem_kernel_stat* slot = nullptr;
+        for (auto& a : agg)
+            if (std::strncmp(a.name, r.name, sizeof a.name) == 0) slot = &a;
+        if (!slot) {
+            agg.emplace_back();
+            slot = &agg.back();
+            std::memset(slot, 0, sizeof *slot);
+            std::snprintf(slot->name, sizeof slot->name, "%s", r.name);
+        }
+        slot->launches++;
+        slot->total_ms += ms;
+        slot->bytes += r.bytes;
+        client->event_pool.push_back(r.start);
+        client->event_pool.push_back(r.stop);
+    }
+    client->records.clear();
+    for (uint32_t i = 0; i < max && i < agg.size(); i++) out[i] = agg[i];
+    if (count) *count = (uint32_t)agg.size();
+    return LUCEM_OK;
+}
+
+double lucem::row_pass_bytes(const RowPass& r) {
+    const double R = r.rows, spec = R * r.Lc * 8.0, real = R * r.nx * 4.0;
+    switch (r.op) {
+        case ROW_LOAD: return real + spec;
+        case ROW_LOAD_CONST: return spec;
+        case ROW_RATIO: return spec + real + spec;
+        default: return spec + (r.first ? 0.0 : real) + real + (r.last ? 0.0 : spec);
+    }
+}
+
+double lucem::col_pass_bytes(const ColPass& p, int blocks) {
+    const double outers = (double)(blocks / p.nkb);
+    const double line = p.Lc * 8.0;
+    double b = outers * line * (p.n_src + p.n_dst);
+    if (p.op == COL_FUSED) b += outers * line * p.f.L;
+    return b;
+}
+
+template <class Launch>
+static lucem_status profiled(lucem_client* c, const char* name, double bytes, Launch&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    lucem_status st;
+    if (c->profiling) {
+        if ((st = take_event(c, &a)) || (st = take_event(c, &b))) return st;
+        (void)hipEventRecord(a, c->stream);
+    }
+    const int err = launch();
+    if (err != hipSuccess) return hip_fail(err, name);
+    if (c->profiling) {
+        (void)hipEventRecord(b, c->stream);
+        c->records.push_back(ProfRecord{name, a, b, bytes});
+    }
+    return LUCEM_OK;
+}
+
+lucem_status lucem::run_row(lucem_client* c, const RowPass& r, int blocks, const char* name) {
+    return profiled(c, name, row_pass_bytes(r), [&] { return launch_row_pass(r, blocks, c->stream); });
+}
+
+lucem_status lucem::run_col(lucem_client* c, const ColPass& p, int blocks, const char* name) {
+    return profiled(c, name, col_pass_bytes(p, blocks), [&] { return launch_col_pass(p, blocks, c->stream); });
+}
 
 // Twiddle table exp(-2 pi i m / L), m < L, computed in double and rounded once to float.
 lucem_status lucem::twiddles_for(lucem_client* c, int L, const float2** out) {
@@ -256,12 +361,6 @@ FftPlan lucem::make_plan(int L) {
     return p;
 }
 
-static int ilog2(int v) {
-    int l = 0;
-    while ((1 << (l + 1)) <= v) l++;
-    return l;
-}
-
 int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
     const int pairs = (rows + 1) / 2;
     int lgP = 3;
@@ -323,7 +422,7 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     r.spec = tmp_a;
     r.tw = tws[2];
     const int pairs = (r.rows + 1) / 2;
-    lucem_status st = launch_check(launch_row_pass(r, (pairs + (1 << r.lgP) - 1) >> r.lgP, c->stream), "row pass");
+    lucem_status st = run_row(c, r, (pairs + (1 << r.lgP) - 1) >> r.lgP, "otf_row_load");
     if (st) return st;
 
     ColPass y{};
@@ -346,14 +445,14 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
         y.dst_axis = g.pitch;
         y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-        return launch_check(launch_col_pass(y, y.nkb, c->stream), "line pass (y)");
+        return run_col(c, y, y.nkb, "otf_line_fwd_y");
     }
     y.dst = tmp_b;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
     y.lgB = choose_lgB(g.L[1], g.Lc, m[0], c->num_cus);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-    st = launch_check(launch_col_pass(y, m[0] * y.nkb, c->stream), "line pass (y)");
+    st = run_col(c, y, m[0] * y.nkb, "otf_line_fwd_y");
     if (st) return st;
 
     ColPass z{};
@@ -375,7 +474,7 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     z.scale = scale;
     z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
-    return launch_check(launch_col_pass(z, g.L[1] * z.nkb, c->stream), "line pass (z)");
+    return run_col(c, z, g.L[1] * z.nkb, "otf_line_fwd_z");
 }
 
 // Runs the line passes of one correlation over spec_a (and spec_b in 3-D) with transfer `otf`.
@@ -400,7 +499,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
         y.otf_axis = g.pitch;
         y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-        return launch_check(launch_col_pass(y, y.nkb, c->stream), "line pass (y)");
+        return run_col(c, y, y.nkb, "line_fused_y");
     }
     // y forward: spec_a [nz][ny] -> spec_b [nz][Ly]
     y.op = COL_FWD;
@@ -414,7 +513,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     y.dst_axis = g.pitch;
     y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], c->num_cus);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-    lucem_status st = launch_check(launch_col_pass(y, g.n[0] * y.nkb, c->stream), "line pass (y)");
+    lucem_status st = run_col(c, y, g.n[0] * y.nkb, "line_fwd_y");
     if (st) return st;
     // z fused, in place on spec_b
     ColPass z{};
@@ -435,7 +534,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     z.otf_axis = (int64_t)g.L[1] * g.pitch;
     z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
-    st = launch_check(launch_col_pass(z, g.L[1] * z.nkb, c->stream), "line pass (z)");
+    st = run_col(c, z, g.L[1] * z.nkb, "line_fused_z");
     if (st) return st;
     // y inverse: spec_b [nz][Ly] -> spec_a [nz][ny]
     ColPass yi = y;
@@ -446,7 +545,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     yi.src_outer = (int64_t)g.L[1] * g.pitch;
     yi.dst = sa;
     yi.dst_outer = (int64_t)g.n[1] * g.pitch;
-    return launch_check(launch_col_pass(yi, g.n[0] * yi.nkb, c->stream), "line pass (y inverse)");
+    return run_col(c, yi, g.n[0] * yi.nkb, "line_inv_y");
 }
 
 lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
@@ -507,18 +606,18 @@ lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* in
 
     // est_0 = 0.5 everywhere (deconvolve.rs:93): transform the constant directly.
     r.op = ROW_LOAD_CONST;
-    if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+    if ((st = run_row(c, r, row_blocks, "row_start"))) return st;
     for (uint64_t it = 0; it < s.num_iter; it++) {
         const bool last = it + 1 == s.num_iter;
         if ((st = correlate_lines(c, g, sa, sb, otf1, tws))) return st;
         r.op = ROW_RATIO;
-        if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+        if ((st = run_row(c, r, row_blocks, "row_ratio"))) return st;
         if ((st = correlate_lines(c, g, sa, sb, otf2, tws))) return st;
         r.op = ROW_UPDATE;
         r.first = it == 0;
         r.last = last;
         r.clip = s.clip ? 1 : 0;
-        if ((st = launch_check(launch_row_pass(r, row_blocks, c->stream), "row pass"))) return st;
+        if ((st = run_row(c, r, row_blocks, last ? "row_update_last" : "row_update"))) return st;
     }
     return LUCEM_OK;
 }
@@ -662,5 +761,5 @@ extern "C" lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const floa
     p.tw = tw;
     p.src = (const float2*)src;
     p.dst = (float2*)dst;
-    return launch_check(launch_col_pass(p, (int)nlines, client->stream), "fft lines");
+    return run_col(client, p, (int)nlines, "fft_lines");
 }

@@ -7,11 +7,21 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/lucem_hip.h"
 #include "rl_internal.hpp"
 
 struct lucem_workspace;
+
+namespace lucem {
+// One profiled launch: events on the client stream around it and its algorithmic HBM bytes.
+struct ProfRecord {
+    const char* name;
+    hipEvent_t start, stop;
+    double bytes;
+};
+}  // namespace lucem
 
 struct lucem_client {
     int device = 0;
@@ -20,6 +30,9 @@ struct lucem_client {
     bool own_stream = false;
     std::map<int, float2*> twiddles;               // FFT length -> device twiddle table
     std::unique_ptr<lucem_workspace> internal_ws;  // used by the allocating-style entry points
+    bool profiling = false;
+    std::vector<lucem::ProfRecord> records;        // launches since profiling was enabled
+    std::vector<hipEvent_t> event_pool;            // events of read records, reused
     ~lucem_client();
 };
 
@@ -60,6 +73,12 @@ FftPlan make_plan(int L);
 int choose_lgP(int L, int nx, int rows, int num_cus);
 int choose_lgB(int L, int Lc, int64_t outers, int num_cus);
 bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g);
+
+// Launch with optional profiling (events around the launch, algorithmic bytes recorded).
+lucem_status run_row(lucem_client* c, const RowPass& r, int blocks, const char* name);
+lucem_status run_col(lucem_client* c, const ColPass& p, int blocks, const char* name);
+double row_pass_bytes(const RowPass& r);
+double col_pass_bytes(const ColPass& p, int blocks);
 
 lucem_status validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
                          const uint64_t* psf_dims, uint32_t psf_ndim, const float* output, const uint64_t* out_dims,

@@ -88,10 +88,39 @@ struct PsfPrep {
     float* rev;
 };
 
+// Batched launches: one kernel over the work items of many tiles. Each workgroup of the
+// persistent grid takes items (tile, block) from an atomic counter until none is left.
+struct RowBatch {
+    const RowPass* tiles;   // per-tile descriptors (geometry, buffers)
+    const int2* items;      // (tile, row-pair block)
+    int nitems;
+    int op, first, last, clip;  // overrides of the per-tile fields for this launch
+    int* counter;
+};
+
+struct ColBatch {
+    const ColPass* tiles;
+    const int2* items;      // (tile, workgroup index within the tile's pass)
+    int nitems;
+    int* counter;
+};
+
+struct PsfBatch {
+    const PsfPrep* psfs;
+    int count;
+};
+
 // Launchers (rl_kernels.hip). Each returns the hipError_t of the launch as int.
 int launch_row_pass(const RowPass& p, int blocks, void* stream);
 int launch_col_pass(const ColPass& p, int blocks, void* stream);
 int launch_psf_prep(const PsfPrep& p, void* stream);
+int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream);
+int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream);
+int launch_psf_batch(const PsfBatch& b, void* stream);
+int launch_zero_ints(int* p, int n, void* stream);
+// Workgroups of the batched kernels that fit on one CU with `lds` bytes of dynamic LDS.
+int row_batch_occupancy(size_t lds);
+int col_batch_occupancy(size_t lds);
 size_t row_pass_lds_bytes(int L, int nx, int lgP);
 size_t col_pass_lds_bytes(int L, int lgB);
 

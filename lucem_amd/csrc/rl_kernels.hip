@@ -90,7 +90,7 @@ __device__ void row_pass_body(const RowPass& d, int g, float2* lds) {
                 sb[x] = ob;
             }
         }
-        if (d.op == ROW_UPDATE && d.last) return;
+        if (d.op == ROW_UPDATE && d.last) return;  // uniform: the whole workgroup leaves
         __syncthreads();
         // pad gather from the staged rows
         for (int b = 0; b < P; b++) {
@@ -222,6 +222,74 @@ __global__ __launch_bounds__(256) void psf_prep_kernel(PsfPrep d) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Batched persistent kernels: workgroups pull (tile, block) items from an atomic counter.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int next_item(int* counter, int* slot) {
+    if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*slot);
+    __syncthreads();
+    return it;
+}
+
+template <int T, int E>
+__global__ __launch_bounds__(T) void row_batch_kernel(RowBatch bt) {
+    extern __shared__ float2 lds[];
+    __shared__ int slot;
+    for (;;) {
+        const int it = next_item(bt.counter, &slot);
+        if (it >= bt.nitems) break;
+        const int2 item = bt.items[it];
+        RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        d.op = bt.op;
+        d.first = bt.first;
+        d.last = bt.last;
+        d.clip = bt.clip;
+        row_pass_body<T, E>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
+        __syncthreads();
+    }
+}
+
+template <int T, int E>
+__global__ __launch_bounds__(T) void col_batch_kernel(ColBatch bt) {
+    extern __shared__ float2 lds[];
+    __shared__ int slot;
+    for (;;) {
+        const int it = next_item(bt.counter, &slot);
+        if (it >= bt.nitems) break;
+        const int2 item = bt.items[it];
+        const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        col_pass_body<T, E>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void psf_batch_kernel(PsfBatch bt) {
+    __shared__ double part[256];
+    const PsfPrep d = bt.psfs[blockIdx.x];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < d.nk; i += 256) s += (double)d.psf[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double sum = part[0];
+    const double norm = fabs(sum) > 1e-12 ? 1.0 / sum : 1.0;
+    for (int64_t i = threadIdx.x; i < d.nk; i += 256) {
+        const float v = (float)((double)d.psf[i] * norm);
+        d.h[i] = v;
+        d.rev[d.nk - 1 - i] = v;
+    }
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
 size_t row_pass_lds_bytes(int L, int nx, int lgP) {
     const size_t P = (size_t)1 << lgP;
     return P * (size_t)L * sizeof(float2) + P * 2 * (size_t)nx * sizeof(float);
@@ -241,6 +309,44 @@ int launch_col_pass(const ColPass& p, int blocks, void* stream) {
     hipLaunchKernelGGL((col_pass_kernel<kThreads, kElemsPerThread>), dim3(blocks), dim3(kThreads), lds,
                        (hipStream_t)stream, p);
     return (int)hipGetLastError();
+}
+
+int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream) {
+    hipLaunchKernelGGL((row_batch_kernel<kThreads, kElemsPerThread>), dim3(grid), dim3(kThreads), lds,
+                       (hipStream_t)stream, b);
+    return (int)hipGetLastError();
+}
+
+int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream) {
+    hipLaunchKernelGGL((col_batch_kernel<kThreads, kElemsPerThread>), dim3(grid), dim3(kThreads), lds,
+                       (hipStream_t)stream, b);
+    return (int)hipGetLastError();
+}
+
+int launch_psf_batch(const PsfBatch& b, void* stream) {
+    hipLaunchKernelGGL(psf_batch_kernel, dim3(b.count), dim3(256), 0, (hipStream_t)stream, b);
+    return (int)hipGetLastError();
+}
+
+int launch_zero_ints(int* p, int n, void* stream) {
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+int row_batch_occupancy(size_t lds) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread>,
+                                                     kThreads, lds) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+int col_batch_occupancy(size_t lds) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread>,
+                                                     kThreads, lds) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
 }
 
 int launch_psf_prep(const PsfPrep& p, void* stream) {

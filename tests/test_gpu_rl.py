@@ -1,0 +1,261 @@
+"""GPU parity of the HIP Richardson-Lucy path against the CPU oracle, through the C ABI.
+
+Bar: relative L2 <= 1e-4 against the FP32 restatement of the reference
+(crates/lucem-ops/src/restore/deconvolve.rs:45-149), with the error against FP64 printed; the
+reference KATs (deconvolve.rs:517-566) at their own tolerances; size-independent properties at
+the full benchmark sizes.
+"""
+
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+RTOL_L2 = 1e-4
+
+
+def _run(y, psf, num_iter, mode=0, clip=False, nd=None, eps=None):
+    import torch
+
+    import lucem_amd as LA
+
+    yt = torch.from_numpy(np.ascontiguousarray(y, np.float32)).cuda()
+    pt = torch.from_numpy(np.ascontiguousarray(psf, np.float32)).cuda()
+    out = torch.empty_like(yt)
+    s = LA.RichardsonLucy(num_iter=num_iter, clip=clip, mode=mode, filter_epsilon=eps)
+    if (nd if nd is not None else y.ndim == 3):
+        LA.richardson_lucy_nd_into(yt, pt, out, s)
+    else:
+        LA.richardson_lucy_into(yt, pt, out, s)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 25, 30, 49, 64, 105, 128, 160, 343, 525, 540,
+                               1000, 1050, 2048, 4096])
+def test_fft_lines_match_numpy(gpu, L):
+    import ctypes
+
+    import torch
+
+    from lucem_amd import _native as N
+
+    rng = np.random.default_rng(L)
+    n = 5
+    x = (rng.standard_normal((n, L)) + 1j * rng.standard_normal((n, L))).astype(np.complex64)
+    t = torch.from_numpy(x.view(np.float32).copy()).cuda()
+    f = torch.empty_like(t)
+    b = torch.empty_like(t)
+    N.check(N.lib().lucem_fft_c2c_lines_f32(gpu.handle, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                                            L, n, 0))
+    N.check(N.lib().lucem_fft_c2c_lines_f32(gpu.handle, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                            L, n, 1))
+    torch.cuda.synchronize()
+    got = f.cpu().numpy().view(np.complex64)
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-6
+    back = b.cpu().numpy().view(np.complex64) / L
+    assert np.linalg.norm(back - x) / np.linalg.norm(x) < 2e-6
+
+
+CASES_2D = [
+    ((8, 8), (3, 3)), ((37, 53), (5, 7)), ((64, 64), (4, 6)), ((40, 33), (9, 9)), ((33, 40), (7, 3)),
+    ((50, 21), (5, 5)), ((1, 1), (3, 3)), ((1, 17), (3, 5)), ((19, 1), (5, 1)), ((5, 6), (11, 13)),
+    ((129, 97), (17, 17)), ((256, 256), (25, 25)),
+]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape,kshape", CASES_2D)
+def test_rl_2d_matches_oracle(gpu, oracle, mode, shape, kshape):
+    rng = np.random.default_rng(hash((shape, kshape, mode)) % 2**32)
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    got = _run(y, psf, 10, mode)
+    st, ref = oracle.richardson_lucy(y, psf, 10, clip=False, mode=mode)
+    assert st == 0
+    _, ref64 = oracle.richardson_lucy(y.astype(np.float64), psf.astype(np.float64), 10, clip=False, mode=mode)
+    e32, e64 = oracle.rel_l2(got, ref), oracle.rel_l2(got, ref64)
+    print(f"{shape} {kshape} mode {mode}: rel-L2 vs FP32 {e32:.2e}, vs FP64 {e64:.2e}")
+    assert e32 < RTOL_L2
+
+
+CASES_3D = [((6, 9, 11), (3, 3, 5)), ((20, 30, 25), (5, 7, 3)), ((16, 32, 32), (7, 5, 5)), ((3, 4, 5), (5, 5, 5)),
+            ((1, 12, 13), (1, 3, 3)), ((24, 40, 36), (31, 15, 15))]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape,kshape", CASES_3D)
+def test_rl_3d_matches_oracle(gpu, oracle, mode, shape, kshape):
+    rng = np.random.default_rng(hash((shape, kshape, mode)) % 2**32)
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    got = _run(y, psf, 6, mode, nd=True)
+    st, ref = oracle.richardson_lucy(y, psf, 6, clip=False, mode=mode, nd=True)
+    assert st == 0
+    e = oracle.rel_l2(got, ref)
+    print(f"{shape} {kshape} mode {mode}: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
+
+
+def test_2d_through_nd_entry_matches(gpu, oracle):
+    rng = np.random.default_rng(1)
+    y = rng.uniform(50, 150, (45, 38)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (5, 6)).astype(np.float32)
+    a = _run(y, psf, 7, nd=False)
+    b = _run(y, psf, 7, nd=True)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_identity_kat(gpu):
+    """deconvolve.rs:519-539 within 1e-4 (assert_close semantics)."""
+    img = ((np.arange(64, dtype=np.float32) / 64.0) * 0.8 + 0.1).reshape(8, 8)
+    delta = np.zeros((3, 3), np.float32)
+    delta[1, 1] = 1.0
+    out = _run(img, delta, 15)
+    scale = np.abs(img).max()
+    assert np.all(np.abs(out - img) <= 1e-4 * np.abs(img) + 1e-4 * scale)
+
+
+def test_box_sharpening_kat(gpu, oracle):
+    """deconvolve.rs:551-566: clip on, 25 iterations, peak gain > 0.15."""
+    spot = np.full((8, 8), 0.1, np.float32)
+    spot[4, 4] = 0.9
+    box = np.full((3, 3), 1.0 / 9.0, np.float32)
+    blurred = oracle.correlate(spot, box, 0, 0.0)
+    out = _run(blurred, box, 25, clip=True)
+    assert out[4, 4] > blurred[4, 4] + 0.15
+    st, ref = oracle.richardson_lucy(blurred, box, 25, clip=True)
+    assert oracle.rel_l2(out, ref) < RTOL_L2
+
+
+def test_clip_only_on_last_iteration(gpu, oracle):
+    rng = np.random.default_rng(2)
+    y = rng.uniform(0.2, 0.9, (30, 31)).astype(np.float32)
+    y[10:14, 10:14] = 3.0
+    psf = rng.uniform(0.1, 1.0, (5, 5)).astype(np.float32)
+    got = _run(y, psf, 6, clip=True)
+    st, ref = oracle.richardson_lucy(y, psf, 6, clip=True)
+    assert got.max() <= 1.0 and oracle.rel_l2(got, ref) < RTOL_L2
+
+
+def test_shifted_delta_closed_form(gpu, oracle):
+    rng = np.random.default_rng(3)
+    y = rng.uniform(1.0, 2.0, (9, 7)).astype(np.float32)
+    h = np.zeros((3, 3), np.float32)
+    h[0, 1] = 1.0
+    out = _run(y, h, 3)
+    want = np.empty_like(y)
+    want[1:] = y[:-1]
+    want[0] = y[0]
+    np.testing.assert_allclose(out, want, rtol=1e-5)
+
+
+def test_zero_regions_and_filter_epsilon(gpu, oracle):
+    """Zero data under Constant padding: the eps replacement keeps ratios finite."""
+    y = np.zeros((40, 40), np.float32)
+    y[15:25, 15:25] = 100.0
+    psf = np.ones((5, 5), np.float32)
+    for eps in (None, 1e-3):
+        got = _run(y, psf, 5, mode=1, eps=eps)
+        st, ref = oracle.richardson_lucy(y, psf, 5, clip=False, mode=1, filter_epsilon=eps)
+        assert np.isfinite(got).all() and oracle.rel_l2(got, ref) < RTOL_L2
+
+
+def test_plain2d_full_size(gpu, oracle):
+    """1024x1024, 25x25 Gaussian sigma 2, 30 iterations (the 2-D benchmark workload)."""
+    from lucem_amd import synth
+
+    p = synth.plain2d()
+    got = _run(p.image, p.psf, p.num_iter)
+    st, ref = oracle.richardson_lucy(p.image, p.psf, p.num_iter, clip=False)
+    e = oracle.rel_l2(got, ref)
+    e64 = oracle.rel_l2(got, oracle.rl_fft64(p.image, p.psf, p.num_iter))
+    print(f"plain2d: rel-L2 vs FP32 {e:.2e}, vs FP64 {e64:.2e}")
+    assert e < RTOL_L2
+
+
+def test_plain3d_reduced_matches_oracle(gpu, oracle):
+    """The 3-D benchmark PSF (31x15x15, sigma (4,1.5,1.5)) on a 32x128x128 crop, 10 iterations."""
+    from lucem_amd import synth
+
+    p = synth.plain3d(shape=(32, 128, 128), count=60, num_iter=10)
+    got = _run(p.image, p.psf, p.num_iter)
+    st, ref = oracle.richardson_lucy(p.image, p.psf, p.num_iter, clip=False, nd=True)
+    e = oracle.rel_l2(got, ref)
+    print(f"plain3d crop: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
+
+
+def test_plain3d_full_size_properties(gpu):
+    """128x512x512 with the 31x15x15 PSF: a flat image is a fixed point of RL (Reflect), and
+    the estimate of positive data stays positive and finite over 50 iterations."""
+    import torch
+
+    from lucem_amd import synth
+
+    psf = synth.gaussian_psf((4.0, 1.5, 1.5), (31, 15, 15)).astype(np.float32)
+    flat = np.full((128, 512, 512), 250.0, np.float32)
+    out = _run(flat, psf, 5)
+    assert np.abs(out - 250.0).max() / 250.0 < 1e-4
+    rng = np.random.default_rng(4)
+    y = rng.uniform(50, 150, (128, 512, 512)).astype(np.float32)
+    out = _run(y, psf, 50)
+    assert np.isfinite(out).all() and out.min() > 0
+    del torch
+
+
+def test_workspace_reuse_and_growth(gpu, oracle):
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+
+    ws = N.Workspace(gpu)
+    rng = np.random.default_rng(5)
+    s = LA.RichardsonLucy(num_iter=4, clip=False)
+    caps = []
+    for shape in [(64, 64), (20, 30), (128, 100), (64, 64)]:
+        y = rng.uniform(50, 150, shape).astype(np.float32)
+        psf = rng.uniform(0.1, 1.0, (5, 5)).astype(np.float32)
+        out = torch.empty(shape, device="cuda")
+        LA.richardson_lucy_into(torch.from_numpy(y).cuda(), torch.from_numpy(psf).cuda(), out, s, ws, gpu)
+        gpu.sync()
+        caps.append(ws.capacity_bytes())
+        st, ref = oracle.richardson_lucy(y, psf, 4, clip=False)
+        assert oracle.rel_l2(out.cpu().numpy(), ref) < RTOL_L2
+    assert caps == sorted(caps) and caps[-1] == caps[-2]
+
+
+def test_errors_with_device(gpu):
+    import torch
+
+    import lucem_amd as LA
+
+    x = torch.ones(8, 8, device="cuda")
+    psf = torch.ones(3, 3, device="cuda")
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_into(x, psf, x)
+    assert e.value.kind == "ALIASED_BUFFERS"
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_into(x, psf, torch.empty(8, 9, device="cuda"))
+    assert e.value.kind == "SHAPE_MISMATCH"
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_into(torch.ones(2, 8, 8, device="cuda"), psf, torch.empty(2, 8, 8, device="cuda"))
+    assert e.value.kind == "INVALID_DIMENSION"
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_into(x, psf, torch.empty_like(x), LA.RichardsonLucy(num_iter=0))
+    assert e.value.kind == "INVALID_PARAMETER"
+
+
+def test_cpp_caller(gpu):
+    exe = ROOT / "tests" / "cpp" / "test_restore"
+    assert exe.exists(), "build() compiles tests/cpp/test_restore"
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and res.stdout.strip() == "ok", res.stderr
