@@ -1,11 +1,14 @@
-// LDS-resident mixed-radix FFT for gfx950 (wave64, 64 KiB of LDS per workgroup).
+// LDS-resident mixed-radix FFT for gfx950 (wave64).
 //
 // B lines of length N are stored interleaved in LDS, element i of line b at buf[i*B + b], so
-// consecutive lanes touch consecutive LDS words in every pass. Each Stockham pass of radix R
-// loads all of its butterfly inputs into registers, waits at a barrier, then computes and writes
-// in place: a pass needs N*B/threads complex values per thread of registers and no second LDS
-// buffer. Twiddles come from a table tw[m] = exp(-2*pi*i*m/N) built on the host in double
-// precision and rounded once to float.
+// consecutive lanes touch consecutive LDS words in every pass. Each Stockham pass of radix
+// R = R1*R2 loads all of its butterfly inputs into registers, waits at a barrier, then computes
+// and writes in place: a pass needs N*B/threads complex values per thread of registers and no
+// second LDS buffer. A composite radix R1*R2 is computed in registers as R2 butterflies of size
+// R1, internal twiddles, then R1 butterflies of size R2, so two radix stages cost one LDS round
+// trip. Inter-pass twiddles come from a table tw[m] = exp(-2*pi*i*m/N) built on the host in
+// double precision and rounded once to float; intra-butterfly constants are evaluated at compile
+// time from the Taylor series of sin and cos in double precision.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,9 +29,43 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// Compile-time sin/cos of 2*pi*m/R (Taylor series on [-pi, pi], double precision).
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int n = 1; n < 40; ++n) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 40; ++n) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double unit_angle(int m, int R) {
+    double a = 2.0 * kPi * (double)(m % R) / (double)R;
+    return a > kPi ? a - 2.0 * kPi : a;
+}
+// exp(-+2*pi*i*m/R) as float constants
+template <int M, int R, bool INV>
+struct Twiddle {
+    static constexpr float re = (float)taylor_cos(unit_angle(M, R));
+    static constexpr float im = (float)(INV ? taylor_sin(unit_angle(M, R)) : -taylor_sin(unit_angle(M, R)));
+};
+
 // Radix butterflies: v[s] <- sum_t v[t] * exp(-+2*pi*i*t*s/R).
 template <int R, bool INV>
 struct Butterfly;
+
+template <bool INV>
+struct Butterfly<1, INV> {
+    __device__ __forceinline__ static void run(float2*) {}
+};
 
 template <bool INV>
 struct Butterfly<2, INV> {
@@ -59,11 +96,10 @@ struct Butterfly<8, INV> {
         float2 o[4] = {v[1], v[3], v[5], v[7]};
         Butterfly<4, INV>::run(e);
         Butterfly<4, INV>::run(o);
-        // o[s] *= exp(-+2*pi*i*s/8)
         const float s1 = INV ? c : -c;
-        o[1] = make_float2(c * (o[1].x) - s1 * (o[1].y), c * (o[1].y) + s1 * (o[1].x));
+        o[1] = make_float2(c * o[1].x - s1 * o[1].y, c * o[1].y + s1 * o[1].x);
         o[2] = mul_mi<INV>(o[2]);
-        o[3] = make_float2(-c * (o[3].x) - s1 * (o[3].y), -c * (o[3].y) + s1 * (o[3].x));
+        o[3] = make_float2(-c * o[3].x - s1 * o[3].y, -c * o[3].y + s1 * o[3].x);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             v[s] = cadd(e[s], o[s]);
@@ -129,12 +165,66 @@ struct Butterfly<7, INV> {
     }
 };
 
-// One in-place Stockham pass of radix R over B = 1 << lgB interleaved lines of length N.
-// Input element j + t*N/R of each butterfly j is multiplied by w^(t*k), k = j mod Ns,
+// Multiplies a by the compile-time constant exp(-+2*pi*i*M/R).
+template <int M, int R, bool INV>
+__device__ __forceinline__ float2 mul_const(float2 a) {
+    if constexpr (M % R == 0) {
+        return a;
+    } else {
+        constexpr float c = Twiddle<M, R, INV>::re, s = Twiddle<M, R, INV>::im;
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// DFT of size R1*R2 in registers. Input t = R2*t1 + t2, output s = s1 + R1*s2.
+template <int R1, int R2, bool INV>
+struct Composite {
+    template <int T2, int S1>
+    __device__ __forceinline__ static void twiddle_row(float2* a) {
+        if constexpr (S1 < R1) {
+            a[S1 * R2 + T2] = mul_const<T2 * S1, R1 * R2, INV>(a[S1 * R2 + T2]);
+            twiddle_row<T2, S1 + 1>(a);
+        }
+    }
+    template <int T2>
+    __device__ __forceinline__ static void stage_a(const float2* v, float2* a) {
+        if constexpr (T2 < R2) {
+            float2 col[R1];
+#pragma unroll
+            for (int t1 = 0; t1 < R1; t1++) col[t1] = v[R2 * t1 + T2];
+            Butterfly<R1, INV>::run(col);
+#pragma unroll
+            for (int s1 = 0; s1 < R1; s1++) a[s1 * R2 + T2] = col[s1];
+            twiddle_row<T2, 1>(a);
+            stage_a<T2 + 1>(v, a);
+        }
+    }
+    __device__ __forceinline__ static void run(float2* v) {
+        if constexpr (R2 == 1) {
+            Butterfly<R1, INV>::run(v);
+        } else {
+            float2 a[R1 * R2];
+            stage_a<0>(v, a);
+#pragma unroll
+            for (int s1 = 0; s1 < R1; s1++) {
+                float2 row[R2];
+#pragma unroll
+                for (int t2 = 0; t2 < R2; t2++) row[t2] = a[s1 * R2 + t2];
+                Butterfly<R2, INV>::run(row);
+#pragma unroll
+                for (int s2 = 0; s2 < R2; s2++) v[s1 + R1 * s2] = row[s2];
+            }
+        }
+    }
+};
+
+// One in-place Stockham pass of radix R = R1*R2 over B = 1 << lgB interleaved lines of
+// length N. Input element j + t*N/R of butterfly j is multiplied by w^(t*k), k = j mod Ns,
 // w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns.
-template <int R, bool INV, int T, int E>
-__device__ __forceinline__ void stockham_pass(float2* __restrict__ buf, int N, int lgB, int Ns,
+template <int R1, int R2, bool INV, int T, int E>
+__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, int lgB, int Ns,
                                               const float2* __restrict__ tw) {
+    constexpr int R = R1 * R2;
     constexpr int S = (E + R - 1) / R;
     const int B = 1 << lgB;
     const int NR = N / R;
@@ -158,14 +248,15 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ buf, int N, i
             const int b = u & (B - 1), j = u >> lgB;
             const int k = j % Ns;
             if (Ns > 1) {
+                const int step = k * tws;
 #pragma unroll
                 for (int t = 1; t < R; t++) {
-                    float2 w = tw[t * k * tws];
+                    float2 w = tw[t * step];
                     if (INV) w.y = -w.y;
                     v[s][t] = cmul(v[s][t], w);
                 }
             }
-            Butterfly<R, INV>::run(v[s]);
+            Composite<R1, R2, INV>::run(v[s]);
             const int o = (j - k) * R + k;
 #pragma unroll
             for (int t = 0; t < R; t++) buf[((o + t * Ns) << lgB) + b] = v[s][t];
@@ -174,23 +265,44 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ buf, int N, i
     __syncthreads();
 }
 
+// Radix codes of a pass: R1 in the low nibble, R2 in the high nibble (R2 = 1 for a simple pass).
+constexpr unsigned char radix_code(int r1, int r2) { return (unsigned char)(r1 | (r2 << 4)); }
+
 // Full transform of the interleaved lines. The caller has written buf and synchronised.
 template <bool INV, int T, int E>
 __device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, const float2* __restrict__ twbase) {
     const float2* tw = twbase + p.tw;
     const int N = p.L;
     int Ns = 1;
-    for (int q = 0; q < p.npass; q++) {
-        const int R = p.rad[q];
-        switch (R) {
-            case 8: stockham_pass<8, INV, T, E>(buf, N, lgB, Ns, tw); break;
-            case 4: stockham_pass<4, INV, T, E>(buf, N, lgB, Ns, tw); break;
-            case 2: stockham_pass<2, INV, T, E>(buf, N, lgB, Ns, tw); break;
-            case 3: stockham_pass<3, INV, T, E>(buf, N, lgB, Ns, tw); break;
-            case 5: stockham_pass<5, INV, T, E>(buf, N, lgB, Ns, tw); break;
-            default: stockham_pass<7, INV, T, E>(buf, N, lgB, Ns, tw); break;
+    // fully unrolled so that the radix codes are read with scalar loads at fixed offsets
+#pragma unroll
+    for (int q = 0; q < kMaxPasses; q++) {
+        if (q >= p.npass) break;
+        const int code = p.rad[q];
+        switch (code) {
+#define LUCEM_PASS(r1, r2)                                                        \
+    case radix_code(r1, r2):                                                      \
+        stockham_pass<r1, r2, INV, T, E>(buf, N, lgB, Ns, tw);                    \
+        Ns *= r1 * r2;                                                            \
+        break;
+            LUCEM_PASS(2, 1)
+            LUCEM_PASS(3, 1)
+            LUCEM_PASS(4, 1)
+            LUCEM_PASS(5, 1)
+            LUCEM_PASS(7, 1)
+            LUCEM_PASS(8, 1)
+            LUCEM_PASS(3, 2)
+            LUCEM_PASS(5, 2)
+            LUCEM_PASS(7, 2)
+            LUCEM_PASS(8, 2)
+            LUCEM_PASS(3, 3)
+            LUCEM_PASS(4, 3)
+            LUCEM_PASS(5, 3)
+            LUCEM_PASS(4, 4)
+#undef LUCEM_PASS
+            default:
+                break;
         }
-        Ns *= R;
     }
 }
 
@@ -212,6 +324,15 @@ __device__ __forceinline__ int pad_index(int p, int len, int mode) {
             return r < len ? r : period - r;
         }
     }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8). Renumbering them so
+// that each XCD gets a contiguous range of logical blocks keeps neighbouring column blocks, which
+// share cache lines, in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    const int x = b & 7, slot = b >> 3;
+    return x * per + min(x, rem) + slot;
 }
 
 }  // namespace lucem

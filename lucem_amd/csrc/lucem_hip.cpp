@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -336,43 +337,82 @@ int lucem::choose_fft_len(int target) {
 }
 
 FftPlan lucem::make_plan(int L) {
+    // Prime factors, then greedy grouping into the largest supported composite radix:
+    // 16 = 4*4, 15 = 5*3, 14 = 7*2, 12 = 4*3, 10 = 5*2, 9 = 3*3, 8, 7, 6 = 3*2, 5, 4, 3, 2.
     FftPlan p{};
     p.L = L;
     p.npass = 0;
     p.tw = 0;
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int r = L;
-    while (r % 8 == 0 && r > 1) {
-        p.rad[p.npass++] = 8;
-        r /= 8;
-    }
-    while (r % 4 == 0) {
-        p.rad[p.npass++] = 4;
-        r /= 4;
-    }
-    while (r % 2 == 0) {
-        p.rad[p.npass++] = 2;
-        r /= 2;
-    }
-    for (int q : {3, 5, 7})
-        while (r % q == 0) {
-            p.rad[p.npass++] = (unsigned char)q;
+    for (int q : {2, 3, 5, 7})
+        while (r > 1 && r % q == 0) {
+            cnt[q]++;
             r /= q;
         }
+    struct Code {
+        int r1, r2, n2, n3, n5, n7;
+    };
+    static const Code codes[] = {
+        {4, 4, 4, 0, 0, 0}, {5, 3, 0, 1, 1, 0}, {7, 2, 1, 0, 0, 1}, {4, 3, 2, 1, 0, 0}, {5, 2, 1, 0, 1, 0},
+        {3, 3, 0, 2, 0, 0}, {8, 1, 3, 0, 0, 0}, {7, 1, 0, 0, 0, 1}, {3, 2, 1, 1, 0, 0}, {5, 1, 0, 0, 1, 0},
+        {4, 1, 2, 0, 0, 0}, {3, 1, 0, 1, 0, 0}, {2, 1, 1, 0, 0, 0},
+    };
+    for (;;) {
+        bool found = false;
+        for (const Code& c : codes) {
+            if (cnt[2] >= c.n2 && cnt[3] >= c.n3 && cnt[5] >= c.n5 && cnt[7] >= c.n7) {
+                cnt[2] -= c.n2;
+                cnt[3] -= c.n3;
+                cnt[5] -= c.n5;
+                cnt[7] -= c.n7;
+                p.rad[p.npass++] = (unsigned char)(c.r1 | (c.r2 << 4));
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
     return p;
 }
 
+// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3,col_wide=1") overrides the choices, for
+// measurement sweeps; the autotuner records its choices per shape in the same terms.
+static int tune_value(const char* key) {
+    const char* env = std::getenv("LUCEM_TUNE");
+    if (!env) return -1;
+    const std::string s(env), k = std::string(key) + "=";
+    size_t pos = 0;
+    while ((pos = s.find(k, pos)) != std::string::npos) {
+        if (pos == 0 || s[pos - 1] == ',') return std::atoi(s.c_str() + pos + k.size());
+        pos += k.size();
+    }
+    return -1;
+}
+
 int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
+    (void)nx;
     const int pairs = (rows + 1) / 2;
     int lgP = 3;
-    while (lgP > 0 && ((size_t)((1 << lgP) * (L + nx)) > (size_t)kLdsComplex)) lgP--;
-    while (lgP > 0 && (pairs >> lgP) < 2 * num_cus) lgP--;
+    while (lgP > 0 && ((size_t)L << lgP) > (size_t)kLdsComplex) lgP--;
+    while (lgP > 0 && (pairs >> lgP) < 4 * num_cus) lgP--;
+    const int t = tune_value("row_lgP");
+    if (t >= 0 && ((size_t)L << t) <= (size_t)kLdsComplex) lgP = t;
     return lgP;
 }
 
-int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus) {
+int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide) {
+    int w = tune_value("col_wide");
+    if (w < 0) w = 0;  // measured: the 32 KiB shape wins even at B = 4 (more workgroups per CU)
+    const size_t cap = w ? kLdsComplexWide : kLdsComplex;
     int lgB = 4;
-    while (lgB > 0 && ((size_t)L << lgB) > (size_t)kLdsComplex) lgB--;
-    while (lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 2 * num_cus) lgB--;
+    while (lgB > 0 && ((size_t)L << lgB) > cap) lgB--;
+    while (lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 4 * num_cus) lgB--;
+    const int t = tune_value("col_lgB");
+    if (t >= 0 && ((size_t)L << t) <= cap) lgB = t;
+    // narrow lines that fit the small shape do not need the wide one
+    if (w && ((size_t)L << lgB) <= (size_t)kLdsComplex && tune_value("col_wide") < 0) w = 0;
+    *wide = w;
     return lgB;
 }
 
@@ -389,9 +429,6 @@ bool lucem::make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geom
     }
     g->Lc = g->L[2] / 2 + 1;
     g->pitch = (g->Lc + 15) & ~15;
-    // the x pass must hold L_x + n_x complex-equivalents for one row pair
-    if (g->L[2] + g->n[2] > kLdsComplex) return false;
-    if (g->L[2] + g->k[2] > kLdsComplex) return false;
     return true;
 }
 
@@ -426,7 +463,7 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     if (st) return st;
 
     ColPass y{};
-    y.op = COL_FWD;
+    y.op = g.ndim == 2 ? COL_FWD_OTF : COL_FWD;
     y.n_src = m[1];
     y.left = 0;
     y.mode = LUCEM_PAD_CONSTANT;
@@ -438,25 +475,24 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     y.src_outer = (int64_t)m[1] * g.pitch;
     y.src_axis = g.pitch;
     if (g.ndim == 2) {
-        y.finalize = 1;
         y.scale = scale;
         y.dst = otf;
         y.dst_outer = 0;
         y.dst_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus, &y.wide);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
         return run_col(c, y, y.nkb, "otf_line_fwd_y");
     }
     y.dst = tmp_b;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], c->num_cus);
+    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], c->num_cus, &y.wide);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
     st = run_col(c, y, m[0] * y.nkb, "otf_line_fwd_y");
     if (st) return st;
 
     ColPass z{};
-    z.op = COL_FWD;
+    z.op = COL_FWD_OTF;
     z.n_src = m[0];
     z.left = 0;
     z.mode = LUCEM_PAD_CONSTANT;
@@ -470,9 +506,8 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     z.dst = otf;
     z.dst_outer = g.pitch;
     z.dst_axis = (int64_t)g.L[1] * g.pitch;
-    z.finalize = 1;
     z.scale = scale;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus, &z.wide);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
     return run_col(c, z, g.L[1] * z.nkb, "otf_line_fwd_z");
 }
@@ -497,7 +532,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
         y.otf = otf;
         y.otf_outer = 0;
         y.otf_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus);
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus, &y.wide);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
         return run_col(c, y, y.nkb, "line_fused_y");
     }
@@ -511,7 +546,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     y.dst = sb;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], c->num_cus);
+    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], c->num_cus, &y.wide);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
     lucem_status st = run_col(c, y, g.n[0] * y.nkb, "line_fwd_y");
     if (st) return st;
@@ -532,7 +567,7 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     z.otf = otf;
     z.otf_outer = g.pitch;
     z.otf_axis = (int64_t)g.L[1] * g.pitch;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus);
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus, &z.wide);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
     st = run_col(c, z, g.L[1] * z.nkb, "line_fused_z");
     if (st) return st;

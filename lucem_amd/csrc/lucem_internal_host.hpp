@@ -71,7 +71,7 @@ lucem_status ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void**
 int choose_fft_len(int target);
 FftPlan make_plan(int L);
 int choose_lgP(int L, int nx, int rows, int num_cus);
-int choose_lgB(int L, int Lc, int64_t outers, int num_cus);
+int choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide);
 bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g);
 
 // Launch with optional profiling (events around the launch, algorithmic bytes recorded).

@@ -14,9 +14,11 @@
 namespace lucem {
 
 constexpr int kMaxPasses = 16;
-constexpr int kThreads = 512;         // threads per workgroup of every FFT kernel
-constexpr int kElemsPerThread = 16;   // complex values held per thread in one radix pass
-constexpr int kLdsComplex = kThreads * kElemsPerThread;  // 8192 complex = 64 KiB of LDS
+constexpr int kThreads = 256;         // threads per workgroup of every FFT kernel
+constexpr int kElemsPerThread = 16;   // complex values per thread: 4096 per workgroup, 32 KiB LDS
+constexpr int kElemsWide = 32;        // long lines: 8192 complex per workgroup, 64 KiB LDS
+constexpr int kLdsComplex = kThreads * kElemsPerThread;
+constexpr int kLdsComplexWide = kThreads * kElemsWide;
 
 // One FFT length and its Stockham radix sequence.
 struct FftPlan {
@@ -37,6 +39,7 @@ enum ColOp : int {
     COL_FWD = 0,    // gather along the axis, forward FFT, store all L bins
     COL_FUSED = 1,  // gather, forward FFT, multiply by the transfer function, inverse FFT, crop
     COL_INV = 2,    // load L bins, inverse FFT, crop
+    COL_FWD_OTF = 3,  // as COL_FWD, storing conj(v) * scale (transfer function build)
 };
 
 // Row (x axis) pass over `rows` real rows of length nx, paired two by two.
@@ -46,6 +49,7 @@ struct RowPass {
     int left, mode;      // pad gather along x
     int Lc, pitch;       // half-spectrum bins per row, row pitch (complex) of `spec`
     int lgP;             // log2(row pairs per workgroup)
+    int wide;            // 1: the 8192-value kernel shape
     int first;           // ROW_UPDATE: the previous estimate is the constant `init`
     int last;            // ROW_UPDATE: write `out`, no forward transform
     int clip;            // ROW_UPDATE && last: clamp to [-1, 1]
@@ -69,8 +73,8 @@ struct ColPass {
     int Lc;               // valid columns
     int lgB;              // log2(columns per workgroup)
     int nkb;              // column blocks per outer index
-    int finalize;         // COL_FWD: store conj(v) * scale (transfer function build)
-    float scale;
+    int wide;             // 1: the 8192-value kernel shape
+    float scale;          // COL_FWD_OTF
     int64_t src_outer, src_axis, dst_outer, dst_axis, otf_outer, otf_axis;
     FftPlan f;
     const float2* src;
@@ -95,6 +99,7 @@ struct RowBatch {
     const int2* items;      // (tile, row-pair block)
     int nitems;
     int op, first, last, clip;  // overrides of the per-tile fields for this launch
+    int wide;
     int* counter;
 };
 
@@ -102,6 +107,8 @@ struct ColBatch {
     const ColPass* tiles;
     const int2* items;      // (tile, workgroup index within the tile's pass)
     int nitems;
+    int op;
+    int wide;
     int* counter;
 };
 
@@ -118,10 +125,9 @@ int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream);
 int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream);
 int launch_psf_batch(const PsfBatch& b, void* stream);
 int launch_zero_ints(int* p, int n, void* stream);
-// Workgroups of the batched kernels that fit on one CU with `lds` bytes of dynamic LDS.
-int row_batch_occupancy(size_t lds);
-int col_batch_occupancy(size_t lds);
-size_t row_pass_lds_bytes(int L, int nx, int lgP);
+// Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
+int batch_occupancy(bool row, bool wide, size_t lds);
+size_t row_pass_lds_bytes(int L, int lgP);
 size_t col_pass_lds_bytes(int L, int lgB);
 
 }  // namespace lucem

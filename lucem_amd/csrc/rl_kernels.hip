@@ -20,110 +20,171 @@ namespace lucem {
 
 // ---------------------------------------------------------------------------------------------
 // Row pass. Workgroup g owns row pairs [g*P, g*P + P); pair p is rows (2p, 2p + 1).
-// LDS: Z[L * P] (interleaved complex lines) followed by stage[P][2][nx] (real rows).
+// LDS: Z[L * P] (interleaved complex lines, element i of pair b at Z[i*P + b]).
 // ---------------------------------------------------------------------------------------------
-template <int T, int E>
-__device__ void row_pass_body(const RowPass& d, int g, float2* lds) {
-    const int P = 1 << d.lgP;
+// Per-launch flags of a row pass (the iteration it belongs to).
+struct RowFlags {
+    int first, last, clip;
+};
+
+template <int T, int E, int OP>
+__device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2* Z) {
+    constexpr int S = E;  // elements per thread in the gather (L * P <= T * E)
+    const int lgP = d.lgP;
+    const int P = 1 << lgP;
     const int L = d.f.L;
     const int nx = d.nx;
-    float2* Z = lds;
-    float* stage = reinterpret_cast<float*>(lds + (size_t)L * P);
     const int tid = threadIdx.x;
     const int pair0 = g * P;
+    const int total = L << lgP;
 
-    if (d.op == ROW_RATIO || d.op == ROW_UPDATE) {
-        // Z[k] = A[k] + i B[k] for k < Lc, and the Hermitian mirror for k >= Lc.
-        for (int b = 0; b < P; b++) {
-            const int ra = 2 * (pair0 + b), rb = ra + 1;
-            const bool va = ra < d.rows, vb = rb < d.rows;
-            const float2* Ar = d.spec + (int64_t)ra * d.pitch;
-            const float2* Br = d.spec + (int64_t)rb * d.pitch;
-            for (int k = tid; k < d.Lc; k += T) {
-                float2 A = va ? Ar[k] : make_float2(0.f, 0.f);
-                float2 Bv = vb ? Br[k] : make_float2(0.f, 0.f);
+    if constexpr (OP == ROW_RATIO || OP == ROW_UPDATE) {
+        // Z[k] = A[k] + i B[k] for k < Lc, and the Hermitian mirror for k >= Lc. All loads are
+        // issued before the first LDS write (item e = b * Lc + k, P * Lc <= T * KS).
+        constexpr int KS = (E + 1) / 2 + 1;
+        const int Lc = d.Lc;
+        const int nload = Lc << lgP;
+        const float inv_lc = 1.0f / (float)Lc;
+        float2 av[KS], bv[KS];
+        int kk[KS], bb[KS];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int e = tid + s * T;
+            int b = __float2int_rz((float)e * inv_lc);
+            int k = e - b * Lc;
+            if (k >= Lc) { k -= Lc; b++; }
+            if (k < 0) { k += Lc; b--; }
+            kk[s] = k;
+            bb[s] = b;
+            av[s] = make_float2(0.f, 0.f);
+            bv[s] = make_float2(0.f, 0.f);
+            if (e < nload) {
+                const int ra = 2 * (pair0 + b), rb = ra + 1;
+                if (ra < d.rows) av[s] = d.spec[(int64_t)ra * d.pitch + k];
+                if (rb < d.rows) bv[s] = d.spec[(int64_t)rb * d.pitch + k];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int e = tid + s * T;
+            if (e < nload) {
+                const int k = kk[s], b = bb[s];
+                float2 A = av[s], Bv = bv[s];
                 if (k == 0 || 2 * k == L) {
                     A.y = 0.f;
                     Bv.y = 0.f;
                 }
-                Z[k * P + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+                Z[(k << lgP) + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
                 const int km = L - k;
-                if (k > 0 && km >= d.Lc) Z[km * P + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+                if (k > 0 && km >= Lc) Z[(km << lgP) + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, d.lgP, d.f, d.tw);
-        // crop to [0, nx) and apply the pointwise step
-        for (int b = 0; b < P; b++) {
-            const int ra = 2 * (pair0 + b), rb = ra + 1;
-            const bool va = ra < d.rows, vb = rb < d.rows;
-            float* sa = stage + (size_t)(2 * b) * nx;
-            float* sb = sa + nx;
-            for (int x = tid; x < nx; x += T) {
-                const float2 c = Z[x * P + b];
-                float oa = 0.f, ob = 0.f;
-                if (d.op == ROW_RATIO) {
-                    if (va) {
-                        const float den = fabsf(c.x) < d.eps ? d.eps : c.x;
-                        oa = d.obs[(int64_t)ra * nx + x] / den;
-                    }
-                    if (vb) {
-                        const float den = fabsf(c.y) < d.eps ? d.eps : c.y;
-                        ob = d.obs[(int64_t)rb * nx + x] / den;
-                    }
-                } else {
-                    float* dst = d.last ? d.out : d.est;
-                    if (va) {
-                        const float prev = d.first ? d.init : d.est[(int64_t)ra * nx + x];
-                        oa = prev * c.x;
-                        if (d.last && d.clip) oa = fminf(fmaxf(oa, -1.f), 1.f);
-                        dst[(int64_t)ra * nx + x] = oa;
-                    }
-                    if (vb) {
-                        const float prev = d.first ? d.init : d.est[(int64_t)rb * nx + x];
-                        ob = prev * c.y;
-                        if (d.last && d.clip) ob = fminf(fmaxf(ob, -1.f), 1.f);
-                        dst[(int64_t)rb * nx + x] = ob;
+        lds_fft<true, T, E>(Z, lgP, d.f, d.tw);
+        // Gather into registers: padded position i reads the cropped result at src = pad(i - left)
+        // and applies the pointwise step there. Every value is read before any is overwritten.
+        float2 val[S];
+        int srcs[S];
+        // global operands first (observed image or previous estimate at the source column)
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = tid + s * T;
+            srcs[s] = -1;
+            val[s] = make_float2(0.f, 0.f);
+            if (e < total) {
+                const int b = e & (P - 1), i = e >> lgP;
+                const int src = pad_index(i - d.left, nx, d.mode);
+                srcs[s] = src;
+                if (src >= 0) {
+                    const int ra = 2 * (pair0 + b), rb = ra + 1;
+                    if constexpr (OP == ROW_RATIO) {
+                        if (ra < d.rows) val[s].x = d.obs[(int64_t)ra * nx + src];
+                        if (rb < d.rows) val[s].y = d.obs[(int64_t)rb * nx + src];
+                    } else {
+                        if (!fl.first) {
+                            if (ra < d.rows) val[s].x = d.est[(int64_t)ra * nx + src];
+                            if (rb < d.rows) val[s].y = d.est[(int64_t)rb * nx + src];
+                        } else {
+                            val[s] = make_float2(d.init, d.init);
+                        }
                     }
                 }
-                sa[x] = oa;
-                sb[x] = ob;
             }
         }
-        if (d.op == ROW_UPDATE && d.last) return;  // uniform: the whole workgroup leaves
-        __syncthreads();
-        // pad gather from the staged rows
-        for (int b = 0; b < P; b++) {
-            const float* sa = stage + (size_t)(2 * b) * nx;
-            const float* sb = sa + nx;
-            for (int i = tid; i < L; i += T) {
-                const int s = pad_index(i - d.left, nx, d.mode);
-                Z[i * P + b] = s < 0 ? make_float2(0.f, 0.f) : make_float2(sa[s], sb[s]);
+        // then the pointwise step with the correlation read from LDS
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = tid + s * T;
+            const int src = srcs[s];
+            if (e < total && src >= 0) {
+                const int b = e & (P - 1);
+                const float2 c = Z[(src << lgP) + b];
+                if constexpr (OP == ROW_RATIO) {
+                    const float da = fabsf(c.x) < d.eps ? d.eps : c.x;
+                    const float db = fabsf(c.y) < d.eps ? d.eps : c.y;
+                    val[s] = make_float2(val[s].x / da, val[s].y / db);
+                } else {
+                    float oa = val[s].x * c.x, ob = val[s].y * c.y;
+                    if (fl.last && fl.clip) {
+                        oa = fminf(fmaxf(oa, -1.f), 1.f);
+                        ob = fminf(fmaxf(ob, -1.f), 1.f);
+                    }
+                    val[s] = make_float2(oa, ob);
+                }
             }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = tid + s * T;
+            if (e < total) {
+                if constexpr (OP == ROW_UPDATE) {
+                    const int i = e >> lgP;
+                    if (srcs[s] >= 0 && i == srcs[s] + d.left) {
+                        const int b = e & (P - 1);
+                        const int x = i - d.left;
+                        const int ra = 2 * (pair0 + b), rb = ra + 1;
+                        float* dst = fl.last ? d.out : d.est;
+                        if (ra < d.rows) dst[(int64_t)ra * nx + x] = val[s].x;
+                        if (rb < d.rows) dst[(int64_t)rb * nx + x] = val[s].y;
+                    }
+                }
+                Z[e] = val[s];
+            }
+        }
+        if constexpr (OP == ROW_UPDATE) {
+            if (fl.last) return;  // uniform: the whole workgroup leaves
         }
     } else {
         // pad gather straight from global memory (or the constant start value)
-        for (int b = 0; b < P; b++) {
-            const int ra = 2 * (pair0 + b), rb = ra + 1;
-            const bool va = ra < d.rows, vb = rb < d.rows;
-            const float* Ar = d.src + (int64_t)ra * nx;
-            const float* Br = d.src + (int64_t)rb * nx;
-            for (int i = tid; i < L; i += T) {
-                const int s = pad_index(i - d.left, nx, d.mode);
-                float2 z = make_float2(0.f, 0.f);
-                if (s >= 0) {
-                    if (d.op == ROW_LOAD_CONST) {
-                        z = make_float2(va ? d.init : 0.f, vb ? d.init : 0.f);
+        float2 z[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = tid + s * T;
+            z[s] = make_float2(0.f, 0.f);
+            if (e < total) {
+                const int b = e & (P - 1), i = e >> lgP;
+                const int ra = 2 * (pair0 + b), rb = ra + 1;
+                const bool va = ra < d.rows, vb = rb < d.rows;
+                const int src = pad_index(i - d.left, nx, d.mode);
+                if (src >= 0) {
+                    if constexpr (OP == ROW_LOAD_CONST) {
+                        z[s] = make_float2(va ? d.init : 0.f, vb ? d.init : 0.f);
                     } else {
-                        z = make_float2(va ? Ar[s] : 0.f, vb ? Br[s] : 0.f);
+                        if (va) z[s].x = d.src[(int64_t)ra * nx + src];
+                        if (vb) z[s].y = d.src[(int64_t)rb * nx + src];
                     }
                 }
-                Z[i * P + b] = z;
             }
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = tid + s * T;
+            if (e < total) Z[e] = z[s];
         }
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, d.lgP, d.f, d.tw);
+    lds_fft<false, T, E>(Z, lgP, d.f, d.tw);
     // split the two real transforms: A = Z[k] + conj(Z[L-k]), B = -i (Z[k] - conj(Z[L-k]))
     // (both carry a factor 2 that the transfer function removes)
     for (int b = 0; b < P; b++) {
@@ -132,70 +193,94 @@ __device__ void row_pass_body(const RowPass& d, int g, float2* lds) {
         float2* Ar = d.spec + (int64_t)ra * d.pitch;
         float2* Br = d.spec + (int64_t)rb * d.pitch;
         for (int k = tid; k < d.Lc; k += T) {
-            const float2 zk = Z[k * P + b];
-            const float2 zm = Z[(k == 0 ? 0 : L - k) * P + b];
+            const float2 zk = Z[(k << lgP) + b];
+            const float2 zm = Z[((k == 0 ? 0 : L - k) << lgP) + b];
             if (va) Ar[k] = make_float2(zk.x + zm.x, zk.y - zm.y);
             if (vb) Br[k] = make_float2(zk.y + zm.y, zm.x - zk.x);
         }
     }
 }
 
-template <int T, int E>
+template <int T, int E, int OP>
 __global__ __launch_bounds__(T) void row_pass_kernel(RowPass d) {
     extern __shared__ float2 lds[];
-    row_pass_body<T, E>(d, blockIdx.x, lds);
+    row_pass_body<T, E, OP>(d, RowFlags{d.first, d.last, d.clip}, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Line pass along y or z. Workgroup g owns columns [kb*B, kb*B + B) of outer index `outer`.
 // ---------------------------------------------------------------------------------------------
-template <int T, int E>
+template <int T, int E, int OP>
 __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
-    const int B = 1 << d.lgB;
+    const int lgB = d.lgB;
+    const int B = 1 << lgB;
     const int outer = g / d.nkb;
     const int kx0 = (g - outer * d.nkb) * B;
     const int L = d.f.L;
     const int tid = threadIdx.x;
     const float2* src = d.src + outer * d.src_outer + kx0;
-    const int total = L << d.lgB;
-    for (int e = tid; e < total; e += T) {
-        const int b = e & (B - 1), i = e >> d.lgB;
-        const int s = d.op == COL_INV ? i : pad_index(i - d.left, d.n_src, d.mode);
-        float2 v = make_float2(0.f, 0.f);
-        if (s >= 0 && kx0 + b < d.Lc) v = src[s * d.src_axis + b];
-        Z[e] = v;
+    const int total = L << lgB;
+    const bool full = kx0 + B <= d.Lc;
+    {
+        float2 v[E];
+#pragma unroll
+        for (int s = 0; s < E; s++) {
+            const int e = tid + s * T;
+            v[s] = make_float2(0.f, 0.f);
+            if (e < total) {
+                const int b = e & (B - 1), i = e >> lgB;
+                const int sr = OP == COL_INV ? i : pad_index(i - d.left, d.n_src, d.mode);
+                if (sr >= 0 && (full || kx0 + b < d.Lc)) v[s] = src[sr * d.src_axis + b];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < E; s++) {
+            const int e = tid + s * T;
+            if (e < total) Z[e] = v[s];
+        }
     }
     __syncthreads();
-    if (d.op == COL_INV) {
-        lds_fft<true, T, E>(Z, d.lgB, d.f, d.tw);
+    if constexpr (OP == COL_INV) {
+        lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
     } else {
-        lds_fft<false, T, E>(Z, d.lgB, d.f, d.tw);
-        if (d.op == COL_FUSED) {
+        lds_fft<false, T, E>(Z, lgB, d.f, d.tw);
+        if constexpr (OP == COL_FUSED) {
             const float2* otf = d.otf + outer * d.otf_outer + kx0;
-            for (int e = tid; e < total; e += T) {
-                const int b = e & (B - 1), i = e >> d.lgB;
-                if (kx0 + b < d.Lc) Z[e] = cmul(Z[e], otf[i * d.otf_axis + b]);
+            float2 w[E];
+#pragma unroll
+            for (int s = 0; s < E; s++) {
+                const int e = tid + s * T;
+                w[s] = make_float2(0.f, 0.f);
+                if (e < total) {
+                    const int b = e & (B - 1), i = e >> lgB;
+                    if (full || kx0 + b < d.Lc) w[s] = otf[i * d.otf_axis + b];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < E; s++) {
+                const int e = tid + s * T;
+                if (e < total) Z[e] = cmul(Z[e], w[s]);
             }
             __syncthreads();
-            lds_fft<true, T, E>(Z, d.lgB, d.f, d.tw);
+            lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
         }
     }
     float2* dst = d.dst + outer * d.dst_outer + kx0;
-    const int nstore = d.n_dst << d.lgB;
+    const int nstore = d.n_dst << lgB;
     for (int e = tid; e < nstore; e += T) {
-        const int b = e & (B - 1), i = e >> d.lgB;
-        if (kx0 + b < d.Lc) {
+        const int b = e & (B - 1), i = e >> lgB;
+        if (full || kx0 + b < d.Lc) {
             float2 v = Z[e];
-            if (d.finalize) v = make_float2(v.x * d.scale, -v.y * d.scale);
+            if constexpr (OP == COL_FWD_OTF) v = make_float2(v.x * d.scale, -v.y * d.scale);
             dst[i * d.dst_axis + b] = v;
         }
     }
 }
 
-template <int T, int E>
+template <int T, int E, int OP>
 __global__ __launch_bounds__(T) void col_pass_kernel(ColPass d) {
     extern __shared__ float2 lds[];
-    col_pass_body<T, E>(d, blockIdx.x, lds);
+    col_pass_body<T, E, OP>(d, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -232,7 +317,7 @@ __device__ __forceinline__ int next_item(int* counter, int* slot) {
     return it;
 }
 
-template <int T, int E>
+template <int T, int E, int OP>
 __global__ __launch_bounds__(T) void row_batch_kernel(RowBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
@@ -240,17 +325,13 @@ __global__ __launch_bounds__(T) void row_batch_kernel(RowBatch bt) {
         const int it = next_item(bt.counter, &slot);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
-        RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
-        d.op = bt.op;
-        d.first = bt.first;
-        d.last = bt.last;
-        d.clip = bt.clip;
-        row_pass_body<T, E>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
+        const RowPass& d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        row_pass_body<T, E, OP>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
 }
 
-template <int T, int E>
+template <int T, int E, int OP>
 __global__ __launch_bounds__(T) void col_batch_kernel(ColBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
@@ -258,8 +339,8 @@ __global__ __launch_bounds__(T) void col_batch_kernel(ColBatch bt) {
         const int it = next_item(bt.counter, &slot);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
-        const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
-        col_pass_body<T, E>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
+        const ColPass& d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        col_pass_body<T, E, OP>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
 }
@@ -290,37 +371,79 @@ __global__ void zero_ints_kernel(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-size_t row_pass_lds_bytes(int L, int nx, int lgP) {
-    const size_t P = (size_t)1 << lgP;
-    return P * (size_t)L * sizeof(float2) + P * 2 * (size_t)nx * sizeof(float);
-}
+// Launchers. Kernels come in two shapes: 256 threads x 16 values (32 KiB of LDS at most) and
+// 256 threads x 32 values (64 KiB) for long lines; `wide` picks the second.
+// ---------------------------------------------------------------------------------------------
+size_t row_pass_lds_bytes(int L, int lgP) { return ((size_t)L << lgP) * sizeof(float2); }
 
 size_t col_pass_lds_bytes(int L, int lgB) { return ((size_t)L << lgB) * sizeof(float2); }
 
+template <int E>
+static int launch_row_e(const RowPass& p, int blocks, void* stream) {
+    const size_t lds = row_pass_lds_bytes(p.f.L, p.lgP);
+    hipStream_t st = (hipStream_t)stream;
+    switch (p.op) {
+        case ROW_LOAD: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_LOAD>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case ROW_LOAD_CONST: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_LOAD_CONST>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case ROW_RATIO: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_RATIO>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        default: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_UPDATE>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+    }
+    return (int)hipGetLastError();
+}
+
 int launch_row_pass(const RowPass& p, int blocks, void* stream) {
-    const size_t lds = row_pass_lds_bytes(p.f.L, p.nx, p.lgP);
-    hipLaunchKernelGGL((row_pass_kernel<kThreads, kElemsPerThread>), dim3(blocks), dim3(kThreads), lds,
-                       (hipStream_t)stream, p);
+    return p.wide ? launch_row_e<kElemsWide>(p, blocks, stream) : launch_row_e<kElemsPerThread>(p, blocks, stream);
+}
+
+template <int E>
+static int launch_col_e(const ColPass& p, int blocks, void* stream) {
+    const size_t lds = col_pass_lds_bytes(p.f.L, p.lgB);
+    hipStream_t st = (hipStream_t)stream;
+    switch (p.op) {
+        case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FWD>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case COL_FUSED: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FUSED>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case COL_INV: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_INV>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        default: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FWD_OTF>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+    }
     return (int)hipGetLastError();
 }
 
 int launch_col_pass(const ColPass& p, int blocks, void* stream) {
-    const size_t lds = col_pass_lds_bytes(p.f.L, p.lgB);
-    hipLaunchKernelGGL((col_pass_kernel<kThreads, kElemsPerThread>), dim3(blocks), dim3(kThreads), lds,
-                       (hipStream_t)stream, p);
+    return p.wide ? launch_col_e<kElemsWide>(p, blocks, stream) : launch_col_e<kElemsPerThread>(p, blocks, stream);
+}
+
+template <int E>
+static int launch_row_batch_e(const RowBatch& b, int grid, size_t lds, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    switch (b.op) {
+        case ROW_LOAD: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_LOAD>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case ROW_LOAD_CONST: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_LOAD_CONST>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case ROW_RATIO: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_RATIO>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        default: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_UPDATE>), dim3(grid), dim3(kThreads), lds, st, b); break;
+    }
     return (int)hipGetLastError();
 }
 
 int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream) {
-    hipLaunchKernelGGL((row_batch_kernel<kThreads, kElemsPerThread>), dim3(grid), dim3(kThreads), lds,
-                       (hipStream_t)stream, b);
+    return b.wide ? launch_row_batch_e<kElemsWide>(b, grid, lds, stream)
+                  : launch_row_batch_e<kElemsPerThread>(b, grid, lds, stream);
+}
+
+template <int E>
+static int launch_col_batch_e(const ColBatch& b, int grid, size_t lds, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    switch (b.op) {
+        case COL_FWD: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FWD>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case COL_FUSED: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FUSED>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case COL_INV: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_INV>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        default: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FWD_OTF>), dim3(grid), dim3(kThreads), lds, st, b); break;
+    }
     return (int)hipGetLastError();
 }
 
 int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream) {
-    hipLaunchKernelGGL((col_batch_kernel<kThreads, kElemsPerThread>), dim3(grid), dim3(kThreads), lds,
-                       (hipStream_t)stream, b);
-    return (int)hipGetLastError();
+    return b.wide ? launch_col_batch_e<kElemsWide>(b, grid, lds, stream)
+                  : launch_col_batch_e<kElemsPerThread>(b, grid, lds, stream);
 }
 
 int launch_psf_batch(const PsfBatch& b, void* stream) {
@@ -333,19 +456,16 @@ int launch_zero_ints(int* p, int n, void* stream) {
     return (int)hipGetLastError();
 }
 
-int row_batch_occupancy(size_t lds) {
+int batch_occupancy(bool row, bool wide, size_t lds) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread>,
-                                                     kThreads, lds) != hipSuccess)
-        return 1;
-    return blocks > 0 ? blocks : 1;
-}
-
-int col_batch_occupancy(size_t lds) {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread>,
-                                                     kThreads, lds) != hipSuccess)
-        return 1;
+    hipError_t e;
+    if (row)
+        e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsWide, ROW_RATIO>, kThreads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>, kThreads, lds);
+    else
+        e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsWide, COL_FUSED>, kThreads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>, kThreads, lds);
+    if (e != hipSuccess) return 1;
     return blocks > 0 ? blocks : 1;
 }
 
