@@ -147,6 +147,44 @@ lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float
                                                uint32_t out_ndim, const lucem_rl_settings* settings,
                                                lucem_workspace* ws);
 
+/* ---- Batched form: many tiles of uneven size in one graph-captured launch sequence ----------
+ * Each tile is an independent RL problem (what a custom::transform closure receives,
+ * engine/lucem-pipeline/src/custom.rs:233-251; Block read geometry crates/lucem-core/src/tiling.rs:13-23)
+ * with its own PSF from a PSF table. All tiles have the PSF table's rank (2 or 3); a 2-D tile uses
+ * dims[0..2) = [h, w]. Tiles sharing an FFT shape and a PSF share transfer functions. Every pass
+ * of every iteration is one persistent kernel over the work items of all tiles (an atomic work
+ * queue, largest tiles first), and the whole sequence is captured once in a hipGraph that
+ * lucem_rl_batch_execute replays. Results equal lucem_richardson_lucy_nd_into_f32 per tile. */
+typedef struct lucem_rl_tile {
+    const float* input;
+    float* output;
+    uint64_t dims[3];
+    uint32_t ndim;
+    uint32_t psf_index;
+} lucem_rl_tile;
+
+typedef struct lucem_rl_batch lucem_rl_batch;
+
+/* Plans a batch for these tile shapes, PSF shapes (num_psfs x psf_ndim) and settings. The
+ * checks of lucem_richardson_lucy_nd_into_f32 apply to every tile (index order), then
+ * psf_index < num_psfs (LUCEM_INVALID_PARAMETER). Pointers in `tiles` are not used here. */
+lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_rl_tile* tiles, uint64_t num_tiles,
+                                   const uint64_t* psf_dims, uint32_t psf_ndim, uint64_t num_psfs,
+                                   const lucem_rl_settings* settings, lucem_rl_batch** out);
+/* Runs the planned batch on these buffers (same shapes and PSF indices as at creation; device
+ * pointers may change from call to call) and `psfs[j]` (device pointers). Enqueues and returns. */
+lucem_status lucem_rl_batch_execute(lucem_rl_batch* batch, const lucem_rl_tile* tiles, const float* const* psfs);
+lucem_status lucem_rl_batch_destroy(lucem_rl_batch* batch);
+/* Device memory the plan holds (workspace, transfer functions, descriptors). */
+uint64_t lucem_rl_batch_device_bytes(const lucem_rl_batch* batch);
+/* Number of kernel launches one execute replays (the captured graph's nodes). */
+uint64_t lucem_rl_batch_launches(const lucem_rl_batch* batch);
+/* One call: create, execute, destroy (the plan is released after the stream drains). */
+lucem_status lucem_richardson_lucy_batched_f32(lucem_client* client, const lucem_rl_tile* tiles, uint64_t num_tiles,
+                                               const float* const* psfs, const uint64_t* psf_dims,
+                                               uint32_t psf_ndim, uint64_t num_psfs,
+                                               const lucem_rl_settings* settings);
+
 /* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
  * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is
  * unnormalised. L must be 7-smooth and <= LUCEM_MAX_FFT_LEN. The engine the RL passes use; no

@@ -38,7 +38,8 @@ EXPORTED = [
     "lucem_client_num_cus", "lucem_workspace_create", "lucem_workspace_destroy",
     "lucem_workspace_capacity_bytes", "lucem_richardson_lucy_into_f32", "lucem_richardson_lucy_f32",
     "lucem_richardson_lucy_nd_into_f32", "lucem_fft_c2c_lines_f32", "lucem_client_profile",
-    "lucem_client_profile_read",
+    "lucem_client_profile_read", "lucem_rl_batch_create", "lucem_rl_batch_execute", "lucem_rl_batch_destroy",
+    "lucem_rl_batch_device_bytes", "lucem_rl_batch_launches", "lucem_richardson_lucy_batched_f32",
 ]
 
 

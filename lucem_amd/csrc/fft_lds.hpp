@@ -277,8 +277,7 @@ __device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, con
     // fully unrolled so that the radix codes are read with scalar loads at fixed offsets
 #pragma unroll
     for (int q = 0; q < kMaxPasses; q++) {
-        if (q >= p.npass) break;
-        const int code = p.rad[q];
+        const int code = q < p.npass ? (int)p.rad[q] : 0;
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                        \
     case radix_code(r1, r2):                                                      \

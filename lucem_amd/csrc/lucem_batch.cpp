@@ -1,0 +1,556 @@
+// Batched Richardson-Lucy: many tiles of uneven size (slide-edge tiles, volumes of different
+// depth), one plan, one hipGraph.
+//
+// The plan gives every tile its own FFT geometry and workspace and every (FFT shape, PSF) pair
+// its own transfer functions. Each pass of the RL iteration (restore/deconvolve.rs:109-147:
+// blur, ratio, adjoint, update) becomes one persistent kernel over the work items of all tiles:
+// workgroups take (tile, block) items from an atomic counter, largest tiles first. The PSF
+// preparation, the transfer-function builds and every iteration are captured once into a
+// hipGraph (the counters are cleared by a kernel at its start, not by a memset node); execute
+// uploads the caller's buffer pointers into the descriptors and replays the graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lucem_hip.h"
+#include "lucem_internal_host.hpp"
+#include "rl_internal.hpp"
+
+using namespace lucem;
+
+namespace {
+
+struct TilePlan {
+    Geometry g;
+    uint32_t psf;
+    int group;
+    int64_t n;
+    int row_blocks;
+    int col_blocks[3];
+};
+
+struct GroupPlan {
+    int L[3];
+    uint32_t psf;
+    int tile;  // a tile of this group (its geometry builds the transfer functions)
+};
+
+// One persistent launch of the captured sequence.
+struct Launch {
+    bool row;
+    int op;
+    int first, last, clip;
+    const void* descs;
+    const int2* items;
+    int nitems;
+    size_t lds;
+    int grid;
+};
+
+}  // namespace
+
+struct lucem_rl_batch {
+    lucem_client* client = nullptr;
+    lucem_rl_settings settings{};
+    uint32_t ndim = 0;
+    std::vector<TilePlan> tiles;
+    std::vector<GroupPlan> groups;
+    std::vector<std::vector<uint64_t>> psf_dims;
+    void* mem = nullptr;
+    size_t mem_bytes = 0;
+    RowPass* d_rows = nullptr;          // per tile
+    PsfPrep* d_psf = nullptr;           // per PSF
+    std::vector<RowPass> h_rows;
+    std::vector<PsfPrep> h_psf;
+    RowPass* h_stage = nullptr;         // pinned upload staging
+    PsfPrep* h_psf_stage = nullptr;
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t launches = 0;
+    bool persistent = true;  // work-queue grid; false: one workgroup per item
+    ~lucem_rl_batch();
+};
+
+lucem_rl_batch::~lucem_rl_batch() {
+    if (client) {
+        (void)hipSetDevice(client->device);
+        (void)hipStreamSynchronize(client->stream);
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture) (void)hipStreamDestroy(capture);
+    if (staged) (void)hipEventDestroy(staged);
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (h_psf_stage) (void)hipHostFree(h_psf_stage);
+    if (mem) (void)hipFree(mem);
+}
+
+namespace {
+
+// Carves aligned sub-buffers out of one allocation: first pass sizes, second pass assigns.
+struct Carver {
+    char* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * std::max<size_t>(count, 1);
+        return p;
+    }
+};
+
+size_t pass_lds(bool row, const void* desc) {
+    if (row) {
+        const RowPass* r = static_cast<const RowPass*>(desc);
+        return row_pass_lds_bytes(r->f.L, r->lgP);
+    }
+    const ColPass* c = static_cast<const ColPass*>(desc);
+    return col_pass_lds_bytes(c->f.L, c->lgB);
+}
+
+}  // namespace
+
+static lucem_status batch_validate(const lucem_rl_tile* tiles, uint64_t num_tiles, const uint64_t* psf_dims,
+                                   uint32_t psf_ndim, uint64_t num_psfs, const lucem_rl_settings* settings,
+                                   std::vector<Geometry>* geos, std::vector<bool>* empty) {
+    if (!settings) return fail(LUCEM_INVALID_HANDLE, "settings is NULL");
+    if (num_tiles && !tiles) return fail(LUCEM_INVALID_HANDLE, "tiles is NULL");
+    if (num_psfs && !psf_dims) return fail(LUCEM_INVALID_HANDLE, "psf_dims is NULL");
+    geos->resize(num_tiles);
+    empty->assign(num_tiles, false);
+    for (uint64_t t = 0; t < num_tiles; t++) {
+        const lucem_rl_tile& tl = tiles[t];
+        if (tl.psf_index >= num_psfs)
+            return fail(LUCEM_INVALID_PARAMETER, "invalid parameter `psf_index`: tile " + std::to_string(t) +
+                                                     " names PSF " + std::to_string(tl.psf_index) + " of " +
+                                                     std::to_string(num_psfs));
+        if (tl.ndim > 3) return fail(LUCEM_INVALID_DIMENSION, "tile " + std::to_string(t) + ": ndim > 3");
+        const uint64_t* pd = psf_dims + (size_t)tl.psf_index * psf_ndim;
+        bool e = false;
+        Geometry g;
+        // buffer pointers are checked at execute time
+        lucem_status st = validate_rl(nullptr, tl.dims, tl.ndim, nullptr, pd, psf_ndim, nullptr, tl.dims, tl.ndim,
+                                      settings, 3, &e, &g);
+        if (st != LUCEM_OK) return fail(st, "tile " + std::to_string(t) + ": " + lucem_last_error());
+        (*empty)[t] = e;
+        (*geos)[t] = g;
+    }
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_rl_tile* tiles, uint64_t num_tiles,
+                                              const uint64_t* psf_dims, uint32_t psf_ndim, uint64_t num_psfs,
+                                              const lucem_rl_settings* settings, lucem_rl_batch** out) {
+    if (!out) return fail(LUCEM_INVALID_HANDLE, "batch output pointer is NULL");
+    *out = nullptr;
+    std::vector<Geometry> geos;
+    std::vector<bool> empty;
+    lucem_status st = batch_validate(tiles, num_tiles, psf_dims, psf_ndim, num_psfs, settings, &geos, &empty);
+    if (st) return st;
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    hipError_t e = hipSetDevice(client->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+
+    auto b = std::make_unique<lucem_rl_batch>();
+    b->client = client;
+    if (const char* sched = std::getenv("LUCEM_BATCH_SCHEDULE")) b->persistent = std::string(sched) != "direct";
+    b->settings = *settings;
+    b->ndim = psf_ndim;
+    for (uint64_t j = 0; j < num_psfs; j++)
+        b->psf_dims.emplace_back(psf_dims + j * psf_ndim, psf_dims + (j + 1) * psf_ndim);
+
+    // tiles and (FFT shape, PSF) groups
+    std::map<std::tuple<int, int, int, uint32_t>, int> gid;
+    for (uint64_t t = 0; t < num_tiles; t++) {
+        TilePlan tp{};
+        tp.g = geos[t];
+        tp.psf = tiles[t].psf_index;
+        tp.n = empty[t] ? 0 : (int64_t)tp.g.n[0] * tp.g.n[1] * tp.g.n[2];
+        tp.group = -1;
+        if (!empty[t]) {
+            auto key = std::make_tuple(tp.g.L[0], tp.g.L[1], tp.g.L[2], tp.psf);
+            auto it = gid.find(key);
+            if (it == gid.end()) {
+                GroupPlan gp{};
+                std::memcpy(gp.L, tp.g.L, sizeof gp.L);
+                gp.psf = tp.psf;
+                gp.tile = (int)t;
+                it = gid.emplace(key, (int)b->groups.size()).first;
+                b->groups.push_back(gp);
+            }
+            tp.group = it->second;
+        }
+        b->tiles.push_back(tp);
+    }
+
+    // twiddle tables for every length used
+    std::map<int, const float2*> tw;
+    for (const auto& tp : b->tiles) {
+        if (tp.group < 0) continue;
+        for (int a = (tp.g.ndim == 2 ? 1 : 0); a < 3; a++)
+            if (!tw.count(tp.g.L[a])) {
+                const float2* p;
+                if ((st = twiddles_for(client, tp.g.L[a], &p))) return st;
+                tw[tp.g.L[a]] = p;
+            }
+    }
+
+    const size_t ntiles = b->tiles.size(), ngroups = b->groups.size(), npsf = num_psfs;
+    const int ncorr = psf_ndim == 2 ? 1 : 3;
+    const int notf = psf_ndim == 2 ? 1 : 2;
+
+    // sizes: one allocation for everything the plan owns
+    struct Ptrs {
+        std::vector<float*> est;
+        std::vector<float2*> sa, sb;
+        std::vector<float2*> otf1, otf2, pa1, pa2, pb1, pb2;
+        std::vector<float*> h, rev;
+        RowPass* rows;
+        PsfPrep* psf;
+        RowPass* otf_rows;
+        ColPass* otf_cols[2];
+        ColPass* corr[2][3];
+        int2* items_row;
+        int2* items_col[3];
+        int2* items_otf_row;
+        int2* items_otf_col[2];
+        int* counters;
+    } P;
+    std::vector<int2> it_row, it_col[3], it_otf_row, it_otf_col[2];
+    // work items, largest tiles first
+    std::vector<size_t> order(ntiles);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t bb) { return b->tiles[a].n > b->tiles[bb].n; });
+
+    // descriptors need the final pointers, so size first with a null carver
+    auto layout = [&](Carver& cv) {
+        P.est.assign(ntiles, nullptr);
+        P.sa.assign(ntiles, nullptr);
+        P.sb.assign(ntiles, nullptr);
+        for (size_t t = 0; t < ntiles; t++) {
+            const TilePlan& tp = b->tiles[t];
+            if (tp.group < 0) continue;
+            const Geometry& g = tp.g;
+            P.est[t] = cv.take<float>(tp.n);
+            P.sa[t] = cv.take<float2>((size_t)g.n[0] * g.n[1] * g.pitch);
+            if (g.ndim == 3) P.sb[t] = cv.take<float2>((size_t)g.n[0] * g.L[1] * g.pitch);
+        }
+        P.otf1.assign(ngroups, nullptr);
+        P.otf2 = P.pa1 = P.pa2 = P.pb1 = P.pb2 = P.otf1;
+        for (size_t gi = 0; gi < ngroups; gi++) {
+            const Geometry& g = b->tiles[b->groups[gi].tile].g;
+            const size_t otf = (size_t)g.L[0] * g.L[1] * g.pitch;
+            P.otf1[gi] = cv.take<float2>(otf);
+            P.otf2[gi] = cv.take<float2>(otf);
+            P.pa1[gi] = cv.take<float2>((size_t)g.k[0] * g.k[1] * g.pitch);
+            P.pa2[gi] = cv.take<float2>((size_t)g.k[0] * g.k[1] * g.pitch);
+            if (g.ndim == 3) {
+                P.pb1[gi] = cv.take<float2>((size_t)g.k[0] * g.L[1] * g.pitch);
+                P.pb2[gi] = cv.take<float2>((size_t)g.k[0] * g.L[1] * g.pitch);
+            }
+        }
+        P.h.assign(npsf, nullptr);
+        P.rev.assign(npsf, nullptr);
+        for (size_t j = 0; j < npsf; j++) {
+            size_t nk = 1;
+            for (auto d : b->psf_dims[j]) nk *= d;
+            P.h[j] = cv.take<float>(nk);
+            P.rev[j] = cv.take<float>(nk);
+        }
+        P.rows = cv.take<RowPass>(ntiles);
+        P.psf = cv.take<PsfPrep>(npsf);
+        P.otf_rows = cv.take<RowPass>(2 * ngroups);
+        for (int i = 0; i < notf; i++) P.otf_cols[i] = cv.take<ColPass>(2 * ngroups);
+        for (int k = 0; k < 2; k++)
+            for (int i = 0; i < ncorr; i++) P.corr[k][i] = cv.take<ColPass>(ntiles);
+        P.items_row = cv.take<int2>(it_row.size());
+        for (int i = 0; i < ncorr; i++) P.items_col[i] = cv.take<int2>(it_col[i].size());
+        P.items_otf_row = cv.take<int2>(it_otf_row.size());
+        for (int i = 0; i < notf; i++) P.items_otf_col[i] = cv.take<int2>(it_otf_col[i].size());
+        P.counters = cv.take<int>(4096);
+    };
+
+    // host descriptors (built with the final pointers after the allocation)
+    std::vector<RowPass> rows(ntiles), otf_rows(2 * ngroups);
+    std::vector<ColPass> otf_cols[2], corr[2][3];
+    for (int i = 0; i < 2; i++) otf_cols[i].resize(2 * ngroups);
+    for (int k = 0; k < 2; k++)
+        for (int i = 0; i < 3; i++) corr[k][i].resize(ntiles);
+    std::vector<PsfPrep> psfp(npsf);
+    const float eps = rl_eps(*settings);
+
+    auto build_descs = [&]() {
+        for (size_t t = 0; t < ntiles; t++) {
+            TilePlan& tp = b->tiles[t];
+            if (tp.group < 0) continue;
+            const Geometry& g = tp.g;
+            const float2* tws[3] = {g.ndim == 3 ? tw[g.L[0]] : nullptr, tw[g.L[1]], tw[g.L[2]]};
+            rows[t] = make_row_desc(g, 0, tws[2], eps, nullptr, P.est[t], nullptr, P.sa[t],
+                                    &tp.row_blocks);
+            ColPass d[3];
+            int blk[3];
+            make_corr_descs(g, 0, P.sa[t], P.sb[t], P.otf1[tp.group], tws, d, blk);
+            for (int i = 0; i < ncorr; i++) {
+                corr[0][i][t] = d[i];
+                tp.col_blocks[i] = blk[i];
+            }
+            make_corr_descs(g, 0, P.sa[t], P.sb[t], P.otf2[tp.group], tws, d, blk);
+            for (int i = 0; i < ncorr; i++) corr[1][i][t] = d[i];
+        }
+        for (size_t gi = 0; gi < ngroups; gi++) {
+            const GroupPlan& gp = b->groups[gi];
+            const Geometry& g = b->tiles[gp.tile].g;
+            const float2* tws[3] = {g.ndim == 3 ? tw[g.L[0]] : nullptr, tw[g.L[1]], tw[g.L[2]]};
+            for (int k = 0; k < 2; k++) {
+                RowPass r;
+                int rb;
+                ColPass cols[2];
+                int cb[2];
+                make_otf_descs(g, 0, k == 0 ? P.rev[gp.psf] : P.h[gp.psf], k == 0 ? P.pa1[gi] : P.pa2[gi],
+                               k == 0 ? P.pb1[gi] : P.pb2[gi], k == 0 ? P.otf1[gi] : P.otf2[gi], tws, &r, &rb, cols, cb);
+                otf_rows[2 * gi + k] = r;
+                for (int i = 0; i < notf; i++) otf_cols[i][2 * gi + k] = cols[i];
+            }
+        }
+        for (size_t j = 0; j < npsf; j++) {
+            int64_t nk = 1;
+            for (auto d : b->psf_dims[j]) nk *= (int64_t)d;
+            psfp[j] = PsfPrep{nk, nullptr, P.h[j], P.rev[j]};
+        }
+    };
+
+    // work items (sizes only depend on the geometry; build them with a sizing pass of the descs)
+    {
+        Carver sizing;
+        layout(sizing);
+        build_descs();
+        for (size_t oi : order) {
+            const TilePlan& tp = b->tiles[oi];
+            if (tp.group < 0) continue;
+            for (int k = 0; k < tp.row_blocks; k++) it_row.push_back(make_int2((int)oi, k));
+            for (int i = 0; i < ncorr; i++)
+                for (int k = 0; k < tp.col_blocks[i]; k++) it_col[i].push_back(make_int2((int)oi, k));
+        }
+        for (size_t gi = 0; gi < ngroups; gi++)
+            for (int k = 0; k < 2; k++) {
+                const RowPass& r = otf_rows[2 * gi + k];
+                const int rb = (((r.rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP;
+                for (int q = 0; q < rb; q++) it_otf_row.push_back(make_int2((int)(2 * gi + k), q));
+                for (int i = 0; i < notf; i++) {
+                    const ColPass& cp = otf_cols[i][2 * gi + k];
+                    const Geometry& g = b->tiles[b->groups[gi].tile].g;
+                    const int outers = i == 0 ? (notf == 1 ? 1 : g.k[0]) : g.L[1];
+                    for (int q = 0; q < outers * cp.nkb; q++) it_otf_col[i].push_back(make_int2((int)(2 * gi + k), q));
+                }
+            }
+    }
+    Carver sizing;
+    layout(sizing);
+    b->mem_bytes = sizing.off;
+    if ((e = hipMalloc(&b->mem, b->mem_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc(batch plan)");
+    Carver real;
+    real.base = static_cast<char*>(b->mem);
+    layout(real);
+    build_descs();
+    b->h_rows = rows;
+    b->h_psf = psfp;
+
+    // upload the plan-owned descriptors (the pointer fields of rows/psf are filled per execute)
+    hipStream_t s = client->stream;
+    auto up = [&](void* dst, const void* src, size_t bytes) -> lucem_status {
+        if (!bytes) return LUCEM_OK;
+        hipError_t err = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+        return err == hipSuccess ? LUCEM_OK : hip_fail(err, "hipMemcpy(batch descriptors)");
+    };
+    (void)s;
+    if ((st = up(P.otf_rows, otf_rows.data(), sizeof(RowPass) * otf_rows.size()))) return st;
+    for (int i = 0; i < notf; i++)
+        if ((st = up(P.otf_cols[i], otf_cols[i].data(), sizeof(ColPass) * otf_cols[i].size()))) return st;
+    for (int k = 0; k < 2; k++)
+        for (int i = 0; i < ncorr; i++)
+            if ((st = up(P.corr[k][i], corr[k][i].data(), sizeof(ColPass) * ntiles))) return st;
+    if ((st = up(P.items_row, it_row.data(), sizeof(int2) * it_row.size()))) return st;
+    for (int i = 0; i < ncorr; i++)
+        if ((st = up(P.items_col[i], it_col[i].data(), sizeof(int2) * it_col[i].size()))) return st;
+    if ((st = up(P.items_otf_row, it_otf_row.data(), sizeof(int2) * it_otf_row.size()))) return st;
+    for (int i = 0; i < notf; i++)
+        if ((st = up(P.items_otf_col[i], it_otf_col[i].data(), sizeof(int2) * it_otf_col[i].size()))) return st;
+    b->d_rows = P.rows;
+    b->d_psf = P.psf;
+
+    // the launch sequence
+    std::vector<Launch> seq;
+    auto add = [&](bool row, int op, const void* descs, const int2* items, int nitems, size_t lds, int f = 0,
+                   int l = 0, int cl = 0) {
+        if (nitems <= 0) return;
+        Launch L{};
+        L.row = row;
+        L.op = op;
+        L.first = f;
+        L.last = l;
+        L.clip = cl;
+        L.descs = descs;
+        L.items = items;
+        L.nitems = nitems;
+        L.lds = lds;
+        const int occ = batch_occupancy(row, false, lds);
+        L.grid = b->persistent ? std::max(1, std::min(nitems, occ * client->num_cus)) : nitems;
+        seq.push_back(L);
+    };
+    auto max_lds = [&](bool row, const auto& descs, const std::vector<int2>& items) {
+        size_t m = 0;
+        for (const auto& it : items) m = std::max(m, pass_lds(row, &descs[it.x]));
+        return m;
+    };
+    add(true, ROW_LOAD, P.otf_rows, P.items_otf_row, (int)it_otf_row.size(), max_lds(true, otf_rows, it_otf_row));
+    for (int i = 0; i < notf; i++)
+        add(false, i + 1 == notf ? COL_FWD_OTF : COL_FWD, P.otf_cols[i], P.items_otf_col[i],
+            (int)it_otf_col[i].size(), max_lds(false, otf_cols[i], it_otf_col[i]));
+    const size_t lds_row = max_lds(true, rows, it_row);
+    size_t lds_col[3];
+    for (int i = 0; i < ncorr; i++) lds_col[i] = max_lds(false, corr[0][i], it_col[i]);
+    add(true, ROW_LOAD_CONST, P.rows, P.items_row, (int)it_row.size(), lds_row);
+    const int corr_ops2[1] = {COL_FUSED};
+    const int corr_ops3[3] = {COL_FWD, COL_FUSED, COL_INV};
+    for (uint64_t it = 0; it < settings->num_iter; it++) {
+        const bool last = it + 1 == settings->num_iter;
+        for (int k = 0; k < 2; k++) {
+            for (int i = 0; i < ncorr; i++)
+                add(false, ncorr == 1 ? corr_ops2[i] : corr_ops3[i], P.corr[k][i], P.items_col[i],
+                    (int)it_col[i].size(), lds_col[i]);
+            if (k == 0)
+                add(true, ROW_RATIO, P.rows, P.items_row, (int)it_row.size(), lds_row);
+            else
+                add(true, ROW_UPDATE, P.rows, P.items_row, (int)it_row.size(), lds_row, it == 0, last,
+                    settings->clip ? 1 : 0);
+        }
+    }
+    if (seq.size() + 1 > 4096) return fail(LUCEM_INVALID_PARAMETER, "invalid parameter `num_iter`: too many launches");
+
+    // capture: counters cleared, PSF prep, transfer functions, iterations
+    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate(capture)");
+    if ((e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    int err = launch_zero_ints(P.counters, (int)seq.size(), b->capture);
+    if (!err && npsf) err = launch_psf_batch(PsfBatch{P.psf, (int)npsf}, b->capture);
+    for (size_t i = 0; i < seq.size() && !err; i++) {
+        const Launch& L = seq[i];
+        if (L.row) {
+            RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, L.first, L.last, L.clip, 0,
+                        b->persistent ? P.counters + i : nullptr};
+            err = launch_row_batch(rb, L.grid, L.lds, b->capture);
+        } else {
+            ColBatch cb{static_cast<const ColPass*>(L.descs), L.items, L.nitems, L.op, 0,
+                        b->persistent ? P.counters + i : nullptr};
+            err = launch_col_batch(cb, L.grid, L.lds, b->capture);
+        }
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(b->capture, &graph);
+    if (err) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(err, "batched launch capture");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = graph;
+    if ((e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+        return hip_fail(e, "hipGraphInstantiate");
+    b->launches = seq.size() + (npsf ? 2 : 1);
+
+    if ((e = hipHostMalloc((void**)&b->h_stage, sizeof(RowPass) * std::max<size_t>(ntiles, 1), hipHostMallocDefault)) !=
+        hipSuccess)
+        return hip_fail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void**)&b->h_psf_stage, sizeof(PsfPrep) * std::max<size_t>(npsf, 1), hipHostMallocDefault)) !=
+        hipSuccess)
+        return hip_fail(e, "hipHostMalloc");
+    if ((e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    *out = b.release();
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const float* const* psfs) {
+    if (!b) return fail(LUCEM_INVALID_HANDLE, "batch is NULL");
+    if (b->tiles.size() && !tiles) return fail(LUCEM_INVALID_HANDLE, "tiles is NULL");
+    if (b->psf_dims.size() && !psfs) return fail(LUCEM_INVALID_HANDLE, "psfs is NULL");
+    lucem_client* c = b->client;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // the previous upload must have left the staging buffers before they are rewritten
+    if (b->staged_pending) {
+        if ((e = hipEventSynchronize(b->staged)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        b->staged_pending = false;
+    }
+    for (size_t t = 0; t < b->tiles.size(); t++) {
+        const lucem_rl_tile& tl = tiles[t];
+        const TilePlan& tp = b->tiles[t];
+        int64_t n3[3] = {1, 1, 1};
+        if (tl.ndim > 3) return fail(LUCEM_SHAPE_MISMATCH, "tile " + std::to_string(t) + " differs from the plan");
+        for (uint32_t i = 0; i < tl.ndim; i++) n3[3 - tl.ndim + i] = (int64_t)tl.dims[i];
+        const bool same = tp.group < 0 ? (n3[0] * n3[1] * n3[2] == 0)
+                                       : (n3[0] == tp.g.n[0] && n3[1] == tp.g.n[1] && n3[2] == tp.g.n[2] &&
+                                          tl.psf_index == tp.psf);
+        if (!same) return fail(LUCEM_SHAPE_MISMATCH, "tile " + std::to_string(t) + " differs from the plan");
+        if (tp.group < 0) continue;
+        const char* a = (const char*)tl.input;
+        const char* o = (const char*)tl.output;
+        const uint64_t bytes = (uint64_t)tp.n * sizeof(float);
+        if (a < o + bytes && o < a + bytes)
+            return fail(LUCEM_ALIASED_BUFFERS, "tile " + std::to_string(t) +
+                                                   ": input and output arrays must not share the same device buffer");
+        if (!tl.input || !tl.output) return fail(LUCEM_INVALID_HANDLE, "tile pointer is NULL");
+        b->h_rows[t].obs = tl.input;
+        b->h_rows[t].out = tl.output;
+    }
+    for (size_t j = 0; j < b->h_psf.size(); j++) {
+        if (!psfs[j]) return fail(LUCEM_INVALID_HANDLE, "psf pointer is NULL");
+        b->h_psf[j].psf = psfs[j];
+    }
+    std::memcpy(b->h_stage, b->h_rows.data(), sizeof(RowPass) * b->h_rows.size());
+    std::memcpy(b->h_psf_stage, b->h_psf.data(), sizeof(PsfPrep) * b->h_psf.size());
+    if (!b->h_rows.empty() &&
+        (e = hipMemcpyAsync(b->d_rows, b->h_stage, sizeof(RowPass) * b->h_rows.size(), hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(descriptors)");
+    if (!b->h_psf.empty() &&
+        (e = hipMemcpyAsync(b->d_psf, b->h_psf_stage, sizeof(PsfPrep) * b->h_psf.size(), hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(descriptors)");
+    if ((e = hipEventRecord(b->staged, c->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    b->staged_pending = true;
+    if ((e = hipGraphLaunch(b->exec, c->stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_rl_batch_destroy(lucem_rl_batch* b) {
+    delete b;
+    return LUCEM_OK;
+}
+
+extern "C" uint64_t lucem_rl_batch_device_bytes(const lucem_rl_batch* b) { return b ? b->mem_bytes : 0; }
+extern "C" uint64_t lucem_rl_batch_launches(const lucem_rl_batch* b) { return b ? b->launches : 0; }
+
+extern "C" lucem_status lucem_richardson_lucy_batched_f32(lucem_client* client, const lucem_rl_tile* tiles,
+                                                          uint64_t num_tiles, const float* const* psfs,
+                                                          const uint64_t* psf_dims, uint32_t psf_ndim,
+                                                          uint64_t num_psfs, const lucem_rl_settings* settings) {
+    lucem_rl_batch* b = nullptr;
+    lucem_status st = lucem_rl_batch_create(client, tiles, num_tiles, psf_dims, psf_ndim, num_psfs, settings, &b);
+    if (st) return st;
+    st = lucem_rl_batch_execute(b, tiles, psfs);
+    lucem_rl_batch_destroy(b);  // waits for the stream before releasing the plan's memory
+    return st;
+}

@@ -376,7 +376,7 @@ FftPlan lucem::make_plan(int L) {
     return p;
 }
 
-// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3,col_wide=1") overrides the choices, for
+// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3") overrides the choices, for
 // measurement sweeps; the autotuner records its choices per shape in the same terms.
 static int tune_value(const char* key) {
     const char* env = std::getenv("LUCEM_TUNE");
@@ -395,23 +395,21 @@ int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
     const int pairs = (rows + 1) / 2;
     int lgP = 3;
     while (lgP > 0 && ((size_t)L << lgP) > (size_t)kLdsComplex) lgP--;
-    while (lgP > 0 && (pairs >> lgP) < 4 * num_cus) lgP--;
+    // one array alone must still fill the device; a batch (num_cus == 0) brings its own parallelism
+    while (num_cus > 0 && lgP > 0 && (pairs >> lgP) < 4 * num_cus) lgP--;
     const int t = tune_value("row_lgP");
     if (t >= 0 && ((size_t)L << t) <= (size_t)kLdsComplex) lgP = t;
     return lgP;
 }
 
 int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide) {
-    int w = tune_value("col_wide");
-    if (w < 0) w = 0;  // measured: the 32 KiB shape wins even at B = 4 (more workgroups per CU)
-    const size_t cap = w ? kLdsComplexWide : kLdsComplex;
+    const int w = 0;
+    const size_t cap = kLdsComplex;
     int lgB = 4;
     while (lgB > 0 && ((size_t)L << lgB) > cap) lgB--;
-    while (lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 4 * num_cus) lgB--;
+    while (num_cus > 0 && lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 4 * num_cus) lgB--;
     const int t = tune_value("col_lgB");
     if (t >= 0 && ((size_t)L << t) <= cap) lgB = t;
-    // narrow lines that fit the small shape do not need the wide one
-    if (w && ((size_t)L << lgB) <= (size_t)kLdsComplex && tune_value("col_wide") < 0) w = 0;
     *wide = w;
     return lgB;
 }
@@ -440,10 +438,12 @@ static lucem_status launch_check(int err, const char* what) {
     return LUCEM_OK;
 }
 
-// Forward-transform a real array (dims m) into the full spectrum `dst` ([L_z][L_y][pitch]) with
-// Constant padding and zero left offset (zero-padding of a kernel), storing conj(v)*scale.
-static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3], const float* w, float2* tmp_a,
-                              float2* tmp_b, float2* otf, const float2* const tws[3]) {
+// Descriptors of the two-step transfer-function build: a real kernel (dims m, at the origin,
+// zero-padded to L) -> row R2C -> line FFTs -> conj(v) * scale stored as [L_z][L_y][pitch].
+int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2* pa, float2* pb, float2* otf,
+                          const float2* const tws[3], RowPass* row, int* row_blocks, ColPass cols[2],
+                          int col_blocks[2]) {
+    const int m[3] = {g.k[0], g.k[1], g.k[2]};
     const float scale = (float)(0.5 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]));
     RowPass r{};
     r.op = ROW_LOAD;
@@ -453,17 +453,15 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     r.mode = LUCEM_PAD_CONSTANT;
     r.Lc = g.Lc;
     r.pitch = g.pitch;
-    r.lgP = choose_lgP(g.L[2], m[2], r.rows, c->num_cus);
+    r.lgP = choose_lgP(g.L[2], m[2], r.rows, num_cus);
     r.f = make_plan(g.L[2]);
     r.src = w;
-    r.spec = tmp_a;
+    r.spec = pa;
     r.tw = tws[2];
-    const int pairs = (r.rows + 1) / 2;
-    lucem_status st = run_row(c, r, (pairs + (1 << r.lgP) - 1) >> r.lgP, "otf_row_load");
-    if (st) return st;
+    *row = r;
+    *row_blocks = (((r.rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP;
 
     ColPass y{};
-    y.op = g.ndim == 2 ? COL_FWD_OTF : COL_FWD;
     y.n_src = m[1];
     y.left = 0;
     y.mode = LUCEM_PAD_CONSTANT;
@@ -471,25 +469,29 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     y.Lc = g.Lc;
     y.f = make_plan(g.L[1]);
     y.tw = tws[1];
-    y.src = tmp_a;
+    y.src = pa;
     y.src_outer = (int64_t)m[1] * g.pitch;
     y.src_axis = g.pitch;
     if (g.ndim == 2) {
+        y.op = COL_FWD_OTF;
         y.scale = scale;
         y.dst = otf;
         y.dst_outer = 0;
         y.dst_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus, &y.wide);
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, num_cus, &y.wide);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-        return run_col(c, y, y.nkb, "otf_line_fwd_y");
+        cols[0] = y;
+        col_blocks[0] = y.nkb;
+        return 1;
     }
-    y.dst = tmp_b;
+    y.op = COL_FWD;
+    y.dst = pb;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], c->num_cus, &y.wide);
+    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], num_cus, &y.wide);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-    st = run_col(c, y, m[0] * y.nkb, "otf_line_fwd_y");
-    if (st) return st;
+    cols[0] = y;
+    col_blocks[0] = m[0] * y.nkb;
 
     ColPass z{};
     z.op = COL_FWD_OTF;
@@ -500,21 +502,24 @@ static lucem_status build_otf(lucem_client* c, const Geometry& g, const int m[3]
     z.Lc = g.Lc;
     z.f = make_plan(g.L[0]);
     z.tw = tws[0];
-    z.src = tmp_b;
+    z.src = pb;
     z.src_outer = g.pitch;
     z.src_axis = (int64_t)g.L[1] * g.pitch;
     z.dst = otf;
     z.dst_outer = g.pitch;
     z.dst_axis = (int64_t)g.L[1] * g.pitch;
     z.scale = scale;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus, &z.wide);
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], num_cus, &z.wide);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
-    return run_col(c, z, g.L[1] * z.nkb, "otf_line_fwd_z");
+    cols[1] = z;
+    col_blocks[1] = g.L[1] * z.nkb;
+    return 2;
 }
 
-// Runs the line passes of one correlation over spec_a (and spec_b in 3-D) with transfer `otf`.
-static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* sa, float2* sb, const float2* otf,
-                                    const float2* const tws[3]) {
+// Line-pass descriptors of one correlation over spec_a (and spec_b in 3-D) with transfer `otf`:
+// 2-D: y fused (in place); 3-D: y forward (a -> b), z fused (in place on b), y inverse (b -> a).
+int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* sb, const float2* otf,
+                           const float2* const tws[3], ColPass out[3], int blocks[3]) {
     ColPass y{};
     y.Lc = g.Lc;
     y.f = make_plan(g.L[1]);
@@ -532,11 +537,12 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
         y.otf = otf;
         y.otf_outer = 0;
         y.otf_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, c->num_cus, &y.wide);
+        y.lgB = choose_lgB(g.L[1], g.Lc, 1, num_cus, &y.wide);
         y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-        return run_col(c, y, y.nkb, "line_fused_y");
+        out[0] = y;
+        blocks[0] = y.nkb;
+        return 1;
     }
-    // y forward: spec_a [nz][ny] -> spec_b [nz][Ly]
     y.op = COL_FWD;
     y.n_src = g.n[1];
     y.n_dst = g.L[1];
@@ -546,11 +552,11 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     y.dst = sb;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], c->num_cus, &y.wide);
+    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], num_cus, &y.wide);
     y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
-    lucem_status st = run_col(c, y, g.n[0] * y.nkb, "line_fwd_y");
-    if (st) return st;
-    // z fused, in place on spec_b
+    out[0] = y;
+    blocks[0] = g.n[0] * y.nkb;
+
     ColPass z{};
     z.op = COL_FUSED;
     z.Lc = g.Lc;
@@ -567,11 +573,11 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     z.otf = otf;
     z.otf_outer = g.pitch;
     z.otf_axis = (int64_t)g.L[1] * g.pitch;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], c->num_cus, &z.wide);
+    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], num_cus, &z.wide);
     z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
-    st = run_col(c, z, g.L[1] * z.nkb, "line_fused_z");
-    if (st) return st;
-    // y inverse: spec_b [nz][Ly] -> spec_a [nz][ny]
+    out[1] = z;
+    blocks[1] = g.L[1] * z.nkb;
+
     ColPass yi = y;
     yi.op = COL_INV;
     yi.n_src = g.L[1];
@@ -580,7 +586,65 @@ static lucem_status correlate_lines(lucem_client* c, const Geometry& g, float2* 
     yi.src_outer = (int64_t)g.L[1] * g.pitch;
     yi.dst = sa;
     yi.dst_outer = (int64_t)g.n[1] * g.pitch;
-    return run_col(c, yi, g.n[0] * yi.nkb, "line_inv_y");
+    out[2] = yi;
+    blocks[2] = g.n[0] * yi.nkb;
+    return 3;
+}
+
+RowPass lucem::make_row_desc(const Geometry& g, int num_cus, const float2* tw_x, float eps, const float* obs,
+                             float* est, float* out, float2* spec, int* blocks) {
+    const int64_t rows = (int64_t)g.n[0] * g.n[1];
+    RowPass r{};
+    r.nx = g.n[2];
+    r.rows = (int)rows;
+    r.left = g.k[2] / 2;
+    r.mode = g.mode;
+    r.Lc = g.Lc;
+    r.pitch = g.pitch;
+    r.lgP = choose_lgP(g.L[2], g.n[2], (int)rows, num_cus);
+    r.eps = eps;
+    r.init = 0.5f;
+    r.f = make_plan(g.L[2]);
+    r.obs = obs;
+    r.est = est;
+    r.out = out;
+    r.spec = spec;
+    r.tw = tw_x;
+    *blocks = (int)((((rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP);
+    return r;
+}
+
+static const char* const kCorrNames2[1] = {"line_fused_y"};
+static const char* const kCorrNames3[3] = {"line_fwd_y", "line_fused_z", "line_inv_y"};
+
+static lucem_status run_corr(lucem_client* c, const Geometry& g, float2* sa, float2* sb, const float2* otf,
+                             const float2* const tws[3]) {
+    ColPass d[3];
+    int b[3];
+    const int n = make_corr_descs(g, c->num_cus, sa, sb, otf, tws, d, b);
+    for (int i = 0; i < n; i++) {
+        lucem_status st = run_col(c, d[i], b[i], g.ndim == 2 ? kCorrNames2[i] : kCorrNames3[i]);
+        if (st) return st;
+    }
+    return LUCEM_OK;
+}
+
+static lucem_status run_otf(lucem_client* c, const Geometry& g, const float* w, float2* pa, float2* pb, float2* otf,
+                            const float2* const tws[3]) {
+    RowPass r;
+    int rb;
+    ColPass cols[2];
+    int cb[2];
+    const int n = make_otf_descs(g, c->num_cus, w, pa, pb, otf, tws, &r, &rb, cols, cb);
+    lucem_status st = run_row(c, r, rb, "otf_row_load");
+    if (st) return st;
+    for (int i = 0; i < n; i++)
+        if ((st = run_col(c, cols[i], cb[i], i + 1 == n ? "otf_line_fwd_last" : "otf_line_fwd_y"))) return st;
+    return LUCEM_OK;
+}
+
+float lucem::rl_eps(const lucem_rl_settings& s) {
+    return (float)std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
 }
 
 lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
@@ -616,38 +680,21 @@ lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* in
     // otf_conv for correlate(est, rev) (:120), otf_mirror for correlate(rel, h) (:133).
     PsfPrep pp{nk, psf, h, rev};
     if ((st = launch_check(launch_psf_prep(pp, c->stream), "psf prep"))) return st;
-    const int m[3] = {g.k[0], g.k[1], g.k[2]};
-    if ((st = build_otf(c, g, m, rev, pa, pb, otf1, tws))) return st;
-    if ((st = build_otf(c, g, m, h, pa, pb, otf2, tws))) return st;
+    if ((st = run_otf(c, g, rev, pa, pb, otf1, tws))) return st;
+    if ((st = run_otf(c, g, h, pa, pb, otf2, tws))) return st;
 
-    const double e64 = std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
-    RowPass r{};
-    r.nx = g.n[2];
-    r.rows = (int)rows;
-    r.left = g.k[2] / 2;
-    r.mode = g.mode;
-    r.Lc = g.Lc;
-    r.pitch = g.pitch;
-    r.lgP = choose_lgP(g.L[2], g.n[2], (int)rows, c->num_cus);
-    r.eps = (float)e64;
-    r.init = 0.5f;
-    r.f = make_plan(g.L[2]);
-    r.obs = input;
-    r.est = est;
-    r.out = output;
-    r.spec = sa;
-    r.tw = tws[2];
-    const int row_blocks = (int)((((rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP);
+    int row_blocks = 0;
+    RowPass r = make_row_desc(g, c->num_cus, tws[2], rl_eps(s), input, est, output, sa, &row_blocks);
 
     // est_0 = 0.5 everywhere (deconvolve.rs:93): transform the constant directly.
     r.op = ROW_LOAD_CONST;
     if ((st = run_row(c, r, row_blocks, "row_start"))) return st;
     for (uint64_t it = 0; it < s.num_iter; it++) {
         const bool last = it + 1 == s.num_iter;
-        if ((st = correlate_lines(c, g, sa, sb, otf1, tws))) return st;
+        if ((st = run_corr(c, g, sa, sb, otf1, tws))) return st;
         r.op = ROW_RATIO;
         if ((st = run_row(c, r, row_blocks, "row_ratio"))) return st;
-        if ((st = correlate_lines(c, g, sa, sb, otf2, tws))) return st;
+        if ((st = run_corr(c, g, sa, sb, otf2, tws))) return st;
         r.op = ROW_UPDATE;
         r.first = it == 0;
         r.last = last;
@@ -680,7 +727,7 @@ lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uin
         if (in_dims[i] && n > UINT64_MAX / in_dims[i]) overflow = true;
         n *= in_dims[i];
     }
-    if (n > 0 && !overflow) {
+    if (n > 0 && !overflow && input && output) {
         const char* a = (const char*)input;
         const char* b = (const char*)output;
         const uint64_t bytes = n * sizeof(float);

@@ -80,6 +80,15 @@ lucem_status run_col(lucem_client* c, const ColPass& p, int blocks, const char* 
 double row_pass_bytes(const RowPass& r);
 double col_pass_bytes(const ColPass& p, int blocks);
 
+// Descriptor builders shared by the plain and batched drivers.
+int make_otf_descs(const Geometry& g, int num_cus, const float* w, float2* pa, float2* pb, float2* otf,
+                   const float2* const tws[3], RowPass* row, int* row_blocks, ColPass cols[2], int col_blocks[2]);
+int make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* sb, const float2* otf,
+                    const float2* const tws[3], ColPass out[3], int blocks[3]);
+RowPass make_row_desc(const Geometry& g, int num_cus, const float2* tw_x, float eps, const float* obs, float* est,
+                      float* out, float2* spec, int* blocks);
+float rl_eps(const lucem_rl_settings& s);
+
 lucem_status validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
                          const uint64_t* psf_dims, uint32_t psf_ndim, const float* output, const uint64_t* out_dims,
                          uint32_t out_ndim, const lucem_rl_settings* s, uint32_t max_ndim, bool* empty, Geometry* g);

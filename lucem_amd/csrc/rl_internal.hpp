@@ -16,9 +16,8 @@ namespace lucem {
 constexpr int kMaxPasses = 16;
 constexpr int kThreads = 256;         // threads per workgroup of every FFT kernel
 constexpr int kElemsPerThread = 16;   // complex values per thread: 4096 per workgroup, 32 KiB LDS
-constexpr int kElemsWide = 32;        // long lines: 8192 complex per workgroup, 64 KiB LDS
 constexpr int kLdsComplex = kThreads * kElemsPerThread;
-constexpr int kLdsComplexWide = kThreads * kElemsWide;
+
 
 // One FFT length and its Stockham radix sequence.
 struct FftPlan {
@@ -49,7 +48,7 @@ struct RowPass {
     int left, mode;      // pad gather along x
     int Lc, pitch;       // half-spectrum bins per row, row pitch (complex) of `spec`
     int lgP;             // log2(row pairs per workgroup)
-    int wide;            // 1: the 8192-value kernel shape
+    int wide;            // reserved (0)
     int first;           // ROW_UPDATE: the previous estimate is the constant `init`
     int last;            // ROW_UPDATE: write `out`, no forward transform
     int clip;            // ROW_UPDATE && last: clamp to [-1, 1]
@@ -73,7 +72,7 @@ struct ColPass {
     int Lc;               // valid columns
     int lgB;              // log2(columns per workgroup)
     int nkb;              // column blocks per outer index
-    int wide;             // 1: the 8192-value kernel shape
+    int wide;             // reserved (0)
     float scale;          // COL_FWD_OTF
     int64_t src_outer, src_axis, dst_outer, dst_axis, otf_outer, otf_axis;
     FftPlan f;

@@ -202,7 +202,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) void row_pass_kernel(RowPass d) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row_pass_kernel(RowPass d) {
     extern __shared__ float2 lds[];
     row_pass_body<T, E, OP>(d, RowFlags{d.first, d.last, d.clip}, blockIdx.x, lds);
 }
@@ -278,7 +278,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) void col_pass_kernel(ColPass d) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void col_pass_kernel(ColPass d) {
     extern __shared__ float2 lds[];
     col_pass_body<T, E, OP>(d, xcd_block(blockIdx.x, gridDim.x), lds);
 }
@@ -318,28 +318,31 @@ __device__ __forceinline__ int next_item(int* counter, int* slot) {
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) void row_batch_kernel(RowBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row_batch_kernel(RowBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
-    for (;;) {
-        const int it = next_item(bt.counter, &slot);
+    for (int round = 0;; round++) {
+        // persistent: items from the atomic queue; direct (counter == nullptr): one item per block
+        const int it = bt.counter ? next_item(bt.counter, &slot)
+                                  : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
-        const RowPass& d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        const RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];  // a private copy: no aliasing
         row_pass_body<T, E, OP>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) void col_batch_kernel(ColBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void col_batch_kernel(ColBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
-    for (;;) {
-        const int it = next_item(bt.counter, &slot);
+    for (int round = 0;; round++) {
+        const int it = bt.counter ? next_item(bt.counter, &slot)
+                                  : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
-        const ColPass& d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+        const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
         col_pass_body<T, E, OP>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
@@ -371,8 +374,9 @@ __global__ void zero_ints_kernel(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Launchers. Kernels come in two shapes: 256 threads x 16 values (32 KiB of LDS at most) and
-// 256 threads x 32 values (64 KiB) for long lines; `wide` picks the second.
+// Launchers. Every FFT kernel is 256 threads x 16 complex values (at most 32 KiB of LDS, at most
+// 128 VGPRs): four workgroups per CU. A 64 KiB shape (32 values per thread) measured slower at
+// every length used here, so only this shape is built.
 // ---------------------------------------------------------------------------------------------
 size_t row_pass_lds_bytes(int L, int lgP) { return ((size_t)L << lgP) * sizeof(float2); }
 
@@ -391,9 +395,7 @@ static int launch_row_e(const RowPass& p, int blocks, void* stream) {
     return (int)hipGetLastError();
 }
 
-int launch_row_pass(const RowPass& p, int blocks, void* stream) {
-    return p.wide ? launch_row_e<kElemsWide>(p, blocks, stream) : launch_row_e<kElemsPerThread>(p, blocks, stream);
-}
+int launch_row_pass(const RowPass& p, int blocks, void* stream) { return launch_row_e<kElemsPerThread>(p, blocks, stream); }
 
 template <int E>
 static int launch_col_e(const ColPass& p, int blocks, void* stream) {
@@ -408,9 +410,7 @@ static int launch_col_e(const ColPass& p, int blocks, void* stream) {
     return (int)hipGetLastError();
 }
 
-int launch_col_pass(const ColPass& p, int blocks, void* stream) {
-    return p.wide ? launch_col_e<kElemsWide>(p, blocks, stream) : launch_col_e<kElemsPerThread>(p, blocks, stream);
-}
+int launch_col_pass(const ColPass& p, int blocks, void* stream) { return launch_col_e<kElemsPerThread>(p, blocks, stream); }
 
 template <int E>
 static int launch_row_batch_e(const RowBatch& b, int grid, size_t lds, void* stream) {
@@ -425,8 +425,7 @@ static int launch_row_batch_e(const RowBatch& b, int grid, size_t lds, void* str
 }
 
 int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream) {
-    return b.wide ? launch_row_batch_e<kElemsWide>(b, grid, lds, stream)
-                  : launch_row_batch_e<kElemsPerThread>(b, grid, lds, stream);
+    return launch_row_batch_e<kElemsPerThread>(b, grid, lds, stream);
 }
 
 template <int E>
@@ -442,8 +441,7 @@ static int launch_col_batch_e(const ColBatch& b, int grid, size_t lds, void* str
 }
 
 int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream) {
-    return b.wide ? launch_col_batch_e<kElemsWide>(b, grid, lds, stream)
-                  : launch_col_batch_e<kElemsPerThread>(b, grid, lds, stream);
+    return launch_col_batch_e<kElemsPerThread>(b, grid, lds, stream);
 }
 
 int launch_psf_batch(const PsfBatch& b, void* stream) {
@@ -457,14 +455,15 @@ int launch_zero_ints(int* p, int n, void* stream) {
 }
 
 int batch_occupancy(bool row, bool wide, size_t lds) {
+    (void)wide;
     int blocks = 0;
     hipError_t e;
     if (row)
-        e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsWide, ROW_RATIO>, kThreads, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>, kThreads, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>,
+                                                         kThreads, lds);
     else
-        e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsWide, COL_FUSED>, kThreads, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>, kThreads, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>,
+                                                         kThreads, lds);
     if (e != hipSuccess) return 1;
     return blocks > 0 ? blocks : 1;
 }
