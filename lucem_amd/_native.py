@@ -9,11 +9,12 @@ library or device raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "liblucem_hip.so"
+LIB_PATH = Path(os.environ.get("LUCEM_HIP_LIB", str(_PKG / "liblucem_hip.so")))
 
 LUCEM_OK = 0
 STATUS_NAMES = {

@@ -274,10 +274,9 @@ __device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, con
     const float2* tw = twbase + p.tw;
     const int N = p.L;
     int Ns = 1;
-    // fully unrolled so that the radix codes are read with scalar loads at fixed offsets
-#pragma unroll
-    for (int q = 0; q < kMaxPasses; q++) {
-        const int code = q < p.npass ? (int)p.rad[q] : 0;
+    for (int q = 0; q < p.npass; q++) {
+        const unsigned long long w = q < 8 ? p.codes_lo : p.codes_hi;
+        const int code = (int)((w >> (8 * (q & 7))) & 0xffull);
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                        \
     case radix_code(r1, r2):                                                      \

@@ -52,6 +52,7 @@ struct Launch {
     const int2* items;
     int nitems;
     size_t lds;
+    int wide;
     int grid;
 };
 
@@ -392,7 +393,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
 
     // the launch sequence
     std::vector<Launch> seq;
-    auto add = [&](bool row, int op, const void* descs, const int2* items, int nitems, size_t lds, int f = 0,
+    auto add = [&](bool row, int op, const void* descs, const int2* items, int nitems, size_t lds, int wide, int f = 0,
                    int l = 0, int cl = 0) {
         if (nitems <= 0) return;
         Launch L{};
@@ -405,7 +406,8 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         L.items = items;
         L.nitems = nitems;
         L.lds = lds;
-        const int occ = batch_occupancy(row, false, lds);
+        L.wide = wide;
+        const int occ = batch_occupancy(row, wide, lds);
         L.grid = b->persistent ? std::max(1, std::min(nitems, occ * client->num_cus)) : nitems;
         seq.push_back(L);
     };
@@ -414,14 +416,25 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         for (const auto& it : items) m = std::max(m, pass_lds(row, &descs[it.x]));
         return m;
     };
-    add(true, ROW_LOAD, P.otf_rows, P.items_otf_row, (int)it_otf_row.size(), max_lds(true, otf_rows, it_otf_row));
+    // one line-pass launch runs one kernel shape: the 512-thread one if any tile asks for it
+    // (a tile planned for 256 threads runs correctly on 512: its items use fewer columns)
+    auto any_wide = [&](const std::vector<ColPass>& descs, const std::vector<int2>& items) {
+        int w = 0;
+        for (const auto& it : items) w |= descs[it.x].wide;
+        return w;
+    };
+    add(true, ROW_LOAD, P.otf_rows, P.items_otf_row, (int)it_otf_row.size(), max_lds(true, otf_rows, it_otf_row), 0);
     for (int i = 0; i < notf; i++)
         add(false, i + 1 == notf ? COL_FWD_OTF : COL_FWD, P.otf_cols[i], P.items_otf_col[i],
-            (int)it_otf_col[i].size(), max_lds(false, otf_cols[i], it_otf_col[i]));
+            (int)it_otf_col[i].size(), max_lds(false, otf_cols[i], it_otf_col[i]), any_wide(otf_cols[i], it_otf_col[i]));
     const size_t lds_row = max_lds(true, rows, it_row);
     size_t lds_col[3];
-    for (int i = 0; i < ncorr; i++) lds_col[i] = max_lds(false, corr[0][i], it_col[i]);
-    add(true, ROW_LOAD_CONST, P.rows, P.items_row, (int)it_row.size(), lds_row);
+    int wide_col[3];
+    for (int i = 0; i < ncorr; i++) {
+        lds_col[i] = max_lds(false, corr[0][i], it_col[i]);
+        wide_col[i] = any_wide(corr[0][i], it_col[i]);
+    }
+    add(true, ROW_LOAD_CONST, P.rows, P.items_row, (int)it_row.size(), lds_row, 0);
     const int corr_ops2[1] = {COL_FUSED};
     const int corr_ops3[3] = {COL_FWD, COL_FUSED, COL_INV};
     for (uint64_t it = 0; it < settings->num_iter; it++) {
@@ -429,11 +442,11 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         for (int k = 0; k < 2; k++) {
             for (int i = 0; i < ncorr; i++)
                 add(false, ncorr == 1 ? corr_ops2[i] : corr_ops3[i], P.corr[k][i], P.items_col[i],
-                    (int)it_col[i].size(), lds_col[i]);
+                    (int)it_col[i].size(), lds_col[i], wide_col[i]);
             if (k == 0)
-                add(true, ROW_RATIO, P.rows, P.items_row, (int)it_row.size(), lds_row);
+                add(true, ROW_RATIO, P.rows, P.items_row, (int)it_row.size(), lds_row, 0);
             else
-                add(true, ROW_UPDATE, P.rows, P.items_row, (int)it_row.size(), lds_row, it == 0, last,
+                add(true, ROW_UPDATE, P.rows, P.items_row, (int)it_row.size(), lds_row, 0, it == 0, last,
                     settings->clip ? 1 : 0);
         }
     }
@@ -453,7 +466,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
                         b->persistent ? P.counters + i : nullptr};
             err = launch_row_batch(rb, L.grid, L.lds, b->capture);
         } else {
-            ColBatch cb{static_cast<const ColPass*>(L.descs), L.items, L.nitems, L.op, 0,
+            ColBatch cb{static_cast<const ColPass*>(L.descs), L.items, L.nitems, L.op, L.wide,
                         b->persistent ? P.counters + i : nullptr};
             err = launch_col_batch(cb, L.grid, L.lds, b->capture);
         }

@@ -366,7 +366,12 @@ FftPlan lucem::make_plan(int L) {
                 cnt[3] -= c.n3;
                 cnt[5] -= c.n5;
                 cnt[7] -= c.n7;
-                p.rad[p.npass++] = (unsigned char)(c.r1 | (c.r2 << 4));
+                const unsigned long long code = (unsigned long long)(c.r1 | (c.r2 << 4));
+                if (p.npass < 8)
+                    p.codes_lo |= code << (8 * p.npass);
+                else
+                    p.codes_hi |= code << (8 * (p.npass - 8));
+                p.npass++;
                 found = true;
                 break;
             }
@@ -376,7 +381,7 @@ FftPlan lucem::make_plan(int L) {
     return p;
 }
 
-// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3") overrides the choices, for
+// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3,col_wide=0") overrides the choices, for
 // measurement sweeps; the autotuner records its choices per shape in the same terms.
 static int tune_value(const char* key) {
     const char* env = std::getenv("LUCEM_TUNE");
@@ -403,10 +408,14 @@ int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
 }
 
 int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide) {
-    const int w = 0;
-    const size_t cap = kLdsComplex;
-    int lgB = 4;
-    while (lgB > 0 && ((size_t)L << lgB) > cap) lgB--;
+    // the 512-thread shape when it doubles the columns of a workgroup below 16
+    int lg_small = 4, lg_big = 4;
+    while (lg_small > 0 && ((size_t)L << lg_small) > (size_t)kLdsComplex) lg_small--;
+    while (lg_big > 0 && ((size_t)L << lg_big) > (size_t)kLdsComplexWide) lg_big--;
+    int w = tune_value("col_wide");
+    if (w < 0) w = lg_big > lg_small ? 1 : 0;
+    const size_t cap = w ? kLdsComplexWide : kLdsComplex;
+    int lgB = w ? lg_big : lg_small;
     while (num_cus > 0 && lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 4 * num_cus) lgB--;
     const int t = tune_value("col_lgB");
     if (t >= 0 && ((size_t)L << t) <= cap) lgB = t;

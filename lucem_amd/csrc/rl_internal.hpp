@@ -17,15 +17,25 @@ constexpr int kMaxPasses = 16;
 constexpr int kThreads = 256;         // threads per workgroup of every FFT kernel
 constexpr int kElemsPerThread = 16;   // complex values per thread: 4096 per workgroup, 32 KiB LDS
 constexpr int kLdsComplex = kThreads * kElemsPerThread;
+constexpr int kThreadsWide = 512;     // line passes over long lines: 8192 complex per workgroup
+constexpr int kLdsComplexWide = kThreadsWide * kElemsPerThread;
 
 
-// One FFT length and its Stockham radix sequence.
+// One FFT length and its Stockham radix sequence: pass q has radix code byte q of the packed
+// words (code = R1 | R2 << 4, R2 = 1 for a simple pass).
 struct FftPlan {
     int L;
     int npass;
     int tw;                            // offset (in float2) of this length's twiddle table
-    unsigned char rad[kMaxPasses];
+    int reserved;
+    unsigned long long codes_lo;       // passes 0..7
+    unsigned long long codes_hi;       // passes 8..15
 };
+
+inline int plan_code(const FftPlan& p, int q) {
+    const unsigned long long w = q < 8 ? p.codes_lo : p.codes_hi;
+    return (int)((w >> (8 * (q & 7))) & 0xffull);
+}
 
 enum RowOp : int {
     ROW_LOAD = 0,        // gather a real array, forward R2C
@@ -72,7 +82,7 @@ struct ColPass {
     int Lc;               // valid columns
     int lgB;              // log2(columns per workgroup)
     int nkb;              // column blocks per outer index
-    int wide;             // reserved (0)
+    int wide;             // 1: the 512-thread line kernel
     float scale;          // COL_FWD_OTF
     int64_t src_outer, src_axis, dst_outer, dst_axis, otf_outer, otf_axis;
     FftPlan f;

@@ -374,9 +374,9 @@ __global__ void zero_ints_kernel(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Launchers. Every FFT kernel is 256 threads x 16 complex values (at most 32 KiB of LDS, at most
-// 128 VGPRs): four workgroups per CU. A 64 KiB shape (32 values per thread) measured slower at
-// every length used here, so only this shape is built.
+// Launchers. FFT kernels hold 16 complex values per thread (at most 128 VGPRs). Row passes use
+// 256 threads (32 KiB of LDS at most, four workgroups per CU); line passes use 256 threads or,
+// for long lines, 512 threads (64 KiB, two workgroups per CU) to double the columns per workgroup.
 // ---------------------------------------------------------------------------------------------
 size_t row_pass_lds_bytes(int L, int lgP) { return ((size_t)L << lgP) * sizeof(float2); }
 
@@ -397,20 +397,23 @@ static int launch_row_e(const RowPass& p, int blocks, void* stream) {
 
 int launch_row_pass(const RowPass& p, int blocks, void* stream) { return launch_row_e<kElemsPerThread>(p, blocks, stream); }
 
-template <int E>
+template <int T, int E>
 static int launch_col_e(const ColPass& p, int blocks, void* stream) {
     const size_t lds = col_pass_lds_bytes(p.f.L, p.lgB);
     hipStream_t st = (hipStream_t)stream;
     switch (p.op) {
-        case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FWD>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        case COL_FUSED: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FUSED>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        case COL_INV: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_INV>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        default: hipLaunchKernelGGL((col_pass_kernel<kThreads, E, COL_FWD_OTF>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD>), dim3(blocks), dim3(T), lds, st, p); break;
+        case COL_FUSED: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FUSED>), dim3(blocks), dim3(T), lds, st, p); break;
+        case COL_INV: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_INV>), dim3(blocks), dim3(T), lds, st, p); break;
+        default: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD_OTF>), dim3(blocks), dim3(T), lds, st, p); break;
     }
     return (int)hipGetLastError();
 }
 
-int launch_col_pass(const ColPass& p, int blocks, void* stream) { return launch_col_e<kElemsPerThread>(p, blocks, stream); }
+int launch_col_pass(const ColPass& p, int blocks, void* stream) {
+    return p.wide ? launch_col_e<kThreadsWide, kElemsPerThread>(p, blocks, stream)
+                  : launch_col_e<kThreads, kElemsPerThread>(p, blocks, stream);
+}
 
 template <int E>
 static int launch_row_batch_e(const RowBatch& b, int grid, size_t lds, void* stream) {
@@ -428,20 +431,21 @@ int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream) {
     return launch_row_batch_e<kElemsPerThread>(b, grid, lds, stream);
 }
 
-template <int E>
+template <int T, int E>
 static int launch_col_batch_e(const ColBatch& b, int grid, size_t lds, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     switch (b.op) {
-        case COL_FWD: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FWD>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        case COL_FUSED: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FUSED>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        case COL_INV: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_INV>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        default: hipLaunchKernelGGL((col_batch_kernel<kThreads, E, COL_FWD_OTF>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case COL_FWD: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD>), dim3(grid), dim3(T), lds, st, b); break;
+        case COL_FUSED: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FUSED>), dim3(grid), dim3(T), lds, st, b); break;
+        case COL_INV: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_INV>), dim3(grid), dim3(T), lds, st, b); break;
+        default: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD_OTF>), dim3(grid), dim3(T), lds, st, b); break;
     }
     return (int)hipGetLastError();
 }
 
 int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream) {
-    return launch_col_batch_e<kElemsPerThread>(b, grid, lds, stream);
+    return b.wide ? launch_col_batch_e<kThreadsWide, kElemsPerThread>(b, grid, lds, stream)
+                  : launch_col_batch_e<kThreads, kElemsPerThread>(b, grid, lds, stream);
 }
 
 int launch_psf_batch(const PsfBatch& b, void* stream) {
@@ -455,12 +459,14 @@ int launch_zero_ints(int* p, int n, void* stream) {
 }
 
 int batch_occupancy(bool row, bool wide, size_t lds) {
-    (void)wide;
     int blocks = 0;
     hipError_t e;
     if (row)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>,
                                                          kThreads, lds);
+    else if (wide)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, col_batch_kernel<kThreadsWide, kElemsPerThread, COL_FUSED>, kThreadsWide, lds);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>,
                                                          kThreads, lds);
