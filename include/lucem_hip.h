@@ -185,6 +185,28 @@ lucem_status lucem_richardson_lucy_batched_f32(lucem_client* client, const lucem
                                                uint32_t psf_ndim, uint64_t num_psfs,
                                                const lucem_rl_settings* settings);
 
+/* ---- Differentiable form --------------------------------------------------------------------
+ * loss_t = sum(u_T - y_t log u_T), u_T = A x_T, after T = num_iter RL iterations (no clip) from a
+ * given initial estimate x0_t, for num_tiles tiles of rank ndim (2 or 3; shapes may differ,
+ * tile_dims is num_tiles x ndim) sharing one PSF. The backward returns d(sum_t w_t loss_t)/d x0_t
+ * and d(...)/d psf (through the reference's normalisation psf / sum(psf), deconvolve.rs:77-79).
+ * The forward keeps a checkpoint of x every `checkpoint_every` iterations (0: only the start);
+ * the backward recomputes the segments before the last from them. x0, y and psf must not change
+ * between a forward and its backward. No reference counterpart. */
+typedef struct lucem_rl_grad lucem_rl_grad;
+lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_t* tile_dims, uint32_t ndim,
+                                  uint64_t num_tiles, const uint64_t* psf_dims, const lucem_rl_settings* settings,
+                                  uint32_t checkpoint_every, lucem_rl_grad** out);
+/* x0[t], y[t]: device pointers; psf: device; x_out: NULL or per-tile device pointers (entries may
+ * be NULL) for x_T; loss: NULL or device double[num_tiles]. */
+lucem_status lucem_rl_grad_forward(lucem_rl_grad* plan, const float* const* x0, const float* const* y,
+                                   const float* psf, float* const* x_out, double* loss);
+/* loss_weight: NULL (all 1) or device float[num_tiles]; grad_x0[t]: device; grad_psf: NULL or device. */
+lucem_status lucem_rl_grad_backward(lucem_rl_grad* plan, const float* loss_weight, float* const* grad_x0,
+                                    float* grad_psf);
+lucem_status lucem_rl_grad_destroy(lucem_rl_grad* plan);
+uint64_t lucem_rl_grad_device_bytes(const lucem_rl_grad* plan);
+
 /* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
  * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is
  * unnormalised. L must be 7-smooth and <= LUCEM_MAX_FFT_LEN. The engine the RL passes use; no

@@ -453,7 +453,9 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
                           const float2* const tws[3], RowPass* row, int* row_blocks, ColPass cols[2],
                           int col_blocks[2]) {
     const int m[3] = {g.k[0], g.k[1], g.k[2]};
-    const float scale = (float)(0.5 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]));
+    // The kernel's own R2C pairs rows and carries a factor 2, and so does the data's: the
+    // transfer function removes both (1/4) and the unnormalised inverse transforms (1/L).
+    const float scale = (float)(0.25 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]));
     RowPass r{};
     r.op = ROW_LOAD;
     r.nx = m[2];

@@ -1,0 +1,341 @@
+// Row (x-axis) passes of the differentiable Richardson-Lucy form (gfx950).
+//
+// Forward model per tile (no clip), for t = 0 .. T-1:
+//   u_t = A x_t          A = correlate with rev = flip(h), the reference's deconvolve.rs:120
+//   r_t = y / safe(u_t)  restore/kernels/deconvolve.rs:7-22
+//   c_t = H r_t          H = correlate with h, deconvolve.rs:133
+//   x_{t+1} = x_t * c_t  restore/kernels/deconvolve.rs:24-43
+// loss = sum(u_T - y log u_T), u_T = A x_T.
+// Backward (reverse mode), with A^T = Pad^T . (correlate with w)^T . Crop^T:
+//   ubar_T = w (1 - y / u_T);  xbar_T = A^T ubar_T
+//   for t = T-1 .. 0:  cbar = xbar_{t+1} x_t;  xa = xbar_{t+1} c_t;  rbar = H^T cbar;
+//                      ubar = -rbar y / u_t^2 (0 where |u_t| < eps);  xbar_t = xa + A^T ubar
+// and the PSF gradient accumulates sum_p ubar[p] P(x)[p - l + k] (and the same for H) as
+// spectral products inside the transposed line passes.
+//
+// Each row pass here is: (C2R of the previous line passes, crop or pad-transpose fold) ->
+// pointwise step -> gather (pad or crop-transpose) -> R2C, with the same two-rows-per-FFT layout
+// as the forward kernels. Storage slots of a checkpoint segment hold x_t, u_t and c_t.
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+#include "rl_grad.hpp"
+#include "rl_internal.hpp"
+
+namespace lucem {
+
+template <int OP>
+struct GradOpTraits {
+    static constexpr bool c2r = OP != G_LOAD_X && OP != G_SEED && OP != G_LOSS_ADJ;
+    static constexpr bool fold = OP == G_ADJ_R || OP == G_ADJ_X;
+    static constexpr bool fwd_gather = OP == G_LOAD_X || OP == G_RATIO || OP == G_UPDATE;
+};
+
+// Folds the halo columns of P interleaved row-pair lines back onto [0, nx) (pad transpose).
+__device__ __forceinline__ void fold_rows(float2* Z, int lgP, int left, int nx, int mode, int end) {
+    const int P = 1 << lgP;
+    const int b = threadIdx.x;
+    if (b < P) {
+        for (int h = 0; h < end; h++) {
+            if (h >= left && h < left + nx) continue;
+            const int t = pad_index(h - left, nx, mode);
+            if (t < 0) continue;
+            const float2 v = Z[(h << lgP) + b];
+            float2& dst = Z[((t + left) << lgP) + b];
+            dst = make_float2(dst.x + v.x, dst.y + v.y);
+        }
+    }
+}
+
+template <int T, int E, int OP>
+__device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item_index, int g, float2* Z,
+                              double* red) {
+    using TR = GradOpTraits<OP>;
+    const RowPass& d = G.r;
+    constexpr int S = E;
+    const int lgP = d.lgP;
+    const int P = 1 << lgP;
+    const int Lf = d.f.L;
+    const int nx = d.nx;
+    const int tid = threadIdx.x;
+    const int pair0 = g * P;
+    const int total = Lf << lgP;
+    const int64_t ns = G.nstride;
+
+    if constexpr (TR::c2r) {
+        constexpr int KS = (E + 1) / 2 + 1;
+        const int Lc = d.Lc;
+        const int nload = Lc << lgP;
+        const float inv_lc = 1.0f / (float)Lc;
+        float2 av[KS], bv[KS];
+        int kk[KS], bb[KS];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int e = tid + s * T;
+            int b = __float2int_rz((float)e * inv_lc);
+            int k = e - b * Lc;
+            if (k >= Lc) { k -= Lc; b++; }
+            if (k < 0) { k += Lc; b--; }
+            kk[s] = k;
+            bb[s] = b;
+            av[s] = make_float2(0.f, 0.f);
+            bv[s] = make_float2(0.f, 0.f);
+            if (e < nload) {
+                const int ra = 2 * (pair0 + b), rb = ra + 1;
+                if (ra < d.rows) av[s] = d.spec[(int64_t)ra * d.pitch + k];
+                if (rb < d.rows) bv[s] = d.spec[(int64_t)rb * d.pitch + k];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int e = tid + s * T;
+            if (e < nload) {
+                const int k = kk[s], b = bb[s];
+                float2 A = av[s], Bv = bv[s];
+                if (k == 0 || 2 * k == Lf) {
+                    A.y = 0.f;
+                    Bv.y = 0.f;
+                }
+                Z[(k << lgP) + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+                const int km = Lf - k;
+                if (k > 0 && km >= Lc) Z[(km << lgP) + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+            }
+        }
+        __syncthreads();
+        lds_fft<true, T, E>(Z, lgP, d.f, d.tw);
+        if constexpr (TR::fold) {
+            fold_rows(Z, lgP, d.left, nx, d.mode, min(Lf, G.kx + nx - 1));
+            __syncthreads();
+        }
+    }
+
+    // pointwise step at every gathered position; owners (i == src + gather_left) store
+    const int gleft = TR::fwd_gather ? d.left : 0;
+    const int gmode = TR::fwd_gather ? d.mode : 1;  // crop transpose: zero outside [0, nx)
+    const int coff = TR::fold ? d.left : 0;         // where the result for column x sits in Z
+    float2 val[S];
+    int srcs[S];
+    double loss = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        srcs[s] = -1;
+        val[s] = make_float2(0.f, 0.f);
+        if (e < total) {
+            const int b = e & (P - 1), i = e >> lgP;
+            const int src = pad_index(i - gleft, nx, gmode);
+            srcs[s] = src;
+            if (src >= 0) {
+                const bool own = i == src + gleft;
+                float2 c = make_float2(0.f, 0.f);
+                if constexpr (TR::c2r) c = Z[((src + coff) << lgP) + b];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int row = 2 * (pair0 + b) + h;
+                    if (row >= d.rows) continue;
+                    const int64_t at = (int64_t)row * nx + src;
+                    const float cv = h ? c.y : c.x;
+                    float v = 0.f;
+                    if constexpr (OP == G_LOAD_X) {
+                        v = L.flags & GF_FROM_X0 ? G.x0[at] : (G.ckpt + L.ck * ns)[at];
+                        if (own) (G.xslots + L.sx * ns)[at] = v;
+                    } else if constexpr (OP == G_RATIO) {
+                        const float den = fabsf(cv) < d.eps ? d.eps : cv;
+                        v = d.obs[at] / den;
+                        if (own && (L.flags & GF_SAVE)) (G.uslots + L.su * ns)[at] = cv;
+                    } else if constexpr (OP == G_UPDATE) {
+                        v = (G.xslots + L.sx * ns)[at] * cv;
+                        if (own) {
+                            (G.xslots + L.sx1 * ns)[at] = v;
+                            if (L.flags & GF_SAVE) (G.cslots + L.su * ns)[at] = cv;
+                            if (L.flags & GF_CKPT) (G.ckpt + L.ck * ns)[at] = v;
+                            if ((L.flags & GF_LAST) && G.x_out) G.x_out[at] = v;
+                        }
+                    } else if constexpr (OP == G_LOSS) {
+                        if (own) {
+                            const float y = d.obs[at];
+                            G.uT[at] = cv;
+#ifdef LUCEM_DEBUG_LOSS_U
+                            if (G.x_out) G.x_out[at] = cv;
+#endif
+                            loss += (double)cv - (double)y * log((double)cv);
+                        }
+                    } else if constexpr (OP == G_LOSS_ADJ) {
+                        const float wgt = G.weight ? G.weight[0] : 1.0f;
+                        v = wgt * (1.0f - d.obs[at] / G.uT[at]);
+                    } else if constexpr (OP == G_ADJ_R) {
+                        const float u = (G.uslots + L.su * ns)[at];
+                        v = fabsf(u) < d.eps ? 0.f : -cv * d.obs[at] / (u * u);
+                    } else if constexpr (OP == G_ADJ_X) {
+                        float xb = cv;
+                        if (!(L.flags & GF_NO_XA)) xb += G.xa[at];
+                        if (L.flags & GF_SEED_NEXT) {
+                            // xbar_t -> the previous iteration: xa = xbar c_{t-1}, cbar = xbar x_{t-1}
+                            if (own) G.xa[at] = xb * (G.cslots + L.scm * ns)[at];
+                            v = xb * (G.xslots + L.sxm * ns)[at];
+                        } else if (own) {
+                            if (L.flags & GF_TO_GX0)
+                                G.gx0[at] = xb;
+                            else
+                                G.xbar[at] = xb;
+                        }
+                    } else if constexpr (OP == G_SEED) {
+                        const float xb = G.xbar[at];
+                        if (own) G.xa[at] = xb * (G.cslots + L.scm * ns)[at];
+                        v = xb * (G.xslots + L.sxm * ns)[at];
+                    } else if constexpr (OP == G_C2R) {
+                        if (own) G.out[at] = cv * G.scale;
+                    }
+                    if (h)
+                        val[s].y = v;
+                    else
+                        val[s].x = v;
+                }
+            }
+        }
+    }
+    if constexpr (OP == G_LOSS) {
+        // deterministic per-workgroup partial of the loss
+        red[tid] = loss;
+        __syncthreads();
+        for (int w = T / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) G.partial[g] = red[0];
+    }
+    if constexpr (OP == G_C2R || OP == G_LOSS) return;
+    if constexpr (OP == G_ADJ_X) {
+        if (!(L.flags & GF_SEED_NEXT)) return;  // uniform
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        if (e < total) Z[e] = val[s];
+    }
+    __syncthreads();
+    lds_fft<false, T, E>(Z, lgP, d.f, d.tw);
+    for (int b = 0; b < P; b++) {
+        const int ra = 2 * (pair0 + b), rb = ra + 1;
+        const bool va = ra < d.rows, vb = rb < d.rows;
+        float2* Ar = d.spec + (int64_t)ra * d.pitch;
+        float2* Br = d.spec + (int64_t)rb * d.pitch;
+        for (int k = tid; k < d.Lc; k += T) {
+            const float2 zk = Z[(k << lgP) + b];
+            const float2 zm = Z[((k == 0 ? 0 : Lf - k) << lgP) + b];
+            if (va) Ar[k] = make_float2(zk.x + zm.x, zk.y - zm.y);
+            if (vb) Br[k] = make_float2(zk.y + zm.y, zm.x - zk.x);
+        }
+    }
+    (void)item_index;
+}
+
+template <int T, int E, int OP>
+__global__ __launch_bounds__(T) void grad_row_kernel(GradRowBatch bt) {
+    extern __shared__ float2 lds[];
+    __shared__ double red[T];
+    const int it = xcd_block(blockIdx.x, gridDim.x);
+    if (it >= bt.nitems) return;
+    const int2 item = bt.items[it];
+    const GradTile G = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
+    grad_row_body<T, E, OP>(G, bt, it, __builtin_amdgcn_readfirstlane(item.y), lds, red);
+}
+
+// Sums the loss partials of each tile in a fixed order.
+__global__ __launch_bounds__(256) void loss_reduce_kernel(const double* partials, const int* first, const int* count,
+                                                          double* loss, int ntiles) {
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count[t]; i += 256) s += partials[first[t] + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[t] = red[0];
+}
+
+// d loss / d psf from the per-tile correlation gradients of h and rev (k elements each):
+// dh = sum_tiles (g_h + flip(g_rev)); through the normalisation h = psf / sum(psf) of
+// deconvolve.rs:77-79: dpsf = (dh - sum(dh * h)) / sum(psf) when |sum| > 1e-12, else dh.
+__global__ __launch_bounds__(256) void psf_grad_kernel(const float* gh, const float* grev, int ntiles, int64_t nk,
+                                                       const float* psf, float* out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < nk; i += 256) s += (double)psf[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double sum = red[0];
+    __syncthreads();
+    const bool normed = fabs(sum) > 1e-12;
+    const double norm = normed ? 1.0 / sum : 1.0;
+    double dot = 0.0;
+    for (int64_t i = threadIdx.x; i < nk; i += 256) {
+        double dh = 0.0;
+        for (int t = 0; t < ntiles; t++) dh += (double)gh[t * nk + i] + (double)grev[t * nk + (nk - 1 - i)];
+        dot += dh * (double)psf[i] * norm;
+    }
+    red[threadIdx.x] = dot;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double total = red[0];
+    for (int64_t i = threadIdx.x; i < nk; i += 256) {
+        double dh = 0.0;
+        for (int t = 0; t < ntiles; t++) dh += (double)gh[t * nk + i] + (double)grev[t * nk + (nk - 1 - i)];
+        out[i] = (float)(normed ? (dh - total) * norm : dh);
+    }
+}
+
+__global__ void zero_floats_kernel(float* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+int launch_grad_row(const GradRowBatch& b, size_t lds, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(b.nitems), block(kThreads);
+    switch (b.op) {
+        case G_LOAD_X: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOAD_X>), grid, block, lds, st, b); break;
+        case G_RATIO: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_RATIO>), grid, block, lds, st, b); break;
+        case G_UPDATE: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_UPDATE>), grid, block, lds, st, b); break;
+        case G_LOSS: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOSS>), grid, block, lds, st, b); break;
+        case G_ADJ_R: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_ADJ_R>), grid, block, lds, st, b); break;
+        case G_ADJ_X: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_ADJ_X>), grid, block, lds, st, b); break;
+        case G_SEED: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_SEED>), grid, block, lds, st, b); break;
+        case G_LOSS_ADJ: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOSS_ADJ>), grid, block, lds, st, b); break;
+        default: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_C2R>), grid, block, lds, st, b); break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_loss_reduce(const double* partials, const int* first, const int* count, double* loss, int ntiles,
+                       void* stream) {
+    hipLaunchKernelGGL(loss_reduce_kernel, dim3(ntiles), dim3(256), 0, (hipStream_t)stream, partials, first, count,
+                       loss, ntiles);
+    return (int)hipGetLastError();
+}
+
+int launch_psf_grad(const float* gh, const float* grev, int ntiles, int64_t nk, const float* psf, float* out,
+                    void* stream) {
+    hipLaunchKernelGGL(psf_grad_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, gh, grev, ntiles, nk, psf, out);
+    return (int)hipGetLastError();
+}
+
+int launch_zero_floats(float* p, int64_t n, void* stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(zero_floats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+}  // namespace lucem

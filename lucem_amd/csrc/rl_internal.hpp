@@ -49,6 +49,8 @@ enum ColOp : int {
     COL_FUSED = 1,  // gather, forward FFT, multiply by the transfer function, inverse FFT, crop
     COL_INV = 2,    // load L bins, inverse FFT, crop
     COL_FWD_OTF = 3,  // as COL_FWD, storing conj(v) * scale (transfer function build)
+    COL_FUSED_T = 4,  // transposed correlation: gather, FFT, (accumulate), * conj(H), IFFT, fold back
+    COL_INV_FOLD = 5, // load L bins, inverse FFT, fold the padded positions back onto [0, n)
 };
 
 // Row (x axis) pass over `rows` real rows of length nx, paired two by two.
@@ -90,6 +92,12 @@ struct ColPass {
     float2* dst;
     const float2* otf;
     const float2* tw;
+    // differentiable form (laid out like otf; slot k at base + k * slot_stride):
+    float2* save;           // COL_FUSED: the padded input's spectrum, stored before the multiply
+    float2* gacc;           // COL_FUSED_T: gacc += save[slot] * conj(spectrum of the input)
+    int64_t slot_stride;
+    int fold_left, fold_mode, fold_n;  // COL_FUSED_T / COL_INV_FOLD: the forward's pad (transposed)
+    int fold_end;                      // n + k - 1: end of the padded support
 };
 
 // PSF preparation (deconvolve.rs:73-90 on the device): f64 sum, norm = 1/sum when |sum| > 1e-12
@@ -119,6 +127,7 @@ struct ColBatch {
     int op;
     int wide;
     int* counter;
+    int slot;               // differentiable form: storage slot of save / gacc operands
 };
 
 struct PsfBatch {

@@ -186,7 +186,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
     __syncthreads();
     lds_fft<false, T, E>(Z, lgP, d.f, d.tw);
     // split the two real transforms: A = Z[k] + conj(Z[L-k]), B = -i (Z[k] - conj(Z[L-k]))
-    // (both carry a factor 2 that the transfer function removes)
+    // (both carry a factor 2; the transfer function is scaled to remove it)
     for (int b = 0; b < P; b++) {
         const int ra = 2 * (pair0 + b), rb = ra + 1;
         const bool va = ra < d.rows, vb = rb < d.rows;
@@ -210,8 +210,28 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row
 // ---------------------------------------------------------------------------------------------
 // Line pass along y or z. Workgroup g owns columns [kb*B, kb*B + B) of outer index `outer`.
 // ---------------------------------------------------------------------------------------------
+// Folds the padded positions of B interleaved lines back onto [0, n) in place (the transpose of
+// the pad gather Q[i] = x[pad(i - left)]): every halo position h < left or h >= left + n (below
+// n + k - 1 <= L) is added to position left + pad(h - left). One thread per line, halo in order.
+__device__ __forceinline__ void fold_lines(float2* Z, int lgB, int nlines_valid, int L, int left, int n, int mode,
+                                           int nhalo_end) {
+    const int B = 1 << lgB;
+    const int b = threadIdx.x;
+    if (b < B && b < nlines_valid) {
+        for (int h = 0; h < nhalo_end; h++) {
+            if (h >= left && h < left + n) continue;
+            const int t = pad_index(h - left, n, mode);
+            if (t < 0) continue;
+            const float2 v = Z[(h << lgB) + b];
+            float2& dst = Z[((t + left) << lgB) + b];
+            dst = make_float2(dst.x + v.x, dst.y + v.y);
+        }
+    }
+    (void)L;
+}
+
 template <int T, int E, int OP>
-__device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
+__device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) {
     const int lgB = d.lgB;
     const int B = 1 << lgB;
     const int outer = g / d.nkb;
@@ -221,6 +241,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
     const float2* src = d.src + outer * d.src_outer + kx0;
     const int total = L << lgB;
     const bool full = kx0 + B <= d.Lc;
+    constexpr bool INV_IN = OP == COL_INV || OP == COL_INV_FOLD;
     {
         float2 v[E];
 #pragma unroll
@@ -229,7 +250,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
             v[s] = make_float2(0.f, 0.f);
             if (e < total) {
                 const int b = e & (B - 1), i = e >> lgB;
-                const int sr = OP == COL_INV ? i : pad_index(i - d.left, d.n_src, d.mode);
+                const int sr = INV_IN ? i : pad_index(i - d.left, d.n_src, d.mode);
                 if (sr >= 0 && (full || kx0 + b < d.Lc)) v[s] = src[sr * d.src_axis + b];
             }
         }
@@ -240,12 +261,13 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
         }
     }
     __syncthreads();
-    if constexpr (OP == COL_INV) {
+    if constexpr (INV_IN) {
         lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
     } else {
         lds_fft<false, T, E>(Z, lgB, d.f, d.tw);
-        if constexpr (OP == COL_FUSED) {
-            const float2* otf = d.otf + outer * d.otf_outer + kx0;
+        if constexpr (OP == COL_FUSED || OP == COL_FUSED_T) {
+            const int64_t off = outer * d.otf_outer + kx0;
+            const float2* otf = d.otf + off;
             float2 w[E];
 #pragma unroll
             for (int s = 0; s < E; s++) {
@@ -256,21 +278,65 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z) {
                     if (full || kx0 + b < d.Lc) w[s] = otf[i * d.otf_axis + b];
                 }
             }
+            if constexpr (OP == COL_FUSED) {
+                if (d.save) {  // the padded input's spectrum, for the gradient of the PSF
+                    float2* sv = d.save + slot * d.slot_stride + off;
+#pragma unroll
+                    for (int s = 0; s < E; s++) {
+                        const int e = tid + s * T;
+                        if (e < total) {
+                            const int b = e & (B - 1), i = e >> lgB;
+                            if (full || kx0 + b < d.Lc) sv[i * d.otf_axis + b] = Z[e];
+                        }
+                    }
+                }
+            } else {
+                if (d.gacc) {  // gacc += saved * conj(this spectrum)
+                    const float2* sv = d.save + slot * d.slot_stride + off;
+                    float2* ga = d.gacc + off;
+#pragma unroll
+                    for (int s = 0; s < E; s++) {
+                        const int e = tid + s * T;
+                        if (e < total) {
+                            const int b = e & (B - 1), i = e >> lgB;
+                            if (full || kx0 + b < d.Lc) {
+                                const float2 a = sv[i * d.otf_axis + b], z = Z[e];
+                                float2 acc = ga[i * d.otf_axis + b];
+                                acc.x += a.x * z.x + a.y * z.y;
+                                acc.y += a.y * z.x - a.x * z.y;
+                                ga[i * d.otf_axis + b] = acc;
+                            }
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int s = 0; s < E; s++) {
                 const int e = tid + s * T;
-                if (e < total) Z[e] = cmul(Z[e], w[s]);
+                if (e < total) {
+                    if constexpr (OP == COL_FUSED_T) {
+                        Z[e] = cmul(Z[e], make_float2(w[s].x, -w[s].y));
+                    } else {
+                        Z[e] = cmul(Z[e], w[s]);
+                    }
+                }
             }
             __syncthreads();
             lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
         }
+    }
+    int row0 = 0;
+    if constexpr (OP == COL_FUSED_T || OP == COL_INV_FOLD) {
+        fold_lines(Z, lgB, d.Lc - kx0, L, d.fold_left, d.fold_n, d.fold_mode, min(L, d.fold_end));
+        __syncthreads();
+        row0 = d.fold_left;
     }
     float2* dst = d.dst + outer * d.dst_outer + kx0;
     const int nstore = d.n_dst << lgB;
     for (int e = tid; e < nstore; e += T) {
         const int b = e & (B - 1), i = e >> lgB;
         if (full || kx0 + b < d.Lc) {
-            float2 v = Z[e];
+            float2 v = Z[e + (row0 << lgB)];
             if constexpr (OP == COL_FWD_OTF) v = make_float2(v.x * d.scale, -v.y * d.scale);
             dst[i * d.dst_axis + b] = v;
         }
@@ -343,7 +409,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void col
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
-        col_pass_body<T, E, OP>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
+        col_pass_body<T, E, OP>(d, __builtin_amdgcn_readfirstlane(item.y), lds, bt.slot);
         __syncthreads();
     }
 }
@@ -405,6 +471,8 @@ static int launch_col_e(const ColPass& p, int blocks, void* stream) {
         case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD>), dim3(blocks), dim3(T), lds, st, p); break;
         case COL_FUSED: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FUSED>), dim3(blocks), dim3(T), lds, st, p); break;
         case COL_INV: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_INV>), dim3(blocks), dim3(T), lds, st, p); break;
+        case COL_FUSED_T: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FUSED_T>), dim3(blocks), dim3(T), lds, st, p); break;
+        case COL_INV_FOLD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_INV_FOLD>), dim3(blocks), dim3(T), lds, st, p); break;
         default: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD_OTF>), dim3(blocks), dim3(T), lds, st, p); break;
     }
     return (int)hipGetLastError();
@@ -438,6 +506,8 @@ static int launch_col_batch_e(const ColBatch& b, int grid, size_t lds, void* str
         case COL_FWD: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD>), dim3(grid), dim3(T), lds, st, b); break;
         case COL_FUSED: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FUSED>), dim3(grid), dim3(T), lds, st, b); break;
         case COL_INV: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_INV>), dim3(grid), dim3(T), lds, st, b); break;
+        case COL_FUSED_T: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FUSED_T>), dim3(grid), dim3(T), lds, st, b); break;
+        case COL_INV_FOLD: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_INV_FOLD>), dim3(grid), dim3(T), lds, st, b); break;
         default: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD_OTF>), dim3(grid), dim3(T), lds, st, b); break;
     }
     return (int)hipGetLastError();

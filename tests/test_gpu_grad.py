@@ -1,0 +1,161 @@
+"""GPU tests of the differentiable form: loss, d/dx0 and d/dpsf against the FP64 oracle (exact
+reverse-mode derivative of the FP64 direct restatement, itself pinned by finite differences in
+tests/test_oracle.py), checkpointing invariance, 3-D and uneven tiles, autograd plumbing.
+
+Tolerance: gradients within 1e-3 relative L2 of FP64; losses 1e-6 relative.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-3
+
+
+def _tiles(rng, n, shape, lo=50.0, hi=150.0):
+    return [rng.uniform(lo, hi, shape).astype(np.float32) for _ in range(n)]
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+def _run(x0s, ys, psf, num_iter, k=5, mode=0):
+    import torch
+
+    from lucem_amd.autograd import rl_nll
+
+    x0 = [torch.from_numpy(x).cuda().requires_grad_(True) for x in x0s]
+    y = [torch.from_numpy(v).cuda() for v in ys]
+    p = torch.from_numpy(psf).cuda().requires_grad_(True)
+    loss = rl_nll(x0, y, p, num_iter, checkpoint_every=k, mode=mode)
+    loss.sum().backward()
+    torch.cuda.synchronize()
+    return loss.detach().cpu().numpy(), [x.grad.cpu().numpy() for x in x0], p.grad.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_grad_2d_matches_fp64_oracle(gpu, oracle, mode):
+    rng = np.random.default_rng(30 + mode)
+    shape, kshape, T = (24, 29), (5, 4), 4
+    x0s = _tiles(rng, 3, shape, 0.5, 2.0)
+    ys = _tiles(rng, 3, shape)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    loss, gx, gp = _run(x0s, ys, psf, T, k=2, mode=mode)
+    gp_ref = np.zeros(kshape)
+    for i in range(3):
+        L, gxr, gpr = oracle.rl_nll_grad64(ys[i], x0s[i], psf, T, mode=mode)
+        assert abs(loss[i] - L) <= 1e-6 * abs(L)
+        e = _rel(gx[i], gxr)
+        print(f"mode {mode} tile {i}: dL/dx0 rel-L2 vs FP64 {e:.2e}")
+        assert e < GRAD_TOL
+        gp_ref += gpr
+    e = _rel(gp, gp_ref)
+    print(f"mode {mode}: dL/dpsf rel-L2 vs FP64 {e:.2e}")
+    assert e < GRAD_TOL
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 7])
+def test_checkpoint_interval_does_not_change_results(gpu, k):
+    rng = np.random.default_rng(40)
+    x0s = _tiles(rng, 2, (33, 20), 0.5, 2.0)
+    ys = _tiles(rng, 2, (33, 20))
+    psf = rng.uniform(0.1, 1.0, (7, 5)).astype(np.float32)
+    ref = _run(x0s, ys, psf, 7, k=7)
+    got = _run(x0s, ys, psf, 7, k=k)
+    np.testing.assert_allclose(got[0], ref[0], rtol=1e-12)
+    for a, b in zip(got[1], ref[1]):
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(got[2], ref[2], rtol=1e-5, atol=1e-9)
+
+
+def test_grad_3d_uneven_depths(gpu, oracle):
+    rng = np.random.default_rng(50)
+    shapes = [(6, 14, 12), (9, 14, 12), (4, 10, 16)]
+    psf = rng.uniform(0.1, 1.0, (3, 5, 3)).astype(np.float32)
+    x0s = [rng.uniform(0.5, 2.0, s).astype(np.float32) for s in shapes]
+    ys = [rng.uniform(50, 150, s).astype(np.float32) for s in shapes]
+    loss, gx, gp = _run(x0s, ys, psf, 3, k=2)
+    gp_ref = np.zeros(psf.shape)
+    for i in range(3):
+        L, gxr, gpr = oracle.rl_nll_grad64(ys[i], x0s[i], psf, 3)
+        assert abs(loss[i] - L) <= 1e-6 * abs(L)
+        assert _rel(gx[i], gxr) < GRAD_TOL
+        gp_ref += gpr
+    assert _rel(gp, gp_ref) < GRAD_TOL
+
+
+def test_grad_loss_weights_and_autograd_chain(gpu, oracle):
+    """Weighted sum of tile losses, and a Gaussian-sigma parameter chained by torch autograd."""
+    import torch
+
+    from lucem_amd.autograd import gaussian_psf, rl_nll
+
+    rng = np.random.default_rng(60)
+    shape = (20, 22)
+    x0s = _tiles(rng, 2, shape, 0.5, 2.0)
+    ys = _tiles(rng, 2, shape)
+    sigma = torch.tensor(1.3, device="cuda", dtype=torch.float32, requires_grad=True)
+    psf = gaussian_psf(sigma, (7, 7))
+    x0 = [torch.from_numpy(x).cuda() for x in x0s]
+    y = [torch.from_numpy(v).cuda() for v in ys]
+    loss = rl_nll(x0, y, psf, 3, checkpoint_every=2)
+    (2.0 * loss[0] + 0.5 * loss[1]).backward()
+    g = float(sigma.grad)
+    # FP64 central difference on sigma through the oracle
+    def total(s):
+        p = gaussian_psf(torch.tensor(s, dtype=torch.float64), (7, 7)).numpy()
+        return (2.0 * oracle.rl_nll64(ys[0], x0s[0], p, 3) + 0.5 * oracle.rl_nll64(ys[1], x0s[1], p, 3))
+    h = 1e-4
+    fd = (total(1.3 + h) - total(1.3 - h)) / (2 * h)
+    print(f"d/dsigma: {g:.6e} vs FP64 FD {fd:.6e}")
+    assert abs(g - fd) <= GRAD_TOL * abs(fd)
+
+
+def test_grad_fd_on_pixels(gpu, oracle):
+    """d loss / d x0 at seeded pixels against FP64 central finite differences (config law,
+    reduced size), step 1e-3 * max(1, |x|)."""
+    from lucem_amd import synth
+
+    ys_all, psf, truths, T = synth.grad2d(count=2, side=48, num_iter=10)
+    x0s = [np.full(ys_all[i].shape, 0.5 * float(ys_all[i].mean()), np.float32) for i in range(2)]
+    ys = [ys_all[i] for i in range(2)]
+    loss, gx, gp = _run(x0s, ys, psf, T, k=5)
+    rng = np.random.default_rng(70)
+    for i in range(2):
+        for _ in range(8):
+            p = tuple(int(v) for v in rng.integers(0, 48, 2))
+            x = x0s[i].astype(np.float64)
+            step = 1e-3 * max(1.0, abs(x[p]))
+            xp, xm = x.copy(), x.copy()
+            xp[p] += step
+            xm[p] -= step
+            fd = (oracle.rl_nll64(ys[i], xp, psf, T) - oracle.rl_nll64(ys[i], xm, psf, T)) / (2 * step)
+            assert abs(gx[i][p] - fd) <= GRAD_TOL * max(abs(fd), 1e-3 * np.abs(gx[i]).max())
+
+
+def test_backward_after_other_forward_recomputes(gpu):
+    import torch
+
+    from lucem_amd.autograd import rl_nll
+
+    rng = np.random.default_rng(80)
+    x0 = [torch.from_numpy(rng.uniform(0.5, 2.0, (16, 16)).astype(np.float32)).cuda().requires_grad_(True)]
+    y1 = [torch.from_numpy(rng.uniform(50, 150, (16, 16)).astype(np.float32)).cuda()]
+    y2 = [torch.from_numpy(rng.uniform(50, 150, (16, 16)).astype(np.float32)).cuda()]
+    psf = torch.from_numpy(rng.uniform(0.1, 1.0, (3, 3)).astype(np.float32)).cuda()
+    l1 = rl_nll(x0, y1, psf, 3)
+    l1.sum().backward()
+    g_ref = x0[0].grad.clone()
+    x0[0].grad = None
+    l1 = rl_nll(x0, y1, psf, 3)
+    l2 = rl_nll(x0, y2, psf, 3)  # same plan, later forward
+    l1.sum().backward()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(x0[0].grad.cpu().numpy(), g_ref.cpu().numpy(), rtol=1e-6)
+    del l2

@@ -168,6 +168,22 @@ def test_zero_regions_and_filter_epsilon(gpu, oracle):
         assert np.isfinite(got).all() and oracle.rel_l2(got, ref) < RTOL_L2
 
 
+def test_eps_rule_uses_the_exact_correlation(gpu, oracle):
+    """|u| < eps replaces u by eps (restore/kernels/deconvolve.rs:17-21). With data near eps the
+    rule fires on some pixels and not others, so a correlation off by any constant factor (which
+    RL alone would not show) changes the result."""
+    rng = np.random.default_rng(8)
+    y = rng.uniform(0.5e-3, 2.0e-3, (30, 34)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (5, 5)).astype(np.float32)
+    for eps in (1.0e-3, 1.5e-3):
+        got = _run(y, psf, 3, eps=eps)
+        st, ref = oracle.richardson_lucy(y, psf, 3, clip=False, filter_epsilon=eps)
+        assert oracle.rel_l2(got, ref) < RTOL_L2
+    # the case discriminates: a correlation twice too large is equivalent to eps / 2
+    _, half = oracle.richardson_lucy(y, psf, 3, clip=False, filter_epsilon=0.75e-3)
+    assert oracle.rel_l2(half, ref) > 1e-2
+
+
 def test_plain2d_full_size(gpu, oracle):
     """1024x1024, 25x25 Gaussian sigma 2, 30 iterations (the 2-D benchmark workload)."""
     from lucem_amd import synth
