@@ -155,9 +155,6 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
                         if (own) {
                             const float y = d.obs[at];
                             G.uT[at] = cv;
-#ifdef LUCEM_DEBUG_LOSS_U
-                            if (G.x_out) G.x_out[at] = cv;
-#endif
                             loss += (double)cv - (double)y * log((double)cv);
                         }
                     } else if constexpr (OP == G_LOSS_ADJ) {
