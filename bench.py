@@ -18,7 +18,12 @@ Alongside the throughput the line carries:
 - ``iteration``: the whole RL iteration's algorithmic bytes (24 n + 80 N_c in 2-D, 24 n + 144 N_c
   in 3-D, on the minimal linear-convolution domain n + k - 1) over the measured time per iteration;
 - ``cpu_baseline``: the CPU restatement of the reference algorithm (direct correlation, FP32,
-  OpenMP) timed on this host on a bounded sample of the same workload.
+  OpenMP) timed on this host on a bounded sample of the same workload;
+- ``batched``: the uneven-tile slide workload (300 tiles of 64..512 px per side, 5 Gaussian PSFs
+  of 11..21 px, 20 iterations) through the batched form (one graph launch per call) against the
+  plain 2-D entry point called once per tile, with both results compared;
+- ``grad``: the differentiable form on 64 tiles of 256x256, PSF 17x17, 10 iterations, checkpoint
+  every 5: forward time, forward + backward (d/dx0 and d/dpsf) time and their ratio.
 """
 
 from __future__ import annotations
@@ -67,7 +72,7 @@ def cpu_baseline(prob, workload: str, want_seconds: float = 15.0):
     O.build()
     y = prob.image
     if workload == "plain3d":
-        iters = 2
+        iters = 6
         sample = f"full 128x512x512 volume, {iters} of the 50 iterations (cost per iteration is constant)"
     else:
         iters = prob.num_iter
@@ -85,6 +90,94 @@ def cpu_baseline(prob, workload: str, want_seconds: float = 15.0):
     }
 
 
+def _timeit(fn, reps: int) -> float:
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def batched_leg(steps: int) -> dict:
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+    from lucem_amd import synth
+    from lucem_amd.batched import RLBatch
+
+    ts = synth.tiles()
+    ys = [torch.from_numpy(t).cuda() for t in ts.tiles]
+    psfs = [torch.from_numpy(p).cuda() for p in ts.psfs]
+    settings = LA.RichardsonLucy(num_iter=ts.num_iter, clip=False)
+    outs = [torch.empty_like(y) for y in ys]
+    b = RLBatch([tuple(y.shape) for y in ys], [tuple(p.shape) for p in psfs], ts.psf_index, settings)
+    tb = _timeit(lambda: b.execute(ys, outs, psfs), max(steps, 3))
+    client = N.default_client(0)
+    ws = N.Workspace(client)
+    outs_p = [torch.empty_like(y) for y in ys]
+
+    def loop():
+        for y, o, i in zip(ys, outs_p, ts.psf_index):
+            LA.richardson_lucy_into(y, psfs[i], o, settings, ws, client)
+
+    tp = _timeit(loop, 2)
+    diff = max(float(((a - c).norm() / c.norm()).item()) for a, c in zip(outs, outs_p))
+    nvox = sum(y.numel() for y in ys)
+    alg = sum(minimal_domain_bytes(tuple(y.shape), tuple(psfs[i].shape)) for y, i in zip(ys, ts.psf_index))
+    gbps = alg * ts.num_iter / tb / 1e9
+    return {
+        "workload": f"{len(ys)} tiles (64..512 px per side), {len(psfs)} PSFs, {ts.num_iter} iterations",
+        "ms_per_step": round(tb * 1e3, 3),
+        "value": round(nvox * ts.num_iter / tb / 1e6, 2),
+        "unit": "Mvoxel-iters/s",
+        "achieved_GBps": round(gbps, 1),
+        "frac": round(gbps / HBM_PEAK_GBPS, 4),
+        "plain_loop_ms": round(tp * 1e3, 3),
+        "plain_loop_value": round(nvox * ts.num_iter / tp / 1e6, 2),
+        "plain_loop_achieved_GBps": round(alg * ts.num_iter / tp / 1e9, 1),
+        "speedup_vs_plain_loop": round(tp / tb, 3),
+        "max_rel_l2_batched_vs_plain": diff,
+        "device_bytes": b.device_bytes,
+    }
+
+
+def grad_leg(steps: int) -> dict:
+    import torch
+
+    from lucem_amd import synth
+    from lucem_amd.autograd import RLGradPlan
+
+    ys, psf, _, T = synth.grad2d()
+    y = torch.from_numpy(ys).cuda()
+    p = torch.from_numpy(psf).cuda()
+    n = y.shape[0]
+    plan = RLGradPlan([tuple(y.shape[1:])] * n, psf.shape, T, 5)
+    xs = list(torch.full_like(y, 0.5).unbind(0))
+    yl = list(y.unbind(0))
+    gx = [torch.empty_like(a) for a in xs]
+    gp = torch.empty_like(p)
+
+    def step():
+        plan.forward(xs, yl, p)
+        plan.backward(None, gx, gp)
+
+    reps = max(steps, 5)
+    tf = _timeit(lambda: plan.forward(xs, yl, p), reps)
+    tg = _timeit(step, reps)
+    return {
+        "workload": f"{n} tiles of 256x256, psf 17x17, {T} iterations, checkpoint every 5",
+        "forward_ms": round(tf * 1e3, 4),
+        "grad_step_ms": round(tg * 1e3, 4),
+        "ratio": round(tg / tf, 3),
+        "device_bytes": plan.device_bytes,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +185,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain3d", choices=["plain3d", "plain2d"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and grad legs")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -186,6 +280,9 @@ def main() -> int:
                     for k, v in stats.items()},
         "cpu_baseline": None,
     }
+    if not args.no_extras:
+        line["batched"] = batched_leg(args.steps)
+        line["grad"] = grad_leg(args.steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))

@@ -218,7 +218,15 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
     P->h_tiles.assign(ntiles, GradTile{});
     int total_blocks = 0;
 
+    int64_t gh_total = 0;  // floats in the spectral PSF-gradient accumulators of all tiles
     auto layout = [&](Carve& cv) {
+        // accumulators of every tile in one block, so that one launch clears them all
+        gh_total = 0;
+        for (int t = 0; t < ntiles; t++)
+            gh_total += 2 * (int64_t)P->geos[t].L[0] * P->geos[t].L[1] * P->geos[t].pitch;
+        float2* gh_pool = cv.take<float2>((size_t)gh_total);
+        gh_total *= 2;
+        int64_t gh_off = 0;
         for (int t = 0; t < ntiles; t++) {
             const Geometry& g = P->geos[t];
             TileMem& m = tm[t];
@@ -229,7 +237,8 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
             m.sb = three ? cv.take<float2>((size_t)g.n[0] * g.L[1] * g.pitch) : nullptr;
             m.sx = cv.take<float2>((size_t)(K + 1) * m.spec_slot);
             m.sr = cv.take<float2>((size_t)K * m.spec_slot);
-            m.gh = cv.take<float2>((size_t)2 * m.spec_slot);  // gh then grev, contiguous
+            m.gh = gh_pool ? gh_pool + gh_off : nullptr;  // gh then grev, contiguous
+            gh_off += 2 * m.spec_slot;
             m.grev = m.gh ? m.gh + m.spec_slot : nullptr;
             m.xs = cv.take<float>((size_t)(K + 1) * m.nstride);
             m.us = cv.take<float>((size_t)K * m.nstride);
@@ -676,8 +685,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
             err = launch_psf_batch(PsfBatch{d_psf, 1}, P->capture);
         } else {
             // the spectral accumulators of every tile start at zero
-            for (int t = 0; t < ntiles && !err; t++)
-                err = launch_zero_floats(reinterpret_cast<float*>(tm[t].gh), 4 * tm[t].spec_slot, P->capture);
+            err = launch_zero_floats(reinterpret_cast<float*>(tm[0].gh), gh_total, P->capture);
         }
         lucem_status rs = err ? hip_fail(err, "capture") : run_seq(P->capture, seq, nullptr);
         if (!rs && forward) {

@@ -294,9 +294,15 @@ __global__ __launch_bounds__(256) void psf_grad_kernel(const float* gh, const fl
     }
 }
 
+// p is 16-byte aligned; grid-stride over float4 stores, then the tail
 __global__ void zero_floats_kernel(float* p, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n4 = n >> 2;
+    float4* q = reinterpret_cast<float4*>(p);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t t = (n4 << 2) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.f;
 }
 
 int launch_grad_row(const GradRowBatch& b, size_t lds, void* stream) {
@@ -331,7 +337,8 @@ int launch_psf_grad(const float* gh, const float* grev, int ntiles, int64_t nk, 
 
 int launch_zero_floats(float* p, int64_t n, void* stream) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(zero_floats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, n);
+    const int64_t blocks = std::min<int64_t>(((n >> 2) + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(zero_floats_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
     return (int)hipGetLastError();
 }
 

@@ -1,0 +1,176 @@
+"""HBM traffic per kernel launch from rocprofv3 counter runs.
+
+Two helpers for the ``traffic`` figure of ``bench.py``:
+
+``python -m lucem_amd.profiling copy``
+    a plain device-to-device copy of 1 GiB, three times: the calibration program. Its
+    ``FETCH_SIZE`` / ``WRITE_SIZE`` readings against the known 1 GiB each way give the factor that
+    turns each counter into bytes on this GPU.
+
+``python -m lucem_amd.profiling run batched|grad [REPS]``
+    a short run of one workload for a kernel trace: the uneven-tile slide batch (300 tiles, 20
+    iterations) or the differentiable form (64 tiles of 256x256, forward + backward), REPS calls
+    (default 3) after one warm-up call.
+
+``python -m lucem_amd.profiling summarize WORKLOAD FETCH_DIR WRITE_DIR FETCH_CAL_DIR WRITE_CAL_DIR``
+    reads the ``run_counter_collection.csv`` files of four ``rocprofv3 --pmc FETCH_SIZE`` /
+    ``--pmc WRITE_SIZE`` runs (the bench and the copy, each counter in a run of its own) and writes
+    ``profiles/traffic_<WORKLOAD>.json``: per pass, the median calibrated HBM bytes per launch.
+
+rocprofv3 names kernels by template (``col_pass_kernel<256, 16, 1>``); the op id is the last
+template argument, and each RL pass of a workload is one (kernel family, op) pair.
+"""
+
+from __future__ import annotations
+
+import csv
+import json
+import re
+import statistics
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+
+# pass name used by the library's own profiling -> (kernel family, op id)
+PASS_KERNELS = {
+    "plain3d": {
+        "line_fwd_y": ("col_pass_kernel", 0),
+        "line_fused_z": ("col_pass_kernel", 1),
+        "line_inv_y": ("col_pass_kernel", 2),
+        "row_ratio": ("row_pass_kernel", 2),
+        "row_update": ("row_pass_kernel", 3),
+    },
+    "plain2d": {
+        "line_fused_y": ("col_pass_kernel", 1),
+        "row_ratio": ("row_pass_kernel", 2),
+        "row_update": ("row_pass_kernel", 3),
+    },
+    "batched": {
+        "line_fused_y": ("col_batch_kernel", 1),
+        "row_ratio": ("row_batch_kernel", 2),
+        "row_update": ("row_batch_kernel", 3),
+    },
+}
+
+_NAME = re.compile(r"lucem::(\w+)<(\d+), (\d+), (\d+)>")
+CAL_BYTES = 1 << 30
+
+
+def copy_calibration() -> None:
+    import torch
+
+    a = torch.empty(CAL_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {CAL_BYTES} bytes 3 times", flush=True)
+
+
+def run_workload(name: str, reps: int = 3) -> None:
+    import torch
+
+    from . import synth
+    from .restore import RichardsonLucy
+
+    if name == "batched":
+        from .batched import RLBatch
+
+        ts = synth.tiles()
+        ys = [torch.from_numpy(t).cuda() for t in ts.tiles]
+        psfs = [torch.from_numpy(p).cuda() for p in ts.psfs]
+        outs = [torch.empty_like(y) for y in ys]
+        b = RLBatch([tuple(y.shape) for y in ys], [tuple(p.shape) for p in psfs], ts.psf_index,
+                    RichardsonLucy(num_iter=ts.num_iter, clip=False))
+        fn = lambda: b.execute(ys, outs, psfs)  # noqa: E731
+    elif name == "grad":
+        from .autograd import RLGradPlan
+
+        ys, psf, _, T = synth.grad2d()
+        y = torch.from_numpy(ys).cuda()
+        p = torch.from_numpy(psf).cuda()
+        plan = RLGradPlan([tuple(y.shape[1:])] * y.shape[0], psf.shape, T, 5)
+        xs = list(torch.full_like(y, 0.5).unbind(0))
+        yl = list(y.unbind(0))
+        gx = [torch.empty_like(a) for a in xs]
+        gp = torch.empty_like(p)
+
+        def fn():
+            plan.forward(xs, yl, p)
+            plan.backward(None, gx, gp)
+    else:
+        raise ValueError(f"unknown workload {name!r}")
+    fn()
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    print(f"{name}: {reps} calls done", flush=True)
+
+
+def _rows(d: Path, counter: str):
+    files = sorted(Path(d).rglob("*counter_collection.csv"))
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {d}")
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] == counter:
+                yield r["Kernel_Name"], float(r["Counter_Value"])
+
+
+def _calibration(d: Path, counter: str) -> float:
+    vals = [v for name, v in _rows(d, counter) if "copyBuffer" in name and v > 0.5 * CAL_BYTES / 1024 / 4]
+    if not vals:
+        raise ValueError(f"no copy kernel of 1 GiB in {d}")
+    return CAL_BYTES / (statistics.median(vals) * 1024.0)  # bytes moved / bytes counted
+
+
+def _per_pass(d: Path, counter: str, workload: str):
+    groups: dict = {}
+    for name, v in _rows(d, counter):
+        m = _NAME.search(name)
+        if m:
+            groups.setdefault((m.group(1), int(m.group(4))), []).append(v)
+    out = {}
+    for pname, key in PASS_KERNELS[workload].items():
+        if key in groups:
+            out[pname] = (statistics.median(groups[key]), len(groups[key]))
+    return out
+
+
+def summarize(workload: str, fetch_dir, write_dir, fetch_cal, write_cal) -> dict:
+    kf = _calibration(Path(fetch_cal), "FETCH_SIZE")
+    kw = _calibration(Path(write_cal), "WRITE_SIZE")
+    f = _per_pass(Path(fetch_dir), "FETCH_SIZE", workload)
+    w = _per_pass(Path(write_dir), "WRITE_SIZE", workload)
+    kernels = {}
+    for p in PASS_KERNELS[workload]:
+        if p in f and p in w:
+            rb, wb = f[p][0] * 1024.0 * kf, w[p][0] * 1024.0 * kw
+            kernels[p] = {"read_bytes": round(rb), "write_bytes": round(wb), "hbm_bytes_per_launch": round(rb + wb),
+                          "dispatches": f[p][1]}
+    return {"workload": workload, "fetch_scale": round(kf, 4), "write_scale": round(kw, 4),
+            "scale_note": "bytes moved by a 1 GiB device copy / counter reading of that copy", "kernels": kernels}
+
+
+def main(argv) -> int:
+    if argv[:1] == ["copy"]:
+        copy_calibration()
+        return 0
+    if argv[:1] == ["run"] and len(argv) in (2, 3):
+        run_workload(argv[1], int(argv[2]) if len(argv) == 3 else 3)
+        return 0
+    if argv[:1] == ["summarize"] and len(argv) == 6:
+        res = summarize(*argv[1:])
+        dst = ROOT / "profiles" / f"traffic_{argv[1]}.json"
+        dst.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res, indent=1))
+        return 0
+    print(__doc__, file=sys.stderr)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
