@@ -53,3 +53,35 @@ def fit_gaussian_sigma(volumes, psf_shape, sigma0, num_iter: int = 10, steps: in
     res.ms_per_step = res.seconds * 1e3 / max(steps, 1)
     res.sigma = (param.exp() if log_sigma else param).detach().tolist()
     return res
+
+
+def main(argv=None) -> int:
+    """The PSF-fitting workload: 16 volumes of 32..128 x 256 x 256 (random spheres blurred by a
+    Gaussian of sigma (3.5, 1.4, 1.6) on 31x15x15, Poisson noise), sigma fitted from (2.5, 1.0,
+    1.0) by Adam (lr 0.05, betas (0.9, 0.999)) through 10 unrolled iterations, 100 steps. Prints
+    one JSON line with the trajectory."""
+    import argparse
+    import json
+
+    from . import synth
+
+    ap = argparse.ArgumentParser(description="fit a Gaussian PSF through the differentiable RL")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--lr", type=float, default=0.05)
+    ap.add_argument("--log-sigma", action="store_true")
+    args = ap.parse_args(argv)
+    vols, _ = synth.psffit3d()
+    vt = [torch.from_numpy(v).cuda() for v in vols]
+    true_sigma = [3.5, 1.4, 1.6]
+    res = fit_gaussian_sigma(vt, (31, 15, 15), (2.5, 1.0, 1.0), num_iter=10, steps=args.steps, lr=args.lr,
+                             log_sigma=args.log_sigma)
+    err = [abs(a - b) / b for a, b in zip(res.sigma, true_sigma)]
+    print(json.dumps({"volumes": [list(v.shape) for v in vols], "steps": args.steps, "lr": args.lr,
+                      "ms_per_step": round(res.ms_per_step, 3), "sigma_true": true_sigma,
+                      "sigma_final": [round(s, 5) for s in res.sigma], "rel_error": [round(e, 4) for e in err],
+                      "trajectory": [[h[0], h[1], [round(s, 5) for s in h[2]]] for h in res.history]}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

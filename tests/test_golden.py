@@ -1,0 +1,92 @@
+"""The committed golden fixtures (tests/golden/, written by make_golden.py from the FP64 oracle).
+
+CPU: the oracle still reproduces every fixture (FP64 to 1e-12, FP32 to 1e-6 relative L2), so a
+later change to the oracle cannot silently move the target. GPU: the HIP entry points reproduce
+the FP64 outputs within the parity bar of 1e-4 relative L2 (the FP32 oracle's own distance to
+FP64 is printed beside it), and the differentiable form reproduces the FP64 loss and gradients
+within 1e-3.
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+G = Path(__file__).resolve().parent / "golden"
+META = json.loads((G / "cases.json").read_text())
+RL = np.load(G / "rl_cases.npz")
+GR = np.load(G / "grad_case.npz")
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["rl"]])
+def test_oracle_reproduces_rl_fixture(oracle, case):
+    m = META["rl"][case]
+    y, p = RL[f"c{case}_image"], RL[f"c{case}_psf"]
+    nd = len(m["shape"]) == 3
+    kw = dict(clip=m["clip"], filter_epsilon=m["filter_epsilon"], mode=m["mode"], nd=nd)
+    st, o64 = oracle.richardson_lucy(y.astype(np.float64), p.astype(np.float64), m["num_iter"], **kw)
+    assert st == 0 and _rel(o64, RL[f"c{case}_out64"]) < 1e-12
+    st, o32 = oracle.richardson_lucy(y, p, m["num_iter"], **kw)
+    assert st == 0 and _rel(o32, RL[f"c{case}_out32"]) < 1e-6
+    assert _rel(RL[f"c{case}_out32"], RL[f"c{case}_out64"]) < 1e-5
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["grad"]])
+def test_oracle_reproduces_grad_fixture(oracle, case):
+    m = META["grad"][case]
+    loss, gx, gp = oracle.rl_nll_grad64(GR[f"g{case}_y"], GR[f"g{case}_x0"], GR[f"g{case}_psf"], m["num_iter"])
+    assert abs(loss - GR[f"g{case}_loss"][0]) <= 1e-12 * abs(loss)
+    assert _rel(gx, GR[f"g{case}_gx0"]) < 1e-12
+    assert _rel(gp, GR[f"g{case}_gpsf"]) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["rl"]])
+def test_hip_matches_rl_fixture(gpu, case):
+    import torch
+
+    import lucem_amd as LA
+
+    m = META["rl"][case]
+    y = torch.from_numpy(RL[f"c{case}_image"]).cuda()
+    p = torch.from_numpy(RL[f"c{case}_psf"]).cuda()
+    out = torch.empty_like(y)
+    s = LA.RichardsonLucy(num_iter=m["num_iter"], clip=m["clip"], mode=m["mode"], filter_epsilon=m["filter_epsilon"])
+    if len(m["shape"]) == 2:
+        LA.richardson_lucy_into(y, p, out, s)
+    else:
+        LA.richardson_lucy_nd_into(y, p, out, s)
+    got = out.cpu().numpy()
+    e64 = _rel(got, RL[f"c{case}_out64"])
+    e32 = _rel(got, RL[f"c{case}_out32"])
+    print(f"case {case} {m['shape']} mode {m['mode']}: HIP vs FP64 {e64:.2e}, vs FP32 oracle {e32:.2e}, "
+          f"FP32 oracle vs FP64 {_rel(RL[f'c{case}_out32'], RL[f'c{case}_out64']):.2e}")
+    assert e64 < 1e-4 and e32 < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["grad"]])
+def test_hip_matches_grad_fixture(gpu, case):
+    import torch
+
+    from lucem_amd.autograd import rl_nll
+
+    m = META["grad"][case]
+    x0 = torch.from_numpy(GR[f"g{case}_x0"]).cuda().requires_grad_(True)
+    y = torch.from_numpy(GR[f"g{case}_y"]).cuda()
+    p = torch.from_numpy(GR[f"g{case}_psf"]).cuda().requires_grad_(True)
+    loss = rl_nll([x0], [y], p, m["num_iter"], checkpoint_every=2)
+    loss.sum().backward()
+    L = float(loss[0])
+    assert abs(L - GR[f"g{case}_loss"][0]) <= 1e-6 * abs(L)
+    assert _rel(x0.grad.cpu().numpy(), GR[f"g{case}_gx0"]) < 1e-3
+    assert _rel(p.grad.cpu().numpy(), GR[f"g{case}_gpsf"]) < 1e-3
