@@ -218,26 +218,36 @@ struct Composite {
     }
 };
 
-// One in-place Stockham pass of radix R = R1*R2 over B = 1 << lgB interleaved lines of
-// length N. Input element j + t*N/R of butterfly j is multiplied by w^(t*k), k = j mod Ns,
-// w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns.
+// Exact division by a small positive divisor d through a multiply-high: q = u / d for every
+// 0 <= u < 2^17 and d <= 4096 (m = floor((2^32 - 1) / d) + 1, checked exhaustively on the host).
+struct Divisor {
+    int d;
+    unsigned m;
+    __device__ __forceinline__ static Divisor of(int d) { return Divisor{d, d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u}; }
+    __device__ __forceinline__ int div(int u) const { return d > 1 ? (int)__umulhi((unsigned)u, m) : u; }
+};
+
+// One in-place Stockham pass of radix R = R1*R2 over B interleaved lines of length N (B is any
+// count >= 1). Input element j + t*N/R of butterfly j is multiplied by w^(t*k), k = j mod Ns,
+// w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns. With the
+// butterfly index u = j*B + b, the inputs sit at u + t*(N/R)*B and the outputs at
+// u + (j - k)*(R - 1)*B + t*Ns*B.
 template <int R1, int R2, bool INV, int T, int E>
-__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, int lgB, int Ns,
+__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns,
                                               const float2* __restrict__ tw) {
     constexpr int R = R1 * R2;
     constexpr int S = (E + R - 1) / R;
-    const int B = 1 << lgB;
+    const int B = lines.d;
     const int NR = N / R;
-    const int nbf = NR << lgB;
+    const int nbf = NR * B;
     const int tws = N / (Ns * R);
     float2 v[S][R];
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const int u = threadIdx.x + s * T;
         if (u < nbf) {
-            const int b = u & (B - 1), j = u >> lgB;
 #pragma unroll
-            for (int t = 0; t < R; t++) v[s][t] = buf[((j + t * NR) << lgB) + b];
+            for (int t = 0; t < R; t++) v[s][t] = buf[u + t * nbf];
         }
     }
     __syncthreads();
@@ -245,7 +255,7 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, int 
     for (int s = 0; s < S; s++) {
         const int u = threadIdx.x + s * T;
         if (u < nbf) {
-            const int b = u & (B - 1), j = u >> lgB;
+            const int j = lines.div(u);
             const int k = j % Ns;
             if (Ns > 1) {
                 const int step = k * tws;
@@ -257,9 +267,10 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, int 
                 }
             }
             Composite<R1, R2, INV>::run(v[s]);
-            const int o = (j - k) * R + k;
+            const int o = u + (j - k) * (R - 1) * B;
+            const int stride = Ns * B;
 #pragma unroll
-            for (int t = 0; t < R; t++) buf[((o + t * Ns) << lgB) + b] = v[s][t];
+            for (int t = 0; t < R; t++) buf[o + t * stride] = v[s][t];
         }
     }
     __syncthreads();
@@ -270,7 +281,7 @@ constexpr unsigned char radix_code(int r1, int r2) { return (unsigned char)(r1 |
 
 // Full transform of the interleaved lines. The caller has written buf and synchronised.
 template <bool INV, int T, int E>
-__device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, const float2* __restrict__ twbase) {
+__device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ twbase) {
     const float2* tw = twbase + p.tw;
     const int N = p.L;
     int Ns = 1;
@@ -280,7 +291,7 @@ __device__ void lds_fft(float2* __restrict__ buf, int lgB, const FftPlan& p, con
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                        \
     case radix_code(r1, r2):                                                      \
-        stockham_pass<r1, r2, INV, T, E>(buf, N, lgB, Ns, tw);                    \
+        stockham_pass<r1, r2, INV, T, E>(buf, N, lines, Ns, tw);                  \
         Ns *= r1 * r2;                                                            \
         break;
             LUCEM_PASS(2, 1)

@@ -115,10 +115,10 @@ struct Carver {
 size_t pass_lds(bool row, const void* desc) {
     if (row) {
         const RowPass* r = static_cast<const RowPass*>(desc);
-        return row_pass_lds_bytes(r->f.L, r->lgP);
+        return row_pass_lds_bytes(r->f.L, r->P);
     }
     const ColPass* c = static_cast<const ColPass*>(desc);
-    return col_pass_lds_bytes(c->f.L, c->lgB);
+    return col_pass_lds_bytes(c->f.L, c->B);
 }
 
 }  // namespace
@@ -235,6 +235,30 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t bb) { return b->tiles[a].n > b->tiles[bb].n; });
 
+    // Launch shapes: every tile joins one launch per pass, so the parallelism of a launch is the
+    // whole batch's; the autotuner may replace the heuristic choices once the buffers exist.
+    // Work-item arrays are sized for the smallest shapes (one line per workgroup) so that any
+    // choice fits.
+    ShapeTable table;
+    ShapePolicy bsp{client->num_cus, 0, 0, nullptr};
+    const ShapePolicy osp{client->num_cus, 0, 0, nullptr};
+    size_t cap_row = 0, cap_col[3] = {0, 0, 0};
+    for (const TilePlan& tp : b->tiles) {
+        if (tp.group < 0) continue;
+        const Geometry& g = tp.g;
+        const int64_t pairs = ((int64_t)g.n[0] * g.n[1] + 1) / 2;
+        bsp.launch_pairs += pairs;
+        bsp.launch_lines += (int64_t)g.Lc * g.n[0];
+        cap_row += (size_t)pairs;
+        if (ncorr == 1) {
+            cap_col[0] += (size_t)g.Lc;
+        } else {
+            cap_col[0] += (size_t)g.Lc * g.n[0];
+            cap_col[1] += (size_t)g.Lc * g.L[1];
+            cap_col[2] += (size_t)g.Lc * g.n[0];
+        }
+    }
+
     // descriptors need the final pointers, so size first with a null carver
     auto layout = [&](Carver& cv) {
         P.est.assign(ntiles, nullptr);
@@ -276,8 +300,8 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         for (int i = 0; i < notf; i++) P.otf_cols[i] = cv.take<ColPass>(2 * ngroups);
         for (int k = 0; k < 2; k++)
             for (int i = 0; i < ncorr; i++) P.corr[k][i] = cv.take<ColPass>(ntiles);
-        P.items_row = cv.take<int2>(it_row.size());
-        for (int i = 0; i < ncorr; i++) P.items_col[i] = cv.take<int2>(it_col[i].size());
+        P.items_row = cv.take<int2>(cap_row);
+        for (int i = 0; i < ncorr; i++) P.items_col[i] = cv.take<int2>(cap_col[i]);
         P.items_otf_row = cv.take<int2>(it_otf_row.size());
         for (int i = 0; i < notf; i++) P.items_otf_col[i] = cv.take<int2>(it_otf_col[i].size());
         P.counters = cv.take<int>(4096);
@@ -292,22 +316,22 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     std::vector<PsfPrep> psfp(npsf);
     const float eps = rl_eps(*settings);
 
+
     auto build_descs = [&]() {
         for (size_t t = 0; t < ntiles; t++) {
             TilePlan& tp = b->tiles[t];
             if (tp.group < 0) continue;
             const Geometry& g = tp.g;
             const float2* tws[3] = {g.ndim == 3 ? tw[g.L[0]] : nullptr, tw[g.L[1]], tw[g.L[2]]};
-            rows[t] = make_row_desc(g, 0, tws[2], eps, nullptr, P.est[t], nullptr, P.sa[t],
-                                    &tp.row_blocks);
+            rows[t] = make_row_desc(g, bsp, tws[2], eps, nullptr, P.est[t], nullptr, P.sa[t], &tp.row_blocks);
             ColPass d[3];
             int blk[3];
-            make_corr_descs(g, 0, P.sa[t], P.sb[t], P.otf1[tp.group], tws, d, blk);
+            make_corr_descs(g, bsp, P.sa[t], P.sb[t], P.otf1[tp.group], tws, d, blk);
             for (int i = 0; i < ncorr; i++) {
                 corr[0][i][t] = d[i];
                 tp.col_blocks[i] = blk[i];
             }
-            make_corr_descs(g, 0, P.sa[t], P.sb[t], P.otf2[tp.group], tws, d, blk);
+            make_corr_descs(g, bsp, P.sa[t], P.sb[t], P.otf2[tp.group], tws, d, blk);
             for (int i = 0; i < ncorr; i++) corr[1][i][t] = d[i];
         }
         for (size_t gi = 0; gi < ngroups; gi++) {
@@ -319,7 +343,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
                 int rb;
                 ColPass cols[2];
                 int cb[2];
-                make_otf_descs(g, 0, k == 0 ? P.rev[gp.psf] : P.h[gp.psf], k == 0 ? P.pa1[gi] : P.pa2[gi],
+                make_otf_descs(g, osp, k == 0 ? P.rev[gp.psf] : P.h[gp.psf], k == 0 ? P.pa1[gi] : P.pa2[gi],
                                k == 0 ? P.pb1[gi] : P.pb2[gi], k == 0 ? P.otf1[gi] : P.otf2[gi], tws, &r, &rb, cols, cb);
                 otf_rows[2 * gi + k] = r;
                 for (int i = 0; i < notf; i++) otf_cols[i][2 * gi + k] = cols[i];
@@ -332,11 +356,12 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         }
     };
 
-    // work items (sizes only depend on the geometry; build them with a sizing pass of the descs)
-    {
-        Carver sizing;
-        layout(sizing);
-        build_descs();
+    // work items of the current descriptors, largest tiles first
+    auto build_items = [&]() {
+        it_row.clear();
+        for (int i = 0; i < 3; i++) it_col[i].clear();
+        it_otf_row.clear();
+        for (int i = 0; i < 2; i++) it_otf_col[i].clear();
         for (size_t oi : order) {
             const TilePlan& tp = b->tiles[oi];
             if (tp.group < 0) continue;
@@ -347,7 +372,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         for (size_t gi = 0; gi < ngroups; gi++)
             for (int k = 0; k < 2; k++) {
                 const RowPass& r = otf_rows[2 * gi + k];
-                const int rb = (((r.rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP;
+                const int rb = (((r.rows + 1) / 2) + r.P - 1) / r.P;
                 for (int q = 0; q < rb; q++) it_otf_row.push_back(make_int2((int)(2 * gi + k), q));
                 for (int i = 0; i < notf; i++) {
                     const ColPass& cp = otf_cols[i][2 * gi + k];
@@ -356,6 +381,13 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
                     for (int q = 0; q < outers * cp.nkb; q++) it_otf_col[i].push_back(make_int2((int)(2 * gi + k), q));
                 }
             }
+    };
+    // the transfer-function items do not depend on tuning: size them with a sizing pass
+    {
+        Carver sizing;
+        layout(sizing);
+        build_descs();
+        build_items();
     }
     Carver sizing;
     layout(sizing);
@@ -365,6 +397,22 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     real.base = static_cast<char*>(b->mem);
     layout(real);
     build_descs();
+    // autotune on the plan's own buffers, then rebuild the descriptors with the measured shapes
+    {
+        std::vector<RowPass> trows;
+        std::vector<TuneCol> tcols;
+        for (size_t t = 0; t < ntiles; t++) {
+            const TilePlan& tp = b->tiles[t];
+            if (tp.group < 0) continue;
+            trows.push_back(rows[t]);
+            for (int i = 0; i < ncorr; i++)
+                tcols.push_back(TuneCol{corr[0][i][t], tp.col_blocks[i] / std::max(corr[0][i][t].nkb, 1)});
+        }
+        if ((st = tune_shapes(client, trows, tcols, b->persistent, &table))) return st;
+        bsp.table = &table;
+        build_descs();
+    }
+    build_items();
     b->h_rows = rows;
     b->h_psf = psfp;
 

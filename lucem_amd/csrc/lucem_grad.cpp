@@ -300,13 +300,20 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
     };
 
     const float eps = rl_eps(*settings);
+    // launch shapes: every tile joins each launch, so a launch's parallelism is the whole batch's
+    ShapePolicy gsp{client->num_cus, 0, 0, nullptr};
+    const ShapePolicy osp{client->num_cus, 0, 0, nullptr};
+    for (int t = 0; t < ntiles; t++) {
+        gsp.launch_pairs += ((int64_t)P->geos[t].n[0] * P->geos[t].n[1] + 1) / 2;
+        gsp.launch_lines += (int64_t)P->geos[t].Lc * P->geos[t].n[0];
+    }
     auto build = [&]() {
         for (int t = 0; t < ntiles; t++) {
             const Geometry& g = P->geos[t];
             TileMem& m = tm[t];
             const float2* tws[3] = {three ? tw[g.L[0]] : nullptr, tw[g.L[1]], tw[g.L[2]]};
             GradTile G{};
-            G.r = make_row_desc(g, 0, tws[2], eps, nullptr, nullptr, nullptr, m.sa, &row_blocks[t]);
+            G.r = make_row_desc(g, gsp, tws[2], eps, nullptr, nullptr, nullptr, m.sa, &row_blocks[t]);
             G.kx = g.k[2];
             G.nstride = m.nstride;
             G.xslots = m.xs;
@@ -321,12 +328,12 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
             ColPass d[3];
             int blk[3];
             const int gi = group_of[t];
-            const int n1 = make_corr_descs(g, 0, m.sa, m.sb, otf1[gi], tws, d, blk);
+            const int n1 = make_corr_descs(g, gsp, m.sa, m.sb, otf1[gi], tws, d, blk);
             for (int i = 0; i < n1; i++) {
                 h_f1[i][t] = d[i];
                 h_f1s[i][t] = d[i];
             }
-            make_corr_descs(g, 0, m.sa, m.sb, otf2[gi], tws, d, blk);
+            make_corr_descs(g, gsp, m.sa, m.sb, otf2[gi], tws, d, blk);
             for (int i = 0; i < n1; i++) {
                 h_f2[i][t] = d[i];
                 h_f2s[i][t] = d[i];
@@ -340,7 +347,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
             for (int w = 0; w < 2; w++) {
                 const float2* otf = w == 0 ? otf1[gi] : otf2[gi];
                 ColPass a[3];
-                make_corr_descs(g, 0, m.sa, m.sb, otf, tws, a, blk);
+                make_corr_descs(g, gsp, m.sa, m.sb, otf, tws, a, blk);
                 const int ax_last = three ? 0 : 1;  // the fused axis
                 ColPass& fu = a[fi];
                 fu.op = COL_FUSED_T;
@@ -370,6 +377,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                 float2* ta = w == 0 ? m.ta_h : m.ta_r;
                 float2* tb = w == 0 ? m.tb_h : m.tb_r;
                 auto& gi_arr = w == 0 ? h_gi_h : h_gi_r;
+                int var = 0;
                 ColPass c{};
                 c.op = COL_INV;
                 c.Lc = g.Lc;
@@ -384,8 +392,9 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                     c.dst = tb;
                     c.dst_outer = g.pitch;
                     c.dst_axis = (int64_t)g.L[1] * g.pitch;
-                    c.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], 0, &c.wide);
-                    c.nkb = (g.Lc + (1 << c.lgB) - 1) >> c.lgB;
+                    c.B = choose_B(g.L[0], g.Lc, g.L[1], SHAPE_COL_INV, osp, &c.wide, &var);
+                    c.f = make_plan(g.L[0], var);
+                    c.nkb = (g.Lc + c.B - 1) / c.B;
                     gi_arr[0][t] = c;
                     ColPass y = c;
                     y.f = make_plan(g.L[1]);
@@ -398,8 +407,9 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                     y.dst = ta;
                     y.dst_outer = (int64_t)g.k[1] * g.pitch;
                     y.dst_axis = g.pitch;
-                    y.lgB = choose_lgB(g.L[1], g.Lc, g.k[0], 0, &y.wide);
-                    y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+                    y.B = choose_B(g.L[1], g.Lc, g.k[0], SHAPE_COL_INV, osp, &y.wide, &var);
+                    y.f = make_plan(g.L[1], var);
+                    y.nkb = (g.Lc + y.B - 1) / y.B;
                     gi_arr[1][t] = y;
                 } else {
                     c.f = make_plan(g.L[1]);
@@ -412,8 +422,9 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                     c.dst = ta;
                     c.dst_outer = 0;
                     c.dst_axis = g.pitch;
-                    c.lgB = choose_lgB(g.L[1], g.Lc, 1, 0, &c.wide);
-                    c.nkb = (g.Lc + (1 << c.lgB) - 1) >> c.lgB;
+                    c.B = choose_B(g.L[1], g.Lc, 1, SHAPE_COL_INV, osp, &c.wide, &var);
+                    c.f = make_plan(g.L[1], var);
+                    c.nkb = (g.Lc + c.B - 1) / c.B;
                     gi_arr[0][t] = c;
                 }
                 GradTile F{};
@@ -422,7 +433,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                 kg.n[1] = g.k[1];
                 kg.n[2] = g.k[2];
                 int fb = 0;
-                F.r = make_row_desc(kg, 0, tws[2], eps, nullptr, nullptr, nullptr, ta, &fb);
+                F.r = make_row_desc(kg, osp, tws[2], eps, nullptr, nullptr, nullptr, ta, &fb);
                 F.out = (w == 0 ? gh_all : gr_all) ? (w == 0 ? gh_all : gr_all) + (int64_t)t * P->nk : nullptr;
                 F.scale = (float)(0.25 / ((double)g.L[0] * (double)g.L[1] * (double)g.L[2]));
                 (w == 0 ? h_fin_h : h_fin_r)[t] = F;
@@ -436,7 +447,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                 int rb;
                 ColPass cols[2];
                 int cb[2];
-                make_otf_descs(g, 0, w == 0 ? rev : h, w == 0 ? pa1[gi] : pa2[gi], w == 0 ? pb1[gi] : pb2[gi],
+                make_otf_descs(g, osp, w == 0 ? rev : h, w == 0 ? pa1[gi] : pa2[gi], w == 0 ? pb1[gi] : pb2[gi],
                                w == 0 ? otf1[gi] : otf2[gi], tws, &r, &rb, cols, cb);
                 h_otf_rows[2 * gi + w] = r;
                 for (int i = 0; i < (three ? 2 : 1); i++) h_otf_cols[i][2 * gi + w] = cols[i];
@@ -467,14 +478,14 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                 for (int q = 0; q < outers * c.nkb; q++) h_it_gi[i].push_back(make_int2(t, q));
             }
             const int frows = g.k[0] * g.k[1];
-            const int fblocks = (((frows + 1) / 2) + (1 << h_fin_h[t].r.lgP) - 1) >> h_fin_h[t].r.lgP;
+            const int fblocks = (((frows + 1) / 2) + h_fin_h[t].r.P - 1) / h_fin_h[t].r.P;
             for (int q = 0; q < fblocks; q++) h_it_fin.push_back(make_int2(t, q));
         }
         for (int gi = 0; gi < ngroups; gi++) {
             const Geometry& g = P->geos[group_tile[gi]];
             for (int w = 0; w < 2; w++) {
                 const RowPass& r = h_otf_rows[2 * gi + w];
-                const int rb = (((r.rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP;
+                const int rb = (((r.rows + 1) / 2) + r.P - 1) / r.P;
                 for (int q = 0; q < rb; q++) h_it_otf_row.push_back(make_int2(2 * gi + w, q));
                 for (int i = 0; i < (three ? 2 : 1); i++) {
                     const ColPass& c = h_otf_cols[i][2 * gi + w];
@@ -533,7 +544,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
         size_t mx = 0;
         for (const auto& it : items) {
             const RowPass& r = get(descs[it.x]);
-            mx = std::max(mx, row_pass_lds_bytes(r.f.L, r.lgP));
+            mx = std::max(mx, row_pass_lds_bytes(r.f.L, r.P));
         }
         return mx;
     };
@@ -541,7 +552,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
         size_t mx = 0;
         int w = 0;
         for (const auto& it : items) {
-            mx = std::max(mx, col_pass_lds_bytes(descs[it.x].f.L, descs[it.x].lgB));
+            mx = std::max(mx, col_pass_lds_bytes(descs[it.x].f.L, descs[it.x].B));
             w |= descs[it.x].wide;
         }
         *wide = w;

@@ -336,9 +336,10 @@ int lucem::choose_fft_len(int target) {
     return L;
 }
 
-FftPlan lucem::make_plan(int L) {
-    // Prime factors, then greedy grouping into the largest supported composite radix:
-    // 16 = 4*4, 15 = 5*3, 14 = 7*2, 12 = 4*3, 10 = 5*2, 9 = 3*3, 8, 7, 6 = 3*2, 5, 4, 3, 2.
+FftPlan lucem::make_plan(int L, int variant) {
+    // Prime factors, then greedy grouping into the largest supported radix: 16 = 4*4, 15 = 5*3,
+    // 14 = 7*2, 12 = 4*3, 10 = 5*2, 9 = 3*3, 8, 7, 6 = 3*2, 5, 4, 3, 2 (variant 2: 8, 7, 5, 4,
+    // 3, 2 only). Variant 1 runs the passes of variant 0 in reverse order.
     FftPlan p{};
     p.L = L;
     p.npass = 0;
@@ -353,36 +354,49 @@ FftPlan lucem::make_plan(int L) {
     struct Code {
         int r1, r2, n2, n3, n5, n7;
     };
-    static const Code codes[] = {
+    static const Code composite[] = {
         {4, 4, 4, 0, 0, 0}, {5, 3, 0, 1, 1, 0}, {7, 2, 1, 0, 0, 1}, {4, 3, 2, 1, 0, 0}, {5, 2, 1, 0, 1, 0},
         {3, 3, 0, 2, 0, 0}, {8, 1, 3, 0, 0, 0}, {7, 1, 0, 0, 0, 1}, {3, 2, 1, 1, 0, 0}, {5, 1, 0, 0, 1, 0},
         {4, 1, 2, 0, 0, 0}, {3, 1, 0, 1, 0, 0}, {2, 1, 1, 0, 0, 0},
     };
+    static const Code single[] = {
+        {8, 1, 3, 0, 0, 0}, {7, 1, 0, 0, 0, 1}, {5, 1, 0, 0, 1, 0}, {4, 1, 2, 0, 0, 0}, {3, 1, 0, 1, 0, 0},
+        {2, 1, 1, 0, 0, 0},
+    };
+    const Code* codes = variant == 2 ? single : composite;
+    const int ncodes = variant == 2 ? (int)(sizeof single / sizeof single[0]) : (int)(sizeof composite / sizeof composite[0]);
+    unsigned char seq[32];
+    int n = 0;
     for (;;) {
         bool found = false;
-        for (const Code& c : codes) {
+        for (int i = 0; i < ncodes; i++) {
+            const Code& c = codes[i];
             if (cnt[2] >= c.n2 && cnt[3] >= c.n3 && cnt[5] >= c.n5 && cnt[7] >= c.n7) {
                 cnt[2] -= c.n2;
                 cnt[3] -= c.n3;
                 cnt[5] -= c.n5;
                 cnt[7] -= c.n7;
-                const unsigned long long code = (unsigned long long)(c.r1 | (c.r2 << 4));
-                if (p.npass < 8)
-                    p.codes_lo |= code << (8 * p.npass);
-                else
-                    p.codes_hi |= code << (8 * (p.npass - 8));
-                p.npass++;
+                seq[n++] = (unsigned char)(c.r1 | (c.r2 << 4));
                 found = true;
                 break;
             }
         }
-        if (!found) break;
+        if (!found || n == 16) break;
     }
+    if (variant == 1) std::reverse(seq, seq + n);
+    for (int q = 0; q < n; q++) {
+        const unsigned long long code = seq[q];
+        if (q < 8)
+            p.codes_lo |= code << (8 * q);
+        else
+            p.codes_hi |= code << (8 * (q - 8));
+    }
+    p.npass = n;
     return p;
 }
 
-// Launch shapes. LUCEM_TUNE (e.g. "row_lgP=2,col_lgB=3,col_wide=0") overrides the choices, for
-// measurement sweeps; the autotuner records its choices per shape in the same terms.
+// Launch shapes. LUCEM_TUNE (e.g. "row_P=4,col_B=8,col_wide=0") overrides the choices, for
+// measurement sweeps.
 static int tune_value(const char* key) {
     const char* env = std::getenv("LUCEM_TUNE");
     if (!env) return -1;
@@ -395,32 +409,50 @@ static int tune_value(const char* key) {
     return -1;
 }
 
-int lucem::choose_lgP(int L, int nx, int rows, int num_cus) {
-    (void)nx;
-    const int pairs = (rows + 1) / 2;
-    int lgP = 3;
-    while (lgP > 0 && ((size_t)L << lgP) > (size_t)kLdsComplex) lgP--;
-    // one array alone must still fill the device; a batch (num_cus == 0) brings its own parallelism
-    while (num_cus > 0 && lgP > 0 && (pairs >> lgP) < 4 * num_cus) lgP--;
-    const int t = tune_value("row_lgP");
-    if (t >= 0 && ((size_t)L << t) <= (size_t)kLdsComplex) lgP = t;
-    return lgP;
+static const ShapeChoice* find_choice(const ShapePolicy& sp, int kind, int L) {
+    if (!sp.table) return nullptr;
+    auto it = sp.table->find(std::make_pair(kind, L));
+    return it == sp.table->end() ? nullptr : &it->second;
 }
 
-int lucem::choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide) {
-    // the 512-thread shape when it doubles the columns of a workgroup below 16
-    int lg_small = 4, lg_big = 4;
-    while (lg_small > 0 && ((size_t)L << lg_small) > (size_t)kLdsComplex) lg_small--;
-    while (lg_big > 0 && ((size_t)L << lg_big) > (size_t)kLdsComplexWide) lg_big--;
+// Row pairs per workgroup: as many lines of length L as fill the workgroup's 4096 complex
+// values (any count, not only powers of two), at most 16, fewer when the launch would not give
+// every CU four workgroups.
+int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* variant) {
+    *variant = 0;
+    if (const ShapeChoice* c = find_choice(sp, SHAPE_ROW, L)) {
+        *variant = c->variant;
+        return c->lines;
+    }
+    const int64_t pairs = std::max<int64_t>((rows + 1) / 2, sp.launch_pairs);
+    int P = std::max(1, std::min(16, kLdsComplex / L));
+    while (sp.num_cus > 0 && P > 1 && (pairs + P - 1) / P < 4 * sp.num_cus) P--;
+    const int t = tune_value("row_P");
+    if (t >= 1 && (size_t)L * t <= (size_t)kLdsComplex) P = t;
+    return P;
+}
+
+// Columns per workgroup of a line pass, any count up to 32. The 512-thread shape serves lines
+// too long for 8 columns in 4096 values.
+int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& sp, int* wide, int* variant) {
+    *variant = 0;
+    if (const ShapeChoice* c = find_choice(sp, kind, L)) {
+        *variant = c->variant;
+        *wide = c->wide;
+        return c->lines;
+    }
+    const int b_small = std::max(1, std::min(32, kLdsComplex / L));
+    const int b_big = std::max(1, std::min(32, kLdsComplexWide / L));
     int w = tune_value("col_wide");
-    if (w < 0) w = lg_big > lg_small ? 1 : 0;
+    if (w < 0) w = (b_small < 8 && b_big > b_small) ? 1 : 0;
     const size_t cap = w ? kLdsComplexWide : kLdsComplex;
-    int lgB = w ? lg_big : lg_small;
-    while (num_cus > 0 && lgB > 2 && outers * ((Lc + (1 << lgB) - 1) >> lgB) < 4 * num_cus) lgB--;
-    const int t = tune_value("col_lgB");
-    if (t >= 0 && ((size_t)L << t) <= cap) lgB = t;
+    int B = w ? b_big : b_small;
+    const int64_t lines = std::max<int64_t>(outers * Lc, sp.launch_lines);
+    while (sp.num_cus > 0 && B > 4 && lines / B < 4 * sp.num_cus) B--;
+    const int t = tune_value("col_B");
+    if (t >= 1 && (size_t)L * t <= cap) B = t;
     *wide = w;
-    return lgB;
+    return B;
 }
 
 bool lucem::make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g) {
@@ -449,7 +481,7 @@ static lucem_status launch_check(int err, const char* what) {
 
 // Descriptors of the two-step transfer-function build: a real kernel (dims m, at the origin,
 // zero-padded to L) -> row R2C -> line FFTs -> conj(v) * scale stored as [L_z][L_y][pitch].
-int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2* pa, float2* pb, float2* otf,
+int lucem::make_otf_descs(const Geometry& g, const ShapePolicy& sp, const float* w, float2* pa, float2* pb, float2* otf,
                           const float2* const tws[3], RowPass* row, int* row_blocks, ColPass cols[2],
                           int col_blocks[2]) {
     const int m[3] = {g.k[0], g.k[1], g.k[2]};
@@ -464,13 +496,14 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
     r.mode = LUCEM_PAD_CONSTANT;
     r.Lc = g.Lc;
     r.pitch = g.pitch;
-    r.lgP = choose_lgP(g.L[2], m[2], r.rows, num_cus);
-    r.f = make_plan(g.L[2]);
+    int var = 0;
+    r.P = choose_P(g.L[2], r.rows, sp, &var);
+    r.f = make_plan(g.L[2], var);
     r.src = w;
     r.spec = pa;
     r.tw = tws[2];
     *row = r;
-    *row_blocks = (((r.rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP;
+    *row_blocks = (((r.rows + 1) / 2) + r.P - 1) / r.P;
 
     ColPass y{};
     y.n_src = m[1];
@@ -478,7 +511,6 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
     y.mode = LUCEM_PAD_CONSTANT;
     y.n_dst = g.L[1];
     y.Lc = g.Lc;
-    y.f = make_plan(g.L[1]);
     y.tw = tws[1];
     y.src = pa;
     y.src_outer = (int64_t)m[1] * g.pitch;
@@ -489,8 +521,9 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
         y.dst = otf;
         y.dst_outer = 0;
         y.dst_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, num_cus, &y.wide);
-        y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+        y.B = choose_B(g.L[1], g.Lc, 1, SHAPE_COL_FWD, sp, &y.wide, &var);
+        y.f = make_plan(g.L[1], var);
+        y.nkb = (g.Lc + y.B - 1) / y.B;
         cols[0] = y;
         col_blocks[0] = y.nkb;
         return 1;
@@ -499,8 +532,9 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
     y.dst = pb;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, m[0], num_cus, &y.wide);
-    y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+    y.B = choose_B(g.L[1], g.Lc, m[0], SHAPE_COL_FWD, sp, &y.wide, &var);
+    y.f = make_plan(g.L[1], var);
+    y.nkb = (g.Lc + y.B - 1) / y.B;
     cols[0] = y;
     col_blocks[0] = m[0] * y.nkb;
 
@@ -511,7 +545,6 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
     z.mode = LUCEM_PAD_CONSTANT;
     z.n_dst = g.L[0];
     z.Lc = g.Lc;
-    z.f = make_plan(g.L[0]);
     z.tw = tws[0];
     z.src = pb;
     z.src_outer = g.pitch;
@@ -520,8 +553,9 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
     z.dst_outer = g.pitch;
     z.dst_axis = (int64_t)g.L[1] * g.pitch;
     z.scale = scale;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], num_cus, &z.wide);
-    z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
+    z.B = choose_B(g.L[0], g.Lc, g.L[1], SHAPE_COL_FWD, sp, &z.wide, &var);
+    z.f = make_plan(g.L[0], var);
+    z.nkb = (g.Lc + z.B - 1) / z.B;
     cols[1] = z;
     col_blocks[1] = g.L[1] * z.nkb;
     return 2;
@@ -529,11 +563,11 @@ int lucem::make_otf_descs(const Geometry& g, int num_cus, const float* w, float2
 
 // Line-pass descriptors of one correlation over spec_a (and spec_b in 3-D) with transfer `otf`:
 // 2-D: y fused (in place); 3-D: y forward (a -> b), z fused (in place on b), y inverse (b -> a).
-int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* sb, const float2* otf,
+int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb, const float2* otf,
                            const float2* const tws[3], ColPass out[3], int blocks[3]) {
+    int var = 0;
     ColPass y{};
     y.Lc = g.Lc;
-    y.f = make_plan(g.L[1]);
     y.tw = tws[1];
     y.left = g.k[1] / 2;
     y.mode = g.mode;
@@ -548,8 +582,9 @@ int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* s
         y.otf = otf;
         y.otf_outer = 0;
         y.otf_axis = g.pitch;
-        y.lgB = choose_lgB(g.L[1], g.Lc, 1, num_cus, &y.wide);
-        y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+        y.B = choose_B(g.L[1], g.Lc, 1, SHAPE_COL_FUSED, sp, &y.wide, &var);
+        y.f = make_plan(g.L[1], var);
+        y.nkb = (g.Lc + y.B - 1) / y.B;
         out[0] = y;
         blocks[0] = y.nkb;
         return 1;
@@ -563,15 +598,15 @@ int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* s
     y.dst = sb;
     y.dst_outer = (int64_t)g.L[1] * g.pitch;
     y.dst_axis = g.pitch;
-    y.lgB = choose_lgB(g.L[1], g.Lc, g.n[0], num_cus, &y.wide);
-    y.nkb = (g.Lc + (1 << y.lgB) - 1) >> y.lgB;
+    y.B = choose_B(g.L[1], g.Lc, g.n[0], SHAPE_COL_FWD, sp, &y.wide, &var);
+    y.f = make_plan(g.L[1], var);
+    y.nkb = (g.Lc + y.B - 1) / y.B;
     out[0] = y;
     blocks[0] = g.n[0] * y.nkb;
 
     ColPass z{};
     z.op = COL_FUSED;
     z.Lc = g.Lc;
-    z.f = make_plan(g.L[0]);
     z.tw = tws[0];
     z.left = g.k[0] / 2;
     z.mode = g.mode;
@@ -584,13 +619,17 @@ int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* s
     z.otf = otf;
     z.otf_outer = g.pitch;
     z.otf_axis = (int64_t)g.L[1] * g.pitch;
-    z.lgB = choose_lgB(g.L[0], g.Lc, g.L[1], num_cus, &z.wide);
-    z.nkb = (g.Lc + (1 << z.lgB) - 1) >> z.lgB;
+    z.B = choose_B(g.L[0], g.Lc, g.L[1], SHAPE_COL_FUSED, sp, &z.wide, &var);
+    z.f = make_plan(g.L[0], var);
+    z.nkb = (g.Lc + z.B - 1) / z.B;
     out[1] = z;
     blocks[1] = g.L[1] * z.nkb;
 
     ColPass yi = y;
     yi.op = COL_INV;
+    yi.B = choose_B(g.L[1], g.Lc, g.n[0], SHAPE_COL_INV, sp, &yi.wide, &var);
+    yi.f = make_plan(g.L[1], var);
+    yi.nkb = (g.Lc + yi.B - 1) / yi.B;
     yi.n_src = g.L[1];
     yi.n_dst = g.n[1];
     yi.src = sb;
@@ -602,7 +641,7 @@ int lucem::make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* s
     return 3;
 }
 
-RowPass lucem::make_row_desc(const Geometry& g, int num_cus, const float2* tw_x, float eps, const float* obs,
+RowPass lucem::make_row_desc(const Geometry& g, const ShapePolicy& sp, const float2* tw_x, float eps, const float* obs,
                              float* est, float* out, float2* spec, int* blocks) {
     const int64_t rows = (int64_t)g.n[0] * g.n[1];
     RowPass r{};
@@ -612,27 +651,28 @@ RowPass lucem::make_row_desc(const Geometry& g, int num_cus, const float2* tw_x,
     r.mode = g.mode;
     r.Lc = g.Lc;
     r.pitch = g.pitch;
-    r.lgP = choose_lgP(g.L[2], g.n[2], (int)rows, num_cus);
+    int var = 0;
+    r.P = choose_P(g.L[2], (int)rows, sp, &var);
     r.eps = eps;
     r.init = 0.5f;
-    r.f = make_plan(g.L[2]);
+    r.f = make_plan(g.L[2], var);
     r.obs = obs;
     r.est = est;
     r.out = out;
     r.spec = spec;
     r.tw = tw_x;
-    *blocks = (int)((((rows + 1) / 2) + (1 << r.lgP) - 1) >> r.lgP);
+    *blocks = (int)((((rows + 1) / 2) + r.P - 1) / r.P);
     return r;
 }
 
 static const char* const kCorrNames2[1] = {"line_fused_y"};
 static const char* const kCorrNames3[3] = {"line_fwd_y", "line_fused_z", "line_inv_y"};
 
-static lucem_status run_corr(lucem_client* c, const Geometry& g, float2* sa, float2* sb, const float2* otf,
-                             const float2* const tws[3]) {
+static lucem_status run_corr(lucem_client* c, const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb,
+                             const float2* otf, const float2* const tws[3]) {
     ColPass d[3];
     int b[3];
-    const int n = make_corr_descs(g, c->num_cus, sa, sb, otf, tws, d, b);
+    const int n = make_corr_descs(g, sp, sa, sb, otf, tws, d, b);
     for (int i = 0; i < n; i++) {
         lucem_status st = run_col(c, d[i], b[i], g.ndim == 2 ? kCorrNames2[i] : kCorrNames3[i]);
         if (st) return st;
@@ -646,7 +686,8 @@ static lucem_status run_otf(lucem_client* c, const Geometry& g, const float* w, 
     int rb;
     ColPass cols[2];
     int cb[2];
-    const int n = make_otf_descs(g, c->num_cus, w, pa, pb, otf, tws, &r, &rb, cols, cb);
+    const ShapePolicy sp{c->num_cus, 0, 0, nullptr};
+    const int n = make_otf_descs(g, sp, w, pa, pb, otf, tws, &r, &rb, cols, cb);
     lucem_status st = run_row(c, r, rb, "otf_row_load");
     if (st) return st;
     for (int i = 0; i < n; i++)
@@ -695,17 +736,19 @@ lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* in
     if ((st = run_otf(c, g, h, pa, pb, otf2, tws))) return st;
 
     int row_blocks = 0;
-    RowPass r = make_row_desc(g, c->num_cus, tws[2], rl_eps(s), input, est, output, sa, &row_blocks);
+    ShapePolicy sp{c->num_cus, 0, 0, nullptr};
+    if ((st = tuned_shapes(c, g, est, sa, sb, otf1, tws, &sp.table))) return st;
+    RowPass r = make_row_desc(g, sp, tws[2], rl_eps(s), input, est, output, sa, &row_blocks);
 
     // est_0 = 0.5 everywhere (deconvolve.rs:93): transform the constant directly.
     r.op = ROW_LOAD_CONST;
     if ((st = run_row(c, r, row_blocks, "row_start"))) return st;
     for (uint64_t it = 0; it < s.num_iter; it++) {
         const bool last = it + 1 == s.num_iter;
-        if ((st = run_corr(c, g, sa, sb, otf1, tws))) return st;
+        if ((st = run_corr(c, g, sp, sa, sb, otf1, tws))) return st;
         r.op = ROW_RATIO;
         if ((st = run_row(c, r, row_blocks, "row_ratio"))) return st;
-        if ((st = run_corr(c, g, sa, sb, otf2, tws))) return st;
+        if ((st = run_corr(c, g, sp, sa, sb, otf2, tws))) return st;
         r.op = ROW_UPDATE;
         r.first = it == 0;
         r.last = last;
@@ -844,7 +887,7 @@ extern "C" lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const floa
     p.mode = LUCEM_PAD_WRAP;
     p.n_dst = (int)L;
     p.Lc = 1;
-    p.lgB = 0;
+    p.B = 1;
     p.nkb = 1;
     p.src_outer = L;
     p.src_axis = 1;

@@ -21,6 +21,16 @@ struct ProfRecord {
     hipEvent_t start, stop;
     double bytes;
 };
+
+// Launch shape of one pass kind at one FFT length: lines per workgroup, the 512-thread line
+// kernel, radix-sequence variant.
+struct ShapeChoice {
+    int lines = 0;
+    int wide = 0;
+    int variant = 0;
+};
+enum ShapeKind : int { SHAPE_ROW = 0, SHAPE_COL_FWD = 1, SHAPE_COL_FUSED = 2, SHAPE_COL_INV = 3 };
+using ShapeTable = std::map<std::pair<int, int>, ShapeChoice>;  // (kind, L) -> choice
 }  // namespace lucem
 
 struct lucem_client {
@@ -33,6 +43,7 @@ struct lucem_client {
     bool profiling = false;
     std::vector<lucem::ProfRecord> records;        // launches since profiling was enabled
     std::vector<hipEvent_t> event_pool;            // events of read records, reused
+    std::map<std::vector<int>, lucem::ShapeTable> tuned;  // plain path: tuned shapes per geometry
     ~lucem_client();
 };
 
@@ -69,9 +80,21 @@ lucem_status twiddles_for(lucem_client* c, int L, const float2** out);
 lucem_status ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out);
 
 int choose_fft_len(int target);
-FftPlan make_plan(int L);
-int choose_lgP(int L, int nx, int rows, int num_cus);
-int choose_lgB(int L, int Lc, int64_t outers, int num_cus, int* wide);
+// Radix sequence of an FFT of length L. Variant 0: greedy, largest composite radix first;
+// 1: the same radices in reverse order; 2: single radices only (8, 7, 5, 4, 3, 2).
+FftPlan make_plan(int L, int variant = 0);
+
+// How the descriptor builders pick launch shapes: the device size, the parallelism of the launch
+// a descriptor joins (row pairs and lines of all arrays launched together; 0 = the array's own)
+// and measured choices that override the heuristics.
+struct ShapePolicy {
+    int num_cus = 0;
+    int64_t launch_pairs = 0;
+    int64_t launch_lines = 0;
+    const ShapeTable* table = nullptr;
+};
+int choose_P(int L, int rows, const ShapePolicy& sp, int* variant);
+int choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& sp, int* wide, int* variant);
 bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g);
 
 // Launch with optional profiling (events around the launch, algorithmic bytes recorded).
@@ -81,13 +104,24 @@ double row_pass_bytes(const RowPass& r);
 double col_pass_bytes(const ColPass& p, int blocks);
 
 // Descriptor builders shared by the plain and batched drivers.
-int make_otf_descs(const Geometry& g, int num_cus, const float* w, float2* pa, float2* pb, float2* otf,
+int make_otf_descs(const Geometry& g, const ShapePolicy& sp, const float* w, float2* pa, float2* pb, float2* otf,
                    const float2* const tws[3], RowPass* row, int* row_blocks, ColPass cols[2], int col_blocks[2]);
-int make_corr_descs(const Geometry& g, int num_cus, float2* sa, float2* sb, const float2* otf,
+int make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb, const float2* otf,
                     const float2* const tws[3], ColPass out[3], int blocks[3]);
-RowPass make_row_desc(const Geometry& g, int num_cus, const float2* tw_x, float eps, const float* obs, float* est,
+RowPass make_row_desc(const Geometry& g, const ShapePolicy& sp, const float2* tw_x, float eps, const float* obs, float* est,
                       float* out, float2* spec, int* blocks);
 float rl_eps(const lucem_rl_settings& s);
+
+// Autotuner: times candidate shapes of these descriptors' passes (batch work-queue kernels when
+// `persistent`, else the plain kernels) and records the fastest per (kind, L) in `out`.
+struct TuneCol {
+    ColPass d;
+    int outers;  // outer lines of the pass (work items = outers * column blocks)
+};
+lucem_status tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, const std::vector<TuneCol>& cols,
+                         bool persistent, ShapeTable* out);
+lucem_status tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb, const float2* otf,
+                          const float2* const tws[3], const ShapeTable** out);
 
 lucem_status validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
                          const uint64_t* psf_dims, uint32_t psf_ndim, const float* output, const uint64_t* out_dims,

@@ -1,0 +1,356 @@
+// Autotuner of the FFT launch shapes: for each pass kind (row pass, y/z line passes) and FFT
+// length it times candidate shapes on the device and keeps the fastest:
+//   - lines per workgroup (row pairs P, or columns B of a line pass), near the largest count that
+//     fills the workgroup;
+//   - the 256- or 512-thread line kernel;
+//   - the radix sequence (greedy composite radices, the same in reverse order, or single radices).
+// Candidates run on the caller's own plan buffers (plan-owned workspace: the tuning launches
+// leave garbage that the captured sequence overwrites before it reads), each one warm-up launch
+// then the best of three timed launches (HIP events on the client's stream). Results are kept
+// per process, keyed by (kind, L, op, launch size, schedule), so later plans of the same shapes
+// skip the measurement. LUCEM_AUTOTUNE=0 turns tuning off (the heuristics of choose_P/choose_B).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lucem_hip.h"
+#include "lucem_internal_host.hpp"
+#include "rl_internal.hpp"
+
+using namespace lucem;
+
+namespace {
+
+// Candidate line counts: the largest few that fill a workgroup, the heuristic's choice and its
+// neighbours, and a few small counts (launches of few lines want more workgroups).
+std::vector<int> line_candidates(int max_lines, int heuristic) {
+    std::vector<int> v;
+    for (int x : {max_lines, max_lines - 1, max_lines - 2, heuristic, heuristic - 1, heuristic + 1, 1, 2, 4, 8})
+        if (x >= 1 && x <= max_lines && std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+    return v;
+}
+
+using CacheKey = std::tuple<int, int, int, int, int>;  // kind, L, op, log2(lines), persistent
+std::mutex g_cache_mu;
+std::map<CacheKey, ShapeChoice> g_cache;
+
+int shape_kind(const ColPass& c) {
+    if (c.op == COL_FUSED) return SHAPE_COL_FUSED;
+    if (c.op == COL_INV) return SHAPE_COL_INV;
+    return SHAPE_COL_FWD;
+}
+
+int ilog2(int64_t v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        r++;
+    }
+    return r;
+}
+
+bool same_plan(const FftPlan& a, const FftPlan& b) {
+    return a.npass == b.npass && a.codes_lo == b.codes_lo && a.codes_hi == b.codes_hi;
+}
+
+// Device scratch of one tuning session: descriptors, work items and a queue counter.
+struct Scratch {
+    char* p = nullptr;
+    size_t bytes = 0;
+    ~Scratch() {
+        if (p) (void)hipFree(p);
+    }
+    lucem_status reserve(size_t b) {
+        if (b <= bytes) return LUCEM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(autotune scratch)");
+        bytes = b;
+        return LUCEM_OK;
+    }
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+bool tune_log() {
+    const char* v = std::getenv("LUCEM_TUNE_LOG");
+    return v && std::string(v) == "1";
+}
+
+bool autotune_enabled() {
+    const char* v = std::getenv("LUCEM_AUTOTUNE");
+    return !(v && std::string(v) == "0");
+}
+
+// Times fn() (which enqueues one launch on `s`): one warm-up run, then the best of three.
+template <class F>
+lucem_status time_launch(hipStream_t s, Timer& t, F fn, float* ms) {
+    hipError_t e;
+    if (!t.a && (e = hipEventCreate(&t.a)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if (!t.b && (e = hipEventCreate(&t.b)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    int err = fn();
+    if (err) return hip_fail(err, "autotune launch");
+    float best = 1e30f;
+    for (int r = 0; r < 3; r++) {
+        if ((e = hipEventRecord(t.a, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((err = fn())) return hip_fail(err, "autotune launch");
+        if ((e = hipEventRecord(t.b, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = hipEventSynchronize(t.b)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        float x = 0.f;
+        if ((e = hipEventElapsedTime(&x, t.a, t.b)) != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+        best = std::min(best, x);
+    }
+    *ms = best;
+    return LUCEM_OK;
+}
+
+}  // namespace
+
+// Tunes every (kind, L) group of these descriptors that carries at least 3% of its kind's work.
+// persistent: time the work-queue batch kernels (as the batched form runs them); otherwise the
+// plain one-block-per-item kernels.
+lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, const std::vector<TuneCol>& cols,
+                                bool persistent, ShapeTable* out) {
+    if (!autotune_enabled()) return LUCEM_OK;
+    hipStream_t s = c->stream;
+    Scratch scratch;
+    Timer timer;
+
+    // ---- row passes, grouped by L
+    std::map<int, std::vector<RowPass>> rgroups;
+    double rtotal = 0.0;
+    for (const RowPass& r : rows) {
+        rgroups[r.f.L].push_back(r);
+        rtotal += (double)r.rows * r.f.L;
+    }
+    for (auto& kv : rgroups) {
+        const int L = kv.first;
+        std::vector<RowPass>& g = kv.second;
+        double work = 0.0;
+        int64_t pairs = 0;
+        for (const RowPass& r : g) {
+            work += (double)r.rows * L;
+            pairs += (r.rows + 1) / 2;
+        }
+        if (work < 0.03 * rtotal || work < 2.0e5) continue;
+        const CacheKey key{SHAPE_ROW, L, ROW_UPDATE, ilog2(pairs), persistent ? 1 : 0};
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            auto it = g_cache.find(key);
+            if (it != g_cache.end()) {
+                (*out)[{SHAPE_ROW, L}] = it->second;
+                continue;
+            }
+        }
+        const int pmax = std::max(1, std::min(16, kLdsComplex / L));
+        ShapeChoice best{};
+        float best_ms = 1e30f;
+        std::vector<FftPlan> plans;
+        for (int v = 0; v < 3; v++) {
+            FftPlan f = make_plan(L, v);
+            bool dup = false;
+            for (const FftPlan& q : plans) dup = dup || same_plan(q, f);
+            if (dup) continue;
+            plans.push_back(f);
+            for (int P : line_candidates(pmax, g[0].P)) {
+                std::vector<RowPass> d = g;
+                std::vector<int2> items;
+                for (size_t i = 0; i < d.size(); i++) {
+                    d[i].P = P;
+                    d[i].f = f;
+                    d[i].op = ROW_UPDATE;
+                    d[i].first = 1;
+                    d[i].last = 0;
+                    d[i].clip = 0;
+                    const int nb = ((d[i].rows + 1) / 2 + P - 1) / P;
+                    for (int q = 0; q < nb; q++) items.push_back(make_int2((int)i, q));
+                }
+                const size_t lds = row_pass_lds_bytes(L, P);
+                float ms = 0.f;
+                lucem_status st;
+                if (persistent) {
+                    const size_t bd = sizeof(RowPass) * d.size(), bi = sizeof(int2) * items.size();
+                    if ((st = scratch.reserve(bd + bi + 1024))) return st;
+                    RowPass* dd = reinterpret_cast<RowPass*>(scratch.p);
+                    int2* di = reinterpret_cast<int2*>(scratch.p + ((bd + 255) & ~size_t(255)));
+                    int* cnt = reinterpret_cast<int*>(scratch.p + ((bd + 255) & ~size_t(255)) + ((bi + 255) & ~size_t(255)));
+                    hipError_t e;
+                    if ((e = hipMemcpy(dd, d.data(), bd, hipMemcpyHostToDevice)) != hipSuccess)
+                        return hip_fail(e, "hipMemcpy(autotune)");
+                    if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
+                        return hip_fail(e, "hipMemcpy(autotune)");
+                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, false, lds) * c->num_cus));
+                    RowBatch rb{dd, di, (int)items.size(), ROW_UPDATE, 1, 0, 0, 0, cnt};
+                    st = time_launch(s, timer, [&]() {
+                        int err = launch_zero_ints(cnt, 1, s);
+                        return err ? err : launch_row_batch(rb, grid, lds, s);
+                    }, &ms);
+                } else {
+                    const int nb = (int)items.size();
+                    st = time_launch(s, timer, [&]() { return launch_row_pass(d[0], nb, s); }, &ms);
+                }
+                if (st) return st;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = ShapeChoice{P, 0, v};
+                }
+            }
+        }
+        (*out)[{SHAPE_ROW, L}] = best;
+        if (tune_log())
+            std::fprintf(stderr, "lucem autotune: row L=%d pairs=%lld -> P=%d variant=%d (%.4f ms)\n", L,
+                         (long long)pairs, best.lines, best.variant, best_ms);
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache[key] = best;
+    }
+
+    // ---- line passes, grouped by (kind, L)
+    std::map<std::pair<int, int>, std::vector<TuneCol>> cgroups;
+    std::map<int, double> ctotal;
+    for (const TuneCol& tc : cols) {
+        const int k = shape_kind(tc.d);
+        cgroups[{k, tc.d.f.L}].push_back(tc);
+        ctotal[k] += (double)tc.outers * tc.d.Lc * tc.d.f.L;
+    }
+    // one line launch runs one thread count: the widest group decides it, the others follow
+    std::map<int, int> kind_wide;
+    std::vector<std::pair<double, std::pair<int, int>>> order;
+    for (auto& kv : cgroups) {
+        double work = 0.0;
+        for (const TuneCol& tc : kv.second) work += (double)tc.outers * tc.d.Lc * tc.d.f.L;
+        order.push_back({-work, kv.first});
+    }
+    std::sort(order.begin(), order.end());
+    for (const auto& o : order) {
+        const int kind = o.second.first, L = o.second.second;
+        const double work = -o.first;
+        std::vector<TuneCol>& g = cgroups[o.second];
+        if (work < 0.03 * ctotal[kind] || work < 2.0e5) continue;
+        int64_t lines = 0;
+        for (const TuneCol& tc : g) lines += (int64_t)tc.outers * tc.d.Lc;
+        const bool fixed_wide = kind_wide.count(kind) > 0;
+        const int wide_key = fixed_wide ? kind_wide[kind] : 2;
+        const CacheKey key{kind * 4 + wide_key, L, g[0].d.op, ilog2(lines), persistent ? 1 : 0};
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            auto it = g_cache.find(key);
+            if (it != g_cache.end()) {
+                (*out)[{kind, L}] = it->second;
+                if (!fixed_wide) kind_wide[kind] = it->second.wide;
+                continue;
+            }
+        }
+        ShapeChoice best{};
+        float best_ms = 1e30f;
+        std::vector<FftPlan> plans;
+        for (int v = 0; v < 3; v++) {
+            FftPlan f = make_plan(L, v);
+            bool dup = false;
+            for (const FftPlan& q : plans) dup = dup || same_plan(q, f);
+            if (dup) continue;
+            plans.push_back(f);
+            for (int w = 0; w < 2; w++) {
+                if (fixed_wide && w != kind_wide[kind]) continue;
+                const int cap = w ? kLdsComplexWide : kLdsComplex;
+                const int bmax = std::max(1, std::min(32, cap / L));
+                if (w == 1 && !fixed_wide && bmax <= std::max(1, std::min(32, kLdsComplex / L))) continue;
+                for (int B : line_candidates(bmax, g[0].d.wide == w ? g[0].d.B : bmax)) {
+                    std::vector<ColPass> d;
+                    std::vector<int2> items;
+                    for (size_t i = 0; i < g.size(); i++) {
+                        ColPass cp = g[i].d;
+                        cp.B = B;
+                        cp.wide = w;
+                        cp.f = f;
+                        cp.nkb = (cp.Lc + B - 1) / B;
+                        for (int q = 0; q < g[i].outers * cp.nkb; q++) items.push_back(make_int2((int)i, q));
+                        d.push_back(cp);
+                    }
+                    const size_t lds = col_pass_lds_bytes(L, B);
+                    float ms = 0.f;
+                    lucem_status st;
+                    if (persistent) {
+                        const size_t bd = sizeof(ColPass) * d.size(), bi = sizeof(int2) * items.size();
+                        if ((st = scratch.reserve(bd + bi + 1024))) return st;
+                        ColPass* dd = reinterpret_cast<ColPass*>(scratch.p);
+                        int2* di = reinterpret_cast<int2*>(scratch.p + ((bd + 255) & ~size_t(255)));
+                        int* cnt =
+                            reinterpret_cast<int*>(scratch.p + ((bd + 255) & ~size_t(255)) + ((bi + 255) & ~size_t(255)));
+                        hipError_t e;
+                        if ((e = hipMemcpy(dd, d.data(), bd, hipMemcpyHostToDevice)) != hipSuccess)
+                            return hip_fail(e, "hipMemcpy(autotune)");
+                        if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
+                            return hip_fail(e, "hipMemcpy(autotune)");
+                        const int grid =
+                            std::max(1, std::min((int)items.size(), batch_occupancy(false, w != 0, lds) * c->num_cus));
+                        ColBatch cb{dd, di, (int)items.size(), d[0].op, w, cnt};
+                        st = time_launch(s, timer, [&]() {
+                            int err = launch_zero_ints(cnt, 1, s);
+                            return err ? err : launch_col_batch(cb, grid, lds, s);
+                        }, &ms);
+                    } else {
+                        const int nb = (int)items.size();
+                        st = time_launch(s, timer, [&]() { return launch_col_pass(d[0], nb, s); }, &ms);
+                    }
+                    if (st) return st;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = ShapeChoice{B, w, v};
+                    }
+                }
+            }
+        }
+        (*out)[{kind, L}] = best;
+        if (!fixed_wide) kind_wide[kind] = best.wide;
+        if (tune_log())
+            std::fprintf(stderr, "lucem autotune: line kind=%d L=%d lines=%lld -> B=%d wide=%d variant=%d (%.4f ms)\n",
+                         kind, L, (long long)lines, best.lines, best.wide, best.variant, best_ms);
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache[key] = best;
+    }
+    return LUCEM_OK;
+}
+
+// The plain path: the tuned shapes of this geometry, measured on the first call of the client
+// for it (on the workspace buffers) and reused by later calls. Arrays too small to matter keep
+// the heuristics (an empty table).
+lucem_status lucem::tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb,
+                                 const float2* otf, const float2* const tws[3], const ShapeTable** out) {
+    *out = nullptr;
+    if (!autotune_enabled()) return LUCEM_OK;
+    const std::vector<int> key{g.ndim, g.n[0], g.n[1], g.n[2], g.k[0], g.k[1], g.k[2]};
+    auto it = c->tuned.find(key);
+    if (it == c->tuned.end()) {
+        const ShapePolicy sp{c->num_cus, 0, 0, nullptr};
+        int rb = 0;
+        const RowPass r = make_row_desc(g, sp, tws[2], 1.0f, nullptr, est, nullptr, sa, &rb);
+        ColPass d[3];
+        int b[3];
+        const int n = make_corr_descs(g, sp, sa, sb, otf, tws, d, b);
+        std::vector<TuneCol> cols;
+        for (int i = 0; i < n; i++) cols.push_back(TuneCol{d[i], b[i] / std::max(d[i].nkb, 1)});
+        ShapeTable t;
+        lucem_status st = tune_shapes(c, std::vector<RowPass>{r}, cols, false, &t);
+        if (st) return st;
+        it = c->tuned.emplace(key, t).first;
+    }
+    *out = &it->second;
+    return LUCEM_OK;
+}

@@ -32,16 +32,15 @@ struct GradOpTraits {
 };
 
 // Folds the halo columns of P interleaved row-pair lines back onto [0, nx) (pad transpose).
-__device__ __forceinline__ void fold_rows(float2* Z, int lgP, int left, int nx, int mode, int end) {
-    const int P = 1 << lgP;
+__device__ __forceinline__ void fold_rows(float2* Z, int P, int left, int nx, int mode, int end) {
     const int b = threadIdx.x;
     if (b < P) {
         for (int h = 0; h < end; h++) {
             if (h >= left && h < left + nx) continue;
             const int t = pad_index(h - left, nx, mode);
             if (t < 0) continue;
-            const float2 v = Z[(h << lgP) + b];
-            float2& dst = Z[((t + left) << lgP) + b];
+            const float2 v = Z[h * P + b];
+            float2& dst = Z[(t + left) * P + b];
             dst = make_float2(dst.x + v.x, dst.y + v.y);
         }
     }
@@ -53,19 +52,19 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
     using TR = GradOpTraits<OP>;
     const RowPass& d = G.r;
     constexpr int S = E;
-    const int lgP = d.lgP;
-    const int P = 1 << lgP;
+    const int P = d.P;
+    const Divisor pd = Divisor::of(P);
     const int Lf = d.f.L;
     const int nx = d.nx;
     const int tid = threadIdx.x;
     const int pair0 = g * P;
-    const int total = Lf << lgP;
+    const int total = Lf * P;
     const int64_t ns = G.nstride;
 
     if constexpr (TR::c2r) {
         constexpr int KS = (E + 1) / 2 + 1;
         const int Lc = d.Lc;
-        const int nload = Lc << lgP;
+        const int nload = Lc * P;
         const float inv_lc = 1.0f / (float)Lc;
         float2 av[KS], bv[KS];
         int kk[KS], bb[KS];
@@ -96,15 +95,15 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
                     A.y = 0.f;
                     Bv.y = 0.f;
                 }
-                Z[(k << lgP) + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+                Z[k * P + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
                 const int km = Lf - k;
-                if (k > 0 && km >= Lc) Z[(km << lgP) + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+                if (k > 0 && km >= Lc) Z[km * P + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, lgP, d.f, d.tw);
+        lds_fft<true, T, E>(Z, pd, d.f, d.tw);
         if constexpr (TR::fold) {
-            fold_rows(Z, lgP, d.left, nx, d.mode, min(Lf, G.kx + nx - 1));
+            fold_rows(Z, P, d.left, nx, d.mode, min(Lf, G.kx + nx - 1));
             __syncthreads();
         }
     }
@@ -122,13 +121,13 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
         srcs[s] = -1;
         val[s] = make_float2(0.f, 0.f);
         if (e < total) {
-            const int b = e & (P - 1), i = e >> lgP;
+            const int i = pd.div(e), b = e - i * P;
             const int src = pad_index(i - gleft, nx, gmode);
             srcs[s] = src;
             if (src >= 0) {
                 const bool own = i == src + gleft;
                 float2 c = make_float2(0.f, 0.f);
-                if constexpr (TR::c2r) c = Z[((src + coff) << lgP) + b];
+                if constexpr (TR::c2r) c = Z[(src + coff) * P + b];
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const int row = 2 * (pair0 + b) + h;
@@ -212,18 +211,17 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
         if (e < total) Z[e] = val[s];
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, lgP, d.f, d.tw);
-    for (int b = 0; b < P; b++) {
+    lds_fft<false, T, E>(Z, pd, d.f, d.tw);
+    // item e = b * Lc + k over all pairs, so that every thread has work whatever P and Lc are
+    const Divisor lcd = Divisor::of(d.Lc);
+    const int nsplit = P * d.Lc;
+    for (int e = tid; e < nsplit; e += T) {
+        const int b = lcd.div(e), k = e - b * d.Lc;
         const int ra = 2 * (pair0 + b), rb = ra + 1;
-        const bool va = ra < d.rows, vb = rb < d.rows;
-        float2* Ar = d.spec + (int64_t)ra * d.pitch;
-        float2* Br = d.spec + (int64_t)rb * d.pitch;
-        for (int k = tid; k < d.Lc; k += T) {
-            const float2 zk = Z[(k << lgP) + b];
-            const float2 zm = Z[((k == 0 ? 0 : Lf - k) << lgP) + b];
-            if (va) Ar[k] = make_float2(zk.x + zm.x, zk.y - zm.y);
-            if (vb) Br[k] = make_float2(zk.y + zm.y, zm.x - zk.x);
-        }
+        const float2 zk = Z[k * P + b];
+        const float2 zm = Z[(k == 0 ? 0 : Lf - k) * P + b];
+        if (ra < d.rows) d.spec[(int64_t)ra * d.pitch + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
+        if (rb < d.rows) d.spec[(int64_t)rb * d.pitch + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
     }
     (void)item_index;
 }

@@ -59,7 +59,7 @@ struct RowPass {
     int nx, rows;
     int left, mode;      // pad gather along x
     int Lc, pitch;       // half-spectrum bins per row, row pitch (complex) of `spec`
-    int lgP;             // log2(row pairs per workgroup)
+    int P;               // row pairs per workgroup (any count, P * L <= threads * 16)
     int wide;            // reserved (0)
     int first;           // ROW_UPDATE: the previous estimate is the constant `init`
     int last;            // ROW_UPDATE: write `out`, no forward transform
@@ -82,7 +82,7 @@ struct ColPass {
     int left, mode;       // gather along the axis (FWD/FUSED)
     int n_dst;            // rows stored along the axis
     int Lc;               // valid columns
-    int lgB;              // log2(columns per workgroup)
+    int B;                // columns per workgroup (any count, B * L <= threads * 16)
     int nkb;              // column blocks per outer index
     int wide;             // 1: the 512-thread line kernel
     float scale;          // COL_FWD_OTF
@@ -145,7 +145,7 @@ int launch_psf_batch(const PsfBatch& b, void* stream);
 int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
 int batch_occupancy(bool row, bool wide, size_t lds);
-size_t row_pass_lds_bytes(int L, int lgP);
-size_t col_pass_lds_bytes(int L, int lgB);
+size_t row_pass_lds_bytes(int L, int P);
+size_t col_pass_lds_bytes(int L, int B);
 
 }  // namespace lucem

@@ -30,20 +30,20 @@ struct RowFlags {
 template <int T, int E, int OP>
 __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2* Z) {
     constexpr int S = E;  // elements per thread in the gather (L * P <= T * E)
-    const int lgP = d.lgP;
-    const int P = 1 << lgP;
+    const int P = d.P;
+    const Divisor pd = Divisor::of(P);
     const int L = d.f.L;
     const int nx = d.nx;
     const int tid = threadIdx.x;
     const int pair0 = g * P;
-    const int total = L << lgP;
+    const int total = L * P;
 
     if constexpr (OP == ROW_RATIO || OP == ROW_UPDATE) {
         // Z[k] = A[k] + i B[k] for k < Lc, and the Hermitian mirror for k >= Lc. All loads are
         // issued before the first LDS write (item e = b * Lc + k, P * Lc <= T * KS).
         constexpr int KS = (E + 1) / 2 + 1;
         const int Lc = d.Lc;
-        const int nload = Lc << lgP;
+        const int nload = Lc * P;
         const float inv_lc = 1.0f / (float)Lc;
         float2 av[KS], bv[KS];
         int kk[KS], bb[KS];
@@ -74,13 +74,13 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                     A.y = 0.f;
                     Bv.y = 0.f;
                 }
-                Z[(k << lgP) + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+                Z[k * P + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
                 const int km = L - k;
-                if (k > 0 && km >= Lc) Z[(km << lgP) + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+                if (k > 0 && km >= Lc) Z[km * P + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, lgP, d.f, d.tw);
+        lds_fft<true, T, E>(Z, pd, d.f, d.tw);
         // Gather into registers: padded position i reads the cropped result at src = pad(i - left)
         // and applies the pointwise step there. Every value is read before any is overwritten.
         float2 val[S];
@@ -92,7 +92,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             srcs[s] = -1;
             val[s] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int b = e & (P - 1), i = e >> lgP;
+                const int i = pd.div(e), b = e - i * P;
                 const int src = pad_index(i - d.left, nx, d.mode);
                 srcs[s] = src;
                 if (src >= 0) {
@@ -117,8 +117,8 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             const int e = tid + s * T;
             const int src = srcs[s];
             if (e < total && src >= 0) {
-                const int b = e & (P - 1);
-                const float2 c = Z[(src << lgP) + b];
+                const int b = e - pd.div(e) * P;
+                const float2 c = Z[src * P + b];
                 if constexpr (OP == ROW_RATIO) {
                     const float da = fabsf(c.x) < d.eps ? d.eps : c.x;
                     const float db = fabsf(c.y) < d.eps ? d.eps : c.y;
@@ -139,9 +139,9 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             const int e = tid + s * T;
             if (e < total) {
                 if constexpr (OP == ROW_UPDATE) {
-                    const int i = e >> lgP;
+                    const int i = pd.div(e);
                     if (srcs[s] >= 0 && i == srcs[s] + d.left) {
-                        const int b = e & (P - 1);
+                        const int b = e - pd.div(e) * P;
                         const int x = i - d.left;
                         const int ra = 2 * (pair0 + b), rb = ra + 1;
                         float* dst = fl.last ? d.out : d.est;
@@ -163,7 +163,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             const int e = tid + s * T;
             z[s] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int b = e & (P - 1), i = e >> lgP;
+                const int i = pd.div(e), b = e - i * P;
                 const int ra = 2 * (pair0 + b), rb = ra + 1;
                 const bool va = ra < d.rows, vb = rb < d.rows;
                 const int src = pad_index(i - d.left, nx, d.mode);
@@ -184,20 +184,19 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
         }
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, lgP, d.f, d.tw);
+    lds_fft<false, T, E>(Z, pd, d.f, d.tw);
     // split the two real transforms: A = Z[k] + conj(Z[L-k]), B = -i (Z[k] - conj(Z[L-k]))
     // (both carry a factor 2; the transfer function is scaled to remove it)
-    for (int b = 0; b < P; b++) {
+    // item e = b * Lc + k over all pairs, so that every thread has work whatever P and Lc are
+    const Divisor lcd = Divisor::of(d.Lc);
+    const int nsplit = P * d.Lc;
+    for (int e = tid; e < nsplit; e += T) {
+        const int b = lcd.div(e), k = e - b * d.Lc;
         const int ra = 2 * (pair0 + b), rb = ra + 1;
-        const bool va = ra < d.rows, vb = rb < d.rows;
-        float2* Ar = d.spec + (int64_t)ra * d.pitch;
-        float2* Br = d.spec + (int64_t)rb * d.pitch;
-        for (int k = tid; k < d.Lc; k += T) {
-            const float2 zk = Z[(k << lgP) + b];
-            const float2 zm = Z[((k == 0 ? 0 : L - k) << lgP) + b];
-            if (va) Ar[k] = make_float2(zk.x + zm.x, zk.y - zm.y);
-            if (vb) Br[k] = make_float2(zk.y + zm.y, zm.x - zk.x);
-        }
+        const float2 zk = Z[k * P + b];
+        const float2 zm = Z[(k == 0 ? 0 : L - k) * P + b];
+        if (ra < d.rows) d.spec[(int64_t)ra * d.pitch + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
+        if (rb < d.rows) d.spec[(int64_t)rb * d.pitch + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
     }
 }
 
@@ -213,17 +212,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row
 // Folds the padded positions of B interleaved lines back onto [0, n) in place (the transpose of
 // the pad gather Q[i] = x[pad(i - left)]): every halo position h < left or h >= left + n (below
 // n + k - 1 <= L) is added to position left + pad(h - left). One thread per line, halo in order.
-__device__ __forceinline__ void fold_lines(float2* Z, int lgB, int nlines_valid, int L, int left, int n, int mode,
+__device__ __forceinline__ void fold_lines(float2* Z, int B, int nlines_valid, int L, int left, int n, int mode,
                                            int nhalo_end) {
-    const int B = 1 << lgB;
     const int b = threadIdx.x;
     if (b < B && b < nlines_valid) {
         for (int h = 0; h < nhalo_end; h++) {
             if (h >= left && h < left + n) continue;
             const int t = pad_index(h - left, n, mode);
             if (t < 0) continue;
-            const float2 v = Z[(h << lgB) + b];
-            float2& dst = Z[((t + left) << lgB) + b];
+            const float2 v = Z[h * B + b];
+            float2& dst = Z[(t + left) * B + b];
             dst = make_float2(dst.x + v.x, dst.y + v.y);
         }
     }
@@ -232,14 +230,14 @@ __device__ __forceinline__ void fold_lines(float2* Z, int lgB, int nlines_valid,
 
 template <int T, int E, int OP>
 __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) {
-    const int lgB = d.lgB;
-    const int B = 1 << lgB;
+    const int B = d.B;
+    const Divisor bd = Divisor::of(B);
     const int outer = g / d.nkb;
     const int kx0 = (g - outer * d.nkb) * B;
     const int L = d.f.L;
     const int tid = threadIdx.x;
     const float2* src = d.src + outer * d.src_outer + kx0;
-    const int total = L << lgB;
+    const int total = L * B;
     const bool full = kx0 + B <= d.Lc;
     constexpr bool INV_IN = OP == COL_INV || OP == COL_INV_FOLD;
     {
@@ -249,7 +247,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
             const int e = tid + s * T;
             v[s] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int b = e & (B - 1), i = e >> lgB;
+                const int i = bd.div(e), b = e - i * B;
                 const int sr = INV_IN ? i : pad_index(i - d.left, d.n_src, d.mode);
                 if (sr >= 0 && (full || kx0 + b < d.Lc)) v[s] = src[sr * d.src_axis + b];
             }
@@ -262,9 +260,9 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
     }
     __syncthreads();
     if constexpr (INV_IN) {
-        lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
+        lds_fft<true, T, E>(Z, bd, d.f, d.tw);
     } else {
-        lds_fft<false, T, E>(Z, lgB, d.f, d.tw);
+        lds_fft<false, T, E>(Z, bd, d.f, d.tw);
         if constexpr (OP == COL_FUSED || OP == COL_FUSED_T) {
             const int64_t off = outer * d.otf_outer + kx0;
             const float2* otf = d.otf + off;
@@ -274,7 +272,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                 const int e = tid + s * T;
                 w[s] = make_float2(0.f, 0.f);
                 if (e < total) {
-                    const int b = e & (B - 1), i = e >> lgB;
+                    const int i = bd.div(e), b = e - i * B;
                     if (full || kx0 + b < d.Lc) w[s] = otf[i * d.otf_axis + b];
                 }
             }
@@ -285,7 +283,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                     for (int s = 0; s < E; s++) {
                         const int e = tid + s * T;
                         if (e < total) {
-                            const int b = e & (B - 1), i = e >> lgB;
+                            const int i = bd.div(e), b = e - i * B;
                             if (full || kx0 + b < d.Lc) sv[i * d.otf_axis + b] = Z[e];
                         }
                     }
@@ -298,7 +296,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                     for (int s = 0; s < E; s++) {
                         const int e = tid + s * T;
                         if (e < total) {
-                            const int b = e & (B - 1), i = e >> lgB;
+                            const int i = bd.div(e), b = e - i * B;
                             if (full || kx0 + b < d.Lc) {
                                 const float2 a = sv[i * d.otf_axis + b], z = Z[e];
                                 float2 acc = ga[i * d.otf_axis + b];
@@ -322,21 +320,21 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                 }
             }
             __syncthreads();
-            lds_fft<true, T, E>(Z, lgB, d.f, d.tw);
+            lds_fft<true, T, E>(Z, bd, d.f, d.tw);
         }
     }
     int row0 = 0;
     if constexpr (OP == COL_FUSED_T || OP == COL_INV_FOLD) {
-        fold_lines(Z, lgB, d.Lc - kx0, L, d.fold_left, d.fold_n, d.fold_mode, min(L, d.fold_end));
+        fold_lines(Z, B, d.Lc - kx0, L, d.fold_left, d.fold_n, d.fold_mode, min(L, d.fold_end));
         __syncthreads();
         row0 = d.fold_left;
     }
     float2* dst = d.dst + outer * d.dst_outer + kx0;
-    const int nstore = d.n_dst << lgB;
+    const int nstore = d.n_dst * B;
     for (int e = tid; e < nstore; e += T) {
-        const int b = e & (B - 1), i = e >> lgB;
+        const int i = bd.div(e), b = e - i * B;
         if (full || kx0 + b < d.Lc) {
-            float2 v = Z[e + (row0 << lgB)];
+            float2 v = Z[e + row0 * B];
             if constexpr (OP == COL_FWD_OTF) v = make_float2(v.x * d.scale, -v.y * d.scale);
             dst[i * d.dst_axis + b] = v;
         }
@@ -444,13 +442,13 @@ __global__ void zero_ints_kernel(int* p, int n) {
 // 256 threads (32 KiB of LDS at most, four workgroups per CU); line passes use 256 threads or,
 // for long lines, 512 threads (64 KiB, two workgroups per CU) to double the columns per workgroup.
 // ---------------------------------------------------------------------------------------------
-size_t row_pass_lds_bytes(int L, int lgP) { return ((size_t)L << lgP) * sizeof(float2); }
+size_t row_pass_lds_bytes(int L, int P) { return (size_t)L * P * sizeof(float2); }
 
-size_t col_pass_lds_bytes(int L, int lgB) { return ((size_t)L << lgB) * sizeof(float2); }
+size_t col_pass_lds_bytes(int L, int B) { return (size_t)L * B * sizeof(float2); }
 
 template <int E>
 static int launch_row_e(const RowPass& p, int blocks, void* stream) {
-    const size_t lds = row_pass_lds_bytes(p.f.L, p.lgP);
+    const size_t lds = row_pass_lds_bytes(p.f.L, p.P);
     hipStream_t st = (hipStream_t)stream;
     switch (p.op) {
         case ROW_LOAD: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_LOAD>), dim3(blocks), dim3(kThreads), lds, st, p); break;
@@ -465,7 +463,7 @@ int launch_row_pass(const RowPass& p, int blocks, void* stream) { return launch_
 
 template <int T, int E>
 static int launch_col_e(const ColPass& p, int blocks, void* stream) {
-    const size_t lds = col_pass_lds_bytes(p.f.L, p.lgB);
+    const size_t lds = col_pass_lds_bytes(p.f.L, p.B);
     hipStream_t st = (hipStream_t)stream;
     switch (p.op) {
         case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD>), dim3(blocks), dim3(T), lds, st, p); break;
