@@ -432,8 +432,9 @@ int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* variant) {
     return P;
 }
 
-// Columns per workgroup of a line pass, any count up to 32. The 512-thread shape serves lines
-// too long for 8 columns in 4096 values.
+// Columns per workgroup of a line pass, any count up to 32, fewer only when the launch would
+// not give every CU one workgroup (wide workgroups beat more of them in the measurements). The
+// 512-thread shape serves lines too long for 8 columns in 4096 values.
 int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& sp, int* wide, int* variant) {
     *variant = 0;
     if (const ShapeChoice* c = find_choice(sp, kind, L)) {
@@ -448,7 +449,7 @@ int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& 
     const size_t cap = w ? kLdsComplexWide : kLdsComplex;
     int B = w ? b_big : b_small;
     const int64_t lines = std::max<int64_t>(outers * Lc, sp.launch_lines);
-    while (sp.num_cus > 0 && B > 4 && lines / B < 4 * sp.num_cus) B--;
+    while (sp.num_cus > 0 && B > 4 && lines / B < sp.num_cus) B--;
     const int t = tune_value("col_B");
     if (t >= 1 && (size_t)L * t <= cap) B = t;
     *wide = w;

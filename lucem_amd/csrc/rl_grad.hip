@@ -35,8 +35,9 @@ struct GradOpTraits {
 __device__ __forceinline__ void fold_rows(float2* Z, int P, int left, int nx, int mode, int end) {
     const int b = threadIdx.x;
     if (b < P) {
-        for (int h = 0; h < end; h++) {
-            if (h >= left && h < left + nx) continue;
+        // the halo only: [0, left), then [left + nx, end), in increasing order
+        const int right = left + nx;
+        for (int h = left > 0 ? 0 : right; h < end; h = h + 1 == left ? right : h + 1) {
             const int t = pad_index(h - left, nx, mode);
             if (t < 0) continue;
             const float2 v = Z[h * P + b];

@@ -216,8 +216,9 @@ __device__ __forceinline__ void fold_lines(float2* Z, int B, int nlines_valid, i
                                            int nhalo_end) {
     const int b = threadIdx.x;
     if (b < B && b < nlines_valid) {
-        for (int h = 0; h < nhalo_end; h++) {
-            if (h >= left && h < left + n) continue;
+        // the halo only: [0, left), then [left + n, nhalo_end), in increasing order
+        const int right = left + n;
+        for (int h = left > 0 ? 0 : right; h < nhalo_end; h = h + 1 == left ? right : h + 1) {
             const int t = pad_index(h - left, n, mode);
             if (t < 0) continue;
             const float2 v = Z[h * B + b];
