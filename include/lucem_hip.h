@@ -66,7 +66,7 @@ enum {
 };
 
 /* Largest padded FFT length per axis handled in one LDS-resident pass. */
-#define LUCEM_MAX_FFT_LEN 4096
+#define LUCEM_MAX_FFT_LEN 16384
 
 /* RichardsonLucy settings (deconvolve.rs:19-41). Defaults: num_iter 30, clip 1,
  * has_filter_epsilon 0 (=> 1e-12), mode LUCEM_PAD_REFLECT. The epsilon used is

@@ -8,9 +8,11 @@ library or device raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import threading
+import weakref
 from pathlib import Path
 
 _PKG = Path(__file__).resolve().parent
@@ -142,6 +144,34 @@ def last_error() -> str:
     return lib().lucem_last_error().decode(errors="replace")
 
 
+_owned = {"plans": weakref.WeakSet(), "workspaces": weakref.WeakSet(), "clients": weakref.WeakSet()}
+_exit_hook = False
+
+
+def track(obj, kind: str) -> None:
+    """Registers a library object (kind "plans", "workspaces" or "clients") for release at exit.
+
+    The exit hook releases plans first and clients last, while the HIP runtime and torch are
+    still whole: it is registered after torch has been imported, so it runs before torch's own
+    exit handlers (atexit runs last-in, first-out), instead of leaving the order to the garbage
+    collection of interpreter shutdown.
+    """
+    global _exit_hook
+    _owned[kind].add(obj)
+    if not _exit_hook:
+        atexit.register(_release_all)
+        _exit_hook = True
+
+
+def _release_all() -> None:
+    for kind in ("plans", "workspaces", "clients"):
+        for obj in list(_owned[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class Client:
     """A device and a HIP stream (mirrors lucem_core::compute::Client).
 
@@ -160,6 +190,7 @@ class Client:
         check(lib().lucem_client_create(int(device), _vp(stream or None), 1 if own_stream else 0, ctypes.byref(h)))
         self._h = h
         self.device = int(device)
+        track(self, "clients")
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -212,6 +243,7 @@ class Workspace:
         check(lib().lucem_workspace_create(client.handle, ctypes.byref(h)))
         self._h = h
         self._client = client
+        track(self, "workspaces")
 
     @property
     def handle(self) -> ctypes.c_void_p:

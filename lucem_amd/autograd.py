@@ -62,6 +62,7 @@ class RLGradPlan:
         N.check(L.lucem_rl_grad_create(self.client.handle, dims, self.ndim, len(self.shapes), pd, ctypes.byref(s),
                                        int(checkpoint_every), ctypes.byref(h)))
         self._h = h
+        N.track(self, "plans")
         self.generation = 0  # bumped by every forward; a backward needs its own forward resident
 
     @property

@@ -129,6 +129,7 @@ class RLBatch:
         N.check(N.lib().lucem_rl_batch_create(self.client.handle, arr, len(self.shapes), self._psf_dims,
                                               self.psf_ndim, len(self.psf_shapes), ctypes.byref(st), ctypes.byref(h)))
         self._h = h
+        N.track(self, "plans")
         del fake
 
     @property

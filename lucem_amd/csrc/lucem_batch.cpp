@@ -468,21 +468,28 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     // (a tile planned for 256 threads runs correctly on 512: its items use fewer columns)
     auto any_wide = [&](const std::vector<ColPass>& descs, const std::vector<int2>& items) {
         int w = 0;
-        for (const auto& it : items) w |= descs[it.x].wide;
+        for (const auto& it : items) w = std::max(w, descs[it.x].wide);
         return w;
     };
-    add(true, ROW_LOAD, P.otf_rows, P.items_otf_row, (int)it_otf_row.size(), max_lds(true, otf_rows, it_otf_row), 0);
+    auto row_tier = [&](const std::vector<RowPass>& descs, const std::vector<int2>& items) {
+        int w = 0;
+        for (const auto& it : items) w = std::max(w, descs[it.x].wide);
+        return w;
+    };
+    add(true, ROW_LOAD, P.otf_rows, P.items_otf_row, (int)it_otf_row.size(), max_lds(true, otf_rows, it_otf_row),
+        row_tier(otf_rows, it_otf_row));
     for (int i = 0; i < notf; i++)
         add(false, i + 1 == notf ? COL_FWD_OTF : COL_FWD, P.otf_cols[i], P.items_otf_col[i],
             (int)it_otf_col[i].size(), max_lds(false, otf_cols[i], it_otf_col[i]), any_wide(otf_cols[i], it_otf_col[i]));
     const size_t lds_row = max_lds(true, rows, it_row);
+    const int tier_row = row_tier(rows, it_row);
     size_t lds_col[3];
     int wide_col[3];
     for (int i = 0; i < ncorr; i++) {
         lds_col[i] = max_lds(false, corr[0][i], it_col[i]);
         wide_col[i] = any_wide(corr[0][i], it_col[i]);
     }
-    add(true, ROW_LOAD_CONST, P.rows, P.items_row, (int)it_row.size(), lds_row, 0);
+    add(true, ROW_LOAD_CONST, P.rows, P.items_row, (int)it_row.size(), lds_row, tier_row);
     const int corr_ops2[1] = {COL_FUSED};
     const int corr_ops3[3] = {COL_FWD, COL_FUSED, COL_INV};
     for (uint64_t it = 0; it < settings->num_iter; it++) {
@@ -492,9 +499,9 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
                 add(false, ncorr == 1 ? corr_ops2[i] : corr_ops3[i], P.corr[k][i], P.items_col[i],
                     (int)it_col[i].size(), lds_col[i], wide_col[i]);
             if (k == 0)
-                add(true, ROW_RATIO, P.rows, P.items_row, (int)it_row.size(), lds_row, 0);
+                add(true, ROW_RATIO, P.rows, P.items_row, (int)it_row.size(), lds_row, tier_row);
             else
-                add(true, ROW_UPDATE, P.rows, P.items_row, (int)it_row.size(), lds_row, 0, it == 0, last,
+                add(true, ROW_UPDATE, P.rows, P.items_row, (int)it_row.size(), lds_row, tier_row, it == 0, last,
                     settings->clip ? 1 : 0);
         }
     }
@@ -510,7 +517,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     for (size_t i = 0; i < seq.size() && !err; i++) {
         const Launch& L = seq[i];
         if (L.row) {
-            RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, L.first, L.last, L.clip, 0,
+            RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, L.first, L.last, L.clip, L.wide,
                         b->persistent ? P.counters + i : nullptr};
             err = launch_row_batch(rb, L.grid, L.lds, b->capture);
         } else {

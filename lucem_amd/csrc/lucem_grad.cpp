@@ -142,6 +142,11 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                                       tile_dims + t * ndim, ndim, settings, 3, &empty, &P->geos[t]);
         if (st) return fail(st, "tile " + std::to_string(t) + ": " + lucem_last_error());
         if (empty) return fail(LUCEM_INVALID_PARAMETER, "tile " + std::to_string(t) + " is empty");
+        for (int a = 0; a < 3; a++)
+            if (P->geos[t].L[a] > kLdsComplex)
+                return fail(LUCEM_UNSUPPORTED_SHAPE, "tile " + std::to_string(t) +
+                                                         ": the differentiable form handles FFT lengths up to 4096 "
+                                                         "(n + k - 1 per axis)");
     }
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     hipError_t e = hipSetDevice(client->device);

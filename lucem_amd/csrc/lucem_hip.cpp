@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -101,6 +102,10 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
     c->num_cus = prop.multiProcessorCount;
+    static std::once_flag lds_once;
+    static int lds_err = 0;
+    std::call_once(lds_once, [] { lds_err = allow_big_lds_kernels(); });
+    if (lds_err) return hip_fail(lds_err, "hipFuncSetAttribute(dynamic LDS)");
     if (!own_stream) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -418,8 +423,11 @@ static const ShapeChoice* find_choice(const ShapePolicy& sp, int kind, int L) {
 // Row pairs per workgroup: as many lines of length L as fill the workgroup's 4096 complex
 // values (any count, not only powers of two), at most 16, fewer when the launch would not give
 // every CU four workgroups.
-int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* variant) {
+int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* tier, int* variant) {
     *variant = 0;
+    // lines longer than 4096 take the 512- or 1024-thread kernel, one pair per workgroup
+    *tier = tier_for_length(L);
+    if (*tier > 0) return 1;
     if (const ShapeChoice* c = find_choice(sp, SHAPE_ROW, L)) {
         *variant = c->variant;
         return c->lines;
@@ -437,6 +445,10 @@ int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* variant) {
 // 512-thread shape serves lines too long for 8 columns in 4096 values.
 int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& sp, int* wide, int* variant) {
     *variant = 0;
+    if (L > kLdsComplexWide) {  // the 1024-thread kernel
+        *wide = 2;
+        return std::max(1, kLdsComplexXL / L);
+    }
     if (const ShapeChoice* c = find_choice(sp, kind, L)) {
         *variant = c->variant;
         *wide = c->wide;
@@ -498,7 +510,7 @@ int lucem::make_otf_descs(const Geometry& g, const ShapePolicy& sp, const float*
     r.Lc = g.Lc;
     r.pitch = g.pitch;
     int var = 0;
-    r.P = choose_P(g.L[2], r.rows, sp, &var);
+    r.P = choose_P(g.L[2], r.rows, sp, &r.wide, &var);
     r.f = make_plan(g.L[2], var);
     r.src = w;
     r.spec = pa;
@@ -653,7 +665,7 @@ RowPass lucem::make_row_desc(const Geometry& g, const ShapePolicy& sp, const flo
     r.Lc = g.Lc;
     r.pitch = g.pitch;
     int var = 0;
-    r.P = choose_P(g.L[2], (int)rows, sp, &var);
+    r.P = choose_P(g.L[2], (int)rows, sp, &r.wide, &var);
     r.eps = eps;
     r.init = 0.5f;
     r.f = make_plan(g.L[2], var);
@@ -819,7 +831,7 @@ lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uin
     if (!make_geometry(n3, k3, (int)in_ndim, g))
         return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with psf " +
                                                  dims_str(psf_dims, psf_ndim) +
-                                                 ": an axis needs an FFT longer than 4096 (n + k - 1 per axis)");
+                                                 ": an axis needs an FFT longer than 16384 (n + k - 1 per axis)");
     g->mode = (int)s->mode;
     return LUCEM_OK;
 }
@@ -874,7 +886,7 @@ extern "C" lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const floa
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     if (L == 0 || nlines == 0) return LUCEM_OK;
     if (L > LUCEM_MAX_FFT_LEN || !smooth7((int)L))
-        return fail(LUCEM_UNSUPPORTED_SHAPE, "FFT length must be 7-smooth and at most 4096");
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "FFT length must be 7-smooth and at most 16384");
     if (nlines > 0x7FFFFFFFull) return fail(LUCEM_TOO_MANY_ELEMENTS, "too many lines");
     hipError_t e = hipSetDevice(client->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -889,6 +901,7 @@ extern "C" lucem_status lucem_fft_c2c_lines_f32(lucem_client* client, const floa
     p.n_dst = (int)L;
     p.Lc = 1;
     p.B = 1;
+    p.wide = tier_for_length((int)L);
     p.nkb = 1;
     p.src_outer = L;
     p.src_axis = 1;

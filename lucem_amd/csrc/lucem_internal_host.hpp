@@ -93,7 +93,7 @@ struct ShapePolicy {
     int64_t launch_lines = 0;
     const ShapeTable* table = nullptr;
 };
-int choose_P(int L, int rows, const ShapePolicy& sp, int* variant);
+int choose_P(int L, int rows, const ShapePolicy& sp, int* tier, int* variant);
 int choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& sp, int* wide, int* variant);
 bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g);
 

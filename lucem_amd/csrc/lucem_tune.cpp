@@ -149,7 +149,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             work += (double)r.rows * L;
             pairs += (r.rows + 1) / 2;
         }
-        if (work < 0.03 * rtotal || work < 2.0e5) continue;
+        if (work < 0.03 * rtotal || work < 2.0e5 || L > kLdsComplex) continue;
         const CacheKey key{SHAPE_ROW, L, ROW_UPDATE, ilog2(pairs), persistent ? 1 : 0};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -196,7 +196,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         return hip_fail(e, "hipMemcpy(autotune)");
                     if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
                         return hip_fail(e, "hipMemcpy(autotune)");
-                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, false, lds) * c->num_cus));
+                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, 0, lds) * c->num_cus));
                     RowBatch rb{dd, di, (int)items.size(), ROW_UPDATE, 1, 0, 0, 0, cnt};
                     st = time_launch(s, timer, [&]() {
                         int err = launch_zero_ints(cnt, 1, s);
@@ -242,7 +242,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         const int kind = o.second.first, L = o.second.second;
         const double work = -o.first;
         std::vector<TuneCol>& g = cgroups[o.second];
-        if (work < 0.03 * ctotal[kind] || work < 2.0e5) continue;
+        if (work < 0.03 * ctotal[kind] || work < 2.0e5 || L > kLdsComplexWide) continue;
         int64_t lines = 0;
         for (const TuneCol& tc : g) lines += (int64_t)tc.outers * tc.d.Lc;
         const bool fixed_wide = kind_wide.count(kind) > 0;
@@ -299,7 +299,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
                             return hip_fail(e, "hipMemcpy(autotune)");
                         const int grid =
-                            std::max(1, std::min((int)items.size(), batch_occupancy(false, w != 0, lds) * c->num_cus));
+                            std::max(1, std::min((int)items.size(), batch_occupancy(false, w, lds) * c->num_cus));
                         ColBatch cb{dd, di, (int)items.size(), d[0].op, w, cnt};
                         st = time_launch(s, timer, [&]() {
                             int err = launch_zero_ints(cnt, 1, s);

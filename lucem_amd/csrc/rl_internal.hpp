@@ -17,8 +17,13 @@ constexpr int kMaxPasses = 16;
 constexpr int kThreads = 256;         // threads per workgroup of every FFT kernel
 constexpr int kElemsPerThread = 16;   // complex values per thread: 4096 per workgroup, 32 KiB LDS
 constexpr int kLdsComplex = kThreads * kElemsPerThread;
-constexpr int kThreadsWide = 512;     // line passes over long lines: 8192 complex per workgroup
+constexpr int kThreadsWide = 512;     // long lines: 8192 complex per workgroup
 constexpr int kLdsComplexWide = kThreadsWide * kElemsPerThread;
+constexpr int kThreadsXL = 1024;      // lines up to 16384: 128 KiB of LDS, one workgroup per CU
+constexpr int kLdsComplexXL = kThreadsXL * kElemsPerThread;
+// Thread-count tier of an FFT kernel (the `wide` field of the descriptors): 0, 1 or 2.
+constexpr int tier_threads(int tier) { return tier == 2 ? kThreadsXL : tier == 1 ? kThreadsWide : kThreads; }
+constexpr int tier_for_length(int L) { return L <= kLdsComplex ? 0 : L <= kLdsComplexWide ? 1 : 2; }
 
 
 // One FFT length and its Stockham radix sequence: pass q has radix code byte q of the packed
@@ -60,7 +65,7 @@ struct RowPass {
     int left, mode;      // pad gather along x
     int Lc, pitch;       // half-spectrum bins per row, row pitch (complex) of `spec`
     int P;               // row pairs per workgroup (any count, P * L <= threads * 16)
-    int wide;            // reserved (0)
+    int wide;            // thread tier: 0 = 256, 1 = 512, 2 = 1024 threads
     int first;           // ROW_UPDATE: the previous estimate is the constant `init`
     int last;            // ROW_UPDATE: write `out`, no forward transform
     int clip;            // ROW_UPDATE && last: clamp to [-1, 1]
@@ -84,7 +89,7 @@ struct ColPass {
     int Lc;               // valid columns
     int B;                // columns per workgroup (any count, B * L <= threads * 16)
     int nkb;              // column blocks per outer index
-    int wide;             // 1: the 512-thread line kernel
+    int wide;             // thread tier: 0 = 256, 1 = 512, 2 = 1024 threads
     float scale;          // COL_FWD_OTF
     int64_t src_outer, src_axis, dst_outer, dst_axis, otf_outer, otf_axis;
     FftPlan f;
@@ -144,7 +149,9 @@ int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream);
 int launch_psf_batch(const PsfBatch& b, void* stream);
 int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
-int batch_occupancy(bool row, bool wide, size_t lds);
+int batch_occupancy(bool row, int tier, size_t lds);
+// Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
+int allow_big_lds_kernels();
 size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
 

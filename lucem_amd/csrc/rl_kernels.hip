@@ -439,82 +439,154 @@ __global__ void zero_ints_kernel(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Launchers. FFT kernels hold 16 complex values per thread (at most 128 VGPRs). Row passes use
-// 256 threads (32 KiB of LDS at most, four workgroups per CU); line passes use 256 threads or,
-// for long lines, 512 threads (64 KiB, two workgroups per CU) to double the columns per workgroup.
+// Launchers. FFT kernels hold 16 complex values per thread (at most 128 VGPRs). The thread tier
+// of a descriptor picks 256 threads (4096 values, 32 KiB of LDS), 512 (8192 values, 64 KiB) or
+// 1024 (16384 values, 128 KiB) per workgroup; a workgroup planned for fewer threads runs
+// correctly on more (its loops stride by the thread count).
 // ---------------------------------------------------------------------------------------------
 size_t row_pass_lds_bytes(int L, int P) { return (size_t)L * P * sizeof(float2); }
 
 size_t col_pass_lds_bytes(int L, int B) { return (size_t)L * B * sizeof(float2); }
 
-template <int E>
+// Dynamic LDS above 64 KiB is allowed once per process for the 1024-thread kernels (called by
+// lucem_client_create, never during stream capture).
+template <class K>
+static int allow_lds(K kernel) {
+    return (int)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(kLdsComplexXL * sizeof(float2)));
+}
+
+int allow_big_lds_kernels() {
+    constexpr int T = kThreadsXL, E = kElemsPerThread;
+    int err = 0;
+    for (int e : {allow_lds(row_pass_kernel<T, E, ROW_LOAD>), allow_lds(row_pass_kernel<T, E, ROW_LOAD_CONST>),
+                  allow_lds(row_pass_kernel<T, E, ROW_RATIO>), allow_lds(row_pass_kernel<T, E, ROW_UPDATE>),
+                  allow_lds(row_batch_kernel<T, E, ROW_LOAD>), allow_lds(row_batch_kernel<T, E, ROW_LOAD_CONST>),
+                  allow_lds(row_batch_kernel<T, E, ROW_RATIO>), allow_lds(row_batch_kernel<T, E, ROW_UPDATE>),
+                  allow_lds(col_pass_kernel<T, E, COL_FWD>), allow_lds(col_pass_kernel<T, E, COL_FUSED>),
+                  allow_lds(col_pass_kernel<T, E, COL_INV>), allow_lds(col_pass_kernel<T, E, COL_FWD_OTF>),
+                  allow_lds(col_pass_kernel<T, E, COL_FUSED_T>), allow_lds(col_pass_kernel<T, E, COL_INV_FOLD>),
+                  allow_lds(col_batch_kernel<T, E, COL_FWD>), allow_lds(col_batch_kernel<T, E, COL_FUSED>),
+                  allow_lds(col_batch_kernel<T, E, COL_INV>), allow_lds(col_batch_kernel<T, E, COL_FWD_OTF>),
+                  allow_lds(col_batch_kernel<T, E, COL_FUSED_T>), allow_lds(col_batch_kernel<T, E, COL_INV_FOLD>)})
+        if (e && !err) err = e;
+    return err;
+}
+
+template <int T, int E>
 static int launch_row_e(const RowPass& p, int blocks, void* stream) {
     const size_t lds = row_pass_lds_bytes(p.f.L, p.P);
     hipStream_t st = (hipStream_t)stream;
     switch (p.op) {
-        case ROW_LOAD: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_LOAD>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        case ROW_LOAD_CONST: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_LOAD_CONST>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        case ROW_RATIO: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_RATIO>), dim3(blocks), dim3(kThreads), lds, st, p); break;
-        default: hipLaunchKernelGGL((row_pass_kernel<kThreads, E, ROW_UPDATE>), dim3(blocks), dim3(kThreads), lds, st, p); break;
+        case ROW_LOAD:
+            hipLaunchKernelGGL((row_pass_kernel<T, E, ROW_LOAD>), dim3(blocks), dim3(T), lds, st, p);
+            break;
+        case ROW_LOAD_CONST:
+            hipLaunchKernelGGL((row_pass_kernel<T, E, ROW_LOAD_CONST>), dim3(blocks), dim3(T), lds, st, p);
+            break;
+        case ROW_RATIO:
+            hipLaunchKernelGGL((row_pass_kernel<T, E, ROW_RATIO>), dim3(blocks), dim3(T), lds, st, p);
+            break;
+        default:
+            hipLaunchKernelGGL((row_pass_kernel<T, E, ROW_UPDATE>), dim3(blocks), dim3(T), lds, st, p);
+            break;
     }
     return (int)hipGetLastError();
 }
 
-int launch_row_pass(const RowPass& p, int blocks, void* stream) { return launch_row_e<kElemsPerThread>(p, blocks, stream); }
+int launch_row_pass(const RowPass& p, int blocks, void* stream) {
+    switch (p.wide) {
+        case 2: return launch_row_e<kThreadsXL, kElemsPerThread>(p, blocks, stream);
+        case 1: return launch_row_e<kThreadsWide, kElemsPerThread>(p, blocks, stream);
+        default: return launch_row_e<kThreads, kElemsPerThread>(p, blocks, stream);
+    }
+}
 
 template <int T, int E>
 static int launch_col_e(const ColPass& p, int blocks, void* stream) {
     const size_t lds = col_pass_lds_bytes(p.f.L, p.B);
     hipStream_t st = (hipStream_t)stream;
+#define LUCEM_COL_CASE(OPC)                                                                    \
+    case OPC: \
+        hipLaunchKernelGGL((col_pass_kernel<T, E, OPC>), dim3(blocks), dim3(T), lds, st, p); \
+        break;
     switch (p.op) {
-        case COL_FWD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD>), dim3(blocks), dim3(T), lds, st, p); break;
-        case COL_FUSED: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FUSED>), dim3(blocks), dim3(T), lds, st, p); break;
-        case COL_INV: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_INV>), dim3(blocks), dim3(T), lds, st, p); break;
-        case COL_FUSED_T: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FUSED_T>), dim3(blocks), dim3(T), lds, st, p); break;
-        case COL_INV_FOLD: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_INV_FOLD>), dim3(blocks), dim3(T), lds, st, p); break;
-        default: hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD_OTF>), dim3(blocks), dim3(T), lds, st, p); break;
+        LUCEM_COL_CASE(COL_FWD)
+        LUCEM_COL_CASE(COL_FUSED)
+        LUCEM_COL_CASE(COL_INV)
+        LUCEM_COL_CASE(COL_FUSED_T)
+        LUCEM_COL_CASE(COL_INV_FOLD)
+        default:
+            hipLaunchKernelGGL((col_pass_kernel<T, E, COL_FWD_OTF>), dim3(blocks), dim3(T), lds, st, p);
+            break;
     }
+#undef LUCEM_COL_CASE
     return (int)hipGetLastError();
 }
 
 int launch_col_pass(const ColPass& p, int blocks, void* stream) {
-    return p.wide ? launch_col_e<kThreadsWide, kElemsPerThread>(p, blocks, stream)
-                  : launch_col_e<kThreads, kElemsPerThread>(p, blocks, stream);
+    switch (p.wide) {
+        case 2: return launch_col_e<kThreadsXL, kElemsPerThread>(p, blocks, stream);
+        case 1: return launch_col_e<kThreadsWide, kElemsPerThread>(p, blocks, stream);
+        default: return launch_col_e<kThreads, kElemsPerThread>(p, blocks, stream);
+    }
 }
 
-template <int E>
+template <int T, int E>
 static int launch_row_batch_e(const RowBatch& b, int grid, size_t lds, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     switch (b.op) {
-        case ROW_LOAD: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_LOAD>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        case ROW_LOAD_CONST: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_LOAD_CONST>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        case ROW_RATIO: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_RATIO>), dim3(grid), dim3(kThreads), lds, st, b); break;
-        default: hipLaunchKernelGGL((row_batch_kernel<kThreads, E, ROW_UPDATE>), dim3(grid), dim3(kThreads), lds, st, b); break;
+        case ROW_LOAD:
+            hipLaunchKernelGGL((row_batch_kernel<T, E, ROW_LOAD>), dim3(grid), dim3(T), lds, st, b);
+            break;
+        case ROW_LOAD_CONST:
+            hipLaunchKernelGGL((row_batch_kernel<T, E, ROW_LOAD_CONST>), dim3(grid), dim3(T), lds, st, b);
+            break;
+        case ROW_RATIO:
+            hipLaunchKernelGGL((row_batch_kernel<T, E, ROW_RATIO>), dim3(grid), dim3(T), lds, st, b);
+            break;
+        default:
+            hipLaunchKernelGGL((row_batch_kernel<T, E, ROW_UPDATE>), dim3(grid), dim3(T), lds, st, b);
+            break;
     }
     return (int)hipGetLastError();
 }
 
 int launch_row_batch(const RowBatch& b, int grid, size_t lds, void* stream) {
-    return launch_row_batch_e<kElemsPerThread>(b, grid, lds, stream);
+    switch (b.wide) {
+        case 2: return launch_row_batch_e<kThreadsXL, kElemsPerThread>(b, grid, lds, stream);
+        case 1: return launch_row_batch_e<kThreadsWide, kElemsPerThread>(b, grid, lds, stream);
+        default: return launch_row_batch_e<kThreads, kElemsPerThread>(b, grid, lds, stream);
+    }
 }
 
 template <int T, int E>
 static int launch_col_batch_e(const ColBatch& b, int grid, size_t lds, void* stream) {
     hipStream_t st = (hipStream_t)stream;
+#define LUCEM_COLB_CASE(OPC)                                                                   \
+    case OPC: \
+        hipLaunchKernelGGL((col_batch_kernel<T, E, OPC>), dim3(grid), dim3(T), lds, st, b);  \
+        break;
     switch (b.op) {
-        case COL_FWD: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD>), dim3(grid), dim3(T), lds, st, b); break;
-        case COL_FUSED: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FUSED>), dim3(grid), dim3(T), lds, st, b); break;
-        case COL_INV: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_INV>), dim3(grid), dim3(T), lds, st, b); break;
-        case COL_FUSED_T: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FUSED_T>), dim3(grid), dim3(T), lds, st, b); break;
-        case COL_INV_FOLD: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_INV_FOLD>), dim3(grid), dim3(T), lds, st, b); break;
-        default: hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD_OTF>), dim3(grid), dim3(T), lds, st, b); break;
+        LUCEM_COLB_CASE(COL_FWD)
+        LUCEM_COLB_CASE(COL_FUSED)
+        LUCEM_COLB_CASE(COL_INV)
+        LUCEM_COLB_CASE(COL_FUSED_T)
+        LUCEM_COLB_CASE(COL_INV_FOLD)
+        default:
+            hipLaunchKernelGGL((col_batch_kernel<T, E, COL_FWD_OTF>), dim3(grid), dim3(T), lds, st, b);
+            break;
     }
+#undef LUCEM_COLB_CASE
     return (int)hipGetLastError();
 }
 
 int launch_col_batch(const ColBatch& b, int grid, size_t lds, void* stream) {
-    return b.wide ? launch_col_batch_e<kThreadsWide, kElemsPerThread>(b, grid, lds, stream)
-                  : launch_col_batch_e<kThreads, kElemsPerThread>(b, grid, lds, stream);
+    switch (b.wide) {
+        case 2: return launch_col_batch_e<kThreadsXL, kElemsPerThread>(b, grid, lds, stream);
+        case 1: return launch_col_batch_e<kThreadsWide, kElemsPerThread>(b, grid, lds, stream);
+        default: return launch_col_batch_e<kThreads, kElemsPerThread>(b, grid, lds, stream);
+    }
 }
 
 int launch_psf_batch(const PsfBatch& b, void* stream) {
@@ -527,18 +599,25 @@ int launch_zero_ints(int* p, int n, void* stream) {
     return (int)hipGetLastError();
 }
 
-int batch_occupancy(bool row, bool wide, size_t lds) {
+int batch_occupancy(bool row, int tier, size_t lds) {
     int blocks = 0;
     hipError_t e;
-    if (row)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>,
-                                                         kThreads, lds);
-    else if (wide)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, col_batch_kernel<kThreadsWide, kElemsPerThread, COL_FUSED>, kThreadsWide, lds);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>,
-                                                         kThreads, lds);
+    const int T = tier_threads(tier);
+    if (row) {
+        if (tier == 2)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreadsXL, kElemsPerThread, ROW_RATIO>, T, lds);
+        else if (tier == 1)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreadsWide, kElemsPerThread, ROW_RATIO>, T, lds);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, row_batch_kernel<kThreads, kElemsPerThread, ROW_RATIO>, T, lds);
+    } else {
+        if (tier == 2)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreadsXL, kElemsPerThread, COL_FUSED>, T, lds);
+        else if (tier == 1)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreadsWide, kElemsPerThread, COL_FUSED>, T, lds);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_batch_kernel<kThreads, kElemsPerThread, COL_FUSED>, T, lds);
+    }
     if (e != hipSuccess) return 1;
     return blocks > 0 ? blocks : 1;
 }

@@ -116,7 +116,7 @@ def test_check_order_nd_without_device(L):
     assert _call(L, (2, 4, 5), (3, 3), (2, 4, 5), nd=True) == 3  # psf rank must match in N-D
     assert _call(L, (2, 2, 4, 5), (3, 3, 3, 3), (2, 2, 4, 5), nd=True) == 3
     assert _call(L, (65536, 65536, 2), (1, 1, 1), (65536, 65536, 2), nd=True, out=1 << 45) == 6  # > u32::MAX
-    assert _call(L, (8, 9000), (3, 3), (8, 9000)) == 9  # axis beyond the LDS FFT limit
+    assert _call(L, (8, 16400), (3, 3), (8, 16400)) == 9  # axis beyond the 16384 FFT limit
 
 
 def test_oracle_is_not_imported_by_the_product():

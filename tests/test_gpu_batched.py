@@ -131,3 +131,24 @@ def test_tiles_workload_matches_oracle(gpu, oracle):
     for t, o, i in zip(ts.tiles, outs, ts.psf_index):
         st, want = oracle.richardson_lucy(t, ts.psfs[i], ts.num_iter, clip=False)
         assert oracle.rel_l2(o.cpu().numpy(), want) < RTOL_L2
+
+
+def test_batched_long_and_short_tiles(gpu, oracle):
+    """A 2 x 9000 tile (1024-thread row kernel) and a 9000 x 2 tile (1024-thread line kernel) in one
+    batch with ordinary tiles: every launch runs at the widest tier its tiles need."""
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import richardson_lucy_batched
+
+    rng = np.random.default_rng(91)
+    shapes = [(2, 9000), (40, 50), (9000, 2), (17, 300)]
+    ys = [rng.uniform(50, 150, s).astype(np.float32) for s in shapes]
+    psf = rng.uniform(0.1, 1.0, (3, 5)).astype(np.float32)
+    s = LA.RichardsonLucy(num_iter=3, clip=False)
+    outs = richardson_lucy_batched([torch.from_numpy(y).cuda() for y in ys], [torch.from_numpy(psf).cuda()],
+                                   [0] * len(ys), s)
+    for y, o in zip(ys, outs):
+        st, ref = oracle.richardson_lucy(y, psf, 3, clip=False)
+        assert st == 0
+        assert oracle.rel_l2(o.cpu().numpy(), ref) < 1e-4

@@ -38,7 +38,7 @@ def _run(y, psf, num_iter, mode=0, clip=False, nd=None, eps=None):
 
 
 @pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 25, 30, 49, 64, 105, 128, 160, 343, 525, 540,
-                               1000, 1050, 2048, 4096])
+                               1000, 1050, 2048, 4096, 6048, 8192, 9072, 16128, 16384])
 def test_fft_lines_match_numpy(gpu, L):
     import ctypes
 
@@ -275,3 +275,35 @@ def test_cpp_caller(gpu):
     assert exe.exists(), "build() compiles tests/cpp/test_restore"
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and res.stdout.strip() == "ok", res.stderr
+
+
+# Lines longer than 4096 after padding take the 512- and 1024-thread kernels (up to 16384).
+LONG_CASES = [
+    ((3, 6000), (3, 5)),     # rows of L = 6048: 512 threads
+    ((2, 9000), (1, 7)),     # rows of L = 9072: 1024 threads, 71 KiB of LDS
+    ((9000, 3), (5, 3)),     # columns of L = 9072: 1024-thread line pass
+    ((2, 16000), (1, 3)),    # rows of L = 16128: 126 KiB of LDS
+]
+
+
+@pytest.mark.parametrize("shape,kshape", LONG_CASES)
+def test_long_lines_match_oracle(gpu, oracle, shape, kshape):
+    rng = np.random.default_rng(shape[0] * 7 + shape[1])
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    p = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    got = _run(y, p, 4)
+    st, ref = oracle.richardson_lucy(y, p, 4, clip=False)
+    assert st == 0
+    e = oracle.rel_l2(got, ref)
+    print(f"{shape} psf {kshape}: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
+
+
+def test_longer_than_16384_is_refused(gpu):
+    import torch
+
+    import lucem_amd as LA
+
+    y = torch.ones((2, 16400), device="cuda")
+    with pytest.raises(Exception, match="16384"):
+        LA.richardson_lucy_into(y, torch.ones((1, 3), device="cuda"), torch.empty_like(y), LA.RichardsonLucy(num_iter=2))
