@@ -218,6 +218,22 @@ struct Composite {
     }
 };
 
+// A twiddle from the LDS-staged table (TWL) or from global memory, with the address space
+// explicit so that the non-inlined passes issue ds_read / global_load rather than flat loads.
+template <bool TWL>
+__device__ __forceinline__ float2 load_twiddle(const float2* tw, int i) {
+    const float* f = reinterpret_cast<const float*>(tw);
+    if constexpr (TWL) {
+        typedef const __attribute__((address_space(3))) float* lds_ptr;
+        const lds_ptr q = (lds_ptr)f;
+        return make_float2(q[2 * i], q[2 * i + 1]);
+    } else {
+        typedef const __attribute__((address_space(1))) float* global_ptr;
+        const global_ptr q = (global_ptr)f;
+        return make_float2(q[2 * i], q[2 * i + 1]);
+    }
+}
+
 // Exact division by a small positive divisor d through a multiply-high: q = u / d for every
 // 0 <= u < 2^17 and d <= 4096 (m = floor((2^32 - 1) / d) + 1, checked exhaustively on the host).
 struct Divisor {
@@ -232,7 +248,7 @@ struct Divisor {
 // w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns. With the
 // butterfly index u = j*B + b, the inputs sit at u + t*(N/R)*B and the outputs at
 // u + (j - k)*(R - 1)*B + t*Ns*B.
-template <int R1, int R2, bool INV, int T, int E>
+template <int R1, int R2, bool INV, int T, int E, bool TWL>
 __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns,
                                               const float2* __restrict__ tw) {
     constexpr int R = R1 * R2;
@@ -261,7 +277,7 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divi
                 const int step = k * tws;
 #pragma unroll
                 for (int t = 1; t < R; t++) {
-                    float2 w = tw[t * step];
+                    float2 w = load_twiddle<TWL>(tw, t * step);
                     if (INV) w.y = -w.y;
                     v[s][t] = cmul(v[s][t], w);
                 }
@@ -279,10 +295,30 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divi
 // Radix codes of a pass: R1 in the low nibble, R2 in the high nibble (R2 = 1 for a simple pass).
 constexpr unsigned char radix_code(int r1, int r2) { return (unsigned char)(r1 | (r2 << 4)); }
 
-// Full transform of the interleaved lines. The caller has written buf and synchronised.
-template <bool INV, int T, int E>
-__device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ twbase) {
-    const float2* tw = twbase + p.tw;
+// Twiddle tables of lengths up to kTwLdsMax are staged in LDS behind the line data (a global
+// twiddle load waits about a thousand cycles under load, an LDS read about a hundred).
+constexpr int kTwLdsMax = 2048;
+__host__ __device__ constexpr int tw_lds_offset(int data_elems) { return (data_elems + 15) & ~15; }
+// LDS elements of `data_elems` line values of length L plus the staged table.
+__host__ __device__ constexpr int lds_elems(int data_elems, int L) {
+    return L <= kTwLdsMax ? tw_lds_offset(data_elems) + L : data_elems;
+}
+
+// Copies the twiddle table of f into LDS behind the data (the caller synchronises before the
+// first pass); null when the table stays in global memory.
+__device__ __forceinline__ const float2* stage_twiddles(float2* Z, int data_elems, const FftPlan& f,
+                                                        const float2* __restrict__ twbase) {
+    if (f.L > kTwLdsMax) return nullptr;
+    float2* t = Z + tw_lds_offset(data_elems);
+    const float2* g = twbase + f.tw;
+    for (int i = threadIdx.x; i < f.L; i += blockDim.x) t[i] = g[i];
+    return t;
+}
+
+// Full transform of the interleaved lines. The caller has written buf (and the staged twiddles,
+// twl) and synchronised.
+template <bool INV, int T, int E, bool TWL>
+__device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ tw) {
     const int N = p.L;
     int Ns = 1;
     for (int q = 0; q < p.npass; q++) {
@@ -291,7 +327,7 @@ __device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& 
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                        \
     case radix_code(r1, r2):                                                      \
-        stockham_pass<r1, r2, INV, T, E>(buf, N, lines, Ns, tw);                  \
+        stockham_pass<r1, r2, INV, T, E, TWL>(buf, N, lines, Ns, tw);            \
         Ns *= r1 * r2;                                                            \
         break;
             LUCEM_PASS(2, 1)
@@ -313,6 +349,15 @@ __device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& 
                 break;
         }
     }
+}
+
+template <bool INV, int T, int E>
+__device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ twbase,
+                        const float2* twl) {
+    if (twl)
+        lds_fft_run<INV, T, E, true>(buf, lines, p, twl);
+    else
+        lds_fft_run<INV, T, E, false>(buf, lines, p, twbase + p.tw);
 }
 
 // Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len).

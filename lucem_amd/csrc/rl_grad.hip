@@ -60,6 +60,7 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
     const int tid = threadIdx.x;
     const int pair0 = g * P;
     const int total = Lf * P;
+    const float2* twl = stage_twiddles(Z, total, d.f, d.tw);
     const int64_t ns = G.nstride;
 
     if constexpr (TR::c2r) {
@@ -102,7 +103,7 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, pd, d.f, d.tw);
+        lds_fft<true, T, E>(Z, pd, d.f, d.tw, twl);
         if constexpr (TR::fold) {
             fold_rows(Z, P, d.left, nx, d.mode, min(Lf, G.kx + nx - 1));
             __syncthreads();
@@ -212,7 +213,7 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
         if (e < total) Z[e] = val[s];
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, pd, d.f, d.tw);
+    lds_fft<false, T, E>(Z, pd, d.f, d.tw, twl);
     // item e = b * Lc + k over all pairs, so that every thread has work whatever P and Lc are
     const Divisor lcd = Divisor::of(d.Lc);
     const int nsplit = P * d.Lc;

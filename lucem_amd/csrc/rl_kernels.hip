@@ -37,6 +37,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
     const int tid = threadIdx.x;
     const int pair0 = g * P;
     const int total = L * P;
+    const float2* twl = stage_twiddles(Z, total, d.f, d.tw);
 
     if constexpr (OP == ROW_RATIO || OP == ROW_UPDATE) {
         // Z[k] = A[k] + i B[k] for k < Lc, and the Hermitian mirror for k >= Lc. All loads are
@@ -80,7 +81,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, pd, d.f, d.tw);
+        lds_fft<true, T, E>(Z, pd, d.f, d.tw, twl);
         // Gather into registers: padded position i reads the cropped result at src = pad(i - left)
         // and applies the pointwise step there. Every value is read before any is overwritten.
         float2 val[S];
@@ -184,7 +185,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
         }
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, pd, d.f, d.tw);
+    lds_fft<false, T, E>(Z, pd, d.f, d.tw, twl);
     // split the two real transforms: A = Z[k] + conj(Z[L-k]), B = -i (Z[k] - conj(Z[L-k]))
     // (both carry a factor 2; the transfer function is scaled to remove it)
     // item e = b * Lc + k over all pairs, so that every thread has work whatever P and Lc are
@@ -239,6 +240,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
     const int tid = threadIdx.x;
     const float2* src = d.src + outer * d.src_outer + kx0;
     const int total = L * B;
+    const float2* twl = stage_twiddles(Z, total, d.f, d.tw);
     const bool full = kx0 + B <= d.Lc;
     constexpr bool INV_IN = OP == COL_INV || OP == COL_INV_FOLD;
     {
@@ -261,9 +263,9 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
     }
     __syncthreads();
     if constexpr (INV_IN) {
-        lds_fft<true, T, E>(Z, bd, d.f, d.tw);
+        lds_fft<true, T, E>(Z, bd, d.f, d.tw, twl);
     } else {
-        lds_fft<false, T, E>(Z, bd, d.f, d.tw);
+        lds_fft<false, T, E>(Z, bd, d.f, d.tw, twl);
         if constexpr (OP == COL_FUSED || OP == COL_FUSED_T) {
             const int64_t off = outer * d.otf_outer + kx0;
             const float2* otf = d.otf + off;
@@ -321,7 +323,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                 }
             }
             __syncthreads();
-            lds_fft<true, T, E>(Z, bd, d.f, d.tw);
+            lds_fft<true, T, E>(Z, bd, d.f, d.tw, twl);
         }
     }
     int row0 = 0;
@@ -444,20 +446,21 @@ __global__ void zero_ints_kernel(int* p, int n) {
 // 1024 (16384 values, 128 KiB) per workgroup; a workgroup planned for fewer threads runs
 // correctly on more (its loops stride by the thread count).
 // ---------------------------------------------------------------------------------------------
-size_t row_pass_lds_bytes(int L, int P) { return (size_t)L * P * sizeof(float2); }
+size_t row_pass_lds_bytes(int L, int P) { return (size_t)lds_elems(L * P, L) * sizeof(float2); }
 
-size_t col_pass_lds_bytes(int L, int B) { return (size_t)L * B * sizeof(float2); }
+size_t col_pass_lds_bytes(int L, int B) { return (size_t)lds_elems(L * B, L) * sizeof(float2); }
 
-// Dynamic LDS above 64 KiB is allowed once per process for the 1024-thread kernels (called by
-// lucem_client_create, never during stream capture).
+// Dynamic LDS above 64 KiB is allowed once per process for the 512- and 1024-thread kernels
+// (called by lucem_client_create, never during stream capture).
 template <class K>
 static int allow_lds(K kernel) {
     return (int)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(kLdsComplexXL * sizeof(float2)));
+                                    (int)(lds_elems(kLdsComplexXL, kLdsComplexXL) * sizeof(float2)));
 }
 
-int allow_big_lds_kernels() {
-    constexpr int T = kThreadsXL, E = kElemsPerThread;
+template <int T>
+static int allow_big_lds_tier() {
+    constexpr int E = kElemsPerThread;
     int err = 0;
     for (int e : {allow_lds(row_pass_kernel<T, E, ROW_LOAD>), allow_lds(row_pass_kernel<T, E, ROW_LOAD_CONST>),
                   allow_lds(row_pass_kernel<T, E, ROW_RATIO>), allow_lds(row_pass_kernel<T, E, ROW_UPDATE>),
@@ -471,6 +474,12 @@ int allow_big_lds_kernels() {
                   allow_lds(col_batch_kernel<T, E, COL_FUSED_T>), allow_lds(col_batch_kernel<T, E, COL_INV_FOLD>)})
         if (e && !err) err = e;
     return err;
+}
+
+int allow_big_lds_kernels() {
+    const int a = allow_big_lds_tier<kThreadsWide>();
+    const int b = allow_big_lds_tier<kThreadsXL>();
+    return a ? a : b;
 }
 
 template <int T, int E>
