@@ -8,7 +8,8 @@
 // leave garbage that the captured sequence overwrites before it reads), each one warm-up launch
 // then the best of three timed launches (HIP events on the client's stream). Results are kept
 // per process, keyed by (kind, L, op, launch size, schedule), so later plans of the same shapes
-// skip the measurement. LUCEM_AUTOTUNE=0 turns tuning off (the heuristics of choose_P/choose_B).
+// skip the measurement (LUCEM_TUNE_CACHE=<file> keeps them across processes). LUCEM_AUTOTUNE=0
+// turns tuning off (the heuristics of choose_P/choose_B).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +43,39 @@ std::vector<int> line_candidates(int max_lines, int heuristic) {
 using CacheKey = std::tuple<int, int, int, int, int>;  // kind, L, op, log2(lines), persistent
 std::mutex g_cache_mu;
 std::map<CacheKey, ShapeChoice> g_cache;
+bool g_cache_loaded = false;
+
+// LUCEM_TUNE_CACHE=<file> keeps the measured choices across processes: read once, appended to
+// after each new measurement (one line per key: kind L op log2lines persistent lines wide variant).
+const char* cache_file() {
+    const char* f = std::getenv("LUCEM_TUNE_CACHE");
+    return f && *f ? f : nullptr;
+}
+
+void load_cache_locked() {
+    if (g_cache_loaded) return;
+    g_cache_loaded = true;
+    const char* path = cache_file();
+    if (!path) return;
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return;
+    int k, L, op, lg, pers, lines, wide, var;
+    while (std::fscanf(fp, "%d %d %d %d %d %d %d %d", &k, &L, &op, &lg, &pers, &lines, &wide, &var) == 8)
+        if (lines >= 1 && wide >= 0 && wide <= 2 && var >= 0 && var <= 2)
+            g_cache[CacheKey{k, L, op, lg, pers}] = ShapeChoice{lines, wide, var};
+    std::fclose(fp);
+}
+
+void store_locked(const CacheKey& key, const ShapeChoice& c) {
+    g_cache[key] = c;
+    const char* path = cache_file();
+    if (!path) return;
+    if (FILE* fp = std::fopen(path, "a")) {
+        std::fprintf(fp, "%d %d %d %d %d %d %d %d\n", std::get<0>(key), std::get<1>(key), std::get<2>(key),
+                     std::get<3>(key), std::get<4>(key), c.lines, c.wide, c.variant);
+        std::fclose(fp);
+    }
+}
 
 int shape_kind(const ColPass& c) {
     if (c.op == COL_FUSED) return SHAPE_COL_FUSED;
@@ -153,6 +187,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         const CacheKey key{SHAPE_ROW, L, ROW_UPDATE, ilog2(pairs), persistent ? 1 : 0};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
+            load_cache_locked();
             auto it = g_cache.find(key);
             if (it != g_cache.end()) {
                 (*out)[{SHAPE_ROW, L}] = it->second;
@@ -218,7 +253,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             std::fprintf(stderr, "lucem autotune: row L=%d pairs=%lld -> P=%d variant=%d (%.4f ms)\n", L,
                          (long long)pairs, best.lines, best.variant, best_ms);
         std::lock_guard<std::mutex> lk(g_cache_mu);
-        g_cache[key] = best;
+        store_locked(key, best);
     }
 
     // ---- line passes, grouped by (kind, L)
@@ -250,6 +285,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         const CacheKey key{kind * 4 + wide_key, L, g[0].d.op, ilog2(lines), persistent ? 1 : 0};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
+            load_cache_locked();
             auto it = g_cache.find(key);
             if (it != g_cache.end()) {
                 (*out)[{kind, L}] = it->second;
@@ -323,7 +359,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             std::fprintf(stderr, "lucem autotune: line kind=%d L=%d lines=%lld -> B=%d wide=%d variant=%d (%.4f ms)\n",
                          kind, L, (long long)lines, best.lines, best.wide, best.variant, best_ms);
         std::lock_guard<std::mutex> lk(g_cache_mu);
-        g_cache[key] = best;
+        store_locked(key, best);
     }
     return LUCEM_OK;
 }

@@ -16,6 +16,11 @@
 #include "fft_lds.hpp"
 #include "rl_internal.hpp"
 
+// Waves per SIMD the FFT kernels are compiled for (sets their VGPR budget: 4 -> 128).
+#ifndef LUCEM_WAVES_PER_EU
+#define LUCEM_WAVES_PER_EU 4
+#endif
+
 namespace lucem {
 
 // ---------------------------------------------------------------------------------------------
@@ -202,7 +207,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row_pass_kernel(RowPass d) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void row_pass_kernel(RowPass d) {
     extern __shared__ float2 lds[];
     row_pass_body<T, E, OP>(d, RowFlags{d.first, d.last, d.clip}, blockIdx.x, lds);
 }
@@ -345,7 +350,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void col_pass_kernel(ColPass d) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_pass_kernel(ColPass d) {
     extern __shared__ float2 lds[];
     col_pass_body<T, E, OP>(d, xcd_block(blockIdx.x, gridDim.x), lds);
 }
@@ -385,7 +390,7 @@ __device__ __forceinline__ int next_item(int* counter, int* slot) {
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row_batch_kernel(RowBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void row_batch_kernel(RowBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void row
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void col_batch_kernel(ColBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_batch_kernel(ColBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
