@@ -235,7 +235,7 @@ __device__ __forceinline__ float2 load_twiddle(const float2* tw, int i) {
 }
 
 // Exact division by a small positive divisor d through a multiply-high: q = u / d for every
-// 0 <= u < 2^17 and d <= 4096 (m = floor((2^32 - 1) / d) + 1, checked exhaustively on the host).
+// 0 <= u < 2^17 and d <= 16384 (m = floor((2^32 - 1) / d) + 1, checked exhaustively on the host).
 struct Divisor {
     int d;
     unsigned m;
