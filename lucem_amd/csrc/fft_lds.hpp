@@ -248,7 +248,9 @@ struct Divisor {
 // w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns. With the
 // butterfly index u = j*B + b, the inputs sit at u + t*(N/R)*B and the outputs at
 // u + (j - k)*(R - 1)*B + t*Ns*B.
-template <int R1, int R2, bool INV, int T, int E, bool TWL>
+// W: waves per SIMD of the calling kernels. Each W gets its own copy of the pass, compiled to
+// that register budget (a copy shared by kernels of different budgets gets the smallest).
+template <int R1, int R2, bool INV, int T, int E, bool TWL, int W>
 __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns,
                                               const float2* __restrict__ tw) {
     constexpr int R = R1 * R2;
@@ -317,7 +319,7 @@ __device__ __forceinline__ const float2* stage_twiddles(float2* Z, int data_elem
 
 // Full transform of the interleaved lines. The caller has written buf (and the staged twiddles,
 // twl) and synchronised.
-template <bool INV, int T, int E, bool TWL>
+template <bool INV, int T, int E, bool TWL, int W>
 __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ tw) {
     const int N = p.L;
     int Ns = 1;
@@ -327,7 +329,7 @@ __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPl
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                        \
     case radix_code(r1, r2):                                                      \
-        stockham_pass<r1, r2, INV, T, E, TWL>(buf, N, lines, Ns, tw);            \
+        stockham_pass<r1, r2, INV, T, E, TWL, W>(buf, N, lines, Ns, tw);         \
         Ns *= r1 * r2;                                                            \
         break;
             LUCEM_PASS(2, 1)
@@ -338,11 +340,9 @@ __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPl
             LUCEM_PASS(8, 1)
             LUCEM_PASS(3, 2)
             LUCEM_PASS(5, 2)
-            LUCEM_PASS(7, 2)
             LUCEM_PASS(8, 2)
             LUCEM_PASS(3, 3)
             LUCEM_PASS(4, 3)
-            LUCEM_PASS(5, 3)
             LUCEM_PASS(4, 4)
 #undef LUCEM_PASS
             default:
@@ -351,13 +351,13 @@ __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPl
     }
 }
 
-template <bool INV, int T, int E>
+template <bool INV, int T, int E, int W = 4>
 __device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ twbase,
                         const float2* twl) {
     if (twl)
-        lds_fft_run<INV, T, E, true>(buf, lines, p, twl);
+        lds_fft_run<INV, T, E, true, W>(buf, lines, p, twl);
     else
-        lds_fft_run<INV, T, E, false>(buf, lines, p, twbase + p.tw);
+        lds_fft_run<INV, T, E, false, W>(buf, lines, p, twbase + p.tw);
 }
 
 // Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len).

@@ -342,9 +342,11 @@ int lucem::choose_fft_len(int target) {
 }
 
 FftPlan lucem::make_plan(int L, int variant) {
-    // Prime factors, then greedy grouping into the largest supported radix: 16 = 4*4, 15 = 5*3,
-    // 14 = 7*2, 12 = 4*3, 10 = 5*2, 9 = 3*3, 8, 7, 6 = 3*2, 5, 4, 3, 2 (variant 2: 8, 7, 5, 4,
-    // 3, 2 only). Variant 1 runs the passes of variant 0 in reverse order.
+    // Prime factors, then greedy grouping into the largest supported radix: 16 = 4*4, 12 = 4*3,
+    // 10 = 5*2, 9 = 3*3, 8, 7, 6 = 3*2, 5, 4, 3, 2 (variant 2: 8, 7, 5, 4, 3, 2 only). Variant 1
+    // runs the passes of variant 0 in reverse order. Radix 15 = 5*3 and 14 = 7*2 are left out:
+    // with two butterflies per thread they need 118 VGPRs, where every other pass needs at most
+    // 97, and the kernels' register budget is the largest pass's.
     FftPlan p{};
     p.L = L;
     p.npass = 0;
@@ -360,7 +362,7 @@ FftPlan lucem::make_plan(int L, int variant) {
         int r1, r2, n2, n3, n5, n7;
     };
     static const Code composite[] = {
-        {4, 4, 4, 0, 0, 0}, {5, 3, 0, 1, 1, 0}, {7, 2, 1, 0, 0, 1}, {4, 3, 2, 1, 0, 0}, {5, 2, 1, 0, 1, 0},
+        {4, 4, 4, 0, 0, 0}, {4, 3, 2, 1, 0, 0}, {5, 2, 1, 0, 1, 0},
         {3, 3, 0, 2, 0, 0}, {8, 1, 3, 0, 0, 0}, {7, 1, 0, 0, 0, 1}, {3, 2, 1, 1, 0, 0}, {5, 1, 0, 0, 1, 0},
         {4, 1, 2, 0, 0, 0}, {3, 1, 0, 1, 0, 0}, {2, 1, 1, 0, 0, 0},
     };

@@ -16,9 +16,14 @@
 #include "fft_lds.hpp"
 #include "rl_internal.hpp"
 
-// Waves per SIMD the FFT kernels are compiled for (sets their VGPR budget: 4 -> 128).
+// Waves per SIMD the FFT kernels are compiled for (their VGPR budget: 4 -> 128, 5 -> 96). The
+// one-array kernels run 5 (measured faster despite a few spilled registers); the persistent
+// batch kernels run 4 (at 5 they spill in the work loop and were slower).
 #ifndef LUCEM_WAVES_PER_EU
-#define LUCEM_WAVES_PER_EU 4
+#define LUCEM_WAVES_PER_EU 5
+#endif
+#ifndef LUCEM_BATCH_WAVES_PER_EU
+#define LUCEM_BATCH_WAVES_PER_EU 4
 #endif
 
 namespace lucem {
@@ -32,7 +37,7 @@ struct RowFlags {
     int first, last, clip;
 };
 
-template <int T, int E, int OP>
+template <int T, int E, int OP, int W>
 __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2* Z) {
     constexpr int S = E;  // elements per thread in the gather (L * P <= T * E)
     const int P = d.P;
@@ -86,7 +91,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             }
         }
         __syncthreads();
-        lds_fft<true, T, E>(Z, pd, d.f, d.tw, twl);
+        lds_fft<true, T, E, W>(Z, pd, d.f, d.tw, twl);
         // Gather into registers: padded position i reads the cropped result at src = pad(i - left)
         // and applies the pointwise step there. Every value is read before any is overwritten.
         float2 val[S];
@@ -190,7 +195,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
         }
     }
     __syncthreads();
-    lds_fft<false, T, E>(Z, pd, d.f, d.tw, twl);
+    lds_fft<false, T, E, W>(Z, pd, d.f, d.tw, twl);
     // split the two real transforms: A = Z[k] + conj(Z[L-k]), B = -i (Z[k] - conj(Z[L-k]))
     // (both carry a factor 2; the transfer function is scaled to remove it)
     // item e = b * Lc + k over all pairs, so that every thread has work whatever P and Lc are
@@ -209,7 +214,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
 template <int T, int E, int OP>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void row_pass_kernel(RowPass d) {
     extern __shared__ float2 lds[];
-    row_pass_body<T, E, OP>(d, RowFlags{d.first, d.last, d.clip}, blockIdx.x, lds);
+    row_pass_body<T, E, OP, LUCEM_WAVES_PER_EU>(d, RowFlags{d.first, d.last, d.clip}, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -235,7 +240,7 @@ __device__ __forceinline__ void fold_lines(float2* Z, int B, int nlines_valid, i
     (void)L;
 }
 
-template <int T, int E, int OP>
+template <int T, int E, int OP, int W>
 __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) {
     const int B = d.B;
     const Divisor bd = Divisor::of(B);
@@ -268,9 +273,9 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
     }
     __syncthreads();
     if constexpr (INV_IN) {
-        lds_fft<true, T, E>(Z, bd, d.f, d.tw, twl);
+        lds_fft<true, T, E, W>(Z, bd, d.f, d.tw, twl);
     } else {
-        lds_fft<false, T, E>(Z, bd, d.f, d.tw, twl);
+        lds_fft<false, T, E, W>(Z, bd, d.f, d.tw, twl);
         if constexpr (OP == COL_FUSED || OP == COL_FUSED_T) {
             const int64_t off = outer * d.otf_outer + kx0;
             const float2* otf = d.otf + off;
@@ -328,7 +333,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
                 }
             }
             __syncthreads();
-            lds_fft<true, T, E>(Z, bd, d.f, d.tw, twl);
+            lds_fft<true, T, E, W>(Z, bd, d.f, d.tw, twl);
         }
     }
     int row0 = 0;
@@ -352,7 +357,7 @@ __device__ void col_pass_body(const ColPass& d, int g, float2* Z, int slot = 0) 
 template <int T, int E, int OP>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_pass_kernel(ColPass d) {
     extern __shared__ float2 lds[];
-    col_pass_body<T, E, OP>(d, xcd_block(blockIdx.x, gridDim.x), lds);
+    col_pass_body<T, E, OP, LUCEM_WAVES_PER_EU>(d, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -390,7 +395,7 @@ __device__ __forceinline__ int next_item(int* counter, int* slot) {
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void row_batch_kernel(RowBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_BATCH_WAVES_PER_EU))) void row_batch_kernel(RowBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
@@ -400,13 +405,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_P
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];  // a private copy: no aliasing
-        row_pass_body<T, E, OP>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
+        row_pass_body<T, E, OP, LUCEM_BATCH_WAVES_PER_EU>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
 }
 
 template <int T, int E, int OP>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_batch_kernel(ColBatch bt) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_BATCH_WAVES_PER_EU))) void col_batch_kernel(ColBatch bt) {
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
@@ -415,7 +420,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_P
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
-        col_pass_body<T, E, OP>(d, __builtin_amdgcn_readfirstlane(item.y), lds, bt.slot);
+        col_pass_body<T, E, OP, LUCEM_BATCH_WAVES_PER_EU>(d, __builtin_amdgcn_readfirstlane(item.y), lds, bt.slot);
         __syncthreads();
     }
 }
