@@ -432,6 +432,7 @@ int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* tier, int* vari
     if (*tier > 0) return 1;
     if (const ShapeChoice* c = find_choice(sp, SHAPE_ROW, L)) {
         *variant = c->variant;
+        *tier = c->wide;
         return c->lines;
     }
     const int64_t pairs = std::max<int64_t>((rows + 1) / 2, sp.launch_pairs);

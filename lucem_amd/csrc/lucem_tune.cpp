@@ -2,8 +2,10 @@
 // length it times candidate shapes on the device and keeps the fastest:
 //   - lines per workgroup (row pairs P, or columns B of a line pass), near the largest count that
 //     fills the workgroup;
-//   - the 256- or 512-thread line kernel;
+//   - the 256- or 512-thread kernel;
 //   - the radix sequence (greedy composite radices, the same in reverse order, or single radices).
+// Row shapes are timed on a ratio pass followed by an update pass (the plan's estimate stands in
+// for the observed image), line shapes on the pass itself.
 // Candidates run on the caller's own plan buffers (plan-owned workspace: the tuning launches
 // leave garbage that the captured sequence overwrites before it reads), each one warm-up launch
 // then the best of three timed launches (HIP events on the client's stream). Results are kept
@@ -174,27 +176,35 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         rgroups[r.f.L].push_back(r);
         rtotal += (double)r.rows * r.f.L;
     }
+    // One row launch runs one thread count (the batch launches every tile together): the group
+    // with the most work decides it and the others are tuned under it.
+    std::vector<std::pair<double, int>> rorder;
     for (auto& kv : rgroups) {
-        const int L = kv.first;
-        std::vector<RowPass>& g = kv.second;
         double work = 0.0;
+        for (const RowPass& r : kv.second) work += (double)r.rows * kv.first;
+        rorder.push_back({-work, kv.first});
+    }
+    std::sort(rorder.begin(), rorder.end());
+    int row_wide = -1;
+    for (const auto& ro : rorder) {
+        const int L = ro.second;
+        std::vector<RowPass>& g = rgroups[L];
+        const double work = -ro.first;
         int64_t pairs = 0;
-        for (const RowPass& r : g) {
-            work += (double)r.rows * L;
-            pairs += (r.rows + 1) / 2;
-        }
+        for (const RowPass& r : g) pairs += (r.rows + 1) / 2;
         if (work < 0.03 * rtotal || work < 2.0e5 || L > kLdsComplex) continue;
-        const CacheKey key{SHAPE_ROW, L, ROW_UPDATE, ilog2(pairs), persistent ? 1 : 0};
+        const CacheKey key{SHAPE_ROW * 4 + (row_wide < 0 ? 2 : row_wide), L, ROW_RATIO, ilog2(pairs),
+                           persistent ? 1 : 0};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
             load_cache_locked();
             auto it = g_cache.find(key);
             if (it != g_cache.end()) {
                 (*out)[{SHAPE_ROW, L}] = it->second;
+                if (row_wide < 0) row_wide = it->second.wide;
                 continue;
             }
         }
-        const int pmax = std::max(1, std::min(16, kLdsComplex / L));
         ShapeChoice best{};
         float best_ms = 1e30f;
         std::vector<FftPlan> plans;
@@ -204,16 +214,23 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             for (const FftPlan& q : plans) dup = dup || same_plan(q, f);
             if (dup) continue;
             plans.push_back(f);
-            for (int P : line_candidates(pmax, g[0].P)) {
+            for (int w = 0; w < 2; w++) {
+            // 256 threads, or 512 when that fits more row pairs into a workgroup
+            if (row_wide >= 0 && w != row_wide) continue;
+            const int pmax = std::max(1, std::min(16, (w ? kLdsComplexWide : kLdsComplex) / L));
+            if (w == 1 && row_wide < 0 && pmax <= std::max(1, std::min(16, kLdsComplex / L))) continue;
+            for (int P : line_candidates(pmax, w == 0 ? g[0].P : pmax)) {
                 std::vector<RowPass> d = g;
                 std::vector<int2> items;
                 for (size_t i = 0; i < d.size(); i++) {
                     d[i].P = P;
+                    d[i].wide = w;
                     d[i].f = f;
                     d[i].op = ROW_UPDATE;
-                    d[i].first = 1;
+                    d[i].first = 0;
                     d[i].last = 0;
                     d[i].clip = 0;
+                    d[i].obs = d[i].est;  // the ratio pass reads the plan's own estimate as "y"
                     const int nb = ((d[i].rows + 1) / 2 + P - 1) / P;
                     for (int q = 0; q < nb; q++) items.push_back(make_int2((int)i, q));
                 }
@@ -231,27 +248,37 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         return hip_fail(e, "hipMemcpy(autotune)");
                     if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
                         return hip_fail(e, "hipMemcpy(autotune)");
-                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, 0, lds) * c->num_cus));
-                    RowBatch rb{dd, di, (int)items.size(), ROW_UPDATE, 1, 0, 0, 0, cnt};
+                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, w, lds) * c->num_cus));
+                    // one ratio and one update pass, as an iteration runs them
+                    RowBatch ra{dd, di, (int)items.size(), ROW_RATIO, 0, 0, 0, w, cnt};
+                    RowBatch up{dd, di, (int)items.size(), ROW_UPDATE, 0, 0, 0, w, cnt + 1};
                     st = time_launch(s, timer, [&]() {
-                        int err = launch_zero_ints(cnt, 1, s);
-                        return err ? err : launch_row_batch(rb, grid, lds, s);
+                        int err = launch_zero_ints(cnt, 2, s);
+                        if (!err) err = launch_row_batch(ra, grid, lds, s);
+                        return err ? err : launch_row_batch(up, grid, lds, s);
                     }, &ms);
                 } else {
                     const int nb = (int)items.size();
-                    st = time_launch(s, timer, [&]() { return launch_row_pass(d[0], nb, s); }, &ms);
+                    RowPass ra = d[0];
+                    ra.op = ROW_RATIO;
+                    st = time_launch(s, timer, [&]() {
+                        int err = launch_row_pass(ra, nb, s);
+                        return err ? err : launch_row_pass(d[0], nb, s);
+                    }, &ms);
                 }
                 if (st) return st;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    best = ShapeChoice{P, 0, v};
+                    best = ShapeChoice{P, w, v};
                 }
+            }
             }
         }
         (*out)[{SHAPE_ROW, L}] = best;
+        if (row_wide < 0) row_wide = best.wide;
         if (tune_log())
-            std::fprintf(stderr, "lucem autotune: row L=%d pairs=%lld -> P=%d variant=%d (%.4f ms)\n", L,
-                         (long long)pairs, best.lines, best.variant, best_ms);
+            std::fprintf(stderr, "lucem autotune: row L=%d pairs=%lld -> P=%d wide=%d variant=%d (%.4f ms)\n", L,
+                         (long long)pairs, best.lines, best.wide, best.variant, best_ms);
         std::lock_guard<std::mutex> lk(g_cache_mu);
         store_locked(key, best);
     }
