@@ -401,6 +401,9 @@ lucem_status lucem::tuned_shapes(lucem_client* c, const Geometry& g, float* est,
     const std::vector<int> key{g.ndim, g.n[0], g.n[1], g.n[2], g.k[0], g.k[1], g.k[2]};
     auto it = c->tuned.find(key);
     if (it == c->tuned.end()) {
+        // measuring synchronises the stream: under stream capture keep the heuristics
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return LUCEM_OK;
         const ShapePolicy sp{c->num_cus, 0, 0, nullptr};
         int rb = 0;
         const RowPass r = make_row_desc(g, sp, tws[2], 1.0f, nullptr, est, nullptr, sa, &rb);
