@@ -3,8 +3,10 @@
 //
 // The plan gives every tile its own FFT geometry and workspace and every (FFT shape, PSF) pair
 // its own transfer functions. Each pass of the RL iteration (restore/deconvolve.rs:109-147:
-// blur, ratio, adjoint, update) becomes one persistent kernel over the work items of all tiles:
-// workgroups take (tile, block) items from an atomic counter, largest tiles first. The PSF
+// blur, ratio, adjoint, update) becomes one launch of a persistent-style kernel over the work
+// items of all tiles, largest tiles first: by default one workgroup per (tile, block) item;
+// LUCEM_BATCH_SCHEDULE=stride runs a grid of occupancy x CUs workgroups striding over the items,
+// =queue one taking them from an atomic counter. The PSF
 // preparation, the transfer-function builds and every iteration are captured once into a
 // hipGraph (the counters are cleared by a kernel at its start, not by a memset node); execute
 // uploads the caller's buffer pointers into the descriptors and replays the graph.
@@ -79,7 +81,11 @@ struct lucem_rl_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t launches = 0;
-    bool persistent = true;  // work-queue grid; false: one workgroup per item
+    // Schedule of the batch launches: one workgroup per item, placed by the hardware dispatcher
+    // (DIRECT, the default: measured 4-5% faster on the slide workload), or a persistent grid of
+    // occupancy x CUs workgroups striding over the items (STRIDE) or taking them from an atomic
+    // queue (QUEUE).
+    enum Schedule { QUEUE, STRIDE, DIRECT } schedule = DIRECT;
     ~lucem_rl_batch();
 };
 
@@ -166,7 +172,10 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
 
     auto b = std::make_unique<lucem_rl_batch>();
     b->client = client;
-    if (const char* sched = std::getenv("LUCEM_BATCH_SCHEDULE")) b->persistent = std::string(sched) != "direct";
+    if (const char* sched = std::getenv("LUCEM_BATCH_SCHEDULE")) {
+        const std::string v(sched);
+        b->schedule = v == "queue" ? lucem_rl_batch::QUEUE : v == "stride" ? lucem_rl_batch::STRIDE : lucem_rl_batch::DIRECT;
+    }
     b->settings = *settings;
     b->ndim = psf_ndim;
     for (uint64_t j = 0; j < num_psfs; j++)
@@ -408,7 +417,9 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
             for (int i = 0; i < ncorr; i++)
                 tcols.push_back(TuneCol{corr[0][i][t], tp.col_blocks[i] / std::max(corr[0][i][t].nkb, 1)});
         }
-        if ((st = tune_shapes(client, trows, tcols, b->persistent, &table))) return st;
+        if ((st = tune_shapes(client, trows, tcols, b->schedule == lucem_rl_batch::DIRECT ? TUNE_BATCH_DIRECT : TUNE_BATCH_GRID,
+                              &table)))
+            return st;
         bsp.table = &table;
         build_descs();
     }
@@ -456,7 +467,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         L.lds = lds;
         L.wide = wide;
         const int occ = batch_occupancy(row, wide, lds);
-        L.grid = b->persistent ? std::max(1, std::min(nitems, occ * client->num_cus)) : nitems;
+        L.grid = b->schedule != lucem_rl_batch::DIRECT ? std::max(1, std::min(nitems, occ * client->num_cus)) : nitems;
         seq.push_back(L);
     };
     auto max_lds = [&](bool row, const auto& descs, const std::vector<int2>& items) {
@@ -518,11 +529,13 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
         const Launch& L = seq[i];
         if (L.row) {
             RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, L.first, L.last, L.clip, L.wide,
-                        b->persistent ? P.counters + i : nullptr};
+                        b->schedule == lucem_rl_batch::QUEUE ? P.counters + i : nullptr,
+                        b->schedule == lucem_rl_batch::STRIDE ? 1 : 0};
             err = launch_row_batch(rb, L.grid, L.lds, b->capture);
         } else {
             ColBatch cb{static_cast<const ColPass*>(L.descs), L.items, L.nitems, L.op, L.wide,
-                        b->persistent ? P.counters + i : nullptr};
+                        b->schedule == lucem_rl_batch::QUEUE ? P.counters + i : nullptr, 0,
+                        b->schedule == lucem_rl_batch::STRIDE ? 1 : 0};
             err = launch_col_batch(cb, L.grid, L.lds, b->capture);
         }
     }

@@ -112,14 +112,15 @@ RowPass make_row_desc(const Geometry& g, const ShapePolicy& sp, const float2* tw
                       float* out, float2* spec, int* blocks);
 float rl_eps(const lucem_rl_settings& s);
 
-// Autotuner: times candidate shapes of these descriptors' passes (batch work-queue kernels when
-// `persistent`, else the plain kernels) and records the fastest per (kind, L) in `out`.
+// Autotuner: times candidate shapes of these descriptors' passes (the one-array kernels, or the
+// batch kernels scheduled as the batched form runs them) and records the fastest per (kind, L).
+enum TuneMode : int { TUNE_PLAIN = 0, TUNE_BATCH_DIRECT = 1, TUNE_BATCH_GRID = 2 };
 struct TuneCol {
     ColPass d;
     int outers;  // outer lines of the pass (work items = outers * column blocks)
 };
 lucem_status tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, const std::vector<TuneCol>& cols,
-                         bool persistent, ShapeTable* out);
+                         int mode, ShapeTable* out);
 lucem_status tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb, const float2* otf,
                           const float2* const tws[3], const ShapeTable** out);
 

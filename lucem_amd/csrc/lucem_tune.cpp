@@ -42,13 +42,13 @@ std::vector<int> line_candidates(int max_lines, int heuristic) {
     return v;
 }
 
-using CacheKey = std::tuple<int, int, int, int, int>;  // kind, L, op, log2(lines), persistent
+using CacheKey = std::tuple<int, int, int, int, int>;  // kind, L, op, log2(lines), mode
 std::mutex g_cache_mu;
 std::map<CacheKey, ShapeChoice> g_cache;
 bool g_cache_loaded = false;
 
 // LUCEM_TUNE_CACHE=<file> keeps the measured choices across processes: read once, appended to
-// after each new measurement (one line per key: kind L op log2lines persistent lines wide variant).
+// after each new measurement (one line per key: kind L op log2lines mode lines wide variant).
 const char* cache_file() {
     const char* f = std::getenv("LUCEM_TUNE_CACHE");
     return f && *f ? f : nullptr;
@@ -160,10 +160,11 @@ lucem_status time_launch(hipStream_t s, Timer& t, F fn, float* ms) {
 }  // namespace
 
 // Tunes every (kind, L) group of these descriptors that carries at least 3% of its kind's work.
-// persistent: time the work-queue batch kernels (as the batched form runs them); otherwise the
-// plain one-block-per-item kernels.
+// mode: TUNE_PLAIN times the one-array kernels on the first descriptor; TUNE_BATCH_DIRECT and
+// TUNE_BATCH_GRID time the batch kernels over all descriptors of a group, one workgroup per item or
+// a persistent grid striding over the items, as the batched form runs them.
 lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, const std::vector<TuneCol>& cols,
-                                bool persistent, ShapeTable* out) {
+                                int mode, ShapeTable* out) {
     if (!autotune_enabled()) return LUCEM_OK;
     hipStream_t s = c->stream;
     Scratch scratch;
@@ -194,7 +195,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         for (const RowPass& r : g) pairs += (r.rows + 1) / 2;
         if (work < 0.03 * rtotal || work < 2.0e5 || L > kLdsComplex) continue;
         const CacheKey key{SHAPE_ROW * 4 + (row_wide < 0 ? 2 : row_wide), L, ROW_RATIO, ilog2(pairs),
-                           persistent ? 1 : 0};
+                           mode};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
             load_cache_locked();
@@ -237,7 +238,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                 const size_t lds = row_pass_lds_bytes(L, P);
                 float ms = 0.f;
                 lucem_status st;
-                if (persistent) {
+                if (mode != TUNE_PLAIN) {
                     const size_t bd = sizeof(RowPass) * d.size(), bi = sizeof(int2) * items.size();
                     if ((st = scratch.reserve(bd + bi + 1024))) return st;
                     RowPass* dd = reinterpret_cast<RowPass*>(scratch.p);
@@ -248,13 +249,15 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         return hip_fail(e, "hipMemcpy(autotune)");
                     if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
                         return hip_fail(e, "hipMemcpy(autotune)");
-                    const int grid = std::max(1, std::min((int)items.size(), batch_occupancy(true, w, lds) * c->num_cus));
+                    const int grid = mode == TUNE_BATCH_DIRECT ? (int)items.size()
+                                                               : std::max(1, std::min((int)items.size(), batch_occupancy(true, w, lds) * c->num_cus));
                     // one ratio and one update pass, as an iteration runs them
-                    RowBatch ra{dd, di, (int)items.size(), ROW_RATIO, 0, 0, 0, w, cnt};
-                    RowBatch up{dd, di, (int)items.size(), ROW_UPDATE, 0, 0, 0, w, cnt + 1};
+                    (void)cnt;
+                    const int stride = mode == TUNE_BATCH_GRID ? 1 : 0;
+                    RowBatch ra{dd, di, (int)items.size(), ROW_RATIO, 0, 0, 0, w, nullptr, stride};
+                    RowBatch up{dd, di, (int)items.size(), ROW_UPDATE, 0, 0, 0, w, nullptr, stride};
                     st = time_launch(s, timer, [&]() {
-                        int err = launch_zero_ints(cnt, 2, s);
-                        if (!err) err = launch_row_batch(ra, grid, lds, s);
+                        int err = launch_row_batch(ra, grid, lds, s);
                         return err ? err : launch_row_batch(up, grid, lds, s);
                     }, &ms);
                 } else {
@@ -309,7 +312,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         for (const TuneCol& tc : g) lines += (int64_t)tc.outers * tc.d.Lc;
         const bool fixed_wide = kind_wide.count(kind) > 0;
         const int wide_key = fixed_wide ? kind_wide[kind] : 2;
-        const CacheKey key{kind * 4 + wide_key, L, g[0].d.op, ilog2(lines), persistent ? 1 : 0};
+        const CacheKey key{kind * 4 + wide_key, L, g[0].d.op, ilog2(lines), mode};
         {
             std::lock_guard<std::mutex> lk(g_cache_mu);
             load_cache_locked();
@@ -349,7 +352,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                     const size_t lds = col_pass_lds_bytes(L, B);
                     float ms = 0.f;
                     lucem_status st;
-                    if (persistent) {
+                    if (mode != TUNE_PLAIN) {
                         const size_t bd = sizeof(ColPass) * d.size(), bi = sizeof(int2) * items.size();
                         if ((st = scratch.reserve(bd + bi + 1024))) return st;
                         ColPass* dd = reinterpret_cast<ColPass*>(scratch.p);
@@ -361,13 +364,12 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                             return hip_fail(e, "hipMemcpy(autotune)");
                         if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
                             return hip_fail(e, "hipMemcpy(autotune)");
-                        const int grid =
-                            std::max(1, std::min((int)items.size(), batch_occupancy(false, w, lds) * c->num_cus));
-                        ColBatch cb{dd, di, (int)items.size(), d[0].op, w, cnt};
-                        st = time_launch(s, timer, [&]() {
-                            int err = launch_zero_ints(cnt, 1, s);
-                            return err ? err : launch_col_batch(cb, grid, lds, s);
-                        }, &ms);
+                        const int grid = mode == TUNE_BATCH_DIRECT
+                                             ? (int)items.size()
+                                             : std::max(1, std::min((int)items.size(), batch_occupancy(false, w, lds) * c->num_cus));
+                        (void)cnt;
+                        ColBatch cb{dd, di, (int)items.size(), d[0].op, w, nullptr, 0, mode == TUNE_BATCH_GRID ? 1 : 0};
+                        st = time_launch(s, timer, [&]() { return launch_col_batch(cb, grid, lds, s); }, &ms);
                     } else {
                         const int nb = (int)items.size();
                         st = time_launch(s, timer, [&]() { return launch_col_pass(d[0], nb, s); }, &ms);
@@ -413,7 +415,7 @@ lucem_status lucem::tuned_shapes(lucem_client* c, const Geometry& g, float* est,
         std::vector<TuneCol> cols;
         for (int i = 0; i < n; i++) cols.push_back(TuneCol{d[i], b[i] / std::max(d[i].nkb, 1)});
         ShapeTable t;
-        lucem_status st = tune_shapes(c, std::vector<RowPass>{r}, cols, false, &t);
+        lucem_status st = tune_shapes(c, std::vector<RowPass>{r}, cols, TUNE_PLAIN, &t);
         if (st) return st;
         it = c->tuned.emplace(key, t).first;
     }

@@ -122,7 +122,8 @@ struct RowBatch {
     int nitems;
     int op, first, last, clip;  // overrides of the per-tile fields for this launch
     int wide;
-    int* counter;
+    int* counter;           // work queue (persistent grid), or null
+    int stride;             // counter null: 1 = persistent grid striding over the items, 0 = one item per block
 };
 
 struct ColBatch {
@@ -131,8 +132,9 @@ struct ColBatch {
     int nitems;
     int op;
     int wide;
-    int* counter;
+    int* counter;           // work queue (persistent grid), or null
     int slot;               // differentiable form: storage slot of save / gacc operands
+    int stride;             // counter null: 1 = persistent grid striding over the items, 0 = one item per block
 };
 
 struct PsfBatch {

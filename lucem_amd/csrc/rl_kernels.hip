@@ -399,9 +399,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_BATCH_W
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
-        // persistent: items from the atomic queue; direct (counter == nullptr): one item per block
-        const int it = bt.counter ? next_item(bt.counter, &slot)
-                                  : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
+        // items from the atomic queue, or a static stride over them (persistent grid), or one
+        // item per block (direct)
+        const int it = bt.counter  ? next_item(bt.counter, &slot)
+                       : bt.stride ? xcd_block(blockIdx.x, gridDim.x) + round * (int)gridDim.x
+                                   : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];  // a private copy: no aliasing
@@ -415,8 +417,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_BATCH_W
     extern __shared__ float2 lds[];
     __shared__ int slot;
     for (int round = 0;; round++) {
-        const int it = bt.counter ? next_item(bt.counter, &slot)
-                                  : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
+        const int it = bt.counter  ? next_item(bt.counter, &slot)
+                       : bt.stride ? xcd_block(blockIdx.x, gridDim.x) + round * (int)gridDim.x
+                                   : (round == 0 ? xcd_block(blockIdx.x, gridDim.x) : bt.nitems);
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const ColPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];
