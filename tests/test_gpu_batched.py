@@ -152,3 +152,23 @@ def test_batched_long_and_short_tiles(gpu, oracle):
         st, ref = oracle.richardson_lucy(y, psf, 3, clip=False)
         assert st == 0
         assert oracle.rel_l2(o.cpu().numpy(), ref) < 1e-4
+
+
+def test_batched_with_empty_tiles(gpu, oracle):
+    """Empty tiles (a zero extent, as the reference's empty input) are skipped; the others are
+    deconvolved as usual."""
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import richardson_lucy_batched
+
+    rng = np.random.default_rng(93)
+    shapes = [(0, 16), (20, 24), (5, 0), (13, 9)]
+    ys = [torch.from_numpy(rng.uniform(50, 150, s).astype(np.float32)).cuda() for s in shapes]
+    psf = rng.uniform(0.1, 1.0, (3, 3)).astype(np.float32)
+    outs = richardson_lucy_batched(ys, [torch.from_numpy(psf).cuda()], [0] * 4, LA.RichardsonLucy(num_iter=4, clip=False))
+    for y, o in zip(ys, outs):
+        assert tuple(o.shape) == tuple(y.shape)
+        if y.numel():
+            st, want = oracle.richardson_lucy(y.cpu().numpy(), psf, 4, clip=False)
+            assert oracle.rel_l2(o.cpu().numpy(), want) < RTOL_L2
