@@ -192,14 +192,27 @@ inline OpsResult<Unit> richardson_lucy_into(const Client& client, const DeviceAr
         (uint32_t)output.dims().size(), &s, ws.handle()));
 }
 
-// richardson_lucy::<f32> (deconvolve.rs:152-162): allocates the output and a workspace.
-inline OpsResult<DeviceArray<float>> richardson_lucy(const Client& client, const DeviceArray<float>& input,
-                                                     const DeviceArray<float>& psf, const RichardsonLucy& settings) {
-    auto out = DeviceArray<float>::empty(client, input.dims());
+// richardson_lucy_into::<f64> (deconvolve.rs:45-149 with F = f64): FP64 direct correlations, 2-D.
+inline OpsResult<Unit> richardson_lucy_into(const Client& client, const DeviceArray<double>& input,
+                                            const DeviceArray<double>& psf, const DeviceArray<double>& output,
+                                            const RichardsonLucy& settings, Workspace& ws) {
+    const lucem_rl_settings s = settings.to_c();
+    return status_result(lucem_richardson_lucy_into_f64(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
+        psf.dims().data(), (uint32_t)psf.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+// richardson_lucy::<F> (deconvolve.rs:152-162), F = float or double: allocates the output and a
+// workspace.
+template <class F>
+inline OpsResult<DeviceArray<F>> richardson_lucy(const Client& client, const DeviceArray<F>& input,
+                                                 const DeviceArray<F>& psf, const RichardsonLucy& settings) {
+    auto out = DeviceArray<F>::empty(client, input.dims());
     if (!out.is_ok()) return out;
     Workspace ws(client);
     auto r = richardson_lucy_into(client, input, psf, out.unwrap(), settings, ws);
-    if (!r.is_ok()) return OpsResult<DeviceArray<float>>(r.status(), r.message());
+    if (!r.is_ok()) return OpsResult<DeviceArray<F>>(r.status(), r.message());
     client.sync();  // the workspace is released here; its buffers must not be in use
     return out;
 }

@@ -20,6 +20,8 @@
  *   lucem_richardson_lucy_f32
  *                         lucem_ops::restore::richardson_lucy::<f32> (deconvolve.rs:152-162): the
  *                         output is caller-allocated here, the workspace is the client's own.
+ *   lucem_richardson_lucy_into_f64 / lucem_richardson_lucy_f64
+ *                         the same two for F = f64 (deconvolve.rs:45-162)
  *   lucem_richardson_lucy_nd_into_f32
  *                         the same operator for 2-D or 3-D arrays (no reference counterpart: the
  *                         reference accepts 2-D only, deconvolve.rs:55-56)
@@ -140,6 +142,17 @@ lucem_status lucem_richardson_lucy_f32(lucem_client* client, const float* input,
                                        uint32_t ndim, const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim,
                                        float* output, const lucem_rl_settings* settings);
 
+/* richardson_lucy_into::<f64> (deconvolve.rs:45-149 with F = f64): the same checks in the same
+ * order, FP64 arithmetic throughout, direct correlations (filter/kernels/convolve.rs:61-118) rather
+ * than the FFT form of the f32 path. 2-D only, as the reference. */
+lucem_status lucem_richardson_lucy_into_f64(lucem_client* client, const double* input, const uint64_t* in_dims,
+                                            uint32_t in_ndim, const double* psf, const uint64_t* psf_dims,
+                                            uint32_t psf_ndim, double* output, const uint64_t* out_dims,
+                                            uint32_t out_ndim, const lucem_rl_settings* settings, lucem_workspace* ws);
+/* richardson_lucy::<f64> with a caller-allocated output and the client's internal workspace. */
+lucem_status lucem_richardson_lucy_f64(lucem_client* client, const double* input, const uint64_t* dims, uint32_t ndim,
+                                       const double* psf, const uint64_t* psf_dims, uint32_t psf_ndim, double* output,
+                                       const lucem_rl_settings* settings);
 /* The same operator for 2-D or 3-D arrays (psf of the same rank as the input). */
 lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                                uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,

@@ -40,7 +40,8 @@ EXPORTED = [
     "lucem_client_create", "lucem_client_destroy", "lucem_client_sync", "lucem_client_stream",
     "lucem_client_num_cus", "lucem_workspace_create", "lucem_workspace_destroy",
     "lucem_workspace_capacity_bytes", "lucem_richardson_lucy_into_f32", "lucem_richardson_lucy_f32",
-    "lucem_richardson_lucy_nd_into_f32", "lucem_fft_c2c_lines_f32", "lucem_client_profile",
+    "lucem_richardson_lucy_nd_into_f32", "lucem_richardson_lucy_into_f64", "lucem_richardson_lucy_f64",
+    "lucem_fft_c2c_lines_f32", "lucem_client_profile",
     "lucem_client_profile_read", "lucem_rl_batch_create", "lucem_rl_batch_execute", "lucem_rl_batch_destroy",
     "lucem_rl_batch_device_bytes", "lucem_rl_batch_launches", "lucem_richardson_lucy_batched_f32",
 ]
@@ -109,6 +110,8 @@ def lib() -> ctypes.CDLL:
             L.lucem_richardson_lucy_nd_into_f32.argtypes = rl
             L.lucem_richardson_lucy_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                     _vp, ctypes.POINTER(RlSettings)]
+            L.lucem_richardson_lucy_into_f64.argtypes = rl
+            L.lucem_richardson_lucy_f64.argtypes = L.lucem_richardson_lucy_f32.argtypes
             L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
@@ -291,14 +294,15 @@ def as_tensor(x):
     raise TypeError(f"expected a torch tensor or a DLPack-exporting array, got {type(x).__name__}")
 
 
-def device_ptr(t, what: str) -> int:
-    """Raw device pointer of a contiguous float32 HIP tensor."""
+def device_ptr(t, what: str, dtype=None) -> int:
+    """Raw device pointer of a contiguous HIP tensor of ``dtype`` (default float32)."""
     import torch
 
+    dtype = torch.float32 if dtype is None else dtype
     if not t.is_cuda:
         raise ValueError(f"{what} must be a device (HIP) tensor")
-    if t.dtype != torch.float32:
-        raise ValueError(f"{what} must be float32, got {t.dtype}")
+    if t.dtype != dtype:
+        raise ValueError(f"{what} must be {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{what} must be C-contiguous")
     return int(t.data_ptr())

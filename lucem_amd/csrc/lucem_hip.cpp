@@ -777,10 +777,10 @@ lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* in
 // ---------------------------------------------------------------------------------------------
 // validation in the reference's order (deconvolve.rs:53-71)
 // ---------------------------------------------------------------------------------------------
-lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
-                                const uint64_t* psf_dims, uint32_t psf_ndim, const float* output,
+lucem_status lucem::validate_rl(const void* input, const uint64_t* in_dims, uint32_t in_ndim, const void* psf,
+                                const uint64_t* psf_dims, uint32_t psf_ndim, const void* output,
                                 const uint64_t* out_dims, uint32_t out_ndim, const lucem_rl_settings* s,
-                                uint32_t max_ndim, bool* empty, Geometry* g) {
+                                uint32_t max_ndim, bool* empty, Geometry* g, size_t elem_size, bool fft) {
     (void)psf;
     *empty = false;
     if (!s) return fail(LUCEM_INVALID_HANDLE, "settings is NULL");
@@ -800,7 +800,7 @@ lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uin
     if (n > 0 && !overflow && input && output) {
         const char* a = (const char*)input;
         const char* b = (const char*)output;
-        const uint64_t bytes = n * sizeof(float);
+        const uint64_t bytes = n * elem_size;
         if (a < b + bytes && b < a + bytes)
             return fail(LUCEM_ALIASED_BUFFERS, "input and output arrays must not share the same device buffer");
     }
@@ -831,6 +831,16 @@ lucem_status lucem::validate_rl(const float* input, const uint64_t* in_dims, uin
     int64_t n3[3] = {1, 1, 1}, k3[3] = {1, 1, 1};
     for (uint32_t i = 0; i < in_ndim; i++) n3[3 - in_ndim + i] = (int64_t)in_dims[i];
     for (uint32_t i = 0; i < psf_ndim; i++) k3[3 - psf_ndim + i] = (int64_t)psf_dims[i];
+    if (!fft) {  // the direct (f64) form: sizes only
+        std::memset(g, 0, sizeof *g);
+        g->ndim = (int)in_ndim;
+        for (int a = 0; a < 3; a++) {
+            g->n[a] = (int)n3[a];
+            g->k[a] = (int)k3[a];
+        }
+        g->mode = (int)s->mode;
+        return LUCEM_OK;
+    }
     if (!make_geometry(n3, k3, (int)in_ndim, g))
         return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with psf " +
                                                  dims_str(psf_dims, psf_ndim) +
@@ -881,6 +891,79 @@ extern "C" lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, 
                                                           lucem_workspace* ws) {
     return rl_entry(client, input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim, settings,
                     ws, 3);
+}
+
+// ---------------------------------------------------------------------------------------------
+// richardson_lucy_into::<f64>: the reference iteration (deconvolve.rs:92-147) with direct FP64
+// correlations: est_a = 0.5; per iteration conv = correlate(src, rev), rel = y / safe(conv),
+// err = correlate(rel, h), dst = src * err (clip on the last), src/dst ping-ponging between
+// est_a and est_b and the last iteration writing the output.
+// ---------------------------------------------------------------------------------------------
+static lucem_status rl_run_f64(lucem_client* c, lucem_workspace* ws, const double* input, const double* psf,
+                               double* output, const Geometry& g, const lucem_rl_settings& s) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const int H = g.n[1], W = g.n[2], kh = g.k[1], kw = g.k[2];
+    const int64_t n = (int64_t)H * W, nk = (int64_t)kh * kw;
+    double *a, *b, *conv, *rel, *err, *h, *rev;
+    lucem_status st;
+    if ((st = ws_slot(ws, "rl64_est_a", sizeof(double) * n, (void**)&a)) ||
+        (st = ws_slot(ws, "rl64_est_b", sizeof(double) * n, (void**)&b)) ||
+        (st = ws_slot(ws, "rl64_est_conv", sizeof(double) * n, (void**)&conv)) ||
+        (st = ws_slot(ws, "rl64_rel_blur", sizeof(double) * n, (void**)&rel)) ||
+        (st = ws_slot(ws, "rl64_err_est", sizeof(double) * n, (void**)&err)) ||
+        (st = ws_slot(ws, "rl64_psf_h", sizeof(double) * nk, (void**)&h)) ||
+        (st = ws_slot(ws, "rl64_psf_rev", sizeof(double) * nk, (void**)&rev)))
+        return st;
+    void* sm = c->stream;
+    if ((st = launch_check(launch_psf_prep_f64(psf, h, rev, nk, sm), "psf prep (f64)"))) return st;
+    if ((st = launch_check(launch_fill_f64(a, 0.5, n, sm), "fill (f64)"))) return st;
+    const double eps = std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
+    bool ping = true;
+    for (uint64_t it = 0; it < s.num_iter; it++) {
+        const bool last = it + 1 == s.num_iter;
+        const double* src = ping ? a : b;
+        double* dst = last ? output : (ping ? b : a);
+        if ((st = launch_check(launch_correlate2d_f64(src, rev, conv, H, W, kh, kw, g.mode, sm), "correlate (f64)")) ||
+            (st = launch_check(launch_ratio_f64(input, conv, rel, eps, n, sm), "ratio (f64)")) ||
+            (st = launch_check(launch_correlate2d_f64(rel, h, err, H, W, kh, kw, g.mode, sm), "correlate (f64)")) ||
+            (st = launch_check(launch_update_f64(src, err, dst, last && s.clip ? 1 : 0, n, sm), "update (f64)")))
+            return st;
+        ping = !ping;
+    }
+    return LUCEM_OK;
+}
+
+static lucem_status rl_entry_f64(lucem_client* client, const double* input, const uint64_t* in_dims, uint32_t in_ndim,
+                                 const double* psf, const uint64_t* psf_dims, uint32_t psf_ndim, double* output,
+                                 const uint64_t* out_dims, uint32_t out_ndim, const lucem_rl_settings* settings,
+                                 lucem_workspace* ws) {
+    bool empty = false;
+    Geometry g;
+    lucem_status st = validate_rl(input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim,
+                                  settings, 2, &empty, &g, sizeof(double), false);
+    if (st != LUCEM_OK || empty) return st;
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    if (!input || !psf || !output) return fail(LUCEM_INVALID_HANDLE, "input, psf or output pointer is NULL");
+    if (!ws) ws = client->internal_ws.get();
+    if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
+    return rl_run_f64(client, ws, input, psf, output, g, *settings);
+}
+
+extern "C" lucem_status lucem_richardson_lucy_into_f64(lucem_client* client, const double* input,
+                                                       const uint64_t* in_dims, uint32_t in_ndim, const double* psf,
+                                                       const uint64_t* psf_dims, uint32_t psf_ndim, double* output,
+                                                       const uint64_t* out_dims, uint32_t out_ndim,
+                                                       const lucem_rl_settings* settings, lucem_workspace* ws) {
+    return rl_entry_f64(client, input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim,
+                        settings, ws);
+}
+
+extern "C" lucem_status lucem_richardson_lucy_f64(lucem_client* client, const double* input, const uint64_t* dims,
+                                                  uint32_t ndim, const double* psf, const uint64_t* psf_dims,
+                                                  uint32_t psf_ndim, double* output,
+                                                  const lucem_rl_settings* settings) {
+    return rl_entry_f64(client, input, dims, ndim, psf, psf_dims, psf_ndim, output, dims, ndim, settings, nullptr);
 }
 
 // Batched complex line FFTs (contiguous lines of length L, interleaved re/im).

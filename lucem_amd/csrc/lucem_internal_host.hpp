@@ -124,9 +124,10 @@ lucem_status tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, cons
 lucem_status tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb, const float2* otf,
                           const float2* const tws[3], const ShapeTable** out);
 
-lucem_status validate_rl(const float* input, const uint64_t* in_dims, uint32_t in_ndim, const float* psf,
-                         const uint64_t* psf_dims, uint32_t psf_ndim, const float* output, const uint64_t* out_dims,
-                         uint32_t out_ndim, const lucem_rl_settings* s, uint32_t max_ndim, bool* empty, Geometry* g);
+lucem_status validate_rl(const void* input, const uint64_t* in_dims, uint32_t in_ndim, const void* psf,
+                         const uint64_t* psf_dims, uint32_t psf_ndim, const void* output, const uint64_t* out_dims,
+                         uint32_t out_ndim, const lucem_rl_settings* s, uint32_t max_ndim, bool* empty, Geometry* g,
+                         size_t elem_size = sizeof(float), bool fft = true);
 lucem_status rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
                     const Geometry& g, const lucem_rl_settings& s);
 

@@ -152,6 +152,13 @@ int launch_psf_batch(const PsfBatch& b, void* stream);
 int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
 int batch_occupancy(bool row, int tier, size_t lds);
+// Double-precision direct form (richardson_lucy_into::<f64>).
+int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int mode,
+                           void* stream);
+int launch_ratio_f64(const double* obs, const double* c, double* rel, double eps, int64_t n, void* stream);
+int launch_update_f64(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
+int launch_fill_f64(double* p, double v, int64_t n, void* stream);
+int launch_psf_prep_f64(const double* psf, double* h, double* rev, int64_t nk, void* stream);
 // Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
 int allow_big_lds_kernels();
 size_t row_pass_lds_bytes(int L, int P);

@@ -62,7 +62,7 @@ class RichardsonLucy:
         return s
 
 
-def _call(fn, input, psf, output, settings: RichardsonLucy, ws, client):
+def _call(fn, input, psf, output, settings: RichardsonLucy, ws, client, dtype=None):
     x = N.as_tensor(input)
     p = N.as_tensor(psf)
     o = N.as_tensor(output)
@@ -70,9 +70,9 @@ def _call(fn, input, psf, output, settings: RichardsonLucy, ws, client):
         client = N.default_client(x.device.index if x.is_cuda else None)
     st = settings.to_c()
     args_ok = x.numel() == 0 or (x.is_cuda and p.is_cuda and o.is_cuda)
-    xp = N.device_ptr(x, "input") if x.numel() else 0
-    pp = N.device_ptr(p, "psf") if p.numel() else 0
-    op = N.device_ptr(o, "output") if o.numel() else 0
+    xp = N.device_ptr(x, "input", dtype) if x.numel() else 0
+    pp = N.device_ptr(p, "psf", dtype) if p.numel() else 0
+    op = N.device_ptr(o, "output", dtype) if o.numel() else 0
     if not args_ok:
         raise ValueError("input, psf and output must be device tensors")
     N.check(fn(client.handle, N._vp(xp or None), N.dims(x.shape), x.dim(), N._vp(pp or None), N.dims(p.shape),
@@ -84,8 +84,16 @@ def richardson_lucy_into(input, psf, output, settings: RichardsonLucy | None = N
                          client: N.Client | None = None) -> None:
     """Restores 2-D ``input`` blurred by 2-D ``psf`` into ``output`` (deconvolve.rs:45-149).
 
-    Enqueues on the client's stream (by default torch's current stream) and returns.
+    Enqueues on the client's stream (by default torch's current stream) and returns. float32
+    tensors take the FFT path; float64 tensors (input, psf and output alike, as ``F`` is one type
+    in the reference) take the FP64 direct path (``richardson_lucy_into::<f64>``).
     """
+    import torch
+
+    if N.as_tensor(input).dtype == torch.float64:
+        _call(N.lib().lucem_richardson_lucy_into_f64, input, psf, output, settings or RichardsonLucy(), ws, client,
+              torch.float64)
+        return
     _call(N.lib().lucem_richardson_lucy_into_f32, input, psf, output, settings or RichardsonLucy(), ws, client)
 
 
@@ -101,7 +109,17 @@ def richardson_lucy(input, psf, settings: RichardsonLucy | None = None, client: 
 
 def richardson_lucy_nd_into(input, psf, output, settings: RichardsonLucy | None = None,
                             ws: N.Workspace | None = None, client: N.Client | None = None) -> None:
-    """The same operator on 2-D or 3-D arrays ``[z, y, x]`` (the PSF has the input's rank)."""
+    """The same operator on 2-D or 3-D arrays ``[z, y, x]`` (the PSF has the input's rank).
+
+    float64 tensors take the FP64 form, which is 2-D as the reference's operator is: a 3-D float64
+    input is refused with ``INVALID_DIMENSION``.
+    """
+    import torch
+
+    if N.as_tensor(input).dtype == torch.float64:
+        _call(N.lib().lucem_richardson_lucy_into_f64, input, psf, output, settings or RichardsonLucy(), ws, client,
+              torch.float64)
+        return
     _call(N.lib().lucem_richardson_lucy_nd_into_f32, input, psf, output, settings or RichardsonLucy(), ws, client)
 
 

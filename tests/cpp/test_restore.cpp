@@ -63,6 +63,13 @@ int main() {
     s2.clip = true;
     auto restored = richardson_lucy(client, bl, box, s2).unwrap().to_host(client);
     CHECK(restored[36] > blurred[36] + 0.15f, "peak %f vs blurred %f", restored[36], blurred[36]);
+    // the f64 instance: the identity KAT in FP64 holds to rounding
+    std::vector<double> img64(img.begin(), img.end()), delta64(delta.begin(), delta.end());
+    auto in64 = DeviceArray<double>::from_host(client, {8, 8}, img64).unwrap();
+    auto psf64 = DeviceArray<double>::from_host(client, {3, 3}, delta64).unwrap();
+    auto out64 = richardson_lucy(client, in64, psf64, s).unwrap().to_host(client);
+    for (int i = 0; i < 64; i++)
+        CHECK(std::fabs(out64[i] - img64[i]) <= 1e-12, "f64 identity index %d: %.17g vs %.17g", i, out64[i], img64[i]);
     // error order
     Workspace ws(client);
     auto out86 = DeviceArray<float>::empty(client, {8, 6}).unwrap();

@@ -124,3 +124,25 @@ def test_oracle_is_not_imported_by_the_product():
     for p in (ROOT / "lucem_amd").rglob("*.py"):
         text = p.read_text()
         assert "oracle" not in re.sub(r"#.*", "", text).replace("oracle/", ""), p
+
+
+def test_check_order_f64_without_device(L):
+    from lucem_amd import _native as N
+
+    s = N.RlSettings()
+    L.lucem_rl_settings_default(ctypes.byref(s))
+
+    def call(in_dims, psf_dims, out_dims, out=0x20000000):
+        return L.lucem_richardson_lucy_into_f64(None, N._vp(0x10000), N.dims(in_dims), len(in_dims),
+                                                N._vp(0x30000000), N.dims(psf_dims), len(psf_dims), N._vp(out),
+                                                N.dims(out_dims), len(out_dims), ctypes.byref(s), None)
+
+    assert call((4, 5), (3, 3), (4, 6)) == 1
+    # 8-byte elements: 4 x 5 doubles starting at 0x10000 overlap an output 80 bytes on
+    assert call((4, 5), (3, 3), (4, 5), out=0x10000 + 80) == 2
+    assert call((4, 5), (3, 3), (4, 5), out=0x10000 + 160) == 11
+    assert call((2, 4, 5), (3, 3, 3), (2, 4, 5)) == 3
+    assert call((4, 5), (0, 3), (4, 5)) == 4
+    assert call((0, 5), (3, 3), (0, 5)) == 0
+    # the direct form has no FFT length limit
+    assert call((8, 16400), (3, 3), (8, 16400)) == 11

@@ -307,3 +307,73 @@ def test_longer_than_16384_is_refused(gpu):
     y = torch.ones((2, 16400), device="cuda")
     with pytest.raises(Exception, match="16384"):
         LA.richardson_lucy_into(y, torch.ones((1, 3), device="cuda"), torch.empty_like(y), LA.RichardsonLucy(num_iter=2))
+
+
+# richardson_lucy_into::<f64>: FP64 direct correlations in the reference's accumulation order, so
+# the result agrees with the FP64 oracle to rounding (FMA contraction and the PSF sum order aside).
+RTOL_F64 = 1e-12
+F64_CASES = [
+    ((37, 53), (5, 7), 0, False),
+    ((64, 64), (3, 3), 1, True),
+    ((29, 41), (4, 6), 2, False),
+    ((50, 33), (9, 2), 3, True),
+    ((45, 45), (7, 7), 4, False),
+    ((1, 30), (3, 3), 0, False),
+    ((100, 90), (81, 81), 0, False),  # the input tile with its halo exceeds 64 KiB of LDS
+]
+
+
+@pytest.mark.parametrize("shape,kshape,mode,clip", F64_CASES)
+def test_rl_f64_matches_fp64_oracle(gpu, oracle, shape, kshape, mode, clip):
+    import torch
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(shape[0] * 31 + kshape[1] + mode)
+    y = rng.uniform(0.05, 0.95, shape) if clip else rng.uniform(50, 150, shape)
+    p = rng.uniform(0.1, 1.0, kshape)
+    out = torch.empty(shape, dtype=torch.float64, device="cuda")
+    LA.richardson_lucy_into(torch.from_numpy(y).cuda(), torch.from_numpy(p).cuda(), out,
+                            LA.RichardsonLucy(num_iter=6, clip=clip, mode=mode))
+    torch.cuda.synchronize()
+    st, ref = oracle.richardson_lucy(y, p, 6, clip=clip, mode=mode)
+    assert st == 0 and ref.dtype == np.float64
+    e = oracle.rel_l2(out.cpu().numpy(), ref)
+    print(f"f64 {shape} psf {kshape} mode {mode}: rel-L2 vs FP64 {e:.2e}")
+    assert e < RTOL_F64
+
+
+def test_rl_f64_filter_epsilon_and_zeros(gpu, oracle):
+    import torch
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(71)
+    y = rng.uniform(0.0, 2.0, (40, 40))
+    y[10:30, 5:25] = 0.0
+    p = rng.uniform(0.0, 1.0, (5, 5))
+    p[2, :] = 0.0
+    out = torch.empty_like(torch.from_numpy(y)).cuda()
+    LA.richardson_lucy_into(torch.from_numpy(y).cuda(), torch.from_numpy(p).cuda(), out,
+                            LA.RichardsonLucy(num_iter=8, clip=False, mode=LA.PadMode.Constant, filter_epsilon=1e-3))
+    torch.cuda.synchronize()
+    st, ref = oracle.richardson_lucy(y, p, 8, clip=False, filter_epsilon=1e-3, mode=1)
+    assert st == 0
+    assert oracle.rel_l2(out.cpu().numpy(), ref) < RTOL_F64
+
+
+def test_rl_f64_errors(gpu):
+    import torch
+
+    import lucem_amd as LA
+
+    y = torch.ones((8, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="float64"):
+        LA.richardson_lucy_into(y, torch.ones((3, 3), device="cuda"), torch.empty_like(y))
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_into(y, torch.ones((3, 3), dtype=torch.float64, device="cuda"), y)
+    assert e.value.kind == "ALIASED_BUFFERS"
+    v = torch.ones((2, 8, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(LA.LucemError) as e:
+        LA.richardson_lucy_nd_into(v, torch.ones((3, 3, 3), dtype=torch.float64, device="cuda"), torch.empty_like(v))
+    assert e.value.kind == "INVALID_DIMENSION"
