@@ -31,6 +31,14 @@
  *                         crates/lucem-core/src/tiling.rs:13-23); no reference counterpart
  *   lucem_rl_grad_*       differentiable unrolled RL (forward with checkpoints, adjoint); no
  *                         reference counterpart
+ *   lucem_wiener_settings lucem_ops::restore::WienerDeconvolve (deconvolve.rs:164-181)
+ *   lucem_wiener_deconvolve_into_f32 / lucem_wiener_deconvolve_f32
+ *                         wiener_deconvolve_into / wiener_deconvolve (deconvolve.rs:213-327)
+ *   lucem_unsupervised_wiener_settings
+ *                         lucem_ops::restore::UnsupervisedWiener (deconvolve.rs:330-353)
+ *   lucem_unsupervised_wiener_f32
+ *                         unsupervised_wiener (deconvolve.rs:357-509): output caller-allocated,
+ *                         the balance chain written to a caller host buffer
  */
 #ifndef LUCEM_HIP_H
 #define LUCEM_HIP_H
@@ -153,6 +161,56 @@ lucem_status lucem_richardson_lucy_into_f64(lucem_client* client, const double* 
 lucem_status lucem_richardson_lucy_f64(lucem_client* client, const double* input, const uint64_t* dims, uint32_t ndim,
                                        const double* psf, const uint64_t* psf_dims, uint32_t psf_ndim, double* output,
                                        const lucem_rl_settings* settings);
+/* Settings of the regularised Wiener filter, WienerDeconvolve (deconvolve.rs:164-181).
+ * Defaults (lucem_wiener_settings_default): balance 0.1, clip 1. */
+typedef struct lucem_wiener_settings {
+    double balance;
+    int32_t clip;
+    uint32_t reserved;
+} lucem_wiener_settings;
+void lucem_wiener_settings_default(lucem_wiener_settings* out);
+
+/* wiener_deconvolve_into::<f32> (deconvolve.rs:213-315): X = conj(H) Y / max(|H|^2 + balance |L|^2,
+ * 1e-12) on the circular H x W domain of the image, L the centred 3x3 Laplacian, then the
+ * inverse transform and the optional clip to [-1, 1]. Checks in the reference's order:
+ * same_shape, not_aliased, input 2-D, psf 2-D, psf non-empty, balance finite and >= 0, empty
+ * input (OK), psf no larger than the image (LUCEM_INVALID_WEIGHTS). Both image sizes must be
+ * 7-smooth and at most 16384 (LUCEM_UNSUPPORTED_SHAPE otherwise). Asynchronous on the client's
+ * stream. */
+lucem_status lucem_wiener_deconvolve_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                              uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
+                                              uint32_t psf_ndim, float* output, const uint64_t* out_dims,
+                                              uint32_t out_ndim, const lucem_wiener_settings* settings,
+                                              lucem_workspace* ws);
+/* wiener_deconvolve::<f32> (deconvolve.rs:318-327) with a caller-allocated output. */
+lucem_status lucem_wiener_deconvolve_f32(lucem_client* client, const float* input, const uint64_t* dims, uint32_t ndim,
+                                         const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim, float* output,
+                                         const lucem_wiener_settings* settings);
+
+/* Settings of the unsupervised Wiener filter, UnsupervisedWiener (deconvolve.rs:330-353).
+ * Defaults (lucem_unsupervised_wiener_settings_default): num_iter 30, burnin 10, clip 1, seed 0. */
+typedef struct lucem_unsupervised_wiener_settings {
+    uint64_t num_iter;
+    uint64_t burnin;
+    int32_t clip;
+    uint32_t reserved;
+    uint64_t seed;
+} lucem_unsupervised_wiener_settings;
+void lucem_unsupervised_wiener_settings_default(lucem_unsupervised_wiener_settings* out);
+
+/* unsupervised_wiener::<f32> (deconvolve.rs:357-509): the empirical-Bayes loop over num_iter
+ * iterations runs on the device; `output` (the input's dims) receives the posterior-mean image
+ * and `balance_chain` (host memory, room for num_iter values) the balance of every iteration;
+ * *chain_len is set to num_iter (0 for an empty input). Checks in the reference's order: input
+ * 2-D, psf 2-D, psf non-empty, num_iter >= 1 and > burnin, empty input (OK); then
+ * not_aliased, chain capacity (LUCEM_INVALID_PARAMETER), psf no larger than the image, sizes as
+ * for the Wiener filter. Returns after the chain is on the host (synchronises the stream). */
+lucem_status lucem_unsupervised_wiener_f32(lucem_client* client, const float* input, const uint64_t* dims,
+                                           uint32_t ndim, const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim,
+                                           float* output, const lucem_unsupervised_wiener_settings* settings,
+                                           double* balance_chain, uint64_t chain_capacity, uint64_t* chain_len,
+                                           lucem_workspace* ws);
+
 /* The same operator for 2-D or 3-D arrays (psf of the same rank as the input). */
 lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                                uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,

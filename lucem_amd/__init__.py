@@ -7,6 +7,9 @@ Public surface (mirrors the reference's names):
                                                    restore/deconvolve.rs:19-41
 - :func:`richardson_lucy_into`, :func:`richardson_lucy`
                                                    restore/deconvolve.rs:45-162
+- :class:`WienerDeconvolve`, :func:`wiener_deconvolve_into`, :func:`wiener_deconvolve`,
+  :class:`UnsupervisedWiener`, :func:`unsupervised_wiener`
+                                                   restore/deconvolve.rs:164-509
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
 """
 
@@ -19,3 +22,10 @@ from .restore import (  # noqa: F401
     richardson_lucy_nd_into,
 )
 from ._native import Client, LucemError, Workspace, version  # noqa: F401
+from .wiener import (  # noqa: F401
+    UnsupervisedWiener,
+    WienerDeconvolve,
+    unsupervised_wiener,
+    wiener_deconvolve,
+    wiener_deconvolve_into,
+)

@@ -44,6 +44,8 @@ EXPORTED = [
     "lucem_fft_c2c_lines_f32", "lucem_client_profile",
     "lucem_client_profile_read", "lucem_rl_batch_create", "lucem_rl_batch_execute", "lucem_rl_batch_destroy",
     "lucem_rl_batch_device_bytes", "lucem_rl_batch_launches", "lucem_richardson_lucy_batched_f32",
+    "lucem_wiener_settings_default", "lucem_wiener_deconvolve_into_f32", "lucem_wiener_deconvolve_f32",
+    "lucem_unsupervised_wiener_settings_default", "lucem_unsupervised_wiener_f32",
 ]
 
 
@@ -73,6 +75,20 @@ class RlSettings(ctypes.Structure):
         ("filter_epsilon", ctypes.c_double),
         ("mode", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+    ]
+
+
+class WienerSettings(ctypes.Structure):
+    _fields_ = [("balance", ctypes.c_double), ("clip", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+
+
+class UnsupervisedWienerSettings(ctypes.Structure):
+    _fields_ = [
+        ("num_iter", ctypes.c_uint64),
+        ("burnin", ctypes.c_uint64),
+        ("clip", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
     ]
 
 
@@ -113,6 +129,17 @@ def lib() -> ctypes.CDLL:
             L.lucem_richardson_lucy_into_f64.argtypes = rl
             L.lucem_richardson_lucy_f64.argtypes = L.lucem_richardson_lucy_f32.argtypes
             L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
+            L.lucem_wiener_settings_default.argtypes = [ctypes.POINTER(WienerSettings)]
+            L.lucem_wiener_settings_default.restype = None
+            L.lucem_wiener_deconvolve_into_f32.argtypes = rl[:-2] + [ctypes.POINTER(WienerSettings), _vp]
+            L.lucem_wiener_deconvolve_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
+                                                      _vp, ctypes.POINTER(WienerSettings)]
+            L.lucem_unsupervised_wiener_settings_default.argtypes = [ctypes.POINTER(UnsupervisedWienerSettings)]
+            L.lucem_unsupervised_wiener_settings_default.restype = None
+            L.lucem_unsupervised_wiener_f32.argtypes = [
+                _vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32, _vp,
+                ctypes.POINTER(UnsupervisedWienerSettings), ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
+                ctypes.POINTER(ctypes.c_uint64), _vp]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

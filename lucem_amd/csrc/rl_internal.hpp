@@ -159,6 +159,24 @@ int launch_ratio_f64(const double* obs, const double* c, double* rel, double eps
 int launch_update_f64(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
 int launch_fill_f64(double* p, double v, int64_t n, void* stream);
 int launch_psf_prep_f64(const double* psf, double* h, double* rev, int64_t nk, void* stream);
+// Wiener forms (wiener.hip). The transfer function of a centred PSF on the circular Hn x Wn
+// domain: tables exp(-2 pi i m / N) into tw_h / tw_w, the PSF rows' DFT into A [kh][Lc], then
+// either the Wiener filter (filter != 0, float2 at G[u * pitch + v], times scale) or H itself
+// (FP64 at Hout[u * Lc + v]).
+int launch_wiener_otf(const float* psf, int kh, int kw, int Hn, int Wn, int Lc, int pitch, double2* tw_h,
+                      double2* tw_w, double2* A, int filter, double balance, double scale, float2* G, double2* Hout,
+                      void* stream);
+// Unsupervised Wiener: workgroups of the reductions (at most kUwMaxBlocks), the image moments and
+// initial state, one sweep + precision update per iteration, and the posterior mean as the half
+// spectrum of the inverse passes.
+constexpr int kUwMaxBlocks = 1024;
+int uw_blocks(int64_t n);
+int launch_uw_init(const float* x, int64_t n, double* part, int nb, double2* mean, int64_t nmean, double* state,
+                   void* stream);
+int launch_uw_iteration(const float2* spec, int pitch, const double2* Hc, const double2* tw_h, const double2* tw_w,
+                        int Hn, int Wn, int Lc, double n, double* state, int post, double2* mean, double* part, int nb,
+                        double jitter, double* chain, int iter, void* stream);
+int launch_uw_finish(const double2* mean, double scale, int Hn, int Lc, int pitch, float2* spec, void* stream);
 // Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
 int allow_big_lds_kernels();
 size_t row_pass_lds_bytes(int L, int P);

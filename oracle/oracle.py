@@ -15,12 +15,20 @@ Functions and the reference code each one restates (paths relative to the refere
                               FP64 oracle); pinned against the direct FP64 oracle on small shapes
 - :func:`rl_nll_grad64`       FP64 reverse-mode gradient of unrolled RL + Poisson NLL (no
                               reference counterpart); pinned by central finite differences
+- :func:`wiener64`            crates/lucem-ops/src/restore/deconvolve.rs:184-315 with
+                              restore/kernels/deconvolve.rs:44-99 in FP64, numpy's FFT in place of
+                              the reference's O(N^2) DFT (core/fft.rs:55-100)
+- :func:`unsupervised_wiener64`
+                              restore/deconvolve.rs:357-509 in FP64 over the full spectrum
 
 Parity pins (see tests/test_oracle.py): pad_index against the reference's own host restatement
 (crates/lucem-ops/src/test_support.rs:21-45, restated in the test) and numpy.pad; correlation
 against scipy.ndimage.correlate for every mode and the reference KAT pattern
 (filter/convolve.rs:211-228); RL against the reference's identity and box-PSF KATs
 (restore/deconvolve.rs:517-566), closed-form fixed points, brute-force FP64 and the FFT form.
+Wiener against the reference's identity KAT (deconvolve.rs:542-550), exact inversion of a
+brute-force circular blur at balance 0, and a literal restatement with DFT matrices; the
+unsupervised chain's first value against Parseval-equivalent sums taken in the image domain.
 """
 
 from __future__ import annotations
@@ -214,6 +222,90 @@ def rl_nll64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps:
     psf = np.ascontiguousarray(psf, dtype=np.float64)
     return float(lib().lo_rl_nll_f64(_ptr(y, _f64p), _ptr(x0, _f64p), _dims(y.shape), y.ndim, _ptr(psf, _f64p),
                                      _dims(psf.shape), int(num_iter), float(eps), int(mode)))
+
+
+def centred_kernel64(kernel: np.ndarray, shape) -> np.ndarray:
+    """fft_shift_2d_kernel (restore/kernels/deconvolve.rs:76-99): kernel[ky, kx] stored at
+    ((ky - kh/2) mod H, (kx - kw/2) mod W) of a zero H x W image (later entries win)."""
+    kernel = np.asarray(kernel, dtype=np.float64)
+    H, W = shape
+    kh, kw = kernel.shape
+    out = np.zeros((H, W))
+    for ky in range(kh):
+        for kx in range(kw):
+            out[(ky + H - kh // 2) % H, (kx + W - kw // 2) % W] = kernel[ky, kx]
+    return out
+
+
+def laplacian_spectrum64(shape) -> np.ndarray:
+    """|L|^2 of the regulariser of deconvolve.rs:266-285: the centred 3x3 Laplacian, or a 1x1
+    kernel of 1 when an axis is shorter than 3."""
+    H, W = shape
+    if min(H, 3) == 3 and min(W, 3) == 3:
+        lap = np.array([[0.0, -1.0, 0.0], [-1.0, 4.0, -1.0], [0.0, -1.0, 0.0]])
+    else:
+        lap = np.ones((1, 1))
+    L = np.fft.fft2(centred_kernel64(lap, shape))
+    return (L * np.conj(L)).real
+
+
+def wiener64(image: np.ndarray, psf: np.ndarray, balance: float = 0.1, clip: bool = True) -> np.ndarray:
+    """wiener_deconvolve_into (deconvolve.rs:213-315) in FP64: the real part of
+    ifft2(conj(H) Y / max(|H|^2 + balance |L|^2, 1e-12)), clipped to [-1, 1] when asked."""
+    y = np.asarray(image, dtype=np.float64)
+    Y = np.fft.fft2(y)
+    Hf = np.fft.fft2(centred_kernel64(psf, y.shape))
+    denom = np.maximum((Hf * np.conj(Hf)).real + float(balance) * laplacian_spectrum64(y.shape), 1e-12)
+    out = np.fft.ifft2(np.conj(Hf) * Y / denom).real
+    return np.clip(out, -1.0, 1.0) if clip else out
+
+
+def unsupervised_wiener_jitter(seed: int, num_iter: int, n: float) -> list:
+    """The hyperparameter jitter of deconvolve.rs:429-431, :472-475 (64-bit LCG)."""
+    mask = (1 << 64) - 1
+    state = (int(seed) + 0x9E3779B97F4A7C15) & mask
+    out = []
+    for _ in range(int(num_iter)):
+        state = (state * 6364136223846793005 + 1) & mask
+        u = (state >> 33) / float(1 << 31) - 0.5
+        out.append(float(np.exp(u * (2.0 / np.sqrt(n)))))
+    return out
+
+
+def unsupervised_wiener64(image: np.ndarray, psf: np.ndarray, num_iter: int = 30, burnin: int = 10,
+                          clip: bool = True, seed: int = 0):
+    """unsupervised_wiener (deconvolve.rs:357-509) in FP64 over the full spectrum: returns
+    ``(image, balance_chain)``."""
+    y = np.asarray(image, dtype=np.float64)
+    n = float(y.size)
+    Y = np.fft.fft2(y)
+    Hf = np.fft.fft2(centred_kernel64(psf, y.shape))
+    hm2 = (Hf * np.conj(Hf)).real
+    lm2 = laplacian_spectrum64(y.shape)
+    hy = Y * np.conj(Hf)
+    img_var = float(np.var(y))
+    balance = 0.05
+    chain = []
+    mean_x = np.zeros_like(Y)
+    count = 0
+    jitter = unsupervised_wiener_jitter(seed, num_iter, n)
+    for it in range(int(num_iter)):
+        denom = np.maximum(hm2 + balance * lm2, 1e-12)
+        X = hy / denom
+        D = Y - Hf * X
+        rss = float(np.sum((D.real ** 2 + D.imag ** 2) / n))
+        reg = float(np.sum(lm2 * (X.real ** 2 + X.imag ** 2) / n))
+        trace = float(np.sum(hm2 / denom))
+        if it >= burnin:
+            mean_x += X
+            count += 1
+        eff_dof = max(n - trace, 1.0)
+        sn2 = max(rss / eff_dof, 1e-8 * max(img_var, 1e-6))
+        sx2 = max(reg / max(trace, 1.0), 1e-8)
+        balance = min(max(sn2 / sx2 * jitter[it], 1e-6), 1e3)
+        chain.append(balance)
+    out = np.fft.ifft2(mean_x / max(count, 1)).real
+    return (np.clip(out, -1.0, 1.0) if clip else out), chain
 
 
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:

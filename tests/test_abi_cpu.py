@@ -146,3 +146,59 @@ def test_check_order_f64_without_device(L):
     assert call((0, 5), (3, 3), (0, 5)) == 0
     # the direct form has no FFT length limit
     assert call((8, 16400), (3, 3), (8, 16400)) == 11
+
+
+def test_wiener_check_order_without_device(L):
+    from lucem_amd import _native as N
+
+    s = N.WienerSettings()
+    L.lucem_wiener_settings_default(ctypes.byref(s))
+    assert (s.balance, s.clip) == (0.1, 1)
+
+    def call(in_dims, psf_dims, out_dims, balance=0.1, out=0x20000000):
+        s.balance = balance
+        return L.lucem_wiener_deconvolve_into_f32(None, N._vp(0x10000), N.dims(in_dims), len(in_dims),
+                                                  N._vp(0x30000000), N.dims(psf_dims), len(psf_dims), N._vp(out),
+                                                  N.dims(out_dims), len(out_dims), ctypes.byref(s), None)
+
+    assert call((8, 8), (3, 3), (8, 9)) == 1
+    assert call((8, 8), (3, 3), (8, 8), out=0x10000 + 100) == 2
+    assert call((2, 8, 8), (3, 3), (2, 8, 8)) == 3
+    assert b"wiener_deconvolve input expects 2-D" in L.lucem_last_error()
+    assert call((8, 8), (3,), (8, 8)) == 3
+    assert call((8, 8), (0, 3), (8, 8)) == 4
+    assert call((8, 8), (3, 3), (8, 8), balance=-0.5) == 5
+    assert call((8, 8), (3, 3), (8, 8), balance=float("nan")) == 5
+    assert call((0, 8), (30, 30), (0, 8)) == 0  # empty input wins over the PSF size check
+    assert call((8, 8), (9, 3), (8, 8)) == 4
+    assert b"must not exceed image" in L.lucem_last_error()
+    assert call((8, 11), (3, 3), (8, 11)) == 9  # 11 is not 7-smooth
+    assert call((8, 8), (3, 3), (8, 8)) == 11  # valid arguments, no client
+
+
+def test_unsupervised_check_order_without_device(L):
+    from lucem_amd import _native as N
+
+    s = N.UnsupervisedWienerSettings()
+    L.lucem_unsupervised_wiener_settings_default(ctypes.byref(s))
+    assert (s.num_iter, s.burnin, s.clip, s.seed) == (30, 10, 1, 0)
+    chain = (ctypes.c_double * 64)()
+    got = ctypes.c_uint64(99)
+
+    def call(dims, psf_dims, num_iter=12, burnin=4, cap=64, out=0x20000000):
+        s.num_iter, s.burnin = num_iter, burnin
+        return L.lucem_unsupervised_wiener_f32(None, N._vp(0x10000), N.dims(dims), len(dims), N._vp(0x30000000),
+                                               N.dims(psf_dims), len(psf_dims), N._vp(out), ctypes.byref(s), chain,
+                                               cap, ctypes.byref(got), None)
+
+    assert call((2, 8, 8), (3, 3)) == 3
+    assert call((8, 8), (3, 3, 1)) == 3
+    assert call((8, 8), (3, 0)) == 4
+    assert call((8, 8), (3, 3), num_iter=0, burnin=0) == 5
+    assert call((8, 8), (3, 3), num_iter=4, burnin=4) == 5
+    assert b"must be > burnin" in L.lucem_last_error()
+    assert call((0, 8), (3, 3)) == 0 and got.value == 0
+    assert call((8, 8), (3, 3), out=0x10000 + 4) == 2
+    assert call((8, 8), (3, 3), cap=11) == 5
+    assert call((8, 8), (9, 3)) == 4
+    assert call((8, 8), (3, 3)) == 11

@@ -269,3 +269,113 @@ def test_planted_flip_mistake_is_caught(oracle):
     want[1:] = y[:-1]
     want[0] = y[0]
     assert not np.allclose(x, want, rtol=1e-6)
+
+
+# ---- Wiener forms (restore/deconvolve.rs:184-509) ------------------------------------------
+
+
+def circular_blur(x, psf):
+    """Brute force: y[i, j] = sum psf[ky, kx] x[(i - (ky - kh//2)) mod H, (j - (kx - kw//2)) mod W],
+    the image whose spectrum is H X for the centred PSF."""
+    H, W = x.shape
+    kh, kw = psf.shape
+    y = np.zeros((H, W))
+    for ky in range(kh):
+        for kx in range(kw):
+            y += psf[ky, kx] * np.roll(x, (ky - kh // 2, kx - kw // 2), axis=(0, 1))
+    return y
+
+
+def test_wiener_identity_kat(oracle):
+    """deconvolve.rs:542-550: a delta PSF with balance 1e-7 returns the image within 1e-3."""
+    img = (np.arange(64) / 64.0 * 0.8 + 0.1).reshape(8, 8)
+    delta = np.zeros((3, 3))
+    delta[1, 1] = 1.0
+    out = oracle.wiener64(img, delta, balance=1e-7, clip=False)
+    assert np.max(np.abs(out - img)) < 1e-3
+    assert np.max(np.abs(out - img)) < 1e-6  # FP64: only the regulariser's 1e-7 remains
+
+
+@pytest.mark.parametrize("shape,kshape", [((16, 20), (5, 3)), ((9, 12), (4, 4)), ((7, 5), (7, 5))])
+def test_wiener_inverts_a_circular_blur_at_balance_zero(oracle, shape, kshape):
+    """With balance 0 the filter is 1 / H: a circularly blurred image comes back exactly. A wrong
+    centre, a missing conjugate or a transposed axis breaks this."""
+    rng = np.random.default_rng(sum(shape) + kshape[0])
+    x = rng.uniform(0.0, 1.0, shape)
+    # a dominant centre keeps every |H| >= 1 - (the other taps' sum) > 0: a well-posed inverse
+    psf = rng.uniform(0.0, 0.02, kshape)
+    psf[kshape[0] // 2, kshape[1] // 2] += 1.0
+    assert np.min(np.abs(np.fft.fft2(oracle.centred_kernel64(psf, shape)))) > 0.3
+    y = circular_blur(x, psf)
+    out = oracle.wiener64(y, psf, balance=0.0, clip=False)
+    np.testing.assert_allclose(out, x, atol=1e-9)
+
+
+def dft_matrix(n, inverse=False):
+    k = np.arange(n)
+    sign = 1.0 if inverse else -1.0
+    m = np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+    return m / n if inverse else m
+
+
+def test_wiener_matches_dft_matrix_restatement(oracle):
+    """The reference's pipeline with explicit DFT matrices per axis (the dft_axis_kernel sum,
+    core/kernels/fft.rs:11-47) and its Laplacian built through the same centring."""
+    rng = np.random.default_rng(5)
+    y = rng.uniform(0, 1, (6, 7))
+    psf = rng.uniform(0, 1, (3, 4))
+    F0, F1 = dft_matrix(6), dft_matrix(7)
+    I0, I1 = dft_matrix(6, True), dft_matrix(7, True)
+    fft2 = lambda a: F0 @ a @ F1.T  # noqa: E731
+    Hf = fft2(oracle.centred_kernel64(psf, (6, 7)))
+    lap = np.array([[0, -1, 0], [-1, 4, -1], [0, -1, 0]], dtype=np.float64)
+    Lf = fft2(oracle.centred_kernel64(lap, (6, 7)))
+    for balance in (0.1, 3.0):
+        X = np.conj(Hf) * fft2(y) / np.maximum(np.abs(Hf) ** 2 + balance * np.abs(Lf) ** 2, 1e-12)
+        want = (I0 @ X @ I1.T).real
+        np.testing.assert_allclose(oracle.wiener64(y, psf, balance, clip=False), want, atol=1e-12)
+    # the closed form of the Laplacian's spectrum: 4 - 2 cos(2 pi u / H) - 2 cos(2 pi v / W)
+    u, v = np.meshgrid(np.arange(6), np.arange(7), indexing="ij")
+    closed = (4 - 2 * np.cos(2 * np.pi * u / 6) - 2 * np.cos(2 * np.pi * v / 7)) ** 2
+    np.testing.assert_allclose(oracle.laplacian_spectrum64((6, 7)), closed, atol=1e-12)
+    np.testing.assert_allclose(oracle.laplacian_spectrum64((2, 7)), np.ones((2, 7)))
+
+
+def conv_matrix(psf, shape):
+    """The circular blur by a centred kernel as an N x N matrix (row-major pixels)."""
+    n = shape[0] * shape[1]
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = circular_blur(e.reshape(shape), psf).ravel()
+    return A
+
+
+def test_unsupervised_first_balance_from_image_domain_sums(oracle):
+    """The first chain value from sums taken in the image domain: by Parseval, rss = |y - A x|^2
+    and reg = |L x|^2 for x the Wiener estimate at balance 0.05, and the trace term is
+    tr(A'A (A'A + b L'L)^-1) for the blur and Laplacian matrices."""
+    rng = np.random.default_rng(8)
+    shape = (6, 5)
+    y = rng.uniform(0, 1, shape)
+    psf = rng.uniform(0.1, 1.0, (3, 3))
+    lap = np.array([[0, -1, 0], [-1, 4, -1], [0, -1, 0]], dtype=np.float64)
+    A, Lm = conv_matrix(psf, shape), conv_matrix(lap, shape)
+    b0, n = 0.05, float(y.size)
+    M = A.T @ A + b0 * Lm.T @ Lm
+    x = np.linalg.solve(M, A.T @ y.ravel())
+    np.testing.assert_allclose(x.reshape(shape), oracle.wiener64(y, psf, b0, clip=False), atol=1e-10)
+    rss = float(np.sum((y.ravel() - A @ x) ** 2))
+    reg = float(np.sum((Lm @ x) ** 2))
+    trace = float(np.trace(A.T @ A @ np.linalg.inv(M)))
+    sn2 = max(rss / max(n - trace, 1.0), 1e-8 * max(float(np.var(y)), 1e-6))
+    sx2 = max(reg / max(trace, 1.0), 1e-8)
+    jit = oracle.unsupervised_wiener_jitter(3, 4, n)
+    want = min(max(sn2 / sx2 * jit[0], 1e-6), 1e3)
+    img, chain = oracle.unsupervised_wiener64(y, psf, num_iter=4, burnin=3, clip=False, seed=3)
+    assert len(chain) == 4
+    assert abs(chain[0] - want) <= 1e-9 * want
+    # with one post-burn-in iteration the image is the Wiener estimate at that iteration's balance
+    np.testing.assert_allclose(img, oracle.wiener64(y, psf, chain[2], clip=False), atol=1e-10)
+    assert all(np.exp(-1 / np.sqrt(n)) <= j <= np.exp(1 / np.sqrt(n)) for j in jit)
