@@ -39,6 +39,9 @@
  *   lucem_unsupervised_wiener_f32
  *                         unsupervised_wiener (deconvolve.rs:357-509): output caller-allocated,
  *                         the balance chain written to a caller host buffer
+ *   lucem_fft_1d_into_f32 / lucem_fft_nd_into_f32 / lucem_complex_mul_into_f32
+ *                         lucem_ops::core::fft::{fft_1d_into, fft_nd_into, complex_mul_into}
+ *                         (crates/lucem-ops/src/core/fft.rs:13-135) on split re/im arrays
  */
 #ifndef LUCEM_HIP_H
 #define LUCEM_HIP_H
@@ -63,7 +66,8 @@ typedef enum lucem_status {
     LUCEM_UNKNOWN_PAD_MODE = 8,    /* OpsError::UnknownPadMode      (core/pad.rs:55)      */
     LUCEM_UNSUPPORTED_SHAPE = 9,   /* an axis longer than this library's single-pass FFT limit */
     LUCEM_HIP_ERROR = 10,          /* a HIP runtime call failed; see lucem_last_error()   */
-    LUCEM_INVALID_HANDLE = 11      /* NULL client / workspace / settings pointer          */
+    LUCEM_INVALID_HANDLE = 11,     /* NULL client / workspace / settings pointer          */
+    LUCEM_AXIS_OUT_OF_BOUNDS = 12  /* CoreError::AxisOutOfBounds    (lucem-core shape.rs:50-58) */
 } lucem_status;
 
 /* PadMode ids (core/pad.rs:29-33). RL always uses cval = 0. */
@@ -210,6 +214,31 @@ lucem_status lucem_unsupervised_wiener_f32(lucem_client* client, const float* in
                                            float* output, const lucem_unsupervised_wiener_settings* settings,
                                            double* balance_chain, uint64_t chain_capacity, uint64_t* chain_len,
                                            lucem_workspace* ws);
+
+/* fft_1d_into (core/fft.rs:13-51): the DFT along `axis` (negative counts from the end) of the
+ * split complex array (re_in, im_in) into (re_out, im_out); forward exp(-2 pi i k m / n), inverse
+ * exp(+...) / n. Checks in the reference's order: the three shapes equal to re_in's, re_in/re_out
+ * and im_in/im_out not overlapping, the axis in range (LUCEM_AXIS_OUT_OF_BOUNDS), empty (OK). Any
+ * length: 7-smooth lengths up to 16384 and others up to 8192 (LUCEM_UNSUPPORTED_SHAPE beyond). */
+lucem_status lucem_fft_1d_into_f32(lucem_client* client, const float* re_in, const uint64_t* re_in_dims,
+                                   uint32_t re_in_ndim, const float* im_in, const uint64_t* im_in_dims,
+                                   uint32_t im_in_ndim, float* re_out, const uint64_t* re_out_dims,
+                                   uint32_t re_out_ndim, float* im_out, const uint64_t* im_out_dims,
+                                   uint32_t im_out_ndim, int64_t axis, int32_t inverse);
+/* fft_nd_into (core/fft.rs:53-100): the DFT over every axis, in order. A 0-D array is copied, a
+ * 1-D array goes through lucem_fft_1d_into_f32 (and its aliasing check); otherwise the outputs may
+ * alias the inputs (two workspace slots hold the intermediate axes). */
+lucem_status lucem_fft_nd_into_f32(lucem_client* client, const float* re_in, const uint64_t* re_in_dims,
+                                   uint32_t re_in_ndim, const float* im_in, const uint64_t* im_in_dims,
+                                   uint32_t im_in_ndim, float* re_out, const uint64_t* re_out_dims,
+                                   uint32_t re_out_ndim, float* im_out, const uint64_t* im_out_dims,
+                                   uint32_t im_out_ndim, int32_t inverse, lucem_workspace* ws);
+/* complex_mul_into (core/fft.rs:102-135): out = a * b, or a * conj(b). Shapes checked as the
+ * reference does (b_re and out_re against a_re); the imaginary parts have the same length. */
+lucem_status lucem_complex_mul_into_f32(lucem_client* client, const float* a_re, const uint64_t* a_dims,
+                                        uint32_t a_ndim, const float* a_im, const float* b_re, const uint64_t* b_dims,
+                                        uint32_t b_ndim, const float* b_im, float* out_re, const uint64_t* out_dims,
+                                        uint32_t out_ndim, float* out_im, int32_t conj_b);
 
 /* The same operator for 2-D or 3-D arrays (psf of the same rank as the input). */
 lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,

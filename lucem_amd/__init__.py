@@ -10,6 +10,8 @@ Public surface (mirrors the reference's names):
 - :class:`WienerDeconvolve`, :func:`wiener_deconvolve_into`, :func:`wiener_deconvolve`,
   :class:`UnsupervisedWiener`, :func:`unsupervised_wiener`
                                                    restore/deconvolve.rs:164-509
+- :mod:`lucem_amd.fft`: ``fft_1d_into``, ``fft_nd_into``, ``complex_mul_into``
+                                                   core/fft.rs:13-135
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
 """
 
@@ -29,3 +31,4 @@ from .wiener import (  # noqa: F401
     wiener_deconvolve,
     wiener_deconvolve_into,
 )
+from . import fft  # noqa: F401,E402

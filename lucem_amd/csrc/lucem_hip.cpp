@@ -104,7 +104,10 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
     c->num_cus = prop.multiProcessorCount;
     static std::once_flag lds_once;
     static int lds_err = 0;
-    std::call_once(lds_once, [] { lds_err = allow_big_lds_kernels(); });
+    std::call_once(lds_once, [] {
+        lds_err = allow_big_lds_kernels();
+        if (!lds_err) lds_err = allow_big_lds_axis_kernels();
+    });
     if (lds_err) return hip_fail(lds_err, "hipFuncSetAttribute(dynamic LDS)");
     if (!own_stream) {
         c->stream = (hipStream_t)stream;
@@ -130,6 +133,7 @@ lucem_client::~lucem_client() {
     }
     for (auto ev : event_pool) (void)hipEventDestroy(ev);
     for (auto& kv : twiddles) (void)hipFree(kv.second);
+    for (auto& kv : chirps) (void)hipFree(kv.second);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 

@@ -39,6 +39,7 @@ struct lucem_client {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::map<int, float2*> twiddles;               // FFT length -> device twiddle table
+    std::map<int, float2*> chirps;                 // Bluestein length n -> chirp [n] + its transform [M]
     std::unique_ptr<lucem_workspace> internal_ws;  // used by the allocating-style entry points
     bool profiling = false;
     std::vector<lucem::ProfRecord> records;        // launches since profiling was enabled
@@ -123,6 +124,11 @@ lucem_status tune_shapes(lucem_client* c, const std::vector<RowPass>& rows, cons
                          int mode, ShapeTable* out);
 lucem_status tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb, const float2* otf,
                           const float2* const tws[3], const ShapeTable** out);
+
+// One axis transform of the general FFT (lucem_fft.cpp): descriptor and grid for an array
+// viewed as [outer, n, inner]; the caller fills in the buffers.
+lucem_status axis_fft_desc(lucem_client* c, int n, int64_t outer, int64_t inner, bool inverse, AxisFft* d,
+                           int* blocks);
 
 lucem_status validate_rl(const void* input, const uint64_t* in_dims, uint32_t in_ndim, const void* psf,
                          const uint64_t* psf_dims, uint32_t psf_ndim, const void* output, const uint64_t* out_dims,

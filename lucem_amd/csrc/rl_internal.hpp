@@ -159,6 +159,32 @@ int launch_ratio_f64(const double* obs, const double* c, double* rel, double eps
 int launch_update_f64(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
 int launch_fill_f64(double* p, double v, int64_t n, void* stream);
 int launch_psf_prep_f64(const double* psf, double* h, double* rev, int64_t nk, void* stream);
+// Complex FFT along one axis of split re/im arrays viewed as [outer, n, inner] (fft_axis.hip).
+struct AxisFft {
+    const float* re_in;
+    const float* im_in;
+    float* re_out;
+    float* im_out;
+    int n;                   // axis length
+    int B;                   // lines per workgroup (B * f.L <= threads * 16)
+    int64_t inner, outer;    // elements after / slabs before the axis
+    int64_t nkb;             // line blocks per slab (inner > 1)
+    int inverse;             // conj(DFT(conj(x))) * scale
+    int wide;                // thread tier: 0 = 256, 1 = 512, 2 = 1024 threads
+    int bluestein;           // n not 7-smooth: circular convolution of length f.L = M
+    float scale;             // 1 forward, 1/n inverse
+    FftPlan f;
+    const float2* tw;
+    const float2* chirp;      // Bluestein: exp(-i pi m^2 / n), m < n
+    const float2* chirp_hat;  // Bluestein: DFT_M of the conjugate chirp sequence, times 1/M
+};
+int launch_axis_fft(const AxisFft& d, int blocks, void* stream);
+size_t axis_fft_lds_bytes(int L, int B);
+int launch_complex_mul(const float* ar, const float* ai, const float* br, const float* bi, float* orr, float* oi,
+                       int64_t n, int conj_b, void* stream);
+int launch_copy_f32(const float* src, float* dst, int64_t n, void* stream);
+int allow_big_lds_axis_kernels();
+
 // Wiener forms (wiener.hip). The transfer function of a centred PSF on the circular Hn x Wn
 // domain: tables exp(-2 pi i m / N) into tw_h / tw_w, the PSF rows' DFT into A [kh][Lc], then
 // either the Wiener filter (filter != 0, float2 at G[u * pitch + v], times scale) or H itself

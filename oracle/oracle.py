@@ -15,6 +15,8 @@ Functions and the reference code each one restates (paths relative to the refere
                               FP64 oracle); pinned against the direct FP64 oracle on small shapes
 - :func:`rl_nll_grad64`       FP64 reverse-mode gradient of unrolled RL + Poisson NLL (no
                               reference counterpart); pinned by central finite differences
+- :func:`fft_axis64`          crates/lucem-ops/src/core/fft.rs:13-100 (the DFT of
+                              core/kernels/fft.rs:11-47) in FP64 through numpy's FFT
 - :func:`wiener64`            crates/lucem-ops/src/restore/deconvolve.rs:184-315 with
                               restore/kernels/deconvolve.rs:44-99 in FP64, numpy's FFT in place of
                               the reference's O(N^2) DFT (core/fft.rs:55-100)
@@ -222,6 +224,15 @@ def rl_nll64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps:
     psf = np.ascontiguousarray(psf, dtype=np.float64)
     return float(lib().lo_rl_nll_f64(_ptr(y, _f64p), _ptr(x0, _f64p), _dims(y.shape), y.ndim, _ptr(psf, _f64p),
                                      _dims(psf.shape), int(num_iter), float(eps), int(mode)))
+
+
+def fft_axis64(re: np.ndarray, im: np.ndarray, axes=None, inverse: bool = False):
+    """fft_1d_into along each of ``axes`` in turn (all axes: fft_nd_into), FP64: forward
+    exp(-2 pi i k m / n) unscaled, inverse exp(+...) / n. Returns ``(re, im)``."""
+    z = np.asarray(re, dtype=np.float64) + 1j * np.asarray(im, dtype=np.float64)
+    for ax in (range(z.ndim) if axes is None else axes):
+        z = np.fft.ifft(z, axis=ax) if inverse else np.fft.fft(z, axis=ax)
+    return z.real.copy(), z.imag.copy()
 
 
 def centred_kernel64(kernel: np.ndarray, shape) -> np.ndarray:

@@ -202,3 +202,33 @@ def test_unsupervised_check_order_without_device(L):
     assert call((8, 8), (3, 3), cap=11) == 5
     assert call((8, 8), (9, 3)) == 4
     assert call((8, 8), (3, 3)) == 11
+
+
+def test_fft_check_order_without_device(L):
+    from lucem_amd import _native as N
+
+    def arr(ptr, dims):
+        return [N._vp(ptr), N.dims(dims), len(dims)]
+
+    def call(dims=(4, 8), im=(4, 8), ro=(4, 8), io=(4, 8), axis=-1, out=0x20000000):
+        return L.lucem_fft_1d_into_f32(None, *arr(0x10000, dims), *arr(0x40000, im), *arr(out, ro),
+                                       *arr(0x60000000, io), axis, 0)
+
+    assert call(im=(4, 9)) == 1
+    assert call(ro=(8, 4)) == 1
+    assert call(io=(4,)) == 1
+    assert call(out=0x10000 + 64) == 2
+    assert call(axis=2) == 12 and b"out of bounds" in L.lucem_last_error()
+    assert call(axis=-3) == 12
+    assert call((0, 8), (0, 8), (0, 8), (0, 8)) == 0
+    assert call((0, 8), (0, 8), (0, 8), (0, 8), axis=5) == 12  # the axis is checked before emptiness
+    assert call(axis=-2) == 11  # valid arguments, no client
+    nd = L.lucem_fft_nd_into_f32(None, *arr(0x10000, (4, 8)), *arr(0x40000, (4, 8)), *arr(0x10000, (4, 8)),
+                                 *arr(0x40000, (4, 8)), 0, None)
+    assert nd == 11  # in place is allowed for ndim >= 2
+    nd1 = L.lucem_fft_nd_into_f32(None, *arr(0x10000, (8,)), *arr(0x40000, (8,)), *arr(0x10000, (8,)),
+                                  *arr(0x40000, (8,)), 0, None)
+    assert nd1 == 2
+    cm = L.lucem_complex_mul_into_f32(None, *arr(0x10000, (3, 4)), N._vp(0x1000), *arr(0x2000, (3, 5)),
+                                      N._vp(0x3000), *arr(0x4000, (3, 4)), N._vp(0x5000), 0)
+    assert cm == 1

@@ -379,3 +379,24 @@ def test_unsupervised_first_balance_from_image_domain_sums(oracle):
     # with one post-burn-in iteration the image is the Wiener estimate at that iteration's balance
     np.testing.assert_allclose(img, oracle.wiener64(y, psf, chain[2], clip=False), atol=1e-10)
     assert all(np.exp(-1 / np.sqrt(n)) <= j <= np.exp(1 / np.sqrt(n)) for j in jit)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 11, 12, 97])
+def test_fft_axis64_matches_dft_sum(oracle, n):
+    """The reference's DFT (core/kernels/fft.rs:11-47): X_k = sum_m x_m exp(-+2 pi i k m / n),
+    1/n on the inverse, evaluated term by term."""
+    rng = np.random.default_rng(n)
+    re, im = rng.normal(size=(3, n)), rng.normal(size=(3, n))
+    for inverse in (False, True):
+        sign = 1.0 if inverse else -1.0
+        m = np.arange(n)
+        W = np.exp(sign * 2j * np.pi * np.outer(m, m) / n) / (n if inverse else 1)
+        want = (re + 1j * im) @ W.T
+        gr, gi = oracle.fft_axis64(re, im, axes=[1], inverse=inverse)
+        np.testing.assert_allclose(gr + 1j * gi, want, atol=1e-10)
+    # a round trip and the N-D form as consecutive axes
+    x = rng.normal(size=(4, n, 3))
+    fr, fi = oracle.fft_axis64(x, np.zeros_like(x))
+    br, bi = oracle.fft_axis64(fr, fi, inverse=True)
+    np.testing.assert_allclose(br, x, atol=1e-10)
+    np.testing.assert_allclose(bi, 0.0, atol=1e-10)
