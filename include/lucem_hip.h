@@ -178,9 +178,9 @@ void lucem_wiener_settings_default(lucem_wiener_settings* out);
  * 1e-12) on the circular H x W domain of the image, L the centred 3x3 Laplacian, then the
  * inverse transform and the optional clip to [-1, 1]. Checks in the reference's order:
  * same_shape, not_aliased, input 2-D, psf 2-D, psf non-empty, balance finite and >= 0, empty
- * input (OK), psf no larger than the image (LUCEM_INVALID_WEIGHTS). Both image sizes must be
- * 7-smooth and at most 16384 (LUCEM_UNSUPPORTED_SHAPE otherwise). Asynchronous on the client's
- * stream. */
+ * input (OK), psf no larger than the image (LUCEM_INVALID_WEIGHTS). Image sizes: 7-smooth up to
+ * 16384 (the R2C passes) or any size up to 8192 (the general FFT); LUCEM_UNSUPPORTED_SHAPE beyond.
+ * Asynchronous on the client's stream. */
 lucem_status lucem_wiener_deconvolve_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                               uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
                                               uint32_t psf_ndim, float* output, const uint64_t* out_dims,

@@ -4,11 +4,12 @@
 //   wiener_deconvolve        deconvolve.rs:318-327
 //   unsupervised_wiener      deconvolve.rs:357-509 (checks :363-383)
 //
-// Both work on the circular H x W domain of the image (no padding), as the reference's DFTs do:
-// the image's half spectrum comes from the row R2C and the line FFT along y, the PSF's transfer
-// function from a direct FP64 DFT of the kernel (wiener.hip), and the result from the inverse
-// line FFT and the row C2R, whose store applies the optional clip. FFT lengths are the image
-// sizes themselves, so both axes must be 7-smooth and at most 16384.
+// Both work on the circular H x W domain of the image (no padding), as the reference's DFTs do.
+// The PSF's transfer function is a direct FP64 DFT of the kernel (wiener.hip). When both image
+// sizes are 7-smooth (at most 16384) the image's half spectrum comes from the row R2C and the
+// line FFT along y, and the result from the inverse line FFT and the row C2R, whose store
+// applies the optional clip. Other sizes (up to 8192) take the general complex FFT of
+// lucem_fft.cpp over the full spectrum, with Bluestein lines where a length is not 7-smooth.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +29,7 @@ namespace {
 
 struct WienerGeom {
     int H, W, kh, kw, Lc, pitch;
+    bool general;  // full spectrum on the general complex FFT (Lc = pitch = W)
 };
 
 bool fft_length_ok(int v) {
@@ -62,16 +64,18 @@ lucem_status finish_geometry(const uint64_t* in_dims, const uint64_t* psf_dims, 
     if (overflow || n > 0xFFFFFFFFull)
         return fail(LUCEM_TOO_MANY_ELEMENTS,
                     "dims " + dims_str(in_dims, 2) + " exceed the device element limit of 4294967295");
-    if (!fft_length_ok((int)std::min<uint64_t>(in_dims[0], 1u << 30)) ||
-        !fft_length_ok((int)std::min<uint64_t>(in_dims[1], 1u << 30)))
+    const int H = (int)std::min<uint64_t>(in_dims[0], 1u << 30), W = (int)std::min<uint64_t>(in_dims[1], 1u << 30);
+    auto general_ok = [](int v) { return fft_length_ok(v) || v <= LUCEM_MAX_FFT_LEN / 2; };
+    if (!general_ok(H) || !general_ok(W))
         return fail(LUCEM_UNSUPPORTED_SHAPE, "image " + dims_str(in_dims, 2) +
-                                                 ": the Wiener forms need 7-smooth sizes of at most 16384 per axis");
-    g->H = (int)in_dims[0];
-    g->W = (int)in_dims[1];
+                                                 ": sizes up to 16384 (7-smooth) or 8192 (any) per axis are supported");
+    g->H = H;
+    g->W = W;
     g->kh = (int)psf_dims[0];
     g->kw = (int)psf_dims[1];
-    g->Lc = g->W / 2 + 1;
-    g->pitch = (g->Lc + 15) & ~15;
+    g->general = !fft_length_ok(H) || !fft_length_ok(W);
+    g->Lc = g->general ? W : W / 2 + 1;
+    g->pitch = g->general ? W : (g->Lc + 15) & ~15;
     return LUCEM_OK;
 }
 
@@ -204,8 +208,30 @@ lucem_status prepare(lucem_client* c, lucem_workspace* ws, const WienerGeom& g, 
     if ((st = ws_slot(ws, "wiener_spec", sizeof(float2) * (size_t)g.H * g.pitch, (void**)spec))) return st;
     if ((st = ws_slot(ws, "wiener_dft_tables", sizeof(double2) * ((size_t)g.H + g.W), (void**)tw))) return st;
     if ((st = ws_slot(ws, "wiener_psf_rows", sizeof(double2) * (size_t)g.kh * g.Lc, (void**)psf_rows))) return st;
+    if (g.general) return LUCEM_OK;  // the general FFT plans its own lines
     if ((st = twiddles_for(c, g.H, &tws[0]))) return st;
     return twiddles_for(c, g.W, &tws[1]);
+}
+
+// General route: (re, im) = the image's full spectrum, through lucem_fft_nd_into_f32.
+lucem_status image_spectrum_general(lucem_client* c, lucem_workspace* ws, const float* input, const WienerGeom& g,
+                                    float** re, float** im) {
+    const int64_t n = (int64_t)g.H * g.W;
+    lucem_status st;
+    if ((st = ws_slot(ws, "wiener_re", sizeof(float) * n, (void**)re))) return st;
+    if ((st = ws_slot(ws, "wiener_im", sizeof(float) * n, (void**)im))) return st;
+    if ((st = check_launch(launch_zero_f32(*im, n, c->stream), "wiener zero"))) return st;
+    const uint64_t d[2] = {(uint64_t)g.H, (uint64_t)g.W};
+    return lucem_fft_nd_into_f32(c, input, d, 2, *im, d, 2, *re, d, 2, *im, d, 2, 0, ws);
+}
+
+// General route: output = real part of the inverse transform of (re, im), clipped when asked.
+lucem_status image_from_spectrum_general(lucem_client* c, lucem_workspace* ws, float* re, float* im, float* output,
+                                         const WienerGeom& g, bool clip) {
+    const uint64_t d[2] = {(uint64_t)g.H, (uint64_t)g.W};
+    lucem_status st = lucem_fft_nd_into_f32(c, re, d, 2, im, d, 2, re, d, 2, im, d, 2, 1, ws);
+    if (st) return st;
+    return check_launch(launch_real_part(re, output, (int64_t)g.H * g.W, clip ? 1 : 0, c->stream), "wiener output");
 }
 
 lucem_status wiener_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
@@ -216,12 +242,20 @@ lucem_status wiener_run(lucem_client* c, lucem_workspace* ws, const float* input
     lucem_status st = prepare(c, ws, g, &spec, &tw, &psf_rows, tws);
     if (st) return st;
     if ((st = ws_slot(ws, "wiener_filter", sizeof(float2) * (size_t)g.H * g.pitch, (void**)&filt))) return st;
-    // The row R2C carries a factor 2 and the unnormalised inverse transforms a factor H * W.
-    const double scale = 0.5 / ((double)g.H * (double)g.W);
+    // The row R2C carries a factor 2 and the unnormalised inverse transforms a factor H * W; the
+    // general FFT's inverse is normalised.
+    const double scale = g.general ? 1.0 : 0.5 / ((double)g.H * (double)g.W);
     if ((st = check_launch(launch_wiener_otf(psf, g.kh, g.kw, g.H, g.W, g.Lc, g.pitch, tw, tw + g.H, psf_rows, 1,
                                              s.balance, scale, filt, nullptr, c->stream),
                            "wiener filter")))
         return st;
+    if (g.general) {
+        float *re, *im;
+        if ((st = image_spectrum_general(c, ws, input, g, &re, &im))) return st;
+        if ((st = check_launch(launch_apply_filter(re, im, filt, (int64_t)g.H * g.W, c->stream), "wiener filter")))
+            return st;
+        return image_from_spectrum_general(c, ws, re, im, output, g, s.clip != 0);
+    }
     int rb = 0, cb = 0;
     RowPass r = image_rows(c, g, tws[1], spec, &rb);
     r.op = ROW_LOAD;
@@ -276,36 +310,50 @@ lucem_status unsupervised_run(lucem_client* c, lucem_workspace* ws, const float*
     if ((st = check_launch(launch_uw_init(input, npix, part, uw_blocks(npix), mean, nh, state, c->stream),
                            "unsupervised_wiener init")))
         return st;
-    // the image's half spectrum (2 Y)
+    // the image's spectrum as 2 Y (half spectrum from the R2C passes, or the full one packed)
     int rb = 0, cb = 0;
-    RowPass r = image_rows(c, g, tws[1], spec, &rb);
-    r.op = ROW_LOAD;
-    r.src = input;
-    if ((st = run_row(c, r, rb, "uw_row_fwd"))) return st;
-    const ColPass yf = image_lines(c, g, COL_FWD, SHAPE_COL_FWD, tws[0], spec, nullptr, &cb);
-    if ((st = run_col(c, yf, cb, "uw_line_fwd_y"))) return st;
+    RowPass r{};
+    float *re = nullptr, *im = nullptr;
+    if (g.general) {
+        if ((st = image_spectrum_general(c, ws, input, g, &re, &im))) return st;
+        if ((st = check_launch(launch_pack_spectrum(re, im, 2.0f, spec, npix, c->stream), "unsupervised_wiener pack")))
+            return st;
+    } else {
+        r = image_rows(c, g, tws[1], spec, &rb);
+        r.op = ROW_LOAD;
+        r.src = input;
+        if ((st = run_row(c, r, rb, "uw_row_fwd"))) return st;
+        const ColPass yf = image_lines(c, g, COL_FWD, SHAPE_COL_FWD, tws[0], spec, nullptr, &cb);
+        if ((st = run_col(c, yf, cb, "uw_line_fwd_y"))) return st;
+    }
 
     const std::vector<double> jit = jitters(s.seed, s.num_iter, n);
     const int nb = uw_blocks(nh);
     for (uint64_t it = 0; it < s.num_iter; it++) {
         if ((st = check_launch(launch_uw_iteration(spec, g.pitch, Hc, tw, tw + g.H, g.H, g.W, g.Lc, n, state,
-                                                   it >= s.burnin ? 1 : 0, mean, part, nb, jit[it], chain, (int)it,
-                                                   c->stream),
+                                                   it >= s.burnin ? 1 : 0, g.general ? 1 : 0, mean, part, nb,
+                                                   jit[it], chain, (int)it, c->stream),
                                "unsupervised_wiener iteration")))
             return st;
     }
-    // posterior mean / count, normalised for the unnormalised inverse transforms, then back
-    const double inv = 1.0 / ((double)(s.num_iter - s.burnin) * n);
+    // posterior mean / count (and 1 / n for the unnormalised inverse passes), then back
+    const double inv = 1.0 / ((double)(s.num_iter - s.burnin) * (g.general ? 1.0 : n));
     if ((st = check_launch(launch_uw_finish(mean, inv, g.H, g.Lc, g.pitch, spec, c->stream), "unsupervised_wiener mean")))
         return st;
-    const ColPass yi = image_lines(c, g, COL_INV, SHAPE_COL_INV, tws[0], spec, nullptr, &cb);
-    if ((st = run_col(c, yi, cb, "uw_line_inv_y"))) return st;
-    r.op = ROW_UPDATE;
-    r.first = 1;
-    r.last = 1;
-    r.clip = s.clip ? 1 : 0;
-    r.out = output;
-    if ((st = run_row(c, r, rb, "uw_row_inv"))) return st;
+    if (g.general) {
+        if ((st = check_launch(launch_unpack_spectrum(spec, re, im, npix, c->stream), "unsupervised_wiener unpack")))
+            return st;
+        if ((st = image_from_spectrum_general(c, ws, re, im, output, g, s.clip != 0))) return st;
+    } else {
+        const ColPass yi = image_lines(c, g, COL_INV, SHAPE_COL_INV, tws[0], spec, nullptr, &cb);
+        if ((st = run_col(c, yi, cb, "uw_line_inv_y"))) return st;
+        r.op = ROW_UPDATE;
+        r.first = 1;
+        r.last = 1;
+        r.clip = s.clip ? 1 : 0;
+        r.out = output;
+        if ((st = run_row(c, r, rb, "uw_row_inv"))) return st;
+    }
     // the chain is returned on the host, as the reference returns its Vec<f64>
     hipError_t e = hipMemcpyAsync(chain_host, chain, sizeof(double) * s.num_iter, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

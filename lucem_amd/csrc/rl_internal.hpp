@@ -199,9 +199,17 @@ constexpr int kUwMaxBlocks = 1024;
 int uw_blocks(int64_t n);
 int launch_uw_init(const float* x, int64_t n, double* part, int nb, double2* mean, int64_t nmean, double* state,
                    void* stream);
+// `full`: the spectrum is the whole H x W one (weight 1 everywhere), not the half spectrum.
 int launch_uw_iteration(const float2* spec, int pitch, const double2* Hc, const double2* tw_h, const double2* tw_w,
-                        int Hn, int Wn, int Lc, double n, double* state, int post, double2* mean, double* part, int nb,
-                        double jitter, double* chain, int iter, void* stream);
+                        int Hn, int Wn, int Lc, double n, double* state, int post, int full, double2* mean, double* part,
+                        int nb, double jitter, double* chain, int iter, void* stream);
+// General-size route: (re + i im) *= G in place; split planes <-> interleaved (times scale);
+// out = re, clamped to [-1, 1] when clip.
+int launch_apply_filter(float* re, float* im, const float2* G, int64_t n, void* stream);
+int launch_pack_spectrum(const float* re, const float* im, float scale, float2* spec, int64_t n, void* stream);
+int launch_unpack_spectrum(const float2* spec, float* re, float* im, int64_t n, void* stream);
+int launch_real_part(const float* re, float* out, int64_t n, int clip, void* stream);
+int launch_zero_f32(float* p, int64_t n, void* stream);
 int launch_uw_finish(const double2* mean, double scale, int Hn, int Lc, int pitch, float2* spec, void* stream);
 // Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
 int allow_big_lds_kernels();

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kWienerThreads) void uw_sweep_kernel(const float2* 
                                                                    const double2* __restrict__ tw_h,
                                                                    const double2* __restrict__ tw_w, int Hn, int Wn,
                                                                    int Lc, double n, const double* __restrict__ state,
-                                                                   int post, double2* __restrict__ mean,
+                                                                   int post, int full, double2* __restrict__ mean,
                                                                    double* __restrict__ part) {
     const double balance = state[0];
     const int64_t total = (int64_t)Hn * Lc;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kWienerThreads) void uw_sweep_kernel(const float2* 
         const double xi = (yi * h.x - yr * h.y) / denom;
         const double dr = yr - (h.x * xr - h.y * xi);
         const double di = yi - (h.x * xi + h.y * xr);
-        const double w = (v == 0 || 2 * v == Wn) ? 1.0 : 2.0;
+        const double w = (full || v == 0 || 2 * v == Wn) ? 1.0 : 2.0;
         v3[0] += w * ((dr * dr + di * di) / n);
         v3[1] += w * (lm2 * ((xr * xr + xi * xi) / n));
         v3[2] += w * (hm2 / denom);
@@ -227,6 +227,45 @@ __global__ __launch_bounds__(kWienerThreads) void uw_update_kernel(const double*
         const double b = fmin(fmax(sn2 / sx2 * jitter, 1.0e-6), 1.0e3);
         state[0] = b;
         chain[iter] = b;
+    }
+}
+
+// Sizes that are not 7-smooth run on the general complex FFT over split re/im planes: the
+// filter is applied in place, the spectrum moves between split planes and the interleaved
+// layout of the unsupervised sweep, and the output is the real part (clipped when asked).
+__global__ void apply_filter_kernel(float* __restrict__ re, float* __restrict__ im, const float2* __restrict__ G,
+                                    int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 g = G[i];
+        const float a = re[i], b = im[i];
+        re[i] = a * g.x - b * g.y;
+        im[i] = a * g.y + b * g.x;
+    }
+}
+
+__global__ void pack_spectrum_kernel(const float* __restrict__ re, const float* __restrict__ im, float scale,
+                                     float2* __restrict__ spec, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) spec[i] = make_float2(re[i] * scale, im[i] * scale);
+}
+
+__global__ void unpack_spectrum_kernel(const float2* __restrict__ spec, float* __restrict__ re,
+                                       float* __restrict__ im, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 v = spec[i];
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+}
+
+__global__ void real_part_kernel(const float* __restrict__ re, float* __restrict__ out, int64_t n, int clip) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float v = re[i];
+        if (clip) v = fminf(fmaxf(v, -1.f), 1.f);
+        out[i] = v;
     }
 }
 
@@ -274,11 +313,11 @@ int launch_uw_init(const float* x, int64_t n, double* part, int nb, double2* mea
 }
 
 int launch_uw_iteration(const float2* spec, int pitch, const double2* Hc, const double2* tw_h, const double2* tw_w,
-                        int Hn, int Wn, int Lc, double n, double* state, int post, double2* mean, double* part, int nb,
-                        double jitter, double* chain, int iter, void* stream) {
+                        int Hn, int Wn, int Lc, double n, double* state, int post, int full, double2* mean, double* part,
+                        int nb, double jitter, double* chain, int iter, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(uw_sweep_kernel, dim3(nb), dim3(kWienerThreads), 0, st, spec, pitch, Hc, tw_h, tw_w, Hn, Wn, Lc,
-                       n, (const double*)state, post, mean, part);
+                       n, (const double*)state, post, full, mean, part);
     hipLaunchKernelGGL(uw_update_kernel, dim3(1), dim3(kWienerThreads), 0, st, (const double*)part, nb, n, jitter,
                        state, chain, iter);
     return (int)hipGetLastError();
@@ -287,6 +326,40 @@ int launch_uw_iteration(const float2* spec, int pitch, const double2* Hc, const 
 int launch_uw_finish(const double2* mean, double scale, int Hn, int Lc, int pitch, float2* spec, void* stream) {
     hipLaunchKernelGGL(uw_finish_kernel, dim3(blocks_for((int64_t)Hn * Lc)), dim3(kWienerThreads), 0,
                        (hipStream_t)stream, mean, scale, Hn, Lc, pitch, spec);
+    return (int)hipGetLastError();
+}
+
+int launch_apply_filter(float* re, float* im, const float2* G, int64_t n, void* stream) {
+    hipLaunchKernelGGL(apply_filter_kernel, dim3(blocks_for(n)), dim3(kWienerThreads), 0, (hipStream_t)stream, re, im,
+                       G, n);
+    return (int)hipGetLastError();
+}
+
+int launch_pack_spectrum(const float* re, const float* im, float scale, float2* spec, int64_t n, void* stream) {
+    hipLaunchKernelGGL(pack_spectrum_kernel, dim3(blocks_for(n)), dim3(kWienerThreads), 0, (hipStream_t)stream, re, im,
+                       scale, spec, n);
+    return (int)hipGetLastError();
+}
+
+int launch_unpack_spectrum(const float2* spec, float* re, float* im, int64_t n, void* stream) {
+    hipLaunchKernelGGL(unpack_spectrum_kernel, dim3(blocks_for(n)), dim3(kWienerThreads), 0, (hipStream_t)stream, spec,
+                       re, im, n);
+    return (int)hipGetLastError();
+}
+
+__global__ void zero_f32_kernel(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+int launch_zero_f32(float* p, int64_t n, void* stream) {
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(n)), dim3(kWienerThreads), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+int launch_real_part(const float* re, float* out, int64_t n, int clip, void* stream) {
+    hipLaunchKernelGGL(real_part_kernel, dim3(blocks_for(n)), dim3(kWienerThreads), 0, (hipStream_t)stream, re, out, n,
+                       clip);
     return (int)hipGetLastError();
 }
 

@@ -8,9 +8,9 @@ Reference (neuro-kitchen/lucem):
 - ``UnsupervisedWiener`` (defaults)         crates/lucem-ops/src/restore/deconvolve.rs:330-353
 - ``unsupervised_wiener``                   crates/lucem-ops/src/restore/deconvolve.rs:357-509
 
-Arrays are float32 device tensors. Both forms work on the circular domain of the image, whose
-sizes must be 7-smooth and at most 16384 per axis. Errors raise :class:`lucem_amd.LucemError`
-in the reference's check order.
+Arrays are float32 device tensors. Both forms work on the circular domain of the image: sizes
+up to 16384 per axis when 7-smooth, up to 8192 otherwise. Errors raise
+:class:`lucem_amd.LucemError` in the reference's check order.
 """
 
 from __future__ import annotations

@@ -172,7 +172,8 @@ def test_wiener_check_order_without_device(L):
     assert call((0, 8), (30, 30), (0, 8)) == 0  # empty input wins over the PSF size check
     assert call((8, 8), (9, 3), (8, 8)) == 4
     assert b"must not exceed image" in L.lucem_last_error()
-    assert call((8, 11), (3, 3), (8, 11)) == 9  # 11 is not 7-smooth
+    assert call((8, 11), (3, 3), (8, 11)) == 11  # 11 is not 7-smooth: the general FFT route
+    assert call((8, 8209), (3, 3), (8, 8209)) == 9  # a prime above 8192
     assert call((8, 8), (3, 3), (8, 8)) == 11  # valid arguments, no client
 
 

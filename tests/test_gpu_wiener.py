@@ -36,6 +36,11 @@ WIENER_CASES = [
     ((3, 3), (3, 3), 0.2, False),
     ((125, 98), (9, 6), 0.05, False),   # odd and 7-smooth sizes
     ((1024, 1024), (25, 25), 0.1, False),
+    # sizes that are not 7-smooth: the general complex FFT with Bluestein lines
+    ((37, 53), (5, 5), 0.1, False),
+    ((100, 97), (9, 7), 0.02, True),
+    ((11, 1), (3, 1), 0.5, False),
+    ((509, 768), (15, 15), 0.1, False),
 ]
 
 
@@ -85,7 +90,7 @@ def test_wiener_errors(gpu):
     with pytest.raises(LA.LucemError) as e:
         LA.wiener_deconvolve(y, torch.ones((17, 3), device="cuda"))
     assert e.value.kind == "INVALID_WEIGHTS"
-    z = torch.ones((11, 16), device="cuda")  # 11 is not 7-smooth
+    z = torch.ones((8209, 16), device="cuda")  # a prime above 8192
     with pytest.raises(LA.LucemError) as e:
         LA.wiener_deconvolve(z, torch.ones((3, 3), device="cuda"))
     assert e.value.kind == "UNSUPPORTED_SHAPE"
@@ -123,7 +128,8 @@ def test_unsupervised_reference_case(gpu, oracle):
     assert oracle.rel_l2(img.cpu().numpy(), want_img) < RTOL_L2
 
 
-@pytest.mark.parametrize("shape,kshape,seed", [((64, 48), (5, 5), 3), ((100, 128), (9, 7), 11), ((2, 30), (1, 3), 0)])
+@pytest.mark.parametrize("shape,kshape,seed", [((64, 48), (5, 5), 3), ((100, 128), (9, 7), 11), ((2, 30), (1, 3), 0),
+                                                ((37, 41), (5, 3), 5), ((61, 64), (7, 7), 9)])
 def test_unsupervised_matches_fp64_oracle(gpu, oracle, shape, kshape, seed):
     import lucem_amd as LA
 
