@@ -24,6 +24,11 @@ Alongside the throughput the line carries:
   plain 2-D entry point called once per tile, with both results compared;
 - ``grad``: the differentiable form on 64 tiles of 256x256, PSF 17x17, 10 iterations, checkpoint
   every 5: forward time, forward + backward (d/dx0 and d/dpsf) time and their ratio.
+- ``wiener``: Wiener deconvolution of a 2048x2048 image with a 25x25 Gaussian PSF (7-smooth sizes:
+  the R2C passes; 2039x2048: the general FFT with Bluestein columns), the unsupervised form on
+  1024x1024 (30 iterations, burn-in 10), and the FP64 numpy restatement timed on this host;
+- ``fft``: the general complex FFT over both axes of a 2048x2048 split re/im array, and of a
+  2039x2039 one (Bluestein lines), against 32 bytes per element (two read+write passes).
 """
 
 from __future__ import annotations
@@ -178,6 +183,81 @@ def grad_leg(steps: int) -> dict:
     }
 
 
+def _gaussian_psf(k: int, sigma: float):
+    import numpy as np
+
+    r = np.arange(k) - k // 2
+    g = np.exp(-(r[:, None] ** 2 + r[None, :] ** 2) / (2 * sigma * sigma))
+    return (g / g.sum()).astype(np.float32)
+
+
+def wiener_leg(steps: int, with_cpu: bool) -> dict:
+    import numpy as np
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+
+    rng = np.random.default_rng(7)
+    client = N.default_client(0)
+    ws = N.Workspace(client)
+    psf_np = _gaussian_psf(25, 2.0)
+    psf = torch.from_numpy(psf_np).cuda()
+    s = LA.WienerDeconvolve(balance=0.1, clip=False)
+    out = {}
+    for name, shape in (("r2c", (2048, 2048)), ("general", (2039, 2048))):
+        y = torch.from_numpy(rng.uniform(0, 1, shape).astype(np.float32)).cuda()
+        o = torch.empty_like(y)
+        t = _timeit(lambda: LA.wiener_deconvolve_into(y, psf, o, s, ws, client), max(steps, 5))
+        n = shape[0] * shape[1]
+        out[name] = {"shape": list(shape), "ms": round(t * 1e3, 4), "Mpixel_per_s": round(n / t / 1e6, 1)}
+    # the R2C route: image read and output write (4 B per pixel each); the half spectrum written,
+    # read and written by the fused line pass, read again (4 transfers of 8 B per bin); the
+    # filter written once and read once (2 more)
+    H, W = 2048, 2048
+    bins = H * (W // 2 + 1)
+    alg = 8 * H * W + 6 * 8 * bins
+    out["r2c"]["algorithmic_bytes"] = alg
+    out["r2c"]["achieved_GBps"] = round(alg / (out["r2c"]["ms"] * 1e-3) / 1e9, 1)
+    y1 = torch.from_numpy(rng.uniform(0, 1, (1024, 1024)).astype(np.float32)).cuda()
+    us = LA.UnsupervisedWiener(num_iter=30, burnin=10, clip=False, seed=1)
+    tu = _timeit(lambda: LA.unsupervised_wiener(y1, psf, us, client, ws), max(steps // 2, 3))
+    out["unsupervised"] = {"shape": [1024, 1024], "num_iter": 30, "ms": round(tu * 1e3, 3)}
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_wiener(rng.uniform(0, 1, (2048, 2048)).astype(np.float32), psf_np)
+        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["ms"] / out["r2c"]["ms"], 1)
+    return out
+
+
+def cpu_baseline_wiener(y, psf) -> dict:
+    """The oracle's FP64 numpy Wiener filter, timed once on this host (one core)."""
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    O.wiener64(y, psf, 0.1, False)
+    tc = time.perf_counter() - t0
+    return {"ms": round(tc * 1e3, 1), "kind": "port", "cores": 1,
+            "sample": f"the FP64 numpy restatement on the same {y.shape[0]}x{y.shape[1]} case"}
+
+
+def fft_leg(steps: int) -> dict:
+    import torch
+
+    from lucem_amd import fft
+
+    out = {}
+    for name, shape in (("smooth", (2048, 2048)), ("bluestein", (2039, 2039))):
+        re = torch.randn(shape, device="cuda")
+        im = torch.randn(shape, device="cuda")
+        ro, io = torch.empty_like(re), torch.empty_like(im)
+        t = _timeit(lambda: fft.fft_nd_into(re, im, ro, io), max(steps, 5))
+        alg = 32 * shape[0] * shape[1]
+        gbps = alg / t / 1e9
+        out[name] = {"shape": list(shape), "ms": round(t * 1e3, 4), "achieved_GBps": round(gbps, 1),
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +363,8 @@ def main() -> int:
     if not args.no_extras:
         line["batched"] = batched_leg(args.steps)
         line["grad"] = grad_leg(args.steps)
+        line["wiener"] = wiener_leg(args.steps, not args.no_cpu_baseline)
+        line["fft"] = fft_leg(args.steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))

@@ -152,9 +152,15 @@ lucem_status lucem::axis_fft_desc(lucem_client* c, int n, int64_t outer, int64_t
     d->outer = outer;
     d->inverse = inverse ? 1 : 0;
     d->scale = inverse ? (float)(1.0 / (double)n) : 1.0f;
+    // Contiguous lines fill the smallest thread tier that holds them. Strided lines want at least
+    // 8 adjacent columns per workgroup (32-byte accesses): a wider tier when the short one holds
+    // fewer, as the RL line passes do.
     d->wide = tier_for_length(L);
-    const int cap = tier_threads(d->wide) * kElemsPerThread;
     const int64_t line_dim = inner == 1 ? outer : inner;
+    const int64_t want = std::min<int64_t>(8, line_dim);
+    if (inner > 1)
+        while (d->wide < 2 && tier_threads(d->wide) * kElemsPerThread / L < want) d->wide++;
+    const int cap = tier_threads(d->wide) * kElemsPerThread;
     int B = (int)std::max<int64_t>(1, std::min<int64_t>(cap / L, line_dim));
     const int64_t lines = outer * inner;
     while (B > 4 && lines / B < c->num_cus) B--;

@@ -62,34 +62,55 @@ __device__ __forceinline__ double laplacian_mag2(const double2* tw_h, const doub
     return l * l;
 }
 
-// H(u, v) = sum_ky A[ky, v] exp(-2 pi i u (ky - kh/2) / Hn) for u < Hn, v < Lc. FILTER: store the
-// Wiener filter conj(H) / max(|H|^2 + balance |L|^2, 1e-12) times `scale` as float2 at
-// G[u * pitch + v] (wiener_freq_filter_kernel, restore/kernels/deconvolve.rs:44-73); otherwise
-// store H itself (FP64) at Hout[u * Lc + v].
+// H(u, v) = sum_ky A[ky, v] exp(-2 pi i u (ky - kh/2) / Hn) for u < Hn, v < Lc. Each thread
+// takes kOtfRows rows u = u0 + j of one column v, so every A[ky, v] it loads serves them all.
+// FILTER: store the Wiener filter conj(H) / max(|H|^2 + balance |L|^2, 1e-12) times `scale` as
+// float2 at G[u * pitch + v] (wiener_freq_filter_kernel, restore/kernels/deconvolve.rs:44-73);
+// otherwise store H itself (FP64) at Hout[u * Lc + v].
+constexpr int kOtfRows = 8;
+
 template <bool FILTER>
 __global__ void psf_otf_kernel(const double2* __restrict__ A, int kh, const double2* __restrict__ tw_h,
                                const double2* __restrict__ tw_w, int Hn, int Wn, int Lc, int pitch, double balance,
                                double scale, float2* __restrict__ G, double2* __restrict__ Hout) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)Hn * Lc) return;
-    const int u = (int)(idx / Lc), v = (int)(idx - (int64_t)u * Lc);
+    const int64_t ublocks = (Hn + kOtfRows - 1) / kOtfRows;
+    if (idx >= ublocks * Lc) return;
+    const int ub = (int)(idx / Lc), v = (int)(idx - (int64_t)ub * Lc);
+    const int u0 = ub * kOtfRows;
     const int cy = kh / 2;
-    int m = (int)((Hn - ((int64_t)u * cy) % Hn) % Hn);
-    double re = 0.0, im = 0.0;
+    int m[kOtfRows];
+    double re[kOtfRows], im[kOtfRows];
+#pragma unroll
+    for (int j = 0; j < kOtfRows; j++) {
+        const int u = min(u0 + j, Hn - 1);
+        m[j] = (int)((Hn - ((int64_t)u * cy) % Hn) % Hn);
+        re[j] = 0.0;
+        im[j] = 0.0;
+    }
     for (int ky = 0; ky < kh; ky++) {
         const double2 a = A[(int64_t)ky * Lc + v];
-        const double2 t = tw_h[m];
-        re += a.x * t.x - a.y * t.y;
-        im += a.x * t.y + a.y * t.x;
-        m += u;
-        if (m >= Hn) m -= Hn;
+#pragma unroll
+        for (int j = 0; j < kOtfRows; j++) {
+            const double2 t = tw_h[m[j]];
+            re[j] += a.x * t.x - a.y * t.y;
+            im[j] += a.x * t.y + a.y * t.x;
+            m[j] += min(u0 + j, Hn - 1);
+            if (m[j] >= Hn) m[j] -= Hn;
+        }
     }
-    if constexpr (FILTER) {
-        const double d = fmax(re * re + im * im + balance * laplacian_mag2(tw_h, tw_w, Hn, Wn, u, v), 1.0e-12);
-        const double f = scale / d;
-        G[(int64_t)u * pitch + v] = make_float2((float)(re * f), (float)(-im * f));
-    } else {
-        Hout[idx] = make_double2(re, im);
+#pragma unroll
+    for (int j = 0; j < kOtfRows; j++) {
+        const int u = u0 + j;
+        if (u >= Hn) break;
+        if constexpr (FILTER) {
+            const double d =
+                fmax(re[j] * re[j] + im[j] * im[j] + balance * laplacian_mag2(tw_h, tw_w, Hn, Wn, u, v), 1.0e-12);
+            const double f = scale / d;
+            G[(int64_t)u * pitch + v] = make_float2((float)(re[j] * f), (float)(-im[j] * f));
+        } else {
+            Hout[(int64_t)u * Lc + v] = make_double2(re[j], im[j]);
+        }
     }
 }
 
@@ -291,7 +312,7 @@ int launch_wiener_otf(const float* psf, int kh, int kw, int Hn, int Wn, int Lc, 
     hipLaunchKernelGGL(dft_table_kernel, dim3(blocks_for(Wn)), dim3(kWienerThreads), 0, st, tw_w, Wn);
     hipLaunchKernelGGL(psf_row_dft_kernel, dim3(blocks_for((int64_t)kh * Lc)), dim3(kWienerThreads), 0, st, psf, kh,
                        kw, (const double2*)tw_w, Wn, Lc, A);
-    const dim3 grid(blocks_for((int64_t)Hn * Lc));
+    const dim3 grid(blocks_for((int64_t)((Hn + kOtfRows - 1) / kOtfRows) * Lc));
     if (filter)
         hipLaunchKernelGGL(psf_otf_kernel<true>, grid, dim3(kWienerThreads), 0, st, (const double2*)A, kh,
                            (const double2*)tw_h, (const double2*)tw_w, Hn, Wn, Lc, pitch, balance, scale, G, Hout);
