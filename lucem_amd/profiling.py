@@ -7,10 +7,11 @@ Two helpers for the ``traffic`` figure of ``bench.py``:
     ``FETCH_SIZE`` / ``WRITE_SIZE`` readings against the known 1 GiB each way give the factor that
     turns each counter into bytes on this GPU.
 
-``python -m lucem_amd.profiling run batched|grad [REPS]``
+``python -m lucem_amd.profiling run batched|grad|wiener|fft [REPS]``
     a short run of one workload for a kernel trace: the uneven-tile slide batch (300 tiles, 20
-    iterations) or the differentiable form (64 tiles of 256x256, forward + backward), REPS calls
-    (default 3) after one warm-up call.
+    iterations), the differentiable form (64 tiles of 256x256, forward + backward), the Wiener
+    filter (2048x2048, 25x25 Gaussian PSF, balance 0.1) or the complex FFT over both axes of a
+    2048x2048 split re/im array; REPS calls (default 3) after one warm-up call.
 
 ``python -m lucem_amd.profiling summarize WORKLOAD FETCH_DIR WRITE_DIR FETCH_CAL_DIR WRITE_CAL_DIR``
     reads the ``run_counter_collection.csv`` files of four ``rocprofv3 --pmc FETCH_SIZE`` /
@@ -100,6 +101,24 @@ def run_workload(name: str, reps: int = 3) -> None:
         def fn():
             plan.forward(xs, yl, p)
             plan.backward(None, gx, gp)
+    elif name == "wiener":
+        import numpy as np
+
+        from .wiener import WienerDeconvolve, wiener_deconvolve_into
+
+        r = np.arange(25) - 12
+        g = np.exp(-(r[:, None] ** 2 + r[None, :] ** 2) / 8.0)
+        psf = torch.from_numpy((g / g.sum()).astype(np.float32)).cuda()
+        y = torch.rand((2048, 2048), device="cuda")
+        o = torch.empty_like(y)
+        s = WienerDeconvolve(balance=0.1, clip=False)
+        fn = lambda: wiener_deconvolve_into(y, psf, o, s)  # noqa: E731
+    elif name == "fft":
+        from .fft import fft_nd_into
+
+        re, im = torch.randn((2048, 2048), device="cuda"), torch.randn((2048, 2048), device="cuda")
+        ro, io = torch.empty_like(re), torch.empty_like(im)
+        fn = lambda: fft_nd_into(re, im, ro, io)  # noqa: E731
     else:
         raise ValueError(f"unknown workload {name!r}")
     fn()
