@@ -39,6 +39,8 @@
  *   lucem_unsupervised_wiener_f32
  *                         unsupervised_wiener (deconvolve.rs:357-509): output caller-allocated,
  *                         the balance chain written to a caller host buffer
+ *   lucem_add_noise_f32   lucem_ops::intensity::add_noise (intensity/noise.rs:35-143), NoiseModel
+ *                         as lucem_noise_model; output caller-allocated
  *   lucem_fft_1d_into_f32 / lucem_fft_nd_into_f32 / lucem_complex_mul_into_f32
  *                         lucem_ops::core::fft::{fft_1d_into, fft_nd_into, complex_mul_into}
  *                         (crates/lucem-ops/src/core/fft.rs:13-135) on split re/im arrays
@@ -67,7 +69,8 @@ typedef enum lucem_status {
     LUCEM_UNSUPPORTED_SHAPE = 9,   /* an axis longer than this library's single-pass FFT limit */
     LUCEM_HIP_ERROR = 10,          /* a HIP runtime call failed; see lucem_last_error()   */
     LUCEM_INVALID_HANDLE = 11,     /* NULL client / workspace / settings pointer          */
-    LUCEM_AXIS_OUT_OF_BOUNDS = 12  /* CoreError::AxisOutOfBounds    (lucem-core shape.rs:50-58) */
+    LUCEM_AXIS_OUT_OF_BOUNDS = 12, /* CoreError::AxisOutOfBounds    (lucem-core shape.rs:50-58) */
+    LUCEM_EMPTY_INPUT = 13         /* OpsError::EmptyInput          (error.rs:54)         */
 } lucem_status;
 
 /* PadMode ids (core/pad.rs:29-33). RL always uses cval = 0. */
@@ -239,6 +242,29 @@ lucem_status lucem_complex_mul_into_f32(lucem_client* client, const float* a_re,
                                         uint32_t a_ndim, const float* a_im, const float* b_re, const uint64_t* b_dims,
                                         uint32_t b_ndim, const float* b_im, float* out_re, const uint64_t* out_dims,
                                         uint32_t out_ndim, float* out_im, int32_t conj_b);
+
+/* NoiseModel (intensity/noise.rs:17-32): kind and its two parameters. */
+enum {
+    LUCEM_NOISE_GAUSSIAN = 0,        /* p0 mean, p1 std: x + mean + std * N(0, 1) */
+    LUCEM_NOISE_SPECKLE = 1,         /* p0 mean, p1 std: x + x * (mean + std * N(0, 1)) */
+    LUCEM_NOISE_SALT = 2,            /* p0 amount: that fraction of pixels set to 1 */
+    LUCEM_NOISE_PEPPER = 3,          /* p0 amount: that fraction of pixels set to 0 */
+    LUCEM_NOISE_SALT_AND_PEPPER = 4, /* p0 amount, p1 salt_vs_pepper */
+    LUCEM_NOISE_POISSON = 5          /* p0 scale: photons per unit intensity (at least 1) */
+};
+typedef struct lucem_noise_model {
+    uint32_t kind;
+    uint32_t reserved;
+    double p0;
+    double p1;
+} lucem_noise_model;
+
+/* add_noise::<f32> (intensity/noise.rs:35-143): `output` (the input's dims; it may be the input)
+ * receives the noisy image. The per-pixel counter generator and the seed folding are the
+ * reference's, so uniform draws and the decisions on them match it bit for bit. An empty input
+ * returns LUCEM_EMPTY_INPUT, as the reference's EmptyInput. Asynchronous on the client's stream. */
+lucem_status lucem_add_noise_f32(lucem_client* client, const float* input, const uint64_t* dims, uint32_t ndim,
+                                 float* output, const lucem_noise_model* model, uint64_t seed, int32_t clip);
 
 /* The same operator for 2-D or 3-D arrays (psf of the same rank as the input). */
 lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,

@@ -12,6 +12,7 @@ Public surface (mirrors the reference's names):
                                                    restore/deconvolve.rs:164-509
 - :mod:`lucem_amd.fft`: ``fft_1d_into``, ``fft_nd_into``, ``complex_mul_into``
                                                    core/fft.rs:13-135
+- :class:`NoiseModel`, :func:`add_noise`          intensity/noise.rs:17-143
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
 """
 
@@ -32,3 +33,4 @@ from .wiener import (  # noqa: F401
     wiener_deconvolve_into,
 )
 from . import fft  # noqa: F401,E402
+from .noise import NoiseModel, add_noise, add_noise_into  # noqa: F401,E402

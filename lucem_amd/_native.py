@@ -33,6 +33,7 @@ STATUS_NAMES = {
     10: "HIP_ERROR",
     11: "INVALID_HANDLE",
     12: "AXIS_OUT_OF_BOUNDS",
+    13: "EMPTY_INPUT",
 }
 
 # Every symbol include/lucem_hip.h declares (checked by the CPU tests).
@@ -47,7 +48,7 @@ EXPORTED = [
     "lucem_rl_batch_device_bytes", "lucem_rl_batch_launches", "lucem_richardson_lucy_batched_f32",
     "lucem_wiener_settings_default", "lucem_wiener_deconvolve_into_f32", "lucem_wiener_deconvolve_f32",
     "lucem_unsupervised_wiener_settings_default", "lucem_unsupervised_wiener_f32",
-    "lucem_fft_1d_into_f32", "lucem_fft_nd_into_f32", "lucem_complex_mul_into_f32",
+    "lucem_fft_1d_into_f32", "lucem_fft_nd_into_f32", "lucem_complex_mul_into_f32", "lucem_add_noise_f32",
 ]
 
 
@@ -94,6 +95,11 @@ class UnsupervisedWienerSettings(ctypes.Structure):
     ]
 
 
+class NoiseModelC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("p0", ctypes.c_double),
+                ("p1", ctypes.c_double)]
+
+
 _lib = None
 _lock = threading.Lock()
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -131,6 +137,8 @@ def lib() -> ctypes.CDLL:
             L.lucem_richardson_lucy_into_f64.argtypes = rl
             L.lucem_richardson_lucy_f64.argtypes = L.lucem_richardson_lucy_f32.argtypes
             L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
+            L.lucem_add_noise_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, ctypes.POINTER(NoiseModelC),
+                                              ctypes.c_uint64, ctypes.c_int32]
             arr = [_vp, _u64p, ctypes.c_uint32]
             L.lucem_fft_1d_into_f32.argtypes = [_vp] + arr * 4 + [ctypes.c_int64, ctypes.c_int32]
             L.lucem_fft_nd_into_f32.argtypes = [_vp] + arr * 4 + [ctypes.c_int32, _vp]

@@ -211,6 +211,9 @@ int launch_unpack_spectrum(const float2* spec, float* re, float* im, int64_t n, 
 int launch_real_part(const float* re, float* out, int64_t n, int clip, void* stream);
 int launch_zero_f32(float* p, int64_t n, void* stream);
 int launch_uw_finish(const double2* mean, double scale, int Hn, int Lc, int pitch, float2* spec, void* stream);
+// add_noise (noise.hip): kind is a LUCEM_NOISE_* id; a, b its parameters as the kernel takes them.
+int launch_add_noise(const float* in, float* out, int kind, uint32_t seed, float a, float b, int clip, uint32_t total,
+                     void* stream);
 // Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
 int allow_big_lds_kernels();
 size_t row_pass_lds_bytes(int L, int P);

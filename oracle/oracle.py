@@ -17,6 +17,9 @@ Functions and the reference code each one restates (paths relative to the refere
                               reference counterpart); pinned by central finite differences
 - :func:`fft_axis64`          crates/lucem-ops/src/core/fft.rs:13-100 (the DFT of
                               core/kernels/fft.rs:11-47) in FP64 through numpy's FFT
+- :func:`add_noise32`         crates/lucem-ops/src/intensity/noise.rs:35-143 with
+                              intensity/kernels/noise.rs:1-135 and core/kernels/random.rs:7-12 in
+                              numpy float32 / uint32
 - :func:`wiener64`            crates/lucem-ops/src/restore/deconvolve.rs:184-315 with
                               restore/kernels/deconvolve.rs:44-99 in FP64, numpy's FFT in place of
                               the reference's O(N^2) DFT (core/fft.rs:55-100)
@@ -233,6 +236,74 @@ def fft_axis64(re: np.ndarray, im: np.ndarray, axes=None, inverse: bool = False)
     for ax in (range(z.ndim) if axes is None else axes):
         z = np.fft.ifft(z, axis=ax) if inverse else np.fft.fft(z, axis=ax)
     return z.real.copy(), z.imag.copy()
+
+
+def pcg_hash32(x: np.ndarray) -> np.ndarray:
+    """core/kernels/random.rs:7-12 on uint32 arrays (arithmetic wraps at 32 bits)."""
+    x = np.asarray(x, dtype=np.uint32)
+    state = x * np.uint32(747796405) + np.uint32(2891336453)
+    word = ((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737)
+    return (word >> np.uint32(22)) ^ word
+
+
+def _uniform32(seed: np.uint32, c: np.ndarray) -> np.ndarray:
+    return pcg_hash32(c ^ seed).astype(np.float32) * np.float32(1.0 / 4294967296.0)
+
+
+def _normal32(seed: np.uint32, idx: np.ndarray) -> np.ndarray:
+    u1 = np.maximum(_uniform32(seed, idx * np.uint32(2)), np.float32(1e-7))
+    u2 = _uniform32(seed, idx * np.uint32(2) + np.uint32(1))
+    return np.sqrt(np.float32(-2.0) * np.log(u1)) * np.cos(np.float32(6.2831855) * u2)
+
+
+def add_noise32(image: np.ndarray, kind: int, p0: float, p1: float = 0.0, seed: int = 0,
+                clip: bool = True) -> np.ndarray:
+    """add_noise (intensity/noise.rs:35-143) in float32: kind 0 Gaussian (mean, std), 1 Speckle
+    (mean, std), 2 Salt (amount), 3 Pepper (amount), 4 SaltAndPepper (amount, salt_vs_pepper),
+    5 Poisson (scale)."""
+    x = np.ascontiguousarray(image, dtype=np.float32).ravel()
+    if x.size == 0:
+        raise ValueError("empty input to add_noise")
+    seed = int(seed) & ((1 << 64) - 1)
+    s32 = np.uint32(((seed & 0xFFFFFFFF) ^ (((seed >> 32) * 0x9E3779B9) & 0xFFFFFFFF)) & 0xFFFFFFFF)
+    idx = np.arange(x.size, dtype=np.uint32)
+    a, b = np.float32(p0), np.float32(p1)
+    if kind in (0, 1):
+        z = _normal32(s32, idx)
+        out = (x + a + b * z) if kind == 0 else (x + x * (a + b * z))
+        out = out.astype(np.float32)
+        if clip:
+            out = np.clip(out, np.float32(0), np.float32(1))
+    elif kind in (2, 3, 4):
+        b = np.float32(1.0) if kind == 2 else (np.float32(0.0) if kind == 3 else b)
+        u = _uniform32(s32, idx * np.uint32(2))
+        v = _uniform32(s32, idx * np.uint32(2) + np.uint32(1))
+        out = np.where(u < a, np.where(v < b, np.float32(1), np.float32(0)), x).astype(np.float32)
+    elif kind == 5:
+        scale = np.float32(max(float(p0), 1.0))
+        lam = (np.maximum(x, np.float32(0)) * scale).astype(np.float32)
+        out = np.zeros_like(x)
+        small = (lam > 0) & (lam < 20)
+        l = np.exp(-lam[small]).astype(np.float32)
+        si = idx[small]
+        p = np.ones(si.size, dtype=np.float32)
+        k = np.zeros(si.size, dtype=np.uint32)
+        alive = np.ones(si.size, dtype=bool)
+        for step in range(1, 129):
+            k = np.where(alive, np.uint32(step), k)
+            u = np.maximum(_uniform32(s32, si * np.uint32(131) + np.uint32(step)), np.float32(1e-7))
+            p = np.where(alive, (p * u).astype(np.float32), p)
+            alive &= ~(p <= l)
+        out[small] = (k - np.uint32(1)).astype(np.float32) / scale
+        big = lam >= 20
+        z = _normal32(s32, idx[big])
+        lb = lam[big]
+        out[big] = np.maximum(np.floor(lb + np.sqrt(lb) * z + np.float32(0.5)), np.float32(0)) / scale
+        if clip:
+            out = np.clip(out, np.float32(0), np.float32(1))
+    else:
+        raise ValueError(f"unknown noise model {kind}")
+    return out.reshape(np.shape(image))
 
 
 def centred_kernel64(kernel: np.ndarray, shape) -> np.ndarray:

@@ -400,3 +400,57 @@ def test_fft_axis64_matches_dft_sum(oracle, n):
     br, bi = oracle.fft_axis64(fr, fi, inverse=True)
     np.testing.assert_allclose(br, x, atol=1e-10)
     np.testing.assert_allclose(bi, 0.0, atol=1e-10)
+
+
+# ---- add_noise (intensity/noise.rs:35-143) -----------------------------------------------
+
+
+def test_noise_uniform_stream_is_uniform(oracle):
+    """The counter hash's uniforms: mean, variance and a 64-bin chi-square over 2^20 counters."""
+    u = oracle._uniform32(np.uint32(12345), np.arange(1 << 20, dtype=np.uint32)).astype(np.float64)
+    n = u.size
+    assert abs(u.mean() - 0.5) < 4 / np.sqrt(12 * n)
+    assert abs(u.var() - 1 / 12) < 2e-3
+    counts = np.bincount(np.minimum((u * 64).astype(int), 63), minlength=64)
+    chi2 = float(((counts - n / 64) ** 2 / (n / 64)).sum())
+    assert chi2 < 120  # 63 degrees of freedom: the 99.99th percentile is about 113
+    # a permutation of the 32-bit counters: no collisions
+    h = oracle.pcg_hash32(np.arange(1 << 20, dtype=np.uint32))
+    assert np.unique(h).size == h.size
+
+
+def test_noise_gaussian_and_speckle_moments(oracle):
+    x = np.full((512, 512), 0.5, np.float32)
+    g = oracle.add_noise32(x, 0, 0.1, 0.05, seed=7, clip=False) - x
+    assert abs(g.mean() - 0.1) < 5e-4 and abs(g.std() - 0.05) < 5e-4
+    s = oracle.add_noise32(x, 1, 0.0, 0.2, seed=7, clip=False)
+    assert abs((s / x - 1).std() - 0.2) < 2e-3
+    # the reference's own check (intensity/mod.rs:52-63): clipped Gaussian noise stays in [0, 1]
+    ramp = (np.arange(64, dtype=np.float32) / 63.0).reshape(8, 8)
+    y = oracle.add_noise32(ramp, 0, 0.0, 0.05, seed=42, clip=True)
+    assert ((y >= 0) & (y <= 1)).all()
+
+
+@pytest.mark.parametrize("lam", [0.5, 3.0, 12.0, 50.0])
+def test_noise_poisson_moments(oracle, lam):
+    """Poisson draws at lambda = x * scale: integer photon counts with mean and variance lambda,
+    from Knuth's product method below 20 and the rounded normal approximation above."""
+    scale = 10.0
+    x = np.full(1 << 18, lam / scale, np.float32)
+    y = oracle.add_noise32(x, 5, scale, seed=3, clip=False) * np.float32(scale)
+    assert np.allclose(y, np.round(y), atol=1e-3)
+    assert abs(y.mean() - lam) < 5 * np.sqrt(lam / y.size) + 1e-3
+    assert abs(y.var() / lam - 1) < 0.02
+
+
+def test_noise_impulse_fractions(oracle):
+    x = np.full((1024, 1024), 0.5, np.float32)
+    salt = oracle.add_noise32(x, 2, 0.1)
+    assert set(np.unique(salt)) == {0.5, 1.0} and abs((salt == 1).mean() - 0.1) < 2e-3
+    pep = oracle.add_noise32(x, 3, 0.2)
+    assert set(np.unique(pep)) == {0.0, 0.5} and abs((pep == 0).mean() - 0.2) < 2e-3
+    sp = oracle.add_noise32(x, 4, 0.3, 0.25)
+    hit = sp != 0.5
+    assert abs(hit.mean() - 0.3) < 2e-3 and abs((sp[hit] == 1).mean() - 0.25) < 5e-3
+    # the 64-bit seed is folded: a change in its high half changes the stream
+    assert (oracle.add_noise32(x, 2, 0.1, seed=1 << 32) != salt).any()
