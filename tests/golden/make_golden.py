@@ -5,8 +5,11 @@
 off and a custom filter epsilon; per case the float32 input and PSF, the FP64 direct-oracle
 output (the reference algorithm, crates/lucem-ops/src/restore/deconvolve.rs:45-149, run in
 double precision) and the FP32 direct-oracle output. ``grad_case.npz``: one 2-D and one 3-D
-case of the unrolled-RL Poisson NLL with its FP64 loss and gradients. The metadata of each case
-is in ``cases.json``. Arrays only (no pickled objects): load with ``numpy.load`` defaults.
+case of the unrolled-RL Poisson NLL with its FP64 loss and gradients. ``restore_cases.npz``: the
+Wiener filter (restore/deconvolve.rs:213-315) and the unsupervised Wiener loop (:357-509) in FP64,
+the complex FFT over all axes (core/fft.rs:53-100) in FP64, and add_noise
+(intensity/noise.rs:35-143) impulse and Poisson draws in float32. The metadata of each case is in
+``cases.json``. Arrays only (no pickled objects): load with ``numpy.load`` defaults.
 """
 
 from __future__ import annotations
@@ -68,7 +71,32 @@ def main() -> None:
                       f"g{j}_gx0": gx, f"g{j}_gpsf": gp})
         gmeta.append({"case": j, "shape": list(shape), "psf": list(kshape), "num_iter": it, "mode": 0})
     np.savez_compressed(HERE / "grad_case.npz", **grads)
-    (HERE / "cases.json").write_text(json.dumps({"rl": meta, "grad": gmeta}, indent=1) + "\n")
+    rest, rmeta = {}, {"wiener": [], "unsupervised": [], "fft": [], "noise": []}
+    for j, (shape, kshape, balance, clip) in enumerate([((24, 30), (5, 5), 0.1, False), ((23, 26), (3, 4), 0.01, True),
+                                                         ((2, 9), (1, 3), 0.5, False)]):
+        y = rng.uniform(0.0, 1.0, shape).astype(np.float32)
+        p = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+        rest.update({f"w{j}_image": y, f"w{j}_psf": p, f"w{j}_out64": O.wiener64(y, p, balance, clip)})
+        rmeta["wiener"].append({"case": j, "shape": list(shape), "psf": list(kshape), "balance": balance, "clip": clip})
+    for j, (shape, kshape, it, burn, seed) in enumerate([((16, 20), (3, 3), 8, 3, 5), ((13, 16), (5, 3), 6, 2, 11)]):
+        y = rng.uniform(0.0, 1.0, shape).astype(np.float32)
+        p = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+        img, chain = O.unsupervised_wiener64(y, p, it, burn, False, seed)
+        rest.update({f"u{j}_image": y, f"u{j}_psf": p, f"u{j}_out64": img, f"u{j}_chain": np.array(chain)})
+        rmeta["unsupervised"].append({"case": j, "shape": list(shape), "psf": list(kshape), "num_iter": it,
+                                      "burnin": burn, "clip": False, "seed": seed})
+    for j, (shape, inverse) in enumerate([((6, 10, 7), False), ((11, 13), True)]):
+        re = rng.normal(size=shape).astype(np.float32)
+        im = rng.normal(size=shape).astype(np.float32)
+        fr, fi = O.fft_axis64(re, im, inverse=inverse)
+        rest.update({f"f{j}_re": re, f"f{j}_im": im, f"f{j}_out_re": fr, f"f{j}_out_im": fi})
+        rmeta["fft"].append({"case": j, "shape": list(shape), "inverse": inverse})
+    for j, (kind, a, b, seed, clip) in enumerate([(4, 0.3, 0.4, (1 << 40) + 3, False), (5, 20.0, 0.0, 9, False)]):
+        x = rng.uniform(0.0, 1.0, (32, 32)).astype(np.float32)
+        rest.update({f"n{j}_image": x, f"n{j}_out": O.add_noise32(x, kind, a, b, seed=seed, clip=clip)})
+        rmeta["noise"].append({"case": j, "kind": kind, "p0": a, "p1": b, "seed": seed, "clip": clip})
+    np.savez_compressed(HERE / "restore_cases.npz", **rest)
+    (HERE / "cases.json").write_text(json.dumps({"rl": meta, "grad": gmeta, **rmeta}, indent=1) + "\n")
 
 
 if __name__ == "__main__":

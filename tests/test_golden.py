@@ -22,8 +22,9 @@ GR = np.load(G / "grad_case.npz")
 
 
 def _rel(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a = np.asarray(a, dt)
+    b = np.asarray(b, dt)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
@@ -90,3 +91,95 @@ def test_hip_matches_grad_fixture(gpu, case):
     assert abs(L - GR[f"g{case}_loss"][0]) <= 1e-6 * abs(L)
     assert _rel(x0.grad.cpu().numpy(), GR[f"g{case}_gx0"]) < 1e-3
     assert _rel(p.grad.cpu().numpy(), GR[f"g{case}_gpsf"]) < 1e-3
+
+
+RS = np.load(G / "restore_cases.npz")
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["wiener"]])
+def test_oracle_reproduces_wiener_fixture(oracle, case):
+    m = META["wiener"][case]
+    out = oracle.wiener64(RS[f"w{case}_image"], RS[f"w{case}_psf"], m["balance"], m["clip"])
+    assert _rel(out, RS[f"w{case}_out64"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["unsupervised"]])
+def test_oracle_reproduces_unsupervised_fixture(oracle, case):
+    m = META["unsupervised"][case]
+    img, chain = oracle.unsupervised_wiener64(RS[f"u{case}_image"], RS[f"u{case}_psf"], m["num_iter"], m["burnin"],
+                                              m["clip"], m["seed"])
+    assert _rel(img, RS[f"u{case}_out64"]) < 1e-12
+    np.testing.assert_allclose(chain, RS[f"u{case}_chain"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["fft"]])
+def test_oracle_reproduces_fft_fixture(oracle, case):
+    m = META["fft"][case]
+    fr, fi = oracle.fft_axis64(RS[f"f{case}_re"], RS[f"f{case}_im"], inverse=m["inverse"])
+    assert _rel(fr + 1j * fi, RS[f"f{case}_out_re"] + 1j * RS[f"f{case}_out_im"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["noise"]])
+def test_oracle_reproduces_noise_fixture(oracle, case):
+    m = META["noise"][case]
+    out = oracle.add_noise32(RS[f"n{case}_image"], m["kind"], m["p0"], m["p1"], seed=m["seed"], clip=m["clip"])
+    np.testing.assert_array_equal(out, RS[f"n{case}_out"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["wiener"]])
+def test_hip_matches_wiener_fixture(gpu, case):
+    import torch
+
+    import lucem_amd as LA
+
+    m = META["wiener"][case]
+    out = LA.wiener_deconvolve(torch.from_numpy(RS[f"w{case}_image"]).cuda(),
+                               torch.from_numpy(RS[f"w{case}_psf"]).cuda(),
+                               LA.WienerDeconvolve(balance=m["balance"], clip=m["clip"]))
+    e = _rel(out.cpu().numpy(), RS[f"w{case}_out64"])
+    print(f"wiener case {case} {m['shape']}: HIP vs FP64 {e:.2e}")
+    assert e < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["unsupervised"]])
+def test_hip_matches_unsupervised_fixture(gpu, case):
+    import torch
+
+    import lucem_amd as LA
+
+    m = META["unsupervised"][case]
+    img, chain = LA.unsupervised_wiener(torch.from_numpy(RS[f"u{case}_image"]).cuda(),
+                                        torch.from_numpy(RS[f"u{case}_psf"]).cuda(),
+                                        LA.UnsupervisedWiener(num_iter=m["num_iter"], burnin=m["burnin"],
+                                                              clip=m["clip"], seed=m["seed"]))
+    assert _rel(img.cpu().numpy(), RS[f"u{case}_out64"]) < 1e-4
+    np.testing.assert_allclose(chain, RS[f"u{case}_chain"], rtol=1e-4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["fft"]])
+def test_hip_matches_fft_fixture(gpu, case):
+    import torch
+
+    from lucem_amd import fft
+
+    m = META["fft"][case]
+    fr, fi = fft.fft_nd(torch.from_numpy(RS[f"f{case}_re"]).cuda(), torch.from_numpy(RS[f"f{case}_im"]).cuda(),
+                        inverse=m["inverse"])
+    got = fr.cpu().numpy() + 1j * fi.cpu().numpy()
+    assert _rel(got, RS[f"f{case}_out_re"] + 1j * RS[f"f{case}_out_im"]) < 2e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["noise"]])
+def test_hip_matches_noise_fixture(gpu, case):
+    import torch
+
+    import lucem_amd as LA
+
+    m = META["noise"][case]
+    out = LA.add_noise(torch.from_numpy(RS[f"n{case}_image"]).cuda(), LA.NoiseModel(m["kind"], m["p0"], m["p1"]),
+                       m["seed"], m["clip"])
+    np.testing.assert_array_equal(out.cpu().numpy(), RS[f"n{case}_out"])
