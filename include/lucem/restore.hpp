@@ -6,6 +6,11 @@
 //   lucem::RichardsonLucy                crates/lucem-ops/src/restore/deconvolve.rs:19-41
 //   lucem::richardson_lucy_into          crates/lucem-ops/src/restore/deconvolve.rs:45-149
 //   lucem::richardson_lucy               crates/lucem-ops/src/restore/deconvolve.rs:152-162
+//   lucem::WienerDeconvolve, wiener_deconvolve_into, wiener_deconvolve
+//                                        crates/lucem-ops/src/restore/deconvolve.rs:164-327
+//   lucem::UnsupervisedWiener, unsupervised_wiener
+//                                        crates/lucem-ops/src/restore/deconvolve.rs:330-509
+//   lucem::NoiseModel, add_noise         crates/lucem-ops/src/intensity/noise.rs:17-143
 //   lucem::Client, Workspace, DeviceArray crates/lucem-core/src/compute/target.rs:95-115,
 //                                        memory/workspace.rs:17-37, memory/array.rs:20-151
 //
@@ -15,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <optional>
@@ -226,6 +232,106 @@ inline OpsResult<Unit> richardson_lucy_nd_into(const Client& client, const Devic
         client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
         psf.dims().data(), (uint32_t)psf.dims().size(), output.data(), output.dims().data(),
         (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+// Settings of wiener_deconvolve (deconvolve.rs:164-181).
+struct WienerDeconvolve {
+    double balance = 0.1;
+    bool clip = true;
+
+    lucem_wiener_settings to_c() const {
+        lucem_wiener_settings s;
+        lucem_wiener_settings_default(&s);
+        s.balance = balance;
+        s.clip = clip ? 1 : 0;
+        return s;
+    }
+};
+
+// wiener_deconvolve_into::<f32> (deconvolve.rs:213-315).
+inline OpsResult<Unit> wiener_deconvolve_into(const Client& client, const DeviceArray<float>& input,
+                                              const DeviceArray<float>& psf, const DeviceArray<float>& output,
+                                              const WienerDeconvolve& settings, Workspace& ws) {
+    const lucem_wiener_settings s = settings.to_c();
+    return status_result(lucem_wiener_deconvolve_into_f32(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
+        psf.dims().data(), (uint32_t)psf.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+// wiener_deconvolve::<f32> (deconvolve.rs:318-327): allocates the output and a workspace.
+inline OpsResult<DeviceArray<float>> wiener_deconvolve(const Client& client, const DeviceArray<float>& input,
+                                                       const DeviceArray<float>& psf, const WienerDeconvolve& settings) {
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    Workspace ws(client);
+    auto r = wiener_deconvolve_into(client, input, psf, out.unwrap(), settings, ws);
+    if (!r.is_ok()) return OpsResult<DeviceArray<float>>(r.status(), r.message());
+    client.sync();
+    return out;
+}
+
+// Settings of unsupervised_wiener (deconvolve.rs:330-353).
+struct UnsupervisedWiener {
+    size_t num_iter = 30;
+    size_t burnin = 10;
+    bool clip = true;
+    uint64_t seed = 0;
+
+    lucem_unsupervised_wiener_settings to_c() const {
+        lucem_unsupervised_wiener_settings s;
+        lucem_unsupervised_wiener_settings_default(&s);
+        s.num_iter = num_iter;
+        s.burnin = burnin;
+        s.clip = clip ? 1 : 0;
+        s.seed = seed;
+        return s;
+    }
+};
+
+// unsupervised_wiener::<f32> (deconvolve.rs:357-509): (restored image, balance chain).
+inline OpsResult<std::pair<DeviceArray<float>, std::vector<double>>> unsupervised_wiener(
+    const Client& client, const DeviceArray<float>& input, const DeviceArray<float>& psf,
+    const UnsupervisedWiener& settings) {
+    using R = OpsResult<std::pair<DeviceArray<float>, std::vector<double>>>;
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return R(out.status(), out.message());
+    const lucem_unsupervised_wiener_settings s = settings.to_c();
+    std::vector<double> chain(std::max<size_t>(settings.num_iter, 1));
+    uint64_t len = 0;
+    Workspace ws(client);
+    const lucem_status st = lucem_unsupervised_wiener_f32(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), psf.data(),
+        psf.dims().data(), (uint32_t)psf.dims().size(), out.unwrap().data(), &s, chain.data(), chain.size(), &len,
+        ws.handle());
+    if (st != LUCEM_OK) return R(st, lucem_last_error());
+    chain.resize(len);
+    return std::make_pair(out.unwrap(), std::move(chain));
+}
+
+// NoiseModel (intensity/noise.rs:17-32).
+struct NoiseModel {
+    uint32_t kind;
+    double p0 = 0.0, p1 = 0.0;
+    static NoiseModel gaussian(double mean, double std) { return {LUCEM_NOISE_GAUSSIAN, mean, std}; }
+    static NoiseModel speckle(double mean, double std) { return {LUCEM_NOISE_SPECKLE, mean, std}; }
+    static NoiseModel salt(double amount) { return {LUCEM_NOISE_SALT, amount, 0.0}; }
+    static NoiseModel pepper(double amount) { return {LUCEM_NOISE_PEPPER, amount, 0.0}; }
+    static NoiseModel salt_and_pepper(double amount, double svp) { return {LUCEM_NOISE_SALT_AND_PEPPER, amount, svp}; }
+    static NoiseModel poisson(double scale) { return {LUCEM_NOISE_POISSON, scale, 0.0}; }
+};
+
+// add_noise::<f32> (intensity/noise.rs:35-143): a new array holding `input` with the noise.
+inline OpsResult<DeviceArray<float>> add_noise(const Client& client, const DeviceArray<float>& input,
+                                               const NoiseModel& model, uint64_t seed, bool clip) {
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    const lucem_noise_model m{model.kind, 0u, model.p0, model.p1};
+    const lucem_status st = lucem_add_noise_f32(client.handle(), input.data(), input.dims().data(),
+                                                (uint32_t)input.dims().size(), out.unwrap().data(), &m, seed,
+                                                clip ? 1 : 0);
+    if (st != LUCEM_OK) return OpsResult<DeviceArray<float>>(st, lucem_last_error());
+    return out;
 }
 
 }  // namespace lucem

@@ -70,6 +70,24 @@ int main() {
     auto out64 = richardson_lucy(client, in64, psf64, s).unwrap().to_host(client);
     for (int i = 0; i < 64; i++)
         CHECK(std::fabs(out64[i] - img64[i]) <= 1e-12, "f64 identity index %d: %.17g vs %.17g", i, out64[i], img64[i]);
+    // Wiener: the identity KAT of deconvolve.rs:542-550 (delta PSF, balance 1e-7, within 1e-3)
+    WienerDeconvolve wsettings;
+    wsettings.balance = 1e-7;
+    wsettings.clip = false;
+    auto wout = wiener_deconvolve(client, in, psf, wsettings).unwrap().to_host(client);
+    for (int i = 0; i < 64; i++) CHECK(std::fabs(wout[i] - img[i]) <= 1e-3, "wiener identity index %d", i);
+    // unsupervised Wiener on the blurred spot (deconvolve.rs:569-578): a chain of 12, an 8x8 image
+    UnsupervisedWiener us;
+    us.num_iter = 12;
+    us.burnin = 4;
+    us.clip = true;
+    us.seed = 7;
+    auto uw = unsupervised_wiener(client, bl, box, us).unwrap();
+    CHECK(uw.second.size() == 12, "chain length %zu", uw.second.size());
+    CHECK(uw.first.dims().size() == 2 && uw.first.dims()[0] == 8 && uw.first.dims()[1] == 8, "uw dims");
+    // add_noise: clipped Gaussian noise stays in [0, 1] (intensity/mod.rs:52-63)
+    auto noisy = add_noise(client, in, NoiseModel::gaussian(0.0, 0.05), 42, true).unwrap().to_host(client);
+    for (float v : noisy) CHECK(v >= 0.0f && v <= 1.0f, "noise out of range %f", v);
     // error order
     Workspace ws(client);
     auto out86 = DeviceArray<float>::empty(client, {8, 6}).unwrap();
