@@ -300,6 +300,24 @@ lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_rl_tile* ti
 /* Runs the planned batch on these buffers (same shapes and PSF indices as at creation; device
  * pointers may change from call to call) and `psfs[j]` (device pointers). Enqueues and returns. */
 lucem_status lucem_rl_batch_execute(lucem_rl_batch* batch, const lucem_rl_tile* tiles, const float* const* psfs);
+/* Where each tile reads its input and stores its result, for tiles that are views into larger
+ * arrays (custom::transform's block walk, engine/lucem-pipeline/src/custom.rs:177-196, with the
+ * owned-part crop fused into the last store). Strides count elements; 0 means the tile's own
+ * contiguous layout. The stored window (origin and size per axis in tile coordinates, z y x; a
+ * 2-D tile uses the last two entries; size 0: to the tile's end) is written at `output`, which
+ * points at the window's first element. No reference counterpart. */
+typedef struct lucem_rl_tile_layout {
+    uint64_t in_row_stride, in_plane_stride;
+    uint64_t out_row_stride, out_plane_stride;
+    uint64_t own_origin[3];
+    uint64_t own_size[3];
+} lucem_rl_tile_layout;
+/* lucem_rl_batch_execute with one layout per tile. A window outside its tile, or a stride shorter
+ * than the rows it spans, is LUCEM_INVALID_PARAMETER; a stored range overlapping any tile's input
+ * range is LUCEM_ALIASED_BUFFERS (every tile reads its input in every iteration). Stored windows
+ * of different tiles must not overlap one another (they are written concurrently). */
+lucem_status lucem_rl_batch_execute_layout(lucem_rl_batch* batch, const lucem_rl_tile* tiles,
+                                           const lucem_rl_tile_layout* layouts, const float* const* psfs);
 lucem_status lucem_rl_batch_destroy(lucem_rl_batch* batch);
 /* Device memory the plan holds (workspace, transfer functions, descriptors). */
 uint64_t lucem_rl_batch_device_bytes(const lucem_rl_batch* batch);

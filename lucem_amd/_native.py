@@ -44,7 +44,7 @@ EXPORTED = [
     "lucem_workspace_capacity_bytes", "lucem_richardson_lucy_into_f32", "lucem_richardson_lucy_f32",
     "lucem_richardson_lucy_nd_into_f32", "lucem_richardson_lucy_into_f64", "lucem_richardson_lucy_f64",
     "lucem_fft_c2c_lines_f32", "lucem_client_profile",
-    "lucem_client_profile_read", "lucem_rl_batch_create", "lucem_rl_batch_execute", "lucem_rl_batch_destroy",
+    "lucem_client_profile_read", "lucem_rl_batch_create", "lucem_rl_batch_execute", "lucem_rl_batch_execute_layout", "lucem_rl_batch_destroy",
     "lucem_rl_batch_device_bytes", "lucem_rl_batch_launches", "lucem_richardson_lucy_batched_f32",
     "lucem_wiener_settings_default", "lucem_wiener_deconvolve_into_f32", "lucem_wiener_deconvolve_f32",
     "lucem_unsupervised_wiener_settings_default", "lucem_unsupervised_wiener_f32",

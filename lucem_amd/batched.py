@@ -12,7 +12,7 @@ from dataclasses import dataclass
 from typing import Sequence
 
 from . import _native as N
-from ._native_ext import RlTile
+from ._native_ext import RlTile, RlTileLayout
 from .restore import RichardsonLucy
 
 
@@ -85,6 +85,27 @@ def schedule(dims, block, halo) -> list[Block]:
     return out
 
 
+@dataclass(frozen=True)
+class TileLayout:
+    """``lucem_rl_tile_layout``: element strides of a tile's input and output views (0: the tile's
+    own contiguous layout) and the stored window (z, y, x; a 2-D tile uses y and x; size 0: to
+    the tile's end)."""
+
+    in_row_stride: int = 0
+    in_plane_stride: int = 0
+    out_row_stride: int = 0
+    out_plane_stride: int = 0
+    own_origin: tuple = (0, 0, 0)
+    own_size: tuple = (0, 0, 0)
+
+    def fill(self, c: RlTileLayout) -> None:
+        c.in_row_stride, c.in_plane_stride = int(self.in_row_stride), int(self.in_plane_stride)
+        c.out_row_stride, c.out_plane_stride = int(self.out_row_stride), int(self.out_plane_stride)
+        for a in range(3):
+            c.own_origin[a] = int(self.own_origin[a])
+            c.own_size[a] = int(self.own_size[a])
+
+
 def _tiles(inputs, outputs, psf_index) -> ctypes.Array:
     arr = (RlTile * max(len(inputs), 1))()
     for i, x in enumerate(inputs):
@@ -150,6 +171,30 @@ class RLBatch:
         arr = _tiles(inputs, outputs, self.psf_index)
         pp = (N._vp * max(len(psfs), 1))(*[N.device_ptr(p, f"psf {j}") for j, p in enumerate(psfs)])
         N.check(N.lib().lucem_rl_batch_execute(self._h, arr, pp))
+
+    def execute_layout(self, in_ptrs: Sequence[int], out_ptrs: Sequence[int], layouts: Sequence["TileLayout"],
+                       psfs) -> None:
+        """Enqueues the batch on views into larger arrays: per tile a device address of its input's
+        first element, of its stored window's first element in the output, and its layout."""
+        n = len(self.shapes)
+        if len(in_ptrs) != n or len(out_ptrs) != n or len(layouts) != n:
+            raise ValueError("one input, output and layout per planned tile")
+        psfs = [N.as_tensor(p) for p in psfs]
+        if len(psfs) != len(self.psf_shapes):
+            raise ValueError("psfs must match the plan")
+        arr = (RlTile * max(n, 1))()
+        lay = (RlTileLayout * max(n, 1))()
+        for i, s in enumerate(self.shapes):
+            t = arr[i]
+            t.input = int(in_ptrs[i]) or None
+            t.output = int(out_ptrs[i]) or None
+            for a, d in enumerate(s):
+                t.dims[a] = d
+            t.ndim = len(s)
+            t.psf_index = self.psf_index[i]
+            layouts[i].fill(lay[i])
+        pp = (N._vp * max(len(psfs), 1))(*[N.device_ptr(p, f"psf {j}") for j, p in enumerate(psfs)])
+        N.check(N.lib().lucem_rl_batch_execute_layout(self._h, arr, lay, pp))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

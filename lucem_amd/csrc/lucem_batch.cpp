@@ -563,7 +563,64 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     return LUCEM_OK;
 }
 
-extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const float* const* psfs) {
+namespace {
+
+// Byte range [lo, hi) a strided view of (nz, ny, nx) elements spans.
+struct Span {
+    uintptr_t lo, hi;
+    size_t tile;
+};
+
+Span view_span(const void* p, const int64_t n3[3], int64_t rs, int64_t ps, size_t t) {
+    const uintptr_t lo = (uintptr_t)p;
+    const uint64_t last = (uint64_t)(n3[0] - 1) * (uint64_t)ps + (uint64_t)(n3[1] - 1) * (uint64_t)rs + (uint64_t)n3[2];
+    return Span{lo, lo + last * sizeof(float), t};
+}
+
+// The layout of tile t applied to its descriptor: strides default to the tile's own, the window
+// to the whole tile; the window must lie inside the tile.
+lucem_status apply_layout(const lucem_rl_tile_layout* lay, const TilePlan& tp, size_t t, RowPass& r, Span* in,
+                          Span* out, const void* input, const void* output) {
+    const int64_t n3[3] = {tp.g.n[0], tp.g.n[1], tp.g.n[2]};
+    r.ny = tp.g.n[1];
+    r.strided_obs = 0;
+    r.window = 0;
+    r.obs_rs = r.out_rs = n3[2];
+    r.obs_ps = r.out_ps = n3[1] * n3[2];
+    for (int a = 0; a < 3; a++) {
+        r.own0[a] = 0;
+        r.own1[a] = (int)n3[a];
+    }
+    int64_t win[3] = {n3[0], n3[1], n3[2]};
+    if (lay) {
+        const int off = 3 - tp.g.ndim;  // 2-D tiles: the layout's z entries are ignored
+        if (lay->in_row_stride) r.obs_rs = (int64_t)lay->in_row_stride;
+        if (lay->in_plane_stride) r.obs_ps = (int64_t)lay->in_plane_stride;
+        if (lay->out_row_stride) r.out_rs = (int64_t)lay->out_row_stride;
+        if (lay->out_plane_stride) r.out_ps = (int64_t)lay->out_plane_stride;
+        for (int a = off; a < 3; a++) {
+            const uint64_t o = lay->own_origin[a];
+            const uint64_t sz = lay->own_size[a] ? lay->own_size[a] : (uint64_t)n3[a] - std::min<uint64_t>(o, n3[a]);
+            if (o >= (uint64_t)n3[a] || sz == 0 || o + sz > (uint64_t)n3[a])
+                return fail(LUCEM_INVALID_PARAMETER, "tile " + std::to_string(t) +
+                                                         ": the stored window lies outside the tile");
+            r.own0[a] = (int)o;
+            r.own1[a] = (int)(o + sz);
+            win[a] = (int64_t)sz;
+        }
+        if (r.obs_rs < n3[2] || (tp.g.ndim == 3 && r.obs_ps < n3[1] * r.obs_rs) || r.out_rs < win[2] ||
+            (tp.g.ndim == 3 && r.out_ps < win[1] * r.out_rs))
+            return fail(LUCEM_INVALID_PARAMETER, "tile " + std::to_string(t) + ": a stride is shorter than the rows it spans");
+        r.strided_obs = r.obs_rs != n3[2] || r.obs_ps != n3[1] * n3[2];
+        r.window = 1;
+    }
+    *in = view_span(input, n3, r.obs_rs, r.obs_ps, t);
+    *out = view_span(output, win, r.out_rs, r.out_ps, t);
+    return LUCEM_OK;
+}
+
+lucem_status batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const lucem_rl_tile_layout* layouts,
+                           const float* const* psfs) {
     if (!b) return fail(LUCEM_INVALID_HANDLE, "batch is NULL");
     if (b->tiles.size() && !tiles) return fail(LUCEM_INVALID_HANDLE, "tiles is NULL");
     if (b->psf_dims.size() && !psfs) return fail(LUCEM_INVALID_HANDLE, "psfs is NULL");
@@ -575,6 +632,9 @@ extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl
         if ((e = hipEventSynchronize(b->staged)) != hipSuccess) return hip_fail(e, "hipEventSynchronize");
         b->staged_pending = false;
     }
+    std::vector<Span> ins, outs;
+    ins.reserve(b->tiles.size());
+    outs.reserve(b->tiles.size());
     for (size_t t = 0; t < b->tiles.size(); t++) {
         const lucem_rl_tile& tl = tiles[t];
         const TilePlan& tp = b->tiles[t];
@@ -586,15 +646,33 @@ extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl
                                           tl.psf_index == tp.psf);
         if (!same) return fail(LUCEM_SHAPE_MISMATCH, "tile " + std::to_string(t) + " differs from the plan");
         if (tp.group < 0) continue;
-        const char* a = (const char*)tl.input;
-        const char* o = (const char*)tl.output;
-        const uint64_t bytes = (uint64_t)tp.n * sizeof(float);
-        if (a < o + bytes && o < a + bytes)
-            return fail(LUCEM_ALIASED_BUFFERS, "tile " + std::to_string(t) +
-                                                   ": input and output arrays must not share the same device buffer");
         if (!tl.input || !tl.output) return fail(LUCEM_INVALID_HANDLE, "tile pointer is NULL");
+        Span in, out;
+        lucem_status st = apply_layout(layouts ? &layouts[t] : nullptr, tp, t, b->h_rows[t], &in, &out, tl.input,
+                                       tl.output);
+        if (st) return st;
+        ins.push_back(in);
+        outs.push_back(out);
         b->h_rows[t].obs = tl.input;
         b->h_rows[t].out = tl.output;
+    }
+    // Every tile reads its input in every iteration while the others store their results, so no
+    // stored range may overlap any input range (strided views are compared by their extents).
+    std::sort(ins.begin(), ins.end(), [](const Span& x, const Span& y) { return x.lo < y.lo; });
+    std::vector<uintptr_t> reach(ins.size());
+    for (size_t i = 0; i < ins.size(); i++) reach[i] = std::max(ins[i].hi, i ? reach[i - 1] : 0);
+    for (const Span& o : outs) {
+        // inputs starting before o.hi: the last of them reaching past o.lo overlaps
+        const size_t k = std::lower_bound(ins.begin(), ins.end(), o.hi,
+                                          [](const Span& x, uintptr_t v) { return x.lo < v; }) -
+                         ins.begin();
+        if (k > 0 && reach[k - 1] > o.lo) {
+            if (!layouts)
+                return fail(LUCEM_ALIASED_BUFFERS, "tile " + std::to_string(o.tile) +
+                                                       ": input and output arrays must not share the same device buffer");
+            return fail(LUCEM_ALIASED_BUFFERS, "tile " + std::to_string(o.tile) +
+                                                   ": its stored window overlaps an input of the batch");
+        }
     }
     for (size_t j = 0; j < b->h_psf.size(); j++) {
         if (!psfs[j]) return fail(LUCEM_INVALID_HANDLE, "psf pointer is NULL");
@@ -614,6 +692,18 @@ extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl
     b->staged_pending = true;
     if ((e = hipGraphLaunch(b->exec, c->stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     return LUCEM_OK;
+}
+
+}  // namespace
+
+extern "C" lucem_status lucem_rl_batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const float* const* psfs) {
+    return batch_execute(b, tiles, nullptr, psfs);
+}
+
+extern "C" lucem_status lucem_rl_batch_execute_layout(lucem_rl_batch* b, const lucem_rl_tile* tiles,
+                                                      const lucem_rl_tile_layout* layouts, const float* const* psfs) {
+    if (b && b->tiles.size() && !layouts) return fail(LUCEM_INVALID_HANDLE, "layouts is NULL");
+    return batch_execute(b, tiles, layouts, psfs);
 }
 
 extern "C" lucem_status lucem_rl_batch_destroy(lucem_rl_batch* b) {

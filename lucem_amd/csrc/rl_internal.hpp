@@ -78,6 +78,14 @@ struct RowPass {
     float* out;          // ROW_UPDATE && last: output
     float2* spec;        // half spectra, read (C2R) and written (R2C) in place
     const float2* tw;    // twiddle tables
+    // Strided views of the batched form's tile layouts (all zero: the tile's own contiguous
+    // arrays). Row r of the tile is (z, y) = (r / ny, r % ny).
+    int ny;              // rows per plane
+    int strided_obs;     // ROW_RATIO reads obs[z * obs_ps + y * obs_rs + x]
+    int window;          // the last ROW_UPDATE stores only own0 <= (z, y, x) < own1, at
+                         // out[(z - own0[0]) * out_ps + (y - own0[1]) * out_rs + (x - own0[2])]
+    int own0[3], own1[3];
+    int64_t obs_rs, obs_ps, out_rs, out_ps;
 };
 
 // Line pass along the y or z axis of the half spectra.

@@ -37,6 +37,13 @@ struct RowFlags {
     int first, last, clip;
 };
 
+// Offset of tile row r in a strided view with row stride rs and plane stride ps.
+__device__ __forceinline__ int64_t view_row(const RowPass& d, int r, int64_t rs, int64_t ps) {
+    if (d.ny >= d.rows) return (int64_t)r * rs;
+    const int z = r / d.ny;
+    return (int64_t)z * ps + (int64_t)(r - z * d.ny) * rs;
+}
+
 template <int T, int E, int OP, int W>
 __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2* Z) {
     constexpr int S = E;  // elements per thread in the gather (L * P <= T * E)
@@ -109,8 +116,13 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                 if (src >= 0) {
                     const int ra = 2 * (pair0 + b), rb = ra + 1;
                     if constexpr (OP == ROW_RATIO) {
-                        if (ra < d.rows) val[s].x = d.obs[(int64_t)ra * nx + src];
-                        if (rb < d.rows) val[s].y = d.obs[(int64_t)rb * nx + src];
+                        if (d.strided_obs) {
+                            if (ra < d.rows) val[s].x = d.obs[view_row(d, ra, d.obs_rs, d.obs_ps) + src];
+                            if (rb < d.rows) val[s].y = d.obs[view_row(d, rb, d.obs_rs, d.obs_ps) + src];
+                        } else {
+                            if (ra < d.rows) val[s].x = d.obs[(int64_t)ra * nx + src];
+                            if (rb < d.rows) val[s].y = d.obs[(int64_t)rb * nx + src];
+                        }
                     } else {
                         if (!fl.first) {
                             if (ra < d.rows) val[s].x = d.est[(int64_t)ra * nx + src];
@@ -155,9 +167,26 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                         const int b = e - pd.div(e) * P;
                         const int x = i - d.left;
                         const int ra = 2 * (pair0 + b), rb = ra + 1;
-                        float* dst = fl.last ? d.out : d.est;
-                        if (ra < d.rows) dst[(int64_t)ra * nx + x] = val[s].x;
-                        if (rb < d.rows) dst[(int64_t)rb * nx + x] = val[s].y;
+                        if (fl.last && d.window) {
+                            if (x >= d.own0[2] && x < d.own1[2]) {
+                                const int rr[2] = {ra, rb};
+                                const float vv[2] = {val[s].x, val[s].y};
+#pragma unroll
+                                for (int q = 0; q < 2; q++) {
+                                    const int r = rr[q];
+                                    if (r >= d.rows) continue;
+                                    const int z = d.ny >= d.rows ? 0 : r / d.ny;
+                                    const int y = r - z * d.ny;
+                                    if (y >= d.own0[1] && y < d.own1[1] && z >= d.own0[0] && z < d.own1[0])
+                                        d.out[(int64_t)(z - d.own0[0]) * d.out_ps + (int64_t)(y - d.own0[1]) * d.out_rs +
+                                              (x - d.own0[2])] = vv[q];
+                                }
+                            }
+                        } else {
+                            float* dst = fl.last ? d.out : d.est;
+                            if (ra < d.rows) dst[(int64_t)ra * nx + x] = val[s].x;
+                            if (rb < d.rows) dst[(int64_t)rb * nx + x] = val[s].y;
+                        }
                     }
                 }
                 Z[e] = val[s];

@@ -4,7 +4,8 @@
 (engine/lucem-pipeline/src/custom.rs:171-251): each block is read with a halo
 (crates/lucem-core/src/tiling.rs:54-78), deconvolved, and only its owned part is kept
 (custom.rs:177-196). Here every block of every channel runs in one batched plan (one captured
-graph) instead of one call per block.
+graph) instead of one call per block, and nothing is copied: each tile reads its block straight
+from the image and its last store writes only the owned part, straight into the result.
 
 ``RichardsonLucyOp`` is the operator a pipeline would register for RL, in the shape of the
 reference's ``Op`` variants (engine/lucem-pipeline/src/ops.rs:97-203): its parameters, the
@@ -23,7 +24,7 @@ from dataclasses import dataclass, field
 from typing import Sequence
 
 from . import _native as N
-from .batched import RLBatch, schedule
+from .batched import RLBatch, TileLayout, schedule
 from .restore import PadMode, RichardsonLucy, richardson_lucy_into, richardson_lucy_nd_into
 
 
@@ -107,28 +108,30 @@ def deconvolve_blocks(image, psfs, op: RichardsonLucyOp, block: Sequence[int] = 
     hal = (0,) * (3 - nd) + (h,) * nd
     blocks = _blocks(dims3, blk, hal)
 
-    def region(b, t):
-        sl = tuple(slice(o, o + s) for o, s in zip(b.read_origin, b.read_size))[3 - nd:]
-        return t[sl]
-
-    reads, outs, index, where = [], [], [], []
-    for c in range(C):
-        for b in blocks:
-            r = region(b, chans[c]).contiguous()
-            reads.append(r)
-            outs.append(torch.empty_like(r))
-            index.append(pidx[c])
-            where.append((c, b))
-    plan = RLBatch([tuple(r.shape) for r in reads], [tuple(p.shape) for p in psf_list], index, op.settings(), client)
-    plan.execute(reads, outs, psf_list)
-    if client is not None:
-        client.sync()  # the copies below run on torch's current stream
+    if not chans.is_cuda or chans.dtype != torch.float32:
+        raise ValueError("image must be a float32 device tensor")
+    chans = chans.contiguous()
     result = torch.empty_like(chans)
-    for o, (c, b) in zip(outs, where):
-        inner = b.inner_offset()[3 - nd:]
-        size = b.size[3 - nd:]
-        src = tuple(slice(i, i + s) for i, s in zip(inner, size))
-        dst = tuple(slice(o0, o0 + s) for o0, s in zip(b.origin[3 - nd:], size))
-        result[c][dst] = o[src]
+    if not blocks:
+        return result if channel_axis else result[0]
+    # Zero-copy block walk: every tile reads its block straight from the image (row and plane
+    # strides of the channel) and stores only its owned part straight into the result.
+    esz = chans.element_size()
+    row = spatial[-1]
+    plane = spatial[-1] * spatial[-2]
+    shapes, in_ptrs, out_ptrs, layouts, index = [], [], [], [], []
+    for c in range(C):
+        base_in, base_out = chans[c].data_ptr(), result[c].data_ptr()
+        for b in blocks:
+            ro, o = b.read_origin, b.origin
+            shapes.append(tuple(b.read_size[3 - nd:]))
+            in_ptrs.append(base_in + esz * (ro[0] * plane + ro[1] * row + ro[2]))
+            out_ptrs.append(base_out + esz * (o[0] * plane + o[1] * row + o[2]))
+            layouts.append(TileLayout(row, plane, row, plane, b.inner_offset(), b.size))
+            index.append(pidx[c])
+    plan = RLBatch(shapes, [tuple(p.shape) for p in psf_list], index, op.settings(), client)
+    plan.execute_layout(in_ptrs, out_ptrs, layouts, psf_list)
+    if client is not None:
+        client.sync()  # the caller's next work may run on torch's current stream
     plan.close()
     return result if channel_axis else result[0]

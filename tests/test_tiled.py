@@ -102,3 +102,79 @@ def test_tiled_short_halo_runs(gpu):
     p = torch.from_numpy(rng.uniform(0.1, 1, (7, 7)).astype(np.float32)).cuda()
     out = deconvolve_blocks(x, p, RichardsonLucyOp(num_iter=3, clip=False), block=(128, 128), halo=8)
     assert out.shape == x.shape and bool(torch.isfinite(out).all())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nd", [2, 3])
+def test_layout_views_equal_plain_tiles(gpu, nd):
+    """Tiles given as strided views into one image, each storing a window into one result, give
+    the plain per-tile results inside their windows, and leave the rest of the result alone."""
+    import numpy as np
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import RLBatch, TileLayout
+
+    rng = np.random.default_rng(30 + nd)
+    full = (9, 40, 52) if nd == 3 else (1, 60, 77)
+    img = torch.from_numpy(rng.uniform(50, 150, full).astype(np.float32)).cuda()
+    res = torch.full(full, -7.0, device="cuda")
+    # (read origin, read size, window origin in the tile, window size): z y x
+    tiles = [((0, 0, 0), (full[0], 23, 31), (0, 2, 3), (full[0], 15, 20)),
+             ((0, 10, 30), (full[0], 30, 22), (0, 5, 1), (full[0], 21, 20)),
+             ((0, 35, 0), (full[0], full[1] - 35, 17), (0, 0, 0), (0, 0, 0))]
+    if nd == 3:
+        tiles.append(((2, 3, 40), (5, 20, 12), (1, 0, 2), (3, 4, 9)))  # windows do not overlap
+    psf = torch.from_numpy(rng.uniform(0.1, 1.0, (3, 5, 3) if nd == 3 else (5, 3)).astype(np.float32)).cuda()
+    s = LA.RichardsonLucy(num_iter=4, clip=False)
+    shapes = [tuple(size[3 - nd:]) for _, size, _, _ in tiles]
+    plan = RLBatch(shapes, [tuple(psf.shape)], [0] * len(tiles), s)
+    H, W = full[1], full[2]
+    esz = 4
+    ins, outs, lays = [], [], []
+    for (ro, size, wo, ws) in tiles:
+        ins.append(img.data_ptr() + esz * (ro[0] * H * W + ro[1] * W + ro[2]))
+        dst = tuple(r + w for r, w in zip(ro, wo))
+        outs.append(res.data_ptr() + esz * (dst[0] * H * W + dst[1] * W + dst[2]))
+        lays.append(TileLayout(W, H * W, W, H * W, wo, ws))
+    plan.execute_layout(ins, outs, lays, [psf])
+    torch.cuda.synchronize()
+    covered = torch.zeros(full, dtype=torch.bool, device="cuda")
+    for (ro, size, wo, ws) in tiles:
+        view = img[ro[0]:ro[0] + size[0], ro[1]:ro[1] + size[1], ro[2]:ro[2] + size[2]].contiguous()
+        if nd == 2:
+            view = view[0]
+        ref = torch.empty_like(view)
+        LA.richardson_lucy_nd_into(view, psf, ref, s)
+        torch.cuda.synchronize()
+        ref3 = ref if nd == 3 else ref.unsqueeze(0)
+        wsz = [w if w else size[a] - wo[a] for a, w in enumerate(ws)]
+        sl = tuple(slice(ro[a] + wo[a], ro[a] + wo[a] + wsz[a]) for a in range(3))
+        want = ref3[tuple(slice(wo[a], wo[a] + wsz[a]) for a in range(3))]
+        np.testing.assert_array_equal(res[sl].cpu().numpy(), want.cpu().numpy())
+        covered[sl] = True
+    assert bool((res[~covered] == -7.0).all())
+    plan.close()
+
+
+@pytest.mark.gpu
+def test_layout_errors(gpu):
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import RLBatch, TileLayout
+
+    img = torch.ones((30, 40), device="cuda")
+    res = torch.zeros((30, 40), device="cuda")
+    psf = torch.ones((3, 3), device="cuda")
+    plan = RLBatch([(10, 12)], [(3, 3)], [0])
+    with pytest.raises(LA.LucemError) as e:  # the window leaves the tile
+        plan.execute_layout([img.data_ptr()], [res.data_ptr()], [TileLayout(40, 0, 40, 0, (0, 5, 0), (0, 6, 12))], [psf])
+    assert e.value.kind == "INVALID_PARAMETER"
+    with pytest.raises(LA.LucemError) as e:  # a row stride shorter than the tile's rows
+        plan.execute_layout([img.data_ptr()], [res.data_ptr()], [TileLayout(8, 0, 40, 0)], [psf])
+    assert e.value.kind == "INVALID_PARAMETER"
+    with pytest.raises(LA.LucemError) as e:  # the stored window writes into the image being read
+        plan.execute_layout([img.data_ptr()], [img.data_ptr() + 4 * 45], [TileLayout(40, 0, 40, 0)], [psf])
+    assert e.value.kind == "ALIASED_BUFFERS"
+    plan.close()
