@@ -44,7 +44,9 @@ __device__ __forceinline__ int64_t view_row(const RowPass& d, int r, int64_t rs,
     return (int64_t)z * ps + (int64_t)(r - z * d.ny) * rs;
 }
 
-template <int T, int E, int OP, int W>
+// VIEWS: honour the strided views of the descriptor (the batched form); the one-array kernels
+// are compiled without them.
+template <int T, int E, int OP, int W, bool VIEWS = false>
 __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2* Z) {
     constexpr int S = E;  // elements per thread in the gather (L * P <= T * E)
     const int P = d.P;
@@ -116,7 +118,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                 if (src >= 0) {
                     const int ra = 2 * (pair0 + b), rb = ra + 1;
                     if constexpr (OP == ROW_RATIO) {
-                        if (d.strided_obs) {
+                        if (VIEWS && d.strided_obs) {
                             if (ra < d.rows) val[s].x = d.obs[view_row(d, ra, d.obs_rs, d.obs_ps) + src];
                             if (rb < d.rows) val[s].y = d.obs[view_row(d, rb, d.obs_rs, d.obs_ps) + src];
                         } else {
@@ -167,7 +169,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                         const int b = e - pd.div(e) * P;
                         const int x = i - d.left;
                         const int ra = 2 * (pair0 + b), rb = ra + 1;
-                        if (fl.last && d.window) {
+                        if (VIEWS && fl.last && d.window) {
                             if (x >= d.own0[2] && x < d.own1[2]) {
                                 const int rr[2] = {ra, rb};
                                 const float vv[2] = {val[s].x, val[s].y};
@@ -436,7 +438,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(LUCEM_BATCH_W
         if (it >= bt.nitems) break;
         const int2 item = bt.items[it];
         const RowPass d = bt.tiles[__builtin_amdgcn_readfirstlane(item.x)];  // a private copy: no aliasing
-        row_pass_body<T, E, OP, LUCEM_BATCH_WAVES_PER_EU>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
+        row_pass_body<T, E, OP, LUCEM_BATCH_WAVES_PER_EU, true>(d, RowFlags{bt.first, bt.last, bt.clip}, __builtin_amdgcn_readfirstlane(item.y), lds);
         __syncthreads();
     }
 }
