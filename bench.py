@@ -72,6 +72,9 @@ def load_traffic(workload: str, kernel: str):
 
 
 def cpu_baseline(prob, workload: str, want_seconds: float = 15.0):
+    # one OpenMP thread per core, pinned (read when the oracle's OpenMP runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
 
     O.build()
@@ -92,7 +95,18 @@ def cpu_baseline(prob, workload: str, want_seconds: float = 15.0):
         "cores": O.num_threads(),
         "kind": "port",
         "sample": sample + f"; FP32 direct correlation, {dt:.1f} s",
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _timeit(fn, reps: int) -> float:
