@@ -106,7 +106,7 @@ static lucem_status run_seq(hipStream_t s, const std::vector<GLaunch>& seq, int*
         int err = 0;
         switch (L.kind) {
             case GLaunch::ROW: {
-                RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, 0, 0, 0, 0, nullptr};
+                RowBatch rb{static_cast<const RowPass*>(L.descs), L.items, L.nitems, L.op, 0, 0, 0, L.wide, nullptr};
                 err = launch_row_batch(rb, L.nitems, L.lds, s);
                 break;
             }
@@ -142,11 +142,6 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
                                       tile_dims + t * ndim, ndim, settings, 3, &empty, &P->geos[t]);
         if (st) return fail(st, "tile " + std::to_string(t) + ": " + lucem_last_error());
         if (empty) return fail(LUCEM_INVALID_PARAMETER, "tile " + std::to_string(t) + " is empty");
-        for (int a = 0; a < 3; a++)
-            if (P->geos[t].L[a] > kLdsComplex)
-                return fail(LUCEM_UNSUPPORTED_SHAPE, "tile " + std::to_string(t) +
-                                                         ": the differentiable form handles FFT lengths up to 4096 "
-                                                         "(n + k - 1 per axis)");
     }
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     hipError_t e = hipSetDevice(client->device);
@@ -565,8 +560,16 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
     };
     auto id = [](const RowPass& r) -> const RowPass& { return r; };  // OTF rows are plain RowPass
     auto gr = [](const GradTile& g) -> const RowPass& { return g.r; };
+    // one thread count per launch: the widest tier any of its items needs (the others fit in it)
+    auto wide_row = [&](const auto& descs, const std::vector<int2>& items, auto get) {
+        int w = 0;
+        for (const auto& it : items) w = std::max(w, get(descs[it.x]).wide);
+        return w;
+    };
     const size_t lds_row = maxlds_row(P->h_tiles, h_it_row, gr);
     const size_t lds_fin = maxlds_row(h_fin_h, h_it_fin, gr);
+    const int wrow = wide_row(P->h_tiles, h_it_row, gr);
+    const int wfin = std::max(wide_row(h_fin_h, h_it_fin, gr), wide_row(h_fin_r, h_it_fin, gr));
     int wide_col[3], wide_gi[2], wide_otf[2];
     size_t lds_col[3], lds_gi[2], lds_otf[2];
     for (int i = 0; i < ncorr; i++) lds_col[i] = maxlds_col(h_a1[i], h_it_col[i], &wide_col[i]);
@@ -575,12 +578,13 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
         lds_otf[i] = maxlds_col(h_otf_cols[i], h_it_otf_col[i], &wide_otf[i]);
     }
     const size_t lds_otf_row = maxlds_row(h_otf_rows, h_it_otf_row, id);
+    const int wotf_row = wide_row(h_otf_rows, h_it_otf_row, id);
     const int nrow = (int)h_it_row.size();
 
     auto grow = [&](int op, int sx, int sx1, int su, int sxm, int scm, int ck, int flags) {
         GLaunch L{};
         L.kind = GLaunch::GROW;
-        L.g = GradRowBatch{d_tiles, it_row, nrow, op, sx, sx1, su, sxm, scm, ck, flags};
+        L.g = GradRowBatch{d_tiles, it_row, nrow, op, sx, sx1, su, sxm, scm, ck, flags, wrow};
         L.lds = lds_row;
         return L;
     };
@@ -622,6 +626,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
         L.items = it_otf_row;
         L.nitems = (int)h_it_otf_row.size();
         L.lds = lds_otf_row;
+        L.wide = wotf_row;
         fwd.push_back(L);
         for (int i = 0; i < (three ? 2 : 1); i++) {
             GLaunch c{};
@@ -685,7 +690,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
         }
         GLaunch L{};
         L.kind = GLaunch::GROW;
-        L.g = GradRowBatch{w == 0 ? d_fin_h : d_fin_r, it_fin, (int)h_it_fin.size(), G_C2R, 0, 0, 0, 0, 0, 0, 0};
+        L.g = GradRowBatch{w == 0 ? d_fin_h : d_fin_r, it_fin, (int)h_it_fin.size(), G_C2R, 0, 0, 0, 0, 0, 0, 0, wfin};
         L.lds = lds_fin;
         bwd.push_back(L);
     }

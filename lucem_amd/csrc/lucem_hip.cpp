@@ -23,6 +23,7 @@
 #include "../../include/lucem_hip.h"
 #include "lucem_internal_host.hpp"
 #include "rl_internal.hpp"
+#include "rl_grad.hpp"
 
 using namespace lucem;
 
@@ -107,6 +108,7 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
     std::call_once(lds_once, [] {
         lds_err = allow_big_lds_kernels();
         if (!lds_err) lds_err = allow_big_lds_axis_kernels();
+        if (!lds_err) lds_err = allow_big_lds_grad_kernels();
     });
     if (lds_err) return hip_fail(lds_err, "hipFuncSetAttribute(dynamic LDS)");
     if (!own_stream) {
@@ -456,7 +458,8 @@ int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& 
         *wide = 2;
         return std::max(1, kLdsComplexXL / L);
     }
-    if (const ShapeChoice* c = find_choice(sp, kind, L)) {
+    const ShapeChoice* c = find_choice(sp, kind, L);
+    if (c && (int64_t)L * c->lines <= (int64_t)tier_threads(c->wide) * kElemsPerThread) {  // a valid measured shape
         *variant = c->variant;
         *wide = c->wide;
         return c->lines;
@@ -465,6 +468,7 @@ int lucem::choose_B(int L, int Lc, int64_t outers, int kind, const ShapePolicy& 
     const int b_big = std::max(1, std::min(32, kLdsComplexWide / L));
     int w = tune_value("col_wide");
     if (w < 0) w = (b_small < 8 && b_big > b_small) ? 1 : 0;
+    if (L > kLdsComplex) w = 1;  // a line longer than the 256-thread kernel holds
     const size_t cap = w ? kLdsComplexWide : kLdsComplex;
     int B = w ? b_big : b_small;
     const int64_t lines = std::max<int64_t>(outers * Lc, sp.launch_lines);

@@ -311,6 +311,9 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         int64_t lines = 0;
         for (const TuneCol& tc : g) lines += (int64_t)tc.outers * tc.d.Lc;
         const bool fixed_wide = kind_wide.count(kind) > 0;
+        // a line longer than 4096 needs 512 threads: not tunable under a 256-thread launch (the
+        // heuristic shape stays, and the launch runs at the widest thread count of its groups)
+        if (L > kLdsComplex && fixed_wide && kind_wide[kind] == 0) continue;
         const int wide_key = fixed_wide ? kind_wide[kind] : 2;
         const CacheKey key{kind * 4 + wide_key, L, g[0].d.op, ilog2(lines), mode};
         {
@@ -334,9 +337,11 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             plans.push_back(f);
             for (int w = 0; w < 2; w++) {
                 if (fixed_wide && w != kind_wide[kind]) continue;
+                if (w == 0 && L > kLdsComplex) continue;  // the line does not fit 256 threads
                 const int cap = w ? kLdsComplexWide : kLdsComplex;
                 const int bmax = std::max(1, std::min(32, cap / L));
-                if (w == 1 && !fixed_wide && bmax <= std::max(1, std::min(32, kLdsComplex / L))) continue;
+                if (w == 1 && !fixed_wide && L <= kLdsComplex && bmax <= std::max(1, std::min(32, kLdsComplex / L)))
+                    continue;
                 for (int B : line_candidates(bmax, g[0].d.wide == w ? g[0].d.B : bmax)) {
                     std::vector<ColPass> d;
                     std::vector<int2> items;

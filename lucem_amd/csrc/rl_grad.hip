@@ -305,21 +305,53 @@ __global__ void zero_floats_kernel(float* p, int64_t n) {
     if (t < n) p[t] = 0.f;
 }
 
-int launch_grad_row(const GradRowBatch& b, size_t lds, void* stream) {
-    hipStream_t st = (hipStream_t)stream;
-    const dim3 grid(b.nitems), block(kThreads);
+template <int T>
+static int launch_grad_row_t(const GradRowBatch& b, size_t lds, hipStream_t st) {
+    constexpr int E = kElemsPerThread;
+    const dim3 grid(b.nitems), block(T);
     switch (b.op) {
-        case G_LOAD_X: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOAD_X>), grid, block, lds, st, b); break;
-        case G_RATIO: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_RATIO>), grid, block, lds, st, b); break;
-        case G_UPDATE: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_UPDATE>), grid, block, lds, st, b); break;
-        case G_LOSS: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOSS>), grid, block, lds, st, b); break;
-        case G_ADJ_R: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_ADJ_R>), grid, block, lds, st, b); break;
-        case G_ADJ_X: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_ADJ_X>), grid, block, lds, st, b); break;
-        case G_SEED: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_SEED>), grid, block, lds, st, b); break;
-        case G_LOSS_ADJ: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_LOSS_ADJ>), grid, block, lds, st, b); break;
-        default: hipLaunchKernelGGL((grad_row_kernel<kThreads, kElemsPerThread, G_C2R>), grid, block, lds, st, b); break;
+        case G_LOAD_X: hipLaunchKernelGGL((grad_row_kernel<T, E, G_LOAD_X>), grid, block, lds, st, b); break;
+        case G_RATIO: hipLaunchKernelGGL((grad_row_kernel<T, E, G_RATIO>), grid, block, lds, st, b); break;
+        case G_UPDATE: hipLaunchKernelGGL((grad_row_kernel<T, E, G_UPDATE>), grid, block, lds, st, b); break;
+        case G_LOSS: hipLaunchKernelGGL((grad_row_kernel<T, E, G_LOSS>), grid, block, lds, st, b); break;
+        case G_ADJ_R: hipLaunchKernelGGL((grad_row_kernel<T, E, G_ADJ_R>), grid, block, lds, st, b); break;
+        case G_ADJ_X: hipLaunchKernelGGL((grad_row_kernel<T, E, G_ADJ_X>), grid, block, lds, st, b); break;
+        case G_SEED: hipLaunchKernelGGL((grad_row_kernel<T, E, G_SEED>), grid, block, lds, st, b); break;
+        case G_LOSS_ADJ: hipLaunchKernelGGL((grad_row_kernel<T, E, G_LOSS_ADJ>), grid, block, lds, st, b); break;
+        default: hipLaunchKernelGGL((grad_row_kernel<T, E, G_C2R>), grid, block, lds, st, b); break;
     }
     return (int)hipGetLastError();
+}
+
+int launch_grad_row(const GradRowBatch& b, size_t lds, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    switch (b.wide) {
+        case 2: return launch_grad_row_t<kThreadsXL>(b, lds, st);
+        case 1: return launch_grad_row_t<kThreadsWide>(b, lds, st);
+        default: return launch_grad_row_t<kThreads>(b, lds, st);
+    }
+}
+
+template <int T>
+static int allow_grad_tier() {
+    constexpr int E = kElemsPerThread;
+    const int bytes = (int)(lds_elems(kLdsComplexXL, kLdsComplexXL) * sizeof(float2));
+    int err = 0;
+    for (const void* k : {(const void*)grad_row_kernel<T, E, G_LOAD_X>, (const void*)grad_row_kernel<T, E, G_RATIO>,
+                          (const void*)grad_row_kernel<T, E, G_UPDATE>, (const void*)grad_row_kernel<T, E, G_LOSS>,
+                          (const void*)grad_row_kernel<T, E, G_ADJ_R>, (const void*)grad_row_kernel<T, E, G_ADJ_X>,
+                          (const void*)grad_row_kernel<T, E, G_SEED>, (const void*)grad_row_kernel<T, E, G_LOSS_ADJ>,
+                          (const void*)grad_row_kernel<T, E, G_C2R>}) {
+        const int e = (int)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e && !err) err = e;
+    }
+    return err;
+}
+
+int allow_big_lds_grad_kernels() {
+    const int a = allow_grad_tier<kThreadsWide>();
+    const int b = allow_grad_tier<kThreadsXL>();
+    return a ? a : b;
 }
 
 int launch_loss_reduce(const double* partials, const int* first, const int* count, double* loss, int ntiles,

@@ -63,9 +63,12 @@ struct GradRowBatch {
     int scm;    // slot of c_{t-1}
     int ck;     // checkpoint index
     int flags;
+    int wide;   // thread tier of the launch: 0 = 256, 1 = 512, 2 = 1024 threads
 };
 
 int launch_grad_row(const GradRowBatch& b, size_t lds, void* stream);
+// Allows the 512- and 1024-thread gradient row kernels their dynamic LDS (once per process).
+int allow_big_lds_grad_kernels();
 int launch_loss_reduce(const double* partials, const int* first, const int* count, double* loss, int ntiles,
                        void* stream);
 int launch_psf_grad(const float* gh, const float* grev, int ntiles, int64_t nk, const float* psf, float* out,

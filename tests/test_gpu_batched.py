@@ -142,7 +142,7 @@ def test_batched_long_and_short_tiles(gpu, oracle):
     from lucem_amd.batched import richardson_lucy_batched
 
     rng = np.random.default_rng(91)
-    shapes = [(2, 9000), (40, 50), (9000, 2), (17, 300)]
+    shapes = [(2, 9000), (40, 50), (9000, 2), (17, 300), (6000, 4), (5000, 30)]
     ys = [rng.uniform(50, 150, s).astype(np.float32) for s in shapes]
     psf = rng.uniform(0.1, 1.0, (3, 5)).astype(np.float32)
     s = LA.RichardsonLucy(num_iter=3, clip=False)

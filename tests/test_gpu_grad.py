@@ -159,3 +159,34 @@ def test_backward_after_other_forward_recomputes(gpu):
     torch.cuda.synchronize()
     np.testing.assert_allclose(x0[0].grad.cpu().numpy(), g_ref.cpu().numpy(), rtol=1e-6)
     del l2
+
+
+# Axes whose FFT is longer than 4096 take the 512- and 1024-thread kernels in every pass of the
+# forward, the adjoint and the PSF-gradient finish.
+LONG_GRAD_CASES = [
+    ((2, 6000), (1, 5), 0),     # rows of L = 6048: 512 threads
+    ((5000, 2), (5, 1), 3),     # columns of L = 5000: 512-thread line passes (transposed ones too)
+    ((2, 12000), (3, 3), 4),    # rows of L = 12005: 1024 threads
+    ((3, 9000, 2), (3, 3, 1), 0),  # 3-D: y lines of L = 9002 -> 9072, 1024 threads
+]
+
+
+@pytest.mark.parametrize("shape,kshape,mode", LONG_GRAD_CASES)
+def test_grad_long_axes_match_fp64_oracle(gpu, oracle, shape, kshape, mode):
+    rng = np.random.default_rng(sum(shape) + mode)
+    x0s = _tiles(rng, 2, shape, 0.5, 2.0)
+    ys = _tiles(rng, 2, shape)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    T = 3
+    loss, gx, gp = _run(x0s, ys, psf, T, k=2, mode=mode)
+    gp_ref = np.zeros(kshape)
+    for i in range(2):
+        L, gxr, gpr = oracle.rl_nll_grad64(ys[i], x0s[i], psf, T, mode=mode)
+        assert abs(loss[i] - L) <= 1e-6 * abs(L)
+        e = _rel(gx[i], gxr)
+        print(f"{shape} tile {i}: dL/dx0 rel-L2 vs FP64 {e:.2e}")
+        assert e < GRAD_TOL
+        gp_ref += gpr
+    e = _rel(gp, gp_ref)
+    print(f"{shape}: dL/dpsf rel-L2 vs FP64 {e:.2e}")
+    assert e < GRAD_TOL

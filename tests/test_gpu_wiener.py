@@ -41,6 +41,8 @@ WIENER_CASES = [
     ((100, 97), (9, 7), 0.02, True),
     ((11, 1), (3, 1), 0.5, False),
     ((509, 768), (15, 15), 0.1, False),
+    ((4200, 16), (7, 3), 0.1, False),   # a y line of 4200 (7-smooth) takes the 512-thread pass
+    ((16, 4200), (3, 7), 0.1, False),   # an x line of 4200 takes the 512-thread row pass
 ]
 
 
