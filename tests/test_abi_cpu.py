@@ -233,3 +233,19 @@ def test_fft_check_order_without_device(L):
     cm = L.lucem_complex_mul_into_f32(None, *arr(0x10000, (3, 4)), N._vp(0x1000), *arr(0x2000, (3, 5)),
                                       N._vp(0x3000), *arr(0x4000, (3, 4)), N._vp(0x5000), 0)
     assert cm == 1
+
+
+def test_launch_shapes_fit_their_tiers(L):
+    """Host-side planning invariants over every 7-smooth FFT length up to 16384 (tests/cpp/
+    test_shapes.cpp, built by build()): row and line shapes fit the thread tier they name, and
+    the radix plans multiply out to the length."""
+    import subprocess
+
+    exe = ROOT / "tests" / "cpp" / "test_shapes"
+    if not exe.exists():
+        from lucem_amd import build
+
+        build.build()
+        pytest.skip("tests/cpp/test_shapes is built by __graft_entry__.build()")
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and res.stdout.strip() == "ok", res.stderr
