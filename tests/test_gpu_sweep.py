@@ -1,0 +1,126 @@
+"""Seeded random sweeps over shapes against the oracles: the plain RL form in 2-D and 3-D, the
+batched form, the Wiener filter and the complex FFT, at sizes from 1 up to a few hundred (odd,
+prime and 7-smooth), PSFs from 1 to 31 taps, every pad mode. Each case prints what it ran."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _size(rng, lo, hi):
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def _dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rl_2d_random_shapes(gpu, oracle, seed):
+    import torch
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(700 + seed)
+    for case in range(8):
+        shape = (_size(rng, 1, 400), _size(rng, 1, 400))
+        kshape = (int(rng.integers(1, 32)), int(rng.integers(1, 32)))
+        mode, clip, it = int(rng.integers(0, 5)), bool(rng.integers(0, 2)), int(rng.integers(1, 4))
+        y = rng.uniform(0.05, 0.95, shape) if clip else rng.uniform(10, 200, shape)
+        y = y.astype(np.float32)
+        p = rng.uniform(0.05, 1.0, kshape).astype(np.float32)
+        out = torch.empty(shape, device="cuda")
+        LA.richardson_lucy_into(_dev(y), _dev(p), out, LA.RichardsonLucy(num_iter=it, clip=clip, mode=mode))
+        torch.cuda.synchronize()
+        st, ref = oracle.richardson_lucy(y, p, it, clip=clip, mode=mode)
+        e = oracle.rel_l2(out.cpu().numpy(), ref)
+        print(f"seed {seed} case {case}: {shape} psf {kshape} mode {mode} clip {clip} iters {it}: {e:.2e}")
+        assert st == 0 and e < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_rl_3d_random_shapes(gpu, oracle, seed):
+    import torch
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(800 + seed)
+    for case in range(5):
+        shape = (_size(rng, 1, 40), _size(rng, 1, 70), _size(rng, 1, 70))
+        kshape = tuple(int(rng.integers(1, 8)) for _ in range(3))
+        mode, it = int(rng.integers(0, 5)), int(rng.integers(1, 3))
+        y = rng.uniform(10, 200, shape).astype(np.float32)
+        p = rng.uniform(0.05, 1.0, kshape).astype(np.float32)
+        out = torch.empty(shape, device="cuda")
+        LA.richardson_lucy_nd_into(_dev(y), _dev(p), out, LA.RichardsonLucy(num_iter=it, clip=False, mode=mode))
+        torch.cuda.synchronize()
+        st, ref = oracle.richardson_lucy(y, p, it, clip=False, mode=mode, nd=True)
+        e = oracle.rel_l2(out.cpu().numpy(), ref)
+        print(f"seed {seed} case {case}: {shape} psf {kshape} mode {mode} iters {it}: {e:.2e}")
+        assert st == 0 and e < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_batched_random_tiles(gpu, oracle, seed):
+    import lucem_amd as LA
+    from lucem_amd.batched import richardson_lucy_batched
+
+    rng = np.random.default_rng(900 + seed)
+    psfs = [rng.uniform(0.05, 1.0, (int(rng.integers(1, 16)), int(rng.integers(1, 16)))).astype(np.float32)
+            for _ in range(3)]
+    shapes = [(_size(rng, 1, 300), _size(rng, 1, 300)) for _ in range(12)]
+    index = [int(rng.integers(0, 3)) for _ in shapes]
+    ys = [rng.uniform(10, 200, s).astype(np.float32) for s in shapes]
+    mode = int(rng.integers(0, 5))
+    outs = richardson_lucy_batched([_dev(y) for y in ys], [_dev(p) for p in psfs], index,
+                                   LA.RichardsonLucy(num_iter=3, clip=False, mode=mode))
+    for y, o, i in zip(ys, outs, index):
+        st, ref = oracle.richardson_lucy(y, psfs[i], 3, clip=False, mode=mode)
+        e = oracle.rel_l2(o.cpu().numpy(), ref)
+        print(f"seed {seed}: tile {y.shape} psf {psfs[i].shape} mode {mode}: {e:.2e}")
+        assert st == 0 and e < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_wiener_random_shapes(gpu, oracle, seed):
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(1000 + seed)
+    for case in range(8):
+        shape = (_size(rng, 1, 300), _size(rng, 1, 300))
+        kshape = (int(rng.integers(1, min(shape[0], 25) + 1)), int(rng.integers(1, min(shape[1], 25) + 1)))
+        balance = float(np.exp(rng.uniform(np.log(1e-3), np.log(1.0))))
+        y = rng.uniform(0, 1, shape).astype(np.float32)
+        p = rng.uniform(0.05, 1.0, kshape).astype(np.float32)
+        out = LA.wiener_deconvolve(_dev(y), _dev(p), LA.WienerDeconvolve(balance=balance, clip=False))
+        ref = oracle.wiener64(y, p, balance, False)
+        e = oracle.rel_l2(out.cpu().numpy(), ref)
+        print(f"seed {seed} case {case}: {shape} psf {kshape} balance {balance:.3g}: {e:.2e}")
+        assert e < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_fft_random_shapes(gpu, oracle, seed):
+    import torch
+
+    from lucem_amd import fft
+
+    rng = np.random.default_rng(1100 + seed)
+    for case in range(8):
+        nd = int(rng.integers(1, 4))
+        shape = tuple(_size(rng, 1, 3000 if nd == 1 else (300 if nd == 2 else 40)) for _ in range(nd))
+        re, im = rng.normal(size=shape).astype(np.float32), rng.normal(size=shape).astype(np.float32)
+        inverse = bool(rng.integers(0, 2))
+        fr, fi = fft.fft_nd(_dev(re), _dev(im), inverse=inverse)
+        torch.cuda.synchronize()
+        wr, wi = oracle.fft_axis64(re, im, inverse=inverse)
+        got = fr.cpu().numpy().astype(np.float64) + 1j * fi.cpu().numpy()
+        want = wr + 1j * wi
+        e = float(np.linalg.norm((got - want).ravel()) / max(np.linalg.norm(want.ravel()), 1e-30))
+        print(f"seed {seed} case {case}: {shape} inverse {inverse}: {e:.2e}")
+        assert e < 2e-6
