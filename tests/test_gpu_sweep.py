@@ -124,3 +124,30 @@ def test_fft_random_shapes(gpu, oracle, seed):
         e = float(np.linalg.norm((got - want).ravel()) / max(np.linalg.norm(want.ravel()), 1e-30))
         print(f"seed {seed} case {case}: {shape} inverse {inverse}: {e:.2e}")
         assert e < 2e-6
+
+
+def test_grad_random_shapes(gpu, oracle):
+    import torch
+
+    from lucem_amd.autograd import rl_nll
+
+    rng = np.random.default_rng(1200)
+    for case in range(6):
+        nd = 2 if case < 4 else 3
+        shape = tuple(_size(rng, 2, 120 if nd == 2 else 24) for _ in range(nd))
+        kshape = tuple(int(rng.integers(1, 10 if nd == 2 else 5)) for _ in range(nd))
+        mode, T = int(rng.integers(0, 5)), int(rng.integers(1, 4))
+        x0 = rng.uniform(0.5, 2.0, shape).astype(np.float32)
+        y = rng.uniform(10, 200, shape).astype(np.float32)
+        p = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+        xt = _dev(x0).requires_grad_(True)
+        pt = _dev(p).requires_grad_(True)
+        loss = rl_nll([xt], [_dev(y)], pt, T, checkpoint_every=2, mode=mode)
+        loss.sum().backward()
+        torch.cuda.synchronize()
+        L, gx, gp = oracle.rl_nll_grad64(y, x0, p, T, mode=mode)
+        ex = oracle.rel_l2(xt.grad.cpu().numpy(), gx)
+        ep = oracle.rel_l2(pt.grad.cpu().numpy(), gp)
+        print(f"case {case}: {shape} psf {kshape} mode {mode} T {T}: loss {abs(float(loss[0]) - L) / abs(L):.1e}, "
+              f"dx0 {ex:.2e}, dpsf {ep:.2e}")
+        assert abs(float(loss[0]) - L) <= 1e-6 * abs(L) and ex < 1e-3 and ep < 1e-3
