@@ -145,6 +145,7 @@ def test_grad_random_shapes(gpu, oracle):
         loss = rl_nll([xt], [_dev(y)], pt, T, checkpoint_every=2, mode=mode)
         loss.sum().backward()
         torch.cuda.synchronize()
+        loss = loss.detach()
         L, gx, gp = oracle.rl_nll_grad64(y, x0, p, T, mode=mode)
         ex = oracle.rel_l2(xt.grad.cpu().numpy(), gx)
         ep = oracle.rel_l2(pt.grad.cpu().numpy(), gp)
