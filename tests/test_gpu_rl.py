@@ -379,3 +379,24 @@ def test_rl_f64_errors(gpu):
     with pytest.raises(LA.LucemError) as e:
         LA.richardson_lucy_nd_into(v, torch.ones((3, 3, 3), dtype=torch.float64, device="cuda"), torch.empty_like(v))
     assert e.value.kind == "INVALID_DIMENSION"
+
+
+@pytest.mark.parametrize("shape,kshape", [((5000, 2, 3), (3, 1, 3)), ((2, 4800, 3), (1, 5, 3)), ((3, 2, 9000), (3, 1, 5))])
+def test_long_axes_3d_match_oracle(gpu, oracle, shape, kshape):
+    """3-D volumes whose z, y or x line is longer than 4096 (512- and 1024-thread passes)."""
+    import torch
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(sum(shape))
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    p = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    out = torch.empty(shape, device="cuda")
+    LA.richardson_lucy_nd_into(torch.from_numpy(y).cuda(), torch.from_numpy(p).cuda(), out,
+                               LA.RichardsonLucy(num_iter=3, clip=False))
+    torch.cuda.synchronize()
+    st, ref = oracle.richardson_lucy(y, p, 3, clip=False, nd=True)
+    assert st == 0
+    e = oracle.rel_l2(out.cpu().numpy(), ref)
+    print(f"{shape} psf {kshape}: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
