@@ -284,6 +284,7 @@ LONG_CASES = [
     ((9000, 3), (5, 3)),     # columns of L = 9072: 1024-thread line pass
     ((5000, 3), (5, 3)),     # columns of L = 5040: 512-thread line pass (one line per workgroup)
     ((4500, 40), (9, 5)),    # columns of L = 4508 -> 4536: 512 threads
+    ((5000, 300), (5, 3)),   # columns of L = 5040 with enough work to be autotuned
     ((2, 16000), (1, 3)),    # rows of L = 16128: 126 KiB of LDS
 ]
 
