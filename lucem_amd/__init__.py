@@ -34,3 +34,4 @@ from .wiener import (  # noqa: F401
 )
 from . import fft  # noqa: F401,E402
 from .noise import NoiseModel, add_noise, add_noise_into  # noqa: F401,E402
+from . import tiled  # noqa: F401,E402
