@@ -33,6 +33,8 @@ AXIS_CASES = [
     ((2, 8191), 1),           # prime: Bluestein of length 16384 (1024 threads)
     ((3, 16384), 1),          # 7-smooth at the 16384 limit
     ((1, 1), 0), ((7, 1), 1),
+    ((16384, 3), 0),          # strided at the 16384 limit (1024 threads)
+    ((3000, 5), 0),           # strided: the thread tier grows until 5 columns fit
 ]
 
 
