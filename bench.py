@@ -28,7 +28,10 @@ Alongside the throughput the line carries:
   the R2C passes; 2039x2048: the general FFT with Bluestein columns), the unsupervised form on
   1024x1024 (30 iterations, burn-in 10), and the FP64 numpy restatement timed on this host;
 - ``fft``: the general complex FFT over both axes of a 2048x2048 split re/im array, and of a
-  2039x2039 one (Bluestein lines), against 32 bytes per element (two read+write passes).
+  2039x2039 one (Bluestein lines), against 32 bytes per element (two read+write passes);
+- ``tiled``: the zero-copy block walk (3 channels of 2048x2048, blocks of 480 with a 32-px halo,
+  15x15 Gaussian PSF, 20 iterations), plan creation included, and the Poisson ``add_noise`` of a
+  2048x2048 image against its 8 bytes per pixel.
 """
 
 from __future__ import annotations
@@ -272,6 +275,30 @@ def fft_leg(steps: int) -> dict:
     return out
 
 
+def tiled_leg(steps: int) -> dict:
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import tiled
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    img = torch.rand((3, 2048, 2048), device="cuda", generator=g) * 100 + 50
+    psf = torch.from_numpy(_gaussian_psf(15, 2.0)).cuda()
+    op = tiled.RichardsonLucyOp(num_iter=20, clip=False)
+    t = _timeit(lambda: tiled.deconvolve_blocks(img, psf, op, block=(480, 480), halo=32), max(steps // 2, 3))
+    x = torch.rand((2048, 2048), device="cuda", generator=g)
+    o = torch.empty_like(x)
+    model = LA.NoiseModel.Poisson(20.0)
+    tn = _timeit(lambda: LA.add_noise_into(x, o, model, 7, False), max(steps, 10))
+    return {
+        "workload": "3 x 2048x2048, blocks 480 + 32-px halo, psf 15x15, 20 iterations",
+        "ms": round(t * 1e3, 3),
+        "Mpixel_iters_per_s": round(img.numel() * 20 / t / 1e6, 1),
+        "add_noise_poisson_2048_ms": round(tn * 1e3, 4),
+        "add_noise_GBps": round(8 * x.numel() / tn / 1e9, 1),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,6 +406,7 @@ def main() -> int:
         line["grad"] = grad_leg(args.steps)
         line["wiener"] = wiener_leg(args.steps, not args.no_cpu_baseline)
         line["fft"] = fft_leg(args.steps)
+        line["tiled"] = tiled_leg(args.steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))
