@@ -44,6 +44,14 @@
  *   lucem_fft_1d_into_f32 / lucem_fft_nd_into_f32 / lucem_complex_mul_into_f32
  *                         lucem_ops::core::fft::{fft_1d_into, fft_nd_into, complex_mul_into}
  *                         (crates/lucem-ops/src/core/fft.rs:13-135) on split re/im arrays
+ *   lucem_correlate_settings
+ *                         lucem_ops::filter::Correlate2d (crates/lucem-ops/src/filter/convolve.rs:113-123)
+ *   lucem_correlate_2d_into_f32 / lucem_correlate_2d_into_f64
+ *                         lucem_ops::filter::correlate_2d_into (filter/convolve.rs:126-166), the
+ *                         correlation A5/A7 of the RL iteration call; f32 through the FFT passes,
+ *                         f64 direct
+ *   lucem_correlate_nd_into_f32
+ *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
  */
 #ifndef LUCEM_HIP_H
 #define LUCEM_HIP_H
@@ -70,7 +78,8 @@ typedef enum lucem_status {
     LUCEM_HIP_ERROR = 10,          /* a HIP runtime call failed; see lucem_last_error()   */
     LUCEM_INVALID_HANDLE = 11,     /* NULL client / workspace / settings pointer          */
     LUCEM_AXIS_OUT_OF_BOUNDS = 12, /* CoreError::AxisOutOfBounds    (lucem-core shape.rs:50-58) */
-    LUCEM_EMPTY_INPUT = 13         /* OpsError::EmptyInput          (error.rs:54)         */
+    LUCEM_EMPTY_INPUT = 13,        /* OpsError::EmptyInput          (error.rs:54)         */
+    LUCEM_INVALID_ORIGIN = 14      /* OpsError::InvalidOrigin       (core/check.rs:59-72) */
 } lucem_status;
 
 /* PadMode ids (core/pad.rs:29-33). RL always uses cval = 0. */
@@ -350,6 +359,42 @@ lucem_status lucem_rl_grad_backward(lucem_rl_grad* plan, const float* loss_weigh
                                     float* grad_psf);
 lucem_status lucem_rl_grad_destroy(lucem_rl_grad* plan);
 uint64_t lucem_rl_grad_device_bytes(const lucem_rl_grad* plan);
+
+/* ---- Correlation (filter/convolve.rs:113-166) ------------------------------------------------
+ * Correlate2d settings: pad mode (PadMode::id), cval (read outside the array in Constant mode)
+ * and the window origin per axis of the input (a 2-D input uses origin[0..2)). Defaults
+ * (lucem_correlate_settings_default): Reflect, cval 0, origin 0. */
+typedef struct lucem_correlate_settings {
+    uint32_t mode;
+    int32_t origin[3];
+    double cval;
+} lucem_correlate_settings;
+void lucem_correlate_settings_default(lucem_correlate_settings* out);
+
+/* correlate_2d_into::<f32> (filter/convolve.rs:126-166): out[p] = sum_t w[t] P(in)[p - (k/2 + origin) + t]
+ * over the window t, P the boundary mode with cval. Checks in the reference's order: input ndim
+ * == 2, weights ndim == 2, same shape (input, output), no overlap of input and output, then per
+ * axis the origin (core/check.rs:59-72: a weights axis of size 0 is LUCEM_INVALID_WEIGHTS, an
+ * origin outside -(k/2)..=(k-1)/2 LUCEM_INVALID_ORIGIN); an empty input returns LUCEM_OK with
+ * nothing done. The sum runs as an FFT linear correlation (the passes of the RL path) on the
+ * padded domain n + k - 1 per axis, at most 16384 (LUCEM_UNSUPPORTED_SHAPE beyond): FP32 with a
+ * rounding error of about 1e-7 of the output's largest value, not the reference's sequential
+ * sum. Asynchronous on the client's stream; ws may be NULL (the client's own). */
+lucem_status lucem_correlate_2d_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                         uint32_t in_ndim, const float* weights, const uint64_t* w_dims,
+                                         uint32_t w_ndim, float* output, const uint64_t* out_dims, uint32_t out_ndim,
+                                         const lucem_correlate_settings* settings, lucem_workspace* ws);
+/* The same operator on 2-D or 3-D arrays (weights of the input's rank). */
+lucem_status lucem_correlate_nd_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                         uint32_t in_ndim, const float* weights, const uint64_t* w_dims,
+                                         uint32_t w_ndim, float* output, const uint64_t* out_dims, uint32_t out_ndim,
+                                         const lucem_correlate_settings* settings, lucem_workspace* ws);
+/* correlate_2d_into::<f64>: the same checks, the direct sum in FP64 accumulated in the order of
+ * filter/kernels/convolve.rs:61-118 (row of the window outer, column inner). */
+lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const double* input, const uint64_t* in_dims,
+                                         uint32_t in_ndim, const double* weights, const uint64_t* w_dims,
+                                         uint32_t w_ndim, double* output, const uint64_t* out_dims,
+                                         uint32_t out_ndim, const lucem_correlate_settings* settings);
 
 /* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
  * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is

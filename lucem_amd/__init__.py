@@ -13,6 +13,8 @@ Public surface (mirrors the reference's names):
 - :mod:`lucem_amd.fft`: ``fft_1d_into``, ``fft_nd_into``, ``complex_mul_into``
                                                    core/fft.rs:13-135
 - :class:`NoiseModel`, :func:`add_noise`          intensity/noise.rs:17-143
+- :class:`Correlate2d`, :func:`correlate_2d_into`, :func:`correlate_2d`
+                                                   filter/convolve.rs:113-177
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
 """
 
@@ -35,3 +37,4 @@ from .wiener import (  # noqa: F401
 from . import fft  # noqa: F401,E402
 from .noise import NoiseModel, add_noise, add_noise_into  # noqa: F401,E402
 from . import tiled  # noqa: F401,E402
+from .filter import Correlate2d, correlate_2d, correlate_2d_into, correlate_nd, correlate_nd_into  # noqa: F401,E402

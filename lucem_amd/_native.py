@@ -34,6 +34,7 @@ STATUS_NAMES = {
     11: "INVALID_HANDLE",
     12: "AXIS_OUT_OF_BOUNDS",
     13: "EMPTY_INPUT",
+    14: "INVALID_ORIGIN",
 }
 
 # Every symbol include/lucem_hip.h declares (checked by the CPU tests).
@@ -49,6 +50,8 @@ EXPORTED = [
     "lucem_wiener_settings_default", "lucem_wiener_deconvolve_into_f32", "lucem_wiener_deconvolve_f32",
     "lucem_unsupervised_wiener_settings_default", "lucem_unsupervised_wiener_f32",
     "lucem_fft_1d_into_f32", "lucem_fft_nd_into_f32", "lucem_complex_mul_into_f32", "lucem_add_noise_f32",
+    "lucem_correlate_settings_default", "lucem_correlate_2d_into_f32", "lucem_correlate_nd_into_f32",
+    "lucem_correlate_2d_into_f64",
 ]
 
 
@@ -93,6 +96,10 @@ class UnsupervisedWienerSettings(ctypes.Structure):
         ("reserved", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
     ]
+
+
+class CorrelateSettings(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("origin", ctypes.c_int32 * 3), ("cval", ctypes.c_double)]
 
 
 class NoiseModelC(ctypes.Structure):
@@ -154,6 +161,11 @@ def lib() -> ctypes.CDLL:
                 _vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32, _vp,
                 ctypes.POINTER(UnsupervisedWienerSettings), ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
                 ctypes.POINTER(ctypes.c_uint64), _vp]
+            L.lucem_correlate_settings_default.argtypes = [ctypes.POINTER(CorrelateSettings)]
+            L.lucem_correlate_settings_default.restype = None
+            L.lucem_correlate_2d_into_f32.argtypes = rl[:-2] + [ctypes.POINTER(CorrelateSettings), _vp]
+            L.lucem_correlate_nd_into_f32.argtypes = L.lucem_correlate_2d_into_f32.argtypes
+            L.lucem_correlate_2d_into_f64.argtypes = rl[:-2] + [ctypes.POINTER(CorrelateSettings)]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

@@ -593,7 +593,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
     ColPass y{};
     y.Lc = g.Lc;
     y.tw = tws[1];
-    y.left = g.k[1] / 2;
+    y.left = g.k[1] / 2 + g.origin[1];
     y.mode = g.mode;
     if (g.ndim == 2) {
         y.op = COL_FUSED;
@@ -632,7 +632,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
     z.op = COL_FUSED;
     z.Lc = g.Lc;
     z.tw = tws[0];
-    z.left = g.k[0] / 2;
+    z.left = g.k[0] / 2 + g.origin[0];
     z.mode = g.mode;
     z.n_src = g.n[0];
     z.n_dst = g.n[0];
@@ -671,7 +671,7 @@ RowPass lucem::make_row_desc(const Geometry& g, const ShapePolicy& sp, const flo
     RowPass r{};
     r.nx = g.n[2];
     r.rows = (int)rows;
-    r.left = g.k[2] / 2;
+    r.left = g.k[2] / 2 + g.origin[2];
     r.mode = g.mode;
     r.Lc = g.Lc;
     r.pitch = g.pitch;
@@ -692,7 +692,7 @@ RowPass lucem::make_row_desc(const Geometry& g, const ShapePolicy& sp, const flo
 static const char* const kCorrNames2[1] = {"line_fused_y"};
 static const char* const kCorrNames3[3] = {"line_fwd_y", "line_fused_z", "line_inv_y"};
 
-static lucem_status run_corr(lucem_client* c, const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb,
+lucem_status lucem::run_corr(lucem_client* c, const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb,
                              const float2* otf, const float2* const tws[3]) {
     ColPass d[3];
     int b[3];
@@ -704,7 +704,7 @@ static lucem_status run_corr(lucem_client* c, const Geometry& g, const ShapePoli
     return LUCEM_OK;
 }
 
-static lucem_status run_otf(lucem_client* c, const Geometry& g, const float* w, float2* pa, float2* pb, float2* otf,
+lucem_status lucem::run_otf(lucem_client* c, const Geometry& g, const float* w, float2* pa, float2* pb, float2* otf,
                             const float2* const tws[3]) {
     RowPass r;
     int rb;
@@ -932,9 +932,11 @@ static lucem_status rl_run_f64(lucem_client* c, lucem_workspace* ws, const doubl
         const bool last = it + 1 == s.num_iter;
         const double* src = ping ? a : b;
         double* dst = last ? output : (ping ? b : a);
-        if ((st = launch_check(launch_correlate2d_f64(src, rev, conv, H, W, kh, kw, g.mode, sm), "correlate (f64)")) ||
+        if ((st = launch_check(launch_correlate2d_f64(src, rev, conv, H, W, kh, kw, kh / 2, kw / 2, g.mode, 0.0, sm),
+                               "correlate (f64)")) ||
             (st = launch_check(launch_ratio_f64(input, conv, rel, eps, n, sm), "ratio (f64)")) ||
-            (st = launch_check(launch_correlate2d_f64(rel, h, err, H, W, kh, kw, g.mode, sm), "correlate (f64)")) ||
+            (st = launch_check(launch_correlate2d_f64(rel, h, err, H, W, kh, kw, kh / 2, kw / 2, g.mode, 0.0, sm),
+                               "correlate (f64)")) ||
             (st = launch_check(launch_update_f64(src, err, dst, last && s.clip ? 1 : 0, n, sm), "update (f64)")))
             return st;
         ping = !ping;

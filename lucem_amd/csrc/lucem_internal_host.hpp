@@ -72,6 +72,7 @@ struct Geometry {
     int Lc;
     int pitch;
     int mode;
+    int origin[3];  // shift of the window per axis: the gather starts at k / 2 + origin before each output
 };
 
 lucem_status fail(lucem_status st, const std::string& msg);
@@ -112,6 +113,13 @@ int make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa, float2
 RowPass make_row_desc(const Geometry& g, const ShapePolicy& sp, const float2* tw_x, float eps, const float* obs, float* est,
                       float* out, float2* spec, int* blocks);
 float rl_eps(const lucem_rl_settings& s);
+// One linear correlation of the row spectra in spec_a (and spec_b in 3-D) with transfer function
+// `otf`, and the build of a transfer function from real weights `w` (dims g.k) through the
+// scratch spectra pa, pb.
+lucem_status run_corr(lucem_client* c, const Geometry& g, const ShapePolicy& sp, float2* sa, float2* sb,
+                      const float2* otf, const float2* const tws[3]);
+lucem_status run_otf(lucem_client* c, const Geometry& g, const float* w, float2* pa, float2* pb, float2* otf,
+                     const float2* const tws[3]);
 
 // Autotuner: times candidate shapes of these descriptors' passes (the one-array kernels, or the
 // batch kernels scheduled as the batched form runs them) and records the fastest per (kind, L).

@@ -4,7 +4,8 @@
 // entry point with FP64 arithmetic throughout (the configurations are all f32).
 //
 // correlate2d_f64: out[y, x] = sum_{ky, kx} w[ky, kx] * P_mode(in)[y - ly + ky, x - lx + kx],
-// accumulated ky-major as in filter/kernels/convolve.rs:61-118 (cval 0). A 16x16 output tile
+// accumulated ky-major as in filter/kernels/convolve.rs:61-118, Constant reading `cval` outside
+// the array (0 in the Richardson-Lucy iteration, deconvolve.rs:99-103). A 16x16 output tile
 // per workgroup reads its input tile with the halo into LDS once when it fits (kernels up to
 // about 74 x 74), otherwise straight from global memory.
 #include <hip/hip_runtime.h>
@@ -21,7 +22,8 @@ template <bool STAGED>
 __global__ __launch_bounds__(kTile* kTile) void correlate2d_f64_kernel(const double* __restrict__ in,
                                                                        const double* __restrict__ w,
                                                                        double* __restrict__ out, int H, int W, int kh,
-                                                                       int kw, int ly, int lx, int mode) {
+                                                                       int kw, int ly, int lx, int mode,
+                                                                       double cval) {
     extern __shared__ double tile[];
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
     const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
@@ -31,7 +33,7 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_f64_kernel(const dou
         for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
             const int r = i / tw, c = i - r * tw;
             const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
-            tile[i] = (py < 0 || px < 0) ? 0.0 : in[(int64_t)py * W + px];
+            tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
         }
         __syncthreads();
         if (x < W && y < H) {
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_f64_kernel(const dou
                 const int py = pad_index(y - ly + ky, H, mode);
                 for (int kx = 0; kx < kw; kx++) {
                     const int px = pad_index(x - lx + kx, W, mode);
-                    const double s = (py < 0 || px < 0) ? 0.0 : in[(int64_t)py * W + px];
+                    const double s = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
                     acc += w[ky * kw + kx] * s;
                 }
             }
@@ -108,17 +110,17 @@ __global__ __launch_bounds__(256) void psf_prep_f64_kernel(const double* __restr
 
 static dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int mode,
-                           void* stream) {
+int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int ly,
+                           int lx, int mode, double cval, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid((unsigned)((W + kTile - 1) / kTile), (unsigned)((H + kTile - 1) / kTile));
     const size_t lds = (size_t)(kTile + kw - 1) * (kTile + kh - 1) * sizeof(double);
     if (lds <= kF64TileLds)
         hipLaunchKernelGGL(correlate2d_f64_kernel<true>, grid, dim3(kTile * kTile), lds, st, in, w, out, H, W, kh, kw,
-                           kh / 2, kw / 2, mode);
+                           ly, lx, mode, cval);
     else
         hipLaunchKernelGGL(correlate2d_f64_kernel<false>, grid, dim3(kTile * kTile), 0, st, in, w, out, H, W, kh, kw,
-                           kh / 2, kw / 2, mode);
+                           ly, lx, mode, cval);
     return (int)hipGetLastError();
 }
 

@@ -161,8 +161,15 @@ int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
 int batch_occupancy(bool row, int tier, size_t lds);
 // Double-precision direct form (richardson_lucy_into::<f64>).
-int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int mode,
-                           void* stream);
+// Constant-mode cval of the f32 correlation: out = in - cval; *sum = sum(w) in f64;
+// out += cval * *sum.
+int launch_cval_shift(const float* in, float* out, float cval, int64_t n, void* stream);
+int launch_weight_sum(const float* w, int64_t nk, double* sum, void* stream);
+int launch_add_weight_sum(float* out, const double* sum, double cval, int64_t n, void* stream);
+
+// out = correlate(in, w) with left offsets (ly, lx) and `cval` outside the array in Constant mode.
+int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int ly,
+                           int lx, int mode, double cval, void* stream);
 int launch_ratio_f64(const double* obs, const double* c, double* rel, double eps, int64_t n, void* stream);
 int launch_update_f64(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
 int launch_fill_f64(double* p, double v, int64_t n, void* stream);

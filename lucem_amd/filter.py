@@ -1,0 +1,103 @@
+"""Host-side mirror of lucem-ops' N-D correlation interface, over the C ABI.
+
+Reference (neuro-kitchen/lucem):
+
+- ``Correlate2d`` (defaults)          crates/lucem-ops/src/filter/convolve.rs:113-123
+- ``correlate_2d_into``               crates/lucem-ops/src/filter/convolve.rs:126-166
+- ``correlate_2d`` (allocating form)  crates/lucem-ops/src/filter/convolve.rs:169-177
+- ``check::origin``                   crates/lucem-ops/src/core/check.rs:59-72
+
+``out[p] = sum_t w[t] * P(in)[p - (k // 2 + origin) + t]`` over the window ``t``, ``P`` the
+boundary mode (``cval`` outside the array in Constant mode). float32 tensors run as an FFT
+linear correlation through the passes of the Richardson-Lucy path (padded domain ``n + k - 1``
+per axis, at most 16384); float64 tensors run the direct sum in the reference's order. Errors
+raise :class:`lucem_amd.LucemError` in the reference's check order.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+from . import _native as N
+from .restore import PadMode
+
+
+@dataclass
+class Correlate2d:
+    """Settings of 2-D / N-D correlation (convolve.rs:113-123): boundary mode, cval and the
+    window origin per axis of the input."""
+
+    mode: PadMode | str = PadMode.Reflect
+    cval: float = 0.0
+    origin: tuple = (0, 0)
+
+    def to_c(self) -> N.CorrelateSettings:
+        s = N.CorrelateSettings()
+        s.mode = int(PadMode.parse(self.mode)) if isinstance(self.mode, str) else int(PadMode(self.mode))
+        org = tuple(int(o) for o in self.origin)
+        if len(org) > 3:
+            raise ValueError("origin has at most 3 entries")
+        for i, o in enumerate(org):
+            s.origin[i] = o
+        s.cval = float(self.cval)
+        return s
+
+
+def _call(fn, input, weights, output, settings, ws, client, dtype=None, with_ws=True):
+    x, w, o = N.as_tensor(input), N.as_tensor(weights), N.as_tensor(output)
+    if client is None:
+        client = N.default_client(x.device.index if x.is_cuda else None)
+    if x.numel() and not (x.is_cuda and w.is_cuda and o.is_cuda):
+        raise ValueError("input, weights and output must be device tensors")
+    xp = N.device_ptr(x, "input", dtype) if x.numel() else 0
+    wp = N.device_ptr(w, "weights", dtype) if w.numel() else 0
+    op = N.device_ptr(o, "output", dtype) if o.numel() else 0
+    st = (settings or Correlate2d()).to_c()
+    args = [client.handle, N._vp(xp or None), N.dims(x.shape), x.dim(), N._vp(wp or None), N.dims(w.shape), w.dim(),
+            N._vp(op or None), N.dims(o.shape), o.dim(), ctypes.byref(st)]
+    if with_ws:
+        args.append(ws.handle if ws is not None else None)
+    N.check(fn(*args))
+
+
+def correlate_2d_into(input, weights, output, settings: Correlate2d | None = None, ws: N.Workspace | None = None,
+                      client: N.Client | None = None) -> None:
+    """Correlates 2-D ``input`` with 2-D ``weights`` into ``output`` (convolve.rs:126-166).
+
+    float32: FFT linear correlation (rounding error about 1e-7 of the output's largest value);
+    float64: the direct sum in FP64 (``ws`` unused). Enqueues on the client's stream.
+    """
+    import torch
+
+    if N.as_tensor(input).dtype == torch.float64:
+        _call(N.lib().lucem_correlate_2d_into_f64, input, weights, output, settings, ws, client, torch.float64,
+              with_ws=False)
+        return
+    _call(N.lib().lucem_correlate_2d_into_f32, input, weights, output, settings, ws, client)
+
+
+def correlate_2d(input, weights, settings: Correlate2d | None = None, client: N.Client | None = None):
+    """Allocates and returns the correlation (convolve.rs:169-177)."""
+    import torch
+
+    x = N.as_tensor(input)
+    out = torch.empty_like(x)
+    correlate_2d_into(x, weights, out, settings, None, client)
+    return out
+
+
+def correlate_nd_into(input, weights, output, settings: Correlate2d | None = None, ws: N.Workspace | None = None,
+                      client: N.Client | None = None) -> None:
+    """The same operator on 2-D or 3-D float32 arrays (weights of the input's rank, origin per
+    axis)."""
+    _call(N.lib().lucem_correlate_nd_into_f32, input, weights, output, settings, ws, client)
+
+
+def correlate_nd(input, weights, settings: Correlate2d | None = None, client: N.Client | None = None):
+    import torch
+
+    x = N.as_tensor(input)
+    out = torch.empty_like(x)
+    correlate_nd_into(x, weights, out, settings, None, client)
+    return out

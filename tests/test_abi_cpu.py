@@ -148,6 +148,52 @@ def test_check_order_f64_without_device(L):
     assert call((8, 16400), (3, 3), (8, 16400)) == 11
 
 
+def test_correlate_check_order_without_device(L):
+    """correlate_2d_into's order (filter/convolve.rs:133-141): input ndim, weights ndim, same
+    shape, aliasing, origin per axis (core/check.rs:59-72), then empty input."""
+    from lucem_amd import _native as N
+
+    s = N.CorrelateSettings()
+    L.lucem_correlate_settings_default(ctypes.byref(s))
+    assert (s.mode, tuple(s.origin), s.cval) == (0, (0, 0, 0), 0.0)
+
+    def call(fn, in_dims, w_dims, out_dims, origin=(0, 0, 0), out=0x20000000, mode=0, f64=False):
+        for i, o in enumerate(origin):
+            s.origin[i] = o
+        s.mode = mode
+        args = [None, N._vp(0x10000), N.dims(in_dims), len(in_dims), N._vp(0x30000000), N.dims(w_dims), len(w_dims),
+                N._vp(out), N.dims(out_dims), len(out_dims), ctypes.byref(s)]
+        return fn(*args) if f64 else fn(*args, None)
+
+    f32, nd, f64 = L.lucem_correlate_2d_into_f32, L.lucem_correlate_nd_into_f32, L.lucem_correlate_2d_into_f64
+    assert call(f32, (2, 8, 8), (3, 3), (2, 8, 9)) == 3
+    assert b"correlate_2d expects 2-D array, got 3-D shape" in L.lucem_last_error()
+    assert call(f32, (8, 8), (3,), (8, 9)) == 3
+    assert b"weights expects 2-D array" in L.lucem_last_error()
+    assert call(f32, (8, 8), (3, 3), (8, 9)) == 1
+    assert call(f32, (8, 8), (3, 3), (8, 8), out=0x10000 + 100) == 2
+    assert call(f32, (8, 8), (0, 3), (8, 8)) == 4
+    assert b"window: size must be at least 1" in L.lucem_last_error()
+    assert call(f32, (8, 8), (3, 4), (8, 8), origin=(2, 0, 0)) == 14
+    assert b"origin 2 is out of bounds for window of size 3 (must be in -1..=1)" in L.lucem_last_error()
+    assert call(f32, (8, 8), (3, 4), (8, 8), origin=(0, -3, 0)) == 14
+    assert call(f32, (8, 8), (3, 4), (8, 8), origin=(-1, -2, 0)) == 11  # in range: reaches the client check
+    assert call(f32, (8, 8), (3, 4), (8, 8), origin=(1, 1, 0)) == 11
+    assert call(f32, (8, 8), (3, 3), (8, 8), mode=7) == 8
+    assert call(f32, (0, 8), (3, 3), (0, 8)) == 0
+    assert call(f32, (8, 16400), (3, 3), (8, 16400)) == 9
+    # the N-D form: 3-D arrays with weights of the same rank; origin per axis
+    assert call(nd, (2, 8, 8), (3, 3, 3), (2, 8, 8)) == 11
+    assert call(nd, (2, 8, 8), (3, 3), (2, 8, 8)) == 3
+    assert call(nd, (2, 2, 8, 8), (3, 3, 3, 3), (2, 2, 8, 8)) == 3
+    assert call(nd, (2, 8, 8), (3, 3, 3), (2, 8, 8), origin=(0, 0, 2)) == 14
+    # the FP64 direct form: 8-byte elements, no FFT length limit
+    assert call(f64, (4, 5), (3, 3), (4, 5), out=0x10000 + 80, f64=True) == 2
+    assert call(f64, (4, 5), (3, 3), (4, 5), out=0x10000 + 160, f64=True) == 11
+    assert call(f64, (8, 16400), (3, 3), (8, 16400), f64=True) == 11
+    assert call(f64, (2, 4, 5), (3, 3, 3), (2, 4, 5), f64=True) == 3
+
+
 def test_wiener_check_order_without_device(L):
     from lucem_amd import _native as N
 

@@ -1,0 +1,151 @@
+"""GPU tests of the correlation entry points (correlate_2d_into, filter/convolve.rs:126-166)
+against the oracle's direct correlation: every boundary mode, window origins, Constant mode with
+a nonzero cval, weights larger than the input, 3-D arrays, the FP64 direct form, and the
+reference's checks.
+
+Tolerances: the reference's own correlation test bound (``assert_close`` with rtol 1e-5 and
+atol = rtol * max|want|, crates/lucem-core/src/test_support.rs:42-50, used at
+filter/convolve.rs:211-228) and 1e-6 relative L2 for the FFT form in FP32; 1e-12 for FP64."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = [0, 1, 2, 3, 4]
+
+
+def _dev(a, dtype=np.float32):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).cuda()
+
+
+def assert_close(got, want, rtol):
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    atol = rtol * float(np.max(np.abs(want))) if want.size else 0.0
+    bad = np.abs(got - want) > rtol * np.abs(want) + atol
+    assert not bad.any(), f"{int(bad.sum())} values differ, worst {float(np.max(np.abs(got - want)))}"
+
+
+def _corr(x, w, mode, cval=0.0, origin=(0, 0), nd=False):
+    import lucem_amd as LA
+
+    s = LA.Correlate2d(mode=mode, cval=cval, origin=origin)
+    fn = LA.correlate_nd if nd else LA.correlate_2d
+    return fn(_dev(x), _dev(w), s).cpu().numpy()
+
+
+CASES_2D = [
+    ((64, 64), (5, 5), (0, 0)),
+    ((37, 53), (4, 6), (-2, 2)),      # even windows, origins at both ends of the range
+    ((48, 80), (7, 3), (3, -1)),
+    ((9, 11), (15, 21), (0, 0)),      # weights larger than the input: folded pads
+    ((1, 64), (1, 7), (0, 3)),
+    ((64, 1), (9, 1), (-4, 0)),
+    ((3, 3), (3, 3), (1, -1)),
+    ((125, 98), (25, 25), (-12, 12)),
+    ((1024, 1024), (25, 25), (0, 0)),
+]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("shape,kshape,origin", CASES_2D)
+def test_correlate_2d_matches_oracle(gpu, oracle, mode, shape, kshape, origin):
+    rng = np.random.default_rng(shape[0] * 31 + shape[1] + kshape[0] * 7 + mode)
+    x = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+    w = rng.standard_normal(kshape).astype(np.float32)
+    cval = 1.5 if mode == 1 else 0.0
+    print(f"correlate_2d {shape} weights {kshape} origin {origin} mode {mode} cval {cval}", flush=True)
+    got = _corr(x, w, mode, cval, origin)
+    want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+    e = oracle.rel_l2(got, want)
+    print(f"  rel-L2 vs FP64 {e:.2e}")
+    assert_close(got, want, 1e-5)
+    assert e < 1e-6
+
+
+CASES_3D = [
+    ((16, 20, 24), (3, 5, 7), (0, 0, 0)),
+    ((9, 12, 10), (4, 3, 6), (-2, 1, 2)),
+    ((5, 7, 6), (7, 9, 11), (0, 0, 0)),    # weights larger than the input
+    ((64, 96, 80), (15, 7, 7), (3, -3, 0)),
+]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("shape,kshape,origin", CASES_3D)
+def test_correlate_nd_matches_oracle(gpu, oracle, mode, shape, kshape, origin):
+    rng = np.random.default_rng(sum(shape) * 13 + sum(kshape) + mode)
+    x = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+    w = rng.standard_normal(kshape).astype(np.float32)
+    cval = -0.75 if mode == 1 else 0.0
+    print(f"correlate_nd {shape} weights {kshape} origin {origin} mode {mode} cval {cval}", flush=True)
+    got = _corr(x, w, mode, cval, origin, nd=True)
+    want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+    e = oracle.rel_l2(got, want)
+    print(f"  rel-L2 vs FP64 {e:.2e}")
+    assert_close(got, want, 1e-5)
+    assert e < 1e-6
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_correlate_2d_f64_matches_oracle(gpu, oracle, mode):
+    import lucem_amd as LA
+
+    for shape, kshape, origin in [((37, 53), (4, 6), (-2, 2)), ((9, 11), (15, 21), (0, 0)),
+                                  ((40, 40), (81, 81), (40, -1))]:  # 81 x 81: the unstaged kernel
+        rng = np.random.default_rng(shape[0] + kshape[1] + mode)
+        x = rng.uniform(-1.0, 1.0, shape)
+        w = rng.standard_normal(kshape)
+        cval = 1.5 if mode == 1 else 0.0
+        print(f"correlate_2d f64 {shape} weights {kshape} origin {origin} mode {mode}", flush=True)
+        got = LA.correlate_2d(_dev(x, np.float64), _dev(w, np.float64),
+                              LA.Correlate2d(mode=mode, cval=cval, origin=origin)).cpu().numpy()
+        want = oracle.correlate(x, w, mode, cval, origin=origin)
+        assert oracle.rel_l2(got, want) < 1e-12
+
+
+def test_correlate_matches_rl_blur(gpu, oracle):
+    """The forward blur of the RL iteration (deconvolve.rs:120): correlate(x, flip(psf / sum))."""
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.0, 2.0, (200, 150)).astype(np.float32)
+    psf = rng.uniform(0.0, 1.0, (11, 9))
+    h = psf / psf.sum()
+    rev = h[::-1, ::-1].astype(np.float32)
+    got = _corr(x, rev, 0)
+    want = oracle.correlate_fft64(x, rev.astype(np.float64), 0)
+    assert oracle.rel_l2(got, want) < 1e-6
+
+
+def test_correlate_delta_shifts_exactly(gpu):
+    """A delta window at (dy, dx) moves the image by (dy - k // 2, dx - k // 2) (Wrap: circular)."""
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0.0, 1.0, (256, 320)).astype(np.float32)
+    w = np.zeros((9, 7), np.float32)
+    w[7, 1] = 1.0
+    got = _corr(x, w, 4)
+    want = np.roll(x, (-(7 - 4), -(1 - 3)), axis=(0, 1))
+    assert np.max(np.abs(got - want)) < 2e-6
+
+
+def test_correlate_empty_and_errors(gpu):
+    import torch
+
+    import lucem_amd as LA
+
+    e = torch.empty((0, 8), device="cuda")
+    LA.correlate_2d_into(e, _dev(np.ones((3, 3))), torch.empty((0, 8), device="cuda"))
+    x = _dev(np.ones((8, 8)))
+    with pytest.raises(LA.LucemError) as ei:
+        LA.correlate_2d(x, _dev(np.ones((3, 4))), LA.Correlate2d(origin=(2, 0)))
+    assert ei.value.kind == "INVALID_ORIGIN"
+    with pytest.raises(LA.LucemError) as ei:
+        LA.correlate_2d_into(x, _dev(np.ones((3, 3))), x)
+    assert ei.value.kind == "ALIASED_BUFFERS"
+    with pytest.raises(LA.LucemError) as ei:
+        LA.correlate_2d(_dev(np.ones((2, 8, 8))), _dev(np.ones((3, 3, 3))))
+    assert ei.value.kind == "INVALID_DIMENSION"

@@ -88,6 +88,21 @@ def test_correlate_matches_scipy(oracle, mode):
         assert_close(got32, want, 1e-5)
 
 
+@pytest.mark.parametrize("mode", list(MODES))
+def test_correlate_origin_and_cval_match_scipy(oracle, mode):
+    """The window origin (left = k // 2 + origin, convolve.rs:146-147) and cval in 2-D and 3-D,
+    against scipy.ndimage.correlate, whose origin and cval have the same meaning."""
+    rng = np.random.default_rng(40 + mode)
+    cases = [((9, 11), (4, 5), (-2, 1)), ((9, 11), (4, 5), (1, -2)), ((12, 7), (3, 6), (-1, 2)),
+             ((5, 6, 7), (3, 4, 5), (1, -2, 2)), ((4, 9, 6), (2, 3, 4), (-1, 0, -2))]
+    for shp, ks, org in cases:
+        x = rng.standard_normal(shp)
+        w = rng.standard_normal(ks)
+        got = oracle.correlate(x, w, mode, 1.5, origin=org)
+        want = ndi.correlate(x, w, mode=MODES[mode], cval=1.5, origin=org)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
 def host_correlate_1d(x, dims, w, axis, mode, cval, origin):
     """filter/convolve.rs:185-209 restated: f64 sums over host_pad_index."""
     x = np.asarray(x, dtype=np.float64).reshape(dims)
