@@ -52,6 +52,10 @@
  *                         f64 direct
  *   lucem_correlate_nd_into_f32
  *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
+ *   lucem_render_ellipses_f32 / lucem_render_spheres_f32
+ *                         the synthetic workloads' ground truth rendered on the device (no
+ *                         reference counterpart; with lucem_add_noise_f32, device-side phantoms
+ *                         and Poisson noise)
  */
 #ifndef LUCEM_HIP_H
 #define LUCEM_HIP_H
@@ -395,6 +399,28 @@ lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const double* inp
                                          uint32_t in_ndim, const double* weights, const uint64_t* w_dims,
                                          uint32_t w_ndim, double* output, const uint64_t* out_dims,
                                          uint32_t out_ndim, const lucem_correlate_settings* settings);
+
+/* ---- Synthetic phantoms on the device (no reference counterpart) ----------------------------
+ * out = background + the sum, in list order and in float64, of the value of every object the
+ * pixel lies inside, rounded to float32 once. Inside tests, in float64 with no contraction:
+ *   ellipse: u = (dx cos + dy sin) / a, v = (-dx sin + dy cos) / b, u*u + v*v <= 1
+ *   sphere:  dz*dz + dy*dy + dx*dx <= radius_sq
+ * with d = (pixel index) - (centre). cos_theta, sin_theta and radius_sq are the caller's (so that a
+ * host generator and the device agree bit for bit); radius bounds the sphere for culling.
+ * `objects` is a DEVICE array of `count` entries; out is [h, w] (ellipses) or [d, h, w] (spheres).
+ * Checks, in order: ndim (LUCEM_INVALID_DIMENSION), an empty output (LUCEM_OK, nothing done),
+ * element limit (LUCEM_TOO_MANY_ELEMENTS), grid limits (LUCEM_UNSUPPORTED_SHAPE: h <= 1048560, or
+ * d <= 262140 and h <= 524280), NULL handles. Asynchronous on the client's stream. */
+typedef struct lucem_ellipse {
+    double cy, cx, a, b, cos_theta, sin_theta, value;
+} lucem_ellipse;
+typedef struct lucem_sphere {
+    double cz, cy, cx, radius, radius_sq, value;
+} lucem_sphere;
+lucem_status lucem_render_ellipses_f32(lucem_client* client, float* out, const uint64_t* dims, uint32_t ndim,
+                                       const lucem_ellipse* objects, uint64_t count, double background);
+lucem_status lucem_render_spheres_f32(lucem_client* client, float* out, const uint64_t* dims, uint32_t ndim,
+                                      const lucem_sphere* objects, uint64_t count, double background);
 
 /* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
  * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is

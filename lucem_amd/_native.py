@@ -51,7 +51,7 @@ EXPORTED = [
     "lucem_unsupervised_wiener_settings_default", "lucem_unsupervised_wiener_f32",
     "lucem_fft_1d_into_f32", "lucem_fft_nd_into_f32", "lucem_complex_mul_into_f32", "lucem_add_noise_f32",
     "lucem_correlate_settings_default", "lucem_correlate_2d_into_f32", "lucem_correlate_nd_into_f32",
-    "lucem_correlate_2d_into_f64",
+    "lucem_correlate_2d_into_f64", "lucem_render_ellipses_f32", "lucem_render_spheres_f32",
 ]
 
 
@@ -100,6 +100,14 @@ class UnsupervisedWienerSettings(ctypes.Structure):
 
 class CorrelateSettings(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("origin", ctypes.c_int32 * 3), ("cval", ctypes.c_double)]
+
+
+class Ellipse(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("cy", "cx", "a", "b", "cos_theta", "sin_theta", "value")]
+
+
+class Sphere(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("cz", "cy", "cx", "radius", "radius_sq", "value")]
 
 
 class NoiseModelC(ctypes.Structure):
@@ -166,6 +174,9 @@ def lib() -> ctypes.CDLL:
             L.lucem_correlate_2d_into_f32.argtypes = rl[:-2] + [ctypes.POINTER(CorrelateSettings), _vp]
             L.lucem_correlate_nd_into_f32.argtypes = L.lucem_correlate_2d_into_f32.argtypes
             L.lucem_correlate_2d_into_f64.argtypes = rl[:-2] + [ctypes.POINTER(CorrelateSettings)]
+            L.lucem_render_ellipses_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, ctypes.c_uint64,
+                                                    ctypes.c_double]
+            L.lucem_render_spheres_f32.argtypes = L.lucem_render_ellipses_f32.argtypes
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

@@ -11,6 +11,8 @@
 
 #include <cstdint>
 
+#include "../../include/lucem_hip.h"
+
 namespace lucem {
 
 constexpr int kMaxPasses = 16;
@@ -161,6 +163,12 @@ int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
 int batch_occupancy(bool row, int tier, size_t lds);
 // Double-precision direct form (richardson_lucy_into::<f64>).
+// Phantoms: background + sum of the objects' values inside which a pixel lies.
+int launch_render_ellipses(float* out, int H, int W, const lucem_ellipse* obj, int64_t count, double background,
+                           void* stream);
+int launch_render_spheres(float* out, int D, int H, int W, const lucem_sphere* obj, int64_t count, double background,
+                          void* stream);
+
 // Constant-mode cval of the f32 correlation: out = in - cval; *sum = sum(w) in f64;
 // out += cval * *sum.
 int launch_cval_shift(const float* in, float* out, float cval, int64_t n, void* stream);

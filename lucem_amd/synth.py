@@ -69,18 +69,27 @@ def blur(x: np.ndarray, psf: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(c[tuple(slice(0, s) for s in x.shape)])
 
 
-def ellipses(shape, count: int, rng: np.random.Generator, axes=(3.0, 20.0), intensity=(200.0, 2000.0),
-             background: float = 100.0) -> np.ndarray:
-    """``background`` plus ``count`` filled ellipses (random centre, semi-axes, angle, intensity)."""
+def ellipse_params(shape, count: int, rng: np.random.Generator, axes=(3.0, 20.0),
+                   intensity=(200.0, 2000.0)) -> dict:
+    """Random ellipses: centre, semi-axes, angle (with its cosine and sine) and intensity."""
     h, w = shape
-    img = np.full((h, w), background, dtype=np.float64)
     cy = rng.uniform(0, h, count)
     cx = rng.uniform(0, w, count)
     a = rng.uniform(axes[0], axes[1], count)
     b = rng.uniform(axes[0], axes[1], count)
     th = rng.uniform(0, np.pi, count)
     val = rng.uniform(intensity[0], intensity[1], count)
-    for i in range(count):
+    cos = np.array([np.cos(t) for t in th], dtype=np.float64)
+    sin = np.array([np.sin(t) for t in th], dtype=np.float64)
+    return {"cy": cy, "cx": cx, "a": a, "b": b, "theta": th, "cos": cos, "sin": sin, "value": val}
+
+
+def render_ellipses(shape, p: dict, background: float = 100.0) -> np.ndarray:
+    """``background`` plus the filled ellipses of ``p`` (float64), in list order."""
+    h, w = shape
+    img = np.full((h, w), background, dtype=np.float64)
+    cy, cx, a, b, val = p["cy"], p["cx"], p["a"], p["b"], p["value"]
+    for i in range(len(cy)):
         r = int(np.ceil(max(a[i], b[i]))) + 1
         y0, y1 = max(int(cy[i]) - r, 0), min(int(cy[i]) + r + 1, h)
         x0, x1 = max(int(cx[i]) - r, 0), min(int(cx[i]) + r + 1, w)
@@ -88,23 +97,37 @@ def ellipses(shape, count: int, rng: np.random.Generator, axes=(3.0, 20.0), inte
             continue
         yy, xx = np.mgrid[y0:y1, x0:x1]
         dy, dx = yy - cy[i], xx - cx[i]
-        c, s = np.cos(th[i]), np.sin(th[i])
+        c, s = p["cos"][i], p["sin"][i]
         u = (dx * c + dy * s) / a[i]
         v = (-dx * s + dy * c) / b[i]
         img[y0:y1, x0:x1] += np.where(u * u + v * v <= 1.0, val[i], 0.0)
     return img
 
 
-def spheres(shape, count: int, rng: np.random.Generator, radius=(2.0, 8.0), intensity=(100.0, 1000.0),
-            background: float = 100.0) -> np.ndarray:
+def ellipses(shape, count: int, rng: np.random.Generator, axes=(3.0, 20.0), intensity=(200.0, 2000.0),
+             background: float = 100.0) -> np.ndarray:
+    """``background`` plus ``count`` filled ellipses (random centre, semi-axes, angle, intensity)."""
+    return render_ellipses(shape, ellipse_params(shape, count, rng, axes, intensity), background)
+
+
+def sphere_params(shape, count: int, rng: np.random.Generator, radius=(2.0, 8.0), intensity=(100.0, 1000.0)) -> dict:
+    """Random spheres: centre, radius (and its square) and intensity."""
     d, h, w = shape
-    vol = np.full((d, h, w), background, dtype=np.float64)
     cz = rng.uniform(0, d, count)
     cy = rng.uniform(0, h, count)
     cx = rng.uniform(0, w, count)
     rad = rng.uniform(radius[0], radius[1], count)
     val = rng.uniform(intensity[0], intensity[1], count)
-    for i in range(count):
+    rad_sq = np.array([r ** 2 for r in rad], dtype=np.float64)
+    return {"cz": cz, "cy": cy, "cx": cx, "radius": rad, "radius_sq": rad_sq, "value": val}
+
+
+def render_spheres(shape, p: dict, background: float = 100.0) -> np.ndarray:
+    """``background`` plus the filled spheres of ``p`` (float64), in list order."""
+    d, h, w = shape
+    vol = np.full((d, h, w), background, dtype=np.float64)
+    cz, cy, cx, rad, val = p["cz"], p["cy"], p["cx"], p["radius"], p["value"]
+    for i in range(len(cz)):
         r = int(np.ceil(rad[i])) + 1
         z0, z1 = max(int(cz[i]) - r, 0), min(int(cz[i]) + r + 1, d)
         y0, y1 = max(int(cy[i]) - r, 0), min(int(cy[i]) + r + 1, h)
@@ -112,9 +135,14 @@ def spheres(shape, count: int, rng: np.random.Generator, radius=(2.0, 8.0), inte
         if z0 >= z1 or y0 >= y1 or x0 >= x1:
             continue
         zz, yy, xx = np.mgrid[z0:z1, y0:y1, x0:x1]
-        inside = (zz - cz[i]) ** 2 + (yy - cy[i]) ** 2 + (xx - cx[i]) ** 2 <= rad[i] ** 2
+        inside = (zz - cz[i]) ** 2 + (yy - cy[i]) ** 2 + (xx - cx[i]) ** 2 <= p["radius_sq"][i]
         vol[z0:z1, y0:y1, x0:x1] += np.where(inside, val[i], 0.0)
     return vol
+
+
+def spheres(shape, count: int, rng: np.random.Generator, radius=(2.0, 8.0), intensity=(100.0, 1000.0),
+            background: float = 100.0) -> np.ndarray:
+    return render_spheres(shape, sphere_params(shape, count, rng, radius, intensity), background)
 
 
 def poisson(blurred: np.ndarray, rng: np.random.Generator) -> np.ndarray:

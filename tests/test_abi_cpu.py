@@ -194,6 +194,27 @@ def test_correlate_check_order_without_device(L):
     assert call(f64, (2, 4, 5), (3, 3, 3), (2, 4, 5), f64=True) == 3
 
 
+def test_render_check_order_without_device(L):
+    """The phantom renderers: rank, empty output, element and grid limits, then the handles."""
+    from lucem_amd import _native as N
+
+    def call(fn, dims, count=4, objects=0x40000):
+        return fn(None, N._vp(0x10000), N.dims(dims), len(dims), N._vp(objects or None), count, 100.0)
+
+    ell, sph = L.lucem_render_ellipses_f32, L.lucem_render_spheres_f32
+    assert call(ell, (2, 8, 8)) == 3
+    assert b"render_ellipses expects 2-D array, got 3-D shape" in L.lucem_last_error()
+    assert call(sph, (8, 8)) == 3
+    assert call(ell, (0, 8)) == 0
+    assert call(sph, (4, 0, 8)) == 0
+    assert call(ell, (1 << 17, 1 << 16)) == 6
+    assert call(ell, (1048561, 1)) == 9
+    assert call(ell, (1048560, 1)) == 11
+    assert call(sph, (262141, 1, 1)) == 9
+    assert call(sph, (1, 524281, 1)) == 9
+    assert call(sph, (4, 8, 8)) == 11
+
+
 def test_wiener_check_order_without_device(L):
     from lucem_amd import _native as N
 
