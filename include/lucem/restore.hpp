@@ -11,6 +11,8 @@
 //   lucem::UnsupervisedWiener, unsupervised_wiener
 //                                        crates/lucem-ops/src/restore/deconvolve.rs:330-509
 //   lucem::NoiseModel, add_noise         crates/lucem-ops/src/intensity/noise.rs:17-143
+//   lucem::Correlate2d, correlate_2d_into, correlate_2d
+//                                        crates/lucem-ops/src/filter/convolve.rs:113-177
 //   lucem::Client, Workspace, DeviceArray crates/lucem-core/src/compute/target.rs:95-115,
 //                                        memory/workspace.rs:17-37, memory/array.rs:20-151
 //
@@ -331,6 +333,65 @@ inline OpsResult<DeviceArray<float>> add_noise(const Client& client, const Devic
                                                 (uint32_t)input.dims().size(), out.unwrap().data(), &m, seed,
                                                 clip ? 1 : 0);
     if (st != LUCEM_OK) return OpsResult<DeviceArray<float>>(st, lucem_last_error());
+    return out;
+}
+
+// Settings of correlate_2d (filter/convolve.rs:113-123): boundary mode, cval, window origin
+// per axis (2-D: (y, x)).
+struct Correlate2d {
+    PadMode mode = PadMode::Reflect;
+    double cval = 0.0;
+    std::vector<int32_t> origin = {0, 0};
+
+    lucem_correlate_settings to_c() const {
+        lucem_correlate_settings s;
+        lucem_correlate_settings_default(&s);
+        s.mode = (uint32_t)mode;
+        s.cval = cval;
+        for (size_t i = 0; i < origin.size() && i < 3; i++) s.origin[i] = origin[i];
+        return s;
+    }
+};
+
+// correlate_2d_into (filter/convolve.rs:126-166); f32 through the FFT passes, f64 direct.
+inline OpsResult<Unit> correlate_2d_into(const Client& client, const DeviceArray<float>& input,
+                                         const DeviceArray<float>& weights, const DeviceArray<float>& output,
+                                         const Correlate2d& settings, Workspace& ws) {
+    const lucem_correlate_settings s = settings.to_c();
+    return status_result(lucem_correlate_2d_into_f32(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), weights.data(),
+        weights.dims().data(), (uint32_t)weights.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s, ws.handle()));
+}
+
+inline OpsResult<Unit> correlate_2d_into(const Client& client, const DeviceArray<double>& input,
+                                         const DeviceArray<double>& weights, const DeviceArray<double>& output,
+                                         const Correlate2d& settings) {
+    const lucem_correlate_settings s = settings.to_c();
+    return status_result(lucem_correlate_2d_into_f64(
+        client.handle(), input.data(), input.dims().data(), (uint32_t)input.dims().size(), weights.data(),
+        weights.dims().data(), (uint32_t)weights.dims().size(), output.data(), output.dims().data(),
+        (uint32_t)output.dims().size(), &s));
+}
+
+// correlate_2d (filter/convolve.rs:169-177): allocates the output.
+inline OpsResult<DeviceArray<float>> correlate_2d(const Client& client, const DeviceArray<float>& input,
+                                                  const DeviceArray<float>& weights, const Correlate2d& settings) {
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    Workspace ws(client);
+    auto r = correlate_2d_into(client, input, weights, out.unwrap(), settings, ws);
+    if (!r.is_ok()) return OpsResult<DeviceArray<float>>(r.status(), r.message());
+    client.sync();
+    return out;
+}
+
+inline OpsResult<DeviceArray<double>> correlate_2d(const Client& client, const DeviceArray<double>& input,
+                                                   const DeviceArray<double>& weights, const Correlate2d& settings) {
+    auto out = DeviceArray<double>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    auto r = correlate_2d_into(client, input, weights, out.unwrap(), settings);
+    if (!r.is_ok()) return OpsResult<DeviceArray<double>>(r.status(), r.message());
     return out;
 }
 

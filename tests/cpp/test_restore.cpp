@@ -1,7 +1,8 @@
 // C++ caller test: drives the library exactly as a host-language binding would, through the
 // C++ mirror of lucem-ops' interface (include/lucem/restore.hpp). Mirrors the reference test
 // crates/lucem-ops/src/restore/deconvolve.rs:517-566 (identity KAT within 1e-4, box-PSF peak
-// gain > 0.15) and the error order of deconvolve.rs:53-71. Prints "ok" and exits 0 on success.
+// gain > 0.15) and the error order of deconvolve.rs:53-71, plus the Wiener, noise and correlation
+// wrappers. Prints "ok" and exits 0 on success.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -88,6 +89,28 @@ int main() {
     // add_noise: clipped Gaussian noise stays in [0, 1] (intensity/mod.rs:52-63)
     auto noisy = add_noise(client, in, NoiseModel::gaussian(0.0, 0.05), 42, true).unwrap().to_host(client);
     for (float v : noisy) CHECK(v >= 0.0f && v <= 1.0f, "noise out of range %f", v);
+    // correlate_2d: a delta window at (2, 0) of a 3x3 window under Wrap reads in[y + 1, x - 1]
+    // (filter/convolve.rs:126-166, left = k / 2 + origin); f32 through the FFT passes, f64 direct
+    {
+        std::vector<float> dw(9, 0.0f);
+        dw[6] = 1.0f;
+        auto win = DeviceArray<float>::from_host(client, {3, 3}, dw).unwrap();
+        Correlate2d cs;
+        cs.mode = PadMode::Wrap;
+        auto c32 = correlate_2d(client, in, win, cs).unwrap().to_host(client);
+        std::vector<double> img64(img.begin(), img.end()), dw64(dw.begin(), dw.end());
+        auto in64 = DeviceArray<double>::from_host(client, {8, 8}, img64).unwrap();
+        auto win64 = DeviceArray<double>::from_host(client, {3, 3}, dw64).unwrap();
+        auto c64 = correlate_2d(client, in64, win64, cs).unwrap().to_host(client);
+        for (int y = 0; y < 8; y++)
+            for (int x = 0; x < 8; x++) {
+                const double want = img[((y + 1) % 8) * 8 + (x + 7) % 8];
+                CHECK(std::fabs(c32[y * 8 + x] - want) < 1e-6, "correlate f32 (%d, %d)", y, x);
+                CHECK(c64[y * 8 + x] == want, "correlate f64 (%d, %d)", y, x);
+            }
+        cs.origin = {2, 0};
+        CHECK(correlate_2d(client, in, win, cs).status() == LUCEM_INVALID_ORIGIN, "origin check");
+    }
     // error order
     Workspace ws(client);
     auto out86 = DeviceArray<float>::empty(client, {8, 6}).unwrap();
