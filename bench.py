@@ -29,6 +29,9 @@ Alongside the throughput the line carries:
   1024x1024 (30 iterations, burn-in 10), and the FP64 numpy restatement timed on this host;
 - ``fft``: the general complex FFT over both axes of a 2048x2048 split re/im array, and of a
   2039x2039 one (Bluestein lines), against 32 bytes per element (two read+write passes);
+- ``filter``: the correlation entry point on 2048x2048 with a 31x31 window, the device phantom
+  renderers on the 2-D and 3-D workloads' truths, and the whole 3-D workload generated on the
+  device (render, blur, Poisson noise);
 - ``tiled``: the zero-copy block walk (3 channels of 2048x2048, blocks of 480 with a 32-px halo,
   15x15 Gaussian PSF, 20 iterations), plan creation included, and the Poisson ``add_noise`` of a
   2048x2048 image against its 8 bytes per pixel.
@@ -299,6 +302,35 @@ def tiled_leg(steps: int) -> dict:
     }
 
 
+def filter_leg(steps: int) -> dict:
+    import numpy as np
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+    from lucem_amd import phantoms, synth
+
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = torch.rand((2048, 2048), device="cuda", generator=g)
+    w = torch.from_numpy(_gaussian_psf(31, 5.0)).cuda()
+    o = torch.empty_like(x)
+    ws = N.Workspace(N.default_client(0))
+    s = LA.Correlate2d()
+    tc = _timeit(lambda: LA.correlate_2d_into(x, w, o, s, ws), max(steps, 10))
+    p2 = synth.ellipse_params((1024, 1024), 200, np.random.default_rng(1000))
+    t2 = _timeit(lambda: phantoms.render_ellipses((1024, 1024), p2), max(steps, 10))
+    p3 = synth.sphere_params((128, 512, 512), 1500, np.random.default_rng(1001))
+    t3 = _timeit(lambda: phantoms.render_spheres((128, 512, 512), p3), max(steps, 3))
+    tg = _timeit(lambda: phantoms.plain3d(), 2)
+    return {
+        "correlate_2048_w31_ms": round(tc * 1e3, 4),
+        "correlate_note": "2048x2048 f32, 31x31 window, Reflect: transfer function built in the call",
+        "render_ellipses_1024_ms": round(t2 * 1e3, 4),
+        "render_spheres_128x512x512_ms": round(t3 * 1e3, 3),
+        "device_workload_plain3d_ms": round(tg * 1e3, 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +439,7 @@ def main() -> int:
         line["wiener"] = wiener_leg(args.steps, not args.no_cpu_baseline)
         line["fft"] = fft_leg(args.steps)
         line["tiled"] = tiled_leg(args.steps)
+        line["filter"] = filter_leg(args.steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))
