@@ -78,7 +78,7 @@ typedef enum lucem_status {
     LUCEM_TOO_MANY_ELEMENTS = 6,   /* CoreError::TooManyElements    (compute/limits.rs:16-22) */
     LUCEM_OUT_OF_DEVICE_MEMORY = 7,/* CoreError::OutOfDeviceMemory  (memory/budget.rs:232-244) */
     LUCEM_UNKNOWN_PAD_MODE = 8,    /* OpsError::UnknownPadMode      (core/pad.rs:55)      */
-    LUCEM_UNSUPPORTED_SHAPE = 9,   /* an axis longer than this library's single-pass FFT limit */
+    LUCEM_UNSUPPORTED_SHAPE = 9,   /* an axis longer than this library's FFT or launch limits */
     LUCEM_HIP_ERROR = 10,          /* a HIP runtime call failed; see lucem_last_error()   */
     LUCEM_INVALID_HANDLE = 11,     /* NULL client / workspace / settings pointer          */
     LUCEM_AXIS_OUT_OF_BOUNDS = 12, /* CoreError::AxisOutOfBounds    (lucem-core shape.rs:50-58) */
@@ -158,7 +158,10 @@ uint64_t lucem_workspace_capacity_bytes(const lucem_workspace* ws);
 
 /* richardson_lucy_into::<f32>. input, psf and output are device pointers, C-contiguous.
  * Checks, in order: same shape (input, output); no overlap of input and output; input ndim == 2;
- * psf ndim == 2; psf non-empty; num_iter >= 1; empty input => LUCEM_OK with nothing done. */
+ * psf ndim == 2; psf non-empty; num_iter >= 1; empty input => LUCEM_OK with nothing done.
+ * Correlations run as FFTs on the padded domain n + k - 1 per axis; when that is longer than
+ * LUCEM_MAX_FFT_LEN, the call runs the direct form instead (the reference's own correlations and
+ * order in FP32), so every 2-D shape the reference accepts is accepted. */
 lucem_status lucem_richardson_lucy_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                             uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
                                             uint32_t psf_ndim, float* output, const uint64_t* out_dims,
@@ -381,9 +384,10 @@ void lucem_correlate_settings_default(lucem_correlate_settings* out);
  * axis the origin (core/check.rs:59-72: a weights axis of size 0 is LUCEM_INVALID_WEIGHTS, an
  * origin outside -(k/2)..=(k-1)/2 LUCEM_INVALID_ORIGIN); an empty input returns LUCEM_OK with
  * nothing done. The sum runs as an FFT linear correlation (the passes of the RL path) on the
- * padded domain n + k - 1 per axis, at most 16384 (LUCEM_UNSUPPORTED_SHAPE beyond): FP32 with a
- * rounding error of about 1e-7 of the output's largest value, not the reference's sequential
- * sum. Asynchronous on the client's stream; ws may be NULL (the client's own). */
+ * padded domain n + k - 1 per axis: FP32 with a rounding error of about 1e-7 of the output's
+ * largest value, not the reference's sequential sum. A domain longer than LUCEM_MAX_FFT_LEN runs
+ * the direct sum instead (2-D; 3-D returns LUCEM_UNSUPPORTED_SHAPE). Asynchronous on the client's
+ * stream; ws may be NULL (the client's own). */
 lucem_status lucem_correlate_2d_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                          uint32_t in_ndim, const float* weights, const uint64_t* w_dims,
                                          uint32_t w_ndim, float* output, const uint64_t* out_dims, uint32_t out_ndim,

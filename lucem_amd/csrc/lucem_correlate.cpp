@@ -8,7 +8,8 @@
 // 1.0f * c == c, so the output is the correlation itself. The origin moves the gather start
 // (k / 2 + origin per axis). Constant mode with cval != 0 correlates in - cval and adds
 // cval * sum(w) (the zero-padded transform sees P_cval(in) - cval).
-// f64: the direct sum in the reference's order (rl_f64.hip).
+// f64, and f32 arrays with an axis longer than the FFT passes take: the direct sum in the
+// reference's order (rl_direct.hip).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -179,9 +180,24 @@ lucem_status correlate_entry(lucem_client* client, const float* input, const uin
     Geometry g;
     lucem_status st = validate_correlate(input, in_dims, in_ndim, w_dims, w_ndim, output, out_dims, out_ndim, s, nd,
                                          sizeof(float), true, &empty, &g);
+    bool direct = false;
+    if (st == LUCEM_UNSUPPORTED_SHAPE && in_ndim == 2) {
+        // an axis longer than the FFT passes take: the direct sum
+        st = validate_correlate(input, in_dims, in_ndim, w_dims, w_ndim, output, out_dims, out_ndim, s, nd,
+                                sizeof(float), false, &empty, &g);
+        direct = true;
+    }
     if (st != LUCEM_OK || empty) return st;
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     if (!input || !w || !output) return fail(LUCEM_INVALID_HANDLE, "input, weights or output pointer is NULL");
+    if (direct) {
+        hipError_t e = hipSetDevice(client->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        const int kh = g.k[1], kw = g.k[2];
+        return check(launch_correlate2d_direct(input, w, output, g.n[1], g.n[2], kh, kw, kh / 2 + g.origin[1],
+                                               kw / 2 + g.origin[2], g.mode, (float)s->cval, client->stream),
+                     "correlate (direct)");
+    }
     if (!ws) ws = client->internal_ws.get();
     if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
     return correlate_run(client, ws, input, w, output, g, s->cval);
@@ -221,7 +237,7 @@ extern "C" lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const 
     hipError_t e = hipSetDevice(client->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const int kh = g.k[1], kw = g.k[2];
-    return check(launch_correlate2d_f64(input, weights, output, g.n[1], g.n[2], kh, kw, kh / 2 + g.origin[1],
-                                        kw / 2 + g.origin[2], g.mode, settings->cval, client->stream),
+    return check(launch_correlate2d_direct(input, weights, output, g.n[1], g.n[2], kh, kw, kh / 2 + g.origin[1],
+                                           kw / 2 + g.origin[2], g.mode, settings->cval, client->stream),
                  "correlate (f64)");
 }

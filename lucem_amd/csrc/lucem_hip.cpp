@@ -857,6 +857,10 @@ lucem_status lucem::validate_rl(const void* input, const uint64_t* in_dims, uint
     return LUCEM_OK;
 }
 
+template <typename T>
+static lucem_status rl_run_direct(lucem_client* c, lucem_workspace* ws, const T* input, const T* psf, T* output,
+                                  const Geometry& g, const lucem_rl_settings& s);
+
 static lucem_status rl_entry(lucem_client* client, const float* input, const uint64_t* in_dims, uint32_t in_ndim,
                              const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim, float* output,
                              const uint64_t* out_dims, uint32_t out_ndim, const lucem_rl_settings* settings,
@@ -867,11 +871,19 @@ static lucem_status rl_entry(lucem_client* client, const float* input, const uin
     Geometry g;
     lucem_status st = validate_rl(input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim,
                                   settings, max_ndim, &empty, &g);
+    bool direct = false;
+    if (st == LUCEM_UNSUPPORTED_SHAPE && in_ndim == 2) {
+        // an axis longer than the FFT passes take: the direct form, in the reference's own order
+        st = validate_rl(input, in_dims, in_ndim, psf, psf_dims, psf_ndim, output, out_dims, out_ndim, settings,
+                         max_ndim, &empty, &g, sizeof(float), false);
+        direct = true;
+    }
     if (st != LUCEM_OK || empty) return st;
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     if (!input || !psf || !output) return fail(LUCEM_INVALID_HANDLE, "input, psf or output pointer is NULL");
     if (!ws) ws = client->internal_ws.get();
     if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
+    if (direct) return rl_run_direct<float>(client, ws, input, psf, output, g, *settings);
     return rl_run(client, ws, input, psf, output, g, *settings);
 }
 
@@ -902,42 +914,54 @@ extern "C" lucem_status lucem_richardson_lucy_nd_into_f32(lucem_client* client, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// richardson_lucy_into::<f64>: the reference iteration (deconvolve.rs:92-147) with direct FP64
-// correlations: est_a = 0.5; per iteration conv = correlate(src, rev), rel = y / safe(conv),
-// err = correlate(rel, h), dst = src * err (clip on the last), src/dst ping-ponging between
-// est_a and est_b and the last iteration writing the output.
+// The direct form: the reference iteration (deconvolve.rs:92-147) with direct correlations, in
+// FP64 for richardson_lucy_into::<f64> and in FP32 for 2-D f32 arrays whose padded domain is
+// longer than the FFT passes take: est_a = 0.5; per iteration conv = correlate(src, rev),
+// rel = y / safe(conv), err = correlate(rel, h), dst = src * err (clip on the last), src/dst
+// ping-ponging between est_a and est_b and the last iteration writing the output.
 // ---------------------------------------------------------------------------------------------
-static lucem_status rl_run_f64(lucem_client* c, lucem_workspace* ws, const double* input, const double* psf,
-                               double* output, const Geometry& g, const lucem_rl_settings& s) {
+static int psf_prep_direct(const double* psf, double* h, double* rev, int64_t nk, void* stream) {
+    return launch_psf_prep_f64(psf, h, rev, nk, stream);
+}
+static int psf_prep_direct(const float* psf, float* h, float* rev, int64_t nk, void* stream) {
+    PsfPrep pp{nk, psf, h, rev};  // f64 sum and norm, h rounded to f32 (deconvolve.rs:73-90)
+    return launch_psf_prep(pp, stream);
+}
+
+template <typename T>
+static lucem_status rl_run_direct(lucem_client* c, lucem_workspace* ws, const T* input, const T* psf, T* output,
+                                  const Geometry& g, const lucem_rl_settings& s) {
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const int H = g.n[1], W = g.n[2], kh = g.k[1], kw = g.k[2];
     const int64_t n = (int64_t)H * W, nk = (int64_t)kh * kw;
-    double *a, *b, *conv, *rel, *err, *h, *rev;
+    const bool f64 = sizeof(T) == sizeof(double);
+    T *a, *b, *conv, *rel, *err, *h, *rev;
     lucem_status st;
-    if ((st = ws_slot(ws, "rl64_est_a", sizeof(double) * n, (void**)&a)) ||
-        (st = ws_slot(ws, "rl64_est_b", sizeof(double) * n, (void**)&b)) ||
-        (st = ws_slot(ws, "rl64_est_conv", sizeof(double) * n, (void**)&conv)) ||
-        (st = ws_slot(ws, "rl64_rel_blur", sizeof(double) * n, (void**)&rel)) ||
-        (st = ws_slot(ws, "rl64_err_est", sizeof(double) * n, (void**)&err)) ||
-        (st = ws_slot(ws, "rl64_psf_h", sizeof(double) * nk, (void**)&h)) ||
-        (st = ws_slot(ws, "rl64_psf_rev", sizeof(double) * nk, (void**)&rev)))
+    if ((st = ws_slot(ws, f64 ? "rl64_est_a" : "rl32_est_a", sizeof(T) * n, (void**)&a)) ||
+        (st = ws_slot(ws, f64 ? "rl64_est_b" : "rl32_est_b", sizeof(T) * n, (void**)&b)) ||
+        (st = ws_slot(ws, f64 ? "rl64_est_conv" : "rl32_est_conv", sizeof(T) * n, (void**)&conv)) ||
+        (st = ws_slot(ws, f64 ? "rl64_rel_blur" : "rl32_rel_blur", sizeof(T) * n, (void**)&rel)) ||
+        (st = ws_slot(ws, f64 ? "rl64_err_est" : "rl32_err_est", sizeof(T) * n, (void**)&err)) ||
+        (st = ws_slot(ws, f64 ? "rl64_psf_h" : "rl32_psf_h", sizeof(T) * nk, (void**)&h)) ||
+        (st = ws_slot(ws, f64 ? "rl64_psf_rev" : "rl32_psf_rev", sizeof(T) * nk, (void**)&rev)))
         return st;
     void* sm = c->stream;
-    if ((st = launch_check(launch_psf_prep_f64(psf, h, rev, nk, sm), "psf prep (f64)"))) return st;
-    if ((st = launch_check(launch_fill_f64(a, 0.5, n, sm), "fill (f64)"))) return st;
-    const double eps = std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
+    if ((st = launch_check(psf_prep_direct(psf, h, rev, nk, sm), "psf prep (direct)"))) return st;
+    if ((st = launch_check(launch_fill_direct(a, T(0.5), n, sm), "fill (direct)"))) return st;
+    // the epsilon of deconvolve.rs:104, cast to F
+    const T eps = f64 ? (T)std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15) : (T)rl_eps(s);
     bool ping = true;
     for (uint64_t it = 0; it < s.num_iter; it++) {
         const bool last = it + 1 == s.num_iter;
-        const double* src = ping ? a : b;
-        double* dst = last ? output : (ping ? b : a);
-        if ((st = launch_check(launch_correlate2d_f64(src, rev, conv, H, W, kh, kw, kh / 2, kw / 2, g.mode, 0.0, sm),
-                               "correlate (f64)")) ||
-            (st = launch_check(launch_ratio_f64(input, conv, rel, eps, n, sm), "ratio (f64)")) ||
-            (st = launch_check(launch_correlate2d_f64(rel, h, err, H, W, kh, kw, kh / 2, kw / 2, g.mode, 0.0, sm),
-                               "correlate (f64)")) ||
-            (st = launch_check(launch_update_f64(src, err, dst, last && s.clip ? 1 : 0, n, sm), "update (f64)")))
+        const T* src = ping ? a : b;
+        T* dst = last ? output : (ping ? b : a);
+        if ((st = launch_check(launch_correlate2d_direct(src, rev, conv, H, W, kh, kw, kh / 2, kw / 2, g.mode, T(0), sm),
+                               "correlate (direct)")) ||
+            (st = launch_check(launch_ratio_direct(input, conv, rel, eps, n, sm), "ratio (direct)")) ||
+            (st = launch_check(launch_correlate2d_direct(rel, h, err, H, W, kh, kw, kh / 2, kw / 2, g.mode, T(0), sm),
+                               "correlate (direct)")) ||
+            (st = launch_check(launch_update_direct(src, err, dst, last && s.clip ? 1 : 0, n, sm), "update (direct)")))
             return st;
         ping = !ping;
     }
@@ -957,7 +981,7 @@ static lucem_status rl_entry_f64(lucem_client* client, const double* input, cons
     if (!input || !psf || !output) return fail(LUCEM_INVALID_HANDLE, "input, psf or output pointer is NULL");
     if (!ws) ws = client->internal_ws.get();
     if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
-    return rl_run_f64(client, ws, input, psf, output, g, *settings);
+    return rl_run_direct<double>(client, ws, input, psf, output, g, *settings);
 }
 
 extern "C" lucem_status lucem_richardson_lucy_into_f64(lucem_client* client, const double* input,

@@ -162,7 +162,6 @@ int launch_psf_batch(const PsfBatch& b, void* stream);
 int launch_zero_ints(int* p, int n, void* stream);
 // Workgroups of a batched kernel that fit on one CU with `lds` bytes of dynamic LDS.
 int batch_occupancy(bool row, int tier, size_t lds);
-// Double-precision direct form (richardson_lucy_into::<f64>).
 // Phantoms: background + sum of the objects' values inside which a pixel lies.
 int launch_render_ellipses(float* out, int H, int W, const lucem_ellipse* obj, int64_t count, double background,
                            void* stream);
@@ -175,12 +174,19 @@ int launch_cval_shift(const float* in, float* out, float cval, int64_t n, void* 
 int launch_weight_sum(const float* w, int64_t nk, double* sum, void* stream);
 int launch_add_weight_sum(float* out, const double* sum, double cval, int64_t n, void* stream);
 
-// out = correlate(in, w) with left offsets (ly, lx) and `cval` outside the array in Constant mode.
-int launch_correlate2d_f64(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int ly,
-                           int lx, int mode, double cval, void* stream);
-int launch_ratio_f64(const double* obs, const double* c, double* rel, double eps, int64_t n, void* stream);
-int launch_update_f64(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
-int launch_fill_f64(double* p, double v, int64_t n, void* stream);
+// Direct form (richardson_lucy_into::<f64>, and f32 arrays beyond the FFT length limit):
+// out = correlate(in, w) with left offsets (ly, lx) and `cval` outside the array in Constant mode,
+// the ratio, the update, a fill, and the f64 PSF normalisation.
+int launch_correlate2d_direct(const double* in, const double* w, double* out, int H, int W, int kh, int kw, int ly,
+                              int lx, int mode, double cval, void* stream);
+int launch_correlate2d_direct(const float* in, const float* w, float* out, int H, int W, int kh, int kw, int ly,
+                              int lx, int mode, float cval, void* stream);
+int launch_ratio_direct(const double* obs, const double* c, double* rel, double eps, int64_t n, void* stream);
+int launch_ratio_direct(const float* obs, const float* c, float* rel, float eps, int64_t n, void* stream);
+int launch_update_direct(const double* src, const double* err, double* dst, int clip, int64_t n, void* stream);
+int launch_update_direct(const float* src, const float* err, float* dst, int clip, int64_t n, void* stream);
+int launch_fill_direct(double* p, double v, int64_t n, void* stream);
+int launch_fill_direct(float* p, float v, int64_t n, void* stream);
 int launch_psf_prep_f64(const double* psf, double* h, double* rev, int64_t nk, void* stream);
 // Complex FFT along one axis of split re/im arrays viewed as [outer, n, inner] (fft_axis.hip).
 struct AxisFft {

@@ -10,8 +10,8 @@ Reference (neuro-kitchen/lucem):
 ``out[p] = sum_t w[t] * P(in)[p - (k // 2 + origin) + t]`` over the window ``t``, ``P`` the
 boundary mode (``cval`` outside the array in Constant mode). float32 tensors run as an FFT
 linear correlation through the passes of the Richardson-Lucy path (padded domain ``n + k - 1``
-per axis, at most 16384); float64 tensors run the direct sum in the reference's order. Errors
-raise :class:`lucem_amd.LucemError` in the reference's check order.
+per axis; past 16384 a 2-D array takes the direct sum); float64 tensors run the direct sum in
+the reference's order. Errors raise :class:`lucem_amd.LucemError` in the reference's check order.
 """
 
 from __future__ import annotations

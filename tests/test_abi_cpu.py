@@ -116,7 +116,11 @@ def test_check_order_nd_without_device(L):
     assert _call(L, (2, 4, 5), (3, 3), (2, 4, 5), nd=True) == 3  # psf rank must match in N-D
     assert _call(L, (2, 2, 4, 5), (3, 3, 3, 3), (2, 2, 4, 5), nd=True) == 3
     assert _call(L, (65536, 65536, 2), (1, 1, 1), (65536, 65536, 2), nd=True, out=1 << 45) == 6  # > u32::MAX
-    assert _call(L, (8, 16400), (3, 3), (8, 16400)) == 9  # axis beyond the 16384 FFT limit
+    # a 2-D axis beyond the 16384 FFT limit takes the direct form (valid arguments, no client);
+    # a 3-D one is refused
+    assert _call(L, (8, 16400), (3, 3), (8, 16400)) == 11
+    assert _call(L, (8, 16400), (3, 3), (8, 16400), nd=True) == 11
+    assert _call(L, (2, 8, 16400), (3, 3, 3), (2, 8, 16400), nd=True) == 9
 
 
 def test_oracle_is_not_imported_by_the_product():
@@ -181,7 +185,8 @@ def test_correlate_check_order_without_device(L):
     assert call(f32, (8, 8), (3, 4), (8, 8), origin=(1, 1, 0)) == 11
     assert call(f32, (8, 8), (3, 3), (8, 8), mode=7) == 8
     assert call(f32, (0, 8), (3, 3), (0, 8)) == 0
-    assert call(f32, (8, 16400), (3, 3), (8, 16400)) == 9
+    assert call(f32, (8, 16400), (3, 3), (8, 16400)) == 11   # the direct sum beyond the FFT limit
+    assert call(nd, (2, 8, 16400), (3, 3, 3), (2, 8, 16400)) == 9
     # the N-D form: 3-D arrays with weights of the same rank; origin per axis
     assert call(nd, (2, 8, 8), (3, 3, 3), (2, 8, 8)) == 11
     assert call(nd, (2, 8, 8), (3, 3), (2, 8, 8)) == 3

@@ -109,6 +109,23 @@ def test_correlate_2d_f64_matches_oracle(gpu, oracle, mode):
         assert oracle.rel_l2(got, want) < 1e-12
 
 
+@pytest.mark.parametrize("shape,kshape,origin,mode", [
+    ((3, 16400), (3, 7), (0, 2), 1),
+    ((16390, 2), (5, 1), (-2, 0), 0),
+    ((2, 16380), (1, 9), (0, -4), 4),   # n fits the FFT limit, n + k - 1 does not
+])
+def test_correlate_longer_than_16384_takes_the_direct_sum(gpu, oracle, shape, kshape, origin, mode):
+    rng = np.random.default_rng(shape[0] + shape[1])
+    x = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+    w = rng.standard_normal(kshape).astype(np.float32)
+    cval = 1.5 if mode == 1 else 0.0
+    print(f"correlate_2d direct {shape} weights {kshape} origin {origin} mode {mode}", flush=True)
+    got = _corr(x, w, mode, cval, origin)
+    want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+    assert_close(got, want, 1e-5)
+    assert oracle.rel_l2(got, want) < 1e-6
+
+
 def test_correlate_matches_rl_blur(gpu, oracle):
     """The forward blur of the RL iteration (deconvolve.rs:120): correlate(x, flip(psf / sum))."""
     rng = np.random.default_rng(5)

@@ -302,14 +302,40 @@ def test_long_lines_match_oracle(gpu, oracle, shape, kshape):
     assert e < RTOL_L2
 
 
-def test_longer_than_16384_is_refused(gpu):
+# A 2-D axis whose padded domain n + k - 1 is longer than 16384 takes the direct form: the
+# reference's own correlations and order in FP32, so it matches the FP32 oracle to rounding
+# (FMA contraction). 3-D arrays with such an axis are refused.
+LONG_DIRECT_CASES = [
+    ((2, 16400), (1, 3), 0),
+    ((3, 16390), (3, 5), 3),
+    ((4, 16380), (3, 9), 1),    # n fits, n + k - 1 does not
+    ((16390, 2), (3, 1), 2),
+]
+
+
+@pytest.mark.parametrize("shape,kshape,mode", LONG_DIRECT_CASES)
+def test_longer_than_16384_takes_the_direct_form(gpu, oracle, shape, kshape, mode):
+    rng = np.random.default_rng(shape[1] + kshape[1])
+    y = rng.uniform(0.1, 1.0, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    print(f"direct f32 {shape} psf {kshape} mode {mode}", flush=True)
+    got = _run(y, psf, 4, mode)
+    st, ref = oracle.richardson_lucy(y, psf, num_iter=4, clip=False, mode=mode)
+    assert st == 0
+    e = oracle.rel_l2(got, ref)
+    print(f"  rel-L2 vs FP32 {e:.2e}")
+    assert e < 1e-6
+
+
+def test_3d_longer_than_16384_is_refused(gpu):
     import torch
 
     import lucem_amd as LA
 
-    y = torch.ones((2, 16400), device="cuda")
+    y = torch.ones((2, 2, 16400), device="cuda")
     with pytest.raises(Exception, match="16384"):
-        LA.richardson_lucy_into(y, torch.ones((1, 3), device="cuda"), torch.empty_like(y), LA.RichardsonLucy(num_iter=2))
+        LA.richardson_lucy_nd_into(y, torch.ones((1, 1, 3), device="cuda"), torch.empty_like(y),
+                                   LA.RichardsonLucy(num_iter=2))
 
 
 # richardson_lucy_into::<f64>: FP64 direct correlations in the reference's accumulation order, so
