@@ -8,8 +8,9 @@ double precision) and the FP32 direct-oracle output. ``grad_case.npz``: one 2-D 
 case of the unrolled-RL Poisson NLL with its FP64 loss and gradients. ``restore_cases.npz``: the
 Wiener filter (restore/deconvolve.rs:213-315) and the unsupervised Wiener loop (:357-509) in FP64,
 the complex FFT over all axes (core/fft.rs:53-100) in FP64, and add_noise
-(intensity/noise.rs:35-143) impulse and Poisson draws in float32. The metadata of each case is in
-``cases.json``. Arrays only (no pickled objects): load with ``numpy.load`` defaults.
+(intensity/noise.rs:35-143) impulse and Poisson draws in float32. ``filter_cases.npz``: the
+correlation (filter/convolve.rs:126-166) in FP64, 2-D and 3-D, every PadMode, window origins and
+Constant cval. The metadata of each case is in ``cases.json``. Arrays only (no pickled objects): load with ``numpy.load`` defaults.
 """
 
 from __future__ import annotations
@@ -96,7 +97,21 @@ def main() -> None:
         rest.update({f"n{j}_image": x, f"n{j}_out": O.add_noise32(x, kind, a, b, seed=seed, clip=clip)})
         rmeta["noise"].append({"case": j, "kind": kind, "p0": a, "p1": b, "seed": seed, "clip": clip})
     np.savez_compressed(HERE / "restore_cases.npz", **rest)
-    (HERE / "cases.json").write_text(json.dumps({"rl": meta, "grad": gmeta, **rmeta}, indent=1) + "\n")
+    frng = np.random.default_rng(535353)
+    filt, fmeta = {}, []
+    for j, (shape, kshape, mode, cval, origin) in enumerate([
+            ((20, 27), (5, 4), 0, 0.0, (0, 0)), ((19, 23), (4, 6), 1, 1.5, (-2, 2)), ((16, 9), (7, 3), 2, 0.0, (3, -1)),
+            ((9, 11), (15, 21), 3, 0.0, (0, 0)), ((25, 17), (3, 3), 4, 0.0, (1, -1)),
+            ((6, 9, 8), (3, 4, 5), 0, 0.0, (1, -2, 2)), ((5, 7, 6), (4, 3, 3), 1, -0.75, (-2, 1, 0))]):
+        x = frng.uniform(-1.0, 1.0, shape).astype(np.float32)
+        w = frng.standard_normal(kshape).astype(np.float32)
+        out = O.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+        filt.update({f"c{j}_input": x, f"c{j}_weights": w, f"c{j}_out64": out})
+        fmeta.append({"case": j, "shape": list(shape), "weights": list(kshape), "mode": mode, "cval": cval,
+                      "origin": list(origin)})
+    np.savez_compressed(HERE / "filter_cases.npz", **filt)
+    (HERE / "cases.json").write_text(json.dumps({"rl": meta, "grad": gmeta, **rmeta, "correlate": fmeta}, indent=1)
+                                     + "\n")
 
 
 if __name__ == "__main__":

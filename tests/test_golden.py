@@ -3,8 +3,8 @@
 CPU: the oracle still reproduces every fixture (FP64 to 1e-12, FP32 to 1e-6 relative L2), so a
 later change to the oracle cannot silently move the target. GPU: the HIP entry points reproduce
 the FP64 outputs within the parity bar of 1e-4 relative L2 (the FP32 oracle's own distance to
-FP64 is printed beside it), and the differentiable form reproduces the FP64 loss and gradients
-within 1e-3.
+FP64 is printed beside it), the differentiable form reproduces the FP64 loss and gradients
+within 1e-3, and the correlation its FP64 fixtures within 1e-6 (FP32) and 1e-12 (FP64).
 """
 
 from __future__ import annotations
@@ -183,3 +183,36 @@ def test_hip_matches_noise_fixture(gpu, case):
     out = LA.add_noise(torch.from_numpy(RS[f"n{case}_image"]).cuda(), LA.NoiseModel(m["kind"], m["p0"], m["p1"]),
                        m["seed"], m["clip"])
     np.testing.assert_array_equal(out.cpu().numpy(), RS[f"n{case}_out"])
+
+
+FC = np.load(G / "filter_cases.npz")
+
+
+@pytest.mark.parametrize("case", [m["case"] for m in META["correlate"]])
+def test_oracle_reproduces_correlate_fixture(oracle, case):
+    m = META["correlate"][case]
+    out = oracle.correlate(FC[f"c{case}_input"].astype(np.float64), FC[f"c{case}_weights"].astype(np.float64),
+                           m["mode"], m["cval"], origin=m["origin"])
+    assert _rel(out, FC[f"c{case}_out64"]) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [m["case"] for m in META["correlate"]])
+def test_hip_matches_correlate_fixture(gpu, case):
+    """FP32 FFT form within 1e-6 relative L2; the FP64 direct form (2-D cases) within 1e-12."""
+    import torch
+
+    import lucem_amd as LA
+
+    m = META["correlate"][case]
+    s = LA.Correlate2d(mode=m["mode"], cval=m["cval"], origin=tuple(m["origin"]))
+    x, w = FC[f"c{case}_input"], FC[f"c{case}_weights"]
+    want = FC[f"c{case}_out64"]
+    got = LA.correlate_nd(torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda(), s).cpu().numpy()
+    e = _rel(got, want)
+    print(f"correlate case {case} {m['shape']}: HIP f32 vs FP64 {e:.2e}")
+    assert e < 1e-6
+    if x.ndim == 2:
+        got64 = LA.correlate_2d(torch.from_numpy(x.astype(np.float64)).cuda(),
+                                torch.from_numpy(w.astype(np.float64)).cuda(), s).cpu().numpy()
+        assert _rel(got64, want) < 1e-12
