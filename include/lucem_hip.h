@@ -198,8 +198,8 @@ void lucem_wiener_settings_default(lucem_wiener_settings* out);
  * inverse transform and the optional clip to [-1, 1]. Checks in the reference's order:
  * same_shape, not_aliased, input 2-D, psf 2-D, psf non-empty, balance finite and >= 0, empty
  * input (OK), psf no larger than the image (LUCEM_INVALID_WEIGHTS). Image sizes: 7-smooth up to
- * 16384 (the R2C passes) or any size up to 8192 (the general FFT); LUCEM_UNSUPPORTED_SHAPE beyond.
- * Asynchronous on the client's stream. */
+ * 16384 take the R2C passes, others the general FFT (any length up to 2^27 per axis;
+ * LUCEM_UNSUPPORTED_SHAPE beyond). Asynchronous on the client's stream. */
 lucem_status lucem_wiener_deconvolve_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                               uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
                                               uint32_t psf_ndim, float* output, const uint64_t* out_dims,
@@ -238,7 +238,8 @@ lucem_status lucem_unsupervised_wiener_f32(lucem_client* client, const float* in
  * split complex array (re_in, im_in) into (re_out, im_out); forward exp(-2 pi i k m / n), inverse
  * exp(+...) / n. Checks in the reference's order: the three shapes equal to re_in's, re_in/re_out
  * and im_in/im_out not overlapping, the axis in range (LUCEM_AXIS_OUT_OF_BOUNDS), empty (OK). Any
- * length: 7-smooth lengths up to 16384 and others up to 8192 (LUCEM_UNSUPPORTED_SHAPE beyond). */
+ * length up to 2^27: one pass for 7-smooth lengths up to 16384 and others up to 8192, four-step
+ * passes beyond (7-smooth lengths as two single-pass lengths, others through Bluestein). */
 lucem_status lucem_fft_1d_into_f32(lucem_client* client, const float* re_in, const uint64_t* re_in_dims,
                                    uint32_t re_in_ndim, const float* im_in, const uint64_t* im_in_dims,
                                    uint32_t im_in_ndim, float* re_out, const uint64_t* re_out_dims,

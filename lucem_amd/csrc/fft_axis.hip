@@ -27,16 +27,20 @@ __global__ __launch_bounds__(T) void axis_fft_kernel(AxisFft d) {
     const int total = L * B;
     const Divisor bd = Divisor::of(B);
     const Divisor ld = Divisor::of(L);
-    int64_t base;
+    int64_t base, line0, col0;  // first element; first line (CONTIG) or slab; first column
     int nlines;
     if constexpr (CONTIG) {
         const int64_t o0 = (int64_t)blockIdx.x * B;
         base = o0 * n;
+        line0 = o0;
+        col0 = 0;
         nlines = (int)(d.outer - o0 < B ? d.outer - o0 : B);
     } else {
         const int64_t o = blockIdx.x / d.nkb;
         const int64_t j0 = ((int64_t)blockIdx.x - o * d.nkb) * B;
         base = o * n * d.inner + j0;
+        line0 = o;
+        col0 = j0;
         nlines = (int)(d.inner - j0 < B ? d.inner - j0 : B);
     }
     const float sgn = d.inverse ? -1.f : 1.f;  // conjugate on the way in and out for the inverse
@@ -89,9 +93,10 @@ __global__ __launch_bounds__(T) void axis_fft_kernel(AxisFft d) {
     }
     const Divisor nd = Divisor::of(n);
     const int nstore = n * B;
+    const bool tr = d.tr_n1 > 0;  // four-step second pass: transposed store, k-major for coalescing
     for (int e = tid; e < nstore; e += T) {
         int k, b;
-        if constexpr (CONTIG) {
+        if (CONTIG && !tr) {
             b = nd.div(e);
             k = e - b * n;
         } else {
@@ -101,10 +106,65 @@ __global__ __launch_bounds__(T) void axis_fft_kernel(AxisFft d) {
         if (b < nlines) {
             float2 z = Z[k * B + b];
             if constexpr (BLU) z = cmul(z, d.chirp[k]);
-            const int64_t a = CONTIG ? base + e : base + (int64_t)k * d.inner + b;
+            if (d.post_tw) {  // four-step first pass (a strided view): twiddle of column j at output k
+                const int64_t j2 = (col0 + b) / d.post_inner;
+                z = cmul(z, d.post_tw[(j2 * k) % d.post_n]);
+            }
+            int64_t a;
+            if (tr) {
+                const int64_t line = CONTIG ? line0 + b : line0;
+                const int64_t j = CONTIG ? 0 : col0 + b;
+                const int64_t ob = line / d.tr_n1, k1 = line - ob * d.tr_n1;
+                a = (ob * d.tr_n + k1 + d.tr_n1 * (int64_t)k) * d.inner + j;
+            } else {
+                a = CONTIG ? base + e : base + (int64_t)k * d.inner + b;
+            }
             d.re_out[a] = z.x * d.scale;
             d.im_out[a] = sgn * z.y * d.scale;
         }
+    }
+}
+
+// Bluestein in several passes: element e of [outer, M, inner] has axis index m = (e / inner) % M.
+__global__ void bluestein_pre_kernel(const float* __restrict__ re, const float* __restrict__ im,
+                                     float* __restrict__ ure, float* __restrict__ uim,
+                                     const float2* __restrict__ chirp, int64_t outer, int64_t n, int64_t M,
+                                     int64_t inner, int inverse) {
+    const int64_t total = outer * M * inner;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e % inner, r = e / inner, m = r % M, o = r / M;
+        float2 v = make_float2(0.f, 0.f);
+        if (m < n) {
+            const int64_t a = (o * n + m) * inner + j;
+            v = cmul(make_float2(re[a], inverse ? -im[a] : im[a]), chirp[m]);
+        }
+        ure[e] = v.x;
+        uim[e] = v.y;
+    }
+}
+
+__global__ void bluestein_mid_kernel(float* __restrict__ ure, float* __restrict__ uim, const float2* __restrict__ hat,
+                                     int64_t outer, int64_t M, int64_t inner) {
+    const int64_t total = outer * M * inner;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = (e / inner) % M;
+        const float2 v = cmul(make_float2(ure[e], uim[e]), hat[m]);
+        ure[e] = v.x;
+        uim[e] = v.y;
+    }
+}
+
+__global__ void bluestein_post_kernel(const float* __restrict__ ure, const float* __restrict__ uim,
+                                      float* __restrict__ re, float* __restrict__ im,
+                                      const float2* __restrict__ chirp, int64_t outer, int64_t n, int64_t M,
+                                      int64_t inner, int inverse, float scale) {
+    const int64_t total = outer * n * inner;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e % inner, r = e / inner, k = r % n, o = r / n;
+        const int64_t a = (o * M + k) * inner + j;
+        const float2 v = cmul(make_float2(ure[a], uim[a]), chirp[k]);
+        re[e] = v.x * scale;
+        im[e] = (inverse ? -v.y : v.y) * scale;
     }
 }
 
@@ -153,6 +213,32 @@ int launch_axis_fft(const AxisFft& d, int blocks, void* stream) {
         case 1: return launch_axis_t<kThreadsWide>(d, blocks, lds, st);
         default: return launch_axis_t<kThreads>(d, blocks, lds, st);
     }
+}
+
+static dim3 stride_grid(int64_t total) {
+    const int64_t blocks = (total + 255) / 256;
+    return dim3((unsigned)(blocks < 65536 ? blocks : 65536));
+}
+
+int launch_bluestein_pre(const float* re, const float* im, float* ure, float* uim, const float2* chirp, int64_t outer,
+                         int64_t n, int64_t M, int64_t inner, int inverse, void* stream) {
+    hipLaunchKernelGGL(bluestein_pre_kernel, stride_grid(outer * M * inner), dim3(256), 0, (hipStream_t)stream, re, im,
+                       ure, uim, chirp, outer, n, M, inner, inverse);
+    return (int)hipGetLastError();
+}
+
+int launch_bluestein_mid(float* ure, float* uim, const float2* hat, int64_t outer, int64_t M, int64_t inner,
+                         void* stream) {
+    hipLaunchKernelGGL(bluestein_mid_kernel, stride_grid(outer * M * inner), dim3(256), 0, (hipStream_t)stream, ure,
+                       uim, hat, outer, M, inner);
+    return (int)hipGetLastError();
+}
+
+int launch_bluestein_post(const float* ure, const float* uim, float* re, float* im, const float2* chirp, int64_t outer,
+                          int64_t n, int64_t M, int64_t inner, int inverse, float scale, void* stream) {
+    hipLaunchKernelGGL(bluestein_post_kernel, stride_grid(outer * n * inner), dim3(256), 0, (hipStream_t)stream, ure,
+                       uim, re, im, chirp, outer, n, M, inner, inverse, scale);
+    return (int)hipGetLastError();
 }
 
 int launch_complex_mul(const float* ar, const float* ai, const float* br, const float* bi, float* orr, float* oi,

@@ -4,14 +4,17 @@
 //   fft_nd_into        core/fft.rs:53-100  every axis in order, through two workspace slots
 //   complex_mul_into   core/fft.rs:102-135 pointwise product, optionally with conj(b)
 //
-// Each axis transform is one launch of the LDS FFT kernel of fft_axis.hip: the Stockham FFT for
-// 7-smooth lengths up to 16384, Bluestein's convolution (power-of-two length up to 16384) for
-// other lengths up to 8192. The reference evaluates every output as an O(n) sum with its own
-// sin/cos (core/kernels/fft.rs:11-47).
+// Each axis transform is one launch of the LDS FFT kernel of fft_axis.hip when a workgroup holds
+// the line: the Stockham FFT for 7-smooth lengths up to 16384, Bluestein's convolution
+// (power-of-two length up to 16384) for other lengths up to 8192. Longer lines run in passes:
+// 7-smooth lengths as a four-step transform of two single-pass lengths, others as Bluestein's
+// convolution whose power-of-two transforms are four-step. The reference evaluates every output
+// as an O(n) sum with its own sin/cos (core/kernels/fft.rs:11-47).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -177,22 +180,148 @@ lucem_status lucem::axis_fft_desc(lucem_client* c, int n, int64_t outer, int64_t
     return LUCEM_OK;
 }
 
-static lucem_status run_axis(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, const uint64_t* dims,
-                             uint32_t ndim, uint32_t axis, bool inverse) {
-    int64_t outer = 1, inner = 1;
-    for (uint32_t a = 0; a < axis; a++) outer *= (int64_t)dims[a];
-    for (uint32_t a = axis + 1; a < ndim; a++) inner *= (int64_t)dims[a];
+namespace {
+
+// exp(-2 pi i m / n), m < n, computed in FP64 and cached per client: the twiddles between the
+// two passes of a four-step transform of length n.
+lucem_status long_twiddles(lucem_client* c, int64_t n, const float2** out) {
+    auto it = c->long_twiddles.find(n);
+    if (it == c->long_twiddles.end()) {
+        const double pi = 3.14159265358979323846264338327950288;
+        std::vector<float2> host((size_t)n);
+        for (int64_t m = 0; m < n; m++) {
+            const std::complex<double> w = std::polar(1.0, -2.0 * pi * (double)m / (double)n);
+            host[(size_t)m] = make_float2((float)w.real(), (float)w.imag());
+        }
+        float2* dev = nullptr;
+        hipError_t e = hipMalloc(&dev, sizeof(float2) * host.size());
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(four-step twiddles)");
+        e = hipMemcpy(dev, host.data(), sizeof(float2) * host.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(dev);
+            return hip_fail(e, "hipMemcpy(four-step twiddles)");
+        }
+        it = c->long_twiddles.emplace(n, dev).first;
+    }
+    *out = it->second;
+    return LUCEM_OK;
+}
+
+// One launch of the single-pass kernel over [outer, n, inner].
+lucem_status one_pass(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, int64_t outer, int n,
+                      int64_t inner, bool inverse, bool unit_scale, const AxisFft* extra) {
     AxisFft d;
     int blocks = 0;
-    lucem_status st = axis_fft_desc(c, (int)dims[axis], outer, inner, inverse, &d, &blocks);
+    lucem_status st = axis_fft_desc(c, n, outer, inner, inverse, &d, &blocks);
     if (st) return st;
+    if (extra) {
+        d.post_tw = extra->post_tw;
+        d.post_n = extra->post_n;
+        d.post_inner = extra->post_inner;
+        d.tr_n1 = extra->tr_n1;
+        d.tr_n = extra->tr_n;
+    }
+    if (unit_scale) d.scale = 1.0f;
     d.re_in = ri;
     d.im_in = ii;
     d.re_out = ro;
     d.im_out = io;
     const int err = launch_axis_fft(d, blocks, c->stream);
-    if (err != hipSuccess) return hip_fail(err, "fft axis pass");
-    return LUCEM_OK;
+    return err == hipSuccess ? LUCEM_OK : hip_fail(err, "fft axis pass");
+}
+
+// The 7-smooth factor of n (n 7-smooth) closest to sqrt(n) with both factors at most the
+// single-pass limit; 0 when there is none.
+int64_t split_smooth(int64_t n) {
+    int64_t best = 0;
+    for (int64_t a = 2; a <= LUCEM_MAX_FFT_LEN && a < n; a++) {
+        if (n % a || n / a > LUCEM_MAX_FFT_LEN || !smooth7((int)a)) continue;
+        if (!best || std::llabs(a * a - n) < std::llabs(best * best - n)) best = a;
+    }
+    return best;
+}
+
+lucem_status fft_lines(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, int64_t outer,
+                       int64_t n, int64_t inner, bool inverse, bool unit_scale);
+
+// Four-step transform of the 7-smooth length n = n1 * n2 (Bailey): with j = j1 * n2 + j2 and
+// k = k1 + n1 * k2, X[k] = sum_j2 w_n2^(j2 k2) w_n^(j2 k1) sum_j1 x[j] w_n1^(j1 k1). Pass 1: the
+// length-n1 transforms over j1 (a strided view), times w_n^(j2 k1), into a scratch buffer; pass 2:
+// the length-n2 transforms over j2, stored in the transposed order k1 + n1 * k2.
+lucem_status four_step(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, int64_t outer,
+                       int64_t n, int64_t inner, bool inverse, bool unit_scale) {
+    const int64_t n1 = split_smooth(n);
+    if (!n1)
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "FFT length " + std::to_string(n) + ": no split into two lengths of at "
+                                                                                   "most 16384");
+    const int64_t n2 = n / n1;
+    const int64_t total = outer * n * inner;
+    float *tr, *ti;
+    lucem_status st;
+    lucem_workspace* ws = c->internal_ws.get();
+    if ((st = ws_slot(ws, "fft4_re", sizeof(float) * total, (void**)&tr))) return st;
+    if ((st = ws_slot(ws, "fft4_im", sizeof(float) * total, (void**)&ti))) return st;
+    AxisFft x;
+    std::memset(&x, 0, sizeof x);
+    if ((st = long_twiddles(c, n, &x.post_tw))) return st;
+    x.post_n = n;
+    x.post_inner = inner;
+    if ((st = one_pass(c, ri, ii, tr, ti, outer, (int)n1, n2 * inner, inverse, unit_scale, &x))) return st;
+    std::memset(&x, 0, sizeof x);
+    x.tr_n1 = n1;
+    x.tr_n = n;
+    return one_pass(c, tr, ti, ro, io, outer * n1, (int)n2, inner, inverse, unit_scale, &x);
+}
+
+// Bluestein's convolution in several passes, for lengths whose power-of-two convolution M is
+// longer than one workgroup holds: the chirped input zero-padded to M, its transform (four-step),
+// the product with the chirp filter's transform, the unnormalised inverse, the output chirp.
+lucem_status bluestein_long(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, int64_t outer,
+                            int64_t n, int64_t inner, bool inverse, bool unit_scale) {
+    int64_t M = 1;
+    while (M < 2 * n - 1) M <<= 1;
+    if (M > (int64_t)LUCEM_MAX_FFT_LEN * LUCEM_MAX_FFT_LEN)
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "FFT length " + std::to_string(n) + " is too long");
+    const float2 *chirp, *hat;
+    lucem_status st = chirp_tables(c, (int)n, (int)M, &chirp, &hat);
+    if (st) return st;
+    float *ur, *ui;
+    lucem_workspace* ws = c->internal_ws.get();
+    const int64_t total = outer * M * inner;
+    if ((st = ws_slot(ws, "fftb_re", sizeof(float) * total, (void**)&ur))) return st;
+    if ((st = ws_slot(ws, "fftb_im", sizeof(float) * total, (void**)&ui))) return st;
+    auto check = [](int err, const char* what) { return err == hipSuccess ? LUCEM_OK : hip_fail(err, what); };
+    if ((st = check(launch_bluestein_pre(ri, ii, ur, ui, chirp, outer, n, M, inner, inverse ? 1 : 0, c->stream),
+                    "bluestein pre")))
+        return st;
+    if ((st = fft_lines(c, ur, ui, ur, ui, outer, M, inner, false, true))) return st;
+    if ((st = check(launch_bluestein_mid(ur, ui, hat, outer, M, inner, c->stream), "bluestein mid"))) return st;
+    if ((st = fft_lines(c, ur, ui, ur, ui, outer, M, inner, true, true))) return st;
+    const float scale = inverse && !unit_scale ? (float)(1.0 / (double)n) : 1.0f;
+    return check(launch_bluestein_post(ur, ui, ro, io, chirp, outer, n, M, inner, inverse ? 1 : 0, scale, c->stream),
+                 "bluestein post");
+}
+
+// The transform of every line of [outer, n, inner] (forward, or inverse with 1/n unless
+// unit_scale): one pass when a workgroup holds the line, else four-step or Bluestein in passes.
+// ro/io may alias ri/ii.
+lucem_status fft_lines(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, int64_t outer,
+                       int64_t n, int64_t inner, bool inverse, bool unit_scale) {
+    const bool smooth = n <= 0x7FFFFFFF && smooth7((int)n);
+    if ((smooth && n <= LUCEM_MAX_FFT_LEN) || (!smooth && n <= LUCEM_MAX_FFT_LEN / 2))
+        return one_pass(c, ri, ii, ro, io, outer, (int)n, inner, inverse, unit_scale, nullptr);
+    if (smooth && split_smooth(n)) return four_step(c, ri, ii, ro, io, outer, n, inner, inverse, unit_scale);
+    return bluestein_long(c, ri, ii, ro, io, outer, n, inner, inverse, unit_scale);
+}
+
+}  // namespace
+
+static lucem_status run_axis(lucem_client* c, const float* ri, const float* ii, float* ro, float* io, const uint64_t* dims,
+                             uint32_t ndim, uint32_t axis, bool inverse) {
+    int64_t outer = 1, inner = 1;
+    for (uint32_t a = 0; a < axis; a++) outer *= (int64_t)dims[a];
+    for (uint32_t a = axis + 1; a < ndim; a++) inner *= (int64_t)dims[a];
+    return fft_lines(c, ri, ii, ro, io, outer, (int64_t)dims[axis], inner, inverse, false);
 }
 
 extern "C" lucem_status lucem_fft_1d_into_f32(lucem_client* client, const float* re_in, const uint64_t* re_in_dims,

@@ -136,6 +136,7 @@ lucem_client::~lucem_client() {
     for (auto ev : event_pool) (void)hipEventDestroy(ev);
     for (auto& kv : twiddles) (void)hipFree(kv.second);
     for (auto& kv : chirps) (void)hipFree(kv.second);
+    for (auto& kv : long_twiddles) (void)hipFree(kv.second);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 

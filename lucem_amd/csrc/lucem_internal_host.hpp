@@ -40,6 +40,7 @@ struct lucem_client {
     bool own_stream = false;
     std::map<int, float2*> twiddles;               // FFT length -> device twiddle table
     std::map<int, float2*> chirps;                 // Bluestein length n -> chirp [n] + its transform [M]
+    std::map<int64_t, float2*> long_twiddles;      // four-step length n -> exp(-2 pi i m / n), m < n
     std::unique_ptr<lucem_workspace> internal_ws;  // used by the allocating-style entry points
     bool profiling = false;
     std::vector<lucem::ProfRecord> records;        // launches since profiling was enabled

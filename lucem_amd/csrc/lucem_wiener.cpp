@@ -8,8 +8,8 @@
 // The PSF's transfer function is a direct FP64 DFT of the kernel (wiener.hip). When both image
 // sizes are 7-smooth (at most 16384) the image's half spectrum comes from the row R2C and the
 // line FFT along y, and the result from the inverse line FFT and the row C2R, whose store
-// applies the optional clip. Other sizes (up to 8192) take the general complex FFT of
-// lucem_fft.cpp over the full spectrum, with Bluestein lines where a length is not 7-smooth.
+// applies the optional clip. Other sizes take the general complex FFT of lucem_fft.cpp over the
+// full spectrum (Bluestein lines where a length is not 7-smooth, passes for long lines).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -65,10 +65,10 @@ lucem_status finish_geometry(const uint64_t* in_dims, const uint64_t* psf_dims, 
         return fail(LUCEM_TOO_MANY_ELEMENTS,
                     "dims " + dims_str(in_dims, 2) + " exceed the device element limit of 4294967295");
     const int H = (int)std::min<uint64_t>(in_dims[0], 1u << 30), W = (int)std::min<uint64_t>(in_dims[1], 1u << 30);
-    auto general_ok = [](int v) { return fft_length_ok(v) || v <= LUCEM_MAX_FFT_LEN / 2; };
+    // the general FFT: 7-smooth lengths that split into two of at most 16384, others up to 2^27
+    auto general_ok = [](int v) { return fft_length_ok(v) || v <= (1 << 27); };
     if (!general_ok(H) || !general_ok(W))
-        return fail(LUCEM_UNSUPPORTED_SHAPE, "image " + dims_str(in_dims, 2) +
-                                                 ": sizes up to 16384 (7-smooth) or 8192 (any) per axis are supported");
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "image " + dims_str(in_dims, 2) + ": an axis is longer than the FFT takes");
     g->H = H;
     g->W = W;
     g->kh = (int)psf_dims[0];

@@ -206,8 +206,25 @@ struct AxisFft {
     const float2* tw;
     const float2* chirp;      // Bluestein: exp(-i pi m^2 / n), m < n
     const float2* chirp_hat;  // Bluestein: DFT_M of the conjugate chirp sequence, times 1/M
+    // Four-step passes of a longer transform (all zero otherwise). First pass: output k of
+    // column j is multiplied by post_tw[((j / post_inner) * k) mod post_n]. Second pass: line
+    // o' = o * tr_n1 + k1 stores its output k at ((o * tr_n + k1 + tr_n1 * k) * inner + j), the
+    // transposed order of the long axis.
+    const float2* post_tw;
+    int64_t post_n, post_inner;
+    int64_t tr_n1, tr_n;
 };
 int launch_axis_fft(const AxisFft& d, int blocks, void* stream);
+// Bluestein in several passes (lines longer than one workgroup holds), over [outer, M, inner]:
+// pre:  u[o][m][j] = m < n ? (x or conj(x))[o][m][j] * chirp[m] : 0
+// mid:  u *= chirp_hat[m]
+// post: y[o][k][j] = (u[o][k][j] * chirp[k]) (conjugated for the inverse) * scale, k < n
+int launch_bluestein_pre(const float* re, const float* im, float* ure, float* uim, const float2* chirp, int64_t outer,
+                         int64_t n, int64_t M, int64_t inner, int inverse, void* stream);
+int launch_bluestein_mid(float* ure, float* uim, const float2* hat, int64_t outer, int64_t M, int64_t inner,
+                         void* stream);
+int launch_bluestein_post(const float* ure, const float* uim, float* re, float* im, const float2* chirp, int64_t outer,
+                          int64_t n, int64_t M, int64_t inner, int inverse, float scale, void* stream);
 size_t axis_fft_lds_bytes(int L, int B);
 int launch_complex_mul(const float* ar, const float* ai, const float* br, const float* bi, float* orr, float* oi,
                        int64_t n, int conj_b, void* stream);

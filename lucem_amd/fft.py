@@ -7,7 +7,8 @@ Reference (neuro-kitchen/lucem): crates/lucem-ops/src/core/fft.rs
 - ``complex_mul_into``   core/fft.rs:102-135 pointwise complex product, optionally with conj(b)
 
 Forward transforms use exp(-2 pi i k m / n) and no scaling; inverse transforms exp(+...) / n.
-Any length: 7-smooth lengths up to 16384 and other lengths up to 8192 (Bluestein).
+Any length up to 2^27: one pass for 7-smooth lengths up to 16384 and other lengths up to 8192
+(Bluestein), four-step passes for longer lines.
 """
 
 from __future__ import annotations

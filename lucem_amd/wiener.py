@@ -8,8 +8,8 @@ Reference (neuro-kitchen/lucem):
 - ``UnsupervisedWiener`` (defaults)         crates/lucem-ops/src/restore/deconvolve.rs:330-353
 - ``unsupervised_wiener``                   crates/lucem-ops/src/restore/deconvolve.rs:357-509
 
-Arrays are float32 device tensors. Both forms work on the circular domain of the image: sizes
-up to 16384 per axis when 7-smooth, up to 8192 otherwise. Errors raise
+Arrays are float32 device tensors. Both forms work on the circular domain of the image: the R2C
+passes for 7-smooth sizes up to 16384, the general complex FFT for any other size. Errors raise
 :class:`lucem_amd.LucemError` in the reference's check order.
 """
 

@@ -245,7 +245,7 @@ def test_wiener_check_order_without_device(L):
     assert call((8, 8), (9, 3), (8, 8)) == 4
     assert b"must not exceed image" in L.lucem_last_error()
     assert call((8, 11), (3, 3), (8, 11)) == 11  # 11 is not 7-smooth: the general FFT route
-    assert call((8, 8209), (3, 3), (8, 8209)) == 9  # a prime above 8192
+    assert call((8, 8209), (3, 3), (8, 8209)) == 11  # a prime above 8192: Bluestein lines in passes
     assert call((8, 8), (3, 3), (8, 8)) == 11  # valid arguments, no client
 
 

@@ -43,6 +43,8 @@ WIENER_CASES = [
     ((509, 768), (15, 15), 0.1, False),
     ((4200, 16), (7, 3), 0.1, False),   # a y line of 4200 (7-smooth) takes the 512-thread pass
     ((16, 4200), (3, 7), 0.1, False),   # an x line of 4200 takes the 512-thread row pass
+    ((8209, 16), (5, 3), 0.1, False),   # a prime above 8192: Bluestein lines in passes
+    ((8, 20000), (3, 9), 0.05, False),  # 7-smooth above 16384: four-step lines
 ]
 
 
@@ -92,10 +94,6 @@ def test_wiener_errors(gpu):
     with pytest.raises(LA.LucemError) as e:
         LA.wiener_deconvolve(y, torch.ones((17, 3), device="cuda"))
     assert e.value.kind == "INVALID_WEIGHTS"
-    z = torch.ones((8209, 16), device="cuda")  # a prime above 8192
-    with pytest.raises(LA.LucemError) as e:
-        LA.wiener_deconvolve(z, torch.ones((3, 3), device="cuda"))
-    assert e.value.kind == "UNSUPPORTED_SHAPE"
     with pytest.raises(LA.LucemError) as e:
         LA.wiener_deconvolve_into(y, torch.ones((3, 3), device="cuda"), y)
     assert e.value.kind == "ALIASED_BUFFERS"
