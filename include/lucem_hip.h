@@ -52,6 +52,10 @@
  *                         f64 direct
  *   lucem_correlate_nd_into_f32
  *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
+ *   lucem_phase_correlation_f32
+ *                         lucem_ops::register::phase_correlation (crates/lucem-ops/src/register/phase.rs:
+ *                         18-250), PhaseCorrelation as lucem_phase_correlation_settings; the result
+ *                         written to a caller host struct
  *   lucem_render_ellipses_f32 / lucem_render_spheres_f32
  *                         the synthetic workloads' ground truth rendered on the device (no
  *                         reference counterpart; with lucem_add_noise_f32, device-side phantoms
@@ -404,6 +408,34 @@ lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const double* inp
                                          uint32_t in_ndim, const double* weights, const uint64_t* w_dims,
                                          uint32_t w_ndim, double* output, const uint64_t* out_dims,
                                          uint32_t out_ndim, const lucem_correlate_settings* settings);
+
+/* ---- Phase correlation (register/phase.rs:18-250) --------------------------------------------
+ * Settings as PhaseCorrelation (defaults: upsample_factor 1, normalize_phase 1). */
+typedef struct lucem_phase_correlation_settings {
+    uint64_t upsample_factor;
+    int32_t normalize_phase;
+    uint32_t reserved;
+} lucem_phase_correlation_settings;
+void lucem_phase_correlation_settings_default(lucem_phase_correlation_settings* out);
+/* PhaseCorrelationResult: shift[0..ndim) (pixels per axis that register moving onto reference),
+ * error (normalised RMS, [0, 1]) and phasediff (radians). */
+typedef struct lucem_phase_correlation_result {
+    double shift[3];
+    uint32_t ndim;
+    uint32_t reserved;
+    double error;
+    double phasediff;
+} lucem_phase_correlation_result;
+/* phase_correlation::<f32> (phase.rs:52-250) for 1-D to 3-D arrays. Checks in the reference's
+ * order: same shape (LUCEM_SHAPE_MISMATCH), non-empty (LUCEM_EMPTY_INPUT), upsample_factor >= 1
+ * (LUCEM_INVALID_PARAMETER); then ndim <= 3 (LUCEM_UNSUPPORTED_SHAPE) and the element limit. The
+ * transforms run on the device (the general FFT), the subpixel sweeps' local inverse DFTs are
+ * FP64 sums on the device. Blocks until the result is written to `out` (host memory), as the
+ * reference returns host values. */
+lucem_status lucem_phase_correlation_f32(lucem_client* client, const float* reference, const uint64_t* ref_dims,
+                                         uint32_t ref_ndim, const float* moving, const uint64_t* mov_dims,
+                                         uint32_t mov_ndim, const lucem_phase_correlation_settings* settings,
+                                         lucem_phase_correlation_result* out, lucem_workspace* ws);
 
 /* ---- Synthetic phantoms on the device (no reference counterpart) ----------------------------
  * out = background + the sum, in list order and in float64, of the value of every object the

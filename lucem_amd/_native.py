@@ -52,6 +52,7 @@ EXPORTED = [
     "lucem_fft_1d_into_f32", "lucem_fft_nd_into_f32", "lucem_complex_mul_into_f32", "lucem_add_noise_f32",
     "lucem_correlate_settings_default", "lucem_correlate_2d_into_f32", "lucem_correlate_nd_into_f32",
     "lucem_correlate_2d_into_f64", "lucem_render_ellipses_f32", "lucem_render_spheres_f32",
+    "lucem_phase_correlation_settings_default", "lucem_phase_correlation_f32",
 ]
 
 
@@ -100,6 +101,16 @@ class UnsupervisedWienerSettings(ctypes.Structure):
 
 class CorrelateSettings(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("origin", ctypes.c_int32 * 3), ("cval", ctypes.c_double)]
+
+
+class PhaseCorrelationSettings(ctypes.Structure):
+    _fields_ = [("upsample_factor", ctypes.c_uint64), ("normalize_phase", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class PhaseCorrelationResult(ctypes.Structure):
+    _fields_ = [("shift", ctypes.c_double * 3), ("ndim", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("error", ctypes.c_double), ("phasediff", ctypes.c_double)]
 
 
 class Ellipse(ctypes.Structure):
@@ -177,6 +188,11 @@ def lib() -> ctypes.CDLL:
             L.lucem_render_ellipses_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, ctypes.c_uint64,
                                                     ctypes.c_double]
             L.lucem_render_spheres_f32.argtypes = L.lucem_render_ellipses_f32.argtypes
+            L.lucem_phase_correlation_settings_default.argtypes = [ctypes.POINTER(PhaseCorrelationSettings)]
+            L.lucem_phase_correlation_settings_default.restype = None
+            L.lucem_phase_correlation_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
+                                                      ctypes.POINTER(PhaseCorrelationSettings),
+                                                      ctypes.POINTER(PhaseCorrelationResult), _vp]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

@@ -168,6 +168,18 @@ int launch_render_ellipses(float* out, int H, int W, const lucem_ellipse* obj, i
 int launch_render_spheres(float* out, int D, int H, int W, const lucem_sphere* obj, int64_t count, double background,
                           void* stream);
 
+// Phase correlation (phase.hip): workgroups of the reductions; A * conj(B) (normalised when
+// asked); per-workgroup largest re^2 + im^2 and its first index; per-workgroup FP64 sums of a
+// spectrum against plane waves exp(+2 pi i f(k) . s) at C positions (3 doubles each, z y x);
+// per-workgroup FP64 sums of squares.
+int phase_blocks(int64_t n);
+int launch_cross_power(const float* ar, const float* ai, const float* br, const float* bi, float* pr, float* pi,
+                       int64_t n, int normalize, void* stream);
+int launch_argmax_mag2(const float* re, const float* im, int64_t n, float* part_v, int64_t* part_i, void* stream);
+int launch_plane_wave_sums(const float* re, const float* im, const int64_t dims3[3], const double* pos, int C,
+                           double2* part, void* stream);
+int launch_sum_squares(const float* x, int64_t n, double* part, void* stream);
+
 // Constant-mode cval of the f32 correlation: out = in - cval; *sum = sum(w) in f64;
 // out += cval * *sum.
 int launch_cval_shift(const float* in, float* out, float cval, int64_t n, void* stream);

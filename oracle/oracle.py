@@ -25,6 +25,8 @@ Functions and the reference code each one restates (paths relative to the refere
                               the reference's O(N^2) DFT (core/fft.rs:55-100)
 - :func:`unsupervised_wiener64`
                               restore/deconvolve.rs:357-509 in FP64 over the full spectrum
+- :func:`phase_correlation64` crates/lucem-ops/src/register/phase.rs:52-250 in FP64, numpy's FFT
+                              in place of the reference's DFT
 
 Parity pins (see tests/test_oracle.py): pad_index against the reference's own host restatement
 (crates/lucem-ops/src/test_support.rs:21-45, restated in the test) and numpy.pad; correlation
@@ -236,6 +238,87 @@ def fft_axis64(re: np.ndarray, im: np.ndarray, axes=None, inverse: bool = False)
     for ax in (range(z.ndim) if axes is None else axes):
         z = np.fft.ifft(z, axis=ax) if inverse else np.fft.fft(z, axis=ax)
     return z.real.copy(), z.imag.copy()
+
+
+def _signed_freqs(n: int) -> np.ndarray:
+    """phase.rs:145-158: k / n for k <= n / 2, else (k - n) / n."""
+    k = np.arange(n)
+    return np.where(k <= n // 2, k, k - n).astype(np.float64) / n
+
+
+def phase_correlation64(reference: np.ndarray, moving: np.ndarray, upsample_factor: int = 1,
+                        normalize_phase: bool = True):
+    """phase_correlation (register/phase.rs:52-250) in FP64. Returns ``(shift, error, phasediff)``.
+
+    The cross-power spectrum A conj(B) (normalised by its magnitude above 1e-12, or 0), its
+    inverse transform, the first index of the largest |cc|^2 as the integer shift (signed), then
+    for ``upsample_factor > 1`` two coordinate sweeps of the local inverse DFT of the cross-power
+    spectrum (radius ceil(1.5 u) at step 1/u, then radius 2 at step 0.25/u, each axis in turn,
+    keeping the first strictly larger |sum|^2), the shift snapped to the 1/u grid; the phase of
+    the peak; the normalised RMS error from the powers of both images and the unnormalised
+    correlation at the peak index.
+    """
+    ref = np.asarray(reference, dtype=np.float64)
+    mov = np.asarray(moving, dtype=np.float64)
+    dims = ref.shape
+    ndim = ref.ndim
+    n_total = ref.size
+    A = np.fft.fftn(ref)
+    B = np.fft.fftn(mov)
+    prod = A * np.conj(B)
+    if normalize_phase:
+        mag = np.abs(prod)
+        prod = np.where(mag > 1e-12, prod / np.where(mag > 1e-12, mag, 1.0), 0.0)
+    cc = np.fft.ifftn(prod)
+    max_idx = int(np.argmax((cc.real ** 2 + cc.imag ** 2).ravel()))
+    coords = np.unravel_index(max_idx, dims)
+    shift = [float(k) if k <= n // 2 else float(k - n) for k, n in zip(coords, dims)]
+    peak = complex(cc.ravel()[max_idx])
+    if upsample_factor > 1:
+        u = float(upsample_factor)
+        half_span = int(np.ceil(1.5 * u))
+        freqs = [_signed_freqs(n) for n in dims]
+        grids = np.meshgrid(*freqs, indexing="ij")
+        pr = prod.ravel()
+        for pas in range(2):
+            radius = half_span if pas == 0 else 2
+            step = 1.0 / u if pas == 0 else 0.25 / u
+            for ax in range(ndim):
+                other = np.zeros(dims)
+                for d in range(ndim):
+                    if d != ax:
+                        other = other + 2.0 * np.pi * grids[d] * shift[d]
+                oc, os_ = np.cos(other).ravel(), np.sin(other).ravel()
+                center = shift[ax]
+                best = (-1.0, center, peak)
+                for m in range(-radius, radius + 1):
+                    cand = center + m * step
+                    a = (2.0 * np.pi * grids[ax] * cand).ravel()
+                    ac, as_ = np.cos(a), np.sin(a)
+                    c = oc * ac - os_ * as_
+                    sn = os_ * ac + oc * as_
+                    sre = float(np.sum(pr.real * c - pr.imag * sn)) / n_total
+                    sim = float(np.sum(pr.real * sn + pr.imag * c)) / n_total
+                    mag2 = sre * sre + sim * sim
+                    if mag2 > best[0]:
+                        best = (mag2, cand, complex(sre, sim))
+                shift[ax] = best[1]
+                peak = best[2]
+        shift = [round(s_ * u) / u for s_ in shift]
+    phasediff = float(np.arctan2(peak.imag, peak.real))
+    ref_power = float(np.sum(ref * ref))
+    mov_power = float(np.sum(mov * mov))
+    denom = ref_power * mov_power
+    if denom <= np.finfo(np.float64).eps:
+        error = 0.0
+    else:
+        if normalize_phase:
+            un = np.fft.ifftn(A * np.conj(B)).ravel()[max_idx]
+        else:
+            un = peak
+        amp2 = (un.real ** 2 + un.imag ** 2) * n_total * n_total
+        error = float(np.sqrt(1.0 - min(max(amp2 / denom, 0.0), 1.0)))
+    return shift, error, phasediff
 
 
 def pcg_hash32(x: np.ndarray) -> np.ndarray:

@@ -220,6 +220,30 @@ def test_render_check_order_without_device(L):
     assert call(sph, (4, 8, 8)) == 11
 
 
+def test_phase_correlation_check_order_without_device(L):
+    """phase.rs:58-68: same shape, non-empty, upsample_factor >= 1; then rank and limits."""
+    from lucem_amd import _native as N
+
+    s = N.PhaseCorrelationSettings()
+    L.lucem_phase_correlation_settings_default(ctypes.byref(s))
+    assert (s.upsample_factor, s.normalize_phase) == (1, 1)
+    out = N.PhaseCorrelationResult()
+
+    def call(a, b, up=1):
+        s.upsample_factor = up
+        return L.lucem_phase_correlation_f32(None, N._vp(0x10000), N.dims(a), len(a), N._vp(0x20000000), N.dims(b),
+                                             len(b), ctypes.byref(s), ctypes.byref(out), None)
+
+    assert call((8, 8), (8, 9), up=0) == 1
+    assert b"reference shape [8, 8] does not match moving shape [8, 9]" in L.lucem_last_error()
+    assert call((0, 8), (0, 8), up=0) == 13
+    assert b"phase_correlation cannot be computed on an empty array" in L.lucem_last_error()
+    assert call((8, 8), (8, 8), up=0) == 5
+    assert call((2, 2, 2, 2), (2, 2, 2, 2)) == 9
+    assert call((8, 8), (8, 8)) == 11
+    assert call((5,), (5,), up=3) == 11
+
+
 def test_wiener_check_order_without_device(L):
     from lucem_amd import _native as N
 

@@ -469,3 +469,53 @@ def test_noise_impulse_fractions(oracle):
     assert abs(hit.mean() - 0.3) < 2e-3 and abs((sp[hit] == 1).mean() - 0.25) < 5e-3
     # the 64-bit seed is folded: a change in its high half changes the stream
     assert (oracle.add_noise32(x, 2, 0.1, seed=1 << 32) != salt).any()
+
+
+def _fourier_shift(x, shift):
+    """x moved by ``shift`` (pixels per axis) through its spectrum, with the signed frequencies
+    of phase.rs:145-158."""
+    X = np.fft.fftn(x)
+    ph = np.zeros(x.shape)
+    for ax, (n, s) in enumerate(zip(x.shape, shift)):
+        k = np.arange(n)
+        f = np.where(k <= n // 2, k, k - n) / n
+        shp = [1] * x.ndim
+        shp[ax] = n
+        ph = ph + (f * s).reshape(shp)
+    return np.fft.ifftn(X * np.exp(-2j * np.pi * ph)).real
+
+
+def test_phase_correlation_reference_kat(oracle):
+    """The reference test (register/phase.rs:262-293): a 16x16 sample_positive_f64 image moved by
+    (2.25, -1.5) in the Fourier domain, upsample 20: the shift (-2.25, 1.5) within assert_close
+    at rtol 0.1."""
+    base = np.array([((i * 6151 + 49999) % 1013) * 0.1 + 0.1 for i in range(256)]).reshape(16, 16)
+    moved = _fourier_shift(base, (2.25, -1.5))
+    shift, _, _ = oracle.phase_correlation64(base.astype(np.float32), moved.astype(np.float32), 20, True)
+    assert_close(np.array(shift), np.array([-2.25, 1.5]), 0.1)
+
+
+@pytest.mark.parametrize("shape,shift", [((40, 50), (3, -7)), ((33,), (-5,)), ((6, 9, 10), (1, -4, 5))])
+def test_phase_correlation_integer_circular_shifts(oracle, shape, shift):
+    """A circular shift is recovered exactly (negated, as the signed index k <= n / 2 of
+    phase.rs:123-131, so -5 on an axis of 10 reads +5), with zero error and zero phase."""
+    x = np.random.default_rng(len(shape)).uniform(size=shape)
+    want = []
+    for s_, n in zip(shift, shape):
+        k = (-s_) % n
+        want.append(float(k if k <= n // 2 else k - n))
+    for normalize in (True, False):
+        got, err, ph = oracle.phase_correlation64(x, np.roll(x, shift, axis=tuple(range(x.ndim))), 1, normalize)
+        assert got == want
+        assert err < 1e-6 and abs(ph) < 1e-9
+
+
+@pytest.mark.parametrize("shape,shift,u", [((33, 35), (1.25, -2.75), 4), ((51,), (-3.5,), 2),
+                                           ((21, 25), (0.4, -1.6), 5)])
+def test_phase_correlation_subpixel_shift_on_the_grid(oracle, shape, shift, u):
+    """Odd sizes (no Nyquist bin) and a shift on the 1/u grid: the normalised cross-power spectrum
+    is an exact phase ramp, so the upsampled peak is exactly the shift (negated) with |peak| = 1."""
+    x = np.random.default_rng(7).uniform(size=shape)
+    got, err, ph = oracle.phase_correlation64(x, _fourier_shift(x, shift), u, True)
+    np.testing.assert_allclose(got, [-s for s in shift], atol=1e-12)
+    assert abs(ph) < 1e-9
