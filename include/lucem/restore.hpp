@@ -13,6 +13,8 @@
 //   lucem::NoiseModel, add_noise         crates/lucem-ops/src/intensity/noise.rs:17-143
 //   lucem::Correlate2d, correlate_2d_into, correlate_2d
 //                                        crates/lucem-ops/src/filter/convolve.rs:113-177
+//   lucem::PhaseCorrelation, PhaseCorrelationResult, phase_correlation
+//                                        crates/lucem-ops/src/register/phase.rs:18-250
 //   lucem::Client, Workspace, DeviceArray crates/lucem-core/src/compute/target.rs:95-115,
 //                                        memory/workspace.rs:17-37, memory/array.rs:20-151
 //
@@ -392,6 +394,37 @@ inline OpsResult<DeviceArray<double>> correlate_2d(const Client& client, const D
     if (!out.is_ok()) return out;
     auto r = correlate_2d_into(client, input, weights, out.unwrap(), settings);
     if (!r.is_ok()) return OpsResult<DeviceArray<double>>(r.status(), r.message());
+    return out;
+}
+
+// Settings and result of phase_correlation (register/phase.rs:18-47).
+struct PhaseCorrelation {
+    size_t upsample_factor = 1;
+    bool normalize_phase = true;
+};
+struct PhaseCorrelationResult {
+    std::vector<double> shift;
+    double error = 0.0;
+    double phasediff = 0.0;
+};
+
+// phase_correlation::<f32> (register/phase.rs:52-250): blocks until the result is on the host.
+inline OpsResult<PhaseCorrelationResult> phase_correlation(const Client& client, const DeviceArray<float>& reference,
+                                                           const DeviceArray<float>& moving,
+                                                           const PhaseCorrelation& settings) {
+    lucem_phase_correlation_settings s;
+    lucem_phase_correlation_settings_default(&s);
+    s.upsample_factor = settings.upsample_factor;
+    s.normalize_phase = settings.normalize_phase ? 1 : 0;
+    lucem_phase_correlation_result r{};
+    const lucem_status st = lucem_phase_correlation_f32(
+        client.handle(), reference.data(), reference.dims().data(), (uint32_t)reference.dims().size(), moving.data(),
+        moving.dims().data(), (uint32_t)moving.dims().size(), &s, &r, nullptr);
+    if (st != LUCEM_OK) return OpsResult<PhaseCorrelationResult>(st, lucem_last_error());
+    PhaseCorrelationResult out;
+    out.shift.assign(r.shift, r.shift + r.ndim);
+    out.error = r.error;
+    out.phasediff = r.phasediff;
     return out;
 }
 

@@ -111,6 +111,12 @@ int main() {
         cs.origin = {2, 0};
         CHECK(correlate_2d(client, in, win, cs).status() == LUCEM_INVALID_ORIGIN, "origin check");
     }
+    // phase_correlation: an image against itself registers at zero shift (register/phase.rs:52-250)
+    {
+        auto pc = phase_correlation(client, in, in, PhaseCorrelation{4, true}).unwrap();
+        CHECK(pc.shift.size() == 2 && pc.shift[0] == 0.0 && pc.shift[1] == 0.0, "phase correlation shift");
+        CHECK(std::fabs(pc.phasediff) < 1e-5, "phase correlation phase %f", pc.phasediff);
+    }
     // error order
     Workspace ws(client);
     auto out86 = DeviceArray<float>::empty(client, {8, 6}).unwrap();
