@@ -52,6 +52,11 @@
  *                         f64 direct
  *   lucem_correlate_nd_into_f32
  *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
+ *   lucem_fourier_gaussian_into_f32 / lucem_fourier_box_into_f32 / lucem_fourier_shift_into_f32 /
+ *   lucem_butterworth_f32
+ *                         lucem_ops::filter::{fourier_gaussian_into, fourier_box_into,
+ *                         fourier_shift_into, butterworth} (crates/lucem-ops/src/filter/spectral.rs:
+ *                         20-212)
  *   lucem_phase_correlation_f32
  *                         lucem_ops::register::phase_correlation (crates/lucem-ops/src/register/phase.rs:
  *                         18-250), PhaseCorrelation as lucem_phase_correlation_settings; the result
@@ -408,6 +413,30 @@ lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const double* inp
                                          uint32_t in_ndim, const double* weights, const uint64_t* w_dims,
                                          uint32_t w_ndim, double* output, const uint64_t* out_dims,
                                          uint32_t out_ndim, const lucem_correlate_settings* settings);
+
+/* ---- Frequency-domain filters (filter/spectral.rs:20-212) -----------------------------------
+ * In place on a split spectrum (re, im), one axis at a time: fourier_gaussian_into multiplies by
+ * exp(-2 pi^2 sigma_d^2 f_d^2), fourier_box_into by sinc(size_d f_d) (1 where |pi size f| < 1e-12),
+ * fourier_shift_into by exp(-2 pi i shift_d f_d), with f = k / n for k <= n / 2, else (k - n) / n.
+ * The transfer values are computed in FP64 and rounded to FP32 as the reference's tables are.
+ * Checks in the reference's order: same shape (re, im), then 1 or ndim values
+ * (LUCEM_INVALID_PARAMETER), empty (OK). */
+lucem_status lucem_fourier_gaussian_into_f32(lucem_client* client, float* re, const uint64_t* re_dims,
+                                             uint32_t re_ndim, float* im, const uint64_t* im_dims, uint32_t im_ndim,
+                                             const double* sigmas, uint32_t num_sigmas);
+lucem_status lucem_fourier_box_into_f32(lucem_client* client, float* re, const uint64_t* re_dims, uint32_t re_ndim,
+                                        float* im, const uint64_t* im_dims, uint32_t im_ndim, const double* sizes,
+                                        uint32_t num_sizes);
+lucem_status lucem_fourier_shift_into_f32(lucem_client* client, float* re, const uint64_t* re_dims, uint32_t re_ndim,
+                                          float* im, const uint64_t* im_dims, uint32_t im_ndim, const double* shifts,
+                                          uint32_t num_shifts);
+/* butterworth::<f32> (spectral.rs:166-212) with a caller-allocated output of the input's shape:
+ * the forward FFT, the radial weight 1 / (1 + |f / cutoff|^(2 order)) (or 1 minus it for the
+ * high-pass), the inverse FFT's real part. cutoff_ratio outside (0, 0.5] is
+ * LUCEM_INVALID_PARAMETER (checked first, as the reference); 1-D to 4-D arrays. */
+lucem_status lucem_butterworth_f32(lucem_client* client, const float* input, const uint64_t* in_dims, uint32_t in_ndim,
+                                   float* output, const uint64_t* out_dims, uint32_t out_ndim, double cutoff_ratio,
+                                   int32_t high_pass, double order, lucem_workspace* ws);
 
 /* ---- Phase correlation (register/phase.rs:18-250) --------------------------------------------
  * Settings as PhaseCorrelation (defaults: upsample_factor 1, normalize_phase 1). */

@@ -15,6 +15,8 @@ Public surface (mirrors the reference's names):
 - :class:`NoiseModel`, :func:`add_noise`          intensity/noise.rs:17-143
 - :class:`Correlate2d`, :func:`correlate_2d_into`, :func:`correlate_2d`
                                                    filter/convolve.rs:113-177
+- :func:`fourier_gaussian_into`, :func:`fourier_box_into`, :func:`fourier_shift_into`,
+  :func:`butterworth`                              filter/spectral.rs:20-212
 - :class:`PhaseCorrelation`, :func:`phase_correlation`
                                                    register/phase.rs:18-250
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
@@ -39,5 +41,15 @@ from .wiener import (  # noqa: F401
 from . import fft  # noqa: F401,E402
 from .noise import NoiseModel, add_noise, add_noise_into  # noqa: F401,E402
 from . import tiled  # noqa: F401,E402
-from .filter import Correlate2d, correlate_2d, correlate_2d_into, correlate_nd, correlate_nd_into  # noqa: F401,E402
+from .filter import (  # noqa: F401,E402
+    Correlate2d,
+    butterworth,
+    correlate_2d,
+    correlate_2d_into,
+    correlate_nd,
+    correlate_nd_into,
+    fourier_box_into,
+    fourier_gaussian_into,
+    fourier_shift_into,
+)
 from .register import PhaseCorrelation, PhaseCorrelationResult, phase_correlation  # noqa: F401,E402

@@ -53,6 +53,8 @@ EXPORTED = [
     "lucem_correlate_settings_default", "lucem_correlate_2d_into_f32", "lucem_correlate_nd_into_f32",
     "lucem_correlate_2d_into_f64", "lucem_render_ellipses_f32", "lucem_render_spheres_f32",
     "lucem_phase_correlation_settings_default", "lucem_phase_correlation_f32",
+    "lucem_fourier_gaussian_into_f32", "lucem_fourier_box_into_f32", "lucem_fourier_shift_into_f32",
+    "lucem_butterworth_f32",
 ]
 
 
@@ -193,6 +195,13 @@ def lib() -> ctypes.CDLL:
             L.lucem_phase_correlation_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                       ctypes.POINTER(PhaseCorrelationSettings),
                                                       ctypes.POINTER(PhaseCorrelationResult), _vp]
+            spec = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                    ctypes.c_uint32]
+            L.lucem_fourier_gaussian_into_f32.argtypes = spec
+            L.lucem_fourier_box_into_f32.argtypes = spec
+            L.lucem_fourier_shift_into_f32.argtypes = spec
+            L.lucem_butterworth_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
+                                                ctypes.c_double, ctypes.c_int32, ctypes.c_double, _vp]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

@@ -168,6 +168,14 @@ int launch_render_ellipses(float* out, int H, int W, const lucem_ellipse* obj, i
 int launch_render_spheres(float* out, int D, int H, int W, const lucem_sphere* obj, int64_t count, double background,
                           void* stream);
 
+// Spectral filters (spectral.hip): per-axis transfer of the signed frequency on [outer, len,
+// inner] (kind 0 Gaussian, 1 box sinc, 2 phase shift; parameter p), and the N-D Butterworth
+// weight on a right-aligned 4-D shape.
+int launch_spectral_axis(float* re, float* im, int64_t total, int64_t len, int64_t inner, int kind, double p,
+                         void* stream);
+int launch_butterworth_weight(float* re, float* im, const int64_t dims4[4], double cutoff, double order, int high_pass,
+                              void* stream);
+
 // Phase correlation (phase.hip): workgroups of the reductions; A * conj(B) (normalised when
 // asked); per-workgroup largest re^2 + im^2 and its first index; per-workgroup FP64 sums of a
 // spectrum against plane waves exp(+2 pi i f(k) . s) at C positions (3 doubles each, z y x);

@@ -6,6 +6,8 @@ Reference (neuro-kitchen/lucem):
 - ``correlate_2d_into``               crates/lucem-ops/src/filter/convolve.rs:126-166
 - ``correlate_2d`` (allocating form)  crates/lucem-ops/src/filter/convolve.rs:169-177
 - ``check::origin``                   crates/lucem-ops/src/core/check.rs:59-72
+- ``fourier_gaussian_into``, ``fourier_box_into``, ``fourier_shift_into``, ``butterworth``
+                                      crates/lucem-ops/src/filter/spectral.rs:20-212
 
 ``out[p] = sum_t w[t] * P(in)[p - (k // 2 + origin) + t]`` over the window ``t``, ``P`` the
 boundary mode (``cval`` outside the array in Constant mode). float32 tensors run as an FFT
@@ -100,4 +102,51 @@ def correlate_nd(input, weights, settings: Correlate2d | None = None, client: N.
     x = N.as_tensor(input)
     out = torch.empty_like(x)
     correlate_nd_into(x, weights, out, settings, None, client)
+    return out
+
+
+def _spectral(fn, re, im, values, client):
+    r, i = N.as_tensor(re), N.as_tensor(im)
+    if client is None:
+        client = N.default_client(r.device.index if r.is_cuda else None)
+    vals = [float(v) for v in (values if hasattr(values, "__len__") else [values])]
+    arr = (ctypes.c_double * max(len(vals), 1))(*vals)
+    rp = N.device_ptr(r, "re") if r.numel() else 0
+    ip = N.device_ptr(i, "im") if i.numel() else 0
+    N.check(fn(client.handle, N._vp(rp or None), N.dims(r.shape), r.dim(), N._vp(ip or None), N.dims(i.shape), i.dim(),
+               arr, len(vals)))
+
+
+def fourier_gaussian_into(re, im, sigmas, client: N.Client | None = None) -> None:
+    """Multiplies the spectrum (re, im) in place by exp(-2 pi^2 sigma_d^2 f_d^2) along each axis
+    (spectral.rs:20-62); ``sigmas``: one value or one per axis."""
+    _spectral(N.lib().lucem_fourier_gaussian_into_f32, re, im, sigmas, client)
+
+
+def fourier_box_into(re, im, sizes, client: N.Client | None = None) -> None:
+    """Multiplies the spectrum in place by sinc(size_d f_d) along each axis (spectral.rs:64-108)."""
+    _spectral(N.lib().lucem_fourier_box_into_f32, re, im, sizes, client)
+
+
+def fourier_shift_into(re, im, shifts, client: N.Client | None = None) -> None:
+    """Multiplies the spectrum in place by exp(-2 pi i shift_d f_d) along each axis
+    (spectral.rs:110-165): the image moves by ``shifts`` pixels."""
+    _spectral(N.lib().lucem_fourier_shift_into_f32, re, im, shifts, client)
+
+
+def butterworth(input, cutoff_ratio: float, high_pass: bool, order: float, ws: N.Workspace | None = None,
+                client: N.Client | None = None):
+    """Butterworth low- or high-pass filter with normalised cutoff in (0, 0.5] (spectral.rs:
+    166-212): returns a new float32 device tensor."""
+    import torch
+
+    x = N.as_tensor(input)
+    if client is None:
+        client = N.default_client(x.device.index if x.is_cuda else None)
+    out = torch.empty_like(x)
+    xp = N.device_ptr(x, "input") if x.numel() else 0
+    op = N.device_ptr(out, "output") if out.numel() else 0
+    N.check(N.lib().lucem_butterworth_f32(client.handle, N._vp(xp or None), N.dims(x.shape), x.dim(),
+                                          N._vp(op or None), N.dims(out.shape), out.dim(), float(cutoff_ratio),
+                                          1 if high_pass else 0, float(order), ws.handle if ws is not None else None))
     return out

@@ -27,6 +27,8 @@ Functions and the reference code each one restates (paths relative to the refere
                               restore/deconvolve.rs:357-509 in FP64 over the full spectrum
 - :func:`phase_correlation64` crates/lucem-ops/src/register/phase.rs:52-250 in FP64, numpy's FFT
                               in place of the reference's DFT
+- :func:`spectral_axes64`, :func:`butterworth64`
+                              crates/lucem-ops/src/filter/spectral.rs:20-212 in FP64
 
 Parity pins (see tests/test_oracle.py): pad_index against the reference's own host restatement
 (crates/lucem-ops/src/test_support.rs:21-45, restated in the test) and numpy.pad; correlation
@@ -319,6 +321,41 @@ def phase_correlation64(reference: np.ndarray, moving: np.ndarray, upsample_fact
         amp2 = (un.real ** 2 + un.imag ** 2) * n_total * n_total
         error = float(np.sqrt(1.0 - min(max(amp2 / denom, 0.0), 1.0)))
     return shift, error, phasediff
+
+
+def spectral_axes64(re: np.ndarray, im: np.ndarray, kind: str, values) -> tuple:
+    """fourier_gaussian_into / fourier_box_into / fourier_shift_into (filter/spectral.rs:20-165)
+    in FP64: per axis, the spectrum times exp(-2 pi^2 p^2 f^2) ("gaussian"), sinc(p f) ("box",
+    1 where |pi p f| < 1e-12) or exp(-2 pi i p f) ("shift"), f the signed frequency."""
+    z = np.asarray(re, dtype=np.float64) + 1j * np.asarray(im, dtype=np.float64)
+    vals = list(values) if hasattr(values, "__len__") else [values]
+    if len(vals) == 1:
+        vals = vals * z.ndim
+    for ax, p in enumerate(vals):
+        n = z.shape[ax]
+        f = _signed_freqs(n)
+        if kind == "gaussian":
+            w = np.exp(-2.0 * np.pi ** 2 * p * p * f * f)
+        elif kind == "box":
+            x = np.pi * p * f
+            w = np.where(np.abs(x) < 1e-12, 1.0, np.sin(x) / np.where(np.abs(x) < 1e-12, 1.0, x))
+        else:
+            w = np.exp(-2j * np.pi * p * f)
+        shp = [1] * z.ndim
+        shp[ax] = n
+        z = z * w.reshape(shp)
+    return z.real.copy(), z.imag.copy()
+
+
+def butterworth64(x: np.ndarray, cutoff_ratio: float, high_pass: bool, order: float) -> np.ndarray:
+    """butterworth (filter/spectral.rs:166-212) in FP64: the real part of the inverse transform of
+    the spectrum times 1 / (1 + |f / cutoff|^(2 order)) (or 1 minus it)."""
+    x = np.asarray(x, dtype=np.float64)
+    grids = np.meshgrid(*[_signed_freqs(n) for n in x.shape], indexing="ij")
+    r2 = sum((g / cutoff_ratio) ** 2 for g in grids)
+    lp = 1.0 / (1.0 + np.sqrt(r2) ** (2.0 * order))
+    w = 1.0 - lp if high_pass else lp
+    return np.fft.ifftn(np.fft.fftn(x) * w).real
 
 
 def pcg_hash32(x: np.ndarray) -> np.ndarray:

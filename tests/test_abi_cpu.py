@@ -244,6 +244,36 @@ def test_phase_correlation_check_order_without_device(L):
     assert call((5,), (5,), up=3) == 11
 
 
+def test_spectral_check_order_without_device(L):
+    """spectral.rs: same_shape(re, im), then 1 or ndim values, then empty; butterworth checks its
+    cutoff first."""
+    from lucem_amd import _native as N
+
+    vals = (ctypes.c_double * 3)(1.0, 2.0, 3.0)
+
+    def call(fn, a, b, nv):
+        return fn(None, N._vp(0x10000), N.dims(a), len(a), N._vp(0x20000000), N.dims(b), len(b), vals, nv)
+
+    for fn in (L.lucem_fourier_gaussian_into_f32, L.lucem_fourier_box_into_f32, L.lucem_fourier_shift_into_f32):
+        assert call(fn, (4, 5), (4, 6), 3) == 1
+        assert call(fn, (4, 5), (4, 5), 3) == 5
+        assert b"expected 1 or 2 values, got 3" in L.lucem_last_error()
+        assert call(fn, (0, 5), (0, 5), 1) == 0
+        assert call(fn, (4, 5), (4, 5), 2) == 11
+        assert call(fn, (4, 5), (4, 5), 1) == 11
+
+    def bw(a, b, c):
+        return L.lucem_butterworth_f32(None, N._vp(0x10000), N.dims(a), len(a), N._vp(0x20000000), N.dims(b), len(b),
+                                       c, 0, 2.0, None)
+
+    assert bw((4, 5), (4, 6), 0.0) == 5
+    assert b"cutoff_ratio must be in (0.0, 0.5], got 0" in L.lucem_last_error()
+    assert bw((4, 5), (4, 6), 0.2) == 1
+    assert bw((0, 5), (0, 5), 0.2) == 0
+    assert bw((2, 2, 2, 2, 2), (2, 2, 2, 2, 2), 0.2) == 9
+    assert bw((4, 5), (4, 5), 0.5) == 11
+
+
 def test_wiener_check_order_without_device(L):
     from lucem_amd import _native as N
 

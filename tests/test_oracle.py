@@ -519,3 +519,39 @@ def test_phase_correlation_subpixel_shift_on_the_grid(oracle, shape, shift, u):
     got, err, ph = oracle.phase_correlation64(x, _fourier_shift(x, shift), u, True)
     np.testing.assert_allclose(got, [-s for s in shift], atol=1e-12)
     assert abs(ph) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["gaussian", "box", "shift"])
+def test_spectral_filters_match_scipy_fourier(oracle, kind):
+    """fourier_gaussian / fourier_uniform / fourier_shift of scipy.ndimage (complex input, the
+    same transfer functions) on odd sizes, where the Nyquist sign convention does not arise."""
+    rng = np.random.default_rng(3)
+    z = rng.normal(size=(15, 21)) + 1j * rng.normal(size=(15, 21))
+    vals = {"gaussian": [1.5, 2.5], "box": [3.0, 5.0], "shift": [2.25, -1.5]}[kind]
+    fn = {"gaussian": ndi.fourier_gaussian, "box": ndi.fourier_uniform, "shift": ndi.fourier_shift}[kind]
+    want = fn(z, vals)
+    gr, gi = oracle.spectral_axes64(z.real, z.imag, kind, vals)
+    np.testing.assert_allclose(gr + 1j * gi, want, rtol=1e-12, atol=1e-12)
+
+
+def test_spectral_shift_by_integers_is_a_circular_roll(oracle):
+    x = np.random.default_rng(4).normal(size=(16, 20))
+    X = np.fft.fft2(x)
+    gr, gi = oracle.spectral_axes64(X.real, X.imag, "shift", [3, -5])
+    np.testing.assert_allclose(np.fft.ifft2(gr + 1j * gi).real, np.roll(x, (3, -5), axis=(0, 1)), atol=1e-12)
+
+
+def test_butterworth_closed_forms(oracle):
+    """A cosine at frequency f0 is scaled by exactly 1 / (1 + (f0 / c)^(2 n)); the low and high
+    passes add up to the input; the low pass keeps the mean."""
+    n = 64
+    t = np.arange(n)
+    for f0, c, order in [(5 / 64, 0.1, 2.0), (12 / 64, 0.25, 3.5), (20 / 64, 0.2, 1.0)]:
+        x = np.cos(2 * np.pi * f0 * t)[:, None] * np.ones((1, 8))
+        got = oracle.butterworth64(x, c, False, order)
+        np.testing.assert_allclose(got, x / (1.0 + (f0 / c) ** (2 * order)), atol=1e-12)
+    x = np.random.default_rng(5).normal(size=(30, 41))
+    lo = oracle.butterworth64(x, 0.2, False, 2.0)
+    hi = oracle.butterworth64(x, 0.2, True, 2.0)
+    np.testing.assert_allclose(lo + hi, x, atol=1e-12)
+    assert abs(lo.mean() - x.mean()) < 1e-12
