@@ -52,6 +52,7 @@
  *                         f64 direct
  *   lucem_correlate_nd_into_f32
  *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
+ *   lucem_gabor_2d_f32    lucem_ops::filter::gabor_2d (filter/spectral.rs:216-268), on the correlation
  *   lucem_fourier_gaussian_into_f32 / lucem_fourier_box_into_f32 / lucem_fourier_shift_into_f32 /
  *   lucem_butterworth_f32
  *                         lucem_ops::filter::{fourier_gaussian_into, fourier_box_into,
@@ -487,6 +488,22 @@ lucem_status lucem_render_ellipses_f32(lucem_client* client, float* out, const u
                                        const lucem_ellipse* objects, uint64_t count, double background);
 lucem_status lucem_render_spheres_f32(lucem_client* client, float* out, const uint64_t* dims, uint32_t ndim,
                                       const lucem_sphere* objects, uint64_t count, double background);
+
+/* gabor_2d::<f32> (filter/spectral.rs:216-268): the complex Gabor kernel of gabor_kernel_2d
+ * (FP64 on the host, rounded to f32) correlated with the 2-D input by lucem_correlate_2d_into_f32,
+ * real part into out_re and imaginary part into out_im (both caller-allocated, the input's
+ * shape). Defaults (lucem_gabor_settings_default): frequency 0.1, theta 0, sigma 1 and 1, n_stds 3,
+ * offset 0, Reflect, cval 0. The correlation's checks apply. Blocks until the kernel is uploaded. */
+typedef struct lucem_gabor_settings {
+    double frequency, theta, sigma_x, sigma_y, n_stds, offset;
+    uint32_t mode;
+    uint32_t reserved;
+    double cval;
+} lucem_gabor_settings;
+void lucem_gabor_settings_default(lucem_gabor_settings* out);
+lucem_status lucem_gabor_2d_f32(lucem_client* client, const float* input, const uint64_t* dims, uint32_t ndim,
+                                float* out_re, float* out_im, const lucem_gabor_settings* settings,
+                                lucem_workspace* ws);
 
 /* Batched complex-to-complex FFTs of `nlines` contiguous lines of length L (interleaved re/im
  * float pairs, device pointers; src == dst allowed). Forward is exp(-2 pi i j k / L), inverse is

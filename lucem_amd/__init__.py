@@ -16,7 +16,7 @@ Public surface (mirrors the reference's names):
 - :class:`Correlate2d`, :func:`correlate_2d_into`, :func:`correlate_2d`
                                                    filter/convolve.rs:113-177
 - :func:`fourier_gaussian_into`, :func:`fourier_box_into`, :func:`fourier_shift_into`,
-  :func:`butterworth`                              filter/spectral.rs:20-212
+  :func:`butterworth`, :func:`gabor_2d`            filter/spectral.rs:20-268
 - :class:`PhaseCorrelation`, :func:`phase_correlation`
                                                    register/phase.rs:18-250
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
@@ -51,5 +51,6 @@ from .filter import (  # noqa: F401,E402
     fourier_box_into,
     fourier_gaussian_into,
     fourier_shift_into,
+    gabor_2d,
 )
 from .register import PhaseCorrelation, PhaseCorrelationResult, phase_correlation  # noqa: F401,E402

@@ -54,7 +54,7 @@ EXPORTED = [
     "lucem_correlate_2d_into_f64", "lucem_render_ellipses_f32", "lucem_render_spheres_f32",
     "lucem_phase_correlation_settings_default", "lucem_phase_correlation_f32",
     "lucem_fourier_gaussian_into_f32", "lucem_fourier_box_into_f32", "lucem_fourier_shift_into_f32",
-    "lucem_butterworth_f32",
+    "lucem_butterworth_f32", "lucem_gabor_settings_default", "lucem_gabor_2d_f32",
 ]
 
 
@@ -121,6 +121,11 @@ class Ellipse(ctypes.Structure):
 
 class Sphere(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("cz", "cy", "cx", "radius", "radius_sq", "value")]
+
+
+class GaborSettings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("frequency", "theta", "sigma_x", "sigma_y", "n_stds", "offset")] + [
+        ("mode", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("cval", ctypes.c_double)]
 
 
 class NoiseModelC(ctypes.Structure):
@@ -202,6 +207,10 @@ def lib() -> ctypes.CDLL:
             L.lucem_fourier_shift_into_f32.argtypes = spec
             L.lucem_butterworth_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                 ctypes.c_double, ctypes.c_int32, ctypes.c_double, _vp]
+            L.lucem_gabor_settings_default.argtypes = [ctypes.POINTER(GaborSettings)]
+            L.lucem_gabor_settings_default.restype = None
+            L.lucem_gabor_2d_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _vp, ctypes.POINTER(GaborSettings),
+                                             _vp]
             L.lucem_client_profile.argtypes = [_vp, ctypes.c_int32]
             L.lucem_client_profile_read.argtypes = [_vp, ctypes.POINTER(KernelStat), ctypes.c_uint32,
                                                     ctypes.POINTER(ctypes.c_uint32)]

@@ -12,8 +12,10 @@
 // reference's order (rl_direct.hip).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/lucem_hip.h"
 #include "lucem_internal_host.hpp"
@@ -240,4 +242,85 @@ extern "C" lucem_status lucem_correlate_2d_into_f64(lucem_client* client, const 
     return check(launch_correlate2d_direct(input, weights, output, g.n[1], g.n[2], kh, kw, kh / 2 + g.origin[1],
                                            kw / 2 + g.origin[2], g.mode, settings->cval, client->stream),
                  "correlate (f64)");
+}
+
+// gabor_kernel_2d (filter/spectral.rs:216-246): the complex Gabor kernel in FP64 on the host,
+// rounded to f32 as the reference's upload casts it.
+static void gabor_kernel(const lucem_gabor_settings& s, int* kh, int* kw, std::vector<float>* re,
+                         std::vector<float>* im) {
+    const double pi = 3.14159265358979323846264338327950288;
+    const double ct = std::cos(s.theta), st = std::sin(s.theta);
+    // fmax drops a NaN as the reference's f64::max does
+    const long rx = (long)std::fmax(std::ceil(s.n_stds * std::hypot(s.sigma_x * ct, s.sigma_y * st)), 1.0);
+    const long ry = (long)std::fmax(std::ceil(s.n_stds * std::hypot(s.sigma_x * st, s.sigma_y * ct)), 1.0);
+    *kh = (int)(2 * ry + 1);
+    *kw = (int)(2 * rx + 1);
+    const double norm = 1.0 / (2.0 * pi * s.sigma_x * s.sigma_y);
+    re->clear();
+    im->clear();
+    for (long y = -ry; y <= ry; y++)
+        for (long x = -rx; x <= rx; x++) {
+            const double xr = (double)x * ct + (double)y * st;
+            const double yr = -(double)x * st + (double)y * ct;
+            const double env =
+                norm * std::exp(-0.5 * (xr * xr / (s.sigma_x * s.sigma_x) + yr * yr / (s.sigma_y * s.sigma_y)));
+            const double phase = 2.0 * pi * s.frequency * xr + s.offset;
+            re->push_back((float)(env * std::cos(phase)));
+            im->push_back((float)(env * std::sin(phase)));
+        }
+}
+
+extern "C" void lucem_gabor_settings_default(lucem_gabor_settings* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof *out);
+    out->frequency = 0.1;
+    out->sigma_x = 1.0;
+    out->sigma_y = 1.0;
+    out->n_stds = 3.0;
+    out->mode = LUCEM_PAD_REFLECT;
+}
+
+extern "C" lucem_status lucem_gabor_2d_f32(lucem_client* client, const float* input, const uint64_t* dims,
+                                           uint32_t ndim, float* out_re, float* out_im,
+                                           const lucem_gabor_settings* settings, lucem_workspace* ws) {
+    if (!settings) return fail(LUCEM_INVALID_HANDLE, "settings is NULL");
+    if (ndim && !dims) return fail(LUCEM_INVALID_HANDLE, "dims pointer is NULL");
+    {
+        // a kernel radius beyond 2^20 is refused (the reference would try to allocate it)
+        const double r = std::fabs(settings->n_stds) * (std::fabs(settings->sigma_x) + std::fabs(settings->sigma_y));
+        if (r > 1048576.0)
+            return fail(LUCEM_INVALID_PARAMETER, "invalid parameter `n_stds`: the Gabor kernel would be too large");
+    }
+    int kh = 0, kw = 0;
+    std::vector<float> kre, kim;
+    gabor_kernel(*settings, &kh, &kw, &kre, &kim);
+    const uint64_t kd[2] = {(uint64_t)kh, (uint64_t)kw};
+    lucem_correlate_settings cs;
+    lucem_correlate_settings_default(&cs);
+    cs.mode = settings->mode;
+    cs.cval = settings->cval;
+    // spectral.rs:262-266: two correlate_2d calls with the kernel's real and imaginary parts (the
+    // checks are theirs); the kernels are uploaded once the arguments pass
+    bool empty = false;
+    Geometry g;
+    lucem_status st = validate_correlate(input, dims, ndim, kd, 2, out_re, dims, ndim, &cs, false, sizeof(float), false,
+                                         &empty, &g);
+    if (st == LUCEM_OK && !empty)
+        st = validate_correlate(input, dims, ndim, kd, 2, out_im, dims, ndim, &cs, false, sizeof(float), false, &empty,
+                                &g);
+    if (st != LUCEM_OK || empty) return st;
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    if (!ws) ws = client->internal_ws.get();
+    if (ws->client != client) return fail(LUCEM_INVALID_HANDLE, "workspace belongs to another client");
+    hipError_t e = hipSetDevice(client->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    float *dre, *dim;
+    const size_t kb = sizeof(float) * kre.size();
+    if ((st = ws_slot(ws, "gabor_re", kb, (void**)&dre)) || (st = ws_slot(ws, "gabor_im", kb, (void**)&dim))) return st;
+    if ((e = hipMemcpyAsync(dre, kre.data(), kb, hipMemcpyHostToDevice, client->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(dim, kim.data(), kb, hipMemcpyHostToDevice, client->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(client->stream)) != hipSuccess)
+        return hip_fail(e, "gabor kernel upload");
+    if ((st = correlate_entry(client, input, dims, ndim, dre, kd, 2, out_re, dims, ndim, &cs, ws, false))) return st;
+    return correlate_entry(client, input, dims, ndim, dim, kd, 2, out_im, dims, ndim, &cs, ws, false);
 }

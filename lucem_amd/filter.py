@@ -8,6 +8,7 @@ Reference (neuro-kitchen/lucem):
 - ``check::origin``                   crates/lucem-ops/src/core/check.rs:59-72
 - ``fourier_gaussian_into``, ``fourier_box_into``, ``fourier_shift_into``, ``butterworth``
                                       crates/lucem-ops/src/filter/spectral.rs:20-212
+- ``gabor_2d`` (``gabor_kernel_2d``)  crates/lucem-ops/src/filter/spectral.rs:216-268
 
 ``out[p] = sum_t w[t] * P(in)[p - (k // 2 + origin) + t]`` over the window ``t``, ``P`` the
 boundary mode (``cval`` outside the array in Constant mode). float32 tensors run as an FFT
@@ -150,3 +151,27 @@ def butterworth(input, cutoff_ratio: float, high_pass: bool, order: float, ws: N
                                           N._vp(op or None), N.dims(out.shape), out.dim(), float(cutoff_ratio),
                                           1 if high_pass else 0, float(order), ws.handle if ws is not None else None))
     return out
+
+
+def gabor_2d(input, frequency: float, theta: float, sigma_x: float, sigma_y: float, n_stds: float = 3.0,
+             offset: float = 0.0, mode: PadMode | str = PadMode.Reflect, cval: float = 0.0,
+             ws: N.Workspace | None = None, client: N.Client | None = None):
+    """The 2-D Gabor filter (spectral.rs:249-268): (real response, imaginary response), the
+    correlations of ``input`` with the two parts of gabor_kernel_2d's kernel."""
+    import torch
+
+    x = N.as_tensor(input)
+    if client is None:
+        client = N.default_client(x.device.index if x.is_cuda else None)
+    s = N.GaborSettings()
+    N.lib().lucem_gabor_settings_default(ctypes.byref(s))
+    s.frequency, s.theta, s.sigma_x, s.sigma_y = float(frequency), float(theta), float(sigma_x), float(sigma_y)
+    s.n_stds, s.offset, s.cval = float(n_stds), float(offset), float(cval)
+    s.mode = int(PadMode.parse(mode)) if isinstance(mode, str) else int(PadMode(mode))
+    ore, oim = torch.empty_like(x), torch.empty_like(x)
+    xp = N.device_ptr(x, "input") if x.numel() else 0
+    N.check(N.lib().lucem_gabor_2d_f32(client.handle, N._vp(xp or None), N.dims(x.shape), x.dim(),
+                                       N._vp(ore.data_ptr() if ore.numel() else None),
+                                       N._vp(oim.data_ptr() if oim.numel() else None), ctypes.byref(s),
+                                       ws.handle if ws is not None else None))
+    return ore, oim

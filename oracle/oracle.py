@@ -27,8 +27,8 @@ Functions and the reference code each one restates (paths relative to the refere
                               restore/deconvolve.rs:357-509 in FP64 over the full spectrum
 - :func:`phase_correlation64` crates/lucem-ops/src/register/phase.rs:52-250 in FP64, numpy's FFT
                               in place of the reference's DFT
-- :func:`spectral_axes64`, :func:`butterworth64`
-                              crates/lucem-ops/src/filter/spectral.rs:20-212 in FP64
+- :func:`spectral_axes64`, :func:`butterworth64`, :func:`gabor_kernel64`
+                              crates/lucem-ops/src/filter/spectral.rs:20-246 in FP64
 
 Parity pins (see tests/test_oracle.py): pad_index against the reference's own host restatement
 (crates/lucem-ops/src/test_support.rs:21-45, restated in the test) and numpy.pad; correlation
@@ -356,6 +356,23 @@ def butterworth64(x: np.ndarray, cutoff_ratio: float, high_pass: bool, order: fl
     lp = 1.0 / (1.0 + np.sqrt(r2) ** (2.0 * order))
     w = 1.0 - lp if high_pass else lp
     return np.fft.ifftn(np.fft.fftn(x) * w).real
+
+
+def gabor_kernel64(frequency: float, theta: float, sigma_x: float, sigma_y: float, n_stds: float = 3.0,
+                   offset: float = 0.0) -> tuple:
+    """gabor_kernel_2d (filter/spectral.rs:216-246) in FP64: (real, imaginary) kernels of shape
+    (2 ry + 1, 2 rx + 1)."""
+    ct, st = np.cos(theta), np.sin(theta)
+    rx = int(max(np.ceil(n_stds * np.hypot(sigma_x * ct, sigma_y * st)), 1.0))
+    ry = int(max(np.ceil(n_stds * np.hypot(sigma_x * st, sigma_y * ct)), 1.0))
+    y, x = np.meshgrid(np.arange(-ry, ry + 1, dtype=np.float64), np.arange(-rx, rx + 1, dtype=np.float64),
+                       indexing="ij")
+    xr = x * ct + y * st
+    yr = -x * st + y * ct
+    norm = 1.0 / (2.0 * np.pi * sigma_x * sigma_y)
+    env = norm * np.exp(-0.5 * (xr * xr / (sigma_x * sigma_x) + yr * yr / (sigma_y * sigma_y)))
+    phase = 2.0 * np.pi * frequency * xr + offset
+    return env * np.cos(phase), env * np.sin(phase)
 
 
 def pcg_hash32(x: np.ndarray) -> np.ndarray:

@@ -81,3 +81,23 @@ def test_spectral_errors(gpu):
     e0 = torch.empty((0, 4), device="cuda")
     LA.fourier_shift_into(e0, e0.clone(), [1.0])
     assert LA.butterworth(e0, 0.2, False, 2.0).shape == (0, 4)
+
+
+@pytest.mark.parametrize("params", [
+    (0.1, 0.0, 1.0, 1.0, 3.0, 0.0, 0, 0.0),
+    (0.25, 0.6, 2.0, 1.5, 3.0, 0.4, 1, 0.5),
+    (0.05, 1.2, 4.0, 2.0, 2.5, 0.0, 3, 0.0),
+])
+def test_gabor_matches_oracle(gpu, oracle, params):
+    """gabor_2d (spectral.rs:249-268): the two correlations of the kernel's parts."""
+    import lucem_amd as LA
+
+    f, th, sx, sy, ns, off, mode, cval = params
+    x = np.random.default_rng(int(f * 100)).uniform(size=(96, 128)).astype(np.float32)
+    gr, gi = LA.gabor_2d(_dev(x), f, th, sx, sy, ns, off, mode, cval)
+    kr, ki = oracle.gabor_kernel64(f, th, sx, sy, ns, off)
+    wr = oracle.correlate(x.astype(np.float64), kr.astype(np.float32).astype(np.float64), mode, cval)
+    wi = oracle.correlate(x.astype(np.float64), ki.astype(np.float32).astype(np.float64), mode, cval)
+    e = max(_rel(gr.cpu().numpy(), wr), _rel(gi.cpu().numpy(), wi))
+    print(f"gabor {params}: rel-L2 {e:.2e}")
+    assert e < 1e-5

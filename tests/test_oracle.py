@@ -555,3 +555,23 @@ def test_butterworth_closed_forms(oracle):
     hi = oracle.butterworth64(x, 0.2, True, 2.0)
     np.testing.assert_allclose(lo + hi, x, atol=1e-12)
     assert abs(lo.mean() - x.mean()) < 1e-12
+
+
+def test_gabor_kernel_closed_forms(oracle):
+    """gabor_kernel_2d: at frequency 0 and offset 0 the real part is the normalised sampled
+    Gaussian (sum near 1 for 4 sigma) and the imaginary part is 0; with offset 0 the real part is
+    even and the imaginary part odd; the centre is norm * (cos offset, sin offset); theta = pi/2
+    swaps the axes."""
+    re, im = oracle.gabor_kernel64(0.0, 0.0, 2.0, 3.0, 4.0, 0.0)
+    assert re.shape == (25, 17) and np.all(im == 0.0)
+    assert abs(re.sum() - 1.0) < 1e-3
+    re, im = oracle.gabor_kernel64(0.15, 0.7, 2.0, 1.5, 3.0, 0.0)
+    np.testing.assert_allclose(re, re[::-1, ::-1], atol=1e-15)
+    np.testing.assert_allclose(im, -im[::-1, ::-1], atol=1e-15)
+    re, im = oracle.gabor_kernel64(0.2, 0.3, 1.0, 2.0, 3.0, 0.9)
+    c = (re.shape[0] // 2, re.shape[1] // 2)
+    norm = 1.0 / (2.0 * np.pi * 2.0)
+    assert abs(re[c] - norm * np.cos(0.9)) < 1e-15 and abs(im[c] - norm * np.sin(0.9)) < 1e-15
+    a, _ = oracle.gabor_kernel64(0.0, 0.0, 1.0, 2.5, 3.0, 0.0)
+    b, _ = oracle.gabor_kernel64(0.0, np.pi / 2, 1.0, 2.5, 3.0, 0.0)
+    np.testing.assert_allclose(a, b.T, atol=1e-12)
