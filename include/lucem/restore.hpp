@@ -15,6 +15,8 @@
 //                                        crates/lucem-ops/src/filter/convolve.rs:113-177
 //   lucem::PhaseCorrelation, PhaseCorrelationResult, phase_correlation
 //                                        crates/lucem-ops/src/register/phase.rs:18-250
+//   lucem::fourier_gaussian_into, fourier_box_into, fourier_shift_into, butterworth, gabor_2d
+//                                        crates/lucem-ops/src/filter/spectral.rs:20-268
 //   lucem::Client, Workspace, DeviceArray crates/lucem-core/src/compute/target.rs:95-115,
 //                                        memory/workspace.rs:17-37, memory/array.rs:20-151
 //
@@ -426,6 +428,68 @@ inline OpsResult<PhaseCorrelationResult> phase_correlation(const Client& client,
     out.error = r.error;
     out.phasediff = r.phasediff;
     return out;
+}
+
+// fourier_{gaussian,box,shift}_into (filter/spectral.rs:20-165): in place on a split spectrum,
+// one value or one per axis.
+inline OpsResult<Unit> fourier_gaussian_into(const Client& client, const DeviceArray<float>& re,
+                                             const DeviceArray<float>& im, const std::vector<double>& sigmas) {
+    return status_result(lucem_fourier_gaussian_into_f32(client.handle(), re.data(), re.dims().data(),
+                                                         (uint32_t)re.dims().size(), im.data(), im.dims().data(),
+                                                         (uint32_t)im.dims().size(), sigmas.data(),
+                                                         (uint32_t)sigmas.size()));
+}
+inline OpsResult<Unit> fourier_box_into(const Client& client, const DeviceArray<float>& re,
+                                        const DeviceArray<float>& im, const std::vector<double>& sizes) {
+    return status_result(lucem_fourier_box_into_f32(client.handle(), re.data(), re.dims().data(),
+                                                    (uint32_t)re.dims().size(), im.data(), im.dims().data(),
+                                                    (uint32_t)im.dims().size(), sizes.data(), (uint32_t)sizes.size()));
+}
+inline OpsResult<Unit> fourier_shift_into(const Client& client, const DeviceArray<float>& re,
+                                          const DeviceArray<float>& im, const std::vector<double>& shifts) {
+    return status_result(lucem_fourier_shift_into_f32(client.handle(), re.data(), re.dims().data(),
+                                                      (uint32_t)re.dims().size(), im.data(), im.dims().data(),
+                                                      (uint32_t)im.dims().size(), shifts.data(),
+                                                      (uint32_t)shifts.size()));
+}
+
+// butterworth::<f32> (filter/spectral.rs:166-212): allocates the output.
+inline OpsResult<DeviceArray<float>> butterworth(const Client& client, const DeviceArray<float>& input,
+                                                 double cutoff_ratio, bool high_pass, double order) {
+    auto out = DeviceArray<float>::empty(client, input.dims());
+    if (!out.is_ok()) return out;
+    auto r = status_result(lucem_butterworth_f32(client.handle(), input.data(), input.dims().data(),
+                                                 (uint32_t)input.dims().size(), out.unwrap().data(),
+                                                 input.dims().data(), (uint32_t)input.dims().size(), cutoff_ratio,
+                                                 high_pass ? 1 : 0, order, nullptr));
+    if (!r.is_ok()) return OpsResult<DeviceArray<float>>(r.status(), r.message());
+    return out;
+}
+
+// gabor_2d::<f32> (filter/spectral.rs:249-268): (real response, imaginary response).
+inline OpsResult<std::pair<DeviceArray<float>, DeviceArray<float>>> gabor_2d(
+    const Client& client, const DeviceArray<float>& input, double frequency, double theta, double sigma_x,
+    double sigma_y, double n_stds, double offset, PadMode mode, double cval) {
+    using R = OpsResult<std::pair<DeviceArray<float>, DeviceArray<float>>>;
+    auto re = DeviceArray<float>::empty(client, input.dims());
+    if (!re.is_ok()) return R(re.status(), re.message());
+    auto im = DeviceArray<float>::empty(client, input.dims());
+    if (!im.is_ok()) return R(im.status(), im.message());
+    lucem_gabor_settings s;
+    lucem_gabor_settings_default(&s);
+    s.frequency = frequency;
+    s.theta = theta;
+    s.sigma_x = sigma_x;
+    s.sigma_y = sigma_y;
+    s.n_stds = n_stds;
+    s.offset = offset;
+    s.mode = (uint32_t)mode;
+    s.cval = cval;
+    auto r = status_result(lucem_gabor_2d_f32(client.handle(), input.data(), input.dims().data(),
+                                              (uint32_t)input.dims().size(), re.unwrap().data(), im.unwrap().data(),
+                                              &s, nullptr));
+    if (!r.is_ok()) return R(r.status(), r.message());
+    return std::make_pair(re.unwrap(), im.unwrap());
 }
 
 }  // namespace lucem

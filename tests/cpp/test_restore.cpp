@@ -117,6 +117,14 @@ int main() {
         CHECK(pc.shift.size() == 2 && pc.shift[0] == 0.0 && pc.shift[1] == 0.0, "phase correlation shift");
         CHECK(std::fabs(pc.phasediff) < 1e-5, "phase correlation phase %f", pc.phasediff);
     }
+    // butterworth: the low and high passes add up to the input (filter/spectral.rs:166-212)
+    {
+        auto lo = butterworth(client, in, 0.2, false, 2.0).unwrap().to_host(client);
+        auto hi = butterworth(client, in, 0.2, true, 2.0).unwrap().to_host(client);
+        for (int i = 0; i < 64; i++) CHECK(std::fabs(lo[i] + hi[i] - img[i]) < 1e-5, "butterworth sum %d", i);
+        auto gb = gabor_2d(client, in, 0.1, 0.0, 1.0, 1.0, 3.0, 0.0, PadMode::Reflect, 0.0);
+        CHECK(gb.is_ok(), "gabor_2d");
+    }
     // error order
     Workspace ws(client);
     auto out86 = DeviceArray<float>::empty(client, {8, 6}).unwrap();
