@@ -189,9 +189,15 @@ def correlate_fft64(x: np.ndarray, w: np.ndarray, mode: int = PAD_REFLECT) -> np
             q[tuple(sl)] = 0.0
     L = q.shape
     axes = list(range(x.ndim))
-    Q = np.fft.rfftn(q, L, axes=axes)
-    W = np.fft.rfftn(w, L, axes=axes)
-    c = np.fft.irfftn(Q * np.conj(W), L, axes=axes)
+    try:  # scipy's FFT runs on every core; numpy's gives the same FP64 result on one
+        import scipy.fft as F
+
+        kw = {"workers": -1}
+    except ImportError:
+        F, kw = np.fft, {}
+    Q = F.rfftn(q, L, axes=axes, **kw)
+    W = F.rfftn(w, L, axes=axes, **kw)
+    c = F.irfftn(Q * np.conj(W), L, axes=axes, **kw)
     return np.ascontiguousarray(c[tuple(slice(0, s) for s in x.shape)])
 
 
