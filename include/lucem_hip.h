@@ -394,11 +394,12 @@ void lucem_correlate_settings_default(lucem_correlate_settings* out);
  * == 2, weights ndim == 2, same shape (input, output), no overlap of input and output, then per
  * axis the origin (core/check.rs:59-72: a weights axis of size 0 is LUCEM_INVALID_WEIGHTS, an
  * origin outside -(k/2)..=(k-1)/2 LUCEM_INVALID_ORIGIN); an empty input returns LUCEM_OK with
- * nothing done. The sum runs as an FFT linear correlation (the passes of the RL path) on the
- * padded domain n + k - 1 per axis: FP32 with a rounding error of about 1e-7 of the output's
- * largest value, not the reference's sequential sum. A domain longer than LUCEM_MAX_FFT_LEN runs
- * the direct sum instead (2-D; 3-D returns LUCEM_UNSUPPORTED_SHAPE). Asynchronous on the client's
- * stream; ws may be NULL (the client's own). */
+ * nothing done. Windows of up to 169 taps (13 x 13) run the direct sum in the reference's order,
+ * the faster form there; larger windows run as an FFT linear correlation (the passes of the RL
+ * path) on the padded domain n + k - 1 per axis: FP32 with a rounding error of about 1e-7 of the
+ * output's largest value. A domain longer than LUCEM_MAX_FFT_LEN runs the direct sum too (2-D;
+ * 3-D returns LUCEM_UNSUPPORTED_SHAPE). Asynchronous on the client's stream; ws may be NULL (the
+ * client's own). */
 lucem_status lucem_correlate_2d_into_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
                                          uint32_t in_ndim, const float* weights, const uint64_t* w_dims,
                                          uint32_t w_ndim, float* output, const uint64_t* out_dims, uint32_t out_ndim,

@@ -166,3 +166,19 @@ def test_correlate_empty_and_errors(gpu):
     with pytest.raises(LA.LucemError) as ei:
         LA.correlate_2d(_dev(np.ones((2, 8, 8))), _dev(np.ones((3, 3, 3))))
     assert ei.value.kind == "INVALID_DIMENSION"
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_correlate_small_windows_through_the_fft_passes(gpu, oracle, mode, monkeypatch):
+    """Windows up to 13 x 13 take the direct sum by default; LUCEM_CORRELATE_DIRECT_MAX=0 sends
+    them through the FFT passes, which must agree as well."""
+    monkeypatch.setenv("LUCEM_CORRELATE_DIRECT_MAX", "0")
+    for shape, kshape, origin in [((64, 64), (5, 5), (0, 0)), ((37, 53), (4, 6), (-2, 2)), ((3, 3), (3, 3), (1, -1))]:
+        rng = np.random.default_rng(shape[0] + kshape[0] + mode)
+        x = rng.uniform(-1.0, 1.0, shape).astype(np.float32)
+        w = rng.standard_normal(kshape).astype(np.float32)
+        cval = 1.5 if mode == 1 else 0.0
+        got = _corr(x, w, mode, cval, origin)
+        want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+        assert_close(got, want, 1e-5)
+        assert oracle.rel_l2(got, want) < 1e-6
