@@ -190,3 +190,34 @@ def test_grad_long_axes_match_fp64_oracle(gpu, oracle, shape, kshape, mode):
     e = _rel(gp, gp_ref)
     print(f"{shape}: dL/dpsf rel-L2 vs FP64 {e:.2e}")
     assert e < GRAD_TOL
+
+
+def test_grad_plan_replays_match_every_time(gpu):
+    """One plan's forward and backward replayed four times with eager work between: losses and
+    gradients equal the first replay's bit for bit."""
+    import torch
+
+    from lucem_amd.autograd import RLGradPlan
+
+    rng = np.random.default_rng(31)
+    shapes = [(48, 40), (48, 40)]
+    ys = [torch.from_numpy(y).cuda() for y in _tiles(rng, 2, shapes[0])]
+    x0 = [torch.from_numpy(x).cuda() for x in _tiles(rng, 2, shapes[0], 0.5, 2.0)]
+    psf = torch.from_numpy(rng.uniform(0.1, 1.0, (7, 5)).astype(np.float32)).cuda()
+    plan = RLGradPlan(shapes, (7, 5), 6, checkpoint_every=3)
+    ref = None
+    for rep in range(4):
+        junk = torch.randn(1 << 18, device="cuda").sum()
+        loss = plan.forward(x0, ys, psf)
+        gx = [torch.full(shapes[0], float(rep), device="cuda") for _ in shapes]
+        gp = torch.full((7, 5), float(rep), device="cuda")
+        plan.backward(None, gx, gp)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(junk))
+        got = [loss.cpu().numpy()] + [g.cpu().numpy() for g in gx] + [gp.cpu().numpy()]
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(got, ref):
+                np.testing.assert_array_equal(a, b)
+    plan.close()
