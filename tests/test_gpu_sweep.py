@@ -1,6 +1,7 @@
 """Seeded random sweeps over shapes against the oracles: the plain RL form in 2-D and 3-D, the
-batched form, the Wiener filter and the complex FFT, at sizes from 1 up to a few hundred (odd,
-prime and 7-smooth), PSFs from 1 to 31 taps, every pad mode. Each case prints what it ran."""
+batched form, the Wiener filter, the complex FFT, the correlation and phase correlation, at sizes
+from 1 up to a few hundred (odd, prime and 7-smooth), PSFs and windows from 1 to 31 taps, every
+pad mode. Each case prints what it ran."""
 
 from __future__ import annotations
 
@@ -152,3 +153,53 @@ def test_grad_random_shapes(gpu, oracle):
         print(f"case {case}: {shape} psf {kshape} mode {mode} T {T}: loss {abs(float(loss[0]) - L) / abs(L):.1e}, "
               f"dx0 {ex:.2e}, dpsf {ep:.2e}")
         assert abs(float(loss[0]) - L) <= 1e-6 * abs(L) and ex < 1e-3 and ep < 1e-3
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_correlate_random_shapes(gpu, oracle, seed):
+    """The correlation in 2-D (direct sum up to 13 x 13, FFT passes above) and 3-D, random
+    windows, origins, modes and cval, against the FP64 oracle at the reference's tolerance."""
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(900 + seed)
+    for case in range(10):
+        nd = 2 if case < 7 else 3
+        shape = tuple(_size(rng, 1, 300 if nd == 2 else 40) for _ in range(nd))
+        kshape = tuple(int(rng.integers(1, 26 if nd == 2 else 9)) for _ in range(nd))
+        origin = tuple(int(rng.integers(-(k // 2), (k - 1) // 2 + 1)) for k in kshape)
+        mode = int(rng.integers(0, 5))
+        cval = float(rng.uniform(-2, 2)) if mode == 1 else 0.0
+        x = rng.uniform(-1, 1, shape).astype(np.float32)
+        w = rng.standard_normal(kshape).astype(np.float32)
+        print(f"correlate {shape} window {kshape} origin {origin} mode {mode} cval {cval:.3f}", flush=True)
+        s = LA.Correlate2d(mode=mode, cval=cval, origin=origin)
+        got = (LA.correlate_2d if nd == 2 else LA.correlate_nd)(_dev(x), _dev(w), s).cpu().numpy()
+        want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
+        scale = float(np.abs(want).max())
+        assert np.all(np.abs(got - want) <= 1e-5 * np.abs(want) + 1e-5 * scale)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_phase_correlation_random_shifts(gpu, oracle, seed):
+    """Random sizes (1-D to 3-D), shifts and upsampling factors: the same shift as the oracle."""
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(950 + seed)
+    for case in range(6):
+        nd = 1 + case % 3
+        shape = tuple(_size(rng, 16, 200 if nd < 3 else 40) for _ in range(nd))
+        u = int(rng.choice([1, 2, 4, 10]))
+        shift = tuple(float(np.round(rng.uniform(-n / 4, n / 4) * u) / u) for n in shape)
+        x = rng.uniform(size=shape)
+        X = np.fft.fftn(x)
+        ph = np.zeros(shape)
+        for ax, (n, s_) in enumerate(zip(shape, shift)):
+            k = np.arange(n)
+            f = np.where(k <= n // 2, k, k - n) / n
+            ph = ph + (f * s_).reshape([n if a == ax else 1 for a in range(nd)])
+        y = np.fft.ifftn(X * np.exp(-2j * np.pi * ph)).real + rng.normal(scale=1e-3, size=shape)
+        x32, y32 = x.astype(np.float32), y.astype(np.float32)
+        print(f"phase_correlation {shape} shift {shift} u {u}", flush=True)
+        got = LA.phase_correlation(_dev(x32), _dev(y32), LA.PhaseCorrelation(upsample_factor=u))
+        want, _, _ = oracle.phase_correlation64(x32, y32, u, True)
+        assert got.shift == want
