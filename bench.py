@@ -29,9 +29,11 @@ Alongside the throughput the line carries:
   1024x1024 (30 iterations, burn-in 10), and the FP64 numpy restatement timed on this host;
 - ``fft``: the general complex FFT over both axes of a 2048x2048 split re/im array, and of a
   2039x2039 one (Bluestein lines), against 32 bytes per element (two read+write passes);
-- ``filter``: the correlation entry point on 2048x2048 with a 31x31 window, the device phantom
-  renderers on the 2-D and 3-D workloads' truths, and the whole 3-D workload generated on the
-  device (render, blur, Poisson noise);
+- ``filter``: the correlation entry point on 2048x2048 with a 31x31 window (FFT passes) and a
+  5x5 one (direct sum), phase correlation of a 1024x1024 pair at upsample 20, Butterworth and Gabor
+  on 2048x2048, the device phantom renderers on the 2-D and 3-D workloads' truths, the whole 3-D
+  workload generated on the device (render, blur, Poisson noise), and the oracle's CPU
+  correlation and phase correlation timed on this host;
 - ``tiled``: the zero-copy block walk (3 channels of 2048x2048, blocks of 480 with a 32-px halo,
   15x15 Gaussian PSF, 20 iterations), plan creation included, and the Poisson ``add_noise`` of a
   2048x2048 image against its 8 bytes per pixel.
@@ -302,7 +304,27 @@ def tiled_leg(steps: int) -> dict:
     }
 
 
-def filter_leg(steps: int) -> dict:
+def cpu_baseline_filters(x, w, ref, mov) -> dict:
+    """The oracle's CPU forms timed once on this host: the direct FP32 correlation (OpenMP, all
+    cores) on a 256-row strip of the 2048x2048 image, scaled to the whole image, and the FP64
+    numpy phase correlation (one core) on the same 1024x1024 pair."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    strip = np.ascontiguousarray(x[:256])
+    t0 = time.perf_counter()
+    O.correlate(strip, w, 0, 0.0)
+    tc = (time.perf_counter() - t0) * x.shape[0] / strip.shape[0]
+    t0 = time.perf_counter()
+    O.phase_correlation64(ref, mov, 20, True)
+    tp = time.perf_counter() - t0
+    return {"correlate_ms": round(tc * 1e3, 1), "correlate_cores": O.num_threads(),
+            "phase_correlation_ms": round(tp * 1e3, 1), "phase_correlation_cores": 1, "kind": "port",
+            "sample": "correlate: 256 of 2048 rows, scaled; phase correlation: the same 1024x1024 pair"}
+
+
+def filter_leg(steps: int, with_cpu: bool) -> dict:
     import numpy as np
     import torch
 
@@ -322,13 +344,30 @@ def filter_leg(steps: int) -> dict:
     p3 = synth.sphere_params((128, 512, 512), 1500, np.random.default_rng(1001))
     t3 = _timeit(lambda: phantoms.render_spheres((128, 512, 512), p3), max(steps, 3))
     tg = _timeit(lambda: phantoms.plain3d(), 2)
-    return {
+    rng = np.random.default_rng(8)
+    ref = rng.uniform(size=(1024, 1024)).astype(np.float32)
+    mov = np.roll(ref, (17, -40), axis=(0, 1))
+    rt, mt = torch.from_numpy(ref).cuda(), torch.from_numpy(mov).cuda()
+    tpc = _timeit(lambda: LA.phase_correlation(rt, mt, LA.PhaseCorrelation(upsample_factor=20)), 3)
+    tbw = _timeit(lambda: LA.butterworth(x, 0.2, False, 2.0), max(steps, 5))
+    tgb = _timeit(lambda: LA.gabor_2d(x, 0.1, 0.5, 3.0, 3.0), max(steps, 5))
+    w5 = torch.from_numpy(_gaussian_psf(5, 1.0)).cuda()
+    t5 = _timeit(lambda: LA.correlate_2d_into(x, w5, o, s, ws), max(steps, 10))
+    out = {
         "correlate_2048_w31_ms": round(tc * 1e3, 4),
         "correlate_note": "2048x2048 f32, 31x31 window, Reflect: transfer function built in the call",
+        "correlate_2048_w5_ms": round(t5 * 1e3, 4),
+        "correlate_w5_GBps": round(8 * x.numel() / t5 / 1e9, 1),
+        "phase_correlation_1024_u20_ms": round(tpc * 1e3, 3),
+        "butterworth_2048_ms": round(tbw * 1e3, 4),
+        "gabor_2048_ms": round(tgb * 1e3, 4),
         "render_ellipses_1024_ms": round(t2 * 1e3, 4),
         "render_spheres_128x512x512_ms": round(t3 * 1e3, 3),
         "device_workload_plain3d_ms": round(tg * 1e3, 2),
     }
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_filters(x.cpu().numpy(), _gaussian_psf(31, 5.0), ref, mov)
+    return out
 
 
 def main() -> int:
@@ -439,7 +478,7 @@ def main() -> int:
         line["wiener"] = wiener_leg(args.steps, not args.no_cpu_baseline)
         line["fft"] = fft_leg(args.steps)
         line["tiled"] = tiled_leg(args.steps)
-        line["filter"] = filter_leg(args.steps)
+        line["filter"] = filter_leg(args.steps, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))
