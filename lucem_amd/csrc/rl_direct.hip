@@ -25,8 +25,11 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_kernel(const T* __re
                                                                    int ly, int lx, int mode, T cval) {
     extern __shared__ __align__(16) unsigned char tile_raw[];
     T* tile = reinterpret_cast<T*>(tile_raw);
+    // one flat grid of blocks, row-major over the output (no limit on the number of block rows)
+    const int nbx = (W + kTile - 1) / kTile;
+    const int bx = (int)(blockIdx.x % (unsigned)nbx), by = (int)(blockIdx.x / (unsigned)nbx);
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+    const int x0 = bx * kTile, y0 = by * kTile;
     const int x = x0 + tx, y = y0 + ty;
     if constexpr (STAGED) {
         const int tw = kTile + kw - 1, th = kTile + kh - 1;
@@ -118,7 +121,7 @@ template <typename T>
 static int correlate2d(const T* in, const T* w, T* out, int H, int W, int kh, int kw, int ly, int lx, int mode, T cval,
                        void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    const dim3 grid((unsigned)((W + kTile - 1) / kTile), (unsigned)((H + kTile - 1) / kTile));
+    const dim3 grid((unsigned)(((int64_t)W + kTile - 1) / kTile * (((int64_t)H + kTile - 1) / kTile)));
     const size_t lds = (size_t)(kTile + kw - 1) * (kTile + kh - 1) * sizeof(T);
     if (lds <= kDirectTileLds)
         hipLaunchKernelGGL((correlate2d_kernel<T, true>), grid, dim3(kTile * kTile), lds, st, in, w, out, H, W, kh, kw,
