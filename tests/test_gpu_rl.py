@@ -310,6 +310,7 @@ LONG_DIRECT_CASES = [
     ((3, 16390), (3, 5), 3),
     ((4, 16380), (3, 9), 1),    # n fits, n + k - 1 does not
     ((16390, 2), (3, 1), 2),
+    ((1100000, 2), (3, 1), 0),  # more than 65535 rows of 16-row blocks: the flat block grid
 ]
 
 
