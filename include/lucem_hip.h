@@ -53,6 +53,8 @@
  *   lucem_correlate_nd_into_f32
  *                         the same operator on 2-D or 3-D arrays (no reference counterpart)
  *   lucem_gabor_2d_f32    lucem_ops::filter::gabor_2d (filter/spectral.rs:216-268), on the correlation
+ *   lucem_correlate1d_settings / lucem_correlate_1d_into_f32 / lucem_correlate_1d_into_f64
+ *                         lucem_ops::filter::{Correlate1d, correlate_1d_into} (filter/convolve.rs:15-75)
  *   lucem_fourier_gaussian_into_f32 / lucem_fourier_box_into_f32 / lucem_fourier_shift_into_f32 /
  *   lucem_butterworth_f32
  *                         lucem_ops::filter::{fourier_gaussian_into, fourier_box_into,
@@ -489,6 +491,30 @@ lucem_status lucem_render_ellipses_f32(lucem_client* client, float* out, const u
                                        const lucem_ellipse* objects, uint64_t count, double background);
 lucem_status lucem_render_spheres_f32(lucem_client* client, float* out, const uint64_t* dims, uint32_t ndim,
                                       const lucem_sphere* objects, uint64_t count, double background);
+
+/* Correlate1d settings (filter/convolve.rs:15-29): axis (negative counts from the end), pad mode,
+ * window origin, cval. Defaults (lucem_correlate1d_settings_default): axis -1, Reflect, 0, 0. */
+typedef struct lucem_correlate1d_settings {
+    int64_t axis;
+    uint32_t mode;
+    int32_t origin;
+    double cval;
+} lucem_correlate1d_settings;
+void lucem_correlate1d_settings_default(lucem_correlate1d_settings* out);
+/* correlate_1d_into (filter/convolve.rs:40-75): out[o, i, k] = sum_t w[t] P(in)[o, i + t - left, k]
+ * along `axis`, left = size / 2 + origin, the direct sum in the reference's order with its
+ * interior fast path (filter/kernels/convolve.rs:8-55). `weights` is a device array of
+ * num_weights taps. Checks in the reference's order: same shape, no overlap of input and output,
+ * the origin (a window of 0 taps is LUCEM_INVALID_WEIGHTS), the axis (LUCEM_AXIS_OUT_OF_BOUNDS),
+ * empty (OK). */
+lucem_status lucem_correlate_1d_into_f32(lucem_client* client, const float* input, const uint64_t* dims, uint32_t ndim,
+                                         const float* weights, uint64_t num_weights, float* output,
+                                         const uint64_t* out_dims, uint32_t out_ndim,
+                                         const lucem_correlate1d_settings* settings);
+lucem_status lucem_correlate_1d_into_f64(lucem_client* client, const double* input, const uint64_t* dims,
+                                         uint32_t ndim, const double* weights, uint64_t num_weights, double* output,
+                                         const uint64_t* out_dims, uint32_t out_ndim,
+                                         const lucem_correlate1d_settings* settings);
 
 /* gabor_2d::<f32> (filter/spectral.rs:216-268): the complex Gabor kernel of gabor_kernel_2d
  * (FP64 on the host, rounded to f32) correlated with the 2-D input by lucem_correlate_2d_into_f32,

@@ -13,8 +13,9 @@ Public surface (mirrors the reference's names):
 - :mod:`lucem_amd.fft`: ``fft_1d_into``, ``fft_nd_into``, ``complex_mul_into``
                                                    core/fft.rs:13-135
 - :class:`NoiseModel`, :func:`add_noise`          intensity/noise.rs:17-143
-- :class:`Correlate2d`, :func:`correlate_2d_into`, :func:`correlate_2d`
-                                                   filter/convolve.rs:113-177
+- :class:`Correlate2d`, :func:`correlate_2d_into`, :func:`correlate_2d`, :class:`Correlate1d`,
+  :func:`correlate_1d_into`, :func:`correlate_1d`, :func:`convolve_1d`
+                                                   filter/convolve.rs:15-177
 - :func:`fourier_gaussian_into`, :func:`fourier_box_into`, :func:`fourier_shift_into`,
   :func:`butterworth`, :func:`gabor_2d`            filter/spectral.rs:20-268
 - :class:`PhaseCorrelation`, :func:`phase_correlation`
@@ -42,7 +43,11 @@ from . import fft  # noqa: F401,E402
 from .noise import NoiseModel, add_noise, add_noise_into  # noqa: F401,E402
 from . import tiled  # noqa: F401,E402
 from .filter import (  # noqa: F401,E402
+    Correlate1d,
     Correlate2d,
+    convolve_1d,
+    correlate_1d,
+    correlate_1d_into,
     butterworth,
     correlate_2d,
     correlate_2d_into,

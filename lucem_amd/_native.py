@@ -55,6 +55,7 @@ EXPORTED = [
     "lucem_phase_correlation_settings_default", "lucem_phase_correlation_f32",
     "lucem_fourier_gaussian_into_f32", "lucem_fourier_box_into_f32", "lucem_fourier_shift_into_f32",
     "lucem_butterworth_f32", "lucem_gabor_settings_default", "lucem_gabor_2d_f32",
+    "lucem_correlate1d_settings_default", "lucem_correlate_1d_into_f32", "lucem_correlate_1d_into_f64",
 ]
 
 
@@ -121,6 +122,11 @@ class Ellipse(ctypes.Structure):
 
 class Sphere(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("cz", "cy", "cx", "radius", "radius_sq", "value")]
+
+
+class Correlate1dSettings(ctypes.Structure):
+    _fields_ = [("axis", ctypes.c_int64), ("mode", ctypes.c_uint32), ("origin", ctypes.c_int32),
+                ("cval", ctypes.c_double)]
 
 
 class GaborSettings(ctypes.Structure):
@@ -207,6 +213,12 @@ def lib() -> ctypes.CDLL:
             L.lucem_fourier_shift_into_f32.argtypes = spec
             L.lucem_butterworth_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                 ctypes.c_double, ctypes.c_int32, ctypes.c_double, _vp]
+            L.lucem_correlate1d_settings_default.argtypes = [ctypes.POINTER(Correlate1dSettings)]
+            L.lucem_correlate1d_settings_default.restype = None
+            c1 = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp, _u64p, ctypes.c_uint32,
+                  ctypes.POINTER(Correlate1dSettings)]
+            L.lucem_correlate_1d_into_f32.argtypes = c1
+            L.lucem_correlate_1d_into_f64.argtypes = c1
             L.lucem_gabor_settings_default.argtypes = [ctypes.POINTER(GaborSettings)]
             L.lucem_gabor_settings_default.restype = None
             L.lucem_gabor_2d_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _vp, ctypes.POINTER(GaborSettings),

@@ -336,3 +336,80 @@ extern "C" lucem_status lucem_gabor_2d_f32(lucem_client* client, const float* in
     if ((st = correlate_entry(client, input, dims, ndim, dre, kd, 2, out_re, dims, ndim, &cs, ws, false))) return st;
     return correlate_entry(client, input, dims, ndim, dim, kd, 2, out_im, dims, ndim, &cs, ws, false);
 }
+
+extern "C" void lucem_correlate1d_settings_default(lucem_correlate1d_settings* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof *out);
+    out->axis = -1;
+    out->mode = LUCEM_PAD_REFLECT;
+}
+
+// correlate_1d_into (filter/convolve.rs:40-75): same shape, not aliased, origin, axis, empty.
+template <typename T>
+static lucem_status correlate1d_entry(lucem_client* client, const T* input, const uint64_t* dims, uint32_t ndim,
+                                      const T* weights, uint64_t num_weights, T* output, const uint64_t* out_dims,
+                                      uint32_t out_ndim, const lucem_correlate1d_settings* s) {
+    if (!s) return fail(LUCEM_INVALID_HANDLE, "settings is NULL");
+    if ((ndim && !dims) || (out_ndim && !out_dims)) return fail(LUCEM_INVALID_HANDLE, "dims pointer is NULL");
+    bool same = ndim == out_ndim;
+    for (uint32_t i = 0; same && i < ndim; i++) same = dims[i] == out_dims[i];
+    if (!same)
+        return fail(LUCEM_SHAPE_MISMATCH, "input shape " + dims_str(dims, ndim) + " does not match output shape " +
+                                              dims_str(out_dims, out_ndim));
+    uint64_t n = 1;
+    bool overflow = false;
+    for (uint32_t i = 0; i < ndim; i++) {
+        if (dims[i] && n > UINT64_MAX / dims[i]) overflow = true;
+        n *= dims[i];
+    }
+    if (n > 0 && !overflow && input && output) {
+        const char* a = (const char*)input;
+        const char* b = (const char*)output;
+        const uint64_t bytes = n * sizeof(T);
+        if (a < b + bytes && b < a + bytes)
+            return fail(LUCEM_ALIASED_BUFFERS, "input and output arrays must not share the same device buffer");
+    }
+    lucem_status st = check_origin(s->origin, num_weights);
+    if (st) return st;
+    const int64_t nd = (int64_t)ndim;
+    const int64_t ax = s->axis < 0 ? s->axis + nd : s->axis;
+    if (ax < 0 || ax >= nd)
+        return fail(LUCEM_AXIS_OUT_OF_BOUNDS,
+                    "axis " + std::to_string(s->axis) + " is out of bounds for " + std::to_string(nd) + "-D array");
+    if (s->mode > 4)
+        return fail(LUCEM_UNKNOWN_PAD_MODE, "boundary pad mode id " + std::to_string(s->mode) +
+                                                " is not supported (expected 0..4: reflect, constant, nearest, "
+                                                "mirror, wrap)");
+    if (n == 0) return LUCEM_OK;
+    if (overflow || n > 0xFFFFFFFFull)
+        return fail(LUCEM_TOO_MANY_ELEMENTS,
+                    "dims " + dims_str(dims, ndim) + " exceed the device element limit of 4294967295");
+    if (dims[ax] > 0x7FFFFFFFull || num_weights > 0x7FFFFFFFull)
+        return fail(LUCEM_UNSUPPORTED_SHAPE, "axis or window longer than 2^31 - 1");
+    if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
+    if (!input || !weights || !output) return fail(LUCEM_INVALID_HANDLE, "input, weights or output pointer is NULL");
+    hipError_t e = hipSetDevice(client->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    int64_t inner = 1;
+    for (int64_t a = ax + 1; a < nd; a++) inner *= (int64_t)dims[a];
+    const int64_t left = (int64_t)(num_weights / 2) + s->origin;
+    return check(launch_correlate1d(input, output, weights, (int64_t)n, (int64_t)dims[ax], inner, left, (T)s->cval,
+                                    (int)num_weights, (int)s->mode, client->stream),
+                 "correlate 1-D");
+}
+
+extern "C" lucem_status lucem_correlate_1d_into_f32(lucem_client* client, const float* input, const uint64_t* dims,
+                                                    uint32_t ndim, const float* weights, uint64_t num_weights,
+                                                    float* output, const uint64_t* out_dims, uint32_t out_ndim,
+                                                    const lucem_correlate1d_settings* settings) {
+    return correlate1d_entry<float>(client, input, dims, ndim, weights, num_weights, output, out_dims, out_ndim,
+                                    settings);
+}
+
+extern "C" lucem_status lucem_correlate_1d_into_f64(lucem_client* client, const double* input, const uint64_t* dims,
+                                                    uint32_t ndim, const double* weights, uint64_t num_weights,
+                                                    double* output, const uint64_t* out_dims, uint32_t out_ndim,
+                                                    const lucem_correlate1d_settings* settings) {
+    return correlate1d_entry<double>(client, input, dims, ndim, weights, num_weights, output, out_dims, out_ndim,
+                                     settings);
+}

@@ -188,6 +188,13 @@ int launch_plane_wave_sums(const float* re, const float* im, const int64_t dims3
                            double2* part, void* stream);
 int launch_sum_squares(const float* x, int64_t n, double* part, void* stream);
 
+// correlate_1d along an axis viewed as [outer, len, inner] (total elements), window of `size`
+// taps starting `left` before each output, `cval` outside the array in Constant mode.
+int launch_correlate1d(const float* in, float* out, const float* w, int64_t total, int64_t len, int64_t inner,
+                       int64_t left, float cval, int size, int mode, void* stream);
+int launch_correlate1d(const double* in, double* out, const double* w, int64_t total, int64_t len, int64_t inner,
+                       int64_t left, double cval, int size, int mode, void* stream);
+
 // Constant-mode cval of the f32 correlation: out = in - cval; *sum = sum(w) in f64;
 // out += cval * *sum.
 int launch_cval_shift(const float* in, float* out, float cval, int64_t n, void* stream);

@@ -6,6 +6,8 @@ Reference (neuro-kitchen/lucem):
 - ``correlate_2d_into``               crates/lucem-ops/src/filter/convolve.rs:126-166
 - ``correlate_2d`` (allocating form)  crates/lucem-ops/src/filter/convolve.rs:169-177
 - ``check::origin``                   crates/lucem-ops/src/core/check.rs:59-72
+- ``Correlate1d``, ``correlate_1d_into``, ``correlate_1d``, ``convolve_1d``
+                                      crates/lucem-ops/src/filter/convolve.rs:15-111
 - ``fourier_gaussian_into``, ``fourier_box_into``, ``fourier_shift_into``, ``butterworth``
                                       crates/lucem-ops/src/filter/spectral.rs:20-212
 - ``gabor_2d`` (``gabor_kernel_2d``)  crates/lucem-ops/src/filter/spectral.rs:216-268
@@ -175,3 +177,74 @@ def gabor_2d(input, frequency: float, theta: float, sigma_x: float, sigma_y: flo
                                        N._vp(oim.data_ptr() if oim.numel() else None), ctypes.byref(s),
                                        ws.handle if ws is not None else None))
     return ore, oim
+
+
+@dataclass
+class Correlate1d:
+    """Settings of 1-D correlation (convolve.rs:15-29): axis, boundary mode, cval, origin."""
+
+    axis: int = -1
+    mode: PadMode | str = PadMode.Reflect
+    cval: float = 0.0
+    origin: int = 0
+
+    def to_c(self) -> N.Correlate1dSettings:
+        s = N.Correlate1dSettings()
+        N.lib().lucem_correlate1d_settings_default(ctypes.byref(s))
+        s.axis = int(self.axis)
+        s.mode = int(PadMode.parse(self.mode)) if isinstance(self.mode, str) else int(PadMode(self.mode))
+        s.origin = int(self.origin)
+        s.cval = float(self.cval)
+        return s
+
+
+def correlate_1d_into(input, weights, output, settings: Correlate1d | None = None,
+                      client: N.Client | None = None) -> None:
+    """1-D correlation along ``settings.axis`` with device-resident ``weights`` (convolve.rs:40-75);
+    float32 or float64 (input, weights and output alike)."""
+    import torch
+
+    x, w, o = N.as_tensor(input), N.as_tensor(weights), N.as_tensor(output)
+    if client is None:
+        client = N.default_client(x.device.index if x.is_cuda else None)
+    f64 = x.dtype == torch.float64
+    dt = torch.float64 if f64 else None
+    xp = N.device_ptr(x, "input", dt) if x.numel() else 0
+    wp = N.device_ptr(w, "weights", dt) if w.numel() else 0
+    op = N.device_ptr(o, "output", dt) if o.numel() else 0
+    fn = N.lib().lucem_correlate_1d_into_f64 if f64 else N.lib().lucem_correlate_1d_into_f32
+    st = (settings or Correlate1d()).to_c()
+    N.check(fn(client.handle, N._vp(xp or None), N.dims(x.shape), x.dim(), N._vp(wp or None), w.numel(),
+               N._vp(op or None), N.dims(o.shape), o.dim(), ctypes.byref(st)))
+
+
+def _check_weights(w) -> list:
+    """check::weights (core/check.rs:74-82): non-empty and finite."""
+    import math
+
+    w = [float(v) for v in w]
+    if not w:
+        raise N.LucemError(4, "weights: filter weights cannot be empty")
+    if not all(math.isfinite(v) for v in w):
+        raise N.LucemError(4, "weights: filter weights must be finite")
+    return w
+
+
+def correlate_1d(input, weights, settings: Correlate1d | None = None, client: N.Client | None = None):
+    """correlate_1d (convolve.rs:77-92): host weights uploaded in the input's precision."""
+    import torch
+
+    x = N.as_tensor(input)
+    w = torch.as_tensor(_check_weights(weights), dtype=x.dtype, device=x.device)
+    out = torch.empty_like(x)
+    correlate_1d_into(x, w, out, settings, client)
+    return out
+
+
+def convolve_1d(input, weights, settings: Correlate1d | None = None, client: N.Client | None = None):
+    """convolve_1d (convolve.rs:94-111): correlation with the reversed weights and the mirrored
+    origin -origin - (1 if the size is even)."""
+    s = settings or Correlate1d()
+    w = _check_weights(weights)[::-1]
+    mirrored = Correlate1d(axis=s.axis, mode=s.mode, cval=s.cval, origin=-s.origin - (1 if len(w) % 2 == 0 else 0))
+    return correlate_1d(input, w, mirrored, client)

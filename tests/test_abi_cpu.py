@@ -274,6 +274,29 @@ def test_spectral_check_order_without_device(L):
     assert bw((4, 5), (4, 5), 0.5) == 11
 
 
+def test_correlate_1d_check_order_without_device(L):
+    """correlate_1d_into (filter/convolve.rs:47-53): same shape, aliasing, origin, axis, empty."""
+    from lucem_amd import _native as N
+
+    s = N.Correlate1dSettings()
+    L.lucem_correlate1d_settings_default(ctypes.byref(s))
+    assert (s.axis, s.mode, s.origin, s.cval) == (-1, 0, 0, 0.0)
+
+    def call(a, b, nw=3, axis=-1, origin=0, out=0x20000000):
+        s.axis, s.origin = axis, origin
+        return L.lucem_correlate_1d_into_f32(None, N._vp(0x10000), N.dims(a), len(a), N._vp(0x30000000), nw,
+                                             N._vp(out), N.dims(b), len(b), ctypes.byref(s))
+
+    assert call((4, 5), (4, 6), nw=0) == 1
+    assert call((4, 5), (4, 5), nw=0, out=0x10000 + 40) == 2
+    assert call((4, 5), (4, 5), nw=0) == 4
+    assert call((4, 5), (4, 5), nw=3, origin=2) == 14
+    assert call((4, 5), (4, 5), nw=3, axis=2) == 12
+    assert call((4, 5), (4, 5), nw=3, axis=-3) == 12
+    assert call((0, 5), (0, 5), nw=3) == 0
+    assert call((4, 5), (4, 5), nw=4, origin=-2, axis=0) == 11
+
+
 def test_wiener_check_order_without_device(L):
     from lucem_amd import _native as N
 

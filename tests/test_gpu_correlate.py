@@ -182,3 +182,65 @@ def test_correlate_small_windows_through_the_fft_passes(gpu, oracle, mode, monke
         want = oracle.correlate(x.astype(np.float64), w.astype(np.float64), mode, cval, origin=origin)
         assert_close(got, want, 1e-5)
         assert oracle.rel_l2(got, want) < 1e-6
+
+
+def _host_correlate_1d(x, dims, w, axis, mode, cval, origin):
+    """The reference test's host_correlate_1d (filter/convolve.rs:181-209), in FP64."""
+    x = np.asarray(x, np.float64).reshape(dims)
+    out = np.zeros_like(x)
+    n = dims[axis]
+    left = len(w) // 2 + origin
+    xm = np.moveaxis(x, axis, -1)
+    om = np.moveaxis(out, axis, -1)
+    for i in range(n):
+        acc = np.zeros(xm.shape[:-1])
+        for t, wt in enumerate(w):
+            p = i - left + t
+            from oracle import oracle as O
+
+            q = O.pad_index(p, n, mode)
+            acc = acc + wt * (cval if q < 0 else xm[..., q])
+        om[..., i] = acc
+    return out
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_correlate_1d_reference_case(gpu, mode):
+    """filter/convolve.rs:211-228: dims [1], [7], [32, 33], [5, 1, 9]; every axis; weights
+    ([1, -2, 3], origin 0) and ([0.5] * 4, origin -1); cval 1.5; assert_close at 1e-5; in f32 and
+    f64 (f64 to 1e-12)."""
+    import torch
+
+    import lucem_amd as LA
+
+    for dims in [(1,), (7,), (32, 33), (5, 1, 9)]:
+        n = int(np.prod(dims))
+        x = np.array([((i * 7919 + 104729) % 1013) * 0.1 - 50.0 for i in range(n)]).reshape(dims)
+        for axis in range(len(dims)):
+            for w, origin in [([1.0, -2.0, 3.0], 0), ([0.5] * 4, -1)]:
+                s = LA.Correlate1d(axis=axis, mode=mode, cval=1.5, origin=origin)
+                want = _host_correlate_1d(x, dims, w, axis, mode, 1.5, origin)
+                got = LA.correlate_1d(torch.from_numpy(x.astype(np.float32)).cuda(), w, s).cpu().numpy()
+                assert_close(got, want, 1e-5)
+                got64 = LA.correlate_1d(torch.from_numpy(x).cuda(), w, s).cpu().numpy()
+                np.testing.assert_allclose(got64, want, rtol=1e-12, atol=1e-12)
+
+
+def test_convolve_1d_is_the_mirrored_correlation(gpu, oracle):
+    """convolve_1d (convolve.rs:94-111) equals scipy's convolve1d (the same reversal and origin
+    rule) on even and odd windows."""
+    import torch
+    from scipy import ndimage as ndi
+
+    import lucem_amd as LA
+
+    x = np.random.default_rng(3).normal(size=(20, 30))
+    for w, origin in [([1.0, 2.0, -1.0], 0), ([0.25, 0.5, 0.75, 1.0], 1), ([1.0, -1.0], 0)]:
+        got = LA.convolve_1d(torch.from_numpy(x).cuda(), w, LA.Correlate1d(axis=1, origin=origin)).cpu().numpy()
+        want = ndi.convolve1d(x, w, axis=1, mode="reflect", origin=origin)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    with pytest.raises(LA.LucemError):
+        LA.correlate_1d(torch.from_numpy(x).cuda(), [1.0, float("nan")])
+    with pytest.raises(LA.LucemError) as e:
+        LA.correlate_1d(torch.from_numpy(x).cuda(), [1.0], LA.Correlate1d(axis=2))
+    assert e.value.kind == "AXIS_OUT_OF_BOUNDS"
