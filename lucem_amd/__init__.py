@@ -18,6 +18,8 @@ Public surface (mirrors the reference's names):
                                                    filter/convolve.rs:15-177
 - :func:`fourier_gaussian_into`, :func:`fourier_box_into`, :func:`fourier_shift_into`,
   :func:`butterworth`, :func:`gabor_2d`            filter/spectral.rs:20-268
+- :class:`GaussianBlur`, :func:`gaussian_blur`, :func:`gaussian_1d`
+                                                   filter/gaussian.rs:18-252
 - :class:`PhaseCorrelation`, :func:`phase_correlation`
                                                    register/phase.rs:18-250
 - :class:`Client`, :class:`Workspace`              lucem-core Client / Workspace
@@ -57,5 +59,12 @@ from .filter import (  # noqa: F401,E402
     fourier_gaussian_into,
     fourier_shift_into,
     gabor_2d,
+    Gaussian1d,
+    GaussianBlur,
+    gaussian_1d,
+    gaussian_1d_into,
+    gaussian_blur,
+    gaussian_blur_into,
+    gaussian_kernel_1d,
 )
 from .register import PhaseCorrelation, PhaseCorrelationResult, phase_correlation  # noqa: F401,E402

@@ -8,6 +8,10 @@ Reference (neuro-kitchen/lucem):
 - ``check::origin``                   crates/lucem-ops/src/core/check.rs:59-72
 - ``Correlate1d``, ``correlate_1d_into``, ``correlate_1d``, ``convolve_1d``
                                       crates/lucem-ops/src/filter/convolve.rs:15-111
+- ``gaussian_kernel_1d``, ``Gaussian1d``, ``gaussian_1d(_into)``, ``GaussianBlur``,
+  ``gaussian_blur(_into)``            crates/lucem-ops/src/filter/gaussian.rs:18-252 (the
+                                      separable passes of core/separable.rs:8-50 on the device's
+                                      1-D correlation)
 - ``fourier_gaussian_into``, ``fourier_box_into``, ``fourier_shift_into``, ``butterworth``
                                       crates/lucem-ops/src/filter/spectral.rs:20-212
 - ``gabor_2d`` (``gabor_kernel_2d``)  crates/lucem-ops/src/filter/spectral.rs:216-268
@@ -22,7 +26,7 @@ the reference's order. Errors raise :class:`lucem_amd.LucemError` in the referen
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 from . import _native as N
 from .restore import PadMode
@@ -235,9 +239,12 @@ def correlate_1d(input, weights, settings: Correlate1d | None = None, client: N.
     import torch
 
     x = N.as_tensor(input)
-    w = torch.as_tensor(_check_weights(weights), dtype=x.dtype, device=x.device)
-    out = torch.empty_like(x)
-    correlate_1d_into(x, w, out, settings, client)
+    taps = _check_weights(weights)
+    client, ctx = _client_stream(x, client)
+    with ctx:  # the weights are freed on the stream that reads them
+        w = torch.as_tensor(taps, dtype=x.dtype, device=x.device)
+        out = torch.empty_like(x)
+        correlate_1d_into(x, w, out, settings, client)
     return out
 
 
@@ -248,3 +255,218 @@ def convolve_1d(input, weights, settings: Correlate1d | None = None, client: N.C
     w = _check_weights(weights)[::-1]
     mirrored = Correlate1d(axis=s.axis, mode=s.mode, cval=s.cval, origin=-s.origin - (1 if len(w) % 2 == 0 else 0))
     return correlate_1d(input, w, mirrored, client)
+
+
+def gaussian_kernel_1d(sigma: float, order: int, radius: int):
+    """gaussian_kernel_1d (gaussian.rs:18-64) in FP64 on the host: the normalised sampled Gaussian
+    of half-width ``radius``, or its ``order``-th derivative through the Hermite recurrence, with
+    odd orders negated for the correlation convention."""
+    import numpy as np
+
+    if radius == 0:
+        return np.array([1.0 if order == 0 else 0.0])
+    s2 = sigma * sigma
+    x = np.arange(-radius, radius + 1, dtype=np.float64)
+    phi = np.exp(-0.5 * x * x / s2)
+    phi = phi / phi.sum()
+    if order == 0:
+        return phi
+    if order == 1:
+        q = -x / s2
+    elif order == 2:
+        q = (x * x - s2) / (s2 * s2)
+    elif order == 3:
+        q = -(x ** 3 - 3.0 * x * s2) / s2 ** 3
+    else:
+        h_prev, h_cur = np.ones_like(x), -x / s2
+        for k in range(1, order):
+            h_prev, h_cur = h_cur, (-x / s2) * h_cur - (k / s2) * h_prev
+        q = h_cur
+    return (-1.0 if order % 2 else 1.0) * q * phi
+
+
+_GAUSSIAN_TAPS: dict = {}
+
+
+def _gaussian_taps(sigma: float, order: int, radius: int, like):
+    """The taps on the device in ``like``'s precision, kept per device under the reference's key
+    (gaussian.rs:106-110: sigma to 12 decimals, order, radius), so that a repeated filter uploads
+    nothing and no call frees taps that queued work still reads."""
+    import numpy as np
+    import torch
+
+    key = (str(like.device), like.dtype, f"gauss1d-{sigma:.12f}-o{order}-r{radius}")
+    t = _GAUSSIAN_TAPS.get(key)
+    if t is None:
+        host = gaussian_kernel_1d(sigma, order, radius)
+        np_dt = np.float64 if like.dtype == torch.float64 else np.float32
+        t = torch.from_numpy(host.astype(np_dt)).to(like.device)
+        _GAUSSIAN_TAPS[key] = t
+    return t
+
+
+@dataclass
+class Gaussian1d:
+    """Settings of the 1-D Gaussian filter (gaussian.rs:66-87)."""
+
+    axis: int = -1
+    order: int = 0
+    mode: PadMode | str = PadMode.Reflect
+    cval: float = 0.0
+    truncate: float = 4.0
+    radius: int | None = None
+
+    def resolved_radius(self, sigma: float) -> int:
+        import math
+
+        return int(self.radius) if self.radius is not None else max(int(math.floor(self.truncate * sigma + 0.5)), 0)
+
+
+def gaussian_1d_into(input, sigma: float, output, settings: Gaussian1d | None = None,
+                     client: N.Client | None = None) -> None:
+    """gaussian_1d_into (gaussian.rs:88-118): sigma finite and >= 0; 0 copies; otherwise the
+    correlation with gaussian_kernel_1d along the axis."""
+    import math
+
+    import torch
+
+    s = settings or Gaussian1d()
+    if not math.isfinite(sigma) or sigma < 0.0:
+        raise N.LucemError(5, f"invalid parameter `sigma`: sigma must be non-negative and finite, got {sigma}")
+    x, o = N.as_tensor(input), N.as_tensor(output)
+    if sigma == 0.0:
+        _copy_into(x, o, client)
+        return
+    wt = _gaussian_taps(sigma, int(s.order), s.resolved_radius(sigma), x)
+    correlate_1d_into(x, wt, o, Correlate1d(axis=s.axis, mode=s.mode, cval=s.cval, origin=0), client)
+
+
+def gaussian_1d(input, sigma: float, settings: Gaussian1d | None = None, client: N.Client | None = None):
+    import torch
+
+    x = N.as_tensor(input)
+    out = torch.empty_like(x)
+    gaussian_1d_into(x, sigma, out, settings, client)
+    return out
+
+
+@dataclass
+class GaussianBlur:
+    """Settings of the N-D separable Gaussian blur (gaussian.rs:133-171)."""
+
+    orders: list = field(default_factory=lambda: [0])
+    mode: PadMode | str = PadMode.Reflect
+    cval: float = 0.0
+    truncate: float = 4.0
+    radii: list | None = None
+    axes: list | None = None
+    channel_axis: int | None = None
+
+
+def _client_stream(x, client):
+    """A torch stream context on ``client``'s HIP stream, so that the copies and scratch arrays of
+    the separable passes are ordered with the library's own work on that stream."""
+    import contextlib
+
+    import torch
+
+    if not x.is_cuda:  # host arrays: the library reports them
+        return client, contextlib.nullcontext()
+    if client is None:
+        client = N.default_client(x.device.index)
+    return client, torch.cuda.stream(torch.cuda.ExternalStream(client.stream, device=x.device))
+
+
+def _copy_into(x, o, client) -> None:
+    """elementwise::copy_into (core/elementwise.rs): same shape, then a device copy."""
+    if tuple(x.shape) != tuple(o.shape):
+        raise N.LucemError(1, f"input shape {list(x.shape)} does not match output shape {list(o.shape)}")
+    if x.numel() == 0:
+        return
+    N.device_ptr(x, "input", x.dtype)
+    N.device_ptr(o, "output", x.dtype)
+    client, ctx = _client_stream(x, client)
+    with ctx:
+        o.copy_(x)
+
+
+def _broadcast(name: str, values, n: int) -> list:
+    """check::broadcast_per_axis (core/check.rs:92-108)."""
+    values = list(values)
+    if n == 0:
+        return []
+    if len(values) == 1:
+        return values * n
+    if len(values) == n:
+        return values
+    raise N.LucemError(5, f"invalid parameter `{name}`: expected 1 or {n} values, got {len(values)}")
+
+
+def gaussian_blur_into(input, sigmas, output, settings: GaussianBlur | None = None,
+                       client: N.Client | None = None) -> None:
+    """gaussian_blur_into (gaussian.rs:174-238): one 1-D Gaussian pass per axis with sigma > 0,
+    chained through two scratch arrays as core/separable.rs:8-50 does (0 passes: a copy)."""
+    import math
+
+    import torch
+
+    s = settings or GaussianBlur()
+    x, o = N.as_tensor(input), N.as_tensor(output)
+    nd = x.dim()
+
+    def norm_axis(a):
+        # Shape::axis / Shape::axes (lucem-core/src/shape.rs:50-75), numpy's messages
+        a = int(a)
+        if a < -nd or a >= nd:
+            raise N.LucemError(12, f"axis {a} is out of bounds for array of dimension {nd}")
+        return a + nd if a < 0 else a
+
+    axes = list(range(nd)) if s.axes is None else []
+    if s.axes is not None:
+        for a in s.axes:
+            na = norm_axis(a)
+            if na in axes:
+                raise N.LucemError(5, f"repeated axis {a}")
+            axes.append(na)
+    if s.channel_axis is not None:
+        ch = norm_axis(s.channel_axis)
+        axes = [a for a in axes if a != ch]
+    sig = _broadcast("sigmas", sigmas if hasattr(sigmas, "__len__") else [sigmas], len(axes))
+    for v in sig:
+        if not math.isfinite(float(v)) or float(v) < 0.0:
+            raise N.LucemError(5, f"invalid parameter `sigmas`: sigma must be non-negative and finite, got {v}")
+    orders = _broadcast("orders", s.orders, len(axes))
+    radii = _broadcast("radii", s.radii, len(axes)) if s.radii is not None else [None] * len(axes)
+    passes = [(a, float(v), int(od), r) for a, v, od, r in zip(axes, sig, orders, radii) if float(v) > 0.0]
+    if tuple(x.shape) != tuple(o.shape):
+        raise N.LucemError(1, f"input shape {list(x.shape)} does not match output shape {list(o.shape)}")
+    if x.numel() and x.data_ptr() == o.data_ptr():
+        raise N.LucemError(2, "input and output arrays must not share the same device buffer")
+    if x.numel() == 0:
+        return
+    if not passes:
+        _copy_into(x, o, client)
+        return
+    client, ctx = _client_stream(x, client)
+
+    def step(src, dst, p):
+        a, v, od, r = p
+        gaussian_1d_into(src, v, dst, Gaussian1d(axis=a, order=od, mode=s.mode, cval=s.cval, truncate=s.truncate,
+                                                 radius=r), client)
+
+    with ctx:
+        bufs = [torch.empty_like(x) for _ in range(min(len(passes) - 1, 2))]
+    src = x
+    for i, p in enumerate(passes):
+        dst = o if i == len(passes) - 1 else bufs[i % 2]
+        step(src, dst, p)
+        src = dst
+
+
+def gaussian_blur(input, sigmas, settings: GaussianBlur | None = None, client: N.Client | None = None):
+    import torch
+
+    x = N.as_tensor(input)
+    out = torch.empty_like(x)
+    gaussian_blur_into(x, sigmas, out, settings, client)
+    return out
