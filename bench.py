@@ -353,11 +353,16 @@ def filter_leg(steps: int, with_cpu: bool) -> dict:
     tgb = _timeit(lambda: LA.gabor_2d(x, 0.1, 0.5, 3.0, 3.0), max(steps, 5))
     w5 = torch.from_numpy(_gaussian_psf(5, 1.0)).cuda()
     t5 = _timeit(lambda: LA.correlate_2d_into(x, w5, o, s, ws), max(steps, 10))
+    gs = LA.GaussianBlur()
+    tgs = _timeit(lambda: LA.gaussian_blur_into(x, [2.0], o, gs), max(steps, 10))
     out = {
         "correlate_2048_w31_ms": round(tc * 1e3, 4),
         "correlate_note": "2048x2048 f32, 31x31 window, Reflect: transfer function built in the call",
         "correlate_2048_w5_ms": round(t5 * 1e3, 4),
         "correlate_w5_GBps": round(8 * x.numel() / t5 / 1e9, 1),
+        "gaussian_blur_2048_s2_ms": round(tgs * 1e3, 4),
+        "gaussian_blur_s2_GBps": round(16 * x.numel() / tgs / 1e9, 1),
+        "gaussian_note": "sigma 2 (17 taps) per axis, Reflect: two 1-D passes, 8 B per element each",
         "phase_correlation_1024_u20_ms": round(tpc * 1e3, 3),
         "butterworth_2048_ms": round(tbw * 1e3, 4),
         "gabor_2048_ms": round(tgb * 1e3, 4),

@@ -59,9 +59,41 @@ int launch_add_weight_sum(float* out, const double* sum, double cval, int64_t n,
 }
 
 
+// pad_index (core/kernels/pad.rs:11-34) in 64-bit arithmetic, for axes up to the element limit;
+// -1 for Constant outside [0, len).
+__device__ __forceinline__ int64_t pad_index64(int64_t p, int64_t len, int mode) {
+    if (p >= 0 && p < len) return p;
+    switch (mode) {
+        case 1: return -1;
+        case 2: return p < 0 ? 0 : len - 1;
+        case 4: return ((p % len) + len) % len;
+        case 0: {
+            const int64_t period = 2 * len;
+            const int64_t r = ((p % period) + period) % period;
+            return r < len ? r : period - 1 - r;
+        }
+        default: {
+            const int64_t period = len > 1 ? 2 * len - 2 : 1;
+            const int64_t r = ((p % period) + period) % period;
+            return r < len ? r : period - r;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T padded_sample(const T* __restrict__ line, int64_t pos, int64_t len, int64_t stride,
+                                           T cval, int mode) {
+    const int64_t q = pad_index64(pos, len, mode);
+    return q < 0 ? cval : line[q * stride];
+}
+
 // correlate_1d (filter/kernels/convolve.rs:8-55): along an axis seen as [outer, len, inner],
-// out[o, i, k] = sum_t w[t] * P(in)[o, i + t - left, k], accumulated in T in window order, with
-// the interior fast path of the reference (no boundary arithmetic when the window fits).
+// out[o, i, k] = sum_t w[t] * P(in)[o, i + t - left, k], accumulated in T in window order. Every
+// form below reads the same padded samples and sums them in the same order, so they agree bit
+// for bit.
+//
+// The general form: one output per thread, each tap one global load (used for short lines and
+// narrow inner extents, and for windows too long for the LDS forms).
 template <typename T>
 __global__ void correlate1d_kernel(const T* __restrict__ in, T* __restrict__ out, const T* __restrict__ w,
                                    int64_t total, int64_t len, int64_t inner, int64_t left, T cval, int size,
@@ -75,21 +107,125 @@ __global__ void correlate1d_kernel(const T* __restrict__ in, T* __restrict__ out
             const T* p = in + base + start * inner;
             for (int t = 0; t < size; t++) acc += w[t] * p[(int64_t)t * inner];
         } else {
-            for (int t = 0; t < size; t++) {
-                const int q = pad_index((int)(start + t), (int)len, mode);
-                const T sample = q < 0 ? cval : in[base + (int64_t)q * inner];
-                acc += w[t] * sample;
-            }
+            for (int t = 0; t < size; t++) acc += w[t] * padded_sample(in + base, start + t, len, inner, cval, mode);
         }
         out[e] = acc;
+    }
+}
+
+constexpr int kRowMaxOut = 1024;  // outputs per block along a contiguous line (up to 4 per thread)
+constexpr int kColLanes = 64; // inner positions per block (one per lane of a wave)
+constexpr int kColRows = 64;  // outputs per block along a strided axis
+constexpr size_t kCorr1dLds = 65536;
+
+// The last axis (inner = 1): a block stages the weights and the 256 U + size - 1 padded samples of
+// one line segment in LDS with coalesced loads, then each thread sums U windows from LDS (at its
+// index and every 256 further on). U = 4, 2 or 1 as the line holds 1024, 512 or 256 outputs.
+template <typename T, int U>
+__global__ __launch_bounds__(256) void correlate1d_row_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                              const T* __restrict__ w, int64_t len, uint32_t nseg,
+                                                              int64_t left, T cval, int size, int mode) {
+    extern __shared__ __align__(16) unsigned char corr1d_raw[];
+    T* tile = reinterpret_cast<T*>(corr1d_raw);
+    constexpr int kOut = 256 * U;
+    T* wl = tile + (kOut + size - 1);
+    const uint32_t seg = blockIdx.x % nseg, line = blockIdx.x / nseg;
+    const T* src = in + (int64_t)line * len;
+    const int64_t i0 = (int64_t)seg * kOut;
+    for (int t = threadIdx.x; t < size; t += 256) wl[t] = w[t];
+    const int span = kOut + size - 1;
+    for (int j = threadIdx.x; j < span; j += 256) {
+        const int64_t pos = i0 - left + j;
+        tile[j] = (pos >= 0 && pos < len) ? src[pos] : padded_sample(src, pos, len, (int64_t)1, cval, mode);
+    }
+    __syncthreads();
+    T acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = T(0);
+    const T* p = tile + threadIdx.x;
+    for (int t = 0; t < size; t++) {
+        const T wt = wl[t];
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[u] += wt * p[t + 256 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t i = i0 + threadIdx.x + 256 * u;
+        if (i < len) out[(int64_t)line * len + i] = acc[u];
+    }
+}
+
+// A strided axis (inner >= 64): a block of four waves covers 64 consecutive inner positions (one
+// per lane, so every global access is a coalesced 64-wide row) and 64 outputs along the axis. It
+// stages the 64 + size - 1 padded rows in LDS, rows interleaved over the waves, then each wave
+// sums 16 outputs from LDS (a lane reads its own column: no bank conflicts).
+template <typename T>
+__global__ __launch_bounds__(256) void correlate1d_col_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                              const T* __restrict__ w, int64_t len, int64_t inner,
+                                                              uint32_t nkb, uint32_t nib, int64_t left, T cval,
+                                                              int size, int mode) {
+    extern __shared__ __align__(16) unsigned char corr1d_raw[];
+    T* tile = reinterpret_cast<T*>(corr1d_raw);
+    T* wl = tile + (int64_t)(kColRows + size - 1) * kColLanes;
+    uint32_t b = blockIdx.x;
+    const uint32_t kb = b % nkb;
+    b /= nkb;
+    const uint32_t ib = b % nib, o = b / nib;
+    const int lane = threadIdx.x & (kColLanes - 1), wave = threadIdx.x / kColLanes;
+    const int64_t k = (int64_t)kb * kColLanes + lane;
+    const T* src = in + (int64_t)o * len * inner + k;
+    T* dst = out + (int64_t)o * len * inner + k;
+    const int64_t i0 = (int64_t)ib * kColRows;
+    for (int t = threadIdx.x; t < size; t += 256) wl[t] = w[t];
+    const int rows = kColRows + size - 1;
+    if (k < inner) {
+        for (int r = wave; r < rows; r += 4) {
+            const int64_t pos = i0 - left + r;
+            tile[r * kColLanes + lane] =
+                (pos >= 0 && pos < len) ? src[pos * inner] : padded_sample(src, pos, len, inner, cval, mode);
+        }
+    }
+    __syncthreads();
+    if (k < inner) {
+        for (int r = wave; r < kColRows; r += 4) {
+            const int64_t i = i0 + r;
+            if (i >= len) break;
+            const T* p = tile + r * kColLanes + lane;
+            T acc = T(0);
+            for (int t = 0; t < size; t++) acc += wl[t] * p[t * kColLanes];
+            dst[i * inner] = acc;
+        }
     }
 }
 
 template <typename T>
 static int correlate1d(const T* in, T* out, const T* w, int64_t total, int64_t len, int64_t inner, int64_t left,
                        T cval, int size, int mode, void* stream) {
-    hipLaunchKernelGGL(correlate1d_kernel<T>, stride_grid(total), dim3(256), 0, (hipStream_t)stream, in, out, w, total,
-                       len, inner, left, cval, size, mode);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t outer = total / (len * inner);
+    const int64_t max_blocks = ((int64_t)1 << 31) / 256;
+    if (inner == 1 && len >= 256) {
+        const int u = len >= 1024 ? 4 : (len >= 512 ? 2 : 1);
+        const size_t lds = (size_t)(256 * u + 2 * size - 1) * sizeof(T);
+        const int64_t nseg = (len + 256 * u - 1) / (256 * u);
+        if (lds <= kCorr1dLds && nseg * outer <= max_blocks) {
+            auto kernel = u == 4 ? correlate1d_row_kernel<T, 4>
+                                 : (u == 2 ? correlate1d_row_kernel<T, 2> : correlate1d_row_kernel<T, 1>);
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(nseg * outer)), dim3(256), lds, st, in, out, w, len,
+                               (uint32_t)nseg, left, cval, size, mode);
+            return (int)hipGetLastError();
+        }
+    } else if (inner >= kColLanes) {
+        const size_t lds = ((size_t)(kColRows + size - 1) * kColLanes + size) * sizeof(T);
+        const int64_t nkb = (inner + kColLanes - 1) / kColLanes, nib = (len + kColRows - 1) / kColRows;
+        if (lds <= kCorr1dLds && nkb * nib * outer <= max_blocks) {
+            hipLaunchKernelGGL(correlate1d_col_kernel<T>, dim3((unsigned)(nkb * nib * outer)), dim3(256), lds, st, in,
+                               out, w, len, inner, (uint32_t)nkb, (uint32_t)nib, left, cval, size, mode);
+            return (int)hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL(correlate1d_kernel<T>, stride_grid(total), dim3(256), 0, st, in, out, w, total, len, inner, left,
+                       cval, size, mode);
     return (int)hipGetLastError();
 }
 

@@ -374,6 +374,8 @@ def _client_stream(x, client):
         return client, contextlib.nullcontext()
     if client is None:
         client = N.default_client(x.device.index)
+    if client.stream == torch.cuda.current_stream(x.device).cuda_stream:
+        return client, contextlib.nullcontext()  # the default client: already torch's stream
     return client, torch.cuda.stream(torch.cuda.ExternalStream(client.stream, device=x.device))
 
 

@@ -15,6 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 MODES = [0, 1, 2, 3, 4]
+SCIPY_MODES = {0: "reflect", 1: "constant", 2: "nearest", 3: "mirror", 4: "wrap"}
 
 
 def _dev(a, dtype=np.float32):
@@ -244,3 +245,34 @@ def test_convolve_1d_is_the_mirrored_correlation(gpu, oracle):
     with pytest.raises(LA.LucemError) as e:
         LA.correlate_1d(torch.from_numpy(x).cuda(), [1.0], LA.Correlate1d(axis=2))
     assert e.value.kind == "AXIS_OUT_OF_BOUNDS"
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_correlate_1d_tiled_forms_match_scipy(gpu, mode):
+    """The LDS-tiled forms of the 1-D correlation: the last axis with lines longer than 64
+    (segments of 256 and a ragged end), strided axes with inner extents of 64 and more (64-wide
+    column blocks, ragged in both directions), windows up to and past what the tiles hold (the
+    general form takes those), origins and cval, against scipy.ndimage.correlate1d in FP64
+    (same origin and mode meanings): f32 to 1e-5 rel L2, f64 to 1e-12."""
+    import torch
+    from scipy import ndimage as ndi
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(11)
+    for dims in [(3, 300, 130), (70, 1000), (4, 2500), (5, 300), (2, 65, 64, 3)]:
+        x = rng.uniform(-1.0, 1.0, dims)
+        for axis in range(len(dims)):
+            for size, origin in [(5, 0), (17, -3), (70, 20), (200, 0)]:
+                if size > dims[axis] * 4:
+                    continue
+                w = rng.normal(size=size)
+                print(f"correlate_1d {dims} axis {axis} size {size} origin {origin} {mode}", flush=True)
+                want = ndi.correlate1d(x, w, axis=axis, mode=SCIPY_MODES[mode], cval=0.5, origin=origin)
+                s = LA.Correlate1d(axis=axis, mode=mode, cval=0.5, origin=origin)
+                got32 = LA.correlate_1d(torch.from_numpy(x.astype(np.float32)).cuda(),
+                                        w.astype(np.float32).tolist(), s).cpu().numpy()
+                err = np.linalg.norm(got32 - want) / np.linalg.norm(want)
+                assert err < 1e-5, (dims, axis, size, origin, err)
+                got64 = LA.correlate_1d(torch.from_numpy(x).cuda(), w.tolist(), s).cpu().numpy()
+                np.testing.assert_allclose(got64, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
