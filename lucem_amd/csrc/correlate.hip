@@ -115,7 +115,6 @@ __global__ void correlate1d_kernel(const T* __restrict__ in, T* __restrict__ out
 
 constexpr int kRowMaxOut = 1024;  // outputs per block along a contiguous line (up to 4 per thread)
 constexpr int kColLanes = 64; // inner positions per block (one per lane of a wave)
-constexpr int kColRows = 64;  // outputs per block along a strided axis
 constexpr size_t kCorr1dLds = 65536;
 
 // The last axis (inner = 1): a block stages the weights and the 256 U + size - 1 padded samples of
@@ -156,10 +155,11 @@ __global__ __launch_bounds__(256) void correlate1d_row_kernel(const T* __restric
 }
 
 // A strided axis (inner >= 64): a block of four waves covers 64 consecutive inner positions (one
-// per lane, so every global access is a coalesced 64-wide row) and 64 outputs along the axis. It
-// stages the 64 + size - 1 padded rows in LDS, rows interleaved over the waves, then each wave
-// sums 16 outputs from LDS (a lane reads its own column: no bank conflicts).
-template <typename T>
+// per lane, so every global access is a coalesced 64-wide row) and ROWS outputs along the axis
+// (64, or 32 where 64 would leave fewer than 8 blocks per CU). It stages the ROWS + size - 1
+// padded rows in LDS, rows interleaved over the waves, then each wave sums ROWS / 4 outputs from
+// LDS (a lane reads its own column: no bank conflicts).
+template <typename T, int kColRows>
 __global__ __launch_bounds__(256) void correlate1d_col_kernel(const T* __restrict__ in, T* __restrict__ out,
                                                               const T* __restrict__ w, int64_t len, int64_t inner,
                                                               uint32_t nkb, uint32_t nib, int64_t left, T cval,
@@ -216,11 +216,14 @@ static int correlate1d(const T* in, T* out, const T* w, int64_t total, int64_t l
             return (int)hipGetLastError();
         }
     } else if (inner >= kColLanes) {
-        const size_t lds = ((size_t)(kColRows + size - 1) * kColLanes + size) * sizeof(T);
-        const int64_t nkb = (inner + kColLanes - 1) / kColLanes, nib = (len + kColRows - 1) / kColRows;
+        const int64_t nkb = (inner + kColLanes - 1) / kColLanes;
+        const int rows = nkb * ((len + 63) / 64) * outer >= 2048 ? 64 : 32;
+        const size_t lds = ((size_t)(rows + size - 1) * kColLanes + size) * sizeof(T);
+        const int64_t nib = (len + rows - 1) / rows;
         if (lds <= kCorr1dLds && nkb * nib * outer <= max_blocks) {
-            hipLaunchKernelGGL(correlate1d_col_kernel<T>, dim3((unsigned)(nkb * nib * outer)), dim3(256), lds, st, in,
-                               out, w, len, inner, (uint32_t)nkb, (uint32_t)nib, left, cval, size, mode);
+            auto kernel = rows == 64 ? correlate1d_col_kernel<T, 64> : correlate1d_col_kernel<T, 32>;
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(nkb * nib * outer)), dim3(256), lds, st, in, out, w, len,
+                               inner, (uint32_t)nkb, (uint32_t)nib, left, cval, size, mode);
             return (int)hipGetLastError();
         }
     }

@@ -276,3 +276,23 @@ def test_correlate_1d_tiled_forms_match_scipy(gpu, mode):
                 assert err < 1e-5, (dims, axis, size, origin, err)
                 got64 = LA.correlate_1d(torch.from_numpy(x).cuda(), w.tolist(), s).cpu().numpy()
                 np.testing.assert_allclose(got64, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("mode", [0, 4])
+def test_correlate_1d_large_strided_axis(gpu, mode):
+    """A strided axis with enough blocks for the 64-row column tiles (4100 x 2100 along axis 0,
+    ragged in both directions), against scipy.ndimage.correlate1d in FP64 to 1e-5 rel L2."""
+    import torch
+    from scipy import ndimage as ndi
+
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1.0, 1.0, (4100, 2100))
+    w = rng.normal(size=17)
+    print(f"correlate_1d (4100, 2100) axis 0 size 17 origin -3 mode {mode}", flush=True)
+    want = ndi.correlate1d(x, w, axis=0, mode=SCIPY_MODES[mode], cval=0.0, origin=-3)
+    s = LA.Correlate1d(axis=0, mode=mode, origin=-3)
+    got = LA.correlate_1d(torch.from_numpy(x.astype(np.float32)).cuda(), w.astype(np.float32).tolist(),
+                          s).cpu().numpy()
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
