@@ -431,10 +431,9 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     hipStream_t s = client->stream;
     auto up = [&](void* dst, const void* src, size_t bytes) -> lucem_status {
         if (!bytes) return LUCEM_OK;
-        hipError_t err = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+        hipError_t err = upload_sync(dst, src, bytes, s);
         return err == hipSuccess ? LUCEM_OK : hip_fail(err, "hipMemcpy(batch descriptors)");
     };
-    (void)s;
     if ((st = up(P.otf_rows, otf_rows.data(), sizeof(RowPass) * otf_rows.size()))) return st;
     for (int i = 0; i < notf; i++)
         if ((st = up(P.otf_cols[i], otf_cols[i].data(), sizeof(ColPass) * otf_cols[i].size()))) return st;

@@ -78,7 +78,7 @@ lucem_status chirp_tables(lucem_client* c, int n, int M, const float2** chirp, c
         float2* dev = nullptr;
         hipError_t e = hipMalloc(&dev, sizeof(float2) * host.size());
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(chirp tables)");
-        e = hipMemcpy(dev, host.data(), sizeof(float2) * host.size(), hipMemcpyHostToDevice);
+        e = upload_sync(dev, host.data(), sizeof(float2) * host.size(), c->stream);
         if (e != hipSuccess) {
             (void)hipFree(dev);
             return hip_fail(e, "hipMemcpy(chirp tables)");
@@ -196,7 +196,7 @@ lucem_status long_twiddles(lucem_client* c, int64_t n, const float2** out) {
         float2* dev = nullptr;
         hipError_t e = hipMalloc(&dev, sizeof(float2) * host.size());
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(four-step twiddles)");
-        e = hipMemcpy(dev, host.data(), sizeof(float2) * host.size(), hipMemcpyHostToDevice);
+        e = upload_sync(dev, host.data(), sizeof(float2) * host.size(), c->stream);
         if (e != hipSuccess) {
             (void)hipFree(dev);
             return hip_fail(e, "hipMemcpy(four-step twiddles)");

@@ -507,7 +507,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
 
     auto up = [&](void* dst, const void* src, size_t bytes) -> lucem_status {
         if (!bytes) return LUCEM_OK;
-        hipError_t err = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+        hipError_t err = upload_sync(dst, src, bytes, client->stream);
         return err == hipSuccess ? LUCEM_OK : hip_fail(err, "hipMemcpy(gradient descriptors)");
     };
     lucem_status st;

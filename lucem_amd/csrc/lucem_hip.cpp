@@ -271,7 +271,7 @@ lucem_status lucem::twiddles_for(lucem_client* c, int L, const float2** out) {
     float2* dev = nullptr;
     hipError_t e = hipMalloc(&dev, sizeof(float2) * (size_t)std::max(L, 1));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
-    e = hipMemcpy(dev, host.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice);
+    e = upload_sync(dev, host.data(), sizeof(float2) * (size_t)L, c->stream);
     if (e != hipSuccess) {
         (void)hipFree(dev);
         return hip_fail(e, "hipMemcpy(twiddles)");

@@ -15,6 +15,14 @@
 struct lucem_workspace;
 
 namespace lucem {
+
+// Host-to-device upload ordered on `s` and waited for. Unlike hipMemcpy it does not touch the
+// legacy stream, so it may run while another thread captures a graph.
+inline hipError_t upload_sync(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 // One profiled launch: events on the client stream around it and its algorithmic HBM bytes.
 struct ProfRecord {
     const char* name;

@@ -245,9 +245,9 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                     int2* di = reinterpret_cast<int2*>(scratch.p + ((bd + 255) & ~size_t(255)));
                     int* cnt = reinterpret_cast<int*>(scratch.p + ((bd + 255) & ~size_t(255)) + ((bi + 255) & ~size_t(255)));
                     hipError_t e;
-                    if ((e = hipMemcpy(dd, d.data(), bd, hipMemcpyHostToDevice)) != hipSuccess)
+                    if ((e = upload_sync(dd, d.data(), bd, s)) != hipSuccess)
                         return hip_fail(e, "hipMemcpy(autotune)");
-                    if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
+                    if ((e = upload_sync(di, items.data(), bi, s)) != hipSuccess)
                         return hip_fail(e, "hipMemcpy(autotune)");
                     const int grid = mode == TUNE_BATCH_DIRECT ? (int)items.size()
                                                                : std::max(1, std::min((int)items.size(), batch_occupancy(true, w, lds) * c->num_cus));
@@ -365,9 +365,9 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         int* cnt =
                             reinterpret_cast<int*>(scratch.p + ((bd + 255) & ~size_t(255)) + ((bi + 255) & ~size_t(255)));
                         hipError_t e;
-                        if ((e = hipMemcpy(dd, d.data(), bd, hipMemcpyHostToDevice)) != hipSuccess)
+                        if ((e = upload_sync(dd, d.data(), bd, s)) != hipSuccess)
                             return hip_fail(e, "hipMemcpy(autotune)");
-                        if ((e = hipMemcpy(di, items.data(), bi, hipMemcpyHostToDevice)) != hipSuccess)
+                        if ((e = upload_sync(di, items.data(), bi, s)) != hipSuccess)
                             return hip_fail(e, "hipMemcpy(autotune)");
                         const int grid = mode == TUNE_BATCH_DIRECT
                                              ? (int)items.size()
