@@ -8,7 +8,7 @@
 // accumulated in T, ky-major as in filter/kernels/convolve.rs:61-118, Constant reading `cval`
 // outside the array (0 in the Richardson-Lucy iteration, deconvolve.rs:99-103). A 16x16 output
 // tile per workgroup reads its input tile with the halo into LDS once when it fits (64 KiB),
-// otherwise straight from global memory.
+// otherwise straight from global memory; arrays 64 or more wide take 64x16 tiles when those fit.
 #include <hip/hip_runtime.h>
 
 #include "fft_lds.hpp"
@@ -61,6 +61,48 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_kernel(const T* __re
             }
             out[(int64_t)y * W + x] = acc;
         }
+    }
+}
+
+// The same sum over a 64x16 output tile: each thread keeps four outputs 16 apart along x, so the
+// halo is staged once per 1024 outputs instead of per 256. Each output still sums ky-major in
+// window order, so the result is the 16x16 form's bit for bit.
+constexpr int kWide = 64;
+
+template <typename T>
+__global__ __launch_bounds__(kTile* kTile) void correlate2d_wide_kernel(const T* __restrict__ in,
+                                                                        const T* __restrict__ w, T* __restrict__ out,
+                                                                        int H, int W, int kh, int kw, int ly, int lx,
+                                                                        int mode, T cval) {
+    extern __shared__ __align__(16) unsigned char tile_raw[];
+    T* tile = reinterpret_cast<T*>(tile_raw);
+    const int nbx = (W + kWide - 1) / kWide;
+    const int bx = (int)(blockIdx.x % (unsigned)nbx), by = (int)(blockIdx.x / (unsigned)nbx);
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const int x0 = bx * kWide, y0 = by * kTile;
+    const int y = y0 + ty;
+    const int tw = kWide + kw - 1, th = kTile + kh - 1;
+    for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
+        const int r = i / tw, c = i - r * tw;
+        const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
+        tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
+    }
+    __syncthreads();
+    if (y >= H) return;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    for (int ky = 0; ky < kh; ky++) {
+        const T* row = tile + (ty + ky) * tw + tx;
+        const T* wr = w + ky * kw;
+        for (int kx = 0; kx < kw; kx++) {
+            const T wt = wr[kx];
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] += wt * row[kx + kTile * u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int x = x0 + tx + kTile * u;
+        if (x < W) out[(int64_t)y * W + x] = acc[u];
     }
 }
 
@@ -121,6 +163,13 @@ template <typename T>
 static int correlate2d(const T* in, const T* w, T* out, int H, int W, int kh, int kw, int ly, int lx, int mode, T cval,
                        void* stream) {
     hipStream_t st = (hipStream_t)stream;
+    const size_t wide_lds = (size_t)(kWide + kw - 1) * (kTile + kh - 1) * sizeof(T);
+    if (W >= kWide && wide_lds <= kDirectTileLds) {
+        const dim3 wgrid((unsigned)(((int64_t)W + kWide - 1) / kWide * (((int64_t)H + kTile - 1) / kTile)));
+        hipLaunchKernelGGL(correlate2d_wide_kernel<T>, wgrid, dim3(kTile * kTile), wide_lds, st, in, w, out, H, W, kh,
+                           kw, ly, lx, mode, cval);
+        return (int)hipGetLastError();
+    }
     const dim3 grid((unsigned)(((int64_t)W + kTile - 1) / kTile * (((int64_t)H + kTile - 1) / kTile)));
     const size_t lds = (size_t)(kTile + kw - 1) * (kTile + kh - 1) * sizeof(T);
     if (lds <= kDirectTileLds)
