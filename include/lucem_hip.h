@@ -180,6 +180,24 @@ lucem_status lucem_richardson_lucy_into_f32(lucem_client* client, const float* i
                                             uint32_t out_ndim, const lucem_rl_settings* settings,
                                             lucem_workspace* ws);
 
+/* Allocation-free variant of lucem_richardson_lucy_into_f32. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same checks in the same order and the same launches,
+ * but every buffer beyond input, psf and output lives in one fixed device array per GPU of
+ * LUCEM_NOALLOC_SCRATCH_BYTES bytes, held by the library until the process ends (set up when a
+ * client is first created on the device). The call creates no buffer, event or stream and takes
+ * no workspace; launch shapes are the heuristics or the ones this client already measured for the
+ * geometry; profiling records nothing. A problem whose buffers exceed the scratch returns
+ * LUCEM_UNSUPPORTED_SHAPE (a 1024 x 1024 image with a 25 x 25 PSF needs about 18 MB, 2048 x 2048
+ * about 70 MB). All calls on one GPU share its scratch: calls that overlap on one GPU race. */
+#define LUCEM_NOALLOC_SCRATCH_BYTES (96ull * 1024ull * 1024ull)
+lucem_status lucem_richardson_lucy_into_noalloc_f32(lucem_client* client, const float* input, const uint64_t* in_dims,
+                                                    uint32_t in_ndim, const float* psf, const uint64_t* psf_dims,
+                                                    uint32_t psf_ndim, float* output, const uint64_t* out_dims,
+                                                    uint32_t out_ndim, const lucem_rl_settings* settings);
+/* Size of that scratch in bytes (LUCEM_NOALLOC_SCRATCH_BYTES). */
+uint64_t lucem_noalloc_scratch_bytes(void);
+
 /* richardson_lucy::<f32> with a caller-allocated output and the client's internal workspace. */
 lucem_status lucem_richardson_lucy_f32(lucem_client* client, const float* input, const uint64_t* dims,
                                        uint32_t ndim, const float* psf, const uint64_t* psf_dims, uint32_t psf_ndim,

@@ -56,6 +56,7 @@ EXPORTED = [
     "lucem_fourier_gaussian_into_f32", "lucem_fourier_box_into_f32", "lucem_fourier_shift_into_f32",
     "lucem_butterworth_f32", "lucem_gabor_settings_default", "lucem_gabor_2d_f32",
     "lucem_correlate1d_settings_default", "lucem_correlate_1d_into_f32", "lucem_correlate_1d_into_f64",
+    "lucem_richardson_lucy_into_noalloc_f32", "lucem_noalloc_scratch_bytes",
 ]
 
 
@@ -174,6 +175,9 @@ def lib() -> ctypes.CDLL:
             L.lucem_richardson_lucy_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, _u64p, ctypes.c_uint32,
                                                     _vp, ctypes.POINTER(RlSettings)]
             L.lucem_richardson_lucy_into_f64.argtypes = rl
+            L.lucem_richardson_lucy_into_noalloc_f32.argtypes = rl[:-1]
+            L.lucem_noalloc_scratch_bytes.argtypes = []
+            L.lucem_noalloc_scratch_bytes.restype = ctypes.c_uint64
             L.lucem_richardson_lucy_f64.argtypes = L.lucem_richardson_lucy_f32.argtypes
             L.lucem_fft_c2c_lines_f32.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32]
             L.lucem_add_noise_f32.argtypes = [_vp, _vp, _u64p, ctypes.c_uint32, _vp, ctypes.POINTER(NoiseModelC),

@@ -111,6 +111,7 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
         if (!lds_err) lds_err = allow_big_lds_grad_kernels();
     });
     if (lds_err) return hip_fail(lds_err, "hipFuncSetAttribute(dynamic LDS)");
+    noalloc_setup(device);
     if (!own_stream) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -230,17 +231,23 @@ double lucem::col_pass_bytes(const ColPass& p, int blocks) {
     return b;
 }
 
+// Calls that may not create events (the allocation-free entry point) record nothing.
+static thread_local int g_no_profile = 0;
+lucem::NoProfileScope::NoProfileScope() { g_no_profile++; }
+lucem::NoProfileScope::~NoProfileScope() { g_no_profile--; }
+
 template <class Launch>
 static lucem_status profiled(lucem_client* c, const char* name, double bytes, Launch&& launch) {
     hipEvent_t a = nullptr, b = nullptr;
     lucem_status st;
-    if (c->profiling) {
+    const bool prof = c->profiling && g_no_profile == 0;
+    if (prof) {
         if ((st = take_event(c, &a)) || (st = take_event(c, &b))) return st;
         (void)hipEventRecord(a, c->stream);
     }
     const int err = launch();
     if (err != hipSuccess) return hip_fail(err, name);
-    if (c->profiling) {
+    if (prof) {
         (void)hipEventRecord(b, c->stream);
         c->records.push_back(ProfRecord{name, a, b, bytes});
     }
@@ -724,56 +731,44 @@ float lucem::rl_eps(const lucem_rl_settings& s) {
     return (float)std::max(s.has_filter_epsilon ? s.filter_epsilon : 1.0e-12, 1.0e-15);
 }
 
-lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
-                           const Geometry& g, const lucem_rl_settings& s) {
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
-    const int64_t nk = (int64_t)g.k[0] * g.k[1] * g.k[2];
-    const int64_t rows = (int64_t)g.n[0] * g.n[1];
-    const int64_t krows = (int64_t)g.k[0] * g.k[1];
+RlSizes lucem::rl_sizes(const Geometry& g) {
     const size_t c8 = sizeof(float2);
-    float *est, *h, *rev;
-    float2 *sa, *sb = nullptr, *otf1, *otf2, *pa, *pb = nullptr;
-    lucem_status st;
-    if ((st = ws_slot(ws, "rl_est", sizeof(float) * n, (void**)&est))) return st;
-    if ((st = ws_slot(ws, "rl_spec_a", c8 * rows * g.pitch, (void**)&sa))) return st;
-    if (g.ndim == 3 && (st = ws_slot(ws, "rl_spec_b", c8 * (int64_t)g.n[0] * g.L[1] * g.pitch, (void**)&sb)))
-        return st;
-    const int64_t otf_elems = (int64_t)g.L[0] * g.L[1] * g.pitch;
-    if ((st = ws_slot(ws, "rl_otf_conv", c8 * otf_elems, (void**)&otf1))) return st;
-    if ((st = ws_slot(ws, "rl_otf_mirror", c8 * otf_elems, (void**)&otf2))) return st;
-    if ((st = ws_slot(ws, "rl_psf_h", sizeof(float) * nk, (void**)&h))) return st;
-    if ((st = ws_slot(ws, "rl_psf_rev", sizeof(float) * nk, (void**)&rev))) return st;
-    if ((st = ws_slot(ws, "rl_psf_spec_a", c8 * krows * g.pitch, (void**)&pa))) return st;
-    if (g.ndim == 3 &&
-        (st = ws_slot(ws, "rl_psf_spec_b", c8 * (int64_t)g.k[0] * g.L[1] * g.pitch, (void**)&pb)))
-        return st;
-    const float2* tws[3] = {nullptr, nullptr, nullptr};
-    for (int a = (g.ndim == 2 ? 1 : 0); a < 3; a++)
-        if ((st = twiddles_for(c, g.L[a], &tws[a]))) return st;
+    RlSizes z{};
+    z.est = sizeof(float) * (size_t)g.n[0] * g.n[1] * g.n[2];
+    z.sa = c8 * (size_t)g.n[0] * g.n[1] * g.pitch;
+    z.sb = g.ndim == 3 ? c8 * (size_t)g.n[0] * g.L[1] * g.pitch : 0;
+    z.otf = c8 * (size_t)g.L[0] * g.L[1] * g.pitch;
+    z.psf = sizeof(float) * (size_t)g.k[0] * g.k[1] * g.k[2];
+    z.pa = c8 * (size_t)g.k[0] * g.k[1] * g.pitch;
+    z.pb = g.ndim == 3 ? c8 * (size_t)g.k[0] * g.L[1] * g.pitch : 0;
+    return z;
+}
 
+lucem_status lucem::rl_launch(lucem_client* c, const Geometry& g, const lucem_rl_settings& s, const float* input,
+                              const float* psf, float* output, const RlBuffers& b, const float2* const tws[3],
+                              const ShapeTable* table) {
+    lucem_status st;
     // PSF normalisation and flip (deconvolve.rs:73-90), then the two transfer functions:
     // otf_conv for correlate(est, rev) (:120), otf_mirror for correlate(rel, h) (:133).
-    PsfPrep pp{nk, psf, h, rev};
+    const int64_t nk = (int64_t)g.k[0] * g.k[1] * g.k[2];
+    PsfPrep pp{nk, psf, b.h, b.rev};
     if ((st = launch_check(launch_psf_prep(pp, c->stream), "psf prep"))) return st;
-    if ((st = run_otf(c, g, rev, pa, pb, otf1, tws))) return st;
-    if ((st = run_otf(c, g, h, pa, pb, otf2, tws))) return st;
+    if ((st = run_otf(c, g, b.rev, b.pa, b.pb, b.otf1, tws))) return st;
+    if ((st = run_otf(c, g, b.h, b.pa, b.pb, b.otf2, tws))) return st;
 
     int row_blocks = 0;
-    ShapePolicy sp{c->num_cus, 0, 0, nullptr};
-    if ((st = tuned_shapes(c, g, est, sa, sb, otf1, tws, &sp.table))) return st;
-    RowPass r = make_row_desc(g, sp, tws[2], rl_eps(s), input, est, output, sa, &row_blocks);
+    const ShapePolicy sp{c->num_cus, 0, 0, table};
+    RowPass r = make_row_desc(g, sp, tws[2], rl_eps(s), input, b.est, output, b.sa, &row_blocks);
 
     // est_0 = 0.5 everywhere (deconvolve.rs:93): transform the constant directly.
     r.op = ROW_LOAD_CONST;
     if ((st = run_row(c, r, row_blocks, "row_start"))) return st;
     for (uint64_t it = 0; it < s.num_iter; it++) {
         const bool last = it + 1 == s.num_iter;
-        if ((st = run_corr(c, g, sp, sa, sb, otf1, tws))) return st;
+        if ((st = run_corr(c, g, sp, b.sa, b.sb, b.otf1, tws))) return st;
         r.op = ROW_RATIO;
         if ((st = run_row(c, r, row_blocks, "row_ratio"))) return st;
-        if ((st = run_corr(c, g, sp, sa, sb, otf2, tws))) return st;
+        if ((st = run_corr(c, g, sp, b.sa, b.sb, b.otf2, tws))) return st;
         r.op = ROW_UPDATE;
         r.first = it == 0;
         r.last = last;
@@ -781,6 +776,31 @@ lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* in
         if ((st = run_row(c, r, row_blocks, last ? "row_update_last" : "row_update"))) return st;
     }
     return LUCEM_OK;
+}
+
+lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
+                           const Geometry& g, const lucem_rl_settings& s) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const RlSizes z = rl_sizes(g);
+    RlBuffers b{};
+    lucem_status st;
+    if ((st = ws_slot(ws, "rl_est", z.est, (void**)&b.est))) return st;
+    if ((st = ws_slot(ws, "rl_spec_a", z.sa, (void**)&b.sa))) return st;
+    if (g.ndim == 3 && (st = ws_slot(ws, "rl_spec_b", z.sb, (void**)&b.sb))) return st;
+    if ((st = ws_slot(ws, "rl_otf_conv", z.otf, (void**)&b.otf1))) return st;
+    if ((st = ws_slot(ws, "rl_otf_mirror", z.otf, (void**)&b.otf2))) return st;
+    if ((st = ws_slot(ws, "rl_psf_h", z.psf, (void**)&b.h))) return st;
+    if ((st = ws_slot(ws, "rl_psf_rev", z.psf, (void**)&b.rev))) return st;
+    if ((st = ws_slot(ws, "rl_psf_spec_a", z.pa, (void**)&b.pa))) return st;
+    if (g.ndim == 3 && (st = ws_slot(ws, "rl_psf_spec_b", z.pb, (void**)&b.pb))) return st;
+    const float2* tws[3] = {nullptr, nullptr, nullptr};
+    for (int a = (g.ndim == 2 ? 1 : 0); a < 3; a++)
+        if ((st = twiddles_for(c, g.L[a], &tws[a]))) return st;
+    // launch shapes measured on this geometry's own buffers (first call of a geometry only)
+    const ShapeTable* table = nullptr;
+    if ((st = tuned_shapes(c, g, b.est, b.sa, b.sb, b.otf1, tws, &table))) return st;
+    return rl_launch(c, g, s, input, psf, output, b, tws, table);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -929,24 +949,15 @@ static int psf_prep_direct(const float* psf, float* h, float* rev, int64_t nk, v
     return launch_psf_prep(pp, stream);
 }
 
+// The launches of the direct form on given buffers: a, b, conv, rel, err of n elements, h and
+// rev of the PSF's size.
 template <typename T>
-static lucem_status rl_run_direct(lucem_client* c, lucem_workspace* ws, const T* input, const T* psf, T* output,
-                                  const Geometry& g, const lucem_rl_settings& s) {
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+static lucem_status rl_direct_launch(lucem_client* c, const T* input, const T* psf, T* output, const Geometry& g,
+                                     const lucem_rl_settings& s, T* a, T* b, T* conv, T* rel, T* err, T* h, T* rev) {
     const int H = g.n[1], W = g.n[2], kh = g.k[1], kw = g.k[2];
     const int64_t n = (int64_t)H * W, nk = (int64_t)kh * kw;
     const bool f64 = sizeof(T) == sizeof(double);
-    T *a, *b, *conv, *rel, *err, *h, *rev;
     lucem_status st;
-    if ((st = ws_slot(ws, f64 ? "rl64_est_a" : "rl32_est_a", sizeof(T) * n, (void**)&a)) ||
-        (st = ws_slot(ws, f64 ? "rl64_est_b" : "rl32_est_b", sizeof(T) * n, (void**)&b)) ||
-        (st = ws_slot(ws, f64 ? "rl64_est_conv" : "rl32_est_conv", sizeof(T) * n, (void**)&conv)) ||
-        (st = ws_slot(ws, f64 ? "rl64_rel_blur" : "rl32_rel_blur", sizeof(T) * n, (void**)&rel)) ||
-        (st = ws_slot(ws, f64 ? "rl64_err_est" : "rl32_err_est", sizeof(T) * n, (void**)&err)) ||
-        (st = ws_slot(ws, f64 ? "rl64_psf_h" : "rl32_psf_h", sizeof(T) * nk, (void**)&h)) ||
-        (st = ws_slot(ws, f64 ? "rl64_psf_rev" : "rl32_psf_rev", sizeof(T) * nk, (void**)&rev)))
-        return st;
     void* sm = c->stream;
     if ((st = launch_check(psf_prep_direct(psf, h, rev, nk, sm), "psf prep (direct)"))) return st;
     if ((st = launch_check(launch_fill_direct(a, T(0.5), n, sm), "fill (direct)"))) return st;
@@ -967,6 +978,32 @@ static lucem_status rl_run_direct(lucem_client* c, lucem_workspace* ws, const T*
         ping = !ping;
     }
     return LUCEM_OK;
+}
+
+lucem_status lucem::rl_direct_launch_f32(lucem_client* c, const float* input, const float* psf, float* output,
+                                         const Geometry& g, const lucem_rl_settings& s, float* const bufs[7]) {
+    return rl_direct_launch<float>(c, input, psf, output, g, s, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5],
+                                   bufs[6]);
+}
+
+template <typename T>
+static lucem_status rl_run_direct(lucem_client* c, lucem_workspace* ws, const T* input, const T* psf, T* output,
+                                  const Geometry& g, const lucem_rl_settings& s) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const int64_t n = (int64_t)g.n[1] * g.n[2], nk = (int64_t)g.k[1] * g.k[2];
+    const bool f64 = sizeof(T) == sizeof(double);
+    T *a, *b, *conv, *rel, *err, *h, *rev;
+    lucem_status st;
+    if ((st = ws_slot(ws, f64 ? "rl64_est_a" : "rl32_est_a", sizeof(T) * n, (void**)&a)) ||
+        (st = ws_slot(ws, f64 ? "rl64_est_b" : "rl32_est_b", sizeof(T) * n, (void**)&b)) ||
+        (st = ws_slot(ws, f64 ? "rl64_est_conv" : "rl32_est_conv", sizeof(T) * n, (void**)&conv)) ||
+        (st = ws_slot(ws, f64 ? "rl64_rel_blur" : "rl32_rel_blur", sizeof(T) * n, (void**)&rel)) ||
+        (st = ws_slot(ws, f64 ? "rl64_err_est" : "rl32_err_est", sizeof(T) * n, (void**)&err)) ||
+        (st = ws_slot(ws, f64 ? "rl64_psf_h" : "rl32_psf_h", sizeof(T) * nk, (void**)&h)) ||
+        (st = ws_slot(ws, f64 ? "rl64_psf_rev" : "rl32_psf_rev", sizeof(T) * nk, (void**)&rev)))
+        return st;
+    return rl_direct_launch<T>(c, input, psf, output, g, s, a, b, conv, rel, err, h, rev);
 }
 
 static lucem_status rl_entry_f64(lucem_client* client, const double* input, const uint64_t* in_dims, uint32_t in_ndim,

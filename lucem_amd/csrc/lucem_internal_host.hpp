@@ -154,4 +154,48 @@ lucem_status validate_rl(const void* input, const uint64_t* in_dims, uint32_t in
 lucem_status rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
                     const Geometry& g, const lucem_rl_settings& s);
 
+// Device buffers of one plain RL call: the workspace slots of rl_run, or carved from the
+// allocation-free scratch (lucem_noalloc.hip).
+struct RlBuffers {
+    float* est;     // current estimate, n floats
+    float2* sa;     // row half spectra, rows * pitch
+    float2* sb;     // 3-D: y-transformed planes, n0 * L1 * pitch
+    float2* otf1;   // transfer function of the flipped PSF (A5), L0 * L1 * pitch
+    float2* otf2;   // transfer function of the PSF (A7)
+    float* h;       // normalised PSF
+    float* rev;     // its flip
+    float2* pa;     // PSF row spectra, k0 * k1 * pitch
+    float2* pb;     // 3-D: PSF y-transformed planes, k0 * L1 * pitch
+};
+// Bytes of each RlBuffers member for geometry g (sb and pb are 0 in 2-D; otf is per transfer
+// function, psf per PSF copy).
+struct RlSizes {
+    size_t est, sa, sb, otf, psf, pa, pb;
+};
+RlSizes rl_sizes(const Geometry& g);
+// The launches of one plain RL call on buffers b: PSF preparation, both transfer functions, the
+// transformed constant start and num_iter iterations. `table`: measured launch shapes, or null
+// for the heuristics. Allocates nothing.
+lucem_status rl_launch(lucem_client* c, const Geometry& g, const lucem_rl_settings& s, const float* input,
+                       const float* psf, float* output, const RlBuffers& b, const float2* const tws[3],
+                       const ShapeTable* table);
+// The direct form's launches (2-D, f32) on bufs = {est_a, est_b, conv, rel, err} (n floats each)
+// and {h, rev} (the PSF's size each). Allocates nothing.
+lucem_status rl_direct_launch_f32(lucem_client* c, const float* input, const float* psf, float* output,
+                                  const Geometry& g, const lucem_rl_settings& s, float* const bufs[7]);
+// The launch shapes measured earlier on this client for geometry g, without measuring (null when
+// none are cached or tuning is off).
+const ShapeTable* tuned_lookup(lucem_client* c, const Geometry& g);
+// While one of these lives on a thread, the client's profiling records nothing (no events are
+// created or taken).
+struct NoProfileScope {
+    NoProfileScope();
+    ~NoProfileScope();
+    NoProfileScope(const NoProfileScope&) = delete;
+    NoProfileScope& operator=(const NoProfileScope&) = delete;
+};
+// Resolves the allocation-free scratch of `device` (that device current); called once per device
+// by lucem_client_create. The status is kept and reported by the allocation-free entry point.
+void noalloc_setup(int device);
+
 }  // namespace lucem

@@ -401,6 +401,13 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
 // The plain path: the tuned shapes of this geometry, measured on the first call of the client
 // for it (on the workspace buffers) and reused by later calls. Arrays too small to matter keep
 // the heuristics (an empty table).
+const ShapeTable* lucem::tuned_lookup(lucem_client* c, const Geometry& g) {
+    if (!autotune_enabled()) return nullptr;
+    const std::vector<int> key{g.ndim, g.n[0], g.n[1], g.n[2], g.k[0], g.k[1], g.k[2]};
+    auto it = c->tuned.find(key);
+    return it == c->tuned.end() ? nullptr : &it->second;
+}
+
 lucem_status lucem::tuned_shapes(lucem_client* c, const Geometry& g, float* est, float2* sa, float2* sb,
                                  const float2* otf, const float2* const tws[3], const ShapeTable** out) {
     *out = nullptr;
