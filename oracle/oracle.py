@@ -15,6 +15,9 @@ Functions and the reference code each one restates (paths relative to the refere
                               FP64 oracle); pinned against the direct FP64 oracle on small shapes
 - :func:`rl_nll_grad64`       FP64 reverse-mode gradient of unrolled RL + Poisson NLL (no
                               reference counterpart); pinned by central finite differences
+- :func:`rl_nll_grad_fft64`, :func:`rl_nll_fft64`
+                              the same gradient and loss with FFT correlations (config-size
+                              volumes); pinned against rl_nll_grad64 on small shapes
 - :func:`fft_axis64`          crates/lucem-ops/src/core/fft.rs:13-100 (the DFT of
                               core/kernels/fft.rs:11-47) in FP64 through numpy's FFT
 - :func:`add_noise32`         crates/lucem-ops/src/intensity/noise.rs:35-143 with
@@ -213,6 +216,133 @@ def rl_fft64(image: np.ndarray, psf: np.ndarray, num_iter: int, mode: int = PAD_
         r = y / np.where(np.abs(u) < eps, eps, u)
         x = x * correlate_fft64(r, h, mode)
     return x
+
+
+def _fft_mod():
+    try:  # scipy's FFT runs on every core; numpy's gives the same FP64 result on one
+        import scipy.fft as F
+
+        return F, {"workers": -1}
+    except ImportError:
+        return np.fft, {}
+
+
+def _gather64(x: np.ndarray, ks, mode: int) -> np.ndarray:
+    """The padded array Q[i] = x[pad(i - k // 2)] on [0, n + k - 1) per axis (0 outside in Constant
+    mode): the values correlate_2d_kernel reads (filter/kernels/convolve.rs:99-115)."""
+    q = x
+    for ax in range(x.ndim):
+        idx = _pad_indices(x.shape[ax], ks[ax], mode)
+        valid = idx >= 0
+        q = np.take(q, np.where(valid, idx, 0), axis=ax)
+        if not valid.all():
+            sl = [slice(None)] * x.ndim
+            sl[ax] = ~valid
+            q[tuple(sl)] = 0.0
+    return q
+
+
+def _fold64(qbar: np.ndarray, ns, ks, mode: int) -> np.ndarray:
+    """Transpose of :func:`_gather64`: every padded position adds into the source it was read from."""
+    g = qbar
+    for ax in range(g.ndim):
+        idx = _pad_indices(ns[ax], ks[ax], mode)
+        valid = np.nonzero(idx >= 0)[0]
+        m = np.zeros((ns[ax], len(idx)))
+        m[idx[valid], valid] = 1.0
+        g = np.moveaxis(np.tensordot(m, g, axes=([1], [ax])), 0, ax)
+    return g
+
+
+class _Fft64:
+    """FFT linear correlations of one geometry (array dims ns, window dims ks) on L >= n + k - 1."""
+
+    def __init__(self, ns, ks):
+        self.F, self.kw = _fft_mod()
+        self.ns, self.ks = tuple(ns), tuple(ks)
+        self.L = tuple(self.F.next_fast_len(n + k - 1) if hasattr(self.F, "next_fast_len") else n + k - 1
+                       for n, k in zip(ns, ks))
+        self.axes = list(range(len(ns)))
+
+    def spec(self, a):
+        return self.F.rfftn(a, self.L, axes=self.axes, **self.kw)
+
+    def back(self, sp, shape):
+        c = self.F.irfftn(sp, self.L, axes=self.axes, **self.kw)
+        return np.ascontiguousarray(c[tuple(slice(0, s) for s in shape)])
+
+    def corr(self, q, w):
+        """out[p] = sum_j w[j] q[p + j], p < n (q padded, w of dims ks)."""
+        return self.back(self.spec(q) * np.conj(self.spec(w)), self.ns)
+
+    def corr_t(self, obar, w):
+        """Transpose of corr in q: qbar[i] = sum_p obar[p] w[i - p], i < n + k - 1."""
+        return self.back(self.spec(obar) * self.spec(w), tuple(n + k - 1 for n, k in zip(self.ns, self.ks)))
+
+    def corr_w(self, q, obar):
+        """Transpose of corr in w: wbar[j] = sum_p obar[p] q[p + j], j < k."""
+        return self.back(self.spec(q) * np.conj(self.spec(obar)), self.ks)
+
+
+def rl_nll_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
+                 mode: int = PAD_REFLECT) -> float:
+    """FP64 loss ``sum(u - y log u)`` after ``num_iter`` RL iterations from ``x0`` (FFT correlations)."""
+    y = np.asarray(y, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    h, rev = _normalised(psf)
+    f = _Fft64(y.shape, h.shape)
+    for _ in range(int(num_iter)):
+        u = f.corr(_gather64(x, h.shape, mode), rev)
+        r = y / np.where(np.abs(u) < eps, eps, u)
+        x = x * f.corr(_gather64(r, h.shape, mode), h)
+    u = f.corr(_gather64(x, h.shape, mode), rev)
+    return float(np.sum(u - y * np.log(u)))
+
+
+def rl_nll_grad_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
+                      mode: int = PAD_REFLECT):
+    """:func:`rl_nll_grad64` with FFT correlations (for volumes the direct FP64 oracle cannot
+    finish): the forward keeps every iteration's padded estimate and ratio, the reverse sweep
+    applies the transposes of the gather (fold), the correlation in its input (full convolution)
+    and in its window, then the chain rule through ``h = psf / sum(psf)`` (deconvolve.rs:77-79).
+    Pinned against the direct :func:`rl_nll_grad64` on small 2-D and 3-D shapes in every mode."""
+    y = np.asarray(y, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    psf = np.asarray(psf, dtype=np.float64)
+    ssum = float(np.sum(psf.ravel()))
+    norm = 1.0 / ssum if abs(ssum) > 1e-12 else 1.0
+    h, rev = _normalised(psf)
+    ks, ns = h.shape, y.shape
+    f = _Fft64(ns, ks)
+    tape = []
+    for _ in range(int(num_iter)):
+        qx = _gather64(x, ks, mode)
+        u = f.corr(qx, rev)
+        small = np.abs(u) < eps
+        safe = np.where(small, eps, u)
+        r = y / safe
+        qr = _gather64(r, ks, mode)
+        c = f.corr(qr, h)
+        tape.append((x, qx, safe, small, qr, c))
+        x = x * c
+    qx = _gather64(x, ks, mode)
+    u = f.corr(qx, rev)
+    loss = float(np.sum(u - y * np.log(u)))
+    ubar = 1.0 - y / u
+    grev = f.corr_w(qx, ubar)
+    gh = np.zeros_like(h)
+    xbar = _fold64(f.corr_t(ubar, rev), ns, ks, mode)
+    for xt, qxt, safe, small, qrt, ct in reversed(tape):
+        cbar = xbar * xt
+        xnew = xbar * ct
+        gh += f.corr_w(qrt, cbar)
+        rbar = _fold64(f.corr_t(cbar, h), ns, ks, mode)
+        ubar_t = np.where(small, 0.0, -rbar * y / (safe * safe))
+        grev += f.corr_w(qxt, ubar_t)
+        xbar = xnew + _fold64(f.corr_t(ubar_t, rev), ns, ks, mode)
+    g = gh + grev[tuple(slice(None, None, -1) for _ in range(grev.ndim))]
+    gpsf = (g - float(np.sum(g * h))) * norm if abs(ssum) > 1e-12 else g
+    return loss, xbar, gpsf
 
 
 def rl_nll_grad64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,

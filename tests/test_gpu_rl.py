@@ -145,6 +145,22 @@ def test_clip_only_on_last_iteration(gpu, oracle):
     assert got.max() <= 1.0 and oracle.rel_l2(got, ref) < RTOL_L2
 
 
+def test_clip_last_iteration_closed_form(gpu):
+    """A 1 x 2 image under Wrap with PSF [[0.75, 0.25]]: two iterations written out by hand give
+    (0.6092, 4.1990), clamped to (0.6092, 1.0) with clip; clamping every iteration would give
+    0.8682 for the first pixel."""
+    y = np.array([[3.0, 0.2]], np.float32)
+    psf = np.array([[0.75, 0.25]], np.float32)
+    a, b = 0.75, 0.25
+    x = [0.5, 0.5]
+    for _ in range(2):
+        u = (b * x[1] + a * x[0], b * x[0] + a * x[1])
+        r = (3.0 / u[0], 0.2 / u[1])
+        x = [x[0] * (a * r[1] + b * r[0]), x[1] * (a * r[0] + b * r[1])]
+    np.testing.assert_allclose(_run(y, psf, 2, mode=4), [x], rtol=1e-5)
+    np.testing.assert_allclose(_run(y, psf, 2, clip=True, mode=4), [np.clip(x, -1, 1)], rtol=1e-5)
+
+
 def test_shifted_delta_closed_form(gpu, oracle):
     rng = np.random.default_rng(3)
     y = rng.uniform(1.0, 2.0, (9, 7)).astype(np.float32)

@@ -269,6 +269,70 @@ def test_gradient_3d_matches_finite_differences(oracle):
     assert abs(fd - gp[idx]) <= 1e-4 * max(abs(fd), 1e-3)
 
 
+@pytest.mark.parametrize("mode", range(5))
+@pytest.mark.parametrize("shape,k", [((23, 17), (5, 7)), ((9, 11, 8), (3, 5, 3)), ((6, 5), (9, 11))])
+def test_fft_gradient_matches_the_direct_gradient(oracle, mode, shape, k):
+    """The FFT form of the FP64 gradient oracle (config-size volumes) against the direct form,
+    which the finite differences above pin: every mode, 2-D and 3-D, a PSF larger than the image."""
+    rng = np.random.default_rng(len(shape) * 10 + mode)
+    y = rng.uniform(10, 100, shape)
+    x0 = rng.uniform(0.5, 2.0, shape)
+    psf = rng.uniform(0.1, 1.0, k)
+    l1, g1, p1 = oracle.rl_nll_grad64(y, x0, psf, 4, mode=mode)
+    l2, g2, p2 = oracle.rl_nll_grad_fft64(y, x0, psf, 4, mode=mode)
+    assert abs(l2 - l1) <= 1e-12 * abs(l1)
+    assert abs(oracle.rl_nll_fft64(y, x0, psf, 4, mode=mode) - l1) <= 1e-12 * abs(l1)
+    assert oracle.rel_l2(g2, g1) < 1e-12
+    assert oracle.rel_l2(p2, p1) < 1e-12
+
+
+def test_fft_gradient_planted_mistake_is_caught(oracle):
+    """Dropping the fold of the padded positions (the transpose of the boundary gather) in the
+    FFT gradient, a plausible mistake, moves d/dx0 far beyond the tolerance above."""
+    rng = np.random.default_rng(5)
+    y = rng.uniform(10, 100, (12, 10))
+    x0 = rng.uniform(0.5, 2.0, (12, 10))
+    psf = rng.uniform(0.1, 1.0, (5, 5))
+    _, g1, _ = oracle.rl_nll_grad64(y, x0, psf, 3)
+    real_fold = oracle._fold64
+    try:  # the planted mistake: keep only the centre of the padded array
+        oracle._fold64 = lambda q, ns, ks, mode: q[tuple(slice(k // 2, k // 2 + n) for n, k in zip(ns, ks))]
+        _, g2, _ = oracle.rl_nll_grad_fft64(y, x0, psf, 3)
+    finally:
+        oracle._fold64 = real_fold
+    assert oracle.rel_l2(g2, g1) > 1e-6
+
+
+def _two_pixel_closed_form(y, a, b, iters, clip_each=False):
+    """RL on a 1 x 2 image, PSF [[a, b]] (a + b = 1), Wrap: with left = 1, correlate(x, w) is
+    (w0 x1 + w1 x0, w0 x0 + w1 x1), so u = (b x1 + a x0, b x0 + a x1) for rev = (b, a) and
+    c = (a r1 + b r0, a r0 + b r1) for h = (a, b), written out by hand."""
+    x0, x1 = 0.5, 0.5
+    for _ in range(iters):
+        u0, u1 = b * x1 + a * x0, b * x0 + a * x1
+        r0, r1 = y[0] / u0, y[1] / u1
+        x0, x1 = x0 * (a * r1 + b * r0), x1 * (a * r0 + b * r1)
+        if clip_each:
+            x0, x1 = min(max(x0, -1.0), 1.0), min(max(x1, -1.0), 1.0)
+    return np.array([[x0, x1]])
+
+
+def test_clip_applies_on_the_last_iteration_only(oracle):
+    """deconvolve.rs:134-145 clamps to [-1, 1] only when the iteration is the last. On a two-pixel
+    image whose first iterate exceeds 1, clamping every iteration (the plausible mistake) gives
+    another first pixel after two iterations; the oracle matches the last-only closed form."""
+    y = np.array([[3.0, 0.2]])
+    psf = np.array([[0.75, 0.25]])
+    want = _two_pixel_closed_form(y[0], 0.75, 0.25, 2)
+    st, free = oracle.richardson_lucy(y, psf, 2, clip=False, mode=4)
+    assert st == 0
+    np.testing.assert_allclose(free, want, rtol=1e-13)
+    st, out = oracle.richardson_lucy(y, psf, 2, clip=True, mode=4)
+    np.testing.assert_allclose(out, np.clip(want, -1.0, 1.0), rtol=1e-13)
+    wrong = _two_pixel_closed_form(y[0], 0.75, 0.25, 2, clip_each=True)
+    assert abs(wrong[0, 0] - out[0, 0]) > 0.2  # 0.868 against 0.609
+
+
 def test_planted_flip_mistake_is_caught(oracle):
     """A restatement that forgets the 180-degree flip (correlating with h in both steps) fails
     the shifted-delta closed form: the pin above is sensitive to that mistake."""
