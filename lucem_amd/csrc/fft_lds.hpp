@@ -29,6 +29,15 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// a / b from the hardware reciprocal with one Newton correction (the residual a - q b is exact
+// with an FMA): within an ulp of the IEEE quotient for the finite, normal divisors the kernels
+// divide by (the eps floor keeps them away from 0), at a third of its instructions.
+__device__ __forceinline__ float div_fast(float a, float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return fmaf(fmaf(-q, b, a), r, q);
+}
+
 // Compile-time sin/cos of 2*pi*m/R (Taylor series on [-pi, pi], double precision).
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr double taylor_sin(double x) {
@@ -250,15 +259,17 @@ struct Divisor {
 // u + (j - k)*(R - 1)*B + t*Ns*B.
 // W: waves per SIMD of the calling kernels. Each W gets its own copy of the pass, compiled to
 // that register budget (a copy shared by kernels of different budgets gets the smallest).
+// Ns = the product of the earlier radices, ns_mag its multiply-high reciprocal (0 when Ns = 1) and
+// rem = N / Ns, so that no pass divides by a value known only at run time.
 template <int R1, int R2, bool INV, int T, int E, bool TWL, int W>
-__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns,
-                                              const float2* __restrict__ tw) {
+__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns, unsigned ns_mag,
+                                              int rem, const float2* __restrict__ tw) {
     constexpr int R = R1 * R2;
     constexpr int S = (E + R - 1) / R;
     const int B = lines.d;
     const int NR = N / R;
     const int nbf = NR * B;
-    const int tws = N / (Ns * R);
+    const int tws = rem / R;  // N / (Ns * R)
     float2 v[S][R];
 #pragma unroll
     for (int s = 0; s < S; s++) {
@@ -274,7 +285,7 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divi
         const int u = threadIdx.x + s * T;
         if (u < nbf) {
             const int j = lines.div(u);
-            const int k = j % Ns;
+            const int k = Ns > 1 ? j - (int)__umulhi((unsigned)j, ns_mag) * Ns : 0;
             if (Ns > 1) {
                 const int step = k * tws;
 #pragma unroll
@@ -322,15 +333,18 @@ __device__ __forceinline__ const float2* stage_twiddles(float2* Z, int data_elem
 template <bool INV, int T, int E, bool TWL, int W>
 __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ tw) {
     const int N = p.L;
-    int Ns = 1;
+    int Ns = 1, rem = N;
     for (int q = 0; q < p.npass; q++) {
         const unsigned long long w = q < 8 ? p.codes_lo : p.codes_hi;
         const int code = (int)((w >> (8 * (q & 7))) & 0xffull);
+        // one division per pass (uniform), none per butterfly
+        const unsigned mag = __builtin_amdgcn_readfirstlane(Divisor::of(Ns).m);
         switch (code) {
-#define LUCEM_PASS(r1, r2)                                                        \
-    case radix_code(r1, r2):                                                      \
-        stockham_pass<r1, r2, INV, T, E, TWL, W>(buf, N, lines, Ns, tw);         \
-        Ns *= r1 * r2;                                                            \
+#define LUCEM_PASS(r1, r2)                                                             \
+    case radix_code(r1, r2):                                                           \
+        stockham_pass<r1, r2, INV, T, E, TWL, W>(buf, N, lines, Ns, mag, rem, tw);   \
+        Ns *= r1 * r2;                                                                 \
+        rem /= r1 * r2;                                                                \
         break;
             LUCEM_PASS(2, 1)
             LUCEM_PASS(3, 1)
@@ -360,19 +374,25 @@ __device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& 
         lds_fft_run<INV, T, E, false, W>(buf, lines, p, twbase + p.tw);
 }
 
-// Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len).
-__device__ __forceinline__ int pad_index(int p, int len, int mode) {
+// Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len). Positions
+// within one fold of the array (every position when the window is no longer than the array)
+// take the closed forms; farther ones fold by the modulo of the period.
+__host__ __device__ __forceinline__ int pad_index(int p, int len, int mode) {
     if (p >= 0 && p < len) return p;
     switch (mode) {
         case 1: return -1;
         case 2: return p < 0 ? 0 : len - 1;
-        case 4: return ((p % len) + len) % len;
+        case 4:
+            if (p >= -len && p < 2 * len) return p < 0 ? p + len : p - len;
+            return ((p % len) + len) % len;
         case 0: {
+            if (p >= -len && p < 2 * len) return p < 0 ? -1 - p : 2 * len - 1 - p;
             const int period = 2 * len;
             const int r = ((p % period) + period) % period;
             return r < len ? r : period - 1 - r;
         }
         default: {
+            if (len > 1 && p > -len && p < 2 * len - 1) return p < 0 ? -p : 2 * len - 2 - p;
             const int period = max(2 * len - 2, 1);
             const int r = ((p % period) + period) % period;
             return r < len ? r : period - r;

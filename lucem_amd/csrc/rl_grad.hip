@@ -142,7 +142,7 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
                         if (own) (G.xslots + L.sx * ns)[at] = v;
                     } else if constexpr (OP == G_RATIO) {
                         const float den = fabsf(cv) < d.eps ? d.eps : cv;
-                        v = d.obs[at] / den;
+                        v = div_fast(d.obs[at], den);
                         if (own && (L.flags & GF_SAVE)) (G.uslots + L.su * ns)[at] = cv;
                     } else if constexpr (OP == G_UPDATE) {
                         v = (G.xslots + L.sx * ns)[at] * cv;

@@ -56,6 +56,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
     const int tid = threadIdx.x;
     const int pair0 = g * P;
     const int total = L * P;
+    const Divisor ld = Divisor::of(L);
     const float2* twl = stage_twiddles(Z, total, d.f, d.tw);
 
     if constexpr (OP == ROW_RATIO || OP == ROW_UPDATE) {
@@ -103,6 +104,8 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
         lds_fft<true, T, E, W>(Z, pd, d.f, d.tw, twl);
         // Gather into registers: padded position i reads the cropped result at src = pad(i - left)
         // and applies the pointwise step there. Every value is read before any is overwritten.
+        // Item e = b * L + i runs along the row (x fastest across lanes), so the global reads and
+        // writes of the real rows are contiguous; the LDS position of (i, b) is i * P + b.
         float2 val[S];
         int srcs[S];
         // global operands first (observed image or previous estimate at the source column)
@@ -112,7 +115,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             srcs[s] = -1;
             val[s] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int i = pd.div(e), b = e - i * P;
+                const int b = ld.div(e), i = e - b * L;
                 const int src = pad_index(i - d.left, nx, d.mode);
                 srcs[s] = src;
                 if (src >= 0) {
@@ -142,12 +145,12 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             const int e = tid + s * T;
             const int src = srcs[s];
             if (e < total && src >= 0) {
-                const int b = e - pd.div(e) * P;
+                const int b = ld.div(e);
                 const float2 c = Z[src * P + b];
                 if constexpr (OP == ROW_RATIO) {
                     const float da = fabsf(c.x) < d.eps ? d.eps : c.x;
                     const float db = fabsf(c.y) < d.eps ? d.eps : c.y;
-                    val[s] = make_float2(val[s].x / da, val[s].y / db);
+                    val[s] = make_float2(div_fast(val[s].x, da), div_fast(val[s].y, db));
                 } else {
                     float oa = val[s].x * c.x, ob = val[s].y * c.y;
                     if (fl.last && fl.clip) {
@@ -163,10 +166,9 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
         for (int s = 0; s < S; s++) {
             const int e = tid + s * T;
             if (e < total) {
+                const int b = ld.div(e), i = e - b * L;
                 if constexpr (OP == ROW_UPDATE) {
-                    const int i = pd.div(e);
                     if (srcs[s] >= 0 && i == srcs[s] + d.left) {
-                        const int b = e - pd.div(e) * P;
                         const int x = i - d.left;
                         const int ra = 2 * (pair0 + b), rb = ra + 1;
                         if (VIEWS && fl.last && d.window) {
@@ -191,7 +193,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
                         }
                     }
                 }
-                Z[e] = val[s];
+                Z[i * P + b] = val[s];
             }
         }
         if constexpr (OP == ROW_UPDATE) {
@@ -205,7 +207,7 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
             const int e = tid + s * T;
             z[s] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int i = pd.div(e), b = e - i * P;
+                const int b = ld.div(e), i = e - b * L;  // x fastest across lanes
                 const int ra = 2 * (pair0 + b), rb = ra + 1;
                 const bool va = ra < d.rows, vb = rb < d.rows;
                 const int src = pad_index(i - d.left, nx, d.mode);
@@ -222,7 +224,10 @@ __device__ void row_pass_body(const RowPass& d, const RowFlags fl, int g, float2
 #pragma unroll
         for (int s = 0; s < S; s++) {
             const int e = tid + s * T;
-            if (e < total) Z[e] = z[s];
+            if (e < total) {
+                const int b = ld.div(e), i = e - b * L;
+                Z[i * P + b] = z[s];
+            }
         }
     }
     __syncthreads();
