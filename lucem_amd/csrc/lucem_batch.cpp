@@ -90,17 +90,20 @@ struct lucem_rl_batch {
 };
 
 lucem_rl_batch::~lucem_rl_batch() {
-    if (client) {
-        (void)hipSetDevice(client->device);
-        (void)hipStreamSynchronize(client->stream);
+    if (!lucem::exiting()) {
+        if (client) {
+            (void)hipSetDevice(client->device);
+            (void)hipStreamSynchronize(client->stream);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture) (void)hipStreamDestroy(capture);
+        if (staged) (void)hipEventDestroy(staged);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (h_psf_stage) (void)hipHostFree(h_psf_stage);
+        if (mem) (void)hipFree(mem);
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (capture) (void)hipStreamDestroy(capture);
-    if (staged) (void)hipEventDestroy(staged);
-    if (h_stage) (void)hipHostFree(h_stage);
-    if (h_psf_stage) (void)hipHostFree(h_psf_stage);
-    if (mem) (void)hipFree(mem);
+    lucem::client_release(client);
 }
 
 namespace {
@@ -172,6 +175,7 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
 
     auto b = std::make_unique<lucem_rl_batch>();
     b->client = client;
+    client_retain(client);  // released by the plan's destructor, also when creation fails below
     if (const char* sched = std::getenv("LUCEM_BATCH_SCHEDULE")) {
         const std::string v(sched);
         b->schedule = v == "queue" ? lucem_rl_batch::QUEUE : v == "stride" ? lucem_rl_batch::STRIDE : lucem_rl_batch::DIRECT;

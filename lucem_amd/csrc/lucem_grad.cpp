@@ -86,18 +86,21 @@ struct lucem_rl_grad {
 };
 
 lucem_rl_grad::~lucem_rl_grad() {
-    if (client) {
-        (void)hipSetDevice(client->device);
-        (void)hipStreamSynchronize(client->stream);
+    if (!lucem::exiting()) {
+        if (client) {
+            (void)hipSetDevice(client->device);
+            (void)hipStreamSynchronize(client->stream);
+        }
+        if (x_fwd) (void)hipGraphExecDestroy(x_fwd);
+        if (x_bwd) (void)hipGraphExecDestroy(x_bwd);
+        if (g_fwd) (void)hipGraphDestroy(g_fwd);
+        if (g_bwd) (void)hipGraphDestroy(g_bwd);
+        if (capture) (void)hipStreamDestroy(capture);
+        if (staged) (void)hipEventDestroy(staged);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (mem) (void)hipFree(mem);
     }
-    if (x_fwd) (void)hipGraphExecDestroy(x_fwd);
-    if (x_bwd) (void)hipGraphExecDestroy(x_bwd);
-    if (g_fwd) (void)hipGraphDestroy(g_fwd);
-    if (g_bwd) (void)hipGraphDestroy(g_bwd);
-    if (capture) (void)hipStreamDestroy(capture);
-    if (staged) (void)hipEventDestroy(staged);
-    if (h_stage) (void)hipHostFree(h_stage);
-    if (mem) (void)hipFree(mem);
+    lucem::client_release(client);
 }
 
 static lucem_status run_seq(hipStream_t s, const std::vector<GLaunch>& seq, int* counters_unused) {
@@ -147,6 +150,7 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
     hipError_t e = hipSetDevice(client->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     P->client = client;
+    client_retain(client);  // released by the plan's destructor, also when creation fails below
     P->settings = *settings;
     P->ndim = (int)ndim;
     P->T = (int)settings->num_iter;

@@ -90,12 +90,35 @@ extern "C" lucem_status lucem_pad_mode_parse(const char* name, uint32_t* mode_ou
 // ---------------------------------------------------------------------------------------------
 // client
 // ---------------------------------------------------------------------------------------------
+static std::atomic<bool> g_exiting{false};
+
+namespace {
+// Made on the first client creation, after the HIP runtime has started: its destructor is
+// registered after the runtime's own exit-time teardown and so runs before it (exit handlers run
+// in reverse order of registration).
+struct ExitMark {
+    ~ExitMark() { g_exiting.store(true); }
+};
+}  // namespace
+
+bool lucem::exiting() { return g_exiting.load(); }
+
+void lucem::client_retain(lucem_client* c) {
+    if (c) c->refs.fetch_add(1);
+}
+
+void lucem::client_release(lucem_client* c) {
+    if (c && c->refs.fetch_sub(1) == 1) delete c;
+}
+
 extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t own_stream, lucem_client** out) {
     if (!out) return fail(LUCEM_INVALID_HANDLE, "client output pointer is NULL");
     *out = nullptr;
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count == 0) return fail(LUCEM_HIP_ERROR, "no HIP device is available");
+    static ExitMark exit_mark;
+    (void)exit_mark;
     if (device < 0 || device >= count) return fail(LUCEM_INVALID_PARAMETER, "device index out of range");
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
     auto c = std::make_unique<lucem_client>();
@@ -122,14 +145,16 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
     }
     c->internal_ws = std::make_unique<lucem_workspace>();
     c->internal_ws->client = c.get();
+    c->internal_ws->internal = true;
     *out = c.release();
     return LUCEM_OK;
 }
 
 lucem_client::~lucem_client() {
+    internal_ws.reset();
+    if (lucem::exiting()) return;  // the runtime may be gone: leave device memory to the process exit
     (void)hipSetDevice(device);
     (void)hipStreamSynchronize(stream);
-    internal_ws.reset();
     for (auto& r : records) {
         (void)hipEventDestroy(r.start);
         (void)hipEventDestroy(r.stop);
@@ -142,7 +167,7 @@ lucem_client::~lucem_client() {
 }
 
 extern "C" lucem_status lucem_client_destroy(lucem_client* client) {
-    delete client;
+    client_release(client);  // freed now, or by the last workspace or plan made on it
     return LUCEM_OK;
 }
 
@@ -292,8 +317,11 @@ lucem_status lucem::twiddles_for(lucem_client* c, int L, const float2** out) {
 // workspace
 // ---------------------------------------------------------------------------------------------
 lucem_workspace::~lucem_workspace() {
-    if (client) (void)hipStreamSynchronize(client->stream);
-    for (auto& kv : slots) (void)hipFree(kv.second.ptr);
+    if (!lucem::exiting()) {
+        if (client) (void)hipStreamSynchronize(client->stream);
+        for (auto& kv : slots) (void)hipFree(kv.second.ptr);
+    }
+    if (!internal) client_release(client);
 }
 
 lucem_status lucem::ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out) {
@@ -324,6 +352,7 @@ extern "C" lucem_status lucem_workspace_create(lucem_client* client, lucem_works
     if (!client || !out) return fail(LUCEM_INVALID_HANDLE, "client or workspace output is NULL");
     auto w = std::make_unique<lucem_workspace>();
     w->client = client;
+    client_retain(client);
     *out = w.release();
     return LUCEM_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -42,6 +43,9 @@ using ShapeTable = std::map<std::pair<int, int>, ShapeChoice>;  // (kind, L) -> 
 }  // namespace lucem
 
 struct lucem_client {
+    // One reference for the caller's handle and one for every workspace and plan made on the
+    // client: lucem_client_destroy drops the caller's, and whichever release comes last frees it.
+    std::atomic<int> refs{1};
     int device = 0;
     int num_cus = 256;
     hipStream_t stream = nullptr;
@@ -66,11 +70,20 @@ struct Slot {
 
 struct lucem_workspace {
     lucem_client* client = nullptr;
+    bool internal = false;  // the client's own workspace: holds no reference to it
     std::map<std::string, lucem::Slot> slots;
     ~lucem_workspace();
 };
 
 namespace lucem {
+
+// Client references held by workspaces and plans (see lucem_client::refs).
+void client_retain(lucem_client* c);
+void client_release(lucem_client* c);
+// True once the process has begun to exit and the HIP runtime may be torn down: the mark is set
+// by the destructor of a static made right after the runtime's first use, so it runs before the
+// runtime's own teardown. Destroy calls made after it free host memory only and call no HIP API.
+bool exiting();
 
 // Geometry of one RL array: sizes per axis (z, y, x), FFT lengths and half-spectrum layout.
 struct Geometry {

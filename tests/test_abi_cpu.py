@@ -398,3 +398,37 @@ def test_launch_shapes_fit_their_tiers(L):
         pytest.skip("tests/cpp/test_shapes is built by __graft_entry__.build()")
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and res.stdout.strip() == "ok", res.stderr
+
+
+def test_exit_hook_releases_plans_then_workspaces_then_clients(monkeypatch):
+    """The interpreter-exit hook closes every live plan, then every workspace, then every client,
+    whatever order they were made in (the C ABI also keeps a client alive while a workspace or
+    plan made on it lives, so any order is safe; this one also frees each plan's graph while its
+    stream is still there)."""
+    import weakref
+
+    from lucem_amd import _native as N
+
+    order = []
+
+    class Stub:
+        def __init__(self, kind, name):
+            self.kind, self.name = kind, name
+
+        def close(self):
+            order.append(self.kind)
+
+    owned = {"plans": weakref.WeakSet(), "workspaces": weakref.WeakSet(), "clients": weakref.WeakSet()}
+    monkeypatch.setattr(N, "_owned", owned)
+    keep = [Stub("clients", "c0"), Stub("workspaces", "w0"), Stub("plans", "p0"), Stub("clients", "c1"),
+            Stub("plans", "p1"), Stub("workspaces", "w1")]
+    for s in keep:
+        owned[s.kind].add(s)
+    N._release_all()
+    assert order == ["plans", "plans", "workspaces", "workspaces", "clients", "clients"]
+
+
+def test_destroy_functions_accept_null(L):
+    """Destroying a NULL handle is a no-op (as freeing a null pointer)."""
+    assert L.lucem_client_destroy(None) == 0
+    assert L.lucem_workspace_destroy(None) == 0
