@@ -30,25 +30,62 @@ from .restore import PadMode, RichardsonLucy, richardson_lucy_into, richardson_l
 
 @dataclass
 class RichardsonLucyOp:
-    """Richardson-Lucy deconvolution as a pipeline operator (tag ``richardson_lucy``)."""
+    """Richardson-Lucy deconvolution as a pipeline operator (tag ``richardson_lucy``): the image on
+    the first port, the PSF on the second, whose extent per axis is at most ``psf_size`` (the
+    bound the halo is computed from, as the reference's ``Op::halo`` takes no input,
+    ops.rs:434)."""
 
     num_iter: int = 30
     clip: bool = True
     filter_epsilon: float | None = None
     mode: PadMode = PadMode.Reflect
+    psf_size: int = 15
     tag: str = field(default="richardson_lucy", init=False)
 
-    def halo(self, psf_shape: Sequence[int]) -> int:
+    def halo(self, psf_shape: Sequence[int] | None = None) -> int:
         """Pixels of context per side for an exact result: ``num_iter * 2 * (k // 2)`` for the
-        largest PSF extent k."""
-        return int(self.num_iter) * 2 * (max(int(k) for k in psf_shape) // 2)
+        largest PSF extent k (``psf_size`` when no PSF shape is given)."""
+        k = max(int(d) for d in psf_shape) if psf_shape is not None else int(self.psf_size)
+        return int(self.num_iter) * 2 * (k // 2)
 
     def validate(self) -> None:
         """Parameter checks of the operator (the reference's RL check: num_iter >= 1)."""
         if int(self.num_iter) < 1:
             raise ValueError("invalid parameter `num_iter`: num_iter must be >= 1")
-        if self.filter_epsilon is not None and math.isnan(float(self.filter_epsilon)):
-            raise ValueError("invalid parameter `filter_epsilon`: not a number")
+        if self.filter_epsilon is not None and not math.isfinite(float(self.filter_epsilon)):
+            raise ValueError("filter epsilon must be a number")
+        if int(self.psf_size) < 1:
+            raise ValueError("psf size must be at least 1")
+
+    def params(self) -> dict:
+        """The operator's parameters as the pipeline's JSON carries them (``op`` tag excluded)."""
+        return {"num_iter": int(self.num_iter), "clip": bool(self.clip),
+                "filter_epsilon": None if self.filter_epsilon is None else float(self.filter_epsilon),
+                "mode": PadMode(self.mode).label, "psf_size": int(self.psf_size)}
+
+    @classmethod
+    def from_params(cls, params: dict) -> "RichardsonLucyOp":
+        """The operator from its JSON parameters: every field given, no other (the reference's
+        serde rules for ``Op``, ops.rs:97-99); the mode by its lowercase name (core/pad.rs:13-14)."""
+        want = {"num_iter", "clip", "filter_epsilon", "mode", "psf_size"}
+        unknown, missing = set(params) - want, want - set(params)
+        if unknown:
+            raise ValueError(f"unknown field(s) {sorted(unknown)} for richardson_lucy")
+        if missing:
+            raise ValueError(f"missing field(s) {sorted(missing)} for richardson_lucy")
+        modes = {m.label: m for m in PadMode}
+        if params["mode"] not in modes:
+            raise ValueError(f"unknown variant `{params['mode']}`, expected one of {sorted(modes)}")
+        for name in ("num_iter", "psf_size"):
+            if isinstance(params[name], bool) or not isinstance(params[name], int) or params[name] < 0:
+                raise ValueError(f"invalid value for `{name}`: expected a non-negative integer")
+        if not isinstance(params["clip"], bool):
+            raise ValueError("invalid value for `clip`: expected a boolean")
+        eps = params["filter_epsilon"]
+        if eps is not None and (isinstance(eps, bool) or not isinstance(eps, (int, float))):
+            raise ValueError("invalid value for `filter_epsilon`: expected a number or null")
+        return cls(num_iter=params["num_iter"], clip=params["clip"], filter_epsilon=eps,
+                   mode=modes[params["mode"]], psf_size=params["psf_size"])
 
     def settings(self) -> RichardsonLucy:
         return RichardsonLucy(num_iter=int(self.num_iter), clip=bool(self.clip), mode=self.mode,
@@ -60,6 +97,8 @@ class RichardsonLucyOp:
 
         self.validate()
         image = N.as_tensor(image)
+        if max(int(d) for d in N.as_tensor(psf).shape) > int(self.psf_size):
+            raise ValueError(f"psf of shape {tuple(N.as_tensor(psf).shape)} is larger than psf_size {self.psf_size}")
         out = torch.empty_like(image)
         if image.dim() == 2:
             richardson_lucy_into(image, psf, out, self.settings())

@@ -125,7 +125,7 @@ def test_check_order_nd_without_device(L):
 
 def test_oracle_is_not_imported_by_the_product():
     """The product path never imports the oracle (it is test infrastructure only)."""
-    for p in (ROOT / "lucem_amd").rglob("*.py"):
+    for p in [*(ROOT / "lucem_amd").rglob("*.py"), *(ROOT / "lucem").rglob("*.py")]:
         text = p.read_text()
         assert "oracle" not in re.sub(r"#.*", "", text).replace("oracle/", ""), p
 
