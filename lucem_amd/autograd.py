@@ -15,6 +15,7 @@ gradients for its parameters by ordinary autograd.
 from __future__ import annotations
 
 import ctypes
+from collections import OrderedDict
 from typing import Sequence
 
 import torch
@@ -101,15 +102,40 @@ class RLGradPlan:
             pass
 
 
-_plans: dict = {}
+_plans: "OrderedDict[tuple, RLGradPlan]" = OrderedDict()
+_plan_cache_size = 4
+
+
+def set_plan_cache_size(n: int) -> None:
+    """Keeps at most ``n`` (>= 1) plans of :func:`rl_nll`, least recently used first out (default
+    4). A plan is keyed by tile shapes, PSF shape and settings, and holds its checkpoints and
+    spectra on the device (about 10 KB per pixel of its tiles), so a fit over changing tile sets
+    would otherwise keep every plan it ever made."""
+    global _plan_cache_size
+    if int(n) < 1:
+        raise ValueError("the plan cache holds at least one plan")
+    _plan_cache_size = int(n)
+    while len(_plans) > _plan_cache_size:
+        _plans.popitem(last=False)
+
+
+def clear_plan_cache() -> None:
+    """Drops every cached plan of :func:`rl_nll`. A plan still needed by a pending backward stays
+    alive through that graph and is freed with it."""
+    _plans.clear()
 
 
 def _plan(shapes, psf_shape, num_iter, checkpoint_every, mode, eps, device) -> RLGradPlan:
-    key = (tuple(shapes), tuple(psf_shape), int(num_iter), int(checkpoint_every), int(mode), eps, device)
+    key = (tuple(tuple(int(d) for d in s) for s in shapes), tuple(psf_shape), int(num_iter), int(checkpoint_every),
+           int(mode), eps, device)
     p = _plans.get(key)
     if p is None:
         p = RLGradPlan(shapes, psf_shape, num_iter, checkpoint_every, mode, eps, N.default_client(device))
         _plans[key] = p
+        while len(_plans) > _plan_cache_size:
+            _plans.popitem(last=False)  # the autograd graphs that use it keep it until they are freed
+    else:
+        _plans.move_to_end(key)
     return p
 
 

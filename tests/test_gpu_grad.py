@@ -221,3 +221,40 @@ def test_grad_plan_replays_match_every_time(gpu):
             for a, b in zip(got, ref):
                 np.testing.assert_array_equal(a, b)
     plan.close()
+
+
+def test_plan_cache_is_bounded(gpu):
+    """rl_nll keeps at most the configured number of plans (least recently used first out); an
+    evicted plan whose backward is still pending stays alive through its graph and gives the same
+    gradient as a fresh one."""
+    import gc
+    import weakref
+
+    import torch
+
+    from lucem_amd import autograd as AG
+
+    AG.clear_plan_cache()
+    AG.set_plan_cache_size(2)
+    try:
+        psf = torch.rand((3, 3), device="cuda") + 0.1
+        y = torch.rand((20, 24), device="cuda") * 50 + 10
+        x0 = torch.full_like(y, 0.5, requires_grad=True)
+        pending = AG.rl_nll([x0], [y], psf, 3).sum()  # its plan is about to be evicted
+        refs = []
+        for h in (21, 22, 23):
+            yy = torch.rand((h, 24), device="cuda") * 50 + 10
+            AG.rl_nll([torch.full_like(yy, 0.5)], [yy], psf, 3)
+            refs.append(weakref.ref(AG._plans[next(reversed(AG._plans))]))
+        assert len(AG._plans) == 2
+        gc.collect()
+        assert refs[0]() is None  # evicted and unreferenced: freed
+        pending.backward()
+        g1 = x0.grad.clone()
+        x1 = torch.full_like(y, 0.5, requires_grad=True)
+        AG.rl_nll([x1], [y], psf, 3).sum().backward()
+        torch.cuda.synchronize()
+        assert torch.equal(g1, x1.grad)
+    finally:
+        AG.set_plan_cache_size(4)
+        AG.clear_plan_cache()
