@@ -392,6 +392,13 @@ lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_t* tile_dim
  * be NULL) for x_T; loss: NULL or device double[num_tiles]. */
 lucem_status lucem_rl_grad_forward(lucem_rl_grad* plan, const float* const* x0, const float* const* y,
                                    const float* psf, float* const* x_out, double* loss);
+/* The same with the loss taken against target[t] instead of y[t]: loss_t = sum(u_T - target_t log u_T)
+ * (target NULL, or an entry NULL: y). RL still runs on y. With y and target two halves of the
+ * same counts (binomial thinning), this held-out likelihood is minimised near the true PSF, where
+ * the likelihood of y itself is minimised by a delta PSF. No reference counterpart. */
+lucem_status lucem_rl_grad_forward_target(lucem_rl_grad* plan, const float* const* x0, const float* const* y,
+                                          const float* const* target, const float* psf, float* const* x_out,
+                                          double* loss);
 /* loss_weight: NULL (all 1) or device float[num_tiles]; grad_x0[t]: device; grad_psf: NULL or device. */
 lucem_status lucem_rl_grad_backward(lucem_rl_grad* plan, const float* loss_weight, float* const* grad_x0,
                                     float* grad_psf);

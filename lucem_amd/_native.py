@@ -56,7 +56,7 @@ EXPORTED = [
     "lucem_fourier_gaussian_into_f32", "lucem_fourier_box_into_f32", "lucem_fourier_shift_into_f32",
     "lucem_butterworth_f32", "lucem_gabor_settings_default", "lucem_gabor_2d_f32",
     "lucem_correlate1d_settings_default", "lucem_correlate_1d_into_f32", "lucem_correlate_1d_into_f64",
-    "lucem_richardson_lucy_into_noalloc_f32", "lucem_noalloc_scratch_bytes",
+    "lucem_richardson_lucy_into_noalloc_f32", "lucem_noalloc_scratch_bytes", "lucem_rl_grad_forward_target",
 ]
 
 

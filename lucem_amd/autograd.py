@@ -33,6 +33,8 @@ def _register(L) -> None:
                                        ctypes.POINTER(N.RlSettings), ctypes.c_uint32, ctypes.POINTER(N._vp)]
     L.lucem_rl_grad_forward.argtypes = [N._vp, ctypes.POINTER(N._vp), ctypes.POINTER(N._vp), N._vp,
                                         ctypes.POINTER(N._vp), N._vp]
+    L.lucem_rl_grad_forward_target.argtypes = [N._vp, ctypes.POINTER(N._vp), ctypes.POINTER(N._vp),
+                                               ctypes.POINTER(N._vp), N._vp, ctypes.POINTER(N._vp), N._vp]
     L.lucem_rl_grad_backward.argtypes = [N._vp, N._vp, ctypes.POINTER(N._vp), N._vp]
     L.lucem_rl_grad_destroy.argtypes = [N._vp]
     L.lucem_rl_grad_device_bytes.argtypes = [N._vp]
@@ -70,17 +72,22 @@ class RLGradPlan:
     def device_bytes(self) -> int:
         return int(N.lib().lucem_rl_grad_device_bytes(self._h))
 
-    def forward(self, x0s, ys, psf, x_outs=None):
-        """Enqueues the forward; returns the per-tile loss as a float64 device tensor."""
+    def forward(self, x0s, ys, psf, x_outs=None, targets=None):
+        """Enqueues the forward; returns the per-tile loss as a float64 device tensor. ``targets``
+        (per tile, or None): the counts the loss compares the re-blurred estimate with, in place
+        of ``ys`` (RL still runs on ``ys``)."""
         n = len(self.shapes)
         xp = (N._vp * n)(*[N.device_ptr(x, "x0") for x in x0s])
         yp = (N._vp * n)(*[N.device_ptr(y, "y") for y in ys])
         op = None
         if x_outs is not None:
             op = (N._vp * n)(*[N.device_ptr(o, "x_out") if o is not None else None for o in x_outs])
+        tp = None
+        if targets is not None:
+            tp = (N._vp * n)(*[N.device_ptr(t, "target") if t is not None else None for t in targets])
         loss = torch.empty(n, dtype=torch.float64, device=psf.device)
-        N.check(N.lib().lucem_rl_grad_forward(self._h, xp, yp, N._vp(N.device_ptr(psf, "psf")), op,
-                                              N._vp(loss.data_ptr())))
+        N.check(N.lib().lucem_rl_grad_forward_target(self._h, xp, yp, tp, N._vp(N.device_ptr(psf, "psf")), op,
+                                                     N._vp(loss.data_ptr())))
         self.generation += 1
         return loss
 
@@ -144,47 +151,58 @@ class RichardsonLucyNLL(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, psf, num_iter, checkpoint_every, mode, eps, n_tiles, *tensors):
+        # tensors: n_tiles x0, n_tiles y, then n_tiles targets or none
         x0s = [t.detach().contiguous() for t in tensors[:n_tiles]]
-        ys = [t.detach().contiguous() for t in tensors[n_tiles:]]
+        ys = [t.detach().contiguous() for t in tensors[n_tiles:2 * n_tiles]]
+        tg = [t.detach().contiguous() for t in tensors[2 * n_tiles:]] or None
         psf_c = psf.detach().contiguous()
         plan = _plan([x.shape for x in x0s], psf_c.shape, num_iter, checkpoint_every, mode, eps, psf.device.index)
-        loss = plan.forward(x0s, ys, psf_c)
+        loss = plan.forward(x0s, ys, psf_c, targets=tg)
         ctx.plan = plan
         ctx.generation = plan.generation
         ctx.n_tiles = n_tiles
-        ctx.save_for_backward(psf_c, *x0s, *ys)
+        ctx.n_inputs = len(tensors)
+        ctx.save_for_backward(psf_c, *x0s, *ys, *(tg or []))
         return loss
 
     @staticmethod
     def backward(ctx, grad_loss):
         plan = ctx.plan
+        n = ctx.n_tiles
         saved = ctx.saved_tensors
-        psf_c, x0s, ys = saved[0], saved[1:1 + ctx.n_tiles], saved[1 + ctx.n_tiles:]
+        psf_c, x0s, ys, tg = saved[0], saved[1:1 + n], saved[1 + n:1 + 2 * n], saved[1 + 2 * n:]
         if plan.generation != ctx.generation:  # another forward used the plan since: redo ours
-            plan.forward(list(x0s), list(ys), psf_c)
+            plan.forward(list(x0s), list(ys), psf_c, targets=list(tg) or None)
             ctx.generation = plan.generation
         w = grad_loss.detach().to(torch.float32).contiguous()
         gx0 = [torch.empty_like(x) for x in x0s]
         gpsf = torch.empty_like(psf_c)
         plan.backward(w, gx0, gpsf)
-        return (gpsf, None, None, None, None, None, *gx0, *([None] * len(ys)))
+        return (gpsf, None, None, None, None, None, *gx0, *([None] * (ctx.n_inputs - n)))
 
 
 def rl_nll(x0, y, psf, num_iter: int, checkpoint_every: int = 5, mode=PadMode.Reflect,
-           filter_epsilon: float | None = None):
+           filter_epsilon: float | None = None, target=None):
     """Per-tile Poisson NLL after unrolled RL (see the module docstring).
 
     ``x0`` and ``y``: a float32 device tensor ``[N, h, w]`` / ``[N, d, h, w]`` (tiles of one shape)
     or lists of tensors (tiles of different shapes, same rank). ``psf``: float32 device tensor.
-    Returns a float64 tensor of N losses, differentiable w.r.t. ``x0`` and ``psf``.
+    ``target`` (same layout, optional): the counts the NLL is taken against in place of ``y``
+    (RL still runs on ``y``). Returns a float64 tensor of N losses, differentiable w.r.t. ``x0``
+    and ``psf``.
     """
     x0s = list(x0.unbind(0)) if isinstance(x0, torch.Tensor) else list(x0)
     ys = list(y.unbind(0)) if isinstance(y, torch.Tensor) else list(y)
     if len(x0s) != len(ys):
         raise ValueError("x0 and y must hold the same number of tiles")
+    tg = []
+    if target is not None:
+        tg = list(target.unbind(0)) if isinstance(target, torch.Tensor) else list(target)
+        if len(tg) != len(ys):
+            raise ValueError("target and y must hold the same number of tiles")
     eps = None if filter_epsilon is None else float(filter_epsilon)
     return RichardsonLucyNLL.apply(psf, int(num_iter), int(checkpoint_every), int(PadMode(mode)), eps, len(x0s),
-                                   *x0s, *ys)
+                                   *x0s, *ys, *tg)
 
 
 def gaussian_psf(sigma: torch.Tensor, shape: Sequence[int]) -> torch.Tensor:

@@ -760,8 +760,9 @@ static lucem_status upload_tiles(lucem_rl_grad* P) {
     return LUCEM_OK;
 }
 
-extern "C" lucem_status lucem_rl_grad_forward(lucem_rl_grad* P, const float* const* x0, const float* const* y,
-                                              const float* psf, float* const* x_out, double* loss) {
+extern "C" lucem_status lucem_rl_grad_forward_target(lucem_rl_grad* P, const float* const* x0, const float* const* y,
+                                                     const float* const* target, const float* psf,
+                                                     float* const* x_out, double* loss) {
     if (!P) return fail(LUCEM_INVALID_HANDLE, "gradient plan is NULL");
     if (!x0 || !y || !psf) return fail(LUCEM_INVALID_HANDLE, "x0, y or psf is NULL");
     lucem_client* c = P->client;
@@ -771,6 +772,7 @@ extern "C" lucem_status lucem_rl_grad_forward(lucem_rl_grad* P, const float* con
         if (!x0[t] || !y[t]) return fail(LUCEM_INVALID_HANDLE, "tile pointer is NULL");
         P->h_tiles[t].x0 = x0[t];
         P->h_tiles[t].r.obs = y[t];
+        P->h_tiles[t].target = target ? target[t] : nullptr;
         P->h_tiles[t].x_out = x_out ? x_out[t] : nullptr;
     }
     lucem_status st = upload_tiles(P);
@@ -784,6 +786,11 @@ extern "C" lucem_status lucem_rl_grad_forward(lucem_rl_grad* P, const float* con
         return hip_fail(e, "hipMemcpyAsync(loss)");
     P->forward_done = true;
     return LUCEM_OK;
+}
+
+extern "C" lucem_status lucem_rl_grad_forward(lucem_rl_grad* P, const float* const* x0, const float* const* y,
+                                              const float* psf, float* const* x_out, double* loss) {
+    return lucem_rl_grad_forward_target(P, x0, y, nullptr, psf, x_out, loss);
 }
 
 extern "C" lucem_status lucem_rl_grad_backward(lucem_rl_grad* P, const float* loss_weight, float* const* grad_x0,

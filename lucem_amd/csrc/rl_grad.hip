@@ -5,7 +5,7 @@
 //   r_t = y / safe(u_t)  restore/kernels/deconvolve.rs:7-22
 //   c_t = H r_t          H = correlate with h, deconvolve.rs:133
 //   x_{t+1} = x_t * c_t  restore/kernels/deconvolve.rs:24-43
-// loss = sum(u_T - y log u_T), u_T = A x_T.
+// loss = sum(u_T - y' log u_T), u_T = A x_T, with y' = y or a separate target of counts.
 // Backward (reverse mode), with A^T = Pad^T . (correlate with w)^T . Crop^T:
 //   ubar_T = w (1 - y / u_T);  xbar_T = A^T ubar_T
 //   for t = T-1 .. 0:  cbar = xbar_{t+1} x_t;  xa = xbar_{t+1} c_t;  rbar = H^T cbar;
@@ -154,13 +154,13 @@ __device__ void grad_row_body(const GradTile& G, const GradRowBatch& L, int item
                         }
                     } else if constexpr (OP == G_LOSS) {
                         if (own) {
-                            const float y = d.obs[at];
+                            const float y = G.target ? G.target[at] : d.obs[at];
                             G.uT[at] = cv;
                             loss += (double)cv - (double)y * log((double)cv);
                         }
                     } else if constexpr (OP == G_LOSS_ADJ) {
                         const float wgt = G.weight ? G.weight[0] : 1.0f;
-                        v = wgt * (1.0f - d.obs[at] / G.uT[at]);
+                        v = wgt * (1.0f - (G.target ? G.target[at] : d.obs[at]) / G.uT[at]);
                     } else if constexpr (OP == G_ADJ_R) {
                         const float u = (G.uslots + L.su * ns)[at];
                         v = fabsf(u) < d.eps ? 0.f : -cv * d.obs[at] / (u * u);

@@ -49,6 +49,7 @@ struct GradTile {
     double* partial;       // loss partials, one per row block
     float* uT;             // u_T = A x_T
     float* out;            // G_C2R output
+    const float* target;   // the counts the loss compares u_T with (null: y, the RL input)
 };
 
 struct GradRowBatch {

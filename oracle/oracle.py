@@ -285,9 +285,11 @@ class _Fft64:
 
 
 def rl_nll_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
-                 mode: int = PAD_REFLECT) -> float:
-    """FP64 loss ``sum(u - y log u)`` after ``num_iter`` RL iterations from ``x0`` (FFT correlations)."""
+                 mode: int = PAD_REFLECT, target: np.ndarray | None = None) -> float:
+    """FP64 loss ``sum(u - t log u)`` after ``num_iter`` RL iterations on ``y`` from ``x0`` (FFT
+    correlations); ``t`` is ``target`` when given, else ``y``."""
     y = np.asarray(y, dtype=np.float64)
+    t = y if target is None else np.asarray(target, dtype=np.float64)
     x = np.array(x0, dtype=np.float64)
     h, rev = _normalised(psf)
     f = _Fft64(y.shape, h.shape)
@@ -296,17 +298,20 @@ def rl_nll_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, 
         r = y / np.where(np.abs(u) < eps, eps, u)
         x = x * f.corr(_gather64(r, h.shape, mode), h)
     u = f.corr(_gather64(x, h.shape, mode), rev)
-    return float(np.sum(u - y * np.log(u)))
+    return float(np.sum(u - t * np.log(u)))
 
 
 def rl_nll_grad_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: int, eps: float = 1e-12,
-                      mode: int = PAD_REFLECT):
+                      mode: int = PAD_REFLECT, target: np.ndarray | None = None):
     """:func:`rl_nll_grad64` with FFT correlations (for volumes the direct FP64 oracle cannot
     finish): the forward keeps every iteration's padded estimate and ratio, the reverse sweep
     applies the transposes of the gather (fold), the correlation in its input (full convolution)
     and in its window, then the chain rule through ``h = psf / sum(psf)`` (deconvolve.rs:77-79).
-    Pinned against the direct :func:`rl_nll_grad64` on small 2-D and 3-D shapes in every mode."""
+    Pinned against the direct :func:`rl_nll_grad64` on small 2-D and 3-D shapes in every mode.
+    ``target``: the counts the loss compares ``u`` with, in place of ``y`` (only the seed of the
+    reverse sweep changes, ``1 - target / u``); pinned by central differences."""
     y = np.asarray(y, dtype=np.float64)
+    tgt = y if target is None else np.asarray(target, dtype=np.float64)
     x = np.array(x0, dtype=np.float64)
     psf = np.asarray(psf, dtype=np.float64)
     ssum = float(np.sum(psf.ravel()))
@@ -327,8 +332,8 @@ def rl_nll_grad_fft64(y: np.ndarray, x0: np.ndarray, psf: np.ndarray, num_iter: 
         x = x * c
     qx = _gather64(x, ks, mode)
     u = f.corr(qx, rev)
-    loss = float(np.sum(u - y * np.log(u)))
-    ubar = 1.0 - y / u
+    loss = float(np.sum(u - tgt * np.log(u)))
+    ubar = 1.0 - tgt / u
     grev = f.corr_w(qx, ubar)
     gh = np.zeros_like(h)
     xbar = _fold64(f.corr_t(ubar, rev), ns, ks, mode)

@@ -258,3 +258,49 @@ def test_plan_cache_is_bounded(gpu):
     finally:
         AG.set_plan_cache_size(4)
         AG.clear_plan_cache()
+
+
+def test_loss_against_a_separate_target(gpu, oracle):
+    """RL on y, the NLL against another array of counts (the held-out half of a thinning): loss,
+    d/dx0 and d/dpsf against the FP64 oracle, 2-D and 3-D tiles of different shapes in one plan."""
+    import torch
+
+    from lucem_amd.autograd import RLGradPlan
+
+    rng = np.random.default_rng(44)
+    for shapes, k in ((((30, 28), (25, 33)), (5, 5)), (((6, 20, 18), (9, 16, 20)), (3, 5, 5))):
+        ys = [rng.uniform(10, 100, s).astype(np.float32) for s in shapes]
+        ts = [rng.poisson(y).astype(np.float32) for y in ys]
+        psf = rng.uniform(0.1, 1.0, k).astype(np.float32)
+        print(f"target loss: tiles {shapes}, psf {k}", flush=True)
+        plan = RLGradPlan(shapes, k, 4, 2)
+        x0 = [torch.full(s, 0.5, device="cuda") for s in shapes]
+        loss = plan.forward(x0, [torch.from_numpy(y).cuda() for y in ys], torch.from_numpy(psf).cuda(),
+                            targets=[torch.from_numpy(t).cuda() for t in ts])
+        gx = [torch.empty_like(x) for x in x0]
+        gp = torch.empty(k, device="cuda")
+        plan.backward(None, gx, gp)
+        torch.cuda.synchronize()
+        gp_want = np.zeros(k)
+        for j in range(len(shapes)):
+            l64, g64, p64 = oracle.rl_nll_grad_fft64(ys[j], np.full(shapes[j], 0.5), psf, 4, target=ts[j])
+            gp_want += p64
+            assert abs(float(loss[j]) - l64) <= 1e-6 * abs(l64)
+            assert oracle.rel_l2(gx[j].cpu().numpy(), g64) < 1e-3
+        assert oracle.rel_l2(gp.cpu().numpy(), gp_want) < 1e-3
+        plan.close()
+
+
+def test_thinning_splits_counts(gpu):
+    """Binomial thinning: the halves add up to the counts, each half is integral and within the
+    counts, and each takes about half of them."""
+    import torch
+
+    from lucem_amd.psf_fit import thin
+
+    y = torch.poisson(torch.full((64, 256, 256), 40.0, device="cuda"))
+    y1, y2 = thin([y], seed=3)
+    assert torch.equal(y1[0] + y2[0], y)
+    assert bool((y1[0] >= 0).all()) and bool((y2[0] >= 0).all()) and bool((y1[0] == y1[0].round()).all())
+    frac = float(y1[0].sum() / y.sum())
+    assert abs(frac - 0.5) < 2e-3

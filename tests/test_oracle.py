@@ -286,6 +286,32 @@ def test_fft_gradient_matches_the_direct_gradient(oracle, mode, shape, k):
     assert oracle.rel_l2(p2, p1) < 1e-12
 
 
+def test_fft_gradient_with_a_target_matches_finite_differences(oracle):
+    """The loss against a separate target (the held-out half of thinned counts): d/dx0 and d/dpsf
+    of the FFT oracle against FP64 central differences of its own loss, 2-D and 3-D."""
+    rng = np.random.default_rng(21)
+    for shape, k in (((11, 9), (3, 5)), ((5, 6, 7), (3, 3, 3))):
+        y = rng.uniform(10, 100, shape)
+        t = rng.uniform(10, 100, shape)
+        x0 = rng.uniform(0.5, 2.0, shape)
+        psf = rng.uniform(0.1, 1.0, k)
+        loss, gx, gp = oracle.rl_nll_grad_fft64(y, x0, psf, 3, target=t)
+        assert abs(loss - oracle.rl_nll_fft64(y, x0, psf, 3, target=t)) <= 1e-12 * abs(loss)
+        for idx in [(0,) * len(shape), tuple(s // 2 for s in shape)]:
+            h = 1e-5
+            xp, xm = x0.copy(), x0.copy()
+            xp[idx] += h
+            xm[idx] -= h
+            fd = (oracle.rl_nll_fft64(y, xp, psf, 3, target=t) - oracle.rl_nll_fft64(y, xm, psf, 3, target=t)) / (2 * h)
+            assert abs(fd - gx[idx]) <= 1e-5 * abs(fd) + 1e-7
+        idx = tuple(s // 2 for s in k)
+        pp, pm = psf.copy(), psf.copy()
+        pp[idx] += 1e-6
+        pm[idx] -= 1e-6
+        fd = (oracle.rl_nll_fft64(y, x0, pp, 3, target=t) - oracle.rl_nll_fft64(y, x0, pm, 3, target=t)) / 2e-6
+        assert abs(fd - gp[idx]) <= 1e-4 * max(abs(fd), 1e-3)
+
+
 def test_fft_gradient_planted_mistake_is_caught(oracle):
     """Dropping the fold of the padded positions (the transpose of the boundary gather) in the
     FFT gradient, a plausible mistake, moves d/dx0 far beyond the tolerance above."""
