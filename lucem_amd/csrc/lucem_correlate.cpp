@@ -102,6 +102,12 @@ lucem_status validate_correlate(const void* input, const uint64_t* in_dims, uint
         o3[3 - in_ndim + i] = s->origin[i];
     }
     if (!fft) {
+        // the direct kernels index an axis's padded extent n + k - 1 with 32-bit integers
+        for (int a = 0; a < 3; a++)
+            if (n3[a] + k3[a] - 1 > 0x7FFFFFFF)
+                return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with weights " +
+                                                         dims_str(w_dims, w_ndim) +
+                                                         ": an axis with its window is longer than 2147483647");
         std::memset(g, 0, sizeof *g);
         g->ndim = (int)in_ndim;
         for (int a = 0; a < 3; a++) {

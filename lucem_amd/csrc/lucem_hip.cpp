@@ -890,6 +890,12 @@ lucem_status lucem::validate_rl(const void* input, const uint64_t* in_dims, uint
     for (uint32_t i = 0; i < in_ndim; i++) n3[3 - in_ndim + i] = (int64_t)in_dims[i];
     for (uint32_t i = 0; i < psf_ndim; i++) k3[3 - psf_ndim + i] = (int64_t)psf_dims[i];
     if (!fft) {  // the direct (f64) form: sizes only
+        // the direct kernels index an axis's padded extent n + k - 1 with 32-bit integers
+        for (int a = 0; a < 3; a++)
+            if (n3[a] + k3[a] - 1 > 0x7FFFFFFF)
+                return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with psf " +
+                                                         dims_str(psf_dims, psf_ndim) +
+                                                         ": an axis with the PSF is longer than 2147483647");
         std::memset(g, 0, sizeof *g);
         g->ndim = (int)in_ndim;
         for (int a = 0; a < 3; a++) {

@@ -11,6 +11,9 @@
 // otherwise straight from global memory; arrays 64 or more wide take 64x16 tiles when those fit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "fft_lds.hpp"
 #include "rl_internal.hpp"
 
@@ -25,41 +28,45 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_kernel(const T* __re
                                                                    int ly, int lx, int mode, T cval) {
     extern __shared__ __align__(16) unsigned char tile_raw[];
     T* tile = reinterpret_cast<T*>(tile_raw);
-    // one flat grid of blocks, row-major over the output (no limit on the number of block rows)
-    const int nbx = (W + kTile - 1) / kTile;
-    const int bx = (int)(blockIdx.x % (unsigned)nbx), by = (int)(blockIdx.x / (unsigned)nbx);
+    // output tiles row-major, the blocks striding over them (any array height or width)
+    const int nbx = (int)(((int64_t)W + kTile - 1) / kTile);
+    const int64_t ntiles = (int64_t)nbx * (((int64_t)H + kTile - 1) / kTile);
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    const int x0 = bx * kTile, y0 = by * kTile;
-    const int x = x0 + tx, y = y0 + ty;
-    if constexpr (STAGED) {
-        const int tw = kTile + kw - 1, th = kTile + kh - 1;
-        for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
-            const int r = i / tw, c = i - r * tw;
-            const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
-            tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
-        }
-        __syncthreads();
-        if (x < W && y < H) {
-            T acc = T(0);
-            for (int ky = 0; ky < kh; ky++) {
-                const T* row = tile + (ty + ky) * tw + tx;
-                const T* wr = w + ky * kw;
-                for (int kx = 0; kx < kw; kx++) acc += wr[kx] * row[kx];
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int bx = (int)(t % nbx), by = (int)(t / nbx);
+        const int x0 = bx * kTile, y0 = by * kTile;
+        const int x = x0 + tx, y = y0 + ty;
+        if constexpr (STAGED) {
+            const int tw = kTile + kw - 1, th = kTile + kh - 1;
+            __syncthreads();  // the previous tile's sums have read the staged values
+            for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
+                const int r = i / tw, c = i - r * tw;
+                const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
+                tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
             }
-            out[(int64_t)y * W + x] = acc;
-        }
-    } else {
-        if (x < W && y < H) {
-            T acc = T(0);
-            for (int ky = 0; ky < kh; ky++) {
-                const int py = pad_index(y - ly + ky, H, mode);
-                for (int kx = 0; kx < kw; kx++) {
-                    const int px = pad_index(x - lx + kx, W, mode);
-                    const T s = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
-                    acc += w[ky * kw + kx] * s;
+            __syncthreads();
+            if (x < W && y < H) {
+                T acc = T(0);
+                for (int ky = 0; ky < kh; ky++) {
+                    const T* row = tile + (ty + ky) * tw + tx;
+                    const T* wr = w + ky * kw;
+                    for (int kx = 0; kx < kw; kx++) acc += wr[kx] * row[kx];
                 }
+                out[(int64_t)y * W + x] = acc;
             }
-            out[(int64_t)y * W + x] = acc;
+        } else {
+            if (x < W && y < H) {
+                T acc = T(0);
+                for (int ky = 0; ky < kh; ky++) {
+                    const int py = pad_index(y - ly + ky, H, mode);
+                    for (int kx = 0; kx < kw; kx++) {
+                        const int px = pad_index(x - lx + kx, W, mode);
+                        const T s = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
+                        acc += w[ky * kw + kx] * s;
+                    }
+                }
+                out[(int64_t)y * W + x] = acc;
+            }
         }
     }
 }
@@ -76,33 +83,38 @@ __global__ __launch_bounds__(kTile* kTile) void correlate2d_wide_kernel(const T*
                                                                         int mode, T cval) {
     extern __shared__ __align__(16) unsigned char tile_raw[];
     T* tile = reinterpret_cast<T*>(tile_raw);
-    const int nbx = (W + kWide - 1) / kWide;
-    const int bx = (int)(blockIdx.x % (unsigned)nbx), by = (int)(blockIdx.x / (unsigned)nbx);
+    const int nbx = (int)(((int64_t)W + kWide - 1) / kWide);
+    const int64_t ntiles = (int64_t)nbx * (((int64_t)H + kTile - 1) / kTile);
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    const int x0 = bx * kWide, y0 = by * kTile;
-    const int y = y0 + ty;
     const int tw = kWide + kw - 1, th = kTile + kh - 1;
-    for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
-        const int r = i / tw, c = i - r * tw;
-        const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
-        tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
-    }
-    __syncthreads();
-    if (y >= H) return;
-    T acc[4] = {T(0), T(0), T(0), T(0)};
-    for (int ky = 0; ky < kh; ky++) {
-        const T* row = tile + (ty + ky) * tw + tx;
-        const T* wr = w + ky * kw;
-        for (int kx = 0; kx < kw; kx++) {
-            const T wt = wr[kx];
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[u] += wt * row[kx + kTile * u];
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int bx = (int)(t % nbx), by = (int)(t / nbx);
+        const int x0 = bx * kWide, y0 = by * kTile;
+        const int y = y0 + ty;
+        __syncthreads();  // the previous tile's sums have read the staged values
+        for (int i = threadIdx.x; i < tw * th; i += kTile * kTile) {
+            const int r = i / tw, c = i - r * tw;
+            const int py = pad_index(y0 + r - ly, H, mode), px = pad_index(x0 + c - lx, W, mode);
+            tile[i] = (py < 0 || px < 0) ? cval : in[(int64_t)py * W + px];
         }
-    }
+        __syncthreads();
+        if (y < H) {
+            T acc[4] = {T(0), T(0), T(0), T(0)};
+            for (int ky = 0; ky < kh; ky++) {
+                const T* row = tile + (ty + ky) * tw + tx;
+                const T* wr = w + ky * kw;
+                for (int kx = 0; kx < kw; kx++) {
+                    const T wt = wr[kx];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int x = x0 + tx + kTile * u;
-        if (x < W) out[(int64_t)y * W + x] = acc[u];
+                    for (int u = 0; u < 4; u++) acc[u] += wt * row[kx + kTile * u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int x = x0 + tx + kTile * u;
+                if (x < W) out[(int64_t)y * W + x] = acc[u];
+            }
+        }
     }
 }
 
@@ -159,18 +171,29 @@ __global__ __launch_bounds__(256) void psf_prep_f64_kernel(const double* __restr
 
 static dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// Blocks of a direct-correlation launch: one per output tile up to 2^20 (256 threads each), which
+// then stride over the rest.
+static dim3 tile_grid(int64_t ntiles) { return dim3((unsigned)std::min<int64_t>(std::max<int64_t>(ntiles, 1), 1 << 20)); }
+
+// LUCEM_DIRECT_TILE=16 keeps the 16x16 output tiles where the 64x16 ones would run (the two sum
+// every output in the same order, so they agree bit for bit; tests compare them).
+static bool wide_tiles_allowed() {
+    const char* v = std::getenv("LUCEM_DIRECT_TILE");
+    return !(v && std::atoi(v) == 16);
+}
+
 template <typename T>
 static int correlate2d(const T* in, const T* w, T* out, int H, int W, int kh, int kw, int ly, int lx, int mode, T cval,
                        void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const size_t wide_lds = (size_t)(kWide + kw - 1) * (kTile + kh - 1) * sizeof(T);
-    if (W >= kWide && wide_lds <= kDirectTileLds) {
-        const dim3 wgrid((unsigned)(((int64_t)W + kWide - 1) / kWide * (((int64_t)H + kTile - 1) / kTile)));
+    if (W >= kWide && wide_lds <= kDirectTileLds && wide_tiles_allowed()) {
+        const dim3 wgrid = tile_grid(((int64_t)W + kWide - 1) / kWide * (((int64_t)H + kTile - 1) / kTile));
         hipLaunchKernelGGL(correlate2d_wide_kernel<T>, wgrid, dim3(kTile * kTile), wide_lds, st, in, w, out, H, W, kh,
                            kw, ly, lx, mode, cval);
         return (int)hipGetLastError();
     }
-    const dim3 grid((unsigned)(((int64_t)W + kTile - 1) / kTile * (((int64_t)H + kTile - 1) / kTile)));
+    const dim3 grid = tile_grid(((int64_t)W + kTile - 1) / kTile * (((int64_t)H + kTile - 1) / kTile));
     const size_t lds = (size_t)(kTile + kw - 1) * (kTile + kh - 1) * sizeof(T);
     if (lds <= kDirectTileLds)
         hipLaunchKernelGGL((correlate2d_kernel<T, true>), grid, dim3(kTile * kTile), lds, st, in, w, out, H, W, kh, kw,

@@ -432,3 +432,23 @@ def test_destroy_functions_accept_null(L):
     """Destroying a NULL handle is a no-op (as freeing a null pointer)."""
     assert L.lucem_client_destroy(None) == 0
     assert L.lucem_workspace_destroy(None) == 0
+
+
+def test_direct_form_rejects_axes_beyond_int32(L):
+    """An axis whose padded extent n + k - 1 exceeds 2^31 - 1 (possible within the 2^32 - 1 element
+    limit when the other axis is 1) is refused before any device work: the direct kernels index
+    within an axis with 32-bit integers. One element less fits (valid arguments, no client)."""
+    from lucem_amd import _native as N
+
+    big = (1, 2147483647)
+    assert _call(L, big, (1, 3), big, out=1 << 45) == 9
+    assert b"2147483647" in L.lucem_last_error()
+    assert _call(L, (1, 2147483645), (1, 3), (1, 2147483645), out=1 << 45) == 11
+    s = N.RlSettings()
+    L.lucem_rl_settings_default(ctypes.byref(s))
+    assert L.lucem_richardson_lucy_into_f64(None, N._vp(0x10000), N.dims(big), 2, N._vp(0x30000000), N.dims((1, 3)),
+                                            2, N._vp(1 << 45), N.dims(big), 2, ctypes.byref(s), None) == 9
+    cs = N.CorrelateSettings()
+    L.lucem_correlate_settings_default(ctypes.byref(cs))
+    assert L.lucem_correlate_2d_into_f64(None, N._vp(0x10000), N.dims(big), 2, N._vp(0x30000000), N.dims((1, 3)), 2,
+                                         N._vp(1 << 45), N.dims(big), 2, ctypes.byref(cs)) == 9

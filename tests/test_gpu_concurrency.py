@@ -75,3 +75,51 @@ def test_two_clients_on_two_threads_match_sequential(gpu, monkeypatch):
         assert results[i] is not None
         for got, want in zip(results[i], alone[i]):
             np.testing.assert_array_equal(got, want)
+
+
+def test_default_stream_client_beside_a_capturing_thread(gpu):
+    """A client on the null stream (torch's default) uploads twiddle tables and tuning inputs through
+    that stream while another thread creates batched plans, whose creation captures a graph on a
+    private non-blocking stream in thread-local capture mode: both sides complete, and the results
+    equal the ones computed alone."""
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+    from lucem_amd.batched import RLBatch
+
+    rng = np.random.default_rng(77)
+    shapes = [(90 + 7 * i, 70 + 5 * i) for i in range(6)]  # new geometries: new twiddles and tuning
+    ys = [_dev(rng.uniform(10, 100, s)) for s in shapes]
+    p = _dev(rng.uniform(0.1, 1.0, (7, 7)))
+    s = LA.RichardsonLucy(num_iter=5, clip=False)
+    torch.cuda.synchronize()
+    default = N.Client(0)  # the null stream: not an own stream
+    assert default.stream in (0, None)
+    outs = [torch.empty_like(y) for y in ys]
+    errors = []
+
+    def plans():
+        try:
+            other = N.Client(0, own_stream=True)
+            for i in range(4):
+                t = [_dev(rng.uniform(10, 100, (40 + i, 50))), _dev(rng.uniform(10, 100, (33, 61 + i)))]
+                b = RLBatch([tuple(x.shape) for x in t], [(7, 7)], [0, 0], s, client=other)
+                b.execute(t, [torch.empty_like(x) for x in t], [p])
+                b.close()
+            other.sync()
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    th = threading.Thread(target=plans)
+    th.start()
+    for y, o in zip(ys, outs):
+        LA.richardson_lucy_into(y, p, o, s, None, default)
+    default.sync()
+    th.join(timeout=120)
+    assert not errors, errors
+    for y, o in zip(ys, outs):
+        ref = torch.empty_like(y)
+        LA.richardson_lucy_into(y, p, ref, s, None, default)
+        default.sync()
+        np.testing.assert_array_equal(o.cpu().numpy(), ref.cpu().numpy())

@@ -296,3 +296,37 @@ def test_correlate_1d_large_strided_axis(gpu, mode):
     got = LA.correlate_1d(torch.from_numpy(x.astype(np.float32)).cuda(), w.astype(np.float32).tolist(),
                           s).cpu().numpy()
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,kshape,mode", [((125, 98), (5, 7), 0), ((48, 80), (3, 3), 3), ((70, 130), (9, 5), 4),
+                                               ((17, 64), (4, 6), 1), ((33, 200), (11, 13), 2)])
+def test_direct_wide_tiles_equal_square_tiles(gpu, monkeypatch, dtype, shape, kshape, mode):
+    """The 64x16 output tiles (four outputs per thread) and the 16x16 ones sum every output in the
+    same order: equal bit for bit, on ragged widths, in f32 and f64 (widths of 64 and more take the
+    wide tiles; LUCEM_DIRECT_TILE=16 forces the square ones)."""
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(sum(shape))
+    x = _dev(rng.uniform(-1, 1, shape), dtype)
+    w = _dev(rng.uniform(-1, 1, kshape), dtype)
+    s = LA.Correlate2d(mode=mode, cval=0.25 if mode == 1 else 0.0)
+    print(f"direct tiles: {shape} window {kshape} mode {mode} {np.dtype(dtype).name}", flush=True)
+    wide = LA.correlate_2d(x, w, s).cpu().numpy()
+    monkeypatch.setenv("LUCEM_DIRECT_TILE", "16")
+    square = LA.correlate_2d(x, w, s).cpu().numpy()
+    np.testing.assert_array_equal(wide, square)
+
+
+def test_direct_grid_strides_past_a_million_tiles(gpu, oracle):
+    """A 20,000,000 x 1 column has 1.25 million 16x16 output tiles, more than one launch's blocks
+    (2^20): the blocks stride over the rest. Checked against the oracle's correlation."""
+    import lucem_amd as LA
+
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0, 1, (20_000_000, 1))
+    w = rng.uniform(0, 1, (5, 1))
+    print("direct grid: 20000000 x 1, window 5 x 1, f64", flush=True)
+    got = LA.correlate_2d(_dev(x, np.float64), _dev(w, np.float64), LA.Correlate2d()).cpu().numpy()
+    want = oracle.correlate(x, w, 0)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
