@@ -21,9 +21,15 @@ Alongside the throughput the line carries:
   OpenMP) timed on this host on a bounded sample of the same workload;
 - ``batched``: the uneven-tile slide workload (300 tiles of 64..512 px per side, 5 Gaussian PSFs
   of 11..21 px, 20 iterations) through the batched form (one graph launch per call) against the
-  plain 2-D entry point called once per tile, with both results compared;
+  plain 2-D entry point called once per tile, with both results compared; the achieved bandwidth
+  from the algorithmic bytes and from the HBM bytes the committed rocprofv3 counters measured;
 - ``grad``: the differentiable form on 64 tiles of 256x256, PSF 17x17, 10 iterations, checkpoint
-  every 5: forward time, forward + backward (d/dx0 and d/dpsf) time and their ratio.
+  every 5: forward time, forward + backward (d/dx0 and d/dpsf) time and their ratio, also against the
+  plain batched forward of the same tiles;
+- ``plain2d`` (with the default ``plain3d`` headline): the same fields for the 1024x1024 workload,
+  with its own roofline and CPU baseline;
+- ``psf_fit``: milliseconds per Adam step of the PSF fit (16 volumes of 32..128 x 256x256, PSF
+  31x15x15, 10 iterations, one batched forward + backward per step);
 - ``wiener``: Wiener deconvolution of a 2048x2048 image with a 25x25 Gaussian PSF (7-smooth sizes:
   the R2C passes; 2039x2048: the general FFT with Bluestein columns), the unsupervised form on
   1024x1024 (30 iterations, burn-in 10), and the FP64 numpy restatement timed on this host;
@@ -157,7 +163,16 @@ def batched_leg(steps: int) -> dict:
     nvox = sum(y.numel() for y in ys)
     alg = sum(minimal_domain_bytes(tuple(y.shape), tuple(psfs[i].shape)) for y, i in zip(ys, ts.psf_index))
     gbps = alg * ts.num_iter / tb / 1e9
+    # HBM bytes the counters measured per launch of each pass (profiles/traffic_batched.json) times
+    # the launches of one call: 2 correlations and 1 ratio and 1 update pass per iteration
+    counted = None
+    per = {k: load_traffic("batched", k) for k in ("line_fused_y", "row_ratio", "row_update")}
+    if all(v is not None for v in per.values()):
+        counted = ts.num_iter * (2 * per["line_fused_y"] + per["row_ratio"] + per["row_update"])
     return {
+        "counter_bytes_per_call": counted,
+        "achieved_GBps_counters": round(counted / tb / 1e9, 1) if counted else None,
+        "frac_counters": round(counted / tb / 1e9 / HBM_PEAK_GBPS, 4) if counted else None,
         "workload": f"{len(ys)} tiles (64..512 px per side), {len(psfs)} PSFs, {ts.num_iter} iterations",
         "ms_per_step": round(tb * 1e3, 3),
         "value": round(nvox * ts.num_iter / tb / 1e6, 2),
@@ -176,8 +191,10 @@ def batched_leg(steps: int) -> dict:
 def grad_leg(steps: int) -> dict:
     import torch
 
+    import lucem_amd as LA
     from lucem_amd import synth
     from lucem_amd.autograd import RLGradPlan
+    from lucem_amd.batched import RLBatch
 
     ys, psf, _, T = synth.grad2d()
     y = torch.from_numpy(ys).cuda()
@@ -193,16 +210,119 @@ def grad_leg(steps: int) -> dict:
         plan.forward(xs, yl, p)
         plan.backward(None, gx, gp)
 
+    # the plain batched forward of the same tiles (RL alone, no checkpoints, loss or saved spectra)
+    fwd = RLBatch([tuple(y.shape[1:])] * n, [tuple(psf.shape)], [0] * n, LA.RichardsonLucy(num_iter=T, clip=False))
+    outs = [torch.empty_like(a) for a in yl]
     reps = max(steps, 5)
     tf = _timeit(lambda: plan.forward(xs, yl, p), reps)
+    tb = _timeit(lambda: fwd.execute(yl, outs, [p]), reps)
     tg = _timeit(step, reps)
     return {
         "workload": f"{n} tiles of 256x256, psf 17x17, {T} iterations, checkpoint every 5",
         "forward_ms": round(tf * 1e3, 4),
+        "batched_forward_ms": round(tb * 1e3, 4),
         "grad_step_ms": round(tg * 1e3, 4),
         "ratio": round(tg / tf, 3),
+        "ratio_vs_batched_forward": round(tg / tb, 3),
         "device_bytes": plan.device_bytes,
     }
+
+
+def psf_fit_leg(steps: int) -> dict:
+    """The PSF-fitting workload: 16 volumes of 32..128 x 256 x 256 (generated on the device from the
+    workload's geometry), sigma of a 31 x 15 x 15 Gaussian PSF, one Adam step = one batched forward +
+    backward through 10 RL iterations."""
+    import torch
+
+    from lucem_amd import phantoms
+    from lucem_amd.psf_fit import fit_gaussian_sigma
+
+    vols, _ = phantoms.psffit3d()
+    fit_gaussian_sigma(vols, (31, 15, 15), (2.5, 1.0, 1.0), steps=1)  # plans, tuning
+    torch.cuda.synchronize()
+    n = max(steps, 3)
+    res = fit_gaussian_sigma(vols, (31, 15, 15), (2.5, 1.0, 1.0), steps=n)
+    return {"workload": "16 volumes of 32..128 x 256 x 256, psf 31x15x15, 10 iterations, Adam",
+            "depths": [int(v.shape[0]) for v in vols], "ms_per_adam_step": round(res.ms_per_step, 3),
+            "steps_timed": n}
+
+
+def plain_leg(workload: str, steps: int, warmup: int, with_cpu: bool) -> dict:
+    """One plain-form workload: throughput, the dominant kernel's roofline, the iteration's
+    algorithmic bandwidth, per-kernel times and (optionally) the CPU baseline."""
+    import numpy as np
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd import _native as N
+    from lucem_amd import synth
+
+    prob = synth.plain3d() if workload == "plain3d" else synth.plain2d()
+    y = torch.from_numpy(prob.image).cuda()
+    psf = torch.from_numpy(prob.psf).cuda()
+    out = torch.empty_like(y)
+    client = N.default_client(0)
+    ws = N.Workspace(client)
+    settings = LA.RichardsonLucy(num_iter=prob.num_iter, clip=False)
+    run = LA.richardson_lucy_nd_into if y.dim() == 3 else LA.richardson_lucy_into
+
+    for _ in range(warmup):
+        run(y, psf, out, settings, ws, client)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(y, psf, out, settings, ws, client)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # per-kernel times in separate calls: the events would otherwise sit in the timed region
+    client.profile(True)
+    for _ in range(max(2, min(steps, 5))):
+        run(y, psf, out, settings, ws, client)
+    stats = client.profile_read()
+    client.profile(False)
+    assert bool(torch.isfinite(out).all()), "non-finite output"
+
+    nvox = int(np.prod(prob.image.shape))
+    iter_kernels = {k: v for k, v in stats.items() if not k.startswith("otf") and k != "psf_prep"}
+    dom_name, dom = max(iter_kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernel_ms = sum(v["total_ms"] for v in stats.values())
+    iter_bytes = minimal_domain_bytes(prob.image.shape, prob.psf.shape)
+    iter_s = elapsed / (steps * prob.num_iter)
+    leg = {
+        "workload": prob.meta["workload"] + f", {prob.num_iter} iterations, Reflect, clip off",
+        "shape": list(prob.image.shape),
+        "psf": list(prob.psf.shape),
+        "num_iter": prob.num_iter,
+        "ms_per_step": round(elapsed * 1e3 / steps, 3),
+        "value": round(nvox * prob.num_iter * steps / elapsed / 1e6, 2),
+        "unit": "Mvoxel-iters/s",
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": load_traffic(workload, dom_name),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(avg_ms, 5),
+        },
+        "iteration": {
+            "algorithmic_bytes": iter_bytes,
+            "ms": round(iter_s * 1e3, 4),
+            "achieved_GBps": round(iter_bytes / iter_s / 1e9, 1),
+            "frac": round(iter_bytes / iter_s / 1e9 / HBM_PEAK_GBPS, 4),
+        },
+        "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 5),
+                        "share": round(v["total_ms"] / kernel_ms, 4),
+                        "GBps": round(v["bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)}
+                    for k, v in stats.items()},
+        "cpu_baseline": cpu_baseline(prob, workload) if with_cpu else None,
+    }
+    return leg
 
 
 def _gaussian_psf(k: int, sigma: float):
@@ -390,102 +510,47 @@ def main() -> int:
               "array per GPU; tiles and volumes shard across GPUs without exchange", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
-
-    import lucem_amd as LA
-    from lucem_amd import _native as N
-    from lucem_amd import synth
 
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         return 1
-    prob = synth.plain3d() if args.workload == "plain3d" else synth.plain2d()
-    y = torch.from_numpy(prob.image).cuda()
-    psf = torch.from_numpy(prob.psf).cuda()
-    out = torch.empty_like(y)
-    client = N.default_client(0)
-    ws = N.Workspace(client)
-    settings = LA.RichardsonLucy(num_iter=prob.num_iter, clip=False)
-    run = LA.richardson_lucy_nd_into if y.dim() == 3 else LA.richardson_lucy_into
-
-    for _ in range(args.warmup):
-        run(y, psf, out, settings, ws, client)
-    torch.cuda.synchronize()
-    client.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run(y, psf, out, settings, ws, client)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats = client.profile_read()
-    client.profile(False)
-    assert bool(torch.isfinite(out).all()), "non-finite output"
-
-    nvox = int(np.prod(prob.image.shape))
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = nvox * prob.num_iter * args.steps / elapsed / 1e6
-    iter_kernels = {k: v for k, v in stats.items() if not k.startswith("otf") and k != "psf_prep"}
-    dom_name, dom = max(iter_kernels.items(), key=lambda kv: kv[1]["total_ms"])
-    avg_ms = dom["total_ms"] / dom["launches"]
-    bytes_per_launch = dom["bytes"] / dom["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = load_traffic(args.workload, dom_name)
-    kernel_ms = sum(v["total_ms"] for v in stats.values())
-    iter_bytes = minimal_domain_bytes(prob.image.shape, prob.psf.shape)
-    iter_s = elapsed / (args.steps * prob.num_iter)
+    head = plain_leg(args.workload, args.steps, args.warmup, not args.no_cpu_baseline)
     line = {
         "metric": "RL Mvoxel-iters/s",
-        "value": round(value, 2),
+        "value": head["value"],
         "unit": "Mvoxel-iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": prob.meta["workload"] + f", {prob.num_iter} iterations, Reflect, clip off",
-            "shape": list(prob.image.shape),
-            "psf": list(prob.psf.shape),
-            "num_iter": prob.num_iter,
+            "workload": head["workload"],
+            "shape": head["shape"],
+            "psf": head["psf"],
+            "num_iter": head["num_iter"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom_name,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": round(avg_ms, 5),
-        },
-        "iteration": {
-            "algorithmic_bytes": iter_bytes,
-            "ms": round(iter_s * 1e3, 4),
-            "achieved_GBps": round(iter_bytes / iter_s / 1e9, 1),
-            "frac": round(iter_bytes / iter_s / 1e9 / HBM_PEAK_GBPS, 4),
-        },
-        "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["total_ms"] / v["launches"], 5),
-                        "share": round(v["total_ms"] / kernel_ms, 4),
-                        "GBps": round(v["bytes"] / (v["total_ms"] * 1e-3) / 1e9, 1)}
-                    for k, v in stats.items()},
-        "cpu_baseline": None,
+        "roofline": head["roofline"],
+        "iteration": head["iteration"],
+        "kernels": head["kernels"],
+        "cpu_baseline": head["cpu_baseline"],
     }
     if not args.no_extras:
+        other = "plain2d" if args.workload == "plain3d" else "plain3d"
+        line[other] = plain_leg(other, max(args.steps, 10) if other == "plain2d" else args.steps, 2,
+                                not args.no_cpu_baseline)
         line["batched"] = batched_leg(args.steps)
         line["grad"] = grad_leg(args.steps)
+        line["psf_fit"] = psf_fit_leg(args.steps)
         line["wiener"] = wiener_leg(args.steps, not args.no_cpu_baseline)
         line["fft"] = fft_leg(args.steps)
         line["tiled"] = tiled_leg(args.steps)
         line["filter"] = filter_leg(args.steps, not args.no_cpu_baseline)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(prob, args.workload)
     print(json.dumps(line))
     return 0
 

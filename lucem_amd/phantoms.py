@@ -126,3 +126,21 @@ def plain3d(shape=(128, 512, 512), count: int = 1500, sigma=(4.0, 1.5, 1.5), k=(
     y = poisson(b, 2001)
     return DeviceProblem(y, torch.from_numpy(psf.astype(np.float32)).cuda(), num_iter, truth,
                          {"workload": f"plain3d {shape[0]}x{shape[1]}x{shape[2]} psf {k[0]}x{k[1]}x{k[2]} (device)"})
+
+
+def psffit3d(count: int = 16, side: int = 256, depth=(32, 128), sigma=(3.5, 1.4, 1.6), k=(31, 15, 15),
+             spheres_per_mvox: float = 1500 / 33.554432, client: N.Client | None = None):
+    """The volumes of :func:`lucem_amd.synth.psffit3d` (geometry seed 1004: the same depths and
+    spheres) rendered, blurred by the true PSF and given Poisson noise on the device (noise seed
+    2004, the reference's generator). Returns (volumes, psf) as float32 device tensors."""
+    import torch
+
+    geo = np.random.default_rng(1004)
+    psf = S.gaussian_psf(sigma, k)
+    vols = []
+    for i in range(count):
+        d = int(geo.integers(depth[0], depth[1] + 1))
+        n = max(int(spheres_per_mvox * d * side * side / 1e6), 1)
+        truth = render_spheres((d, side, side), S.sphere_params((d, side, side), n, geo), 100.0, client)
+        vols.append(poisson(blur(truth, psf), 2004 + i))
+    return vols, torch.from_numpy(psf.astype(np.float32)).cuda()
