@@ -378,10 +378,27 @@ static bool smooth7(int v) {
     return v == 1;
 }
 
+// The smallest 7-smooth length >= target, or a longer one within 10% when it needs fewer radix
+// passes. Every pass kernel streams its lines through HBM once, which costs about as much as two
+// LDS passes, so a length costs about L * (passes + 2): at 1048 that picks 1080 (radices 12, 10,
+// 9) over 1050 (10, 7, 5, 3). The search stays within the first length's thread tier and within
+// LUCEM_MAX_FFT_LEN.
 int lucem::choose_fft_len(int target) {
     int L = std::max(target, 1);
     while (!smooth7(L)) L++;
-    return L;
+    if (L > LUCEM_MAX_FFT_LEN) return L;
+    const int lim = std::min(LUCEM_MAX_FFT_LEN, L + L / 10);
+    int best = L;
+    int64_t cost = (int64_t)L * (make_plan(L).npass + 2);
+    for (int c = L + 1; c <= lim; c++) {
+        if (!smooth7(c) || tier_for_length(c) != tier_for_length(L)) continue;
+        const int64_t cc = (int64_t)c * (make_plan(c).npass + 2);
+        if (cc < cost) {
+            best = c;
+            cost = cc;
+        }
+    }
+    return best;
 }
 
 FftPlan lucem::make_plan(int L, int variant) {

@@ -53,9 +53,9 @@ class RichardsonLucyOp:
         if int(self.num_iter) < 1:
             raise ValueError("invalid parameter `num_iter`: num_iter must be >= 1")
         if self.filter_epsilon is not None and not math.isfinite(float(self.filter_epsilon)):
-            raise ValueError("filter epsilon must be a number")
+            raise ValueError("invalid parameter `filter_epsilon`: not a finite number")
         if int(self.psf_size) < 1:
-            raise ValueError("psf size must be at least 1")
+            raise ValueError("invalid parameter `psf_size`: psf_size must be >= 1")
 
     def params(self) -> dict:
         """The operator's parameters as the pipeline's JSON carries them (``op`` tag excluded)."""

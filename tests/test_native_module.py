@@ -36,7 +36,7 @@ def test_halo_and_spec_checks():
     assert M.op_halo(_spec(num_iter=3, psf_size=31)) == 3 * 2 * 15
     assert M.op_halo(_spec(num_iter=2, psf_size=4)) == 2 * 2 * 2
     for bad, msg in [(_spec(num_iter=0), "num_iter"), (_spec(mode="edge"), "unknown variant"),
-                     (_spec(filter_epsilon="x"), "filter_epsilon"), (_spec(psf_size=0), "psf size"),
+                     (_spec(filter_epsilon="x"), "filter_epsilon"), (_spec(psf_size=0), "psf_size"),
                      (json.dumps({"op": "richardson_lucy", "num_iter": 3}), "missing field"),
                      (_spec(sigma=2.0), "unknown field"), (json.dumps({"op": "gaussian_blur", "sigma": 2.0}),
                                                           "unknown variant"), ("{", "JSON")]:
