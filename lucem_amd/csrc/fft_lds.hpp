@@ -252,6 +252,30 @@ struct Divisor {
     __device__ __forceinline__ int div(int u) const { return d > 1 ? (int)__umulhi((unsigned)u, m) : u; }
 };
 
+// The arguments of a non-inlined call arrive in VGPRs even when every lane holds the same value.
+// Reading them back with readfirstlane puts them in SGPRs, so the index arithmetic on them runs on
+// the scalar unit and the branches on them are scalar branches.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ unsigned uniform(unsigned x) { return (unsigned)__builtin_amdgcn_readfirstlane((int)x); }
+// An empty asm statement that the compiler cannot see through: a scalar offset kept in an SGPR as
+// computed (not folded back into a per-lane multiply-add), or a per-lane running sum kept as adds.
+__device__ __forceinline__ int in_sgpr(int x) {
+    asm("" : "+s"(x));
+    return x;
+}
+__device__ __forceinline__ int in_vgpr(int x) {
+    asm("" : "+v"(x));
+    return x;
+}
+// Product of two values below 2^24 on the 24-bit multiplier (full rate).
+__device__ __forceinline__ int mul24(int a, int b) { return (int)(((unsigned)a & 0xFFFFFFu) * ((unsigned)b & 0xFFFFFFu)); }
+template <class P>
+__device__ __forceinline__ P* uniform(P* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned long long lo = uniform((unsigned)v), hi = uniform((unsigned)(v >> 32));
+    return (P*)((hi << 32) | lo);
+}
+
 // One in-place Stockham pass of radix R = R1*R2 over B interleaved lines of length N (B is any
 // count >= 1). Input element j + t*N/R of butterfly j is multiplied by w^(t*k), k = j mod Ns,
 // w = exp(-+2*pi*i/(Ns*R)), transformed, and written to (j - k)*R + k + t*Ns. With the
@@ -261,22 +285,45 @@ struct Divisor {
 // that register budget (a copy shared by kernels of different budgets gets the smallest).
 // Ns = the product of the earlier radices, ns_mag its multiply-high reciprocal (0 when Ns = 1) and
 // rem = N / Ns, so that no pass divides by a value known only at run time.
+// Index arithmetic: the per-line strides (N/R)*B, Ns*B and (R - 1)*B are uniform (SGPRs); the
+// per-lane products take the 24-bit multiplier (every index is below 2^17), and the line data is
+// addressed as LDS, so that each access costs one vector add.
+typedef float lds_vec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) lds_vec2 lds_c;
+__device__ __forceinline__ float2 lds_get(const lds_c* p, int i) {
+    const lds_vec2 x = p[i];
+    return make_float2(x.x, x.y);
+}
+__device__ __forceinline__ void lds_put(lds_c* p, int i, float2 v) {
+    lds_vec2 x;
+    x.x = v.x;
+    x.y = v.y;
+    p[i] = x;
+}
+
 template <int R1, int R2, bool INV, int T, int E, bool TWL, int W>
-__device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divisor lines, int Ns, unsigned ns_mag,
-                                              int rem, const float2* __restrict__ tw) {
+__device__ __noinline__ void stockham_pass(float2* __restrict__ buf_, int N_, Divisor lines_, int Ns_, unsigned ns_mag_,
+                                              int rem_, const float2* __restrict__ tw_) {
     constexpr int R = R1 * R2;
     constexpr int S = (E + R - 1) / R;
+    lds_c* const buf = (lds_c*)(size_t)uniform((unsigned)(size_t)(lds_c*)buf_);
+    const float2* const tw = uniform(tw_);
+    const int N = uniform(N_);
+    const Divisor lines{uniform(lines_.d), uniform(lines_.m)};
+    const int Ns = uniform(Ns_);
+    const unsigned ns_mag = uniform(ns_mag_);
     const int B = lines.d;
-    const int NR = N / R;
-    const int nbf = NR * B;
-    const int tws = rem / R;  // N / (Ns * R)
+    const int nbf = (N / R) * B;
+    const int tws = uniform(rem_) / R;  // N / (Ns * R)
+    const int ostride = Ns * B;
+    const int jstride = (R - 1) * B;
     float2 v[S][R];
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const int u = threadIdx.x + s * T;
         if (u < nbf) {
 #pragma unroll
-            for (int t = 0; t < R; t++) v[s][t] = buf[u + t * nbf];
+            for (int t = 0; t < R; t++) v[s][t] = lds_get(buf, u + in_sgpr(t * nbf));
         }
     }
     __syncthreads();
@@ -285,21 +332,22 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf, int N, Divi
         const int u = threadIdx.x + s * T;
         if (u < nbf) {
             const int j = lines.div(u);
-            const int k = Ns > 1 ? j - (int)__umulhi((unsigned)j, ns_mag) * Ns : 0;
+            const int k = Ns > 1 ? j - mul24((int)__umulhi((unsigned)j, ns_mag), Ns) : 0;
             if (Ns > 1) {
-                const int step = k * tws;
+                const int step = mul24(k, tws);
+                int widx = 0;
 #pragma unroll
                 for (int t = 1; t < R; t++) {
-                    float2 w = load_twiddle<TWL>(tw, t * step);
+                    widx = in_vgpr(widx + step);
+                    float2 w = load_twiddle<TWL>(tw, widx);
                     if (INV) w.y = -w.y;
                     v[s][t] = cmul(v[s][t], w);
                 }
             }
             Composite<R1, R2, INV>::run(v[s]);
-            const int o = u + (j - k) * (R - 1) * B;
-            const int stride = Ns * B;
+            const int o = u + mul24(j - k, jstride);
 #pragma unroll
-            for (int t = 0; t < R; t++) buf[o + t * stride] = v[s][t];
+            for (int t = 0; t < R; t++) lds_put(buf, o + in_sgpr(t * ostride), v[s][t]);
         }
     }
     __syncthreads();
