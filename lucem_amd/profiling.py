@@ -7,9 +7,10 @@ Two helpers for the ``traffic`` figure of ``bench.py``:
     ``FETCH_SIZE`` / ``WRITE_SIZE`` readings against the known 1 GiB each way give the factor that
     turns each counter into bytes on this GPU.
 
-``python -m lucem_amd.profiling run batched|grad|wiener|fft [REPS]``
-    a short run of one workload for a kernel trace: the uneven-tile slide batch (300 tiles, 20
-    iterations), the differentiable form (64 tiles of 256x256, forward + backward), the Wiener
+``python -m lucem_amd.profiling run plain3d|plain2d|batched|grad|wiener|fft [REPS]``
+    a short run of one workload for a kernel trace: the plain call on 128x512x512 (PSF 31x15x15,
+    50 iterations) or 1024x1024 (PSF 25x25, 30 iterations), the uneven-tile slide batch (300
+    tiles, 20 iterations), the differentiable form (64 tiles of 256x256, forward + backward), the Wiener
     filter (2048x2048, 25x25 Gaussian PSF, balance 0.1) or the complex FFT over both axes of a
     2048x2048 split re/im array; REPS calls (default 3) after one warm-up call.
 
@@ -76,7 +77,20 @@ def run_workload(name: str, reps: int = 3) -> None:
     from . import synth
     from .restore import RichardsonLucy
 
-    if name == "batched":
+    if name in ("plain3d", "plain2d"):
+        from . import _native as N
+        from .restore import richardson_lucy_into, richardson_lucy_nd_into
+
+        prob = synth.plain3d() if name == "plain3d" else synth.plain2d()
+        y = torch.from_numpy(prob.image).cuda()
+        p = torch.from_numpy(prob.psf).cuda()
+        o = torch.empty_like(y)
+        client = N.default_client(0)
+        ws = N.Workspace(client)
+        s = RichardsonLucy(num_iter=prob.num_iter, clip=False)
+        run = richardson_lucy_nd_into if y.dim() == 3 else richardson_lucy_into
+        fn = lambda: run(y, p, o, s, ws, client)  # noqa: E731
+    elif name == "batched":
         from .batched import RLBatch
 
         ts = synth.tiles()

@@ -232,14 +232,18 @@ def grad2d(count: int = 64, side: int = 256, sigma: float = 1.8, k: int = 17, nu
 
 
 def psffit3d(count: int = 16, side: int = 256, depth=(32, 128), sigma=(3.5, 1.4, 1.6), k=(31, 15, 15),
-             spheres_per_mvox: float = 1500 / 33.554432):
+             spheres_per_mvox: float = 1500 / 33.554432, return_truth: bool = False):
     geo = np.random.default_rng(1004)
     noise = np.random.default_rng(2004)
     psf = gaussian_psf(sigma, k)
-    vols = []
+    vols, truths = [], []
     for _ in range(count):
         d = int(geo.integers(depth[0], depth[1] + 1))
         n = max(int(spheres_per_mvox * d * side * side / 1e6), 1)
         t = spheres((d, side, side), n, geo)
         vols.append(poisson(blur(t, psf), noise))
+        if return_truth:
+            truths.append(t.astype(np.float32))
+    if return_truth:
+        return vols, psf.astype(np.float32), truths
     return vols, psf.astype(np.float32)

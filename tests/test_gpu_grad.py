@@ -304,3 +304,23 @@ def test_thinning_splits_counts(gpu):
     assert bool((y1[0] >= 0).all()) and bool((y2[0] >= 0).all()) and bool((y1[0] == y1[0].round()).all())
     frac = float(y1[0].sum() / y.sum())
     assert abs(frac - 0.5) < 2e-3
+
+
+def test_known_object_fit_recovers_sigma(gpu):
+    """The known-object objective (held-out likelihood, RL started from the true object) identifies
+    the PSF: 4 volumes of the PSF-fitting law (31x15x15 Gaussian of sigma (3.5, 1.4, 1.6)), 2
+    unrolled iterations, 100 Adam steps from (2.5, 1.0, 1.0) end within 3% on every axis."""
+    import torch
+
+    from lucem_amd import synth
+    from lucem_amd.psf_fit import fit_gaussian_sigma
+
+    vols, _, objs = synth.psffit3d(count=4, return_truth=True)
+    print(f"known object: volumes {[v.shape for v in vols]}", flush=True)
+    vt = [torch.from_numpy(v).cuda() for v in vols]
+    ot = [torch.from_numpy(o).cuda() for o in objs]
+    res = fit_gaussian_sigma(vt, (31, 15, 15), (2.5, 1.0, 1.0), num_iter=2, steps=100, lr=0.05,
+                             objective="known", objects=ot)
+    err = [abs(a - b) / b for a, b in zip(res.sigma, (3.5, 1.4, 1.6))]
+    print(f"known object: sigma {res.sigma}, relative errors {err}, {res.ms_per_step:.1f} ms per step")
+    assert max(err) < 0.03

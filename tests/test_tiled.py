@@ -209,3 +209,4 @@ def test_layout_errors(gpu):
         plan.execute_layout([img.data_ptr()], [img.data_ptr() + 4 * 45], [TileLayout(40, 0, 40, 0)], [psf])
     assert e.value.kind == "ALIASED_BUFFERS"
     plan.close()
+
