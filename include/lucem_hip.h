@@ -168,6 +168,20 @@ lucem_status lucem_workspace_create(lucem_client* client, lucem_workspace** out)
 lucem_status lucem_workspace_destroy(lucem_workspace* ws);
 uint64_t lucem_workspace_capacity_bytes(const lucem_workspace* ws);
 
+/* Device memory budget (crates/lucem-core/src/memory/budget.rs). A limit comes from
+ * LUCEM_DEVICE_MEMORY, else LUCEM_DEVICE_MEMORY_LIMIT (read once, budget.rs:246-261; values as
+ * parse_bytes takes them, e.g. "7GiB", "6500MB", "7516192768"), or from
+ * lucem_set_device_memory_limit (set_device_memory_limit, budget.rs:175-179; has_limit 0 clears it).
+ * With a limit, a workspace slot's growth or a batched / gradient plan of 64 MiB or more fails
+ * with LUCEM_OUT_OF_DEVICE_MEMORY when it exceeds the limit minus what this library already holds
+ * on that device (check_allocation, budget.rs:232-244; message as CoreError::OutOfDeviceMemory).
+ * The caller's own arrays are not counted. None of these calls touch the device. */
+int32_t lucem_parse_bytes(const char* text, uint64_t* out); /* parse_bytes, budget.rs:263-279; 1 = parsed */
+void lucem_set_device_memory_limit(int32_t has_limit, uint64_t bytes);
+int32_t lucem_device_memory_limit(uint64_t* out);           /* 1 and *out = the limit, or 0 (no limit) */
+uint64_t lucem_device_bytes_in_use(int device);             /* bytes this library holds on `device` */
+lucem_status lucem_check_allocation(int device, const char* what, uint64_t bytes);
+
 /* richardson_lucy_into::<f32>. input, psf and output are device pointers, C-contiguous.
  * Checks, in order: same shape (input, output); no overlap of input and output; input ndim == 2;
  * psf ndim == 2; psf non-empty; num_iter >= 1; empty input => LUCEM_OK with nothing done.

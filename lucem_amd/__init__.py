@@ -34,6 +34,7 @@ from .restore import (  # noqa: F401
     richardson_lucy_nd_into,
 )
 from ._native import Client, LucemError, Workspace, version  # noqa: F401
+from ._native import device_memory_limit, parse_bytes, set_device_memory_limit  # noqa: F401
 from .wiener import (  # noqa: F401
     UnsupervisedWiener,
     WienerDeconvolve,

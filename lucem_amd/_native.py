@@ -57,6 +57,8 @@ EXPORTED = [
     "lucem_butterworth_f32", "lucem_gabor_settings_default", "lucem_gabor_2d_f32",
     "lucem_correlate1d_settings_default", "lucem_correlate_1d_into_f32", "lucem_correlate_1d_into_f64",
     "lucem_richardson_lucy_into_noalloc_f32", "lucem_noalloc_scratch_bytes", "lucem_rl_grad_forward_target",
+    "lucem_parse_bytes", "lucem_set_device_memory_limit", "lucem_device_memory_limit", "lucem_device_bytes_in_use",
+    "lucem_check_allocation",
 ]
 
 
@@ -158,6 +160,15 @@ def lib() -> ctypes.CDLL:
             L.lucem_version.restype = ctypes.c_char_p
             L.lucem_rl_settings_default.argtypes = [ctypes.POINTER(RlSettings)]
             L.lucem_pad_mode_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
+            L.lucem_parse_bytes.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+            L.lucem_parse_bytes.restype = ctypes.c_int32
+            L.lucem_set_device_memory_limit.argtypes = [ctypes.c_int32, ctypes.c_uint64]
+            L.lucem_set_device_memory_limit.restype = None
+            L.lucem_device_memory_limit.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+            L.lucem_device_memory_limit.restype = ctypes.c_int32
+            L.lucem_device_bytes_in_use.argtypes = [ctypes.c_int]
+            L.lucem_device_bytes_in_use.restype = ctypes.c_uint64
+            L.lucem_check_allocation.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64]
             L.lucem_client_create.argtypes = [ctypes.c_int, _vp, ctypes.c_int32, ctypes.POINTER(_vp)]
             L.lucem_client_destroy.argtypes = [_vp]
             L.lucem_client_sync.argtypes = [_vp]
@@ -255,6 +266,36 @@ def dims(shape) -> ctypes.Array:
 
 def version() -> str:
     return lib().lucem_version().decode()
+
+
+def parse_bytes(text: str) -> int | None:
+    """Bytes of a size such as ``"7GiB"``, ``" 6.5 GB "`` or ``"7516192768"`` (``None`` if
+    unparsable), as lucem-core's ``parse_bytes`` (memory/budget.rs:263-279)."""
+    out = ctypes.c_uint64()
+    return int(out.value) if lib().lucem_parse_bytes(text.encode(), ctypes.byref(out)) else None
+
+
+def set_device_memory_limit(limit: int | None) -> None:
+    """Sets (or with ``None`` clears) the device memory budget of this library's workspace slots
+    and plans; overrides ``LUCEM_DEVICE_MEMORY`` / ``LUCEM_DEVICE_MEMORY_LIMIT`` (budget.rs:175-179)."""
+    lib().lucem_set_device_memory_limit(0 if limit is None else 1, 0 if limit is None else int(limit))
+
+
+def device_memory_limit() -> int | None:
+    """The device memory limit in bytes, or ``None`` without one (budget.rs:182-187)."""
+    out = ctypes.c_uint64()
+    return int(out.value) if lib().lucem_device_memory_limit(ctypes.byref(out)) else None
+
+
+def device_bytes_in_use(device: int = 0) -> int:
+    """Bytes this library holds on ``device`` in workspace slots and plans."""
+    return int(lib().lucem_device_bytes_in_use(int(device)))
+
+
+def check_allocation(what: str, nbytes: int, device: int = 0) -> None:
+    """Raises ``LucemError`` (``OUT_OF_DEVICE_MEMORY``) when ``nbytes`` more would not fit the
+    budget (budget.rs:232-244); no-op without a limit or below 64 MiB."""
+    check(lib().lucem_check_allocation(int(device), what.encode(), int(nbytes)))
 
 
 def last_error() -> str:

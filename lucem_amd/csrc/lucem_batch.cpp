@@ -101,7 +101,9 @@ lucem_rl_batch::~lucem_rl_batch() {
         if (staged) (void)hipEventDestroy(staged);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_psf_stage) (void)hipHostFree(h_psf_stage);
-        if (mem) (void)hipFree(mem);
+        if (mem) lucem::dev_free(client ? client->device : 0, mem, mem_bytes);
+    } else if (mem) {
+        lucem::dev_forget(client ? client->device : 0, mem_bytes);
     }
     lucem::client_release(client);
 }
@@ -405,7 +407,11 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
     Carver sizing;
     layout(sizing);
     b->mem_bytes = sizing.off;
-    if ((e = hipMalloc(&b->mem, b->mem_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc(batch plan)");
+    {
+        void* m = nullptr;
+        if (lucem_status st = dev_malloc(client->device, &m, b->mem_bytes, "batch plan")) return st;
+        b->mem = m;
+    }
     Carver real;
     real.base = static_cast<char*>(b->mem);
     layout(real);

@@ -98,7 +98,9 @@ lucem_rl_grad::~lucem_rl_grad() {
         if (capture) (void)hipStreamDestroy(capture);
         if (staged) (void)hipEventDestroy(staged);
         if (h_stage) (void)hipHostFree(h_stage);
-        if (mem) (void)hipFree(mem);
+        if (mem) lucem::dev_free(client ? client->device : 0, mem, mem_bytes);
+    } else if (mem) {
+        lucem::dev_forget(client ? client->device : 0, mem_bytes);
     }
     lucem::client_release(client);
 }
@@ -502,7 +504,11 @@ extern "C" lucem_status lucem_rl_grad_create(lucem_client* client, const uint64_
     Carve sizing;
     layout(sizing);
     P->mem_bytes = sizing.off;
-    if ((e = hipMalloc(&P->mem, P->mem_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc(gradient plan)");
+    {
+        void* m = nullptr;
+        if (lucem_status st = dev_malloc(client->device, &m, P->mem_bytes, "gradient plan")) return st;
+        P->mem = m;
+    }
     Carve real;
     real.base = static_cast<char*>(P->mem);
     layout(real);

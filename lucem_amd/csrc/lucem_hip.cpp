@@ -317,9 +317,12 @@ lucem_status lucem::twiddles_for(lucem_client* c, int L, const float2** out) {
 // workspace
 // ---------------------------------------------------------------------------------------------
 lucem_workspace::~lucem_workspace() {
+    const int dev = client ? client->device : 0;
     if (!lucem::exiting()) {
         if (client) (void)hipStreamSynchronize(client->stream);
-        for (auto& kv : slots) (void)hipFree(kv.second.ptr);
+        for (auto& kv : slots) dev_free(dev, kv.second.ptr, kv.second.bytes);
+    } else {
+        for (auto& kv : slots) dev_forget(dev, kv.second.bytes);
     }
     if (!internal) client_release(client);
 }
@@ -327,20 +330,20 @@ lucem_workspace::~lucem_workspace() {
 lucem_status lucem::ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out) {
     Slot& s = ws->slots[name];
     if (s.bytes < bytes) {
+        const int dev = ws->client->device;
+        // the growth is checked against the budget (workspace.rs:32-35), before the old buffer goes
+        if (lucem_status st = budget_check(dev, name, bytes - s.bytes)) return st;
         if (s.ptr) {
             // the old buffer may still be read by queued work
             (void)hipStreamSynchronize(ws->client->stream);
-            (void)hipFree(s.ptr);
+            dev_free(dev, s.ptr, s.bytes);
             s.ptr = nullptr;
             s.bytes = 0;
         }
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            char buf[160];
-            std::snprintf(buf, sizeof buf, "workspace slot '%s' of %zu bytes", name, bytes);
-            return hip_fail(e, buf);
-        }
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "workspace slot '%s' of %zu bytes", name, bytes);
+        if (lucem_status st = dev_malloc(dev, &p, bytes, buf)) return st;
         s.ptr = p;
         s.bytes = bytes;
     }

@@ -103,6 +103,15 @@ std::string dims_str(const uint64_t* dims, uint32_t ndim);
 lucem_status twiddles_for(lucem_client* c, int L, const float2** out);
 lucem_status ws_slot(lucem_workspace* ws, const char* name, size_t bytes, void** out);
 
+// Device memory budget (lucem_budget.cpp). budget_check: LUCEM_OUT_OF_DEVICE_MEMORY when a limit is
+// set, `bytes` is at least 64 MiB and more than the limit minus what the library holds on `device`.
+// dev_malloc checks, allocates and counts; dev_free frees and uncounts; dev_forget uncounts memory
+// that is left to the process (freed without a HIP call at exit).
+lucem_status budget_check(int device, const char* what, uint64_t bytes);
+lucem_status dev_malloc(int device, void** p, size_t bytes, const char* what);
+void dev_free(int device, void* p, size_t bytes);
+void dev_forget(int device, size_t bytes);
+
 int choose_fft_len(int target);
 // Radix sequence of an FFT of length L. Variant 0: greedy, largest composite radix first;
 // 1: the same radices in reverse order; 2: single radices only (8, 7, 5, 4, 3, 2).
