@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "liblucem_hip.so"
-SOURCES = ["lucem_budget.cpp", "rl_kernels.hip", "rl_grad.hip", "rl_direct.hip", "wiener.hip", "fft_axis.hip", "noise.hip", "correlate.hip",
+SOURCES = ["lucem_budget.cpp", "rl_tiled.hip", "lucem_tiled.cpp", "rl_kernels.hip", "rl_grad.hip", "rl_direct.hip", "wiener.hip", "fft_axis.hip", "noise.hip", "correlate.hip",
            "phantom.hip", "phase.hip", "spectral.hip", "lucem_hip.cpp", "lucem_batch.cpp", "lucem_grad.cpp", "lucem_tune.cpp", "lucem_wiener.cpp",
            "lucem_fft.cpp", "lucem_noise.cpp", "lucem_correlate.cpp", "lucem_phantom.cpp",
            "lucem_phase.cpp", "lucem_spectral.cpp", "lucem_noalloc.hip"]

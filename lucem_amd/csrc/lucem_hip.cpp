@@ -132,6 +132,7 @@ extern "C" lucem_status lucem_client_create(int device, void* stream, int32_t ow
         lds_err = allow_big_lds_kernels();
         if (!lds_err) lds_err = allow_big_lds_axis_kernels();
         if (!lds_err) lds_err = allow_big_lds_grad_kernels();
+        if (!lds_err) lds_err = allow_tile_lds();
     });
     if (lds_err) return hip_fail(lds_err, "hipFuncSetAttribute(dynamic LDS)");
     noalloc_setup(device);
@@ -285,6 +286,10 @@ lucem_status lucem::run_row(lucem_client* c, const RowPass& r, int blocks, const
 
 lucem_status lucem::run_col(lucem_client* c, const ColPass& p, int blocks, const char* name) {
     return profiled(c, name, col_pass_bytes(p, blocks), [&] { return launch_col_pass(p, blocks, c->stream); });
+}
+
+lucem_status lucem::run_tile(lucem_client* c, const TileIter& d, int blocks, double bytes, const char* name) {
+    return profiled(c, name, bytes, [&] { return launch_tile_iter(d, blocks, c->stream); });
 }
 
 // Twiddle table exp(-2 pi i m / L), m < L, computed in double and rounded once to float.
@@ -829,6 +834,10 @@ lucem_status lucem::rl_launch(lucem_client* c, const Geometry& g, const lucem_rl
 
 lucem_status lucem::rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
                            const Geometry& g, const lucem_rl_settings& s) {
+    if (g.ndim == 2) {
+        TileGrid tp;
+        if (use_tiled(g, c->num_cus, &tp)) return rl_run_tiled(c, ws, input, psf, output, g, s, tp);
+    }
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const RlSizes z = rl_sizes(g);

@@ -133,6 +133,7 @@ bool make_geometry(const int64_t n[3], const int64_t k[3], int ndim, Geometry* g
 // Launch with optional profiling (events around the launch, algorithmic bytes recorded).
 lucem_status run_row(lucem_client* c, const RowPass& r, int blocks, const char* name);
 lucem_status run_col(lucem_client* c, const ColPass& p, int blocks, const char* name);
+lucem_status run_tile(lucem_client* c, const TileIter& d, int blocks, double bytes, const char* name);
 double row_pass_bytes(const RowPass& r);
 double col_pass_bytes(const ColPass& p, int blocks);
 
@@ -175,6 +176,33 @@ lucem_status validate_rl(const void* input, const uint64_t* in_dims, uint32_t in
                          size_t elem_size = sizeof(float), bool fft = true);
 lucem_status rl_run(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
                     const Geometry& g, const lucem_rl_settings& s);
+
+// The one-launch iteration of 2-D arrays (lucem_tiled.cpp, rl_tiled.hip): output tiles of T0 x T1
+// on transform domains of L0 x L1, tiles0 x tiles1 of them.
+struct TileGrid {
+    int L0, L1, T0, T1, tiles0, tiles1;
+};
+// The cheapest admissible tile geometry of a 2-D array (false: none; the whole-array passes run).
+bool tile_plan(const Geometry& g, int num_cus, TileGrid* out);
+// Whether the plain 2-D path runs tiled (LUCEM_RL_TILED=0 / 1 force it; otherwise small arrays).
+bool use_tiled(const Geometry& g, int num_cus, TileGrid* plan);
+lucem_status rl_run_tiled(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf, float* output,
+                          const Geometry& g, const lucem_rl_settings& s, const TileGrid& tp);
+// Device buffers of the tiled call (workspace slots, or the allocation-free scratch): two
+// estimates (ping-pong), the transfer functions [L0][pitch], the normalised PSF and its flip, and
+// the PSF row spectra of the transfer-function build. Sizes in bytes of one of each.
+struct TileBufs {
+    float *est_a, *est_b, *h, *rev;
+    float2 *otf1, *otf2, *pa;
+};
+struct TileSizes {
+    size_t est, otf, psf, pa;
+};
+TileSizes tile_sizes(const Geometry& g, const TileGrid& tp);
+// The launches of one tiled call on buffers b with the twiddle tables of L0 and L1. Allocates nothing.
+lucem_status rl_tiled_launch(lucem_client* c, const Geometry& g, const lucem_rl_settings& s, const float* input,
+                             const float* psf, float* output, const TileBufs& b, const float2* tw0, const float2* tw1,
+                             const TileGrid& tp);
 
 // Device buffers of one plain RL call: the workspace slots of rl_run, or carved from the
 // allocation-free scratch (lucem_noalloc.hip).

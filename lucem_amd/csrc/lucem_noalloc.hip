@@ -2,7 +2,8 @@
 // memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
 //
 // lucem_richardson_lucy_into_noalloc_f32 runs exactly the launches of
-// lucem_richardson_lucy_into_f32 (the FFT form, or the direct form past the FFT length limit),
+// lucem_richardson_lucy_into_f32 (the tiled form, the FFT form, or the direct form past the FFT
+// length limit),
 // but every buffer beyond the caller's input, PSF and output is carved from `scratch`, one
 // fixed-size device array per GPU that the library holds until the process ends. The FFT twiddle
 // tables are built into it by a kernel, the launch shapes are the heuristics or the ones this
@@ -74,10 +75,15 @@ void lucem::noalloc_setup(int device) {
     g_done[device] = true;
 }
 
-// Scratch bytes the call needs for geometry g: the FFT form's buffers and twiddle tables, or the
-// direct form's five images and two PSF copies.
-static size_t scratch_need(const Geometry& g, bool direct) {
+// Scratch bytes the call needs for geometry g: the tiled form's buffers and twiddle tables, the
+// FFT form's, or the direct form's five images and two PSF copies.
+static size_t scratch_need(const Geometry& g, bool direct, const TileGrid* tiled) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    if (tiled) {
+        const TileSizes z = tile_sizes(g, *tiled);
+        return 2 * al(z.est) + 2 * al(z.otf) + 2 * al(z.psf) + al(z.pa) + al(sizeof(float2) * (size_t)tiled->L0) +
+               al(sizeof(float2) * (size_t)tiled->L1);
+    }
     if (direct) {
         const size_t n = sizeof(float) * (size_t)g.n[1] * g.n[2], nk = sizeof(float) * (size_t)g.k[1] * g.k[2];
         return 5 * al(n) + 2 * al(nk);
@@ -109,7 +115,9 @@ extern "C" lucem_status lucem_richardson_lucy_into_noalloc_f32(lucem_client* cli
     if (st != LUCEM_OK || empty) return st;
     if (!client) return fail(LUCEM_INVALID_HANDLE, "client is NULL");
     if (!input || !psf || !output) return fail(LUCEM_INVALID_HANDLE, "input, psf or output pointer is NULL");
-    const size_t need = scratch_need(g, direct);
+    TileGrid grid{};
+    const bool tiled = !direct && g.ndim == 2 && use_tiled(g, client->num_cus, &grid);
+    const size_t need = scratch_need(g, direct, tiled ? &grid : nullptr);
     if (need > kScratchBytes)
         return fail(LUCEM_UNSUPPORTED_SHAPE, "input " + dims_str(in_dims, in_ndim) + " with psf " +
                                                  dims_str(psf_dims, psf_ndim) + " needs " + std::to_string(need) +
@@ -135,6 +143,23 @@ extern "C" lucem_status lucem_richardson_lucy_into_noalloc_f32(lucem_client* cli
         for (int i = 0; i < 5; i++) bufs[i] = cv.take<float>(n);
         for (int i = 5; i < 7; i++) bufs[i] = cv.take<float>(nk);
         return rl_direct_launch_f32(client, input, psf, output, g, *settings, bufs);
+    }
+    if (tiled) {
+        const TileSizes z = tile_sizes(g, grid);
+        TileBufs b{};
+        b.est_a = cv.take<float>(z.est);
+        b.est_b = cv.take<float>(z.est);
+        b.otf1 = cv.take<float2>(z.otf);
+        b.otf2 = cv.take<float2>(z.otf);
+        b.h = cv.take<float>(z.psf);
+        b.rev = cv.take<float>(z.psf);
+        b.pa = cv.take<float2>(z.pa);
+        float2* tw0 = cv.take<float2>(sizeof(float2) * (size_t)grid.L0);
+        float2* tw1 = cv.take<float2>(sizeof(float2) * (size_t)grid.L1);
+        hipLaunchKernelGGL(twiddle_table_kernel, dim3((grid.L0 + 255) / 256), dim3(256), 0, client->stream, tw0, grid.L0);
+        hipLaunchKernelGGL(twiddle_table_kernel, dim3((grid.L1 + 255) / 256), dim3(256), 0, client->stream, tw1, grid.L1);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "twiddle table");
+        return rl_tiled_launch(client, g, *settings, input, psf, output, b, tw0, tw1, grid);
     }
     const RlSizes z = rl_sizes(g);
     RlBuffers b{};

@@ -1,0 +1,245 @@
+// Host side of the one-launch RL iteration for 2-D arrays (rl_tiled.hip): tile geometry, the
+// check that every tile's boundary folds stay inside the tile, and the calls.
+//
+// Same operator as rl_launch (restore/deconvolve.rs:73-147): PSF normalisation and flip, the two
+// transfer functions, then num_iter iterations from the constant 0.5, the last into `output` with
+// the optional clip. Only the transform domain differs: each output tile is correlated on its own
+// L0 x L1 domain (overlap-save) instead of the whole array's n + k - 1, and one launch carries
+// both correlations, the ratio and the update. Values agree with the whole-array transforms to
+// rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lucem_hip.h"
+#include "fft_lds.hpp"
+#include "lucem_internal_host.hpp"
+#include "rl_internal.hpp"
+
+using namespace lucem;
+
+namespace {
+// Even 7-smooth lengths the tile kernel takes (at most kTileMaxLen).
+const int kTileLens[] = {16, 18, 20, 24, 28, 30, 32, 36, 40, 42, 48, 50, 54, 56, 60,
+                         64, 70, 72, 80, 84, 90, 96, 98, 100, 108, 112, 120, 126, 128};
+
+// Every pad target of every tile's rel window lies in that window: rel at a reflected position is
+// then a value the tile computed itself. Interior windows map to themselves; only folds can fail
+// (Wrap on a multi-tile axis, or an axis shorter than a fold).
+bool axis_admits(int n, int k, int T, int mode) {
+    const int left = k / 2, m = T + k - 1;
+    for (int t0 = 0; t0 < n; t0 += T) {
+        const int q0 = t0 - left;
+        if (q0 >= 0 && q0 + m <= n) continue;
+        for (int i = 0; i < m; i++) {
+            const int s = pad_index(q0 + i, n, mode);
+            if (s >= 0 && (s < q0 || s >= q0 + m)) return false;
+        }
+    }
+    return true;
+}
+
+struct AxisChoice {
+    int L, T;
+};
+
+// Candidate (L, T) of one axis: the longest tile each length allows, or the whole axis in the
+// shortest length that holds it.
+std::vector<AxisChoice> axis_candidates(int n, int k, int mode) {
+    std::vector<AxisChoice> out;
+    const int halo = 2 * (k - 1);
+    bool whole = false;
+    for (int L : kTileLens) {
+        int T = L - halo;
+        if (T < 1) continue;
+        if (T >= n) {
+            if (whole) continue;
+            whole = true;
+            T = n;
+        }
+        if (!axis_admits(n, k, T, mode)) continue;
+        out.push_back({L, T});
+    }
+    return out;
+}
+
+// Both transforms of an L0 x L1 tile fit the kernel's threads (lines x length <= threads x values)
+// and its row pairs fit the sweeps (at most kTileMaxLen / 2 pairs, columns no more than threads).
+bool tile_fits(int L0, int L1) {
+    const int64_t cap = (int64_t)kTileThreads * kTileElems;
+    return L0 % 2 == 0 && L0 <= kTileMaxLen && L1 <= kTileMaxLen && (int64_t)L1 * tile_lines(L0 / 2) <= cap &&
+           (int64_t)L0 * tile_lines(L1 / 2 + 1) <= cap;
+}
+
+bool tile_len_ok(int L) {
+    for (int x : kTileLens)
+        if (x == L) return true;
+    return false;
+}
+
+bool tiled_mode_env(int* mode) {
+    const char* v = std::getenv("LUCEM_RL_TILED");
+    if (!v) return false;
+    *mode = std::atoi(v);
+    return true;
+}
+}  // namespace
+
+bool lucem::tile_plan(const Geometry& g, int num_cus, TileGrid* out) {
+    if (g.ndim != 2 || g.origin[1] != 0 || g.origin[2] != 0) return false;
+    const int n0 = g.n[1], n1 = g.n[2], k0 = g.k[1], k1 = g.k[2];
+    const std::vector<AxisChoice> a0 = axis_candidates(n0, k0, g.mode), a1 = axis_candidates(n1, k1, g.mode);
+    const size_t lds_cu = 160 * 1024;
+    double best = 0.0;
+    bool found = false;
+    for (const AxisChoice& y : a0) {
+        if (y.L % 2) continue;
+        for (const AxisChoice& x : a1) {
+            if (!tile_fits(y.L, x.L)) continue;
+            const size_t lds = tile_iter_lds_bytes(y.L, x.L);
+            const int per_cu = lds * 2 <= lds_cu ? 2 : 1;
+            const int64_t tiles = (int64_t)((n0 + y.T - 1) / y.T) * ((n1 + x.T - 1) / x.T);
+            const int64_t rounds = (tiles + (int64_t)num_cus * per_cu - 1) / ((int64_t)num_cus * per_cu);
+            // a workgroup's time grows with its transform work; two sharing a CU take about 1.6x one
+            const double work = (double)y.L * x.L * (make_plan(y.L).npass + make_plan(x.L).npass + 2);
+            const double cost = (double)rounds * work * (per_cu == 2 ? 1.6 : 1.0);
+            if (!found || cost < best) {
+                best = cost;
+                found = true;
+                *out = TileGrid{y.L, x.L, y.T, x.T, (int)((n0 + y.T - 1) / y.T), (int)((n1 + x.T - 1) / x.T)};
+            }
+        }
+    }
+    return found;
+}
+
+// Whether the plain 2-D path takes the tiled iteration: LUCEM_RL_TILED=0 never, =1 whenever a tile
+// plan exists; otherwise when the plan's tiles fill the device in at most two rounds (arrays small
+// enough that the whole-array passes are bound by latency, not bandwidth).
+bool lucem::use_tiled(const Geometry& g, int num_cus, TileGrid* plan) {
+    if (!tile_plan(g, num_cus, plan)) return false;
+    // LUCEM_TILE_GRID="L0,L1,T0,T1" replaces the chosen grid when it is admissible (measurement)
+    if (const char* v = std::getenv("LUCEM_TILE_GRID")) {
+        int L0 = 0, L1 = 0, T0 = 0, T1 = 0;
+        if (std::sscanf(v, "%d,%d,%d,%d", &L0, &L1, &T0, &T1) == 4 && tile_len_ok(L0) && tile_len_ok(L1) && T0 >= 1 && T1 >= 1 && T0 + 2 * (g.k[1] - 1) <= L0 && T1 + 2 * (g.k[2] - 1) <= L1 &&
+            tile_fits(L0, L1) &&
+            axis_admits(g.n[1], g.k[1], T0, g.mode) && axis_admits(g.n[2], g.k[2], T1, g.mode))
+            *plan = TileGrid{L0, L1, T0, T1, (g.n[1] + T0 - 1) / T0, (g.n[2] + T1 - 1) / T1};
+    }
+    int mode = 0;
+    if (tiled_mode_env(&mode)) return mode != 0;
+    const int64_t tiles = (int64_t)plan->tiles0 * plan->tiles1;
+    const int per_cu = tile_iter_lds_bytes(plan->L0, plan->L1) * 2 <= 160 * 1024 ? 2 : 1;
+    return tiles <= 2 * (int64_t)num_cus * per_cu;
+}
+
+// The canonical algorithmic bytes of one iteration on the whole array's minimal domain
+// (24 n + 80 N_c), the figure the whole-array passes are measured against.
+static double canonical_iteration_bytes(const Geometry& g) {
+    const double Ly = g.n[1] + g.k[1] - 1, Lx = g.n[2] + g.k[2] - 1;
+    const double Nc = Ly * (std::floor(Lx / 2) + 1);
+    return 24.0 * g.n[1] * g.n[2] + 80.0 * Nc;
+}
+
+// The tile domain's geometry, on which the transfer functions are built: (1, L0, L1).
+static Geometry tile_geometry(const Geometry& g, const TileGrid& tp) {
+    Geometry gt = g;
+    gt.L[0] = 1;
+    gt.L[1] = tp.L0;
+    gt.L[2] = tp.L1;
+    gt.Lc = tp.L1 / 2 + 1;
+    gt.pitch = (gt.Lc + 15) & ~15;
+    return gt;
+}
+
+TileSizes lucem::tile_sizes(const Geometry& g, const TileGrid& tp) {
+    const Geometry gt = tile_geometry(g, tp);
+    TileSizes z{};
+    z.est = sizeof(float) * (size_t)g.n[1] * g.n[2];
+    z.otf = sizeof(float2) * (size_t)tp.L0 * gt.pitch;
+    z.psf = sizeof(float) * (size_t)g.k[1] * g.k[2];
+    z.pa = sizeof(float2) * (size_t)g.k[1] * gt.pitch;
+    return z;
+}
+
+lucem_status lucem::rl_tiled_launch(lucem_client* c, const Geometry& g, const lucem_rl_settings& s, const float* input,
+                                    const float* psf, float* output, const TileBufs& b, const float2* tw0,
+                                    const float2* tw1, const TileGrid& tp) {
+    const Geometry gt = tile_geometry(g, tp);
+    const float2* tws[3] = {nullptr, tw0, tw1};
+    lucem_status st;
+    // PSF normalisation and flip (deconvolve.rs:73-90), the transfer functions of rev (:120) and
+    // h (:133) on the tile domain
+    const int64_t nk = (int64_t)g.k[1] * g.k[2];
+    PsfPrep pp{nk, psf, b.h, b.rev};
+    const hipError_t e = (hipError_t)launch_psf_prep(pp, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "psf prep");
+    if ((st = run_otf(c, gt, b.rev, b.pa, nullptr, b.otf1, tws))) return st;
+    if ((st = run_otf(c, gt, b.h, b.pa, nullptr, b.otf2, tws))) return st;
+
+    TileIter d{};
+    d.n0 = g.n[1];
+    d.n1 = g.n[2];
+    d.T0 = tp.T0;
+    d.T1 = tp.T1;
+    d.left0 = g.k[1] / 2;
+    d.left1 = g.k[2] / 2;
+    d.k0 = g.k[1];
+    d.k1 = g.k[2];
+    d.L0 = tp.L0;
+    d.L1 = tp.L1;
+    d.tiles1 = tp.tiles1;
+    d.mode = g.mode;
+    d.pitch = gt.pitch;
+    d.clip = s.clip ? 1 : 0;
+    d.eps = rl_eps(s);
+    d.init = 0.5f;  // deconvolve.rs:93
+    d.f0 = make_plan(tp.L0);
+    d.f1 = make_plan(tp.L1);
+    d.obs = input;
+    d.out = output;
+    d.otf1 = b.otf1;
+    d.otf2 = b.otf2;
+    d.tw0 = tw0;
+    d.tw1 = tw1;
+    const int blocks = tp.tiles0 * tp.tiles1;
+    const double bytes = canonical_iteration_bytes(g);
+    // ping-pong: a tile reads its neighbours' estimates while it writes its own
+    float* cur = b.est_a;
+    float* nxt = b.est_b;
+    for (uint64_t it = 0; it < s.num_iter; it++) {
+        d.first = it == 0;
+        d.last = it + 1 == s.num_iter;
+        d.est_in = cur;
+        d.est_out = nxt;
+        if ((st = run_tile(c, d, blocks, bytes, "tile_iter"))) return st;
+        std::swap(cur, nxt);
+    }
+    return LUCEM_OK;
+}
+
+lucem_status lucem::rl_run_tiled(lucem_client* c, lucem_workspace* ws, const float* input, const float* psf,
+                                 float* output, const Geometry& g, const lucem_rl_settings& s, const TileGrid& tp) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const TileSizes z = tile_sizes(g, tp);
+    TileBufs b{};
+    lucem_status st;
+    if ((st = ws_slot(ws, "rl_est", z.est, (void**)&b.est_a))) return st;
+    if ((st = ws_slot(ws, "rl_est_b", z.est, (void**)&b.est_b))) return st;
+    if ((st = ws_slot(ws, "rl_otf_conv", z.otf, (void**)&b.otf1))) return st;
+    if ((st = ws_slot(ws, "rl_otf_mirror", z.otf, (void**)&b.otf2))) return st;
+    if ((st = ws_slot(ws, "rl_psf_h", z.psf, (void**)&b.h))) return st;
+    if ((st = ws_slot(ws, "rl_psf_rev", z.psf, (void**)&b.rev))) return st;
+    if ((st = ws_slot(ws, "rl_psf_spec_a", z.pa, (void**)&b.pa))) return st;
+    const float2 *tw0 = nullptr, *tw1 = nullptr;
+    if ((st = twiddles_for(c, tp.L0, &tw0))) return st;
+    if ((st = twiddles_for(c, tp.L1, &tw1))) return st;
+    return rl_tiled_launch(c, g, s, input, psf, output, b, tw0, tw1, tp);
+}

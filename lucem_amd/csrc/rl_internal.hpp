@@ -289,6 +289,48 @@ int launch_add_noise(const float* in, float* out, int kind, uint32_t seed, float
                      void* stream);
 // Allows the 1024-thread kernels their 128 KiB of dynamic LDS (once per process).
 int allow_big_lds_kernels();
+
+// One whole RL iteration of a 2-D array in one launch (rl_tiled.hip). Workgroup (ty, tx) owns the
+// output tile [ty*T0, ty*T0 + T0) x [tx*T1, tx*T1 + T1). Both correlations run as overlap-save FFT
+// correlations on an L0 x L1 domain held in LDS, L_a >= T_a + 2 (k_a - 1) per axis:
+//   Q1[i] = P(x)(t0 - 2 left + i), i < L        the estimate with the pad rule, 2 halos deep
+//   u(t0 - left + p) = corr(Q1, rev)[p],  p < T + k - 1
+//   Q2[i] = P(rel)(t0 - left + i) for i < T + k - 1, 0 beyond; rel = y / safe(u)
+//   c(t0 + p)        = corr(Q2, h)[p],    p < T
+//   x'(t0 + p)       = x(t0 + p) * c(t0 + p)
+// so the iteration reads x with a halo of 2 (k - 1), y with one of k - 1, and writes x' once. The
+// host admits a geometry only when every pad target of rel lies inside the tile's own u window.
+struct TileIter {
+    int n0, n1;          // image rows, columns
+    int T0, T1;          // output tile extent per axis
+    int left0, left1;    // window offsets k / 2
+    int k0, k1;          // PSF extent
+    int L0, L1;          // FFT lengths (L0 even: rows are transformed in pairs)
+    int tiles1;          // tiles along x (blockIdx.x = ty * tiles1 + tx)
+    int mode;            // pad mode of both correlations
+    int pitch;           // row pitch (complex) of the transfer functions, [L0][pitch]
+    int first, last, clip;
+    float eps, init;
+    FftPlan f0, f1;      // column (L0) and row (L1) transforms
+    const float* est_in;  // estimate (unused when first: the constant init)
+    float* est_out;       // next estimate (unless last)
+    const float* obs;     // observed image
+    float* out;           // output (last)
+    const float2* otf1;   // transfer function of rev (est -> u)
+    const float2* otf2;   // transfer function of h (rel -> c)
+    const float2* tw0;    // twiddle tables of L0 and L1
+    const float2* tw1;
+};
+int launch_tile_iter(const TileIter& d, int blocks, void* stream);
+// Allows the tiled kernel its dynamic LDS (once per process, at client creation).
+int allow_tile_lds();
+size_t tile_iter_lds_bytes(int L0, int L1);
+// Threads of the tiled kernel and the most complex values one of its transforms holds (L * lines).
+constexpr int kTileThreads = 1024;
+constexpr int kTileElems = 9;
+constexpr int kTileMaxLen = 128;
+// Interleaved lines of the tile's LDS layouts: n data lines, plus one when n is even.
+__host__ __device__ constexpr int tile_lines(int n) { return n | 1; }
 size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
 

@@ -34,3 +34,10 @@ def gpu():
     from lucem_amd import _native as N
 
     return N.default_client(0)
+
+
+@pytest.fixture
+def whole_array_rl(monkeypatch):
+    """The plain 2-D entry point on its whole-array passes, the kernels the batched form shares (small
+    2-D arrays otherwise take the one-launch tiled iteration, equal to rounding but not bit for bit)."""
+    monkeypatch.setenv("LUCEM_RL_TILED", "0")

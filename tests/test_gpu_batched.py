@@ -21,7 +21,7 @@ def _plain(y, psf, settings):
     return out
 
 
-def test_batched_2d_uneven_tiles_match_plain_and_oracle(gpu, oracle):
+def test_batched_2d_uneven_tiles_match_plain_and_oracle(gpu, oracle, whole_array_rl):
     import torch
 
     import lucem_amd as LA

@@ -1,0 +1,135 @@
+"""The one-launch tiled iteration of 2-D arrays (rl_tiled.hip) against the FP32 oracle of the
+reference (crates/lucem-ops/src/restore/deconvolve.rs:45-149), through the plain 2-D entry point.
+
+Each output tile is correlated on its own small domain (overlap-save), both correlations, the
+ratio and the update in one launch; the host admits a tile geometry only where every boundary fold
+of rel stays inside the tile. The cases force the tiled form (LUCEM_RL_TILED=1) and check from the
+profile that it ran, or that it declined (Wrap on an axis of several tiles) and the whole-array
+passes ran instead; both against the oracle at the same 1e-4 relative L2 bar."""
+
+from __future__ import annotations
+
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L2 = 1e-4
+
+
+@pytest.fixture
+def tiled_mode():
+    old = os.environ.get("LUCEM_RL_TILED")
+
+    def set_mode(v):
+        if v is None:
+            os.environ.pop("LUCEM_RL_TILED", None)
+        else:
+            os.environ["LUCEM_RL_TILED"] = v
+
+    yield set_mode
+    set_mode(old)
+
+
+def _run(client, y, psf, num_iter, mode=0, clip=False, eps=None):
+    """Output and the kernel names the call launched."""
+    import torch
+
+    import lucem_amd as LA
+
+    yt = torch.from_numpy(np.ascontiguousarray(y, np.float32)).cuda()
+    pt = torch.from_numpy(np.ascontiguousarray(psf, np.float32)).cuda()
+    out = torch.empty_like(yt)
+    s = LA.RichardsonLucy(num_iter=num_iter, clip=clip, mode=mode, filter_epsilon=eps)
+    client.profile(True)
+    LA.richardson_lucy_into(yt, pt, out, s, None, client)
+    stats = client.profile_read()
+    client.profile(False)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), stats
+
+
+CASES = [
+    ((200, 300), (7, 9)), ((256, 256), (25, 25)), ((130, 70), (4, 6)), ((97, 129), (17, 17)), ((600, 40), (3, 5)),
+    ((5, 6), (11, 13)), ((1, 300), (1, 9)), ((300, 1), (9, 1)), ((64, 64), (1, 1)), ((180, 190), (31, 30)),
+    ((2, 2), (3, 3)), ((33, 500), (2, 3)),
+]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape,kshape", CASES)
+def test_tiled_matches_oracle(gpu, oracle, tiled_mode, mode, shape, kshape):
+    tiled_mode("1")
+    rng = np.random.default_rng(zlib.crc32(repr((shape, kshape, mode)).encode()))
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    got, stats = _run(gpu, y, psf, 8, mode)
+    assert "tile_iter" in stats and stats["tile_iter"]["launches"] == 8, stats.keys()
+    st, ref = oracle.richardson_lucy(y, psf, 8, clip=False, mode=mode)
+    assert st == 0
+    e = oracle.rel_l2(got, ref)
+    print(f"{shape} {kshape} mode {mode}: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
+
+
+def test_wrap_declines_on_several_tiles(gpu, oracle, tiled_mode):
+    """Wrap folds a left-edge window onto the right edge, outside the tile: the whole-array passes
+    run instead."""
+    tiled_mode("1")
+    rng = np.random.default_rng(5)
+    y = rng.uniform(50, 150, (300, 280)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (9, 9)).astype(np.float32)
+    got, stats = _run(gpu, y, psf, 5, mode=4)
+    assert "tile_iter" not in stats and "line_fused_y" in stats
+    _, ref = oracle.richardson_lucy(y, psf, 5, clip=False, mode=4)
+    assert oracle.rel_l2(got, ref) < RTOL_L2
+
+
+def test_wrap_on_one_tile(gpu, oracle, tiled_mode):
+    """An array that one tile covers keeps every Wrap target inside the tile."""
+    tiled_mode("1")
+    rng = np.random.default_rng(6)
+    y = rng.uniform(50, 150, (40, 50)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (5, 5)).astype(np.float32)
+    got, stats = _run(gpu, y, psf, 6, mode=4)
+    assert stats["tile_iter"]["launches"] == 6
+    _, ref = oracle.richardson_lucy(y, psf, 6, clip=False, mode=4)
+    assert oracle.rel_l2(got, ref) < RTOL_L2
+
+
+def test_tiled_clip_eps_and_one_iteration(gpu, oracle, tiled_mode):
+    tiled_mode("1")
+    rng = np.random.default_rng(7)
+    y = rng.uniform(0.2, 0.9, (90, 131)).astype(np.float32)
+    y[40:50, 60:70] = 3.0
+    psf = rng.uniform(0.1, 1.0, (5, 5)).astype(np.float32)
+    got, stats = _run(gpu, y, psf, 6, clip=True)
+    _, ref = oracle.richardson_lucy(y, psf, 6, clip=True)
+    assert "tile_iter" in stats and got.max() <= 1.0 and oracle.rel_l2(got, ref) < RTOL_L2
+    got, _ = _run(gpu, y, psf, 1)
+    _, ref = oracle.richardson_lucy(y, psf, 1, clip=False)
+    assert oracle.rel_l2(got, ref) < RTOL_L2
+    # data near eps: the rule fires on some pixels and not on others
+    z = rng.uniform(0.5e-3, 2.0e-3, (70, 74)).astype(np.float32)
+    got, stats = _run(gpu, z, psf, 3, eps=1.0e-3)
+    _, ref = oracle.richardson_lucy(z, psf, 3, clip=False, filter_epsilon=1.0e-3)
+    assert "tile_iter" in stats and oracle.rel_l2(got, ref) < RTOL_L2
+
+
+def test_tiled_against_whole_array_at_full_size(gpu, tiled_mode):
+    """The 1024 x 1024 workload (25 x 25 Gaussian PSF, 30 iterations): the tiled and the
+    whole-array transforms agree to rounding."""
+    from lucem_amd import synth
+
+    prob = synth.plain2d()
+    tiled_mode("1")
+    a, sa = _run(gpu, prob.image, prob.psf, prob.num_iter)
+    tiled_mode("0")
+    b, sb = _run(gpu, prob.image, prob.psf, prob.num_iter)
+    assert "tile_iter" in sa and "tile_iter" not in sb
+    e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print(f"tiled vs whole-array: rel-L2 {e:.2e}")
+    assert np.isfinite(a).all() and e < 2e-6

@@ -137,7 +137,7 @@ def test_tiled_short_halo_runs(gpu):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nd", [2, 3])
-def test_layout_views_equal_plain_tiles(gpu, nd):
+def test_layout_views_equal_plain_tiles(gpu, nd, whole_array_rl):
     """Tiles given as strided views into one image, each storing a window into one result, give
     the plain per-tile results inside their windows, and leave the rest of the result alone."""
     import numpy as np
