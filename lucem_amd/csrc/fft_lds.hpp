@@ -301,12 +301,15 @@ __device__ __forceinline__ void lds_put(lds_c* p, int i, float2 v) {
     p[i] = x;
 }
 
-template <int R1, int R2, bool INV, int T, int E, bool TWL, int W>
-__device__ __noinline__ void stockham_pass(float2* __restrict__ buf_, int N_, Divisor lines_, int Ns_, unsigned ns_mag_,
-                                              int rem_, const float2* __restrict__ tw_) {
+// OOP: out of place, reading buf_ and writing out_ (another LDS buffer), with no barrier between
+// the loads and the stores: a wave stores as soon as its own butterflies are done.
+template <int R1, int R2, bool INV, int T, int E, bool TWL, int W, bool OOP = false>
+__device__ __noinline__ void stockham_pass(float2* __restrict__ buf_, float2* __restrict__ out_, int N_, Divisor lines_,
+                                           int Ns_, unsigned ns_mag_, int rem_, const float2* __restrict__ tw_) {
     constexpr int R = R1 * R2;
     constexpr int S = (E + R - 1) / R;
     lds_c* const buf = (lds_c*)(size_t)uniform((unsigned)(size_t)(lds_c*)buf_);
+    lds_c* const out = OOP ? (lds_c*)(size_t)uniform((unsigned)(size_t)(lds_c*)out_) : buf;
     const float2* const tw = uniform(tw_);
     const int N = uniform(N_);
     const Divisor lines{uniform(lines_.d), uniform(lines_.m)};
@@ -326,7 +329,7 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf_, int N_, Di
             for (int t = 0; t < R; t++) v[s][t] = lds_get(buf, u + in_sgpr(t * nbf));
         }
     }
-    __syncthreads();
+    if (!OOP) __syncthreads();
 #pragma unroll
     for (int s = 0; s < S; s++) {
         const int u = threadIdx.x + s * T;
@@ -347,7 +350,7 @@ __device__ __noinline__ void stockham_pass(float2* __restrict__ buf_, int N_, Di
             Composite<R1, R2, INV>::run(v[s]);
             const int o = u + mul24(j - k, jstride);
 #pragma unroll
-            for (int t = 0; t < R; t++) lds_put(buf, o + in_sgpr(t * ostride), v[s][t]);
+            for (int t = 0; t < R; t++) lds_put(out, o + in_sgpr(t * ostride), v[s][t]);
         }
     }
     __syncthreads();
@@ -378,11 +381,16 @@ __device__ __forceinline__ const float2* stage_twiddles(float2* Z, int data_elem
 
 // Full transform of the interleaved lines. The caller has written buf (and the staged twiddles,
 // twl) and synchronised.
-template <bool INV, int T, int E, bool TWL, int W>
-__device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ tw) {
+// OOP: passes alternate between buf and alt (pass q reads buf when q is even); the result is in
+// buf after an even number of passes, in alt after an odd one.
+template <bool INV, int T, int E, bool TWL, int W, bool OOP = false>
+__device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPlan& p, const float2* __restrict__ tw,
+                            float2* __restrict__ alt = nullptr) {
     const int N = p.L;
     int Ns = 1, rem = N;
     for (int q = 0; q < p.npass; q++) {
+        float2* const src = OOP && (q & 1) ? alt : buf;
+        float2* const dst = OOP ? ((q & 1) ? buf : alt) : buf;
         const unsigned long long w = q < 8 ? p.codes_lo : p.codes_hi;
         const int code = (int)((w >> (8 * (q & 7))) & 0xffull);
         // one division per pass (uniform), none per butterfly
@@ -390,7 +398,7 @@ __device__ void lds_fft_run(float2* __restrict__ buf, Divisor lines, const FftPl
         switch (code) {
 #define LUCEM_PASS(r1, r2)                                                             \
     case radix_code(r1, r2):                                                           \
-        stockham_pass<r1, r2, INV, T, E, TWL, W>(buf, N, lines, Ns, mag, rem, tw);   \
+        stockham_pass<r1, r2, INV, T, E, TWL, W, OOP>(src, dst, N, lines, Ns, mag, rem, tw); \
         Ns *= r1 * r2;                                                                 \
         rem /= r1 * r2;                                                                \
         break;
@@ -420,6 +428,15 @@ __device__ void lds_fft(float2* __restrict__ buf, Divisor lines, const FftPlan& 
         lds_fft_run<INV, T, E, true, W>(buf, lines, p, twl);
     else
         lds_fft_run<INV, T, E, false, W>(buf, lines, p, twbase + p.tw);
+}
+
+// Out-of-place transform of the lines in `buf` with `alt` (same size) as the other buffer, the
+// twiddles staged in LDS; returns the buffer that holds the result.
+template <bool INV, int T, int E, int W = 4>
+__device__ float2* lds_fft_oop(float2* __restrict__ buf, float2* __restrict__ alt, Divisor lines, const FftPlan& p,
+                               const float2* twl) {
+    lds_fft_run<INV, T, E, true, W, true>(buf, lines, p, twl, alt);
+    return (p.npass & 1) ? alt : buf;
 }
 
 // Reference pad_index (core/kernels/pad.rs:11-34); -1 for Constant outside [0, len). Positions

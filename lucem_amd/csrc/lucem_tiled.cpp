@@ -198,6 +198,10 @@ lucem_status lucem::rl_tiled_launch(lucem_client* c, const Geometry& g, const lu
     d.mode = g.mode;
     d.pitch = gt.pitch;
     d.clip = s.clip ? 1 : 0;
+    {
+        const char* v = std::getenv("LUCEM_TILE_GENERIC");  // the run-time-length build, for comparison
+        d.generic = v && std::atoi(v) != 0;
+    }
     d.eps = rl_eps(s);
     d.init = 0.5f;  // deconvolve.rs:93
     d.f0 = make_plan(tp.L0);

@@ -310,6 +310,7 @@ struct TileIter {
     int mode;            // pad mode of both correlations
     int pitch;           // row pitch (complex) of the transfer functions, [L0][pitch]
     int first, last, clip;
+    int generic;         // run the kernel compiled for run-time lengths even where a constant one exists
     float eps, init;
     FftPlan f0, f1;      // column (L0) and row (L1) transforms
     const float* est_in;  // estimate (unused when first: the constant init)
@@ -322,6 +323,8 @@ struct TileIter {
     const float2* tw1;
 };
 int launch_tile_iter(const TileIter& d, int blocks, void* stream);
+// Whether the tile kernel has a build with these lengths as compile-time constants.
+bool tile_len_static(int L0, int L1);
 // Allows the tiled kernel its dynamic LDS (once per process, at client creation).
 int allow_tile_lds();
 size_t tile_iter_lds_bytes(int L0, int L1);
@@ -331,6 +334,10 @@ constexpr int kTileElems = 9;
 constexpr int kTileMaxLen = 128;
 // Interleaved lines of the tile's LDS layouts: n data lines, plus one when n is even.
 __host__ __device__ constexpr int tile_lines(int n) { return n | 1; }
+// Complex values of one of the tile's two LDS buffers: either layout of an L0 x L1 tile.
+__host__ __device__ constexpr int tile_buffer_elems(int L0, int L1) {
+    return L1 * tile_lines(L0 / 2) > L0 * tile_lines(L1 / 2 + 1) ? L1 * tile_lines(L0 / 2) : L0 * tile_lines(L1 / 2 + 1);
+}
 size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
 

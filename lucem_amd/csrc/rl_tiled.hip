@@ -12,15 +12,21 @@
 // the values kept. The whole chain stays in LDS: the iteration reads x with its halo, y on the rel
 // window and the two transfer functions, and writes x' on the tile.
 //
-// LDS: Zr, the tile's rows transformed in pairs (L0 / 2 interleaved lines of length L1, element
-// c of pair b at Zr[c * B + b]); Sc, its half spectrum as column lines (L1 / 2 + 1 of length L0,
-// element r of column k at Sc[r * C + k]); the twiddle tables of L1 and L0. Line counts B and C
-// are made odd with a padding line, so that a sweep along a line's elements (the gathers, the
-// split and merge between the two layouts) spreads over the LDS banks instead of striding a
-// multiple of 16 words.
+// LDS: two buffers that the transform passes alternate between (read one, write the other, one
+// barrier per pass), each holding the tile either as row-pair lines (L0 / 2 interleaved lines of
+// length L1, element c of pair b at [c * B + b]) or as its half spectrum in column lines
+// (L1 / 2 + 1 lines of length L0, element r of column k at [r * C + k]); then the twiddle tables
+// of L1 and L0. Line counts B and C are made odd with a padding line, so that a sweep along a
+// line's elements (the gathers, the split and merge between the two layouts) spreads over the LDS
+// banks instead of striding a multiple of 16 words.
+//
+// The kernel is compiled twice over: with the tile's lengths known at compile time for the common
+// domains (constant strides, immediate LDS offsets, a fixed radix sequence: fft_static.hpp), and
+// with run-time lengths for every other admissible domain.
 #include <hip/hip_runtime.h>
 
 #include "fft_lds.hpp"
+#include "fft_static.hpp"
 #include "rl_internal.hpp"
 
 namespace lucem {
@@ -31,74 +37,114 @@ constexpr int kE = kTileElems;
 constexpr int kG = kTileMaxLen / 2 / (kT / kTileMaxLen);  // row-pair sweeps of a tile: 8
 constexpr int kW = 4;                                     // waves per SIMD: 16 per workgroup
 
-struct TileLds {
-    float2* Zr;  // row-pair lines: element c of pair b at Zr[c * B + b]
-    float2* Sc;  // column lines: element r of column k at Sc[r * C + k]
-    const float2* tw0;
-    const float2* tw1;
-    Divisor bd;  // B = tile_lines(L0 / 2) lines (odd: a padding line when L0 / 2 is even)
-    Divisor cd;  // C = tile_lines(L1 / 2 + 1) lines
-    int nb, nc;  // the lines that hold data: L0 / 2 row pairs, L1 / 2 + 1 columns
+// Tile dimensions at run time (SL0 = SL1 = 0) or at compile time.
+template <int SL0, int SL1>
+struct Dims {
+    static constexpr bool is_static = true;
+    static constexpr int L0 = SL0, L1 = SL1, nb = SL0 / 2, nc = SL1 / 2 + 1;
+    static constexpr int B = tile_lines(nb), C = tile_lines(nc);
+    static constexpr int nbuf = tile_buffer_elems(SL0, SL1);
+    __device__ explicit Dims(const TileIter&) {}
+    __device__ int div_nc(int e) const { return e / nc; }
+    __device__ int div_C(int e) const { return e / C; }
+};
+template <>
+struct Dims<0, 0> {
+    static constexpr bool is_static = false;
+    int L0, L1, nb, nc, B, C, nbuf;
+    Divisor ncd, Cd;
+    __device__ explicit Dims(const TileIter& d)
+        : L0(d.L0), L1(d.L1), nb(d.L0 / 2), nc(d.L1 / 2 + 1), B(tile_lines(d.L0 / 2)), C(tile_lines(d.L1 / 2 + 1)),
+          nbuf(tile_buffer_elems(d.L0, d.L1)), ncd(Divisor::of(d.L1 / 2 + 1)), Cd(Divisor::of(tile_lines(d.L1 / 2 + 1))) {}
+    __device__ int div_nc(int e) const { return ncd.div(e); }
+    __device__ int div_C(int e) const { return Cd.div(e); }
 };
 
-// The real tile in Zr -> its 2-D spectrum in Sc. The row split carries a factor 2, as the
-// transfer functions (built by the same row transform) expect.
-__device__ void forward_2d(const TileIter& d, const TileLds& t) {
-    lds_fft<false, kT, kE, kW>(t.Zr, t.bd, d.f1, d.tw1, t.tw1);
-    const int B = t.bd.d, C = t.cd.d, nc = t.nc, L1 = d.L1;
-    const Divisor ncd = Divisor::of(nc);
-    for (int e = threadIdx.x; e < t.nb * nc; e += kT) {
-        const int b = ncd.div(e), k = e - b * nc;
-        const float2 zk = t.Zr[k * B + b];
-        const float2 zm = t.Zr[(k == 0 ? 0 : L1 - k) * B + b];
-        t.Sc[(2 * b) * C + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
-        t.Sc[(2 * b + 1) * C + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
-    }
-    __syncthreads();
-    lds_fft<false, kT, kE, kW>(t.Sc, t.cd, d.f0, d.tw0, t.tw0);
+struct TileLds {
+    float2* X;
+    float2* Y;
+    const float2* tw0;
+    const float2* tw1;
+    __device__ float2* other(const float2* p) const { return p == X ? Y : X; }
+};
+
+// Length-L1 transforms of the row-pair lines in `a` (rows) or length-L0 transforms of the column
+// lines (cols); the result's buffer is returned.
+template <bool INV, class D>
+__device__ float2* rows_fft(const D& dm, const TileIter& d, const TileLds& t, float2* a) {
+    if constexpr (D::is_static)
+        return static_fft<D::L1, D::B, kT, INV>(a, t.other(a), t.tw1);
+    else
+        return lds_fft_oop<INV, kT, kE, kW>(a, t.other(a), Divisor::of(dm.B), d.f1, t.tw1);
+}
+template <bool INV, class D>
+__device__ float2* cols_fft(const D& dm, const TileIter& d, const TileLds& t, float2* a) {
+    if constexpr (D::is_static)
+        return static_fft<D::L0, D::C, kT, INV>(a, t.other(a), t.tw0);
+    else
+        return lds_fft_oop<INV, kT, kE, kW>(a, t.other(a), Divisor::of(dm.C), d.f0, t.tw0);
 }
 
-// Sc *= otf ([L0][pitch] in global memory; conj(FFT(w)) * scale) on the data columns, loads
+// The real tile in `zr` (row-pair lines) -> its 2-D spectrum (column lines); returns the buffer
+// that holds it. The row split carries a factor 2, as the transfer functions (built by the same
+// row transform) expect.
+template <class D>
+__device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, float2* zr) {
+    const float2* r = rows_fft<false>(dm, d, t, zr);
+    float2* sc = t.other(r);
+    for (int e = threadIdx.x; e < dm.nb * dm.nc; e += kT) {
+        const int b = dm.div_nc(e), k = e - b * dm.nc;
+        const float2 zk = r[k * dm.B + b];
+        const float2 zm = r[(k == 0 ? 0 : dm.L1 - k) * dm.B + b];
+        sc[(2 * b) * dm.C + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
+        sc[(2 * b + 1) * dm.C + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
+    }
+    __syncthreads();
+    return cols_fft<false>(dm, d, t, sc);
+}
+
+// sc *= otf ([L0][pitch] in global memory; conj(FFT(w)) * scale) on the data columns, loads
 // issued before the LDS reads.
-__device__ void multiply(const TileIter& d, const TileLds& t, const float2* __restrict__ otf) {
-    const int C = t.cd.d, n = d.L0 * C;
+template <class D>
+__device__ void multiply(const D& dm, const TileIter& d, float2* sc, const float2* __restrict__ otf) {
+    const int n = dm.L0 * dm.C;
     float2 w[kE];
 #pragma unroll
     for (int s = 0; s < kE; s++) {
         const int e = threadIdx.x + s * kT;
         w[s] = make_float2(0.f, 0.f);
         if (e < n) {
-            const int r = t.cd.div(e), k = e - r * C;
-            if (k < t.nc) w[s] = otf[(int64_t)r * d.pitch + k];
+            const int r = dm.div_C(e), k = e - r * dm.C;
+            if (k < dm.nc) w[s] = otf[(int64_t)r * d.pitch + k];
         }
     }
 #pragma unroll
     for (int s = 0; s < kE; s++) {
         const int e = threadIdx.x + s * kT;
-        if (e < n) t.Sc[e] = cmul(t.Sc[e], w[s]);
+        if (e < n) sc[e] = cmul(sc[e], w[s]);
     }
     __syncthreads();
 }
 
-// The spectrum in Sc -> the real tile in Zr (unnormalised: the transfer function carries the
-// 1 / (4 L0 L1)).
-__device__ void inverse_2d(const TileIter& d, const TileLds& t) {
-    lds_fft<true, kT, kE, kW>(t.Sc, t.cd, d.f0, d.tw0, t.tw0);
-    const int B = t.bd.d, C = t.cd.d, nc = t.nc, L1 = d.L1;
-    const Divisor ncd = Divisor::of(nc);
-    for (int e = threadIdx.x; e < t.nb * nc; e += kT) {
-        const int b = ncd.div(e), k = e - b * nc;
-        float2 A = t.Sc[(2 * b) * C + k], Bv = t.Sc[(2 * b + 1) * C + k];
-        if (k == 0 || 2 * k == L1) {
+// The spectrum in `sc` -> the real tile as row-pair lines; returns the buffer that holds it
+// (unnormalised: the transfer function carries the 1 / (4 L0 L1)).
+template <class D>
+__device__ float2* inverse_2d(const D& dm, const TileIter& d, const TileLds& t, float2* sc) {
+    const float2* r = cols_fft<true>(dm, d, t, sc);
+    float2* zr = t.other(r);
+    for (int e = threadIdx.x; e < dm.nb * dm.nc; e += kT) {
+        const int b = dm.div_nc(e), k = e - b * dm.nc;
+        float2 A = r[(2 * b) * dm.C + k], Bv = r[(2 * b + 1) * dm.C + k];
+        if (k == 0 || 2 * k == dm.L1) {
             A.y = 0.f;
             Bv.y = 0.f;
         }
-        t.Zr[k * B + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
-        const int km = L1 - k;
-        if (k > 0 && km >= nc) t.Zr[km * B + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+        zr[k * dm.B + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+        const int km = dm.L1 - k;
+        if (k > 0 && km >= dm.nc) zr[km * dm.B + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
     }
     __syncthreads();
-    lds_fft<true, kT, kE, kW>(t.Zr, t.bd, d.f1, d.tw1, t.tw1);
+    return rows_fft<true>(dm, d, t, zr);
 }
 
 // A box of the image (rows y0 + r, columns x0 + c) lies inside it: no boundary rule applies.
@@ -106,43 +152,39 @@ __device__ __forceinline__ bool box_inside(int y0, int x0, int h, int w, int n0,
     return y0 >= 0 && x0 >= 0 && y0 + h <= n0 && x0 + w <= n1;
 }
 
+template <int SL0, int SL1>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void tile_iter_kernel(TileIter d) {
     extern __shared__ float2 lds[];
-    const int nb = d.L0 / 2, nc = d.L1 / 2 + 1;
-    const int B = tile_lines(nb), C = tile_lines(nc);
+    const Dims<SL0, SL1> dm(d);
+    const int B = dm.B, nb = dm.nb;
     TileLds t;
-    t.Zr = lds;
-    t.Sc = lds + d.L1 * B;
-    float2* tw1 = t.Sc + d.L0 * C;
-    float2* tw0 = tw1 + d.L1;
+    t.X = lds;
+    t.Y = lds + dm.nbuf;
+    float2* tw1 = t.Y + dm.nbuf;
+    float2* tw0 = tw1 + dm.L1;
     t.tw1 = tw1;
     t.tw0 = tw0;
-    t.bd = Divisor::of(B);
-    t.cd = Divisor::of(C);
-    t.nb = nb;
-    t.nc = nc;
     const int tid = threadIdx.x;
     const int ty = blockIdx.x / d.tiles1, tx = blockIdx.x - ty * d.tiles1;
     const int t0y = ty * d.T0, t0x = tx * d.T1;
     // Tile sweeps: thread (pr, c) handles row pairs pr, pr + pstep, ... at column c (columns
     // fastest across lanes, so global rows are read whole and a pair is one 8-byte LDS access).
-    const int pstep = kT / d.L1;
-    const int c = tid - (tid / d.L1) * d.L1;
-    const int pr0 = tid / d.L1;
+    const int pstep = kT / dm.L1;
+    const int pr0 = tid / dm.L1;
+    const int c = tid - pr0 * dm.L1;
     const bool active = pr0 < pstep;
 
-    for (int i = tid; i < d.L1; i += kT) tw1[i] = d.tw1[d.f1.tw + i];
-    for (int i = tid; i < d.L0; i += kT) tw0[i] = d.tw0[d.f0.tw + i];
-    // padding lines hold zeros (transforms of them stay finite and are never read)
-    if (B > nb)
-        for (int i = tid; i < d.L1; i += kT) t.Zr[i * B + nb] = make_float2(0.f, 0.f);
-    if (C > nc)
-        for (int i = tid; i < d.L0; i += kT) t.Sc[i * C + nc] = make_float2(0.f, 0.f);
+    for (int i = tid; i < dm.L1; i += kT) tw1[i] = d.tw1[d.f1.tw + i];
+    for (int i = tid; i < dm.L0; i += kT) tw0[i] = d.tw0[d.f0.tw + i];
+    // both buffers start at zero, so that padding lines hold finite values (their transforms are
+    // never read)
+    for (int i = tid; i < 2 * dm.nbuf; i += kT) lds[i] = make_float2(0.f, 0.f);
+    __syncthreads();
 
-    // Q1: the estimate with the pad rule, two halos deep
+    // Q1: the estimate with the pad rule, two halos deep, into X
     {
         const int gy0 = t0y - 2 * d.left0, gx0 = t0x - 2 * d.left1;
-        const bool inside = box_inside(gy0, gx0, d.L0, d.L1, d.n0, d.n1);
+        const bool inside = box_inside(gy0, gx0, dm.L0, dm.L1, d.n0, d.n1);
         float2 v[kG];
 #pragma unroll
         for (int s = 0; s < kG; s++) {
@@ -173,20 +215,22 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
 #pragma unroll
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
-            if (active && b < nb) t.Zr[c * B + b] = v[s];
+            if (active && b < nb) t.X[c * B + b] = v[s];
         }
     }
     __syncthreads();
-    forward_2d(d, t);
-    multiply(d, t, d.otf1);
-    inverse_2d(d, t);
+    float2* sp = forward_2d(dm, d, t, t.X);
+    multiply(dm, d, sp, d.otf1);
+    const float2* U = inverse_2d(dm, d, t, sp);
 
-    // Zr now holds u(t0 - left + p) at p. Q2 = P(rel) on the rel window (my x mx), 0 beyond.
+    // U holds u(t0 - left + p) at p. Q2 = P(rel) on the rel window (my x mx), 0 beyond, into the
+    // other buffer (its last reader, the final inverse pass, is behind a barrier).
+    float2* Q = t.other(U);
     {
         const int my = d.T0 + d.k0 - 1, mx = d.T1 + d.k1 - 1;
         const int qy0 = t0y - d.left0, qx0 = t0x - d.left1;
         const bool inside = box_inside(qy0, qx0, my, mx, d.n0, d.n1);
-        const float* Zf = reinterpret_cast<const float*>(t.Zr);
+        const float* Uf = reinterpret_cast<const float*>(U);
         float2 v[kG];
         if (inside) {
             // every target is its own position: y and u of the pair, rel in place
@@ -205,7 +249,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
             for (int s = 0; s < kG; s++) {
                 const int b = pr0 + s * pstep;
                 u[s] = make_float2(1.f, 1.f);
-                if (active && b < nb && c < mx) u[s] = t.Zr[c * B + b];
+                if (active && b < nb && c < mx) u[s] = U[c * B + b];
             }
 #pragma unroll
             for (int s = 0; s < kG; s++) {
@@ -240,28 +284,27 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
 #pragma unroll
             for (int s = 0; s < kG; s++) {
                 if (at[2 * s] >= 0) {
-                    const float u = Zf[at[2 * s]];
+                    const float u = Uf[at[2 * s]];
                     v[s].x = div_fast(v[s].x, fabsf(u) < d.eps ? d.eps : u);
                 }
                 if (at[2 * s + 1] >= 0) {
-                    const float u = Zf[at[2 * s + 1]];
+                    const float u = Uf[at[2 * s + 1]];
                     v[s].y = div_fast(v[s].y, fabsf(u) < d.eps ? d.eps : u);
                 }
             }
         }
-        __syncthreads();
 #pragma unroll
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
-            if (active && b < nb) t.Zr[c * B + b] = v[s];
+            if (active && b < nb) Q[c * B + b] = v[s];
         }
     }
     __syncthreads();
-    forward_2d(d, t);
-    multiply(d, t, d.otf2);
-    inverse_2d(d, t);
+    sp = forward_2d(dm, d, t, Q);
+    multiply(dm, d, sp, d.otf2);
+    const float2* Cv = inverse_2d(dm, d, t, sp);
 
-    // Zr now holds c(t0 + p) at p: the update on the tile's own pixels
+    // Cv holds c(t0 + p) at p: the update on the tile's own pixels
     {
         const int h = min(d.T0, d.n0 - t0y), w = min(d.T1, d.n1 - t0x);
         float2 x[kG];
@@ -280,7 +323,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
             if (active && 2 * b < h && c < w) {
-                const float2 cv = t.Zr[c * B + b];
+                const float2 cv = Cv[c * B + b];
                 float xa = x[s].x * cv.x, xb = x[s].y * cv.y;
                 if (d.last && d.clip) {
                     xa = fminf(fmaxf(xa, -1.f), 1.f);
@@ -293,23 +336,52 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
         }
     }
 }
+
+// The square domains compiled with their lengths as constants.
+#define LUCEM_TILE_STATIC(X) X(64) X(80) X(96) X(112) X(120) X(128)
 }  // namespace
 
+bool tile_len_static(int L0, int L1) {
+    if (L0 != L1) return false;
+#define LUCEM_CASE(n) \
+    if (L0 == n) return true;
+    LUCEM_TILE_STATIC(LUCEM_CASE)
+#undef LUCEM_CASE
+    return false;
+}
+
 size_t tile_iter_lds_bytes(int L0, int L1) {
-    return sizeof(float2) * ((size_t)L1 * tile_lines(L0 / 2) + (size_t)L0 * tile_lines(L1 / 2 + 1) + (size_t)L1 +
-                             (size_t)L0);
+    return sizeof(float2) * (2 * (size_t)tile_buffer_elems(L0, L1) + (size_t)L1 + (size_t)L0);
 }
 
 int allow_tile_lds() {
-    return (int)hipFuncSetAttribute(reinterpret_cast<const void*>(tile_iter_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)tile_iter_lds_bytes(kTileMaxLen, kTileMaxLen));
+    const int bytes = (int)tile_iter_lds_bytes(kTileMaxLen, kTileMaxLen);
+    int err = (int)hipFuncSetAttribute(reinterpret_cast<const void*>(tile_iter_kernel<0, 0>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+#define LUCEM_CASE(n)                                                                         \
+    if (!err)                                                                                 \
+        err = (int)hipFuncSetAttribute(reinterpret_cast<const void*>(tile_iter_kernel<n, n>), \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    LUCEM_TILE_STATIC(LUCEM_CASE)
+#undef LUCEM_CASE
+    return err;
 }
 
 int launch_tile_iter(const TileIter& d, int blocks, void* stream) {
-    hipLaunchKernelGGL(tile_iter_kernel, dim3(blocks), dim3(kT), tile_iter_lds_bytes(d.L0, d.L1), (hipStream_t)stream,
-                       d);
-    return (int)hipGetLastError();
+    const size_t lds = tile_iter_lds_bytes(d.L0, d.L1);
+    const hipStream_t s = (hipStream_t)stream;
+    if (d.generic || !tile_len_static(d.L0, d.L1)) {
+        hipLaunchKernelGGL((tile_iter_kernel<0, 0>), dim3(blocks), dim3(kT), lds, s, d);
+        return (int)hipGetLastError();
+    }
+#define LUCEM_CASE(n)                                                                     \
+    if (d.L0 == n) {                                                                      \
+        hipLaunchKernelGGL((tile_iter_kernel<n, n>), dim3(blocks), dim3(kT), lds, s, d); \
+        return (int)hipGetLastError();                                                    \
+    }
+    LUCEM_TILE_STATIC(LUCEM_CASE)
+#undef LUCEM_CASE
+    return (int)hipErrorInvalidValue;
 }
 
 }  // namespace lucem

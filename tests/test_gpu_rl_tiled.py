@@ -133,3 +133,25 @@ def test_tiled_against_whole_array_at_full_size(gpu, tiled_mode):
     e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     print(f"tiled vs whole-array: rel-L2 {e:.2e}")
     assert np.isfinite(a).all() and e < 2e-6
+
+
+@pytest.mark.parametrize("shape,kshape,grid", [((1024, 1024), (25, 25), None), ((300, 200), (9, 9), "64,64,48,48"),
+                                              ((500, 420), (17, 13), "96,96,64,72"), ((90, 70), (3, 3), "128,128,124,124")])
+def test_constant_length_build_equals_run_time_build(gpu, tiled_mode, monkeypatch, shape, kshape, grid):
+    """The kernel compiled with the tile lengths as constants runs the same radix sequence as the
+    run-time-length build; the compiler fuses their multiply-adds differently, so they agree to
+    rounding."""
+    tiled_mode("1")
+    if grid:
+        monkeypatch.setenv("LUCEM_TILE_GRID", grid)
+    rng = np.random.default_rng(zlib.crc32(repr(shape).encode()))
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    monkeypatch.setenv("LUCEM_TILE_GENERIC", "1")
+    a, sa = _run(gpu, y, psf, 4)
+    monkeypatch.setenv("LUCEM_TILE_GENERIC", "0")
+    b, sb = _run(gpu, y, psf, 4)
+    assert "tile_iter" in sa and "tile_iter" in sb
+    e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print(f"{shape} {kshape}: constant vs run-time lengths rel-L2 {e:.2e}")
+    assert e < 1e-6
