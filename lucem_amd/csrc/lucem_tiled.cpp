@@ -91,24 +91,35 @@ bool tiled_mode_env(int* mode) {
 }
 }  // namespace
 
+// Modelled time of one iteration on a grid: workgroups run in rounds of (CUs x workgroups per CU);
+// a round costs a fixed latency (launch, the global reads and writes, the phase barriers) plus the
+// tile's transform work, and two workgroups sharing a CU each run about 1.6x slower. Measured on
+// MI355X (30-iteration calls): with constant lengths a 64 x 64 tile takes 16.5 us and a 112 x 112
+// one 31 us (9.5 us + 1.7 ns per element-pass); the run-time-length build takes 27 us on 32 x 32
+// and 46 us on 112 x 112 (26 us + 1.6 ns per element-pass).
+static double grid_cost(int L0, int L1, int64_t tiles, int num_cus) {
+    const size_t lds_cu = 160 * 1024;
+    const int per_cu = tile_iter_lds_bytes(L0, L1) * 2 <= lds_cu ? 2 : 1;
+    const int64_t slots = (int64_t)num_cus * per_cu;
+    const int64_t rounds = (tiles + slots - 1) / slots;
+    const double elem_passes = (double)L0 * L1 * (make_plan(L0).npass + make_plan(L1).npass + 2) / 6.0;
+    const double round_us = tile_len_static(L0, L1) ? 9.5 + 1.72e-3 * elem_passes : 26.0 + 1.6e-3 * elem_passes;
+    const bool shared = tiles > (int64_t)num_cus && per_cu == 2;
+    return (double)rounds * round_us * (shared ? 1.6 : 1.0);
+}
+
 bool lucem::tile_plan(const Geometry& g, int num_cus, TileGrid* out) {
     if (g.ndim != 2 || g.origin[1] != 0 || g.origin[2] != 0) return false;
     const int n0 = g.n[1], n1 = g.n[2], k0 = g.k[1], k1 = g.k[2];
     const std::vector<AxisChoice> a0 = axis_candidates(n0, k0, g.mode), a1 = axis_candidates(n1, k1, g.mode);
-    const size_t lds_cu = 160 * 1024;
     double best = 0.0;
     bool found = false;
     for (const AxisChoice& y : a0) {
         if (y.L % 2) continue;
         for (const AxisChoice& x : a1) {
             if (!tile_fits(y.L, x.L)) continue;
-            const size_t lds = tile_iter_lds_bytes(y.L, x.L);
-            const int per_cu = lds * 2 <= lds_cu ? 2 : 1;
             const int64_t tiles = (int64_t)((n0 + y.T - 1) / y.T) * ((n1 + x.T - 1) / x.T);
-            const int64_t rounds = (tiles + (int64_t)num_cus * per_cu - 1) / ((int64_t)num_cus * per_cu);
-            // a workgroup's time grows with its transform work; two sharing a CU take about 1.6x one
-            const double work = (double)y.L * x.L * (make_plan(y.L).npass + make_plan(x.L).npass + 2);
-            const double cost = (double)rounds * work * (per_cu == 2 ? 1.6 : 1.0);
+            const double cost = grid_cost(y.L, x.L, tiles, num_cus);
             if (!found || cost < best) {
                 best = cost;
                 found = true;
@@ -119,26 +130,6 @@ bool lucem::tile_plan(const Geometry& g, int num_cus, TileGrid* out) {
     return found;
 }
 
-// Whether the plain 2-D path takes the tiled iteration: LUCEM_RL_TILED=0 never, =1 whenever a tile
-// plan exists; otherwise when the plan's tiles fill the device in at most two rounds (arrays small
-// enough that the whole-array passes are bound by latency, not bandwidth).
-bool lucem::use_tiled(const Geometry& g, int num_cus, TileGrid* plan) {
-    if (!tile_plan(g, num_cus, plan)) return false;
-    // LUCEM_TILE_GRID="L0,L1,T0,T1" replaces the chosen grid when it is admissible (measurement)
-    if (const char* v = std::getenv("LUCEM_TILE_GRID")) {
-        int L0 = 0, L1 = 0, T0 = 0, T1 = 0;
-        if (std::sscanf(v, "%d,%d,%d,%d", &L0, &L1, &T0, &T1) == 4 && tile_len_ok(L0) && tile_len_ok(L1) && T0 >= 1 && T1 >= 1 && T0 + 2 * (g.k[1] - 1) <= L0 && T1 + 2 * (g.k[2] - 1) <= L1 &&
-            tile_fits(L0, L1) &&
-            axis_admits(g.n[1], g.k[1], T0, g.mode) && axis_admits(g.n[2], g.k[2], T1, g.mode))
-            *plan = TileGrid{L0, L1, T0, T1, (g.n[1] + T0 - 1) / T0, (g.n[2] + T1 - 1) / T1};
-    }
-    int mode = 0;
-    if (tiled_mode_env(&mode)) return mode != 0;
-    const int64_t tiles = (int64_t)plan->tiles0 * plan->tiles1;
-    const int per_cu = tile_iter_lds_bytes(plan->L0, plan->L1) * 2 <= 160 * 1024 ? 2 : 1;
-    return tiles <= 2 * (int64_t)num_cus * per_cu;
-}
-
 // The canonical algorithmic bytes of one iteration on the whole array's minimal domain
 // (24 n + 80 N_c), the figure the whole-array passes are measured against.
 static double canonical_iteration_bytes(const Geometry& g) {
@@ -146,6 +137,33 @@ static double canonical_iteration_bytes(const Geometry& g) {
     const double Nc = Ly * (std::floor(Lx / 2) + 1);
     return 24.0 * g.n[1] * g.n[2] + 80.0 * Nc;
 }
+
+// Whether the plain 2-D path takes the tiled iteration: LUCEM_RL_TILED=0 never, =1 whenever a tile
+// grid is admissible; otherwise when its modelled time is below that of the whole-array passes.
+// Those take four launches of about 12 us each plus the canonical bytes at the 2.7 TB/s they reach
+// (measured on MI355X from 256 x 256 to 3072 x 3072), so the tiled form takes every size measured.
+bool lucem::use_tiled(const Geometry& g, int num_cus, TileGrid* plan) {
+    if (!tile_plan(g, num_cus, plan)) return false;
+    // LUCEM_TILE_GRID="L0,L1,T0,T1" replaces the chosen grid when it is admissible (measurement)
+    if (const char* v = std::getenv("LUCEM_TILE_GRID")) {
+        int L0 = 0, L1 = 0, T0 = 0, T1 = 0;
+        if (std::sscanf(v, "%d,%d,%d,%d", &L0, &L1, &T0, &T1) == 4 && tile_len_ok(L0) && tile_len_ok(L1) && T0 >= 1 &&
+            T1 >= 1 && T0 + 2 * (g.k[1] - 1) <= L0 && T1 + 2 * (g.k[2] - 1) <= L1 && tile_fits(L0, L1) &&
+            axis_admits(g.n[1], g.k[1], T0, g.mode) && axis_admits(g.n[2], g.k[2], T1, g.mode))
+            *plan = TileGrid{L0, L1, T0, T1, (g.n[1] + T0 - 1) / T0, (g.n[2] + T1 - 1) / T1};
+    }
+    int mode = 0;
+    if (tiled_mode_env(&mode)) return mode != 0;
+    const double tiled_us = grid_cost(plan->L0, plan->L1, (int64_t)plan->tiles0 * plan->tiles1, num_cus);
+    const double whole_us = 4 * 12.0 + canonical_iteration_bytes(g) / 2.7e6;
+    if (const char* v = std::getenv("LUCEM_TUNE_LOG"))
+        if (std::atoi(v) == 1)
+            std::fprintf(stderr, "lucem tiled: %dx%d psf %dx%d -> L %dx%d, tiles %dx%d of %dx%d (model %.1f us; whole %.1f us)\n",
+                         g.n[1], g.n[2], g.k[1], g.k[2], plan->L0, plan->L1, plan->tiles0, plan->tiles1, plan->T0,
+                         plan->T1, tiled_us, whole_us);
+    return tiled_us < whole_us;
+}
+
 
 // The tile domain's geometry, on which the transfer functions are built: (1, L0, L1).
 static Geometry tile_geometry(const Geometry& g, const TileGrid& tp) {

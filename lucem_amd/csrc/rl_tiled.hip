@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
 }
 
 // The square domains compiled with their lengths as constants.
-#define LUCEM_TILE_STATIC(X) X(64) X(80) X(96) X(112) X(120) X(128)
+#define LUCEM_TILE_STATIC(X) X(32) X(48) X(64) X(80) X(96) X(112) X(120) X(128)
 }  // namespace
 
 bool tile_len_static(int L0, int L1) {
