@@ -281,6 +281,22 @@ def plain_leg(workload: str, steps: int, warmup: int, with_cpu: bool) -> dict:
     stats = client.profile_read()
     client.profile(False)
     assert bool(torch.isfinite(out).all()), "non-finite output"
+    whole_ms = None
+    if "tile_iter" in stats:
+        # the same call on the whole-array passes (four launches per iteration), for comparison
+        os.environ["LUCEM_RL_TILED"] = "0"
+        try:
+            ws_whole = N.Workspace(client)
+            run(y, psf, out, settings, ws_whole, client)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                run(y, psf, out, settings, ws_whole, client)
+            torch.cuda.synchronize()
+            whole_ms = (time.perf_counter() - t1) * 1e3 / steps
+            ws_whole.close()
+        finally:
+            del os.environ["LUCEM_RL_TILED"]
 
     nvox = int(np.prod(prob.image.shape))
     iter_kernels = {k: v for k, v in stats.items() if not k.startswith("otf") and k != "psf_prep"}
@@ -322,6 +338,11 @@ def plain_leg(workload: str, steps: int, warmup: int, with_cpu: bool) -> dict:
                     for k, v in stats.items()},
         "cpu_baseline": cpu_baseline(prob, workload) if with_cpu else None,
     }
+    if whole_ms is not None:
+        leg["path"] = ("tiled: one launch per iteration, each workgroup an output tile with its halo "
+                       "(overlap-save FFT correlations in LDS); roofline bytes are the whole-array "
+                       "iteration's canonical bytes, the tile kernel's HBM traffic is `traffic`")
+        leg["whole_array_ms_per_step"] = round(whole_ms, 3)
     return leg
 
 
@@ -540,6 +561,9 @@ def main() -> int:
         "kernels": head["kernels"],
         "cpu_baseline": head["cpu_baseline"],
     }
+    for key in ("path", "whole_array_ms_per_step"):
+        if key in head:
+            line[key] = head[key]
     if not args.no_extras:
         other = "plain2d" if args.workload == "plain3d" else "plain3d"
         line[other] = plain_leg(other, max(args.steps, 10) if other == "plain2d" else args.steps, 2,
