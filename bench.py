@@ -160,6 +160,13 @@ def batched_leg(steps: int) -> dict:
 
     tp = _timeit(loop, 2)
     diff = max(float(((a - c).norm() / c.norm()).item()) for a, c in zip(outs, outs_p))
+    # the same loop with the plain call on its whole-array passes (four launches per iteration),
+    # the per-tile baseline before the plain call took the one-launch tiled iteration
+    os.environ["LUCEM_RL_TILED"] = "0"
+    try:
+        tw = _timeit(loop, 2)
+    finally:
+        del os.environ["LUCEM_RL_TILED"]
     nvox = sum(y.numel() for y in ys)
     alg = sum(minimal_domain_bytes(tuple(y.shape), tuple(psfs[i].shape)) for y, i in zip(ys, ts.psf_index))
     gbps = alg * ts.num_iter / tb / 1e9
@@ -183,6 +190,8 @@ def batched_leg(steps: int) -> dict:
         "plain_loop_value": round(nvox * ts.num_iter / tp / 1e6, 2),
         "plain_loop_achieved_GBps": round(alg * ts.num_iter / tp / 1e9, 1),
         "speedup_vs_plain_loop": round(tp / tb, 3),
+        "plain_loop_whole_array_ms": round(tw * 1e3, 3),
+        "speedup_vs_whole_array_plain_loop": round(tw / tb, 3),
         "max_rel_l2_batched_vs_plain": diff,
         "device_bytes": b.device_bytes,
     }

@@ -86,6 +86,13 @@ struct lucem_rl_batch {
     // occupancy x CUs workgroups striding over the items (STRIDE) or taking them from an atomic
     // queue (QUEUE).
     enum Schedule { QUEUE, STRIDE, DIRECT } schedule = DIRECT;
+    // Tiled mode (2-D tiles): one launch per iteration over the output tiles of every array
+    // (rl_tiled.hip), one job descriptor per non-empty tile.
+    bool tiled = false;
+    std::vector<int> job_of;            // tile -> job (-1: empty tile)
+    std::vector<TileIter> h_jobs;
+    TileIter* d_jobs = nullptr;
+    TileIter* h_job_stage = nullptr;
     ~lucem_rl_batch();
 };
 
@@ -101,6 +108,7 @@ lucem_rl_batch::~lucem_rl_batch() {
         if (staged) (void)hipEventDestroy(staged);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_psf_stage) (void)hipHostFree(h_psf_stage);
+        if (h_job_stage) (void)hipHostFree(h_job_stage);
         if (mem) lucem::dev_free(client ? client->device : 0, mem, mem_bytes);
     } else if (mem) {
         lucem::dev_forget(client ? client->device : 0, mem_bytes);
@@ -162,6 +170,179 @@ static lucem_status batch_validate(const lucem_rl_tile* tiles, uint64_t num_tile
     return LUCEM_OK;
 }
 
+// The tiled plan: per tile its two estimates, per PSF its normalised copies and the two transfer
+// functions on the common L x L domain; captured: PSF preparation, the transfer-function builds,
+// then one tile launch per iteration. *done is false (and nothing is made) when the tiles do not
+// all admit one compile-time length or the per-pass plan is modelled faster (batches with much
+// work in all: the per-pass kernels then stream at HBM speed), and the per-pass plan is built.
+static lucem_status create_tiled(lucem_rl_batch* b, const std::vector<Geometry>& geos, const std::vector<bool>& empty,
+                                 const lucem_rl_settings& s, bool* done) {
+    *done = false;
+    if (b->ndim != 2 || tiled_mode_env() == 0) return LUCEM_OK;
+    lucem_client* client = b->client;
+    std::vector<Geometry> gs;
+    for (size_t t = 0; t < geos.size(); t++)
+        if (!empty[t]) gs.push_back(geos[t]);
+    std::vector<TileGrid> grids;
+    double tiled_us = 0.0, whole_us = 0.0;
+    if (gs.empty() || !tile_batch_plan(gs, client->num_cus, &grids, &tiled_us, &whole_us)) return LUCEM_OK;
+    // LUCEM_RL_TILED=1 takes the tiled form whenever it is admissible; otherwise when it is modelled faster
+    if (tiled_mode_env() != 1 && tiled_us >= whole_us) return LUCEM_OK;
+    const int L = grids[0].L0;
+    const size_t ntiles = geos.size(), npsf = b->psf_dims.size();
+    b->tiled = true;
+    b->job_of.assign(ntiles, -1);
+    int njobs = 0;
+    for (size_t t = 0; t < ntiles; t++)
+        if (!empty[t]) b->job_of[t] = njobs++;
+
+    const float2* tw = nullptr;
+    lucem_status st;
+    if ((st = twiddles_for(client, L, &tw))) return st;
+    // transfer functions per PSF on the (1, L, L) domain
+    std::vector<Geometry> gpsf(npsf);
+    for (size_t j = 0; j < npsf; j++) {
+        Geometry gt{};
+        gt.ndim = 2;
+        gt.n[0] = 1;
+        gt.n[1] = gt.n[2] = L;
+        gt.k[0] = 1;
+        gt.k[1] = (int)b->psf_dims[j][0];
+        gt.k[2] = (int)b->psf_dims[j][1];
+        gt.L[0] = 1;
+        gt.L[1] = gt.L[2] = L;
+        gt.Lc = L / 2 + 1;
+        gt.pitch = (gt.Lc + 15) & ~15;
+        gpsf[j] = gt;
+    }
+    // work items, largest tiles first
+    std::vector<size_t> order;
+    for (size_t t = 0; t < ntiles; t++)
+        if (!empty[t]) order.push_back(t);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return (int64_t)geos[x].n[1] * geos[x].n[2] > (int64_t)geos[y].n[1] * geos[y].n[2];
+    });
+    std::vector<int2> items;
+    for (size_t t : order) {
+        const TileGrid& tg = grids[b->job_of[t]];
+        for (int q = 0; q < tg.tiles0 * tg.tiles1; q++) items.push_back(make_int2(b->job_of[t], q));
+    }
+
+    struct Ptrs {
+        std::vector<float*> est_a, est_b, h, rev;
+        std::vector<float2*> otf1, otf2, pa;
+        TileIter* jobs;
+        PsfPrep* psf;
+        int2* items;
+    } P;
+    auto layout = [&](Carver& cv) {
+        P.est_a.assign(njobs, nullptr);
+        P.est_b.assign(njobs, nullptr);
+        for (size_t t = 0; t < ntiles; t++) {
+            if (b->job_of[t] < 0) continue;
+            const size_t n = (size_t)geos[t].n[1] * geos[t].n[2];
+            P.est_a[b->job_of[t]] = cv.take<float>(n);
+            P.est_b[b->job_of[t]] = cv.take<float>(n);
+        }
+        P.h.assign(npsf, nullptr);
+        P.rev = P.h;
+        P.otf1.assign(npsf, nullptr);
+        P.otf2 = P.pa = P.otf1;
+        for (size_t j = 0; j < npsf; j++) {
+            const Geometry& gt = gpsf[j];
+            const size_t nk = (size_t)gt.k[1] * gt.k[2];
+            P.h[j] = cv.take<float>(nk);
+            P.rev[j] = cv.take<float>(nk);
+            P.otf1[j] = cv.take<float2>((size_t)L * gt.pitch);
+            P.otf2[j] = cv.take<float2>((size_t)L * gt.pitch);
+            P.pa[j] = cv.take<float2>((size_t)gt.k[1] * gt.pitch);
+        }
+        P.jobs = cv.take<TileIter>(njobs);
+        P.psf = cv.take<PsfPrep>(npsf);
+        P.items = cv.take<int2>(items.size());
+    };
+    Carver sizing;
+    layout(sizing);
+    b->mem_bytes = sizing.off;
+    {
+        void* m = nullptr;
+        if ((st = dev_malloc(client->device, &m, b->mem_bytes, "batch plan"))) return st;
+        b->mem = m;
+    }
+    Carver real;
+    real.base = static_cast<char*>(b->mem);
+    layout(real);
+
+    b->h_jobs.resize(njobs);
+    for (size_t t = 0; t < ntiles; t++) {
+        const int j = b->job_of[t];
+        if (j < 0) continue;
+        const uint32_t pi = b->tiles[t].psf;
+        TileIter d = tile_desc(geos[t], grids[j], s, P.otf1[pi], P.otf2[pi], tw, tw);
+        d.est_in = P.est_a[j];
+        d.est_out = P.est_b[j];
+        b->h_jobs[j] = d;
+    }
+    b->h_psf.resize(npsf);
+    for (size_t j = 0; j < npsf; j++)
+        b->h_psf[j] = PsfPrep{(int64_t)gpsf[j].k[1] * gpsf[j].k[2], nullptr, P.h[j], P.rev[j]};
+    hipError_t e;
+    if (!items.empty() && (e = upload_sync(P.items, items.data(), sizeof(int2) * items.size(), client->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(batch descriptors)");
+    b->d_jobs = P.jobs;
+    b->d_psf = P.psf;
+
+    // capture: PSF preparation, transfer functions, iterations
+    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate(capture)");
+    if ((e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    int err = npsf ? launch_psf_batch(PsfBatch{P.psf, (int)npsf}, b->capture) : 0;
+    const ShapePolicy osp{client->num_cus, 0, 0, nullptr};
+    const float2* tws[3] = {nullptr, tw, tw};
+    uint64_t launches = npsf ? 1 : 0;
+    for (size_t j = 0; j < npsf && !err; j++) {
+        for (int k = 0; k < 2 && !err; k++) {
+            RowPass r;
+            int rb;
+            ColPass cols[2];
+            int cb[2];
+            make_otf_descs(gpsf[j], osp, k == 0 ? P.rev[j] : P.h[j], P.pa[j], nullptr, k == 0 ? P.otf1[j] : P.otf2[j],
+                           tws, &r, &rb, cols, cb);
+            err = launch_row_pass(r, rb, b->capture);
+            if (!err) err = launch_col_pass(cols[0], cb[0], b->capture);
+            launches += 2;
+        }
+    }
+    for (uint64_t it = 0; it < s.num_iter && !err; it++) {
+        const TileBatch tb{P.jobs, P.items, (int)items.size(), it == 0 ? 1 : 0, it + 1 == s.num_iter ? 1 : 0,
+                           s.clip ? 1 : 0, (int)(it & 1), L, L};
+        err = launch_tile_batch(tb, b->capture);
+        launches++;
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(b->capture, &graph);
+    if (err) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(err, "batched launch capture");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = graph;
+    if ((e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+        return hip_fail(e, "hipGraphInstantiate");
+    b->launches = launches;
+    if ((e = hipHostMalloc((void**)&b->h_job_stage, sizeof(TileIter) * std::max<int>(njobs, 1), hipHostMallocDefault)) !=
+        hipSuccess)
+        return hip_fail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void**)&b->h_psf_stage, sizeof(PsfPrep) * std::max<size_t>(npsf, 1), hipHostMallocDefault)) !=
+        hipSuccess)
+        return hip_fail(e, "hipHostMalloc");
+    if ((e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    *done = true;
+    return LUCEM_OK;
+}
+
 extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_rl_tile* tiles, uint64_t num_tiles,
                                               const uint64_t* psf_dims, uint32_t psf_ndim, uint64_t num_psfs,
                                               const lucem_rl_settings* settings, lucem_rl_batch** out) {
@@ -209,6 +390,15 @@ extern "C" lucem_status lucem_rl_batch_create(lucem_client* client, const lucem_
             tp.group = it->second;
         }
         b->tiles.push_back(tp);
+    }
+
+    {
+        bool done = false;
+        if ((st = create_tiled(b.get(), geos, empty, *settings, &done))) return st;
+        if (done) {
+            *out = b.release();
+            return LUCEM_OK;
+        }
     }
 
     // twiddle tables for every length used
@@ -657,13 +847,26 @@ lucem_status batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const 
         if (tp.group < 0) continue;
         if (!tl.input || !tl.output) return fail(LUCEM_INVALID_HANDLE, "tile pointer is NULL");
         Span in, out;
-        lucem_status st = apply_layout(layouts ? &layouts[t] : nullptr, tp, t, b->h_rows[t], &in, &out, tl.input,
-                                       tl.output);
+        RowPass view{};
+        RowPass& r = b->tiled ? view : b->h_rows[t];
+        lucem_status st = apply_layout(layouts ? &layouts[t] : nullptr, tp, t, r, &in, &out, tl.input, tl.output);
         if (st) return st;
         ins.push_back(in);
         outs.push_back(out);
-        b->h_rows[t].obs = tl.input;
-        b->h_rows[t].out = tl.output;
+        if (b->tiled) {
+            TileIter& d = b->h_jobs[b->job_of[t]];
+            d.obs = tl.input;
+            d.out = tl.output;
+            d.obs_rs = r.obs_rs;
+            d.out_rs = r.out_rs;
+            d.own0y = r.own0[1];
+            d.own0x = r.own0[2];
+            d.own1y = r.own1[1];
+            d.own1x = r.own1[2];
+        } else {
+            r.obs = tl.input;
+            r.out = tl.output;
+        }
     }
     // Every tile reads its input in every iteration while the others store their results, so no
     // stored range may overlap any input range (strided views are compared by their extents).
@@ -687,12 +890,20 @@ lucem_status batch_execute(lucem_rl_batch* b, const lucem_rl_tile* tiles, const 
         if (!psfs[j]) return fail(LUCEM_INVALID_HANDLE, "psf pointer is NULL");
         b->h_psf[j].psf = psfs[j];
     }
-    std::memcpy(b->h_stage, b->h_rows.data(), sizeof(RowPass) * b->h_rows.size());
     std::memcpy(b->h_psf_stage, b->h_psf.data(), sizeof(PsfPrep) * b->h_psf.size());
-    if (!b->h_rows.empty() &&
-        (e = hipMemcpyAsync(b->d_rows, b->h_stage, sizeof(RowPass) * b->h_rows.size(), hipMemcpyHostToDevice,
-                            c->stream)) != hipSuccess)
-        return hip_fail(e, "hipMemcpyAsync(descriptors)");
+    if (b->tiled) {
+        std::memcpy(b->h_job_stage, b->h_jobs.data(), sizeof(TileIter) * b->h_jobs.size());
+        if (!b->h_jobs.empty() &&
+            (e = hipMemcpyAsync(b->d_jobs, b->h_job_stage, sizeof(TileIter) * b->h_jobs.size(), hipMemcpyHostToDevice,
+                                c->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync(descriptors)");
+    } else {
+        std::memcpy(b->h_stage, b->h_rows.data(), sizeof(RowPass) * b->h_rows.size());
+        if (!b->h_rows.empty() &&
+            (e = hipMemcpyAsync(b->d_rows, b->h_stage, sizeof(RowPass) * b->h_rows.size(), hipMemcpyHostToDevice,
+                                c->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync(descriptors)");
+    }
     if (!b->h_psf.empty() &&
         (e = hipMemcpyAsync(b->d_psf, b->h_psf_stage, sizeof(PsfPrep) * b->h_psf.size(), hipMemcpyHostToDevice,
                             c->stream)) != hipSuccess)

@@ -199,6 +199,19 @@ struct TileSizes {
     size_t est, otf, psf, pa;
 };
 TileSizes tile_sizes(const Geometry& g, const TileGrid& tp);
+// The tile domain's geometry (1, L0, L1), on which the transfer functions are built.
+Geometry tile_geometry(const Geometry& g, const TileGrid& tp);
+// The descriptor of one array's tiled iteration, without its array pointers (obs, out, est_in,
+// est_out) and with the whole array as its output window.
+TileIter tile_desc(const Geometry& g, const TileGrid& tp, const lucem_rl_settings& s, const float2* otf1,
+                   const float2* otf2, const float2* tw0, const float2* tw1);
+// One compile-time length for a batch of 2-D arrays: the grids (same L0 = L1 for all) that admit
+// every array at the least modelled time (false when no such length exists), that time and the
+// modelled time of the per-pass batch of the same arrays, in us per iteration.
+bool tile_batch_plan(const std::vector<Geometry>& gs, int num_cus, std::vector<TileGrid>* grids, double* tiled_us,
+                     double* whole_us);
+// LUCEM_RL_TILED: -1 unset, 0 off, 1 on.
+int tiled_mode_env();
 // The launches of one tiled call on buffers b with the twiddle tables of L0 and L1. Allocates nothing.
 lucem_status rl_tiled_launch(lucem_client* c, const Geometry& g, const lucem_rl_settings& s, const float* input,
                              const float* psf, float* output, const TileBufs& b, const float2* tw0, const float2* tw1,

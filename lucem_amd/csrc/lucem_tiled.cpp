@@ -30,11 +30,13 @@ const int kTileLens[] = {16, 18, 20, 24, 28, 30, 32, 36, 40, 42, 48, 50, 54, 56,
                          64, 70, 72, 80, 84, 90, 96, 98, 100, 108, 112, 120, 126, 128};
 
 // Every pad target of every tile's rel window lies in that window: rel at a reflected position is
-// then a value the tile computed itself. Interior windows map to themselves; only folds can fail
-// (Wrap on a multi-tile axis, or an axis shorter than a fold).
+// then a value the tile computed itself. A tile's window covers its own outputs (the last tile's
+// may be shorter than T) with the PSF's reach on either side. Interior windows map to themselves;
+// only folds can fail (Wrap on a multi-tile axis, or an axis shorter than a fold).
 bool axis_admits(int n, int k, int T, int mode) {
-    const int left = k / 2, m = T + k - 1;
+    const int left = k / 2;
     for (int t0 = 0; t0 < n; t0 += T) {
+        const int m = std::min(T, n - t0) + k - 1;
         const int q0 = t0 - left;
         if (q0 >= 0 && q0 + m <= n) continue;
         for (int i = 0; i < m; i++) {
@@ -83,7 +85,7 @@ bool tile_len_ok(int L) {
     return false;
 }
 
-bool tiled_mode_env(int* mode) {
+bool tiled_mode_env_impl(int* mode) {
     const char* v = std::getenv("LUCEM_RL_TILED");
     if (!v) return false;
     *mode = std::atoi(v);
@@ -153,7 +155,7 @@ bool lucem::use_tiled(const Geometry& g, int num_cus, TileGrid* plan) {
             *plan = TileGrid{L0, L1, T0, T1, (g.n[1] + T0 - 1) / T0, (g.n[2] + T1 - 1) / T1};
     }
     int mode = 0;
-    if (tiled_mode_env(&mode)) return mode != 0;
+    if (tiled_mode_env_impl(&mode)) return mode != 0;
     const double tiled_us = grid_cost(plan->L0, plan->L1, (int64_t)plan->tiles0 * plan->tiles1, num_cus);
     const double whole_us = 4 * 12.0 + canonical_iteration_bytes(g) / 2.7e6;
     if (const char* v = std::getenv("LUCEM_TUNE_LOG"))
@@ -165,8 +167,43 @@ bool lucem::use_tiled(const Geometry& g, int num_cus, TileGrid* plan) {
 }
 
 
+TileIter lucem::tile_desc(const Geometry& g, const TileGrid& tp, const lucem_rl_settings& s, const float2* otf1,
+                         const float2* otf2, const float2* tw0, const float2* tw1) {
+    TileIter d{};
+    d.n0 = g.n[1];
+    d.n1 = g.n[2];
+    d.T0 = tp.T0;
+    d.T1 = tp.T1;
+    d.left0 = g.k[1] / 2;
+    d.left1 = g.k[2] / 2;
+    d.k0 = g.k[1];
+    d.k1 = g.k[2];
+    d.L0 = tp.L0;
+    d.L1 = tp.L1;
+    d.tiles1 = tp.tiles1;
+    d.mode = g.mode;
+    d.pitch = ((tp.L1 / 2 + 1) + 15) & ~15;
+    d.clip = s.clip ? 1 : 0;
+    {
+        const char* v = std::getenv("LUCEM_TILE_GENERIC");  // the run-time-length build, for comparison
+        d.generic = v && std::atoi(v) != 0;
+    }
+    d.eps = rl_eps(s);
+    d.init = 0.5f;  // deconvolve.rs:93
+    d.f0 = make_plan(tp.L0);
+    d.f1 = make_plan(tp.L1);
+    d.otf1 = otf1;
+    d.otf2 = otf2;
+    d.tw0 = tw0;
+    d.tw1 = tw1;
+    d.own0y = d.own0x = 0;
+    d.own1y = g.n[1];
+    d.own1x = g.n[2];
+    return d;
+}
+
 // The tile domain's geometry, on which the transfer functions are built: (1, L0, L1).
-static Geometry tile_geometry(const Geometry& g, const TileGrid& tp) {
+Geometry lucem::tile_geometry(const Geometry& g, const TileGrid& tp) {
     Geometry gt = g;
     gt.L[0] = 1;
     gt.L[1] = tp.L0;
@@ -201,35 +238,10 @@ lucem_status lucem::rl_tiled_launch(lucem_client* c, const Geometry& g, const lu
     if ((st = run_otf(c, gt, b.rev, b.pa, nullptr, b.otf1, tws))) return st;
     if ((st = run_otf(c, gt, b.h, b.pa, nullptr, b.otf2, tws))) return st;
 
-    TileIter d{};
-    d.n0 = g.n[1];
-    d.n1 = g.n[2];
-    d.T0 = tp.T0;
-    d.T1 = tp.T1;
-    d.left0 = g.k[1] / 2;
-    d.left1 = g.k[2] / 2;
-    d.k0 = g.k[1];
-    d.k1 = g.k[2];
-    d.L0 = tp.L0;
-    d.L1 = tp.L1;
-    d.tiles1 = tp.tiles1;
-    d.mode = g.mode;
-    d.pitch = gt.pitch;
-    d.clip = s.clip ? 1 : 0;
-    {
-        const char* v = std::getenv("LUCEM_TILE_GENERIC");  // the run-time-length build, for comparison
-        d.generic = v && std::atoi(v) != 0;
-    }
-    d.eps = rl_eps(s);
-    d.init = 0.5f;  // deconvolve.rs:93
-    d.f0 = make_plan(tp.L0);
-    d.f1 = make_plan(tp.L1);
+    TileIter d = tile_desc(g, tp, s, b.otf1, b.otf2, tw0, tw1);
     d.obs = input;
     d.out = output;
-    d.otf1 = b.otf1;
-    d.otf2 = b.otf2;
-    d.tw0 = tw0;
-    d.tw1 = tw1;
+    d.obs_rs = d.out_rs = g.n[2];
     const int blocks = tp.tiles0 * tp.tiles1;
     const double bytes = canonical_iteration_bytes(g);
     // ping-pong: a tile reads its neighbours' estimates while it writes its own
@@ -264,4 +276,52 @@ lucem_status lucem::rl_run_tiled(lucem_client* c, lucem_workspace* ws, const flo
     if ((st = twiddles_for(c, tp.L0, &tw0))) return st;
     if ((st = twiddles_for(c, tp.L1, &tw1))) return st;
     return rl_tiled_launch(c, g, s, input, psf, output, b, tw0, tw1, tp);
+}
+
+bool lucem::tile_batch_plan(const std::vector<Geometry>& gs, int num_cus, std::vector<TileGrid>* grids,
+                            double* tiled_us, double* whole_us) {
+    static const int kLens[] = {32, 48, 64, 80, 96, 112, 120, 128};
+    // The per-pass batch streams its canonical bytes at about 3.3 TB/s (measured on the 300-tile
+    // slide workload), four launches per iteration.
+    double bytes = 0.0;
+    for (const Geometry& g : gs) bytes += canonical_iteration_bytes(g);
+    *whole_us = 4 * 8.0 + bytes / 3.3e6;
+    double best = 0.0;
+    bool found = false;
+    for (int L : kLens) {
+        if (!tile_fits(L, L) || !tile_len_static(L, L)) continue;
+        std::vector<TileGrid> gr;
+        int64_t items = 0;
+        bool ok = true;
+        for (const Geometry& g : gs) {
+            if (g.ndim != 2 || g.origin[1] != 0 || g.origin[2] != 0) return false;
+            int T[2];
+            for (int a = 0; a < 2 && ok; a++) {
+                const int n = g.n[1 + a], k = g.k[1 + a];
+                const int t = std::min(L - 2 * (k - 1), n);
+                ok = t >= 1 && axis_admits(n, k, t, g.mode);
+                T[a] = t;
+            }
+            if (!ok) break;
+            const TileGrid tg{L, L, T[0], T[1], (g.n[1] + T[0] - 1) / T[0], (g.n[2] + T[1] - 1) / T[1]};
+            items += (int64_t)tg.tiles0 * tg.tiles1;
+            gr.push_back(tg);
+        }
+        if (!ok) continue;
+        // A batch workgroup first reads its array's descriptor; the 300-tile slide (8828 workgroups
+        // of 128 x 128) measured 1.35x the single-array model.
+        const double cost = 1.35 * grid_cost(L, L, items, num_cus);
+        if (!found || cost < best) {
+            best = cost;
+            found = true;
+            *grids = gr;
+        }
+    }
+    *tiled_us = best;
+    return found;
+}
+
+int lucem::tiled_mode_env() {
+    int mode = 0;
+    return tiled_mode_env_impl(&mode) ? (mode != 0 ? 1 : 0) : -1;
 }

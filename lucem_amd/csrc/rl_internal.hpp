@@ -315,14 +315,27 @@ struct TileIter {
     FftPlan f0, f1;      // column (L0) and row (L1) transforms
     const float* est_in;  // estimate (unused when first: the constant init)
     float* est_out;       // next estimate (unless last)
-    const float* obs;     // observed image
-    float* out;           // output (last)
+    const float* obs;     // observed image, rows obs_rs elements apart
+    float* out;           // output (last): pixels own0 <= (y, x) < own1 at out[(y - own0y) * out_rs + x - own0x]
+    int64_t obs_rs, out_rs;
+    int own0y, own0x, own1y, own1x;
     const float2* otf1;   // transfer function of rev (est -> u)
     const float2* otf2;   // transfer function of h (rel -> c)
     const float2* tw0;    // twiddle tables of L0 and L1
     const float2* tw1;
 };
 int launch_tile_iter(const TileIter& d, int blocks, void* stream);
+// One iteration of many arrays in one launch: workgroup i runs tile items[i].y of array
+// jobs[items[i].x]; the launch sets first / last / clip, and `swap` exchanges each job's est_in and
+// est_out (the ping-pong of the iterations). Every job shares the lengths L0 x L1.
+struct TileBatch {
+    const TileIter* jobs;
+    const int2* items;
+    int nitems;
+    int first, last, clip, swap;
+    int L0, L1;
+};
+int launch_tile_batch(const TileBatch& b, void* stream);
 // Whether the tile kernel has a build with these lengths as compile-time constants.
 bool tile_len_static(int L0, int L1);
 // Allows the tiled kernel its dynamic LDS (once per process, at client creation).

@@ -5,8 +5,9 @@
 //   u   = correlate(x, rev)      (:120)      rel = y / safe(u)  (:121-132)
 //   c   = correlate(rel, h)      (:133)      x'  = x * c        (:134-145)
 // with both correlations padded by the boundary rule (filter/kernels/convolve.rs:61-118). For an
-// output tile O = [t0, t0 + T) per axis, c on O needs P(rel) on [t0 - left, t0 + T - 1 + right],
-// hence u on the pad targets of that window, which the host checks lie inside the window itself,
+// output tile O = [t0, t0 + T) per axis (fewer on an array's last tile), c on O needs P(rel) on
+// [t0 - left, t0 + |O| - 1 + right], hence u on the pad targets of that window, which the host
+// checks lie inside the window itself,
 // hence P(x) on [t0 - 2 left, t0 + T - 1 + 2 right]. Each correlation is an overlap-save FFT
 // correlation on an L0 x L1 domain with L >= T + 2 (k - 1) per axis, so no circular wrap reaches
 // the values kept. The whole chain stays in LDS: the iteration reads x with its halo, y on the rel
@@ -152,9 +153,9 @@ __device__ __forceinline__ bool box_inside(int y0, int x0, int h, int w, int n0,
     return y0 >= 0 && x0 >= 0 && y0 + h <= n0 && x0 + w <= n1;
 }
 
+// One iteration of output tile `tile` (row-major over the tiles1 columns of tiles) of array d.
 template <int SL0, int SL1>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void tile_iter_kernel(TileIter d) {
-    extern __shared__ float2 lds[];
+__device__ void tile_body(const TileIter& d, int tile, float2* lds) {
     const Dims<SL0, SL1> dm(d);
     const int B = dm.B, nb = dm.nb;
     TileLds t;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
     t.tw1 = tw1;
     t.tw0 = tw0;
     const int tid = threadIdx.x;
-    const int ty = blockIdx.x / d.tiles1, tx = blockIdx.x - ty * d.tiles1;
+    const int ty = tile / d.tiles1, tx = tile - ty * d.tiles1;
     const int t0y = ty * d.T0, t0x = tx * d.T1;
     // Tile sweeps: thread (pr, c) handles row pairs pr, pr + pstep, ... at column c (columns
     // fastest across lanes, so global rows are read whole and a pair is one 8-byte LDS access).
@@ -223,11 +224,12 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
     multiply(dm, d, sp, d.otf1);
     const float2* U = inverse_2d(dm, d, t, sp);
 
-    // U holds u(t0 - left + p) at p. Q2 = P(rel) on the rel window (my x mx), 0 beyond, into the
-    // other buffer (its last reader, the final inverse pass, is behind a barrier).
+    // U holds u(t0 - left + p) at p. Q2 = P(rel) on the rel window (my x mx: the tile's own outputs,
+    // fewer than T on the last tile of an axis, and the PSF's reach), 0 beyond, into the other
+    // buffer (its last reader, the final inverse pass, is behind a barrier).
     float2* Q = t.other(U);
     {
-        const int my = d.T0 + d.k0 - 1, mx = d.T1 + d.k1 - 1;
+        const int my = min(d.T0, d.n0 - t0y) + d.k0 - 1, mx = min(d.T1, d.n1 - t0x) + d.k1 - 1;
         const int qy0 = t0y - d.left0, qx0 = t0x - d.left1;
         const bool inside = box_inside(qy0, qx0, my, mx, d.n0, d.n1);
         const float* Uf = reinterpret_cast<const float*>(U);
@@ -240,9 +242,9 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
                 const int b = pr0 + s * pstep;
                 v[s] = make_float2(0.f, 0.f);
                 if (active && b < nb && c < mx) {
-                    const float* p = d.obs + (int64_t)(qy0 + 2 * b) * d.n1 + qx0 + c;
+                    const float* p = d.obs + (int64_t)(qy0 + 2 * b) * d.obs_rs + qx0 + c;
                     if (2 * b < my) v[s].x = p[0];
-                    if (2 * b + 1 < my) v[s].y = p[d.n1];
+                    if (2 * b + 1 < my) v[s].y = p[d.obs_rs];
                 }
             }
 #pragma unroll
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
                         // bound keeps every LDS read inside the tile regardless
                         if (sy >= 0 && sx >= 0 && (unsigned)(sy - qy0) < (unsigned)my &&
                             (unsigned)(sx - qx0) < (unsigned)mx) {
-                            const float y = d.obs[(int64_t)sy * d.n1 + sx];
+                            const float y = d.obs[(int64_t)sy * d.obs_rs + sx];
                             if (q == 0) v[s].x = y; else v[s].y = y;
                             const int ry = sy - qy0;
                             at[2 * s + q] = 2 * ((sx - qx0) * B + (ry >> 1)) + (ry & 1);
@@ -308,7 +310,6 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
     {
         const int h = min(d.T0, d.n0 - t0y), w = min(d.T1, d.n1 - t0x);
         float2 x[kG];
-        float* dst = d.last ? d.out : d.est_out;
 #pragma unroll
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
@@ -329,12 +330,42 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void t
                     xa = fminf(fmaxf(xa, -1.f), 1.f);
                     xb = fminf(fmaxf(xb, -1.f), 1.f);
                 }
-                float* p = dst + (int64_t)(t0y + 2 * b) * d.n1 + t0x + c;
-                p[0] = xa;
-                if (2 * b + 1 < h) p[d.n1] = xb;
+                const int gy = t0y + 2 * b, gx = t0x + c;
+                if (!d.last) {
+                    float* p = d.est_out + (int64_t)gy * d.n1 + gx;
+                    p[0] = xa;
+                    if (2 * b + 1 < h) p[d.n1] = xb;
+                } else if (gx >= d.own0x && gx < d.own1x) {
+                    // the output window (the whole array unless the caller stores a part of it)
+                    float* p = d.out + (int64_t)(gy - d.own0y) * d.out_rs + (gx - d.own0x);
+                    if (gy >= d.own0y && gy < d.own1y) p[0] = xa;
+                    if (2 * b + 1 < h && gy + 1 >= d.own0y && gy + 1 < d.own1y) p[d.out_rs] = xb;
+                }
             }
         }
     }
+}
+
+template <int SL0, int SL1>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void tile_iter_kernel(TileIter d) {
+    extern __shared__ float2 lds[];
+    tile_body<SL0, SL1>(d, blockIdx.x, lds);
+}
+
+template <int SL0, int SL1>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(kW))) void tile_batch_kernel(TileBatch bt) {
+    extern __shared__ float2 lds[];
+    const int2 item = bt.items[blockIdx.x];
+    TileIter d = bt.jobs[__builtin_amdgcn_readfirstlane(item.x)];  // a private copy
+    d.first = bt.first;
+    d.last = bt.last;
+    d.clip = bt.clip;
+    if (bt.swap) {
+        float* t = const_cast<float*>(d.est_in);
+        d.est_in = d.est_out;
+        d.est_out = t;
+    }
+    tile_body<SL0, SL1>(d, __builtin_amdgcn_readfirstlane(item.y), lds);
 }
 
 // The square domains compiled with their lengths as constants.
@@ -364,6 +395,15 @@ int allow_tile_lds() {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     LUCEM_TILE_STATIC(LUCEM_CASE)
 #undef LUCEM_CASE
+    if (!err)
+        err = (int)hipFuncSetAttribute(reinterpret_cast<const void*>(tile_batch_kernel<0, 0>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+#define LUCEM_CASE(n)                                                                          \
+    if (!err)                                                                                  \
+        err = (int)hipFuncSetAttribute(reinterpret_cast<const void*>(tile_batch_kernel<n, n>), \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    LUCEM_TILE_STATIC(LUCEM_CASE)
+#undef LUCEM_CASE
     return err;
 }
 
@@ -382,6 +422,21 @@ int launch_tile_iter(const TileIter& d, int blocks, void* stream) {
     LUCEM_TILE_STATIC(LUCEM_CASE)
 #undef LUCEM_CASE
     return (int)hipErrorInvalidValue;
+}
+
+int launch_tile_batch(const TileBatch& b, void* stream) {
+    if (b.nitems <= 0) return 0;
+    const size_t lds = tile_iter_lds_bytes(b.L0, b.L1);
+    const hipStream_t s = (hipStream_t)stream;
+#define LUCEM_CASE(n)                                                                             \
+    if (b.L0 == n && b.L1 == n) {                                                                 \
+        hipLaunchKernelGGL((tile_batch_kernel<n, n>), dim3(b.nitems), dim3(kT), lds, s, b); \
+        return (int)hipGetLastError();                                                            \
+    }
+    LUCEM_TILE_STATIC(LUCEM_CASE)
+#undef LUCEM_CASE
+    hipLaunchKernelGGL((tile_batch_kernel<0, 0>), dim3(b.nitems), dim3(kT), lds, s, b);
+    return (int)hipGetLastError();
 }
 
 }  // namespace lucem

@@ -155,3 +155,99 @@ def test_constant_length_build_equals_run_time_build(gpu, tiled_mode, monkeypatc
     e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     print(f"{shape} {kshape}: constant vs run-time lengths rel-L2 {e:.2e}")
     assert e < 1e-6
+
+
+def _tiled_batch_launches(num_iter, npsf):
+    """Launches of a tiled batch: PSF preparation, two transfer-function builds of two kernels per
+    PSF, one launch per iteration (the per-pass batch has four per iteration)."""
+    return 1 + 4 * npsf + num_iter
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_tiled_batch_matches_oracle(gpu, oracle, tiled_mode, mode):
+    """Uneven 2-D tiles with three PSFs in one tiled batch (one launch per iteration over every
+    output tile of every array) against the FP32 oracle, and against the plain call of each tile."""
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import RLBatch
+
+    tiled_mode("1")
+    rng = np.random.default_rng(40 + mode)
+    shapes = [(64, 64), (17, 93), (128, 40), (5, 7), (150, 130), (200, 150), (33, 1), (1, 29)]
+    psfs_np = [rng.uniform(0.1, 1.0, k).astype(np.float32) for k in [(5, 5), (7, 3), (11, 11)]]
+    index = [0, 1, 2, 0, 2, 1, 0, 1]
+    ys_np = [rng.uniform(50, 150, s).astype(np.float32) for s in shapes]
+    ys = [torch.from_numpy(y).cuda() for y in ys_np]
+    psfs = [torch.from_numpy(p).cuda() for p in psfs_np]
+    s = LA.RichardsonLucy(num_iter=6, clip=False, mode=mode)
+    b = RLBatch(shapes, [p.shape for p in psfs_np], index, s)
+    assert b.launches == _tiled_batch_launches(6, 3)
+    outs = [torch.empty_like(y) for y in ys]
+    b.execute(ys, outs, psfs)
+    torch.cuda.synchronize()
+    for y, o, i in zip(ys_np, outs, index):
+        got = o.cpu().numpy()
+        _, ref = oracle.richardson_lucy(y, psfs_np[i], 6, clip=False, mode=mode)
+        assert oracle.rel_l2(got, ref) < RTOL_L2
+        plain, _ = _run(gpu, y, psfs_np[i], 6, mode)
+        assert float(np.linalg.norm(got - plain) / np.linalg.norm(plain)) < 2e-6
+    b.close()
+
+
+def test_tiled_batch_layout_windows(gpu, tiled_mode):
+    """Tiles given as strided views into one image, each storing a window into one result, through
+    the tiled batch: the plain per-tile results inside the windows, the rest of the result untouched."""
+    import torch
+
+    import lucem_amd as LA
+    from lucem_amd.batched import RLBatch, TileLayout
+
+    tiled_mode("1")
+    rng = np.random.default_rng(44)
+    H, W = 160, 190
+    img = torch.from_numpy(rng.uniform(50, 150, (H, W)).astype(np.float32)).cuda()
+    res = torch.full((H, W), -7.0, device="cuda")
+    # (read origin, read size, window origin in the tile, window size): y x
+    tiles = [((0, 0), (90, 110), (4, 6), (70, 90)), ((30, 100), (130, 90), (10, 5), (100, 60)),
+             ((100, 0), (60, 95), (0, 0), (0, 0))]
+    psf = torch.from_numpy(rng.uniform(0.1, 1.0, (7, 5)).astype(np.float32)).cuda()
+    s = LA.RichardsonLucy(num_iter=5, clip=False)
+    plan = RLBatch([size for _, size, _, _ in tiles], [tuple(psf.shape)], [0] * len(tiles), s)
+    assert plan.launches == _tiled_batch_launches(5, 1)
+    ins, outs, lays = [], [], []
+    for ro, size, wo, ws in tiles:
+        ins.append(img.data_ptr() + 4 * (ro[0] * W + ro[1]))
+        outs.append(res.data_ptr() + 4 * ((ro[0] + wo[0]) * W + ro[1] + wo[1]))
+        lays.append(TileLayout(W, 0, W, 0, (0,) + wo, (0,) + ws))
+    plan.execute_layout(ins, outs, lays, [psf])
+    torch.cuda.synchronize()
+    covered = torch.zeros((H, W), dtype=torch.bool, device="cuda")
+    for ro, size, wo, ws in tiles:
+        view = img[ro[0]:ro[0] + size[0], ro[1]:ro[1] + size[1]].contiguous()
+        ref, _ = _run(gpu, view.cpu().numpy(), psf.cpu().numpy(), 5)
+        wsz = [w if w else size[a] - wo[a] for a, w in enumerate(ws)]
+        sl = (slice(ro[0] + wo[0], ro[0] + wo[0] + wsz[0]), slice(ro[1] + wo[1], ro[1] + wo[1] + wsz[1]))
+        want = ref[wo[0]:wo[0] + wsz[0], wo[1]:wo[1] + wsz[1]]
+        got = res[sl].cpu().numpy()
+        assert float(np.linalg.norm(got - want) / np.linalg.norm(want)) < 2e-6
+        covered[sl] = True
+    assert bool((res[~covered] == -7.0).all())
+    plan.close()
+
+
+def test_large_batch_keeps_the_per_pass_form(gpu, tiled_mode):
+    """With much work in all (here 60 full 512 x 512 tiles) the per-pass batch is modelled faster
+    and is built; LUCEM_RL_TILED=1 forces the tiled one."""
+    from lucem_amd.batched import RLBatch
+    from lucem_amd.restore import RichardsonLucy
+
+    tiled_mode(None)
+    s = RichardsonLucy(num_iter=3, clip=False)
+    b = RLBatch([(512, 512)] * 60, [(15, 15)], [0] * 60, s)
+    assert b.launches != _tiled_batch_launches(3, 1)
+    b.close()
+    tiled_mode("1")
+    b = RLBatch([(512, 512)] * 60, [(15, 15)], [0] * 60, s)
+    assert b.launches == _tiled_batch_launches(3, 1)
+    b.close()
