@@ -95,7 +95,7 @@ bool tiled_mode_env_impl(int* mode) {
 
 // Modelled time of one iteration on a grid: workgroups run in rounds of (CUs x workgroups per CU);
 // a round costs a fixed latency (launch, the global reads and writes, the phase barriers) plus the
-// tile's transform work, and two workgroups sharing a CU each run about 1.6x slower. Measured on
+// tile's transform work, and two workgroups sharing a CU each run about 1.85x slower. Measured on
 // MI355X (30-iteration calls): with constant lengths a 64 x 64 tile takes 16.5 us and a 112 x 112
 // one 31 us (9.5 us + 1.7 ns per element-pass); the run-time-length build takes 27 us on 32 x 32
 // and 46 us on 112 x 112 (26 us + 1.6 ns per element-pass).
@@ -107,7 +107,7 @@ static double grid_cost(int L0, int L1, int64_t tiles, int num_cus) {
     const double elem_passes = (double)L0 * L1 * (make_plan(L0).npass + make_plan(L1).npass + 2) / 6.0;
     const double round_us = tile_len_static(L0, L1) ? 9.5 + 1.72e-3 * elem_passes : 26.0 + 1.6e-3 * elem_passes;
     const bool shared = tiles > (int64_t)num_cus && per_cu == 2;
-    return (double)rounds * round_us * (shared ? 1.6 : 1.0);
+    return (double)rounds * round_us * (shared ? 1.85 : 1.0);
 }
 
 bool lucem::tile_plan(const Geometry& g, int num_cus, TileGrid* out) {
@@ -308,9 +308,9 @@ bool lucem::tile_batch_plan(const std::vector<Geometry>& gs, int num_cus, std::v
             gr.push_back(tg);
         }
         if (!ok) continue;
-        // A batch workgroup first reads its array's descriptor; the 300-tile slide (8828 workgroups
-        // of 128 x 128) measured 1.35x the single-array model.
-        const double cost = 1.35 * grid_cost(L, L, items, num_cus);
+        // The 300-tile slide (8828 workgroups of 128 x 128, 34 rounds) measured 0.9x the
+        // single-array model: the dispatcher starts a CU's next workgroup while others still run.
+        const double cost = 0.9 * grid_cost(L, L, items, num_cus);
         if (!found || cost < best) {
             best = cost;
             found = true;

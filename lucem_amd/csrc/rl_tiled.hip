@@ -155,7 +155,7 @@ __device__ __forceinline__ bool box_inside(int y0, int x0, int h, int w, int n0,
 
 // One iteration of output tile `tile` (row-major over the tiles1 columns of tiles) of array d.
 template <int SL0, int SL1>
-__device__ void tile_body(const TileIter& d, int tile, float2* lds) {
+__device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* lds) {
     const Dims<SL0, SL1> dm(d);
     const int B = dm.B, nb = dm.nb;
     TileLds t;

@@ -236,14 +236,18 @@ def test_tiled_batch_layout_windows(gpu, tiled_mode):
     plan.close()
 
 
-def test_large_batch_keeps_the_per_pass_form(gpu, tiled_mode):
-    """With much work in all (here 60 full 512 x 512 tiles) the per-pass batch is modelled faster
-    and is built; LUCEM_RL_TILED=1 forces the tiled one."""
+def test_batch_form_choice(gpu, tiled_mode):
+    """Wrap across several tiles admits no tile grid: the per-pass batch is built. LUCEM_RL_TILED=0
+    forces the per-pass batch, =1 the tiled one where it is admissible."""
     from lucem_amd.batched import RLBatch
     from lucem_amd.restore import RichardsonLucy
 
     tiled_mode(None)
+    b = RLBatch([(300, 280)] * 4, [(9, 9)], [0] * 4, RichardsonLucy(num_iter=3, clip=False, mode=4))
+    assert b.launches != _tiled_batch_launches(3, 1)
+    b.close()
     s = RichardsonLucy(num_iter=3, clip=False)
+    tiled_mode("0")
     b = RLBatch([(512, 512)] * 60, [(15, 15)], [0] * 60, s)
     assert b.launches != _tiled_batch_launches(3, 1)
     b.close()
