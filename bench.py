@@ -171,12 +171,19 @@ def batched_leg(steps: int) -> dict:
     alg = sum(minimal_domain_bytes(tuple(y.shape), tuple(psfs[i].shape)) for y, i in zip(ys, ts.psf_index))
     gbps = alg * ts.num_iter / tb / 1e9
     # HBM bytes the counters measured per launch of each pass (profiles/traffic_batched.json) times
-    # the launches of one call: 2 correlations and 1 ratio and 1 update pass per iteration
+    # the launches of one call (the tiled batch: one launch per iteration; the per-pass batch: 2
+    # correlations, 1 ratio and 1 update pass per iteration)
+    tiled = b.launches == 1 + 4 * len(psfs) + ts.num_iter
     counted = None
-    per = {k: load_traffic("batched", k) for k in ("line_fused_y", "row_ratio", "row_update")}
-    if all(v is not None for v in per.values()):
-        counted = ts.num_iter * (2 * per["line_fused_y"] + per["row_ratio"] + per["row_update"])
+    if tiled:
+        per_launch = load_traffic("batched", "tile_batch")
+        counted = ts.num_iter * per_launch if per_launch is not None else None
+    else:
+        per = {k: load_traffic("batched", k) for k in ("line_fused_y", "row_ratio", "row_update")}
+        if all(v is not None for v in per.values()):
+            counted = ts.num_iter * (2 * per["line_fused_y"] + per["row_ratio"] + per["row_update"])
     return {
+        "path": "tiled: one launch per iteration over the output tiles of every tile" if tiled else "per-pass",
         "counter_bytes_per_call": counted,
         "achieved_GBps_counters": round(counted / tb / 1e9, 1) if counted else None,
         "frac_counters": round(counted / tb / 1e9 / HBM_PEAK_GBPS, 4) if counted else None,

@@ -21,7 +21,8 @@ Two helpers for the ``traffic`` figure of ``bench.py``:
 
 rocprofv3 names kernels by template (``col_pass_kernel<256, 16, 1>``); the op id is the last
 template argument, and each RL pass of a workload is one (kernel family, op) pair. The tiled 2-D
-iteration is one kernel family (``tile_iter_kernel<L0, L1>``) whatever its lengths.
+iteration is one kernel family (``tile_iter_kernel<L0, L1>``, and ``tile_batch_kernel<L0, L1>`` for
+the batched form) whatever its lengths.
 """
 
 from __future__ import annotations
@@ -51,6 +52,7 @@ PASS_KERNELS = {
         "row_update": ("row_pass_kernel", 3),
     },
     "batched": {
+        "tile_batch": ("tile_batch_kernel", None),
         "line_fused_y": ("col_batch_kernel", 1),
         "row_ratio": ("row_batch_kernel", 2),
         "row_update": ("row_batch_kernel", 3),
@@ -58,7 +60,7 @@ PASS_KERNELS = {
 }
 
 _NAME = re.compile(r"lucem::(\w+)<(\d+), (\d+), (\d+)>")
-_TILE = re.compile(r"(tile_iter_kernel)<\d+, \d+>")
+_TILE = re.compile(r"(tile_iter_kernel|tile_batch_kernel)<\d+, \d+>")
 CAL_BYTES = 1 << 30
 
 
@@ -169,8 +171,8 @@ def _per_pass(d: Path, counter: str, workload: str):
         m = _NAME.search(name)
         if m:
             groups.setdefault((m.group(1), int(m.group(4))), []).append(v)
-        elif _TILE.search(name):
-            groups.setdefault(("tile_iter_kernel", None), []).append(v)
+        elif (t := _TILE.search(name)):
+            groups.setdefault((t.group(1), None), []).append(v)
     out = {}
     for pname, key in PASS_KERNELS[workload].items():
         if key in groups:
