@@ -123,6 +123,35 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def device_info() -> dict:
+    """The device's name and CU count, and its memory clock and bus width as the HIP runtime reports
+    them (``hipDeviceGetAttribute``; attribute ids 60 and 59 of ROCm 7.2's ``hipDeviceAttribute_t``),
+    with the peak they imply at two transfers per clock, beside the published 8 TB/s the roofline
+    uses."""
+    import ctypes
+
+    import torch
+
+    p = torch.cuda.get_device_properties(0)
+    out = {"name": p.name, "gcn_arch": getattr(p, "gcnArchName", ""), "cus": p.multi_processor_count,
+           "hbm_bytes": p.total_memory}
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        v = ctypes.c_int()
+        clk = bw = None
+        if hip.hipDeviceGetAttribute(ctypes.byref(v), 60, 0) == 0:
+            clk = v.value
+        if hip.hipDeviceGetAttribute(ctypes.byref(v), 59, 0) == 0:
+            bw = v.value
+        out.update({"memory_clock_khz": clk, "memory_bus_bits": bw})
+        if clk and bw:
+            out["implied_peak_GBps"] = round(2 * clk * 1e3 * bw / 8 / 1e9, 1)
+    except OSError:
+        pass
+    out["peak_used_GBps"] = HBM_PEAK_GBPS
+    return out
+
+
 def _timeit(fn, reps: int) -> float:
     import torch
 
@@ -632,6 +661,7 @@ def main() -> int:
         "iteration": head["iteration"],
         "kernels": head["kernels"],
         "cpu_baseline": head["cpu_baseline"],
+        "device": device_info(),
     }
     for key in ("path", "whole_array_ms_per_step"):
         if key in head:

@@ -104,12 +104,13 @@ __device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, 
     return cols_fft<false>(dm, d, t, sc);
 }
 
-// sc *= otf ([L0][pitch] in global memory; conj(FFT(w)) * scale) on the data columns, loads
-// issued before the LDS reads.
+// The transfer-function values of this thread's spectrum elements ([L0][pitch] in global memory;
+// conj(FFT(w)) * scale; zero on the padding column). Issued a transform ahead of their use, so
+// that their latency hides behind it.
 template <class D>
-__device__ void multiply(const D& dm, const TileIter& d, float2* sc, const float2* __restrict__ otf) {
+__device__ __forceinline__ void load_otf(const D& dm, const TileIter& d, const float2* __restrict__ otf,
+                                         float2 (&w)[kE]) {
     const int n = dm.L0 * dm.C;
-    float2 w[kE];
 #pragma unroll
     for (int s = 0; s < kE; s++) {
         const int e = threadIdx.x + s * kT;
@@ -119,6 +120,12 @@ __device__ void multiply(const D& dm, const TileIter& d, float2* sc, const float
             if (k < dm.nc) w[s] = otf[(int64_t)r * d.pitch + k];
         }
     }
+}
+
+// sc *= w on the spectrum elements (column lines).
+template <class D>
+__device__ __forceinline__ void apply_otf(const D& dm, float2* sc, const float2 (&w)[kE]) {
+    const int n = dm.L0 * dm.C;
 #pragma unroll
     for (int s = 0; s < kE; s++) {
         const int e = threadIdx.x + s * kT;
@@ -175,14 +182,8 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
     const int c = tid - pr0 * dm.L1;
     const bool active = pr0 < pstep;
 
-    for (int i = tid; i < dm.L1; i += kT) tw1[i] = d.tw1[d.f1.tw + i];
-    for (int i = tid; i < dm.L0; i += kT) tw0[i] = d.tw0[d.f0.tw + i];
-    // both buffers start at zero, so that padding lines hold finite values (their transforms are
-    // never read)
-    for (int i = tid; i < 2 * dm.nbuf; i += kT) lds[i] = make_float2(0.f, 0.f);
-    __syncthreads();
-
-    // Q1: the estimate with the pad rule, two halos deep, into X
+    // Q1: the estimate with the pad rule, two halos deep, into X; the twiddle tables are staged
+    // while its loads are in flight
     {
         const int gy0 = t0y - 2 * d.left0, gx0 = t0x - 2 * d.left1;
         const bool inside = box_inside(gy0, gx0, dm.L0, dm.L1, d.n0, d.n1);
@@ -213,76 +214,94 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
                 }
             }
         }
+        for (int i = tid; i < dm.L1; i += kT) tw1[i] = d.tw1[d.f1.tw + i];
+        for (int i = tid; i < dm.L0; i += kT) tw0[i] = d.tw0[d.f0.tw + i];
+        // Every position a transform reads holds a finite value: X's padding row line (the Q1
+        // stores fill the other lines), and in both buffers the part of the column layout beyond
+        // the row layout, which no row pass writes. Neither overlaps a Q1 store. The padding
+        // lines' transforms are never read.
+        if (B > nb)
+            for (int i = tid; i < dm.L1; i += kT) t.X[i * B + nb] = make_float2(0.f, 0.f);
+        for (int i = dm.L1 * B + tid; i < dm.L0 * dm.C; i += kT) {
+            t.X[i] = make_float2(0.f, 0.f);
+            t.Y[i] = make_float2(0.f, 0.f);
+        }
 #pragma unroll
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
             if (active && b < nb) t.X[c * B + b] = v[s];
         }
     }
+    // the first transfer function's values, loaded while the tile is transformed
+    float2 w[kE];
+    load_otf(dm, d, d.otf1, w);
     __syncthreads();
     float2* sp = forward_2d(dm, d, t, t.X);
-    multiply(dm, d, sp, d.otf1);
-    const float2* U = inverse_2d(dm, d, t, sp);
+    apply_otf(dm, sp, w);
 
-    // U holds u(t0 - left + p) at p. Q2 = P(rel) on the rel window (my x mx: the tile's own outputs,
-    // fewer than T on the last tile of an axis, and the PSF's reach), 0 beyond, into the other
-    // buffer (its last reader, the final inverse pass, is behind a barrier).
-    float2* Q = t.other(U);
-    {
-        const int my = min(d.T0, d.n0 - t0y) + d.k0 - 1, mx = min(d.T1, d.n1 - t0x) + d.k1 - 1;
-        const int qy0 = t0y - d.left0, qx0 = t0x - d.left1;
-        const bool inside = box_inside(qy0, qx0, my, mx, d.n0, d.n1);
-        const float* Uf = reinterpret_cast<const float*>(U);
-        float2 v[kG];
-        if (inside) {
-            // every target is its own position: y and u of the pair, rel in place
-            float2 u[kG];
+    // Q2 = P(rel) on the rel window (my x mx: the tile's own outputs, fewer than T on the last tile
+    // of an axis, and the PSF's reach), 0 beyond. The observed values and the second transfer
+    // function are loaded before the inverse transform that gives u, whose latency they hide behind.
+    const int my = min(d.T0, d.n0 - t0y) + d.k0 - 1, mx = min(d.T1, d.n1 - t0x) + d.k1 - 1;
+    const int qy0 = t0y - d.left0, qx0 = t0x - d.left1;
+    const bool q_inside = box_inside(qy0, qx0, my, mx, d.n0, d.n1);
+    float2 v[kG];
+    int at[2 * kG];  // border tiles: the float offset in U of each value's pad target, or -1
+    if (q_inside) {
+        // every target is its own position: y of the pair, rel in place
 #pragma unroll
-            for (int s = 0; s < kG; s++) {
-                const int b = pr0 + s * pstep;
-                v[s] = make_float2(0.f, 0.f);
-                if (active && b < nb && c < mx) {
-                    const float* p = d.obs + (int64_t)(qy0 + 2 * b) * d.obs_rs + qx0 + c;
-                    if (2 * b < my) v[s].x = p[0];
-                    if (2 * b + 1 < my) v[s].y = p[d.obs_rs];
-                }
+        for (int s = 0; s < kG; s++) {
+            const int b = pr0 + s * pstep;
+            v[s] = make_float2(0.f, 0.f);
+            if (active && b < nb && c < mx) {
+                const float* p = d.obs + (int64_t)(qy0 + 2 * b) * d.obs_rs + qx0 + c;
+                if (2 * b < my) v[s].x = p[0];
+                if (2 * b + 1 < my) v[s].y = p[d.obs_rs];
             }
+        }
+    } else {
 #pragma unroll
-            for (int s = 0; s < kG; s++) {
-                const int b = pr0 + s * pstep;
-                u[s] = make_float2(1.f, 1.f);
-                if (active && b < nb && c < mx) u[s] = U[c * B + b];
-            }
+        for (int s = 0; s < kG; s++) {
+            const int b = pr0 + s * pstep;
+            v[s] = make_float2(0.f, 0.f);
+            at[2 * s] = at[2 * s + 1] = -1;
+            if (active && b < nb && c < mx) {
+                const int sx = pad_index(qx0 + c, d.n1, d.mode);
 #pragma unroll
-            for (int s = 0; s < kG; s++) {
-                v[s].x = div_fast(v[s].x, fabsf(u[s].x) < d.eps ? d.eps : u[s].x);
-                v[s].y = div_fast(v[s].y, fabsf(u[s].y) < d.eps ? d.eps : u[s].y);
-            }
-        } else {
-            int at[2 * kG];
-#pragma unroll
-            for (int s = 0; s < kG; s++) {
-                const int b = pr0 + s * pstep;
-                v[s] = make_float2(0.f, 0.f);
-                at[2 * s] = at[2 * s + 1] = -1;
-                if (active && b < nb && c < mx) {
-                    const int sx = pad_index(qx0 + c, d.n1, d.mode);
-#pragma unroll
-                    for (int q = 0; q < 2; q++) {
-                        const int r = 2 * b + q;
-                        const int sy = r < my ? pad_index(qy0 + r, d.n0, d.mode) : -1;
-                        // the host admits only geometries whose targets lie in the window; the
-                        // bound keeps every LDS read inside the tile regardless
-                        if (sy >= 0 && sx >= 0 && (unsigned)(sy - qy0) < (unsigned)my &&
-                            (unsigned)(sx - qx0) < (unsigned)mx) {
-                            const float y = d.obs[(int64_t)sy * d.obs_rs + sx];
-                            if (q == 0) v[s].x = y; else v[s].y = y;
-                            const int ry = sy - qy0;
-                            at[2 * s + q] = 2 * ((sx - qx0) * B + (ry >> 1)) + (ry & 1);
-                        }
+                for (int q = 0; q < 2; q++) {
+                    const int r = 2 * b + q;
+                    const int sy = r < my ? pad_index(qy0 + r, d.n0, d.mode) : -1;
+                    // the host admits only geometries whose targets lie in the window; the
+                    // bound keeps every LDS read inside the tile regardless
+                    if (sy >= 0 && sx >= 0 && (unsigned)(sy - qy0) < (unsigned)my &&
+                        (unsigned)(sx - qx0) < (unsigned)mx) {
+                        const float y = d.obs[(int64_t)sy * d.obs_rs + sx];
+                        if (q == 0) v[s].x = y; else v[s].y = y;
+                        const int ry = sy - qy0;
+                        at[2 * s + q] = 2 * ((sx - qx0) * B + (ry >> 1)) + (ry & 1);
                     }
                 }
             }
+        }
+    }
+    load_otf(dm, d, d.otf2, w);
+    const float2* U = inverse_2d(dm, d, t, sp);
+
+    // U holds u(t0 - left + p) at p; Q2 goes into the other buffer (its last reader, the final
+    // inverse pass, is behind a barrier).
+    float2* Q = t.other(U);
+    {
+        if (q_inside) {
+#pragma unroll
+            for (int s = 0; s < kG; s++) {
+                const int b = pr0 + s * pstep;
+                float2 u = make_float2(1.f, 1.f);
+                if (active && b < nb && c < mx) u = U[c * B + b];
+                v[s].x = div_fast(v[s].x, fabsf(u.x) < d.eps ? d.eps : u.x);
+                v[s].y = div_fast(v[s].y, fabsf(u.y) < d.eps ? d.eps : u.y);
+            }
+        } else {
+            const float* Uf = reinterpret_cast<const float*>(U);
 #pragma unroll
             for (int s = 0; s < kG; s++) {
                 if (at[2 * s] >= 0) {
@@ -303,27 +322,29 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
     }
     __syncthreads();
     sp = forward_2d(dm, d, t, Q);
-    multiply(dm, d, sp, d.otf2);
+    apply_otf(dm, sp, w);
+
+    // the estimate on the tile's own pixels, loaded before the last inverse transform
+    const int h = min(d.T0, d.n0 - t0y), wd = min(d.T1, d.n1 - t0x);
+    float2 x[kG];
+#pragma unroll
+    for (int s = 0; s < kG; s++) {
+        const int b = pr0 + s * pstep;
+        x[s] = make_float2(d.init, d.init);
+        if (!d.first && active && 2 * b < h && c < wd) {
+            const float* p = d.est_in + (int64_t)(t0y + 2 * b) * d.n1 + t0x + c;
+            x[s].x = p[0];
+            if (2 * b + 1 < h) x[s].y = p[d.n1];
+        }
+    }
     const float2* Cv = inverse_2d(dm, d, t, sp);
 
     // Cv holds c(t0 + p) at p: the update on the tile's own pixels
     {
-        const int h = min(d.T0, d.n0 - t0y), w = min(d.T1, d.n1 - t0x);
-        float2 x[kG];
 #pragma unroll
         for (int s = 0; s < kG; s++) {
             const int b = pr0 + s * pstep;
-            x[s] = make_float2(d.init, d.init);
-            if (!d.first && active && 2 * b < h && c < w) {
-                const float* p = d.est_in + (int64_t)(t0y + 2 * b) * d.n1 + t0x + c;
-                x[s].x = p[0];
-                if (2 * b + 1 < h) x[s].y = p[d.n1];
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < kG; s++) {
-            const int b = pr0 + s * pstep;
-            if (active && 2 * b < h && c < w) {
+            if (active && 2 * b < h && c < wd) {
                 const float2 cv = Cv[c * B + b];
                 float xa = x[s].x * cv.x, xb = x[s].y * cv.y;
                 if (d.last && d.clip) {
