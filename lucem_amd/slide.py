@@ -18,8 +18,10 @@ Plans are kept for group compositions that repeat (the interior blocks of a regu
 The reader (``Pyramid``) covers the stores this driver and the reference's writer make and the
 common Zarr v2 layouts: ``.zarray`` arrays of unsigned, signed or floating samples of either
 byte order, ``.`` or ``/`` chunk keys, fill values for missing chunks, axes ``[c,] [z,] y, x``, and
-chunks stored raw or with zlib or gzip (crates/lucem-io/src/pyramid.rs:1001-1086). Blosc and zstd
-chunks need decoders this package does not carry; such stores raise a clear error.
+chunks stored raw, with zlib, gzip, zstd or lz4, or in Blosc frames of LZ4, zstd, zlib or snappy
+streams with or without byte shuffle (crates/lucem-io/src/pyramid.rs:1001-1086,
+crates/lucem-io/src/ome_zarr.rs:146-343). LZ4, zstd and snappy are decoded by pyarrow's bundled
+codecs. BloscLZ streams and bit shuffle raise a clear error.
 """
 
 from __future__ import annotations
@@ -73,6 +75,130 @@ def _to_micrometres(v: float, unit: str | None) -> float:
 
 
 # --------------------------------------------------------------------------------------------
+# Chunk codecs
+
+
+def _arrow_decompress(codec: str, data: bytes, size: int) -> bytes:
+    """``size`` bytes from an LZ4 block (``lz4_raw``), a zstd frame or a raw snappy block, through
+    pyarrow's bundled codecs."""
+    try:
+        import pyarrow as pa
+    except ImportError as e:  # pragma: no cover - the image carries pyarrow
+        raise IoError(f"{codec} chunks need pyarrow's decoders") from e
+    try:
+        out = pa.decompress(data, decompressed_size=size, codec=codec, asbytes=True)
+    except Exception as e:
+        raise IoError(f"{codec}: {e}") from e
+    if len(out) != size:
+        raise IoError(f"{codec}: {len(out)} bytes decoded, {size} expected")
+    return out
+
+
+# Blosc's compressor codes (bits 5-7 of its flags) and the stream decoders used for them.
+_BLOSC_STREAMS = {1: "lz4_raw", 2: "snappy", 3: "zlib", 4: "zstd"}
+
+
+def blosc_decompress(raw: bytes, expected: int) -> bytes:
+    """A Blosc frame (format version 1, as the reference reads it: crates/lucem-io/src/ome_zarr.rs:
+    237-343) of ``expected`` bytes: a 16-byte header (flags, type size, bytes, block size, frame
+    size), then either the bytes as they are (flag 0x02) or a table of block offsets and per block
+    one stream, or one stream per byte of the type when the block was split (found, as the
+    reference does, from whether the first stream fills the block), each prefixed by its size and
+    stored raw when that size is the stream's length; byte shuffle (flag 0x01) undone per block,
+    the bytes past whole elements left in place. Streams of LZ4, zstd, zlib and snappy decode;
+    BloscLZ streams and bit shuffle raise."""
+    import struct
+
+    if len(raw) < 16:
+        raise IoError("Blosc header shorter than 16 bytes")
+    flags, typesize = raw[2], max(raw[3], 1)
+    nbytes, blocksize, cbytes = struct.unpack_from("<III", raw, 4)
+    if cbytes > len(raw):
+        raise IoError("Blosc frame size exceeds the chunk")
+    if nbytes != expected:
+        raise IoError(f"Blosc frame holds {nbytes} bytes, its array needs {expected}")
+    if flags & 0x02:
+        if len(raw) < 16 + nbytes:
+            raise IoError("Blosc stored frame truncated")
+        return bytes(raw[16:16 + nbytes])
+    if flags & 0x04:
+        raise IoError("Blosc bit shuffle is not supported")
+    codec = _BLOSC_STREAMS.get((flags >> 5) & 7)
+    if codec is None:
+        raise IoError(f"Blosc compressor code {(flags >> 5) & 7} (BloscLZ) cannot be decoded here")
+    if nbytes == 0:
+        return b""
+    if blocksize == 0:
+        raise IoError("Blosc block size 0")
+    nblocks = -(-nbytes // blocksize)
+    if len(raw) < 16 + 4 * nblocks:
+        raise IoError("Blosc block table truncated")
+    starts = struct.unpack_from(f"<{nblocks}I", raw, 16)
+    out = np.empty(nbytes, np.uint8)
+    for b in range(nblocks):
+        off = starts[b]
+        end = starts[b + 1] if b + 1 < nblocks else cbytes
+        bsize = min(blocksize, nbytes - b * blocksize)
+        if off + 4 > len(raw):
+            raise IoError("Blosc stream size truncated")
+        first = struct.unpack_from("<I", raw, off)[0]
+        nstreams = 1 if typesize <= 1 or off + 4 + first == end else typesize
+        if bsize % nstreams:
+            raise IoError("Blosc block does not split into its streams")
+        slen = bsize // nstreams
+        parts = []
+        for _ in range(nstreams):
+            if off + 4 > len(raw):
+                raise IoError("Blosc stream size truncated")
+            csize = struct.unpack_from("<I", raw, off)[0]
+            off += 4
+            if off + csize > len(raw):
+                raise IoError("Blosc stream overruns the chunk")
+            if csize == 0:
+                parts.append(bytes(slen))
+            elif csize == slen:
+                parts.append(raw[off:off + csize])
+            elif codec == "zlib":
+                data = zlib.decompress(raw[off:off + csize])
+                if len(data) != slen:
+                    raise IoError("Blosc zlib stream of the wrong size")
+                parts.append(data)
+            else:
+                parts.append(_arrow_decompress(codec, raw[off:off + csize], slen))
+            off += csize
+        blk = np.frombuffer(b"".join(parts), np.uint8)
+        dst = out[b * blocksize:b * blocksize + bsize]
+        if flags & 0x01 and typesize > 1:
+            ne = bsize // typesize
+            dst[:ne * typesize] = blk[:ne * typesize].reshape(typesize, ne).T.reshape(-1)
+            dst[ne * typesize:] = blk[ne * typesize:]
+        else:
+            dst[:] = blk
+    return out.tobytes()
+
+
+def _decode_chunk(comp: dict | None, raw: bytes, expected: int) -> bytes:
+    """A chunk's bytes under its Zarr v2 compressor: none, zlib, gzip, zstd, lz4 (a 4-byte size,
+    then an LZ4 block) or blosc."""
+    cid = None if comp is None else comp.get("id")
+    if cid is None:
+        return raw
+    if cid == "zlib":
+        return zlib.decompress(raw)
+    if cid == "gzip":
+        return gzip.decompress(raw)
+    if cid == "zstd":
+        return _arrow_decompress("zstd", raw, expected)
+    if cid == "lz4":
+        if len(raw) < 4 or int.from_bytes(raw[:4], "little") != expected:
+            raise IoError("lz4 chunk declares another size than its array needs")
+        return _arrow_decompress("lz4_raw", raw[4:], expected)
+    if cid == "blosc":
+        return blosc_decompress(raw, expected)
+    raise IoError(f"Zarr compressor {cid!r} is not supported")
+
+
+# --------------------------------------------------------------------------------------------
 # Reading
 
 
@@ -122,9 +248,15 @@ class _ZarrArray:
         if self.dtype.kind not in "uifb" or self.dtype.itemsize > 8:
             raise IoError(f"unsupported Zarr dtype {meta['dtype']!r}")
         comp = meta.get("compressor")
-        self.compressor = None if comp is None else str(comp.get("id"))
-        if self.compressor not in (None, "zlib", "gzip"):
-            raise IoError(f"Zarr compressor {self.compressor!r} cannot be decoded here (none, zlib and gzip can)")
+        self.compressor = comp
+        cid = None if comp is None else comp.get("id")
+        if cid not in (None, "zlib", "gzip", "zstd", "lz4", "blosc"):
+            raise IoError(f"Zarr compressor {cid!r} is not supported (none, zlib, gzip, zstd, lz4 and blosc are)")
+        if cid == "blosc":
+            if comp.get("cname", "lz4") not in ("lz4", "zstd", "zlib", "snappy"):
+                raise IoError(f"Blosc with {comp.get('cname')} is not supported (lz4, zstd, zlib and snappy are)")
+            if comp.get("shuffle", 1) == 2:
+                raise IoError("Blosc bit shuffle is not supported")
         if meta.get("filters"):
             raise IoError("Zarr filters are not supported")
         sep = meta.get("dimension_separator") or "."
@@ -166,11 +298,8 @@ class _ZarrArray:
             raw = path.read_bytes()
         except FileNotFoundError:
             return None
-        if self.compressor == "zlib":
-            raw = zlib.decompress(raw)
-        elif self.compressor == "gzip":
-            raw = gzip.decompress(raw)
         n = math.prod(self.chunks)
+        raw = _decode_chunk(self.compressor, raw, n * self.dtype.itemsize)
         if len(raw) != n * self.dtype.itemsize:
             raise IoError(f"{path} holds {len(raw)} bytes, its array needs {n * self.dtype.itemsize}")
         return np.frombuffer(raw, dtype=self.dtype).reshape(self.chunks)

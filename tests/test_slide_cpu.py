@@ -163,6 +163,62 @@ def test_labels_are_sampled_not_averaged(tmp_path):
     np.testing.assert_array_equal(py.read_block(1, 0, (0, 0, 0), (1, 3, 3)), lab[:, ::2, ::2].astype(np.float32))
 
 
+def _blosc_frame(data: bytes, typesize: int, blocksize: int, shuffle: bool = True, code: int = 1) -> bytes:
+    """A Blosc format-1 frame as c-blosc lays one out: header, block offsets, per block the byte
+    planes of the shuffled block as separate streams (one stream for the last, partial block),
+    each stream compressed by the real codec (pyarrow's LZ4 block, zstd, snappy; zlib), stored raw
+    when that does not make it smaller."""
+    import struct
+
+    import pyarrow as pa
+
+    codec = {1: "lz4_raw", 2: "snappy", 3: "zlib", 4: "zstd"}[code]
+
+    def comp(x: bytes) -> bytes:
+        return zlib.compress(x) if codec == "zlib" else pa.compress(x, codec=codec, asbytes=True)
+
+    nblocks = -(-len(data) // blocksize) if data else 0
+    body, starts = b"", []
+    head = 16 + 4 * nblocks
+    for b in range(nblocks):
+        blk = np.frombuffer(data[b * blocksize:(b + 1) * blocksize], np.uint8)
+        n = len(blk)
+        if shuffle and typesize > 1:
+            ne = n // typesize
+            blk = np.concatenate([blk[:ne * typesize].reshape(ne, typesize).T.reshape(-1), blk[ne * typesize:]])
+        split = typesize > 1 and n == blocksize and n % typesize == 0
+        streams = np.split(blk, typesize) if split else [blk]
+        starts.append(head + len(body))
+        for st in streams:
+            c = comp(st.tobytes())
+            if len(c) >= len(st):
+                c = st.tobytes()
+            body += struct.pack("<I", len(c)) + c
+    flags = (0x01 if shuffle else 0) | (code << 5)
+    hdr = bytes([2, 1, flags, typesize]) + struct.pack("<III", len(data), blocksize, head + len(body))
+    return hdr + struct.pack(f"<{nblocks}I", *starts) + body
+
+
+def _encode(comp: dict | None, raw: bytes, typesize: int) -> bytes:
+    import pyarrow as pa
+
+    cid = None if comp is None else comp["id"]
+    if cid is None:
+        return raw
+    if cid == "zlib":
+        return zlib.compress(raw)
+    if cid == "gzip":
+        return gzip.compress(raw)
+    if cid == "zstd":
+        return pa.compress(raw, codec="zstd", asbytes=True)
+    if cid == "lz4":
+        return len(raw).to_bytes(4, "little") + pa.compress(raw, codec="lz4_raw", asbytes=True)
+    code = {"lz4": 1, "snappy": 2, "zlib": 3, "zstd": 4}.get(comp.get("cname", "lz4"))
+    if cid != "blosc" or code is None or comp.get("shuffle", 1) == 2:
+        return raw  # a store the reader refuses when it opens it
+    return _blosc_frame(raw, typesize, comp.get("_blocksize", 1024), bool(comp.get("shuffle", 1)), code)
+
+
 def _hand_store(root, arr, chunks, sep, compressor, fill=0, axes=("c", "y", "x"), drop=()):
     root.mkdir()
     (root / ".zgroup").write_text('{"zarr_format": 2}')
@@ -170,9 +226,10 @@ def _hand_store(root, arr, chunks, sep, compressor, fill=0, axes=("c", "y", "x")
                                                                "datasets": [{"path": "0"}]}]}))
     d = root / "0"
     d.mkdir()
+    comp = {"id": compressor} if isinstance(compressor, str) else compressor
     meta = {"zarr_format": 2, "shape": list(arr.shape), "chunks": list(chunks), "dtype": arr.dtype.str,
-            "compressor": None if compressor is None else {"id": compressor}, "fill_value": fill, "order": "C",
-            "filters": None}
+            "compressor": None if comp is None else {k: v for k, v in comp.items() if not k.startswith("_")},
+            "fill_value": fill, "order": "C", "filters": None}
     if sep is not None:
         meta["dimension_separator"] = sep
     (d / ".zarray").write_text(json.dumps(meta))
@@ -184,8 +241,7 @@ def _hand_store(root, arr, chunks, sep, compressor, fill=0, axes=("c", "y", "x")
         sl = tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, chunks, arr.shape))
         part = arr[sl]
         full[tuple(slice(0, p) for p in part.shape)] = part
-        raw = full.tobytes()
-        raw = zlib.compress(raw) if compressor == "zlib" else gzip.compress(raw) if compressor == "gzip" else raw
+        raw = _encode(comp, full.tobytes(), arr.dtype.itemsize)
         p = d / (sep or ".").join(str(i) for i in idx)
         p.parent.mkdir(parents=True, exist_ok=True)
         p.write_bytes(raw)
@@ -205,11 +261,94 @@ def test_reader_layouts(tmp_path, sep, compressor, dtype):
     np.testing.assert_array_equal(py.read_block(0, 0, (0, 5, 9), (1, 30, 20))[0], want[0, 5:35, 9:29])
 
 
+@pytest.mark.parametrize("comp", [{"id": "zstd"}, {"id": "lz4"},
+                                  {"id": "blosc", "cname": "lz4", "shuffle": 1, "_blocksize": 1000},
+                                  {"id": "blosc", "cname": "zstd", "shuffle": 1, "_blocksize": 4096},
+                                  {"id": "blosc", "cname": "zlib", "shuffle": 0, "_blocksize": 512},
+                                  {"id": "blosc", "cname": "snappy", "shuffle": 1, "_blocksize": 333}])
+@pytest.mark.parametrize("dtype", ["<f4", "<u2", "|u1"])
+def test_reader_codecs(tmp_path, comp, dtype):
+    """Stores whose chunks carry real LZ4, zstd, snappy and zlib payloads (pyarrow's and zlib's
+    encoders), in Blosc frames with shuffled, split streams and partial last blocks, read back
+    exactly."""
+    rng = np.random.default_rng(4)
+    arr = (rng.integers(0, 60, (2, 45, 38)) * (rng.random((2, 45, 38)) < 0.7)).astype(dtype)
+    _hand_store(tmp_path / "h", arr, (1, 16, 16), "/", comp, drop={(0, 2, 2)})
+    py = Pyramid.open(tmp_path / "h")
+    want = arr.astype(np.float32)
+    want[0, 32:45, 32:38] = 0
+    for c in range(2):
+        np.testing.assert_array_equal(py.read_block(0, c, (0, 0, 0), (1, 45, 38))[0], want[c])
+
+
+def test_blosc_known_frame():
+    """A frame laid out by hand: 16 bytes as 4 elements of 4 bytes, shuffled into 4 byte planes,
+    each plane a raw 4-byte stream (its size equals its length), LZ4 code and shuffle flag."""
+    import struct
+
+    from lucem_amd.slide import blosc_decompress
+
+    planes = [bytes([0, 4, 8, 12]), bytes([1, 5, 9, 13]), bytes([2, 6, 10, 14]), bytes([3, 7, 11, 15])]
+    body = b"".join(struct.pack("<I", 4) + p for p in planes)
+    frame = bytes([2, 1, 0x01 | (1 << 5), 4]) + struct.pack("<III", 16, 16, 20 + len(body)) + \
+        struct.pack("<I", 20) + body
+    assert blosc_decompress(frame, 16) == bytes(range(16))
+    # the same bytes stored as they are (flag 0x02)
+    stored = bytes([2, 1, 0x02, 4]) + struct.pack("<III", 16, 16, 32) + bytes(range(16))
+    assert blosc_decompress(stored, 16) == bytes(range(16))
+    # a block of 10 bytes of 4-byte elements keeps its 2 leftover bytes in place
+    data = bytes(range(10))
+    f = _blosc_frame(data, 4, 16)
+    assert blosc_decompress(f, 10) == data
+
+
+@pytest.mark.parametrize("typesize", [1, 2, 4, 8])
+@pytest.mark.parametrize("code", [1, 2, 3, 4])
+def test_blosc_round_trip(typesize, code):
+    from lucem_amd.slide import blosc_decompress
+
+    rng = np.random.default_rng(typesize * 10 + code)
+    data = (rng.integers(0, 5, 5000) * 37).astype(np.uint8).tobytes() + rng.bytes(123)
+    for blocksize in (64, 1000, 4096, 8192):
+        bs = blocksize - blocksize % typesize
+        assert blosc_decompress(_blosc_frame(data, typesize, bs, True, code), len(data)) == data
+        assert blosc_decompress(_blosc_frame(data, typesize, bs, False, code), len(data)) == data
+
+
+def test_blosc_refusals():
+    import struct
+
+    from lucem_amd.slide import blosc_decompress
+
+    good = _blosc_frame(bytes(range(200)) * 10, 4, 512)
+    with pytest.raises(IoError, match="header"):
+        blosc_decompress(good[:10], 2000)
+    with pytest.raises(IoError, match="needs 2001"):
+        blosc_decompress(good, 2001)
+    with pytest.raises(IoError, match="frame size"):
+        blosc_decompress(good[:-1], 2000)
+    bit = bytes([2, 1, good[2] | 0x04]) + good[3:]
+    with pytest.raises(IoError, match="bit shuffle"):
+        blosc_decompress(bit, 2000)
+    blz = bytes([2, 1, good[2] & 0x1F]) + good[3:]
+    with pytest.raises(IoError, match="BloscLZ"):
+        blosc_decompress(blz, 2000)
+    trunc = bytes([2, 1, good[2], 4]) + struct.pack("<III", 2000, 512, 18) + b"\0\0"
+    with pytest.raises(IoError, match="table"):
+        blosc_decompress(trunc, 2000)
+
+
 def test_reader_refusals(tmp_path):
     arr = np.zeros((1, 8, 8), np.float32)
-    _hand_store(tmp_path / "b", arr, (1, 8, 8), "/", "blosc")
-    with pytest.raises(IoError, match="blosc"):
+    _hand_store(tmp_path / "b", arr, (1, 8, 8), "/", {"id": "bz2"})
+    with pytest.raises(IoError, match="bz2"):
         Pyramid.open(tmp_path / "b")
+    _hand_store(tmp_path / "bl", arr, (1, 8, 8), "/", {"id": "blosc", "cname": "blosclz"})
+    with pytest.raises(IoError, match="blosclz"):
+        Pyramid.open(tmp_path / "bl")
+    _hand_store(tmp_path / "bs", arr, (1, 8, 8), "/", {"id": "blosc", "cname": "lz4", "shuffle": 2})
+    with pytest.raises(IoError, match="bit shuffle"):
+        Pyramid.open(tmp_path / "bs")
     _hand_store(tmp_path / "ok", arr, (1, 8, 8), "/", None)
     py = Pyramid.open(tmp_path / "ok")
     with pytest.raises(IoError, match="outside level"):
