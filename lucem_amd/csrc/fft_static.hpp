@@ -47,9 +47,11 @@ template <bool INV> struct StaticButterfly<2, INV> : Composite<2, 1, INV> {};
 // One pass: butterfly u = j * B + b (j < N / R, line b) takes elements u + t * (N / R) * B,
 // multiplies element t by w^(t k), k = j mod Ns, w = exp(-+2 pi i / (Ns R)), and writes element s
 // of its DFT to (j - k) * R * B + b + (k + s * Ns) * B. Twiddles from tw[m] = exp(-2 pi i m / N).
-template <int N, int B, int T, bool INV, int Ns, int R>
+// With PRUNE only the lines b < nl are transformed: the butterflies are numbered w = j * nl + b,
+// so that the threads past the last one idle as whole waves.
+template <int N, int B, int T, bool INV, int Ns, int R, bool PRUNE>
 __device__ __forceinline__ void static_pass(const float2* __restrict__ src, float2* __restrict__ dst,
-                                            const float2* __restrict__ tw) {
+                                            const float2* __restrict__ tw, const Divisor nl) {
     constexpr int nbf = (N / R) * B;
     constexpr int S = (nbf + T - 1) / T;
     constexpr int tws = N / (Ns * R);
@@ -58,17 +60,25 @@ __device__ __forceinline__ void static_pass(const float2* __restrict__ src, floa
     const csrc_t sp = (csrc_t)(size_t)src;
     const cdst_t dp = (cdst_t)(size_t)dst;
     const csrc_t tp = (csrc_t)(size_t)tw;
+    const int nact = PRUNE ? (N / R) * nl.d : nbf;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        const int u = (int)threadIdx.x + s * T;
-        if (nbf % T == 0 || u < nbf) {
+        const int w = (int)threadIdx.x + s * T;
+        if (PRUNE ? w < nact : (nbf % T == 0 || w < nbf)) {
+            int u, j;
+            if constexpr (PRUNE) {
+                j = nl.div(w);
+                u = j * B + (w - j * nl.d);
+            } else {
+                u = w;
+                j = u / B;
+            }
             float2 v[R];
 #pragma unroll
             for (int t = 0; t < R; t++) {
                 const lds_vec2 x = sp[u + t * nbf];
                 v[t] = make_float2(x.x, x.y);
             }
-            const int j = u / B;
             const int k = Ns > 1 ? j % Ns : 0;
             if constexpr (Ns > 1) {
                 const int step = k * tws;
@@ -92,19 +102,21 @@ __device__ __forceinline__ void static_pass(const float2* __restrict__ src, floa
 }
 
 // All passes of a length-N transform of the B lines in `a`, alternating with `b`; returns the
-// buffer that holds the result. Each pass ends with a barrier.
-template <int N, int B, int T, bool INV, int Ns = 1>
-__device__ __forceinline__ float2* static_fft(float2* a, float2* b, const float2* tw) {
+// buffer that holds the result. Each pass ends with a barrier. With PRUNE only the lines below
+// nl.d are transformed; the others of the result buffer keep what they held.
+template <int N, int B, int T, bool INV, bool PRUNE = false, int Ns = 1>
+__device__ __forceinline__ float2* static_fft(float2* a, float2* b, const float2* tw, const Divisor nl = Divisor{B, 0u}) {
     static_assert(static_len_ok(N), "transform length must be 7-smooth");
     if constexpr (Ns >= N) {
         (void)b;
         (void)tw;
+        (void)nl;
         return a;
     } else {
         constexpr int R = static_radix(N / Ns);
-        static_pass<N, B, T, INV, Ns, R>(a, b, tw);
+        static_pass<N, B, T, INV, Ns, R, PRUNE>(a, b, tw, nl);
         __syncthreads();
-        return static_fft<N, B, T, INV, Ns * R>(b, a, tw);
+        return static_fft<N, B, T, INV, PRUNE, Ns * R>(b, a, tw, nl);
     }
 }
 

@@ -70,11 +70,13 @@ struct TileLds {
 };
 
 // Length-L1 transforms of the row-pair lines in `a` (rows) or length-L0 transforms of the column
-// lines (cols); the result's buffer is returned.
+// lines (cols); the result's buffer is returned. Rows: only the lines below nl are needed (the
+// compile-time build transforms just those; the others hold what they held before, or their
+// transforms, which no caller reads).
 template <bool INV, class D>
-__device__ float2* rows_fft(const D& dm, const TileIter& d, const TileLds& t, float2* a) {
+__device__ float2* rows_fft(const D& dm, const TileIter& d, const TileLds& t, float2* a, int nl) {
     if constexpr (D::is_static)
-        return static_fft<D::L1, D::B, kT, INV>(a, t.other(a), t.tw1);
+        return static_fft<D::L1, D::B, kT, INV, true>(a, t.other(a), t.tw1, Divisor::of(nl));
     else
         return lds_fft_oop<INV, kT, kE, kW>(a, t.other(a), Divisor::of(dm.B), d.f1, t.tw1);
 }
@@ -87,18 +89,23 @@ __device__ float2* cols_fft(const D& dm, const TileIter& d, const TileLds& t, fl
 }
 
 // The real tile in `zr` (row-pair lines) -> its 2-D spectrum (column lines); returns the buffer
-// that holds it. The row split carries a factor 2, as the transfer functions (built by the same
-// row transform) expect.
+// that holds it. Row pairs from nl on are zero (rows past the data) and are not transformed. The
+// row split carries a factor 2, as the transfer functions (built by the same row transform) expect.
 template <class D>
-__device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, float2* zr) {
-    const float2* r = rows_fft<false>(dm, d, t, zr);
+__device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, float2* zr, int nl) {
+    const float2* r = rows_fft<false>(dm, d, t, zr, nl);
     float2* sc = t.other(r);
     for (int e = threadIdx.x; e < dm.nb * dm.nc; e += kT) {
         const int b = dm.div_nc(e), k = e - b * dm.nc;
-        const float2 zk = r[k * dm.B + b];
-        const float2 zm = r[(k == 0 ? 0 : dm.L1 - k) * dm.B + b];
-        sc[(2 * b) * dm.C + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
-        sc[(2 * b + 1) * dm.C + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
+        float2 za = make_float2(0.f, 0.f), zb = za;
+        if (b < nl) {
+            const float2 zk = r[k * dm.B + b];
+            const float2 zm = r[(k == 0 ? 0 : dm.L1 - k) * dm.B + b];
+            za = make_float2(zk.x + zm.x, zk.y - zm.y);
+            zb = make_float2(zk.y + zm.y, zm.x - zk.x);
+        }
+        sc[(2 * b) * dm.C + k] = za;
+        sc[(2 * b + 1) * dm.C + k] = zb;
     }
     __syncthreads();
     return cols_fft<false>(dm, d, t, sc);
@@ -135,12 +142,13 @@ __device__ __forceinline__ void apply_otf(const D& dm, float2* sc, const float2 
 }
 
 // The spectrum in `sc` -> the real tile as row-pair lines; returns the buffer that holds it
-// (unnormalised: the transfer function carries the 1 / (4 L0 L1)).
+// (unnormalised: the transfer function carries the 1 / (4 L0 L1)). Only the row pairs below nl,
+// the ones the caller reads, are produced.
 template <class D>
-__device__ float2* inverse_2d(const D& dm, const TileIter& d, const TileLds& t, float2* sc) {
+__device__ float2* inverse_2d(const D& dm, const TileIter& d, const TileLds& t, float2* sc, int nl) {
     const float2* r = cols_fft<true>(dm, d, t, sc);
     float2* zr = t.other(r);
-    for (int e = threadIdx.x; e < dm.nb * dm.nc; e += kT) {
+    for (int e = threadIdx.x; e < nl * dm.nc; e += kT) {
         const int b = dm.div_nc(e), k = e - b * dm.nc;
         float2 A = r[(2 * b) * dm.C + k], Bv = r[(2 * b + 1) * dm.C + k];
         if (k == 0 || 2 * k == dm.L1) {
@@ -152,7 +160,7 @@ __device__ float2* inverse_2d(const D& dm, const TileIter& d, const TileLds& t, 
         if (k > 0 && km >= dm.nc) zr[km * dm.B + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
     }
     __syncthreads();
-    return rows_fft<true>(dm, d, t, zr);
+    return rows_fft<true>(dm, d, t, zr, nl);
 }
 
 // A box of the image (rows y0 + r, columns x0 + c) lies inside it: no boundary rule applies.
@@ -236,7 +244,7 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
     float2 w[kE];
     load_otf(dm, d, d.otf1, w);
     __syncthreads();
-    float2* sp = forward_2d(dm, d, t, t.X);
+    float2* sp = forward_2d(dm, d, t, t.X, nb);
     apply_otf(dm, sp, w);
 
     // Q2 = P(rel) on the rel window (my x mx: the tile's own outputs, fewer than T on the last tile
@@ -285,7 +293,8 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
         }
     }
     load_otf(dm, d, d.otf2, w);
-    const float2* U = inverse_2d(dm, d, t, sp);
+    const int nq = (my + 1) / 2;  // row pairs of the rel window
+    const float2* U = inverse_2d(dm, d, t, sp, nq);
 
     // U holds u(t0 - left + p) at p; Q2 goes into the other buffer (its last reader, the final
     // inverse pass, is behind a barrier).
@@ -296,7 +305,7 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
             for (int s = 0; s < kG; s++) {
                 const int b = pr0 + s * pstep;
                 float2 u = make_float2(1.f, 1.f);
-                if (active && b < nb && c < mx) u = U[c * B + b];
+                if (active && b < nq && c < mx) u = U[c * B + b];
                 v[s].x = div_fast(v[s].x, fabsf(u.x) < d.eps ? d.eps : u.x);
                 v[s].y = div_fast(v[s].y, fabsf(u.y) < d.eps ? d.eps : u.y);
             }
@@ -321,7 +330,7 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
         }
     }
     __syncthreads();
-    sp = forward_2d(dm, d, t, Q);
+    sp = forward_2d(dm, d, t, Q, nq);
     apply_otf(dm, sp, w);
 
     // the estimate on the tile's own pixels, loaded before the last inverse transform
@@ -337,7 +346,7 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
             if (2 * b + 1 < h) x[s].y = p[d.n1];
         }
     }
-    const float2* Cv = inverse_2d(dm, d, t, sp);
+    const float2* Cv = inverse_2d(dm, d, t, sp, (h + 1) / 2);
 
     // Cv holds c(t0 + p) at p: the update on the tile's own pixels
     {
