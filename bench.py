@@ -126,8 +126,8 @@ def _cpu_model() -> str:
 def device_info() -> dict:
     """The device's name and CU count, and its memory clock and bus width as the HIP runtime reports
     them (``hipDeviceGetAttribute``; attribute ids 60 and 59 of ROCm 7.2's ``hipDeviceAttribute_t``),
-    with the peak they imply at two transfers per clock, beside the published 8 TB/s the roofline
-    uses."""
+    with the peak they imply, beside the published 8 TB/s the roofline uses. HBM3 and HBM3E move
+    4 bits per pin per reported memory clock (MI300X: 1.3 GHz, 5.2 Gb/s per pin)."""
     import ctypes
 
     import torch
@@ -145,7 +145,7 @@ def device_info() -> dict:
             bw = v.value
         out.update({"memory_clock_khz": clk, "memory_bus_bits": bw})
         if clk and bw:
-            out["implied_peak_GBps"] = round(2 * clk * 1e3 * bw / 8 / 1e9, 1)
+            out["implied_peak_GBps"] = round(4 * clk * 1e3 * bw / 8 / 1e9, 1)
     except OSError:
         pass
     out["peak_used_GBps"] = HBM_PEAK_GBPS
