@@ -135,6 +135,26 @@ def test_tiled_against_whole_array_at_full_size(gpu, tiled_mode):
     assert np.isfinite(a).all() and e < 2e-6
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape,kshape,grid", [((57, 85), (3, 4), "32,32,28,26"), ((129, 65), (5, 2), "48,48,40,46"),
+                                              ((97, 97), (2, 7), "64,64,62,52")])
+def test_edge_tiles_of_one_row(gpu, oracle, tiled_mode, monkeypatch, mode, shape, kshape, grid):
+    """Grids whose last tiles are 1 to a few rows or columns high, with odd and even PSF extents:
+    the row transforms run only over the row pairs each step reads (the rel window's, the output's),
+    whose counts are odd or even here, on the compile-time builds."""
+    tiled_mode("1")
+    monkeypatch.setenv("LUCEM_TILE_GRID", grid)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, kshape, mode)).encode()))
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    got, stats = _run(gpu, y, psf, 5, mode)
+    assert "tile_iter" in stats
+    _, ref = oracle.richardson_lucy(y, psf, 5, clip=False, mode=mode)
+    e = oracle.rel_l2(got, ref)
+    print(f"{shape} {kshape} grid {grid} mode {mode}: rel-L2 vs FP32 {e:.2e}")
+    assert e < RTOL_L2
+
+
 @pytest.mark.parametrize("shape,kshape,grid", [((1024, 1024), (25, 25), None), ((300, 200), (9, 9), "64,64,48,48"),
                                               ((500, 420), (17, 13), "96,96,64,72"), ((90, 70), (3, 3), "128,128,124,124")])
 def test_constant_length_build_equals_run_time_build(gpu, tiled_mode, monkeypatch, shape, kshape, grid):
