@@ -29,6 +29,18 @@ constexpr int static_radix(int rem) {
                            : 1;
 }
 constexpr bool static_len_ok(int n) { return n == 1 ? true : static_radix(n) > 1 && static_len_ok(n / static_radix(n)); }
+// The radix of a length-n transform's last pass.
+constexpr int static_last_radix(int n) { return n / static_radix(n) == 1 ? n : static_last_radix(n / static_radix(n)); }
+
+// A factor per output element of a transform's last pass, read from a [row][pitch] table in global
+// memory (element i of line b takes tab[i * pitch + b]; lines from nl on take 0). The first round
+// of butterflies takes its factors from registers loaded beforehand (pre[t] for output t of
+// butterfly threadIdx.x); later rounds, when there are any, read the table.
+struct OutScale {
+    const float2* tab = nullptr;
+    int pitch = 0;
+    int nl = 0;
+};
 
 template <int R, bool INV>
 struct StaticButterfly;
@@ -49,9 +61,11 @@ template <bool INV> struct StaticButterfly<2, INV> : Composite<2, 1, INV> {};
 // of its DFT to (j - k) * R * B + b + (k + s * Ns) * B. Twiddles from tw[m] = exp(-2 pi i m / N).
 // With PRUNE only the lines b < nl are transformed: the butterflies are numbered w = j * nl + b,
 // so that the threads past the last one idle as whole waves.
-template <int N, int B, int T, bool INV, int Ns, int R, bool PRUNE>
+template <int N, int B, int T, bool INV, int Ns, int R, bool PRUNE, bool MUL>
 __device__ __forceinline__ void static_pass(const float2* __restrict__ src, float2* __restrict__ dst,
-                                            const float2* __restrict__ tw, const Divisor nl) {
+                                            const float2* __restrict__ tw, const Divisor nl, const OutScale& ms,
+                                            const float2* pre) {
+    static_assert(!MUL || (Ns * R == N && !PRUNE), "output factors apply to a whole last pass");
     constexpr int nbf = (N / R) * B;
     constexpr int S = (nbf + T - 1) / T;
     constexpr int tws = N / (Ns * R);
@@ -89,6 +103,19 @@ __device__ __forceinline__ void static_pass(const float2* __restrict__ src, floa
                 }
             }
             StaticButterfly<R, INV>::run(v);
+            if constexpr (MUL) {
+                // the last pass writes output t of butterfly (j, b) to element j + t N / R of line b
+                const int b = u - j * B;
+#pragma unroll
+                for (int t = 0; t < R; t++) {
+                    float2 f;
+                    if (s == 0)
+                        f = pre[t];
+                    else
+                        f = b < ms.nl ? ms.tab[(int64_t)(j + t * (N / R)) * ms.pitch + b] : make_float2(0.f, 0.f);
+                    v[t] = cmul(v[t], f);
+                }
+            }
             const int o = u + (j - k) * (R - 1) * B;
 #pragma unroll
             for (int t = 0; t < R; t++) {
@@ -103,20 +130,24 @@ __device__ __forceinline__ void static_pass(const float2* __restrict__ src, floa
 
 // All passes of a length-N transform of the B lines in `a`, alternating with `b`; returns the
 // buffer that holds the result. Each pass ends with a barrier. With PRUNE only the lines below
-// nl.d are transformed; the others of the result buffer keep what they held.
-template <int N, int B, int T, bool INV, bool PRUNE = false, int Ns = 1>
-__device__ __forceinline__ float2* static_fft(float2* a, float2* b, const float2* tw, const Divisor nl = Divisor{B, 0u}) {
+// nl.d are transformed; the others of the result buffer keep what they held. With MUL the last
+// pass multiplies its outputs by the factors of `ms` (the first round's from `pre`).
+template <int N, int B, int T, bool INV, bool PRUNE = false, bool MUL = false, int Ns = 1>
+__device__ __forceinline__ float2* static_fft(float2* a, float2* b, const float2* tw, const Divisor nl = Divisor{B, 0u},
+                                              const OutScale& ms = OutScale{}, const float2* pre = nullptr) {
     static_assert(static_len_ok(N), "transform length must be 7-smooth");
     if constexpr (Ns >= N) {
         (void)b;
         (void)tw;
         (void)nl;
+        (void)ms;
+        (void)pre;
         return a;
     } else {
         constexpr int R = static_radix(N / Ns);
-        static_pass<N, B, T, INV, Ns, R, PRUNE>(a, b, tw, nl);
+        static_pass<N, B, T, INV, Ns, R, PRUNE, MUL && Ns * R == N>(a, b, tw, nl, ms, pre);
         __syncthreads();
-        return static_fft<N, B, T, INV, PRUNE, Ns * R>(b, a, tw, nl);
+        return static_fft<N, B, T, INV, PRUNE, MUL, Ns * R>(b, a, tw, nl, ms, pre);
     }
 }
 

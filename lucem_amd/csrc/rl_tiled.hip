@@ -91,8 +91,12 @@ __device__ float2* cols_fft(const D& dm, const TileIter& d, const TileLds& t, fl
 // The real tile in `zr` (row-pair lines) -> its 2-D spectrum (column lines); returns the buffer
 // that holds it. Row pairs from nl on are zero (rows past the data) and are not transformed. The
 // row split carries a factor 2, as the transfer functions (built by the same row transform) expect.
+// The compile-time build multiplies the spectrum by the transfer function `otf` in the column
+// transform's last pass (`pre`: this thread's first-round factors, from load_last_pass); the
+// run-time build leaves that to apply_otf.
 template <class D>
-__device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, float2* zr, int nl) {
+__device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, float2* zr, int nl,
+                              const float2* otf, const float2* pre) {
     const float2* r = rows_fft<false>(dm, d, t, zr, nl);
     float2* sc = t.other(r);
     for (int e = threadIdx.x; e < dm.nb * dm.nc; e += kT) {
@@ -108,7 +112,28 @@ __device__ float2* forward_2d(const D& dm, const TileIter& d, const TileLds& t, 
         sc[(2 * b + 1) * dm.C + k] = zb;
     }
     __syncthreads();
-    return cols_fft<false>(dm, d, t, sc);
+    if constexpr (D::is_static) {
+        const OutScale ms{otf, d.pitch, D::nc};
+        return static_fft<D::L0, D::C, kT, false, false, true>(sc, t.other(sc), t.tw0, Divisor{D::C, 0u}, ms, pre);
+    } else {
+        (void)otf;
+        (void)pre;
+        return cols_fft<false>(dm, d, t, sc);
+    }
+}
+
+// The compile-time build's factors for the first round of the forward column transform's last
+// pass: output t of butterfly (j, b), b = threadIdx.x mod C, is element j + t L0 / R of column b.
+template <class D, int RL>
+__device__ __forceinline__ void load_last_pass(const TileIter& d, const float2* __restrict__ otf, float2 (&pre)[RL]) {
+    constexpr int nbf = (D::L0 / RL) * D::C;
+    const int w = threadIdx.x;
+    const int j = w / D::C, b = w - j * D::C;
+#pragma unroll
+    for (int t = 0; t < RL; t++) {
+        pre[t] = make_float2(0.f, 0.f);
+        if ((nbf >= kT || w < nbf) && b < D::nc) pre[t] = otf[(int64_t)(j + t * (D::L0 / RL)) * d.pitch + b];
+    }
 }
 
 // The transfer-function values of this thread's spectrum elements ([L0][pitch] in global memory;
@@ -241,11 +266,15 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
         }
     }
     // the first transfer function's values, loaded while the tile is transformed
-    float2 w[kE];
-    load_otf(dm, d, d.otf1, w);
+    constexpr bool kStatic = Dims<SL0, SL1>::is_static;
+    constexpr int kRL = kStatic ? static_last_radix(SL0) : 1;
+    float2 w[kE];     // run-time build: this thread's elements for apply_otf
+    float2 pre[kRL];  // compile-time build: the first round of the last column pass
+    if constexpr (kStatic) load_last_pass<Dims<SL0, SL1>, kRL>(d, d.otf1, pre);
+    else load_otf(dm, d, d.otf1, w);
     __syncthreads();
-    float2* sp = forward_2d(dm, d, t, t.X, nb);
-    apply_otf(dm, sp, w);
+    float2* sp = forward_2d(dm, d, t, t.X, nb, d.otf1, pre);
+    if constexpr (!kStatic) apply_otf(dm, sp, w);
 
     // Q2 = P(rel) on the rel window (my x mx: the tile's own outputs, fewer than T on the last tile
     // of an axis, and the PSF's reach), 0 beyond. The observed values and the second transfer
@@ -292,7 +321,8 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
             }
         }
     }
-    load_otf(dm, d, d.otf2, w);
+    if constexpr (kStatic) load_last_pass<Dims<SL0, SL1>, kRL>(d, d.otf2, pre);
+    else load_otf(dm, d, d.otf2, w);
     const int nq = (my + 1) / 2;  // row pairs of the rel window
     const float2* U = inverse_2d(dm, d, t, sp, nq);
 
@@ -330,8 +360,8 @@ __device__ __forceinline__ void tile_body(const TileIter& d, int tile, float2* l
         }
     }
     __syncthreads();
-    sp = forward_2d(dm, d, t, Q, nq);
-    apply_otf(dm, sp, w);
+    sp = forward_2d(dm, d, t, Q, nq, d.otf2, pre);
+    if constexpr (!kStatic) apply_otf(dm, sp, w);
 
     // the estimate on the tile's own pixels, loaded before the last inverse transform
     const int h = min(d.T0, d.n0 - t0y), wd = min(d.T1, d.n1 - t0x);
