@@ -24,6 +24,12 @@
 // The kernel is compiled twice over: with the tile's lengths known at compile time for the common
 // domains (constant strides, immediate LDS offsets, a fixed radix sequence: fft_static.hpp), and
 // with run-time lengths for every other admissible domain.
+//
+// Latency: every global load is issued a transform ahead of its use (the transfer functions, the
+// observed window, the estimate on the tile), and the twiddle tables are staged under the first
+// gather's loads. Work: the row transforms run only over the row pairs that hold data or are read
+// next, and the compile-time build multiplies by the transfer function in the last pass of the
+// forward column transform rather than in a sweep of its own.
 #include <hip/hip_runtime.h>
 
 #include "fft_lds.hpp"
