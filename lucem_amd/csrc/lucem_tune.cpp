@@ -125,10 +125,12 @@ struct Timer {
     }
 };
 
-bool tune_log() {
+// LUCEM_TUNE_LOG=1 prints each choice, 2 also every candidate's time.
+int tune_log_level() {
     const char* v = std::getenv("LUCEM_TUNE_LOG");
-    return v && std::string(v) == "1";
+    return v ? std::atoi(v) : 0;
 }
+bool tune_log() { return tune_log_level() >= 1; }
 
 bool autotune_enabled() {
     const char* v = std::getenv("LUCEM_AUTOTUNE");
@@ -270,6 +272,8 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                     }, &ms);
                 }
                 if (st) return st;
+                if (tune_log_level() >= 2)
+                    std::fprintf(stderr, "lucem autotune:   row L=%d P=%d wide=%d variant=%d %.4f ms\n", L, P, w, v, ms);
                 if (ms < best_ms) {
                     best_ms = ms;
                     best = ShapeChoice{P, w, v};
@@ -380,6 +384,9 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         st = time_launch(s, timer, [&]() { return launch_col_pass(d[0], nb, s); }, &ms);
                     }
                     if (st) return st;
+                    if (tune_log_level() >= 2)
+                        std::fprintf(stderr, "lucem autotune:   line kind=%d L=%d B=%d wide=%d variant=%d %.4f ms\n", kind,
+                                     L, B, w, v, ms);
                     if (ms < best_ms) {
                         best_ms = ms;
                         best = ShapeChoice{B, w, v};
