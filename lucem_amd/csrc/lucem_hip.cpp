@@ -511,6 +511,15 @@ int lucem::choose_P(int L, int rows, const ShapePolicy& sp, int* tier, int* vari
     return P;
 }
 
+// Whether a COL_FUSED pass runs its compile-time build: when the tuner chose it (variant 3), or
+// as LUCEM_COL_STATIC=1 / 0 forces, wherever a build of (L, B) at 256 threads exists.
+int lucem::col_static_choice(bool tuned, int L, int B, int wide) {
+    if (wide != 0 || !col_static_available(L, B)) return 0;
+    const char* v = std::getenv("LUCEM_COL_STATIC");
+    if (v && *v) return std::atoi(v) != 0 ? 1 : 0;
+    return tuned ? 1 : 0;
+}
+
 // Columns per workgroup of a line pass, any count up to 32, fewer only when the launch would
 // not give every CU one workgroup (wide workgroups beat more of them in the measurements). The
 // 512-thread shape serves lines too long for 8 columns in 4096 values.
@@ -671,6 +680,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
         y.B = choose_B(g.L[1], g.Lc, 1, SHAPE_COL_FUSED, sp, &y.wide, &var);
         y.f = make_plan(g.L[1], var);
         y.nkb = (g.Lc + y.B - 1) / y.B;
+        y.stat = col_static_choice(var == kVariantStatic, g.L[1], y.B, y.wide);
         out[0] = y;
         blocks[0] = y.nkb;
         return 1;
@@ -708,6 +718,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
     z.B = choose_B(g.L[0], g.Lc, g.L[1], SHAPE_COL_FUSED, sp, &z.wide, &var);
     z.f = make_plan(g.L[0], var);
     z.nkb = (g.Lc + z.B - 1) / z.B;
+    z.stat = col_static_choice(var == kVariantStatic, g.L[0], z.B, z.wide);
     out[1] = z;
     blocks[1] = g.L[1] * z.nkb;
 

@@ -63,7 +63,7 @@ void load_cache_locked() {
     if (!fp) return;
     int k, L, op, lg, pers, lines, wide, var;
     while (std::fscanf(fp, "%d %d %d %d %d %d %d %d", &k, &L, &op, &lg, &pers, &lines, &wide, &var) == 8)
-        if (lines >= 1 && wide >= 0 && wide <= 2 && var >= 0 && var <= 2)
+        if (lines >= 1 && wide >= 0 && wide <= 2 && var >= 0 && var <= kVariantStatic)
             g_cache[CacheKey{k, L, op, lg, pers}] = ShapeChoice{lines, wide, var};
     std::fclose(fp);
 }
@@ -333,20 +333,32 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         ShapeChoice best{};
         float best_ms = 1e30f;
         std::vector<FftPlan> plans;
-        for (int v = 0; v < 3; v++) {
-            FftPlan f = make_plan(L, v);
+        // variant 3: the compile-time COL_FUSED build, for the column counts it is built for
+        const bool try_static = mode == TUNE_PLAIN && g[0].d.op == COL_FUSED && !g[0].d.save;
+        for (int v = 0; v < 4; v++) {
+            const bool stat = v == kVariantStatic;
+            if (stat && !try_static) continue;
+            FftPlan f = make_plan(L, stat ? 0 : v);
             bool dup = false;
             for (const FftPlan& q : plans) dup = dup || same_plan(q, f);
-            if (dup) continue;
-            plans.push_back(f);
+            if (dup && !stat) continue;
+            if (!stat) plans.push_back(f);
             for (int w = 0; w < 2; w++) {
                 if (fixed_wide && w != kind_wide[kind]) continue;
+                if (stat && w != 0) continue;
                 if (w == 0 && L > kLdsComplex) continue;  // the line does not fit 256 threads
                 const int cap = w ? kLdsComplexWide : kLdsComplex;
                 const int bmax = std::max(1, std::min(32, cap / L));
                 if (w == 1 && !fixed_wide && L <= kLdsComplex && bmax <= std::max(1, std::min(32, kLdsComplex / L)))
                     continue;
-                for (int B : line_candidates(bmax, g[0].d.wide == w ? g[0].d.B : bmax)) {
+                std::vector<int> cands;
+                if (stat) {
+                    for (int B = 1; B <= bmax; B++)
+                        if (col_static_available(L, B)) cands.push_back(B);
+                } else {
+                    cands = line_candidates(bmax, g[0].d.wide == w ? g[0].d.B : bmax);
+                }
+                for (int B : cands) {
                     std::vector<ColPass> d;
                     std::vector<int2> items;
                     for (size_t i = 0; i < g.size(); i++) {
@@ -354,6 +366,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                         cp.B = B;
                         cp.wide = w;
                         cp.f = f;
+                        cp.stat = stat ? 1 : 0;
                         cp.nkb = (cp.Lc + B - 1) / B;
                         for (int q = 0; q < g[i].outers * cp.nkb; q++) items.push_back(make_int2((int)i, q));
                         d.push_back(cp);

@@ -113,6 +113,7 @@ struct ColPass {
     int64_t slot_stride;
     int fold_left, fold_mode, fold_n;  // COL_FUSED_T / COL_INV_FOLD: the forward's pad (transposed)
     int fold_end;                      // n + k - 1: end of the padded support
+    int stat;                          // COL_FUSED: run the compile-time build of (f.L, B)
 };
 
 // PSF preparation (deconvolve.rs:73-90 on the device): f64 sum, norm = 1/sum when |sum| > 1e-12
@@ -353,5 +354,7 @@ __host__ __device__ constexpr int tile_buffer_elems(int L0, int L1) {
 }
 size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
+// A compile-time COL_FUSED build exists for lines of length L in blocks of B columns (256 threads).
+bool col_static_available(int L, int B);
 
 }  // namespace lucem

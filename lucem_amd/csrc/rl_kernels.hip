@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_lds.hpp"
+#include "fft_static.hpp"
 #include "rl_internal.hpp"
 
 // Waves per SIMD the FFT kernels are compiled for (their VGPR budget: 4 -> 128, 5 -> 96). The
@@ -490,6 +491,101 @@ __global__ void zero_ints_kernel(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// COL_FUSED with the line length L and the column count B known at compile time (fft_static.hpp).
+// The passes are inlined, so the transfer-function factors of the last forward pass, loaded with
+// the gather, arrive while the forward transform runs (a call to an out-of-line pass would wait
+// for them), and multiply in that pass. LDS: two buffers of L x B values (out-of-place passes, one
+// barrier each), then the twiddle table. Same descriptor, grid and results (to rounding) as
+// col_pass_kernel<256, 16, COL_FUSED>; no PSF-gradient save.
+// ---------------------------------------------------------------------------------------------
+template <int L, int B>
+constexpr size_t col_static_lds_bytes() {
+    return (size_t)(2 * tw_lds_offset(L * B) + L) * sizeof(float2);
+}
+
+template <int L, int B>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_fused_static_kernel(
+    ColPass d) {
+    constexpr int T = kThreads;
+    constexpr int n = L * B;
+    constexpr int S = (n + T - 1) / T;
+    constexpr int RL = static_last_radix(L);
+    constexpr int nbf = (L / RL) * B;  // butterflies of the last pass
+    static_assert(n <= T * kElemsPerThread, "one block of lines per workgroup");
+    extern __shared__ float2 lds[];
+    float2* X = lds;
+    float2* Y = lds + tw_lds_offset(n);
+    float2* tw = Y + tw_lds_offset(n);
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int outer = g / d.nkb;
+    const int kx0 = (g - outer * d.nkb) * B;
+    const int nl = min(B, d.Lc - kx0);  // columns of this block that hold data
+    const int tid = threadIdx.x;
+    const float2* src = d.src + outer * d.src_outer + kx0;
+    const float2* otf = d.otf + outer * d.otf_outer + kx0;
+    float2 v[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        v[s] = make_float2(0.f, 0.f);
+        if (n % T == 0 || e < n) {
+            const int i = e / B, b = e - i * B;
+            const int sr = pad_index(i - d.left, d.n_src, d.mode);
+            if (sr >= 0 && b < nl) v[s] = src[sr * d.src_axis + b];
+        }
+    }
+    // output t of the last pass's butterfly (j, b) = tid is element j + t L / RL of column b
+    float2 pre[RL];
+    {
+        const int j = tid / B, b = tid - j * B;
+#pragma unroll
+        for (int t = 0; t < RL; t++) {
+            pre[t] = make_float2(0.f, 0.f);
+            if ((nbf >= T || tid < nbf) && b < nl) pre[t] = otf[(int64_t)(j + t * (L / RL)) * d.otf_axis + b];
+        }
+    }
+    for (int i = tid; i < L; i += T) tw[i] = d.tw[d.f.tw + i];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        if (n % T == 0 || e < n) X[e] = v[s];
+    }
+    __syncthreads();
+    const OutScale ms{otf, (int)d.otf_axis, nl};
+    float2* sp = static_fft<L, B, T, false, false, true>(X, Y, tw, Divisor{B, 0u}, ms, pre);
+    const float2* r = static_fft<L, B, T, true>(sp, sp == X ? Y : X, tw);
+    float2* dst = d.dst + outer * d.dst_outer + kx0;
+    for (int e = tid; e < d.n_dst * B; e += T) {
+        const int i = e / B, b = e - i * B;
+        if (b < nl) dst[i * d.dst_axis + b] = r[e];
+    }
+}
+
+// The compile-time builds of COL_FUSED: (L, B) pairs, tried by the autotuner as variant 3.
+#define LUCEM_COL_STATIC_SHAPES(X) \
+    X(64, 32) X(96, 21) X(128, 16) X(160, 8) X(160, 12) X(192, 10) X(256, 8)
+
+bool col_static_available(int L, int B) {
+#define LUCEM_COL_STATIC_AVAIL(l, b) \
+    if (L == l && B == b) return true;
+    LUCEM_COL_STATIC_SHAPES(LUCEM_COL_STATIC_AVAIL)
+#undef LUCEM_COL_STATIC_AVAIL
+    return false;
+}
+
+static int launch_col_static(const ColPass& p, int blocks, hipStream_t st) {
+#define LUCEM_COL_STATIC_LAUNCH(l, b)                                                                 \
+    if (p.f.L == l && p.B == b) {                                                                     \
+        const size_t lds = col_static_lds_bytes<l, b>();                                             \
+        hipLaunchKernelGGL((col_fused_static_kernel<l, b>), dim3(blocks), dim3(kThreads), lds, st, p); \
+        return (int)hipGetLastError();                                                                \
+    }
+    LUCEM_COL_STATIC_SHAPES(LUCEM_COL_STATIC_LAUNCH)
+#undef LUCEM_COL_STATIC_LAUNCH
+    return (int)hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers. FFT kernels hold 16 complex values per thread (at most 128 VGPRs). The thread tier
 // of a descriptor picks 256 threads (4096 values, 32 KiB of LDS), 512 (8192 values, 64 KiB) or
 // 1024 (16384 values, 128 KiB) per workgroup; a workgroup planned for fewer threads runs
@@ -583,6 +679,7 @@ static int launch_col_e(const ColPass& p, int blocks, void* stream) {
 }
 
 int launch_col_pass(const ColPass& p, int blocks, void* stream) {
+    if (p.stat && p.op == COL_FUSED && !p.save && p.wide == 0) return launch_col_static(p, blocks, (hipStream_t)stream);
     switch (p.wide) {
         case 2: return launch_col_e<kThreadsXL, kElemsPerThread>(p, blocks, stream);
         case 1: return launch_col_e<kThreadsWide, kElemsPerThread>(p, blocks, stream);

@@ -22,7 +22,8 @@ Two helpers for the ``traffic`` figure of ``bench.py``:
 rocprofv3 names kernels by template (``col_pass_kernel<256, 16, 1>``); the op id is the last
 template argument, and each RL pass of a workload is one (kernel family, op) pair. The tiled 2-D
 iteration is one kernel family (``tile_iter_kernel<L0, L1>``, and ``tile_batch_kernel<L0, L1>`` for
-the batched form) whatever its lengths.
+the batched form) whatever its lengths, and so is the compile-time build of the fused line pass
+(``col_fused_static_kernel<L, B>``, counted as ``COL_FUSED``).
 """
 
 from __future__ import annotations
@@ -61,6 +62,8 @@ PASS_KERNELS = {
 
 _NAME = re.compile(r"lucem::(\w+)<(\d+), (\d+), (\d+)>")
 _TILE = re.compile(r"(tile_iter_kernel|tile_batch_kernel)<\d+, \d+>")
+# the compile-time build of the fused line pass (COL_FUSED, op 1), named by its length and columns
+_COL_STATIC = re.compile(r"col_fused_static_kernel<\d+, \d+>")
 CAL_BYTES = 1 << 30
 
 
@@ -173,6 +176,8 @@ def _per_pass(d: Path, counter: str, workload: str):
             groups.setdefault((m.group(1), int(m.group(4))), []).append(v)
         elif (t := _TILE.search(name)):
             groups.setdefault((t.group(1), None), []).append(v)
+        elif _COL_STATIC.search(name):
+            groups.setdefault(("col_pass_kernel", 1), []).append(v)
     out = {}
     for pname, key in PASS_KERNELS[workload].items():
         if key in groups:

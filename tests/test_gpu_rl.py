@@ -225,6 +225,31 @@ def test_plain3d_reduced_matches_oracle(gpu, oracle):
     assert e < RTOL_L2
 
 
+@pytest.mark.parametrize("depth,B,mode", [(128, 8, 0), (128, 12, 3), (130, 12, 1), (34, 32, 0), (66, 21, 4),
+                                          (98, 16, 2), (162, 10, 0), (226, 8, 0)])
+def test_plain3d_compile_time_z_pass(gpu, oracle, monkeypatch, depth, B, mode):
+    """The compile-time builds of the fused z pass (lines of depth + 30 = 64 to 256 points with a
+    31-slice PSF, B columns per workgroup) against the FP32 oracle and against the run-time build
+    of the same pass, which they must match to rounding."""
+    from lucem_amd import synth
+
+    rng = np.random.default_rng(40 + depth + B + mode)
+    y = rng.uniform(50, 150, (depth, 24, 20)).astype(np.float32)
+    psf = synth.gaussian_psf((4.0, 1.5, 1.5), (31, 5, 5)).astype(np.float32)
+    monkeypatch.setenv("LUCEM_AUTOTUNE", "0")
+    monkeypatch.setenv("LUCEM_TUNE", f"col_B={B}")
+    monkeypatch.setenv("LUCEM_COL_STATIC", "0")
+    print(f"z pass depth={depth} B={B} mode={mode}: run-time build", flush=True)
+    generic = _run(y, psf, 4, mode=mode)
+    monkeypatch.setenv("LUCEM_COL_STATIC", "1")
+    print(f"z pass depth={depth} B={B} mode={mode}: compile-time build", flush=True)
+    static = _run(y, psf, 4, mode=mode)
+    st, ref = oracle.richardson_lucy(y, psf, 4, mode=mode, clip=False, nd=True)
+    e, d = oracle.rel_l2(static, ref), oracle.rel_l2(static, generic)
+    print(f"z pass depth={depth} B={B} mode={mode}: rel-L2 vs FP32 oracle {e:.2e}, vs the run-time build {d:.2e}")
+    assert e < RTOL_L2 and d < 1e-6
+
+
 def test_plain3d_full_size_properties(gpu):
     """128x512x512 with the 31x15x15 PSF: a flat image is a fixed point of RL (Reflect), and
     the estimate of positive data stays positive and finite over 50 iterations."""
