@@ -334,7 +334,10 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         float best_ms = 1e30f;
         std::vector<FftPlan> plans;
         // variant 3: the compile-time COL_FUSED build, for the column counts it is built for
-        const bool try_static = mode == TUNE_PLAIN && g[0].d.op == COL_FUSED && !g[0].d.save;
+        // (not a candidate where LUCEM_COL_STATIC=0 switches it off)
+        const char* cs = std::getenv("LUCEM_COL_STATIC");
+        const bool try_static =
+            mode == TUNE_PLAIN && g[0].d.op == COL_FUSED && !g[0].d.save && !(cs && *cs && std::atoi(cs) == 0);
         for (int v = 0; v < 4; v++) {
             const bool stat = v == kVariantStatic;
             if (stat && !try_static) continue;
