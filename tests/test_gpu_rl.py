@@ -250,6 +250,23 @@ def test_plain3d_compile_time_z_pass(gpu, oracle, monkeypatch, depth, B, mode):
     assert e < RTOL_L2 and d < 1e-6
 
 
+def test_plain2d_whole_array_compile_time_y_pass(gpu, oracle, monkeypatch):
+    """The 2-D whole-array passes with the compile-time fused y pass (130 rows and a 31-row PSF:
+    160-point lines, 12 columns per workgroup) against the FP32 oracle."""
+    rng = np.random.default_rng(48)
+    y = rng.uniform(50, 150, (130, 90)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (31, 7)).astype(np.float32)
+    monkeypatch.setenv("LUCEM_RL_TILED", "0")
+    monkeypatch.setenv("LUCEM_AUTOTUNE", "0")
+    monkeypatch.setenv("LUCEM_TUNE", "col_B=12")
+    monkeypatch.setenv("LUCEM_COL_STATIC", "1")
+    got = _run(y, psf, 5)
+    st, ref = oracle.richardson_lucy(y, psf, 5, clip=False)
+    e = oracle.rel_l2(got, ref)
+    print(f"2-D whole-array, compile-time y pass: rel-L2 vs FP32 oracle {e:.2e}")
+    assert e < RTOL_L2
+
+
 def test_plain3d_full_size_properties(gpu):
     """128x512x512 with the 31x15x15 PSF: a flat image is a fixed point of RL (Reflect), and
     the estimate of positive data stays positive and finite over 50 iterations."""
