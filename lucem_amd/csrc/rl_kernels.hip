@@ -504,61 +504,85 @@ constexpr size_t col_static_lds_bytes() {
 }
 
 template <int L, int B>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_fused_static_kernel(
-    ColPass d) {
-    constexpr int T = kThreads;
-    constexpr int n = L * B;
-    constexpr int S = (n + T - 1) / T;
-    constexpr int RL = static_last_radix(L);
-    constexpr int nbf = (L / RL) * B;  // butterflies of the last pass
+struct ColStatic {
+    static constexpr int T = kThreads;
+    static constexpr int n = L * B;
+    static constexpr int S = (n + T - 1) / T;                   // gathered values per thread
+    static constexpr int RL = static_last_radix(L);             // radix of the last pass
+    static constexpr int nbf = (L / RL) * B;                    // butterflies of the last pass
     static_assert(n <= T * kElemsPerThread, "one block of lines per workgroup");
-    extern __shared__ float2 lds[];
-    float2* X = lds;
-    float2* Y = lds + tw_lds_offset(n);
-    float2* tw = Y + tw_lds_offset(n);
-    const int g = xcd_block(blockIdx.x, gridDim.x);
+};
+
+// Block g's padded lines (v) and the factors of the first round of its last forward pass (pre):
+// output t of butterfly (j, b) = threadIdx.x is element j + t L / RL of column b.
+template <int L, int B>
+__device__ __forceinline__ void col_static_load(const ColPass& d, int g, float2 (&v)[ColStatic<L, B>::S],
+                                                float2 (&pre)[ColStatic<L, B>::RL]) {
+    using K = ColStatic<L, B>;
     const int outer = g / d.nkb;
     const int kx0 = (g - outer * d.nkb) * B;
-    const int nl = min(B, d.Lc - kx0);  // columns of this block that hold data
+    const int nl = min(B, d.Lc - kx0);
     const int tid = threadIdx.x;
     const float2* src = d.src + outer * d.src_outer + kx0;
     const float2* otf = d.otf + outer * d.otf_outer + kx0;
-    float2 v[S];
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-        const int e = tid + s * T;
+    for (int s = 0; s < K::S; s++) {
+        const int e = tid + s * K::T;
         v[s] = make_float2(0.f, 0.f);
-        if (n % T == 0 || e < n) {
+        if (K::n % K::T == 0 || e < K::n) {
             const int i = e / B, b = e - i * B;
             const int sr = pad_index(i - d.left, d.n_src, d.mode);
             if (sr >= 0 && b < nl) v[s] = src[sr * d.src_axis + b];
         }
     }
-    // output t of the last pass's butterfly (j, b) = tid is element j + t L / RL of column b
-    float2 pre[RL];
-    {
-        const int j = tid / B, b = tid - j * B;
+    const int j = tid / B, b = tid - j * B;
 #pragma unroll
-        for (int t = 0; t < RL; t++) {
-            pre[t] = make_float2(0.f, 0.f);
-            if ((nbf >= T || tid < nbf) && b < nl) pre[t] = otf[(int64_t)(j + t * (L / RL)) * d.otf_axis + b];
-        }
+    for (int t = 0; t < K::RL; t++) {
+        pre[t] = make_float2(0.f, 0.f);
+        if ((K::nbf >= K::T || tid < K::nbf) && b < nl) pre[t] = otf[(int64_t)(j + t * (L / K::RL)) * d.otf_axis + b];
     }
-    for (int i = tid; i < L; i += T) tw[i] = d.tw[d.f.tw + i];
+}
+
+// Block g from its loaded values: into X, forward transform with the factors in its last pass,
+// inverse transform, store. Ends with the store; the caller synchronises before X is written again.
+template <int L, int B>
+__device__ __forceinline__ void col_static_run(const ColPass& d, int g, const float2 (&v)[ColStatic<L, B>::S],
+                                               const float2 (&pre)[ColStatic<L, B>::RL], float2* X, float2* Y,
+                                               const float2* tw) {
+    using K = ColStatic<L, B>;
+    const int outer = g / d.nkb;
+    const int kx0 = (g - outer * d.nkb) * B;
+    const int nl = min(B, d.Lc - kx0);
+    const int tid = threadIdx.x;
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-        const int e = tid + s * T;
-        if (n % T == 0 || e < n) X[e] = v[s];
+    for (int s = 0; s < K::S; s++) {
+        const int e = tid + s * K::T;
+        if (K::n % K::T == 0 || e < K::n) X[e] = v[s];
     }
     __syncthreads();
-    const OutScale ms{otf, (int)d.otf_axis, nl};
-    float2* sp = static_fft<L, B, T, false, false, true>(X, Y, tw, Divisor{B, 0u}, ms, pre);
-    const float2* r = static_fft<L, B, T, true>(sp, sp == X ? Y : X, tw);
+    const OutScale ms{d.otf + outer * d.otf_outer + kx0, (int)d.otf_axis, nl};
+    float2* sp = static_fft<L, B, K::T, false, false, true>(X, Y, tw, Divisor{B, 0u}, ms, pre);
+    const float2* r = static_fft<L, B, K::T, true>(sp, sp == X ? Y : X, tw);
     float2* dst = d.dst + outer * d.dst_outer + kx0;
-    for (int e = tid; e < d.n_dst * B; e += T) {
+    for (int e = tid; e < d.n_dst * B; e += K::T) {
         const int i = e / B, b = e - i * B;
         if (b < nl) dst[i * d.dst_axis + b] = r[e];
     }
+}
+
+template <int L, int B>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_fused_static_kernel(
+    ColPass d) {
+    using K = ColStatic<L, B>;
+    extern __shared__ float2 lds[];
+    float2* X = lds;
+    float2* Y = lds + tw_lds_offset(K::n);
+    float2* tw = Y + tw_lds_offset(K::n);
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    float2 v[K::S], pre[K::RL];
+    col_static_load<L, B>(d, g, v, pre);
+    for (int i = threadIdx.x; i < L; i += K::T) tw[i] = d.tw[d.f.tw + i];
+    col_static_run<L, B>(d, g, v, pre, X, Y, tw);
 }
 
 // The compile-time builds of COL_FUSED: (L, B) pairs, tried by the autotuner as variant 3.
