@@ -520,6 +520,15 @@ int lucem::col_static_choice(bool tuned, int L, int B, int wide) {
     return tuned ? 1 : 0;
 }
 
+// Whether the ratio and update row passes run their compile-time build: when the tuner chose it
+// (variant 3), or as LUCEM_ROW_STATIC=1 / 0 forces, wherever a build of (L, P) at 256 threads exists.
+int lucem::row_static_choice(bool tuned, int L, int P, int wide) {
+    if (wide != 0 || !row_static_available(L, P)) return 0;
+    const char* v = std::getenv("LUCEM_ROW_STATIC");
+    if (v && *v) return std::atoi(v) != 0 ? 1 : 0;
+    return tuned ? 1 : 0;
+}
+
 // Columns per workgroup of a line pass, any count up to 32, fewer only when the launch would
 // not give every CU one workgroup (wide workgroups beat more of them in the measurements). The
 // 512-thread shape serves lines too long for 8 columns in 4096 values.
@@ -753,6 +762,7 @@ RowPass lucem::make_row_desc(const Geometry& g, const ShapePolicy& sp, const flo
     r.eps = eps;
     r.init = 0.5f;
     r.f = make_plan(g.L[2], var);
+    r.stat = row_static_choice(var == kVariantStatic, g.L[2], r.P, r.wide);
     r.obs = obs;
     r.est = est;
     r.out = out;

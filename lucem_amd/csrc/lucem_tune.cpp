@@ -211,24 +211,39 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         ShapeChoice best{};
         float best_ms = 1e30f;
         std::vector<FftPlan> plans;
-        for (int v = 0; v < 3; v++) {
-            FftPlan f = make_plan(L, v);
+        // variant 3: the compile-time ratio / update build, for the pair counts it is built for
+        // (not a candidate where LUCEM_ROW_STATIC=0 switches it off)
+        const char* rs = std::getenv("LUCEM_ROW_STATIC");
+        const bool try_static = mode == TUNE_PLAIN && !(rs && *rs && std::atoi(rs) == 0);
+        for (int v = 0; v < 4; v++) {
+            const bool stat = v == kVariantStatic;
+            if (stat && !try_static) continue;
+            FftPlan f = make_plan(L, stat ? 0 : v);
             bool dup = false;
             for (const FftPlan& q : plans) dup = dup || same_plan(q, f);
-            if (dup) continue;
-            plans.push_back(f);
+            if (dup && !stat) continue;
+            if (!stat) plans.push_back(f);
             for (int w = 0; w < 2; w++) {
             // 256 threads, or 512 when that fits more row pairs into a workgroup
             if (row_wide >= 0 && w != row_wide) continue;
+            if (stat && w != 0) continue;
             const int pmax = std::max(1, std::min(16, (w ? kLdsComplexWide : kLdsComplex) / L));
             if (w == 1 && row_wide < 0 && pmax <= std::max(1, std::min(16, kLdsComplex / L))) continue;
-            for (int P : line_candidates(pmax, w == 0 ? g[0].P : pmax)) {
+            std::vector<int> cands;
+            if (stat) {
+                for (int P = 1; P <= pmax; P++)
+                    if (row_static_available(L, P)) cands.push_back(P);
+            } else {
+                cands = line_candidates(pmax, w == 0 ? g[0].P : pmax);
+            }
+            for (int P : cands) {
                 std::vector<RowPass> d = g;
                 std::vector<int2> items;
                 for (size_t i = 0; i < d.size(); i++) {
                     d[i].P = P;
                     d[i].wide = w;
                     d[i].f = f;
+                    d[i].stat = stat ? 1 : 0;
                     d[i].op = ROW_UPDATE;
                     d[i].first = 0;
                     d[i].last = 0;

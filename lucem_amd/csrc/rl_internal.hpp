@@ -88,6 +88,7 @@ struct RowPass {
                          // out[(z - own0[0]) * out_ps + (y - own0[1]) * out_rs + (x - own0[2])]
     int own0[3], own1[3];
     int64_t obs_rs, obs_ps, out_rs, out_ps;
+    int stat;            // ROW_RATIO / ROW_UPDATE: run the compile-time build of (f.L, P)
 };
 
 // Line pass along the y or z axis of the half spectra.
@@ -356,5 +357,7 @@ size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
 // A compile-time COL_FUSED build exists for lines of length L in blocks of B columns (256 threads).
 bool col_static_available(int L, int B);
+// A compile-time ROW_RATIO / ROW_UPDATE build exists for rows of length L, P pairs per workgroup.
+bool row_static_available(int L, int P);
 
 }  // namespace lucem

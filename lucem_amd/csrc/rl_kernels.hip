@@ -585,6 +585,168 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_
     col_static_run<L, B>(d, g, v, pre, X, Y, tw);
 }
 
+// ROW_RATIO / ROW_UPDATE with the row length L and the pair count P known at compile time (the
+// one-array form, no strided views). As in row_pass_body: the pairs' half spectra are merged into
+// complex lines, inverse-transformed, the pointwise step runs at each padded position's source
+// column, and the padded real pair is transformed forward and split. The pointwise operands (the
+// observed rows, or the estimate) are loaded with the spectra, before the inlined inverse
+// transform, which does not wait for them. LDS: two buffers of L x P values, then the twiddles.
+template <int L, int P>
+struct RowStatic {
+    static constexpr int T = kThreads;
+    static constexpr int n = L * P;
+    static constexpr int Lc = L / 2 + 1;
+    static constexpr int S = (n + T - 1) / T;            // padded positions per thread
+    static constexpr int KS = (P * Lc + T - 1) / T;      // spectrum bins per thread
+    static_assert(L % 2 == 0 && n <= T * kElemsPerThread, "row pairs of an even length");
+};
+
+template <int L, int P>
+constexpr size_t row_static_lds_bytes() {
+    return (size_t)(2 * tw_lds_offset(L * P) + L) * sizeof(float2);
+}
+
+template <int L, int P, int OP>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void row_static_kernel(
+    RowPass d) {
+    using K = RowStatic<L, P>;
+    static_assert(OP == ROW_RATIO || OP == ROW_UPDATE, "the iteration's two row passes");
+    extern __shared__ float2 lds[];
+    float2* X = lds;
+    float2* Y = lds + tw_lds_offset(K::n);
+    float2* tw = Y + tw_lds_offset(K::n);
+    const int tid = threadIdx.x;
+    const int pair0 = blockIdx.x * P;
+    const int nx = d.nx;
+    float2 av[K::KS], bv[K::KS];
+#pragma unroll
+    for (int s = 0; s < K::KS; s++) {
+        const int e = tid + s * K::T;
+        av[s] = make_float2(0.f, 0.f);
+        bv[s] = make_float2(0.f, 0.f);
+        if (e < P * K::Lc) {
+            const int b = e / K::Lc, k = e - b * K::Lc;
+            const int ra = 2 * (pair0 + b), rb = ra + 1;
+            if (ra < d.rows) av[s] = d.spec[(int64_t)ra * d.pitch + k];
+            if (rb < d.rows) bv[s] = d.spec[(int64_t)rb * d.pitch + k];
+        }
+    }
+    float2 val[K::S];
+    int srcs[K::S];
+#pragma unroll
+    for (int s = 0; s < K::S; s++) {
+        const int e = tid + s * K::T;
+        srcs[s] = -1;
+        val[s] = make_float2(0.f, 0.f);
+        if (K::n % K::T == 0 || e < K::n) {
+            const int b = e / L, i = e - b * L;
+            const int src = pad_index(i - d.left, nx, d.mode);
+            srcs[s] = src;
+            if (src >= 0) {
+                const int ra = 2 * (pair0 + b), rb = ra + 1;
+                if constexpr (OP == ROW_RATIO) {
+                    if (ra < d.rows) val[s].x = d.obs[(int64_t)ra * nx + src];
+                    if (rb < d.rows) val[s].y = d.obs[(int64_t)rb * nx + src];
+                } else if (!d.first) {
+                    if (ra < d.rows) val[s].x = d.est[(int64_t)ra * nx + src];
+                    if (rb < d.rows) val[s].y = d.est[(int64_t)rb * nx + src];
+                } else {
+                    val[s] = make_float2(d.init, d.init);
+                }
+            }
+        }
+    }
+    for (int i = tid; i < L; i += K::T) tw[i] = d.tw[d.f.tw + i];
+#pragma unroll
+    for (int s = 0; s < K::KS; s++) {
+        const int e = tid + s * K::T;
+        if (e < P * K::Lc) {
+            const int b = e / K::Lc, k = e - b * K::Lc;
+            float2 A = av[s], Bv = bv[s];
+            if (k == 0 || 2 * k == L) {
+                A.y = 0.f;
+                Bv.y = 0.f;
+            }
+            X[k * P + b] = make_float2(A.x - Bv.y, A.y + Bv.x);
+            const int km = L - k;
+            if (k > 0 && km >= K::Lc) X[km * P + b] = make_float2(A.x + Bv.y, Bv.x - A.y);
+        }
+    }
+    __syncthreads();
+    float2* r = static_fft<L, P, K::T, true>(X, Y, tw);
+    float2* W = r == X ? Y : X;  // free since the last inverse pass
+#pragma unroll
+    for (int s = 0; s < K::S; s++) {
+        const int e = tid + s * K::T;
+        if (K::n % K::T == 0 || e < K::n) {
+            const int b = e / L, i = e - b * L;
+            const int src = srcs[s];
+            if (src >= 0) {
+                const float2 c = r[src * P + b];
+                if constexpr (OP == ROW_RATIO) {
+                    const float da = fabsf(c.x) < d.eps ? d.eps : c.x;
+                    const float db = fabsf(c.y) < d.eps ? d.eps : c.y;
+                    val[s] = make_float2(div_fast(val[s].x, da), div_fast(val[s].y, db));
+                } else {
+                    float oa = val[s].x * c.x, ob = val[s].y * c.y;
+                    if (d.last && d.clip) {
+                        oa = fminf(fmaxf(oa, -1.f), 1.f);
+                        ob = fminf(fmaxf(ob, -1.f), 1.f);
+                    }
+                    val[s] = make_float2(oa, ob);
+                    if (i == src + d.left) {
+                        const int x = i - d.left;
+                        const int ra = 2 * (pair0 + b), rb = ra + 1;
+                        float* dst = d.last ? d.out : d.est;
+                        if (ra < d.rows) dst[(int64_t)ra * nx + x] = val[s].x;
+                        if (rb < d.rows) dst[(int64_t)rb * nx + x] = val[s].y;
+                    }
+                }
+            }
+            W[i * P + b] = val[s];
+        }
+    }
+    if constexpr (OP == ROW_UPDATE) {
+        if (d.last) return;  // uniform: the whole workgroup leaves
+    }
+    __syncthreads();
+    const float2* f = static_fft<L, P, K::T, false>(W, r, tw);
+    for (int e = tid; e < P * K::Lc; e += K::T) {
+        const int b = e / K::Lc, k = e - b * K::Lc;
+        const int ra = 2 * (pair0 + b), rb = ra + 1;
+        const float2 zk = f[k * P + b];
+        const float2 zm = f[(k == 0 ? 0 : L - k) * P + b];
+        if (ra < d.rows) d.spec[(int64_t)ra * d.pitch + k] = make_float2(zk.x + zm.x, zk.y - zm.y);
+        if (rb < d.rows) d.spec[(int64_t)rb * d.pitch + k] = make_float2(zk.y + zm.y, zm.x - zk.x);
+    }
+}
+
+// The compile-time builds of ROW_RATIO / ROW_UPDATE: (L, P) pairs, tried by the autotuner as variant 3.
+#define LUCEM_ROW_STATIC_SHAPES(X) X(270, 4) X(270, 6) X(280, 6) X(540, 2) X(540, 3) X(1080, 1)
+
+bool row_static_available(int L, int P) {
+#define LUCEM_ROW_STATIC_AVAIL(l, p) \
+    if (L == l && P == p) return true;
+    LUCEM_ROW_STATIC_SHAPES(LUCEM_ROW_STATIC_AVAIL)
+#undef LUCEM_ROW_STATIC_AVAIL
+    return false;
+}
+
+static int launch_row_static(const RowPass& r, int blocks, hipStream_t st) {
+#define LUCEM_ROW_STATIC_LAUNCH(l, p)                                                                  \
+    if (r.f.L == l && r.P == p) {                                                                      \
+        const size_t lds = row_static_lds_bytes<l, p>();                                              \
+        if (r.op == ROW_RATIO)                                                                         \
+            hipLaunchKernelGGL((row_static_kernel<l, p, ROW_RATIO>), dim3(blocks), dim3(kThreads), lds, st, r); \
+        else                                                                                           \
+            hipLaunchKernelGGL((row_static_kernel<l, p, ROW_UPDATE>), dim3(blocks), dim3(kThreads), lds, st, r); \
+        return (int)hipGetLastError();                                                                 \
+    }
+    LUCEM_ROW_STATIC_SHAPES(LUCEM_ROW_STATIC_LAUNCH)
+#undef LUCEM_ROW_STATIC_LAUNCH
+    return (int)hipErrorInvalidValue;
+}
+
 // The compile-time builds of COL_FUSED: (L, B) pairs, tried by the autotuner as variant 3.
 #define LUCEM_COL_STATIC_SHAPES(X) \
     X(64, 32) X(96, 21) X(128, 16) X(160, 8) X(160, 12) X(192, 10) X(256, 8)
@@ -673,6 +835,8 @@ static int launch_row_e(const RowPass& p, int blocks, void* stream) {
 }
 
 int launch_row_pass(const RowPass& p, int blocks, void* stream) {
+    if (p.stat && p.wide == 0 && (p.op == ROW_RATIO || p.op == ROW_UPDATE))
+        return launch_row_static(p, blocks, (hipStream_t)stream);
     switch (p.wide) {
         case 2: return launch_row_e<kThreadsXL, kElemsPerThread>(p, blocks, stream);
         case 1: return launch_row_e<kThreadsWide, kElemsPerThread>(p, blocks, stream);

@@ -23,7 +23,8 @@ rocprofv3 names kernels by template (``col_pass_kernel<256, 16, 1>``); the op id
 template argument, and each RL pass of a workload is one (kernel family, op) pair. The tiled 2-D
 iteration is one kernel family (``tile_iter_kernel<L0, L1>``, and ``tile_batch_kernel<L0, L1>`` for
 the batched form) whatever its lengths, and so is the compile-time build of the fused line pass
-(``col_fused_static_kernel<L, B>``, counted as ``COL_FUSED``).
+(``col_fused_static_kernel<L, B>``, counted as ``COL_FUSED``); the compile-time row passes
+(``row_static_kernel<L, P, OP>``) count as their ``row_pass_kernel`` op.
 """
 
 from __future__ import annotations
@@ -173,7 +174,9 @@ def _per_pass(d: Path, counter: str, workload: str):
     for name, v in _rows(d, counter):
         m = _NAME.search(name)
         if m:
-            groups.setdefault((m.group(1), int(m.group(4))), []).append(v)
+            # the compile-time row build (row_static_kernel<L, P, OP>) counts as its row pass
+            fam = "row_pass_kernel" if m.group(1) == "row_static_kernel" else m.group(1)
+            groups.setdefault((fam, int(m.group(4))), []).append(v)
         elif (t := _TILE.search(name)):
             groups.setdefault((t.group(1), None), []).append(v)
         elif _COL_STATIC.search(name):

@@ -250,6 +250,53 @@ def test_plain3d_compile_time_z_pass(gpu, oracle, monkeypatch, depth, B, mode):
     assert e < RTOL_L2 and d < 1e-6
 
 
+@pytest.mark.parametrize("P,mode,clip", [(2, 0, False), (3, 1, False), (3, 3, True), (2, 4, False), (3, 2, True)])
+def test_rows_compile_time_pass(gpu, oracle, monkeypatch, P, mode, clip):
+    """The compile-time ratio and update row passes (540-point rows: 526 columns and a 15-column
+    PSF, P row pairs per workgroup, an odd row count) in 3-D and in the 2-D whole-array path,
+    against the FP32 oracle and against the run-time build of the same passes."""
+    rng = np.random.default_rng(60 + P + mode)
+    lo, hi = (0.2, 1.6) if clip else (50, 150)
+    cases = [(rng.uniform(lo, hi, (3, 7, 526)).astype(np.float32), rng.uniform(0.1, 1.0, (3, 3, 15)).astype(np.float32)),
+             (rng.uniform(lo, hi, (21, 526)).astype(np.float32), rng.uniform(0.1, 1.0, (5, 15)).astype(np.float32))]
+    monkeypatch.setenv("LUCEM_RL_TILED", "0")
+    monkeypatch.setenv("LUCEM_AUTOTUNE", "0")
+    monkeypatch.setenv("LUCEM_TUNE", f"row_P={P}")
+    for y, psf in cases:
+        monkeypatch.setenv("LUCEM_ROW_STATIC", "0")
+        print(f"rows P={P} mode={mode} clip={clip} {y.shape}: run-time build", flush=True)
+        generic = _run(y, psf, 4, mode=mode, clip=clip)
+        monkeypatch.setenv("LUCEM_ROW_STATIC", "1")
+        print(f"rows P={P} mode={mode} clip={clip} {y.shape}: compile-time build", flush=True)
+        static = _run(y, psf, 4, mode=mode, clip=clip)
+        st, ref = oracle.richardson_lucy(y, psf, 4, mode=mode, clip=clip, nd=y.ndim == 3)
+        e, dg = oracle.rel_l2(static, ref), oracle.rel_l2(static, generic)
+        print(f"rows P={P} mode={mode} clip={clip} {y.shape}: rel-L2 vs FP32 oracle {e:.2e}, vs the run-time build {dg:.2e}")
+        assert e < RTOL_L2 and dg < 1e-6
+
+
+@pytest.mark.parametrize("shape,kshape,P", [((3, 5, 256), (3, 3, 15), 6), ((4, 7, 256), (1, 3, 15), 4),
+                                            ((9, 266), (3, 15), 6), ((2, 3, 1066), (3, 3, 15), 1)])
+def test_rows_compile_time_pass_lengths(gpu, oracle, monkeypatch, shape, kshape, P):
+    """The other compile-time row builds (270-, 280- and 1080-point rows) against the FP32 oracle
+    and the run-time build."""
+    rng = np.random.default_rng(70 + P + len(shape))
+    y = rng.uniform(50, 150, shape).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, kshape).astype(np.float32)
+    monkeypatch.setenv("LUCEM_RL_TILED", "0")
+    monkeypatch.setenv("LUCEM_AUTOTUNE", "0")
+    monkeypatch.setenv("LUCEM_TUNE", f"row_P={P}")
+    monkeypatch.setenv("LUCEM_ROW_STATIC", "0")
+    generic = _run(y, psf, 4, mode=0)
+    monkeypatch.setenv("LUCEM_ROW_STATIC", "1")
+    print(f"rows {shape} P={P}: compile-time build", flush=True)
+    static = _run(y, psf, 4, mode=0)
+    st, ref = oracle.richardson_lucy(y, psf, 4, mode=0, clip=False, nd=y.ndim == 3)
+    e, dg = oracle.rel_l2(static, ref), oracle.rel_l2(static, generic)
+    print(f"rows {shape} P={P}: rel-L2 vs FP32 oracle {e:.2e}, vs the run-time build {dg:.2e}")
+    assert e < RTOL_L2 and dg < 1e-6
+
+
 def test_plain2d_whole_array_compile_time_y_pass(gpu, oracle, monkeypatch):
     """The 2-D whole-array passes with the compile-time fused y pass (130 rows and a 31-row PSF:
     160-point lines, 12 columns per workgroup) against the FP32 oracle."""
