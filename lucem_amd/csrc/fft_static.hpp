@@ -151,4 +151,68 @@ __device__ __forceinline__ float2* static_fft(float2* a, float2* b, const float2
     }
 }
 
+// In-place form of the same passes, for a workgroup that holds one buffer of B lines: each pass
+// reads every butterfly's inputs into registers, computes them, and writes after a barrier (two
+// barriers per pass). Returns nothing: the result is in `a`.
+template <int N, int B, int T, bool INV, int Ns, int R>
+__device__ __forceinline__ void static_pass_inplace(float2* __restrict__ a, const float2* __restrict__ tw) {
+    constexpr int nbf = (N / R) * B;
+    constexpr int S = (nbf + T - 1) / T;
+    constexpr int tws = N / (Ns * R);
+    typedef __attribute__((address_space(3))) lds_vec2* lds_t;
+    typedef const __attribute__((address_space(3))) lds_vec2* clds_t;
+    const lds_t ap = (lds_t)(size_t)a;
+    const clds_t tp = (clds_t)(size_t)tw;
+    float2 v[S][R];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u = (int)threadIdx.x + s * T;
+        if (nbf % T == 0 || u < nbf) {
+            const int j = u / B;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                const lds_vec2 x = ap[u + t * nbf];
+                v[s][t] = make_float2(x.x, x.y);
+            }
+            if constexpr (Ns > 1) {
+                const int step = (j % Ns) * tws;
+#pragma unroll
+                for (int t = 1; t < R; t++) {
+                    const lds_vec2 x = tp[t * step];
+                    v[s][t] = cmul(v[s][t], make_float2(x.x, INV ? -x.y : x.y));
+                }
+            }
+            StaticButterfly<R, INV>::run(v[s]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int u = (int)threadIdx.x + s * T;
+        if (nbf % T == 0 || u < nbf) {
+            const int j = u / B;
+            const int k = Ns > 1 ? j % Ns : 0;
+            const int o = u + (j - k) * (R - 1) * B;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                lds_vec2 x;
+                x.x = v[s][t].x;
+                x.y = v[s][t].y;
+                ap[o + t * Ns * B] = x;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int B, int T, bool INV, int Ns = 1>
+__device__ __forceinline__ void static_fft_inplace(float2* a, const float2* tw) {
+    static_assert(static_len_ok(N), "transform length must be 7-smooth");
+    if constexpr (Ns < N) {
+        constexpr int R = static_radix(N / Ns);
+        static_pass_inplace<N, B, T, INV, Ns, R>(a, tw);
+        static_fft_inplace<N, B, T, INV, Ns * R>(a, tw);
+    }
+}
+
 }  // namespace lucem

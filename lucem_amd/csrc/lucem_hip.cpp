@@ -520,6 +520,15 @@ int lucem::col_static_choice(bool tuned, int L, int B, int wide) {
     return tuned ? 1 : 0;
 }
 
+// The same for the y passes (COL_FWD / COL_INV) of 3-D arrays, built at 512 threads; forced by
+// LUCEM_COL_STATIC as well.
+int lucem::line_static_choice(bool tuned, int L, int B, int wide) {
+    if (wide != 1 || !line_static_available(L, B)) return 0;
+    const char* v = std::getenv("LUCEM_COL_STATIC");
+    if (v && *v) return std::atoi(v) != 0 ? 1 : 0;
+    return tuned ? 1 : 0;
+}
+
 // Whether the ratio and update row passes run their compile-time build: when the tuner chose it
 // (variant 3), or as LUCEM_ROW_STATIC=1 / 0 forces, wherever a build of (L, P) at 256 threads exists.
 int lucem::row_static_choice(bool tuned, int L, int P, int wide) {
@@ -706,6 +715,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
     y.B = choose_B(g.L[1], g.Lc, g.n[0], SHAPE_COL_FWD, sp, &y.wide, &var);
     y.f = make_plan(g.L[1], var);
     y.nkb = (g.Lc + y.B - 1) / y.B;
+    y.stat = line_static_choice(var == kVariantStatic, g.L[1], y.B, y.wide);
     out[0] = y;
     blocks[0] = g.n[0] * y.nkb;
 
@@ -736,6 +746,7 @@ int lucem::make_corr_descs(const Geometry& g, const ShapePolicy& sp, float2* sa,
     yi.B = choose_B(g.L[1], g.Lc, g.n[0], SHAPE_COL_INV, sp, &yi.wide, &var);
     yi.f = make_plan(g.L[1], var);
     yi.nkb = (g.Lc + yi.B - 1) / yi.B;
+    yi.stat = line_static_choice(var == kVariantStatic, g.L[1], yi.B, yi.wide);
     yi.n_src = g.L[1];
     yi.n_dst = g.n[1];
     yi.src = sb;

@@ -120,6 +120,7 @@ FftPlan make_plan(int L, int variant = 0);
 constexpr int kVariantStatic = 3;
 int col_static_choice(bool tuned, int L, int B, int wide);
 int row_static_choice(bool tuned, int L, int P, int wide);
+int line_static_choice(bool tuned, int L, int B, int wide);
 
 // How the descriptor builders pick launch shapes: the device size, the parallelism of the launch
 // a descriptor joins (row pairs and lines of all arrays launched together; 0 = the array's own)

@@ -351,8 +351,10 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
         // variant 3: the compile-time COL_FUSED build, for the column counts it is built for
         // (not a candidate where LUCEM_COL_STATIC=0 switches it off)
         const char* cs = std::getenv("LUCEM_COL_STATIC");
-        const bool try_static =
-            mode == TUNE_PLAIN && g[0].d.op == COL_FUSED && !g[0].d.save && !(cs && *cs && std::atoi(cs) == 0);
+        const int op0 = g[0].d.op;
+        const bool try_static = mode == TUNE_PLAIN && (op0 == COL_FUSED || op0 == COL_FWD || op0 == COL_INV) &&
+                                !g[0].d.save && !(cs && *cs && std::atoi(cs) == 0);
+        const int static_tier = op0 == COL_FUSED ? 0 : 1;  // the thread count the builds have
         for (int v = 0; v < 4; v++) {
             const bool stat = v == kVariantStatic;
             if (stat && !try_static) continue;
@@ -363,7 +365,7 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
             if (!stat) plans.push_back(f);
             for (int w = 0; w < 2; w++) {
                 if (fixed_wide && w != kind_wide[kind]) continue;
-                if (stat && w != 0) continue;
+                if (stat && w != static_tier) continue;
                 if (w == 0 && L > kLdsComplex) continue;  // the line does not fit 256 threads
                 const int cap = w ? kLdsComplexWide : kLdsComplex;
                 const int bmax = std::max(1, std::min(32, cap / L));
@@ -372,7 +374,8 @@ lucem_status lucem::tune_shapes(lucem_client* c, const std::vector<RowPass>& row
                 std::vector<int> cands;
                 if (stat) {
                     for (int B = 1; B <= bmax; B++)
-                        if (col_static_available(L, B)) cands.push_back(B);
+                        if (op0 == COL_FUSED ? col_static_available(L, B) : line_static_available(L, B))
+                            cands.push_back(B);
                 } else {
                     cands = line_candidates(bmax, g[0].d.wide == w ? g[0].d.B : bmax);
                 }

@@ -357,6 +357,8 @@ size_t row_pass_lds_bytes(int L, int P);
 size_t col_pass_lds_bytes(int L, int B);
 // A compile-time COL_FUSED build exists for lines of length L in blocks of B columns (256 threads).
 bool col_static_available(int L, int B);
+// A compile-time COL_FWD / COL_INV build exists for lines of length L in blocks of B columns (512 threads).
+bool line_static_available(int L, int B);
 // A compile-time ROW_RATIO / ROW_UPDATE build exists for rows of length L, P pairs per workgroup.
 bool row_static_available(int L, int P);
 

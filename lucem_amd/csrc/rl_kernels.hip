@@ -585,6 +585,80 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_
     col_static_run<L, B>(d, g, v, pre, X, Y, tw);
 }
 
+// COL_FWD (gather with the pad rule, forward transform, all L rows stored) and COL_INV (L rows,
+// inverse transform, the first n_dst stored) with the line length and column count known at
+// compile time, on 512 threads and one LDS buffer (in-place passes), the launch shape the
+// run-time y passes take; same descriptor, grid and results (to rounding).
+template <int L, int B, int OP>
+__global__ __launch_bounds__(kThreadsWide) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_line_static_kernel(
+    ColPass d) {
+    constexpr int T = kThreadsWide;
+    constexpr int n = L * B;
+    constexpr int S = (n + T - 1) / T;
+    static_assert(OP == COL_FWD || OP == COL_INV, "the plain line passes");
+    static_assert(n <= T * kElemsPerThread, "one block of lines per workgroup");
+    extern __shared__ float2 lds[];
+    float2* Z = lds;
+    float2* tw = lds + tw_lds_offset(n);
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int outer = g / d.nkb;
+    const int kx0 = (g - outer * d.nkb) * B;
+    const int nl = min(B, d.Lc - kx0);
+    const int tid = threadIdx.x;
+    const float2* src = d.src + outer * d.src_outer + kx0;
+    float2 v[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        v[s] = make_float2(0.f, 0.f);
+        if (n % T == 0 || e < n) {
+            const int i = e / B, b = e - i * B;
+            const int sr = OP == COL_INV ? i : pad_index(i - d.left, d.n_src, d.mode);
+            if (sr >= 0 && b < nl) v[s] = src[sr * d.src_axis + b];
+        }
+    }
+    for (int i = tid; i < L; i += T) tw[i] = d.tw[d.f.tw + i];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        if (n % T == 0 || e < n) Z[e] = v[s];
+    }
+    __syncthreads();
+    static_fft_inplace<L, B, T, OP == COL_INV>(Z, tw);
+    float2* dst = d.dst + outer * d.dst_outer + kx0;
+    for (int e = tid; e < d.n_dst * B; e += T) {
+        const int i = e / B, b = e - i * B;
+        if (b < nl) dst[i * d.dst_axis + b] = Z[e];
+    }
+}
+
+// The compile-time builds of COL_FWD / COL_INV at 512 threads: (L, B) pairs, tried by the autotuner
+// as variant 3.
+#define LUCEM_LINE_STATIC_SHAPES(X) X(540, 12) X(540, 14)
+
+bool line_static_available(int L, int B) {
+#define LUCEM_LINE_STATIC_AVAIL(l, b) \
+    if (L == l && B == b) return true;
+    LUCEM_LINE_STATIC_SHAPES(LUCEM_LINE_STATIC_AVAIL)
+#undef LUCEM_LINE_STATIC_AVAIL
+    return false;
+}
+
+static int launch_line_static(const ColPass& p, int blocks, hipStream_t st) {
+#define LUCEM_LINE_STATIC_LAUNCH(l, b)                                                                 \
+    if (p.f.L == l && p.B == b) {                                                                      \
+        const size_t lds = (size_t)(tw_lds_offset(l * b) + l) * sizeof(float2);                       \
+        if (p.op == COL_FWD)                                                                           \
+            hipLaunchKernelGGL((col_line_static_kernel<l, b, COL_FWD>), dim3(blocks), dim3(kThreadsWide), lds, st, p); \
+        else                                                                                           \
+            hipLaunchKernelGGL((col_line_static_kernel<l, b, COL_INV>), dim3(blocks), dim3(kThreadsWide), lds, st, p); \
+        return (int)hipGetLastError();                                                                 \
+    }
+    LUCEM_LINE_STATIC_SHAPES(LUCEM_LINE_STATIC_LAUNCH)
+#undef LUCEM_LINE_STATIC_LAUNCH
+    return (int)hipErrorInvalidValue;
+}
+
 // ROW_RATIO / ROW_UPDATE with the row length L and the pair count P known at compile time (the
 // one-array form, no strided views). As in row_pass_body: the pairs' half spectra are merged into
 // complex lines, inverse-transformed, the pointwise step runs at each padded position's source
@@ -868,6 +942,8 @@ static int launch_col_e(const ColPass& p, int blocks, void* stream) {
 
 int launch_col_pass(const ColPass& p, int blocks, void* stream) {
     if (p.stat && p.op == COL_FUSED && !p.save && p.wide == 0) return launch_col_static(p, blocks, (hipStream_t)stream);
+    if (p.stat && (p.op == COL_FWD || p.op == COL_INV) && p.wide == 1)
+        return launch_line_static(p, blocks, (hipStream_t)stream);
     switch (p.wide) {
         case 2: return launch_col_e<kThreadsXL, kElemsPerThread>(p, blocks, stream);
         case 1: return launch_col_e<kThreadsWide, kElemsPerThread>(p, blocks, stream);

@@ -297,6 +297,27 @@ def test_rows_compile_time_pass_lengths(gpu, oracle, monkeypatch, shape, kshape,
     assert e < RTOL_L2 and dg < 1e-6
 
 
+@pytest.mark.parametrize("B,mode", [(14, 0), (12, 1), (14, 4), (12, 3)])
+def test_plain3d_compile_time_y_passes(gpu, oracle, monkeypatch, B, mode):
+    """The compile-time forward and inverse y passes of 3-D arrays (540-point lines: 526 rows and a
+    15-row PSF, B columns per 512-thread workgroup) against the FP32 oracle and against the
+    run-time builds."""
+    rng = np.random.default_rng(80 + B + mode)
+    y = rng.uniform(50, 150, (3, 526, 21)).astype(np.float32)
+    psf = rng.uniform(0.1, 1.0, (3, 15, 3)).astype(np.float32)
+    monkeypatch.setenv("LUCEM_AUTOTUNE", "0")
+    monkeypatch.setenv("LUCEM_TUNE", f"col_B={B},col_wide=1")
+    monkeypatch.setenv("LUCEM_COL_STATIC", "0")
+    generic = _run(y, psf, 4, mode=mode)
+    monkeypatch.setenv("LUCEM_COL_STATIC", "1")
+    print(f"y passes B={B} mode={mode}: compile-time build", flush=True)
+    static = _run(y, psf, 4, mode=mode)
+    st, ref = oracle.richardson_lucy(y, psf, 4, mode=mode, clip=False, nd=True)
+    e, dg = oracle.rel_l2(static, ref), oracle.rel_l2(static, generic)
+    print(f"y passes B={B} mode={mode}: rel-L2 vs FP32 oracle {e:.2e}, vs the run-time build {dg:.2e}")
+    assert e < RTOL_L2 and dg < 1e-6
+
+
 def test_plain2d_whole_array_compile_time_y_pass(gpu, oracle, monkeypatch):
     """The 2-D whole-array passes with the compile-time fused y pass (130 rows and a 31-row PSF:
     160-point lines, 12 columns per workgroup) against the FP32 oracle."""
