@@ -24,7 +24,8 @@ template argument, and each RL pass of a workload is one (kernel family, op) pai
 iteration is one kernel family (``tile_iter_kernel<L0, L1>``, and ``tile_batch_kernel<L0, L1>`` for
 the batched form) whatever its lengths, and so is the compile-time build of the fused line pass
 (``col_fused_static_kernel<L, B>``, counted as ``COL_FUSED``); the compile-time row passes
-(``row_static_kernel<L, P, OP>``) count as their ``row_pass_kernel`` op.
+(``row_static_kernel<L, P, OP>``) and y passes (``col_line_static_kernel<L, B, OP>``) count as their
+``row_pass_kernel`` / ``col_pass_kernel`` op.
 """
 
 from __future__ import annotations
@@ -174,8 +175,10 @@ def _per_pass(d: Path, counter: str, workload: str):
     for name, v in _rows(d, counter):
         m = _NAME.search(name)
         if m:
-            # the compile-time row build (row_static_kernel<L, P, OP>) counts as its row pass
-            fam = "row_pass_kernel" if m.group(1) == "row_static_kernel" else m.group(1)
+            # the compile-time builds (row_static_kernel<L, P, OP>, col_line_static_kernel<L, B, OP>)
+            # count as their run-time pass of the same op
+            fam = {"row_static_kernel": "row_pass_kernel", "col_line_static_kernel": "col_pass_kernel"}.get(
+                m.group(1), m.group(1))
             groups.setdefault((fam, int(m.group(4))), []).append(v)
         elif (t := _TILE.search(name)):
             groups.setdefault((t.group(1), None), []).append(v)
