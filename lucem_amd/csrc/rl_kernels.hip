@@ -821,6 +821,65 @@ static int launch_row_static(const RowPass& r, int blocks, hipStream_t st) {
     return (int)hipErrorInvalidValue;
 }
 
+// COL_FUSED on one LDS buffer (in-place compile-time passes), for wider column blocks than two
+// buffers allow: the transfer-function values of this thread's elements are loaded with the gather,
+// in flight during the forward transform, and multiplied in a sweep after it.
+template <int L, int B>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LUCEM_WAVES_PER_EU))) void col_fused_inplace_kernel(
+    ColPass d) {
+    constexpr int T = kThreads;
+    constexpr int n = L * B;
+    constexpr int S = (n + T - 1) / T;
+    static_assert(n <= T * kElemsPerThread, "one block of lines per workgroup");
+    extern __shared__ float2 lds[];
+    float2* Z = lds;
+    float2* tw = lds + tw_lds_offset(n);
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int outer = g / d.nkb;
+    const int kx0 = (g - outer * d.nkb) * B;
+    const int nl = min(B, d.Lc - kx0);
+    const int tid = threadIdx.x;
+    const float2* src = d.src + outer * d.src_outer + kx0;
+    const float2* otf = d.otf + outer * d.otf_outer + kx0;
+    float2 v[S], w[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        v[s] = make_float2(0.f, 0.f);
+        w[s] = make_float2(0.f, 0.f);
+        if (n % T == 0 || e < n) {
+            const int i = e / B, b = e - i * B;
+            const int sr = pad_index(i - d.left, d.n_src, d.mode);
+            if (b < nl) {
+                if (sr >= 0) v[s] = src[sr * d.src_axis + b];
+                w[s] = otf[(int64_t)i * d.otf_axis + b];
+            }
+        }
+    }
+    for (int i = tid; i < L; i += T) tw[i] = d.tw[d.f.tw + i];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        if (n % T == 0 || e < n) Z[e] = v[s];
+    }
+    __syncthreads();
+    static_fft_inplace<L, B, T, false>(Z, tw);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int e = tid + s * T;
+        if (n % T == 0 || e < n) Z[e] = cmul(Z[e], w[s]);
+    }
+    __syncthreads();
+    static_fft_inplace<L, B, T, true>(Z, tw);
+    float2* dst = d.dst + outer * d.dst_outer + kx0;
+    for (int e = tid; e < d.n_dst * B; e += T) {
+        const int i = e / B, b = e - i * B;
+        if (b < nl) dst[i * d.dst_axis + b] = Z[e];
+    }
+}
+
+#define LUCEM_COL_INPLACE_SHAPES(X) X(160, 16) X(160, 20) X(160, 24)
+
 // The compile-time builds of COL_FUSED: (L, B) pairs, tried by the autotuner as variant 3.
 #define LUCEM_COL_STATIC_SHAPES(X) \
     X(64, 32) X(96, 21) X(128, 16) X(160, 8) X(160, 12) X(192, 10) X(256, 8)
@@ -829,6 +888,7 @@ bool col_static_available(int L, int B) {
 #define LUCEM_COL_STATIC_AVAIL(l, b) \
     if (L == l && B == b) return true;
     LUCEM_COL_STATIC_SHAPES(LUCEM_COL_STATIC_AVAIL)
+    LUCEM_COL_INPLACE_SHAPES(LUCEM_COL_STATIC_AVAIL)
 #undef LUCEM_COL_STATIC_AVAIL
     return false;
 }
@@ -842,6 +902,14 @@ static int launch_col_static(const ColPass& p, int blocks, hipStream_t st) {
     }
     LUCEM_COL_STATIC_SHAPES(LUCEM_COL_STATIC_LAUNCH)
 #undef LUCEM_COL_STATIC_LAUNCH
+#define LUCEM_COL_INPLACE_LAUNCH(l, b)                                                                \
+    if (p.f.L == l && p.B == b) {                                                                     \
+        const size_t lds = (size_t)(tw_lds_offset(l * b) + l) * sizeof(float2);                      \
+        hipLaunchKernelGGL((col_fused_inplace_kernel<l, b>), dim3(blocks), dim3(kThreads), lds, st, p); \
+        return (int)hipGetLastError();                                                                \
+    }
+    LUCEM_COL_INPLACE_SHAPES(LUCEM_COL_INPLACE_LAUNCH)
+#undef LUCEM_COL_INPLACE_LAUNCH
     return (int)hipErrorInvalidValue;
 }
 

@@ -226,11 +226,13 @@ def test_plain3d_reduced_matches_oracle(gpu, oracle):
 
 
 @pytest.mark.parametrize("depth,B,mode", [(128, 8, 0), (128, 12, 3), (130, 12, 1), (34, 32, 0), (66, 21, 4),
-                                          (98, 16, 2), (162, 10, 0), (226, 8, 0)])
+                                          (98, 16, 2), (162, 10, 0), (226, 8, 0), (130, 16, 0), (128, 20, 3),
+                                          (128, 24, 1)])
 def test_plain3d_compile_time_z_pass(gpu, oracle, monkeypatch, depth, B, mode):
     """The compile-time builds of the fused z pass (lines of depth + 30 = 64 to 256 points with a
-    31-slice PSF, B columns per workgroup) against the FP32 oracle and against the run-time build
-    of the same pass, which they must match to rounding."""
+    31-slice PSF, B columns per workgroup; two LDS buffers, or one for 16 to 24 columns of 160
+    points) against the FP32 oracle and against the run-time build of the same pass, which they
+    must match to rounding."""
     from lucem_amd import synth
 
     rng = np.random.default_rng(40 + depth + B + mode)
