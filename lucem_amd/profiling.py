@@ -23,7 +23,7 @@ rocprofv3 names kernels by template (``col_pass_kernel<256, 16, 1>``); the op id
 template argument, and each RL pass of a workload is one (kernel family, op) pair. The tiled 2-D
 iteration is one kernel family (``tile_iter_kernel<L0, L1>``, and ``tile_batch_kernel<L0, L1>`` for
 the batched form) whatever its lengths, and so is the compile-time build of the fused line pass
-(``col_fused_static_kernel<L, B>``, counted as ``COL_FUSED``); the compile-time row passes
+(``col_fused_static_kernel<L, B>`` and ``col_fused_inplace_kernel<L, B>``, counted as ``COL_FUSED``); the compile-time row passes
 (``row_static_kernel<L, P, OP>``) and y passes (``col_line_static_kernel<L, B, OP>``) count as their
 ``row_pass_kernel`` / ``col_pass_kernel`` op.
 """
@@ -65,7 +65,7 @@ PASS_KERNELS = {
 _NAME = re.compile(r"lucem::(\w+)<(\d+), (\d+), (\d+)>")
 _TILE = re.compile(r"(tile_iter_kernel|tile_batch_kernel)<\d+, \d+>")
 # the compile-time build of the fused line pass (COL_FUSED, op 1), named by its length and columns
-_COL_STATIC = re.compile(r"col_fused_static_kernel<\d+, \d+>")
+_COL_STATIC = re.compile(r"col_fused_(static|inplace)_kernel<\d+, \d+>")
 CAL_BYTES = 1 << 30
 
 
